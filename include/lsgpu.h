@@ -1,0 +1,211 @@
+/*
+ * lsgpu.h -- C ABI of liblsgpu, the MI355X (gfx950) ray-casting and
+ * lightmap-gather backend for LightsprintCore.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status returns, no
+ * exceptions and no HIP/torch types in any signature (streams and events are
+ * void*).  Every entry point states the reference interface it replaces.
+ *
+ * Status codes: 0 = ok, negative = error; lsg_last_error() gives the message
+ * of the last failure on the calling thread.
+ */
+#ifndef LSGPU_H
+#define LSGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LSG_OK              0
+#define LSG_ERR_NO_DEVICE  -1   /* no usable gfx950 device: caller falls back (RRCollider.cpp:236-239) */
+#define LSG_ERR_OOM        -2   /* device allocation failed (RRCollider.cpp:221-225 bad_alloc path) */
+#define LSG_ERR_INVALID    -3   /* bad argument */
+#define LSG_ERR_ABORTED    -4   /* *aborting was set (RRCollider.h:178, bsp.cpp:705) */
+#define LSG_ERR_DEVICE     -5   /* HIP runtime error */
+#define LSG_ERR_LIMIT      -6   /* size beyond a documented limit */
+
+/* RRRay::Flags, include/Lightsprint/RRCollider.h:98-106 */
+#define LSG_FILL_DISTANCE  (1u << 0)
+#define LSG_FILL_POINT3D   (1u << 1)
+#define LSG_FILL_POINT2D   (1u << 2)
+#define LSG_FILL_PLANE     (1u << 3)
+#define LSG_FILL_TRIANGLE  (1u << 4)
+#define LSG_FILL_SIDE      (1u << 5)
+
+/* Byte offsets inside the reference's 128-byte RRRay (RRCollider.h:85-137,
+ * LP64), used by lsg_collider_intersect_rrrays. */
+#define LSG_RRRAY_SIZE            128
+#define LSG_RRRAY_ORIGIN            0
+#define LSG_RRRAY_DIR              16
+#define LSG_RRRAY_LENGTH_MIN       32
+#define LSG_RRRAY_LENGTH_MAX       36
+#define LSG_RRRAY_FLAGS            40
+#define LSG_RRRAY_HIT_DISTANCE     44
+#define LSG_RRRAY_HIT_TRIANGLE     48
+#define LSG_RRRAY_HIT_POINT2D      52
+#define LSG_RRRAY_HIT_PLANE        60
+#define LSG_RRRAY_HIT_POINT3D      76
+#define LSG_RRRAY_HIT_FRONTSIDE    88
+
+typedef struct lsg_collider lsg_collider;
+typedef struct lsg_scene lsg_scene;
+
+/* ---- runtime ---------------------------------------------------------------------- */
+const char* lsg_last_error(void);
+const char* lsg_version(void);
+/* number of usable gfx950 devices (0 when none); never fails */
+int lsg_device_count(void);
+int lsg_set_device(int device);
+
+/* device-memory plumbing for callers that keep inputs resident in HBM */
+int lsg_malloc(void** ptr, size_t bytes);
+int lsg_free(void* ptr);
+int lsg_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int lsg_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int lsg_memset(void* dst, int value, size_t bytes);
+int lsg_synchronize(void);
+int lsg_stream_create(void** stream);
+int lsg_stream_destroy(void* stream);
+int lsg_stream_synchronize(void* stream);
+int lsg_event_create(void** event);
+int lsg_event_destroy(void* event);
+int lsg_event_record(void* event, void* stream);
+int lsg_event_elapsed_ms(void* start, void* stop, float* ms);
+
+/* ---- collider ------------------------------------------------------------------------
+ * Replaces RRCollider::create(mesh, nullptr, technique, aborting, cacheLocation,
+ * buildParams) (include/Lightsprint/RRCollider.h:183, RRCollider.cpp:186-241) for the
+ * technique IT_HIP_LBVH: builds a linear BVH on the GPU (Morton codes, LSD radix sort,
+ * Karras radix tree, bottom-up refit) over the mesh given as host arrays.
+ * Triangles whose SRLNP precompute is degenerate are left out, as the reference's
+ * build does (IntersectBsp.h:252-262, IntersectBspFast.cpp:775-780). */
+typedef struct lsg_build_params {
+    uint32_t leaf_size;        /* max triangles per leaf, 1..16 (default 4) */
+    uint32_t flags;            /* reserved, 0 */
+} lsg_build_params;
+
+typedef struct lsg_collider_info {
+    uint32_t num_triangles;    /* as given */
+    uint32_t num_valid;        /* non-degenerate triangles in the tree */
+    uint32_t num_nodes;        /* internal node slots (num_valid-1, >= 0) */
+    int32_t  root_ref;         /* root reference (leaf refs have bit 31 set) */
+    uint32_t depth;            /* longest chain of internal nodes */
+    uint32_t leaf_size;
+    float    box_min[3];       /* inflated mesh box (geometry.cpp:20-31) */
+    float    box_max[3];
+    float    node_margin;      /* absolute inflation of node boxes */
+    uint64_t device_bytes;     /* RRCollider::getMemoryOccupied (RRCollider.h:265) analogue */
+    double   build_ms;
+} lsg_collider_info;
+
+int  lsg_collider_create(const float* verts, uint32_t nverts, const uint32_t* indices, uint32_t ntri,
+                         const lsg_build_params* params, const volatile int* aborting, lsg_collider** out);
+void lsg_collider_destroy(lsg_collider* collider);
+int  lsg_collider_info_get(const lsg_collider* collider, lsg_collider_info* info);
+/* copies the tree to the host: nodes = 16 floats per node slot, prim = original
+ * triangle index per leaf-order slot (num_valid). Either may be NULL. */
+int  lsg_collider_download(const lsg_collider* collider, float* nodes, uint32_t* prim);
+
+/* intersectBatch over the reference's own RRRay records in host memory.
+ * Re-adds RRCollider::intersectBatch (doc/doxygen/doc.news:3212, removed :154)
+ * with the per-ray contract of RRCollider::intersect (RRCollider.h:197-228):
+ * closest hit in <rayLengthMin,rayLengthMax>, outputs per rayFlags.  Rays
+ * carrying a collisionHandler are not supported here (the C++ collider routes
+ * them to its CPU path).  hit_mask (optional, n bytes) receives 1/0 per ray. */
+int  lsg_collider_intersect_rrrays(const lsg_collider* collider, void* rays, uint32_t n,
+                                   uint8_t* hit_mask, uint32_t* num_hits);
+
+/* Device-resident batch (the benchmark path).  ray_orig_tmin / ray_dir_tmax:
+ * device float4[n] (origin.xyz, rayLengthMin) and (dir.xyz, rayLengthMax).
+ * hits: device int4[n] = (triangle or -1, bits of t, bits of u, bits of v).
+ * front (optional device uint8[n]) receives hitFrontSide.  Asynchronous on
+ * stream (NULL = default stream). */
+int  lsg_collider_intersect_device(const lsg_collider* collider, const void* ray_orig_tmin,
+                                   const void* ray_dir_tmax, uint32_t n, void* hits, void* front,
+                                   void* stream);
+
+/* Traversal work counters for n device rays (node visits, triangle tests);
+ * used for the roofline bytes-per-ray figure.  Synchronous. */
+int  lsg_collider_trace_stats(const lsg_collider* collider, const void* ray_orig_tmin,
+                              const void* ray_dir_tmax, uint32_t n, double* node_visits, double* tri_tests);
+
+/* ---- lightmap gather -------------------------------------------------------------------
+ * A scene is a collider plus what RRSolver's final gather reads at a hit: per-vertex
+ * normals (RRMesh::getTriangleNormals, RRMesh.cpp:77-90), per-triangle side/material
+ * bits (RRMaterial sideBits, RRMaterial.cpp:154-166), per-triangle exitance of each
+ * side (PathtracerWorker::getIncidentRadiance with solver shortcuts at depth 0,
+ * pathtracer.cpp:235-350) and the lights (RRLight::getIrradiance, RRLight.cpp:345-386). */
+#define LSG_SIDE_FRONT_RENDERS  1u   /* sideBits[0].renderFrom */
+#define LSG_SIDE_BACK_RENDERS   2u   /* sideBits[1].renderFrom */
+#define LSG_SIDE_CASTS_SHADOW   4u   /* getTriangleMaterial(t, light, ...) != nullptr */
+
+typedef struct lsg_light {
+    int32_t type;              /* 0 = point (REALISTIC attenuation), 1 = directional */
+    float   position[3];
+    float   direction[3];
+    float   color[3];          /* linear, already scaled by the light's multiplier */
+    int32_t cast_shadows;
+} lsg_light;
+
+/* SubTexel, RRSolver/gather.h:60-65: triangle, uv of its three corners in
+ * triangle space, area in map space.  32 bytes. */
+typedef struct lsg_subtexel {
+    uint32_t triangle;
+    float    uv[6];
+    float    area;
+} lsg_subtexel;
+
+/* One texel to gather: 16 bytes.  sub_count_flags = count (low 16 bits) |
+ * texelFlags << 16 (gather.h:68-80).  key = stable id of the texel for the
+ * seeded ray set (object/texel hash). */
+typedef struct lsg_texel {
+    uint32_t out_index;        /* float4 slot in the output buffer */
+    uint32_t sub_begin;
+    uint32_t sub_count_flags;
+    uint32_t key;
+} lsg_texel;
+
+/* RRSolver::UpdateParameters subset (RRSolver.h:468-592) */
+typedef struct lsg_gather_params {
+    uint32_t quality;          /* hemisphere rays; light rounds = quality/10+1 (gather.cpp:197,396) */
+    int32_t  hemisphere;       /* 1 when emissive/environment/useCurrentSolution (gather.cpp:197) */
+    float    ray_length_min;   /* minimalSafeDistance (RRSolver.cpp:333) */
+    float    locality;         /* default 100000 */
+    float    rug_distance;     /* default 0.001 */
+    float    light_multiplier; /* direct.lightMultiplier */
+    uint32_t seed;             /* seeded replacement of rand() (gather.h:102, gather.cpp:742) */
+    uint32_t reserved;
+} lsg_gather_params;
+
+int  lsg_scene_create(const lsg_collider* collider, const float* vertex_normals, const uint8_t* side_bits,
+                      const float* exitance, const lsg_light* lights, uint32_t nlights, lsg_scene** out);
+void lsg_scene_destroy(lsg_scene* scene);
+/* exitance: host float[ntri*8] = per triangle (front r,g,b,pad, back r,g,b,pad) */
+int  lsg_scene_set_exitance(lsg_scene* scene, const float* exitance);
+int  lsg_scene_set_lights(lsg_scene* scene, const lsg_light* lights, uint32_t nlights);
+
+/* The per-texel final gather, processTexel (RRSolver/gather.cpp:600-836) for every
+ * texel at once.  texels/subs are device arrays; out_rgba is a device float4 array
+ * indexed by lsg_texel.out_index (RGB irradiance in physical units, alpha =
+ * texelFlags/255 when reliable, gather.cpp:795-807).  Asynchronous on stream. */
+int  lsg_gather_device(lsg_scene* scene, const lsg_texel* texels, uint32_t ntexels,
+                       const lsg_subtexel* subs, uint32_t nsubs, const lsg_gather_params* params,
+                       float* out_rgba, void* stream);
+
+/* Texel rasterisation of one object's lightmap, RRSolver/lightmap.cpp:83-335
+ * (enumerateTexelsPartial, steps 1-2): clips every texel square against every
+ * triangle of the object in its unwrap and emits SubTexels and texelFlags.
+ * tri_uv: float[ntri*6] lightmap uv of each triangle; tri_ids: the triangle's
+ * index in the collider's mesh.  Call with texels == NULL to get the counts. */
+int  lsg_rasterize_texels(const float* tri_uv, const uint32_t* tri_ids, uint32_t ntri,
+                          uint32_t width, uint32_t height, uint32_t object_id,
+                          lsg_texel* texels, uint32_t* ntexels, lsg_subtexel* subs, uint32_t* nsubs,
+                          uint32_t out_base);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

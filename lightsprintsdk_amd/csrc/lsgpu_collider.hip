@@ -1,0 +1,672 @@
+// lsgpu_collider.hip -- GPU LBVH build and closest-hit traversal for gfx950.
+//
+// Replaces the reference's CPU BSP/kd collider (IntersectBspFast, bsp.cpp) behind
+// RRCollider::create / intersect.  Build: mesh box -> per-triangle SRLNP precompute
+// + Morton key -> LSD radix sort (LDS histograms, wave-ballot stable ranking) ->
+// Karras radix tree -> bottom-up refit with arrival counters -> 64-byte BVH2 nodes
+// with up to leaf_size triangles per leaf, triangles stored in leaf order.
+// Traversal: one ray per lane, near-child-first, per-lane stack in LDS, the
+// reference's SRLNP triangle test, closest hit with ties to the lowest index.
+#include "lsgpu_internal.h"
+#include "lsgpu_trace.h"
+#include <vector>
+#include <chrono>
+#include <cstring>
+#include <cmath>
+
+namespace lsg {
+
+static inline void dfree(void* p) { if (p) (void)hipFree(p); }
+
+// ---------------------------------------------------------------------------------------
+// build kernels
+
+// RRMesh::getAABB (RRMesh/RRMesh.cpp:164-194): min/max over finite vertex coordinates.
+__global__ __launch_bounds__(256) void k_vertex_aabb(const float* __restrict__ v, uint32_t nv, float* __restrict__ partial)
+{
+    __shared__ float s[6][256];
+    float mn[3] = {1e37f, 1e37f, 1e37f}, mx[3] = {-1e37f, -1e37f, -1e37f};
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < nv; i += gridDim.x * 256)
+        for (int j = 0; j < 3; j++) {
+            float c = v[3 * (size_t)i + j];
+            if (isfinite(c)) { mn[j] = rminf(mn[j], c); mx[j] = rmaxf(mx[j], c); }
+        }
+    for (int j = 0; j < 3; j++) { s[j][threadIdx.x] = mn[j]; s[3 + j][threadIdx.x] = mx[j]; }
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int j = 0; j < 3; j++) {
+                s[j][threadIdx.x] = rminf(s[j][threadIdx.x], s[j][threadIdx.x + w]);
+                s[3 + j][threadIdx.x] = rmaxf(s[3 + j][threadIdx.x], s[3 + j][threadIdx.x + w]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) partial[6 * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
+}
+
+__device__ inline uint32_t expand10(uint32_t v)
+{
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+__device__ inline uint32_t quant10(float c, float mn, float scale)
+{
+    float q = (c - mn) * scale;
+    if (!(q > 0)) return 0;
+    if (q >= 1023.f) return 1023;
+    return (uint32_t)q;
+}
+
+// TriangleSRLNP::setGeometry (RRCollider/IntersectBspFast.cpp:130-187), bit-exact,
+// plus the triangle box and its Morton key.  Degenerate triangles (byte 10) get key
+// 0xffffffff and sort behind every valid one (valid keys use 30 bits).
+__global__ __launch_bounds__(256) void k_tri_setup(const float* __restrict__ verts, const uint32_t* __restrict__ idx, uint32_t ntri,
+                                                   float3 mn, float3 scale, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                                   float4* __restrict__ plane, float4* __restrict__ sv, float4* __restrict__ ev,
+                                                   uint32_t* __restrict__ bytes, float4* __restrict__ blo, float4* __restrict__ bhi,
+                                                   uint32_t* __restrict__ nvalid)
+{
+    uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= ntri) return;
+    const float* a = verts + 3 * (size_t)idx[3 * (size_t)t + 0];
+    const float* b = verts + 3 * (size_t)idx[3 * (size_t)t + 1];
+    const float* c = verts + 3 * (size_t)idx[3 * (size_t)t + 2];
+    float s[3], r[3], l[3], ir[3];
+    for (int i = 0; i < 3; i++) { s[i] = a[i]; r[i] = b[i] - a[i]; l[i] = c[i] - a[i]; }
+    for (int i = 0; i < 3; i++) ir[i] = 1 / r[i];
+    float rxy = l[1] * r[0] - l[0] * r[1];
+    float ryz = l[2] * r[1] - l[1] * r[2];
+    float rzx = l[0] * r[2] - l[2] * r[0];
+    int byte; float ireal;
+    if (fabsf(rxy) >= fabsf(ryz)) {
+        if (fabsf(rxy) >= fabsf(rzx)) { byte = 0; ireal = 1 / rxy; }
+        else                          { byte = 2; ireal = 1 / rzx; }
+    } else {
+        if (fabsf(ryz) >= fabsf(rzx)) { byte = 1; ireal = 1 / ryz; }
+        else                          { byte = 2; ireal = 1 / rzx; }
+    }
+    if (fabsf(r[0]) >= fabsf(r[1])) { if (fabsf(r[0]) >= fabsf(r[2])) byte += 3; }
+    else { if (fabsf(r[1]) >= fabsf(r[2])) byte += 6; }
+    float nx = r[1] * l[2] - r[2] * l[1];
+    float ny = -r[0] * l[2] + r[2] * l[0];
+    float nz = r[0] * l[1] - r[1] * l[0];
+    float len = sqrtf(nx * nx + ny * ny + nz * nz);
+    nx = nx / len; ny = ny / len; nz = nz / len;
+    float nw = -(s[0] * nx + s[1] * ny + s[2] * nz);
+    if (!(isfinite(nx) && isfinite(ny) && isfinite(nz))) byte = 10;
+    int x, y, z;
+    srlnp_axes(byte, x, y, z);
+    plane[t] = make_float4(nx, ny, nz, nw);
+    sv[t] = make_float4(s[x], s[y], s[z], ireal);
+    ev[t] = make_float4(r[x], r[y], l[z], ir[z]);
+    bytes[t] = (uint32_t)byte;
+    float lo[3], hi[3];
+    for (int j = 0; j < 3; j++) {
+        lo[j] = rminf(rminf(a[j], b[j]), c[j]);
+        hi[j] = rmaxf(rmaxf(a[j], b[j]), c[j]);
+    }
+    // oracle: l = a<b?a:b; l = l<c?l:c  (same as rminf chain)
+    blo[t] = make_float4(lo[0], lo[1], lo[2], 0.f);
+    bhi[t] = make_float4(hi[0], hi[1], hi[2], 0.f);
+    uint32_t key = 0xffffffffu;
+    if (byte <= 8) {
+        uint32_t qx = quant10((lo[0] + hi[0]) * 0.5f, mn.x, scale.x);
+        uint32_t qy = quant10((lo[1] + hi[1]) * 0.5f, mn.y, scale.y);
+        uint32_t qz = quant10((lo[2] + hi[2]) * 0.5f, mn.z, scale.z);
+        key = (expand10(qx) << 2) | (expand10(qy) << 1) | expand10(qz);
+        atomicAdd(nvalid, 1u);
+    }
+    keys[t] = key;
+    vals[t] = t;
+}
+
+// ---- LSD radix sort, 8-bit digits, tiles of 2048 keys per 256-thread block ------------
+constexpr int RS_THREADS = 256, RS_ITEMS = 8, RS_TILE = RS_THREADS * RS_ITEMS;
+
+__global__ __launch_bounds__(256) void k_rs_hist(const uint32_t* __restrict__ keys, uint32_t n, int shift, uint32_t* __restrict__ hist)
+{
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t base = blockIdx.x * RS_TILE;
+    for (int k = 0; k < RS_ITEMS; k++) {
+        uint32_t i = base + k * RS_THREADS + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
+}
+
+// exclusive scan of m counters in place, one block of 1024 threads
+__global__ __launch_bounds__(1024) void k_rs_scan(uint32_t* __restrict__ a, uint32_t m)
+{
+    __shared__ uint32_t s[1024];
+    uint32_t chunk = (m + 1023) / 1024;
+    uint32_t b = threadIdx.x * chunk, e = min(m, b + chunk);
+    uint32_t sum = 0;
+    for (uint32_t i = b; i < e; i++) sum += a[i];
+    s[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        uint32_t v = threadIdx.x >= (uint32_t)off ? s[threadIdx.x - off] : 0;
+        __syncthreads();
+        s[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = s[threadIdx.x] - sum;
+    for (uint32_t i = b; i < e; i++) { uint32_t v = a[i]; a[i] = run; run += v; }
+}
+
+__global__ __launch_bounds__(256) void k_rs_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                    uint32_t n, int shift, const uint32_t* __restrict__ offs)
+{
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcnt[4][256];
+    const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
+    run[tid] = offs[tid * gridDim.x + blockIdx.x];
+    for (int w = 0; w < 4; w++) wcnt[w][tid] = 0;
+    __syncthreads();
+    uint32_t base = blockIdx.x * RS_TILE;
+    for (int k = 0; k < RS_ITEMS; k++) {
+        uint32_t i = base + k * RS_THREADS + tid;
+        bool valid = i < n;
+        uint32_t key = valid ? kin[i] : 0u;
+        uint32_t dig = (key >> shift) & 255u;
+        uint64_t m = __ballot(valid);
+        for (int bit = 0; bit < 8; bit++) {
+            uint64_t bal = __ballot((dig >> bit) & 1u);
+            m &= ((dig >> bit) & 1u) ? bal : ~bal;
+        }
+        uint64_t lower = m & ((1ull << lane) - 1ull);
+        uint32_t rank = (uint32_t)__popcll(lower);
+        if (valid && lower == 0) wcnt[wave][dig] = (uint32_t)__popcll(m);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = run[dig] + rank;
+            for (uint32_t w = 0; w < wave; w++) pos += wcnt[w][dig];
+            kout[pos] = key;
+            vout[pos] = vin[i];
+        }
+        __syncthreads();
+        run[tid] += wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
+        for (int w = 0; w < 4; w++) wcnt[w][tid] = 0;
+        __syncthreads();
+    }
+}
+
+// ---- Karras radix tree (T. Karras, "Maximizing parallelism in the construction of
+// BVHs, octrees and k-d trees", HPG 2012) -------------------------------------------------
+__device__ inline int kdelta(const uint32_t* __restrict__ k, int n, int i, int j)
+{
+    if (j < 0 || j >= n) return -1;
+    uint32_t a = k[i], b = k[j];
+    if (a == b) return 32 + __clz((int)((uint32_t)i ^ (uint32_t)j));
+    return __clz((int)(a ^ b));
+}
+
+__global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ keys, int n, int* __restrict__ first, int* __restrict__ last,
+                                                int* __restrict__ split, int* __restrict__ parent_node, int* __restrict__ parent_leaf)
+{
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n - 1) return;
+    int d = (kdelta(keys, n, i, i + 1) - kdelta(keys, n, i, i - 1)) > 0 ? 1 : -1;
+    int dmin = kdelta(keys, n, i, i - d);
+    int lmax = 2;
+    while (kdelta(keys, n, i, i + lmax * d) > dmin) lmax *= 2;
+    int l = 0;
+    for (int t = lmax / 2; t >= 1; t /= 2)
+        if (kdelta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    int j = i + l * d;
+    int dnode = kdelta(keys, n, i, j);
+    int s = 0, t = l;
+    do { t = (t + 1) >> 1; if (kdelta(keys, n, i, i + (s + t) * d) > dnode) s += t; } while (t > 1);
+    int g = i + s * d + (d < 0 ? d : 0);
+    int f = min(i, j), la = max(i, j);
+    first[i] = f; last[i] = la; split[i] = g;
+    if (f == g) parent_leaf[g] = i; else parent_node[g] = i;
+    if (la == g + 1) parent_leaf[g + 1] = i; else parent_node[g + 1] = i;
+    if (i == 0) parent_node[0] = -1;
+}
+
+// Bottom-up union of child boxes; the second thread to arrive at a node computes it.
+__global__ __launch_bounds__(256) void k_refit(int n, const int* __restrict__ first, const int* __restrict__ last, const int* __restrict__ split,
+                                               const int* __restrict__ parent_node, const int* __restrict__ parent_leaf,
+                                               const float4* __restrict__ plo, const float4* __restrict__ phi,
+                                               float4* nlo, float4* nhi, uint32_t* arrive)
+{
+    int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= n) return;
+    int node = parent_leaf[p];
+    while (node >= 0) {
+        __threadfence();
+        uint32_t old = atomicAdd(&arrive[node], 1u);
+        if (old == 0) return;
+        __threadfence();
+        int g = split[node];
+        float4 al, ah, bl, bh;
+        if (first[node] == g) { al = plo[g]; ah = phi[g]; }
+        else { al = nlo[g]; ah = nhi[g]; }
+        if (last[node] == g + 1) { bl = plo[g + 1]; bh = phi[g + 1]; }
+        else { bl = nlo[g + 1]; bh = nhi[g + 1]; }
+        float4 lo = make_float4(fminf(al.x, bl.x), fminf(al.y, bl.y), fminf(al.z, bl.z), 0.f);
+        float4 hi = make_float4(fmaxf(ah.x, bh.x), fmaxf(ah.y, bh.y), fmaxf(ah.z, bh.z), 0.f);
+        nlo[node] = lo;
+        nhi[node] = hi;
+        node = parent_node[node];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_write_nodes(int n, uint32_t leafmax, float margin, const int* __restrict__ first, const int* __restrict__ last,
+                                                     const int* __restrict__ split, const int* __restrict__ parent_node,
+                                                     const float4* __restrict__ plo, const float4* __restrict__ phi,
+                                                     const float4* __restrict__ nlo, const float4* __restrict__ nhi,
+                                                     Node* __restrict__ nodes, uint32_t* __restrict__ depth)
+{
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n - 1) return;
+    int g = split[i];
+    float b[2][6]; int32_t ref[2];
+    for (int c = 0; c < 2; c++) {
+        int cf = c == 0 ? first[i] : g + 1;
+        int cl = c == 0 ? g : last[i];
+        int cnt = cl - cf + 1;
+        float4 lo, hi;
+        if (cnt == 1) { lo = plo[cf]; hi = phi[cf]; }
+        else { lo = nlo[c == 0 ? g : g + 1]; hi = nhi[c == 0 ? g : g + 1]; }
+        b[c][0] = lo.x - margin; b[c][1] = hi.x + margin;
+        b[c][2] = lo.y - margin; b[c][3] = hi.y + margin;
+        b[c][4] = lo.z - margin; b[c][5] = hi.z + margin;
+        ref[c] = ((uint32_t)cnt <= leafmax) ? make_leaf((uint32_t)cf, (uint32_t)cnt) : (c == 0 ? g : g + 1);
+    }
+    Node nd;
+    nd.q0 = make_float4(b[0][0], b[0][1], b[0][2], b[0][3]);
+    nd.q1 = make_float4(b[0][4], b[0][5], b[1][0], b[1][1]);
+    nd.q2 = make_float4(b[1][2], b[1][3], b[1][4], b[1][5]);
+    nd.q3 = make_float4(__int_as_float(ref[0]), __int_as_float(ref[1]), 0.f, 0.f);
+    nodes[i] = nd;
+    if ((uint32_t)(last[i] - first[i] + 1) > leafmax) {
+        uint32_t dd = 1; int p = parent_node[i];
+        while (p >= 0) { dd++; p = parent_node[p]; }
+        atomicMax(depth, dd);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_write_tris(uint32_t n, const uint32_t* __restrict__ order, const float4* __restrict__ plane_u,
+                                                    const float4* __restrict__ sv_u, const float4* __restrict__ ev_u,
+                                                    const uint32_t* __restrict__ bytes, const float4* __restrict__ blo_u,
+                                                    const float4* __restrict__ bhi_u, float4* __restrict__ plane, float4* __restrict__ sv,
+                                                    float4* __restrict__ ev, uint2* __restrict__ meta, float4* __restrict__ plo, float4* __restrict__ phi)
+{
+    uint32_t p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= n) return;
+    uint32_t t = order[p];
+    plane[p] = plane_u[t]; sv[p] = sv_u[t]; ev[p] = ev_u[t];
+    meta[p] = make_uint2(t, bytes[t]);
+    plo[p] = blo_u[t]; phi[p] = bhi_u[t];
+}
+
+// ---------------------------------------------------------------------------------------
+// traversal
+
+template <int STACK>
+__global__ __launch_bounds__(256) void k_trace_closest(ColliderView c, const float4* __restrict__ ro, const float4* __restrict__ rd,
+                                                       uint32_t n, int4* __restrict__ hits, uint8_t* __restrict__ front,
+                                                       uint32_t* __restrict__ overflow)
+{
+    __shared__ int32_t stk[STACK * 256];
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = ro[i], b = rd[i];
+    TraceResult r; uint32_t nv = 0, nt = 0;
+    bool ok = trace_closest<STACK, false>(NoFilter(), c, make_float3(a.x, a.y, a.z), make_float3(b.x, b.y, b.z), a.w, b.w,
+                                          stk + threadIdx.x, 256, r, nv, nt);
+    if (!ok && overflow) atomicOr(overflow, 1u);
+    hits[i] = r.tri >= 0 ? make_int4(r.tri, __float_as_int(r.t), __float_as_int(r.u), __float_as_int(r.v))
+                         : make_int4(-1, 0, 0, 0);
+    if (front) front[i] = (r.tri >= 0 && r.front) ? 1 : 0;
+}
+
+template <int STACK>
+__global__ __launch_bounds__(256) void k_trace_stats(ColliderView c, const float4* __restrict__ ro, const float4* __restrict__ rd,
+                                                     uint32_t n, unsigned long long* __restrict__ counters)
+{
+    __shared__ int32_t stk[STACK * 256];
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = ro[i], b = rd[i];
+    TraceResult r; uint32_t nv = 0, nt = 0;
+    trace_closest<STACK, true>(NoFilter(), c, make_float3(a.x, a.y, a.z), make_float3(b.x, b.y, b.z), a.w, b.w, stk + threadIdx.x, 256, r, nv, nt);
+    atomicAdd(&counters[0], (unsigned long long)nv);
+    atomicAdd(&counters[1], (unsigned long long)nt);
+}
+
+// RRRay-shaped results for lsg_collider_intersect_rrrays: (tri,t,u,v) + plane of the hit.
+template <int STACK>
+__global__ __launch_bounds__(256) void k_trace_full(ColliderView c, const float4* __restrict__ ro, const float4* __restrict__ rd,
+                                                    uint32_t n, int4* __restrict__ hits, float4* __restrict__ planes,
+                                                    uint8_t* __restrict__ front, uint32_t* __restrict__ overflow)
+{
+    __shared__ int32_t stk[STACK * 256];
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = ro[i], b = rd[i];
+    TraceResult r; uint32_t nv = 0, nt = 0;
+    bool ok = trace_closest<STACK, false>(NoFilter(), c, make_float3(a.x, a.y, a.z), make_float3(b.x, b.y, b.z), a.w, b.w, stk + threadIdx.x, 256, r, nv, nt);
+    if (!ok) atomicOr(overflow, 1u);
+    if (r.tri >= 0) {
+        hits[i] = make_int4(r.tri, __float_as_int(r.t), __float_as_int(r.u), __float_as_int(r.v));
+        planes[i] = c.plane[r.slot];
+        front[i] = r.front ? 1 : 0;
+    } else {
+        hits[i] = make_int4(-1, 0, 0, 0);
+        front[i] = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// host side
+
+template <class T> static int dalloc(T** p, size_t n, uint64_t* acc = nullptr)
+{
+    hipError_t e = hipMalloc((void**)p, sizeof(T) * (n ? n : 1));
+    if (e != hipSuccess) { *p = nullptr; return hip_fail(e, "hipMalloc", __LINE__); }
+    if (acc) *acc += sizeof(T) * (n ? n : 1);
+    return LSG_OK;
+}
+
+static void free_impl(ColliderImpl& c)
+{
+    dfree(c.d_verts); dfree(c.d_idx); dfree(c.d_nodes);
+    dfree(c.d_plane); dfree(c.d_sv); dfree(c.d_ev); dfree(c.d_meta);
+    c = ColliderImpl();
+}
+
+static inline unsigned blocks(uint64_t n, unsigned per = 256) { return (unsigned)((n + per - 1) / per); }
+
+static int build(ColliderImpl& C, const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt, uint32_t leafmax,
+                 const volatile int* aborting)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    int st;
+    C.ntri = nt; C.nverts = nv; C.leaf = leafmax;
+    if ((st = dalloc(&C.d_verts, 3 * (size_t)nv, &C.bytes))) return st;
+    if ((st = dalloc(&C.d_idx, 3 * (size_t)nt, &C.bytes))) return st;
+    LSG_HIP(hipMemcpy(C.d_verts, verts, 12 * (size_t)nv, hipMemcpyHostToDevice));
+    LSG_HIP(hipMemcpy(C.d_idx, idx, 12 * (size_t)nt, hipMemcpyHostToDevice));
+
+    // mesh box (RRMesh::getAABB) then Box::init inflation, DELTA_BSP-sized node margin
+    float mn[3] = {1e37f, 1e37f, 1e37f}, mx[3] = {-1e37f, -1e37f, -1e37f};
+    if (nv) {
+        unsigned nb = std::min(blocks(nv), 1024u);
+        float* d_part; if ((st = dalloc(&d_part, 6 * (size_t)nb))) return st;
+        k_vertex_aabb<<<nb, 256>>>(C.d_verts, nv, d_part);
+        std::vector<float> part(6 * (size_t)nb);
+        hipError_t e = hipMemcpy(part.data(), d_part, part.size() * 4, hipMemcpyDeviceToHost);
+        dfree(d_part);
+        if (e != hipSuccess) return hip_fail(e, "aabb", __LINE__);
+        for (unsigned b = 0; b < nb; b++)
+            for (int j = 0; j < 3; j++) { mn[j] = rminf(mn[j], part[6 * b + j]); mx[j] = rmaxf(mx[j], part[6 * b + 3 + j]); }
+    }
+    for (int j = 0; j < 3; j++) if (mn[j] > mx[j]) mn[j] = mx[j] = 0;
+    float scale[3];
+    for (int j = 0; j < 3; j++) { float e = mx[j] - mn[j]; scale[j] = e > 0 ? 1024.0f / e : 0.0f; }
+    float imn[3] = {mn[0], mn[1], mn[2]}, imx[3] = {mx[0], mx[1], mx[2]};
+    {
+        float d = (imx[0] - imn[0] + imx[1] - imn[1] + imx[2] - imn[2]) * 1e-5f;   // geometry.cpp:26
+        for (int j = 0; j < 3; j++) { imn[j] -= d; imx[j] += d; }
+    }
+    {
+        // IntersectLinear.cpp:133-139 DELTA_BSP on the inflated box
+        float tx = std::max(fabsf(imx[0]), fabsf(imn[0])), ty = std::max(fabsf(imx[1]), fabsf(imn[1])), tz = std::max(fabsf(imx[2]), fabsf(imn[2]));
+        float m = tx > ty ? (tx > tz ? tx : tz) : (ty > tz ? ty : tz);
+        if (m == 0 || std::isnan(m)) m = 1;
+        C.margin = m * 1e-5f;
+    }
+    memcpy(C.bmin, imn, sizeof imn); memcpy(C.bmax, imx, sizeof imx);
+    if (aborting && *aborting) return fail(LSG_ERR_ABORTED, "collider build aborted");
+
+    // per-triangle precompute + keys
+    uint32_t *d_keys = nullptr, *d_vals = nullptr, *d_k2 = nullptr, *d_v2 = nullptr, *d_bytes = nullptr, *d_nvalid = nullptr;
+    float4 *d_plane_u = nullptr, *d_sv_u = nullptr, *d_ev_u = nullptr, *d_blo_u = nullptr, *d_bhi_u = nullptr;
+    uint32_t* d_hist = nullptr;
+    int *d_first = nullptr, *d_last = nullptr, *d_split = nullptr, *d_pn = nullptr, *d_pl = nullptr;
+    float4 *d_plo = nullptr, *d_phi = nullptr, *d_nlo = nullptr, *d_nhi = nullptr;
+    uint32_t *d_arrive = nullptr, *d_depth = nullptr;
+    auto cleanup = [&]() {
+        dfree(d_keys); dfree(d_vals); dfree(d_k2); dfree(d_v2); dfree(d_bytes); dfree(d_nvalid);
+        dfree(d_plane_u); dfree(d_sv_u); dfree(d_ev_u); dfree(d_blo_u); dfree(d_bhi_u); dfree(d_hist);
+        dfree(d_first); dfree(d_last); dfree(d_split); dfree(d_pn); dfree(d_pl);
+        dfree(d_plo); dfree(d_phi); dfree(d_nlo); dfree(d_nhi); dfree(d_arrive); dfree(d_depth);
+    };
+#define CK(x) do { if ((st = (x))) { cleanup(); return st; } } while (0)
+#define CKH(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { cleanup(); return hip_fail(e_, #x, __LINE__); } } while (0)
+    const size_t N = nt ? nt : 1;
+    CK(dalloc(&d_keys, N)); CK(dalloc(&d_vals, N)); CK(dalloc(&d_k2, N)); CK(dalloc(&d_v2, N));
+    CK(dalloc(&d_bytes, N)); CK(dalloc(&d_nvalid, 1));
+    CK(dalloc(&d_plane_u, N)); CK(dalloc(&d_sv_u, N)); CK(dalloc(&d_ev_u, N)); CK(dalloc(&d_blo_u, N)); CK(dalloc(&d_bhi_u, N));
+    CKH(hipMemset(d_nvalid, 0, 4));
+    if (nt)
+        k_tri_setup<<<blocks(nt), 256>>>(C.d_verts, C.d_idx, nt, make_float3(mn[0], mn[1], mn[2]),
+                                         make_float3(scale[0], scale[1], scale[2]), d_keys, d_vals, d_plane_u, d_sv_u, d_ev_u,
+                                         d_bytes, d_blo_u, d_bhi_u, d_nvalid);
+    CKH(hipGetLastError());
+    uint32_t nvalid = 0;
+    CKH(hipMemcpy(&nvalid, d_nvalid, 4, hipMemcpyDeviceToHost));
+    C.nvalid = nvalid;
+    if (aborting && *aborting) { cleanup(); return fail(LSG_ERR_ABORTED, "collider build aborted"); }
+
+    // sort (key, triangle) pairs: 4 passes of 8 bits
+    if (nt > 1) {
+        unsigned nb = blocks(nt, RS_TILE);
+        CK(dalloc(&d_hist, 256 * (size_t)nb));
+        uint32_t *ki = d_keys, *vi = d_vals, *ko = d_k2, *vo = d_v2;
+        for (int shift = 0; shift < 32; shift += 8) {
+            k_rs_hist<<<nb, 256>>>(ki, nt, shift, d_hist);
+            k_rs_scan<<<1, 1024>>>(d_hist, 256 * nb);
+            k_rs_scatter<<<nb, 256>>>(ki, vi, ko, vo, nt, shift, d_hist);
+            std::swap(ki, ko); std::swap(vi, vo);
+        }
+        CKH(hipGetLastError());
+        // after 4 passes results are back in d_keys/d_vals
+    }
+    const uint32_t n = nvalid;
+    CK(dalloc(&C.d_plane, n, &C.bytes)); CK(dalloc(&C.d_sv, n, &C.bytes)); CK(dalloc(&C.d_ev, n, &C.bytes)); CK(dalloc(&C.d_meta, n, &C.bytes));
+    CK(dalloc(&d_plo, n)); CK(dalloc(&d_phi, n));
+    if (n) k_write_tris<<<blocks(n), 256>>>(n, d_vals, d_plane_u, d_sv_u, d_ev_u, d_bytes, d_blo_u, d_bhi_u,
+                                            C.d_plane, C.d_sv, C.d_ev, C.d_meta, d_plo, d_phi);
+    CKH(hipGetLastError());
+    C.nnodes = n > 1 ? n - 1 : 0;
+    CK(dalloc(&C.d_nodes, C.nnodes ? C.nnodes : 1, &C.bytes));
+    C.depth = 0;
+    if (n == 0) C.root = 0;
+    else if (n <= leafmax) C.root = make_leaf(0, n);
+    else {
+        const int ni = (int)n - 1;
+        CK(dalloc(&d_first, ni)); CK(dalloc(&d_last, ni)); CK(dalloc(&d_split, ni)); CK(dalloc(&d_pn, ni)); CK(dalloc(&d_pl, n));
+        CK(dalloc(&d_nlo, ni)); CK(dalloc(&d_nhi, ni)); CK(dalloc(&d_arrive, ni)); CK(dalloc(&d_depth, 1));
+        CKH(hipMemset(d_arrive, 0, 4 * (size_t)ni));
+        CKH(hipMemset(d_depth, 0, 4));
+        k_karras<<<blocks(ni), 256>>>(d_keys, (int)n, d_first, d_last, d_split, d_pn, d_pl);
+        k_refit<<<blocks(n), 256>>>((int)n, d_first, d_last, d_split, d_pn, d_pl, d_plo, d_phi, d_nlo, d_nhi, d_arrive);
+        k_write_nodes<<<blocks(ni), 256>>>((int)n, leafmax, C.margin, d_first, d_last, d_split, d_pn, d_plo, d_phi, d_nlo, d_nhi,
+                                           C.d_nodes, d_depth);
+        CKH(hipGetLastError());
+        CKH(hipMemcpy(&C.depth, d_depth, 4, hipMemcpyDeviceToHost));
+        C.root = 0;
+    }
+    CKH(hipDeviceSynchronize());
+    cleanup();
+#undef CK
+#undef CKH
+    C.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (!stack_size_for_depth(C.depth))
+        return fail(LSG_ERR_LIMIT, "LBVH depth %u exceeds the traversal stack (63)", C.depth);
+    return LSG_OK;
+}
+
+} // namespace lsg
+
+using namespace lsg;
+
+extern "C" {
+
+int lsg_collider_create(const float* verts, uint32_t nverts, const uint32_t* indices, uint32_t ntri,
+                        const lsg_build_params* params, const volatile int* aborting, lsg_collider** out)
+{
+    if (!out) return fail(LSG_ERR_INVALID, "lsg_collider_create: out is null");
+    *out = nullptr;
+    if ((!verts && nverts) || (!indices && ntri)) return fail(LSG_ERR_INVALID, "lsg_collider_create: null mesh arrays");
+    if (ntri >= (1u << 27)) return fail(LSG_ERR_LIMIT, "lsg_collider_create: %u triangles exceeds 2^27", ntri);
+    for (uint64_t i = 0; i < 3ull * ntri; i++)
+        if (indices[i] >= nverts) return fail(LSG_ERR_INVALID, "lsg_collider_create: index %u out of range at %llu", indices[i], (unsigned long long)i);
+    uint32_t leaf = params && params->leaf_size ? params->leaf_size : 4;
+    if (leaf < 1 || leaf > 16) return fail(LSG_ERR_INVALID, "lsg_collider_create: leaf_size %u not in 1..16", leaf);
+    if (!device_ready()) return fail(LSG_ERR_NO_DEVICE, "lsg_collider_create: no gfx950 device");
+    lsg_collider* c = new (std::nothrow) lsg_collider;
+    if (!c) return fail(LSG_ERR_OOM, "lsg_collider_create: host allocation failed");
+    int st = build(c->impl, verts, nverts, indices, ntri, leaf, aborting);
+    if (st) { free_impl(c->impl); delete c; return st; }
+    *out = c;
+    return LSG_OK;
+}
+
+void lsg_collider_destroy(lsg_collider* c)
+{
+    if (!c) return;
+    free_impl(c->impl);
+    delete c;
+}
+
+int lsg_collider_info_get(const lsg_collider* c, lsg_collider_info* info)
+{
+    if (!c || !info) return fail(LSG_ERR_INVALID, "lsg_collider_info_get: null argument");
+    const ColliderImpl& C = c->impl;
+    info->num_triangles = C.ntri; info->num_valid = C.nvalid; info->num_nodes = C.nnodes;
+    info->root_ref = C.root; info->depth = C.depth; info->leaf_size = C.leaf;
+    for (int j = 0; j < 3; j++) { info->box_min[j] = C.bmin[j]; info->box_max[j] = C.bmax[j]; }
+    info->node_margin = C.margin; info->device_bytes = C.bytes; info->build_ms = C.build_ms;
+    return LSG_OK;
+}
+
+int lsg_collider_download(const lsg_collider* c, float* nodes, uint32_t* prim)
+{
+    if (!c) return fail(LSG_ERR_INVALID, "lsg_collider_download: null collider");
+    const ColliderImpl& C = c->impl;
+    if (nodes && C.nnodes) LSG_HIP(hipMemcpy(nodes, C.d_nodes, sizeof(Node) * (size_t)C.nnodes, hipMemcpyDeviceToHost));
+    if (prim && C.nvalid) {
+        std::vector<uint2> m(C.nvalid);
+        LSG_HIP(hipMemcpy(m.data(), C.d_meta, sizeof(uint2) * (size_t)C.nvalid, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < C.nvalid; i++) prim[i] = m[i].x;
+    }
+    return LSG_OK;
+}
+
+int lsg_collider_intersect_device(const lsg_collider* c, const void* ro, const void* rd, uint32_t n, void* hits, void* front, void* stream)
+{
+    if (!c || (!n)) return c ? LSG_OK : fail(LSG_ERR_INVALID, "lsg_collider_intersect_device: null collider");
+    if (!ro || !rd || !hits) return fail(LSG_ERR_INVALID, "lsg_collider_intersect_device: null buffer");
+    const ColliderImpl& C = c->impl;
+    int stack = stack_size_for_depth(C.depth);
+    if (!stack) return fail(LSG_ERR_LIMIT, "tree too deep");
+    uint32_t* d_over = nullptr;   // stack >= depth+1 by construction; the kernel guard only prevents overrun
+    hipStream_t s = (hipStream_t)stream;
+    if (stack <= 32)
+        k_trace_closest<32><<<blocks(n), 256, 0, s>>>(C.view(), (const float4*)ro, (const float4*)rd, n, (int4*)hits, (uint8_t*)front, d_over);
+    else
+        k_trace_closest<64><<<blocks(n), 256, 0, s>>>(C.view(), (const float4*)ro, (const float4*)rd, n, (int4*)hits, (uint8_t*)front, d_over);
+    LSG_HIP(hipGetLastError());
+    return LSG_OK;
+}
+
+int lsg_collider_trace_stats(const lsg_collider* c, const void* ro, const void* rd, uint32_t n, double* node_visits, double* tri_tests)
+{
+    if (!c || !ro || !rd) return fail(LSG_ERR_INVALID, "lsg_collider_trace_stats: null argument");
+    const ColliderImpl& C = c->impl;
+    unsigned long long* d_cnt = nullptr;
+    LSG_HIP(hipMalloc(&d_cnt, 16));
+    LSG_HIP(hipMemset(d_cnt, 0, 16));
+    if (n) {
+        if (stack_size_for_depth(C.depth) <= 32)
+            k_trace_stats<32><<<blocks(n), 256>>>(C.view(), (const float4*)ro, (const float4*)rd, n, d_cnt);
+        else
+            k_trace_stats<64><<<blocks(n), 256>>>(C.view(), (const float4*)ro, (const float4*)rd, n, d_cnt);
+    }
+    unsigned long long h[2] = {0, 0};
+    hipError_t e = hipMemcpy(h, d_cnt, 16, hipMemcpyDeviceToHost);
+    dfree(d_cnt);
+    if (e != hipSuccess) return hip_fail(e, "stats", __LINE__);
+    if (node_visits) *node_visits = (double)h[0];
+    if (tri_tests) *tri_tests = (double)h[1];
+    return LSG_OK;
+}
+
+int lsg_collider_intersect_rrrays(const lsg_collider* c, void* rays, uint32_t n, uint8_t* hit_mask, uint32_t* num_hits)
+{
+    if (!c || (!rays && n)) return fail(LSG_ERR_INVALID, "lsg_collider_intersect_rrrays: null argument");
+    if (num_hits) *num_hits = 0;
+    if (!n) return LSG_OK;
+    const ColliderImpl& C = c->impl;
+    uint8_t* R = (uint8_t*)rays;
+    std::vector<float4> ho(n), hd(n);
+    for (uint32_t i = 0; i < n; i++) {
+        const uint8_t* r = R + (size_t)i * LSG_RRRAY_SIZE;
+        float o[4], d[4], mn, mx;
+        memcpy(o, r + LSG_RRRAY_ORIGIN, 16); memcpy(d, r + LSG_RRRAY_DIR, 16);
+        memcpy(&mn, r + LSG_RRRAY_LENGTH_MIN, 4); memcpy(&mx, r + LSG_RRRAY_LENGTH_MAX, 4);
+        ho[i] = make_float4(o[0], o[1], o[2], mn);
+        hd[i] = make_float4(d[0], d[1], d[2], mx);
+    }
+    float4 *d_o = nullptr, *d_d = nullptr, *d_pl = nullptr; int4* d_h = nullptr; uint8_t* d_f = nullptr; uint32_t* d_over = nullptr;
+    auto cleanup = [&]() { dfree(d_o); dfree(d_d); dfree(d_pl); dfree(d_h); dfree(d_f); dfree(d_over); };
+    int st;
+#define CK(x) do { if ((st = (x))) { cleanup(); return st; } } while (0)
+#define CKH(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { cleanup(); return hip_fail(e_, #x, __LINE__); } } while (0)
+    CK(dalloc(&d_o, n)); CK(dalloc(&d_d, n)); CK(dalloc(&d_pl, n)); CK(dalloc(&d_h, n)); CK(dalloc(&d_f, n)); CK(dalloc(&d_over, 1));
+    CKH(hipMemcpy(d_o, ho.data(), 16 * (size_t)n, hipMemcpyHostToDevice));
+    CKH(hipMemcpy(d_d, hd.data(), 16 * (size_t)n, hipMemcpyHostToDevice));
+    CKH(hipMemset(d_over, 0, 4));
+    if (stack_size_for_depth(C.depth) <= 32)
+        k_trace_full<32><<<blocks(n), 256>>>(C.view(), d_o, d_d, n, d_h, d_pl, d_f, d_over);
+    else
+        k_trace_full<64><<<blocks(n), 256>>>(C.view(), d_o, d_d, n, d_h, d_pl, d_f, d_over);
+    CKH(hipGetLastError());
+    std::vector<int4> hh(n); std::vector<float4> hp(n); std::vector<uint8_t> hf(n); uint32_t over = 0;
+    CKH(hipMemcpy(hh.data(), d_h, 16 * (size_t)n, hipMemcpyDeviceToHost));
+    CKH(hipMemcpy(hp.data(), d_pl, 16 * (size_t)n, hipMemcpyDeviceToHost));
+    CKH(hipMemcpy(hf.data(), d_f, n, hipMemcpyDeviceToHost));
+    CKH(hipMemcpy(&over, d_over, 4, hipMemcpyDeviceToHost));
+    cleanup();
+#undef CK
+#undef CKH
+    if (over) return fail(LSG_ERR_LIMIT, "traversal stack overflow");
+    uint32_t hitsn = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        uint8_t* r = R + (size_t)i * LSG_RRRAY_SIZE;
+        bool hit = hh[i].x >= 0;
+        if (hit_mask) hit_mask[i] = hit ? 1 : 0;
+        if (!hit) continue;
+        hitsn++;
+        uint32_t flags; memcpy(&flags, r + LSG_RRRAY_FLAGS, 4);
+        float t, u, v; memcpy(&t, &hh[i].y, 4); memcpy(&u, &hh[i].z, 4); memcpy(&v, &hh[i].w, 4);
+        if (flags & LSG_FILL_DISTANCE) memcpy(r + LSG_RRRAY_HIT_DISTANCE, &t, 4);
+        if (flags & LSG_FILL_TRIANGLE) { uint32_t tri = (uint32_t)hh[i].x; memcpy(r + LSG_RRRAY_HIT_TRIANGLE, &tri, 4); }
+        if (flags & LSG_FILL_POINT2D) { float uv[2] = {u, v}; memcpy(r + LSG_RRRAY_HIT_POINT2D, uv, 8); }
+        if (flags & LSG_FILL_PLANE) { float pl[4] = {hp[i].x, hp[i].y, hp[i].z, hp[i].w}; memcpy(r + LSG_RRRAY_HIT_PLANE, pl, 16); }
+        if (flags & LSG_FILL_POINT3D) {
+            // IntersectLinear.cpp:40-45 update_hitPoint3d
+            float o[4], d[4]; memcpy(o, r + LSG_RRRAY_ORIGIN, 16); memcpy(d, r + LSG_RRRAY_DIR, 16);
+            float p[3] = { o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t };
+            memcpy(r + LSG_RRRAY_HIT_POINT3D, p, 12);
+        }
+        if (flags & LSG_FILL_SIDE) { bool fs = hf[i] != 0; memcpy(r + LSG_RRRAY_HIT_FRONTSIDE, &fs, 1); }
+    }
+    if (num_hits) *num_hits = hitsn;
+    return LSG_OK;
+}
+
+} // extern "C"

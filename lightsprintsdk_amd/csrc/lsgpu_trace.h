@@ -1,0 +1,112 @@
+// lsgpu_trace.h -- closest-hit BVH2 traversal for one ray per lane (gfx950).
+//
+// Near-child-first stack traversal over 64-byte nodes whose child boxes are
+// inflated by a DELTA_BSP-sized margin, so node culling is conservative and the
+// result depends only on the per-triangle SRLNP test (IntersectBspFast.cpp:189-252)
+// and the plane distance (:349-352): among accepted triangles with distance in
+// [box-clipped rayLengthMin, current best], the smallest distance wins, ties to the
+// lowest triangle index -- independent of traversal order.
+//
+// A Filter plays the role of RRCollisionHandler::collides (RRCollider.h:59): it
+// sees each candidate (triangle, side, distance) and may reject it, and it may end
+// the search early or narrow the interval once a hit is accepted.
+#pragma once
+#include "lsgpu_internal.h"
+
+namespace lsg {
+
+struct TraceResult { int tri; float t, u, v; bool front; uint32_t slot; };
+
+struct NoFilter {
+    __device__ bool accept(uint32_t, bool, float) const { return true; }
+    // after accepting a hit at t: true = stop searching
+    __device__ bool done(float) const { return false; }
+    // after accepting a hit: upper bound of the interval still worth searching
+    __device__ float clamp(float best) const { return best; }
+};
+
+__device__ inline void slab2(const Node& nd, const float3 o, const float3 idir, float tlo, float thi,
+                             float& n0, float& f0, float& n1, float& f1)
+{
+    float a, b, ta, tb, tc, td, te, tf;
+    a = (nd.q0.x - o.x) * idir.x; b = (nd.q0.y - o.x) * idir.x; ta = fminf(a, b); tb = fmaxf(a, b);
+    a = (nd.q0.z - o.y) * idir.y; b = (nd.q0.w - o.y) * idir.y; tc = fminf(a, b); td = fmaxf(a, b);
+    a = (nd.q1.x - o.z) * idir.z; b = (nd.q1.y - o.z) * idir.z; te = fminf(a, b); tf = fmaxf(a, b);
+    n0 = fmaxf(fmaxf(ta, tc), fmaxf(te, tlo)); f0 = fminf(fminf(tb, td), fminf(tf, thi));
+    a = (nd.q1.z - o.x) * idir.x; b = (nd.q1.w - o.x) * idir.x; ta = fminf(a, b); tb = fmaxf(a, b);
+    a = (nd.q2.x - o.y) * idir.y; b = (nd.q2.y - o.y) * idir.y; tc = fminf(a, b); td = fmaxf(a, b);
+    a = (nd.q2.z - o.z) * idir.z; b = (nd.q2.w - o.z) * idir.z; te = fminf(a, b); tf = fmaxf(a, b);
+    n1 = fmaxf(fmaxf(ta, tc), fmaxf(te, tlo)); f1 = fminf(fminf(tb, td), fminf(tf, thi));
+}
+
+__device__ inline float3 safe_inv(float3 d)
+{
+    float x = fabsf(d.x) < 1e-30f ? copysignf(1e-30f, d.x) : d.x;
+    float y = fabsf(d.y) < 1e-30f ? copysignf(1e-30f, d.y) : d.y;
+    float z = fabsf(d.z) < 1e-30f ? copysignf(1e-30f, d.z) : d.z;
+    return make_float3(1.0f / x, 1.0f / y, 1.0f / z);
+}
+
+// Closest accepted hit of one ray.  stk = this lane's first stack slot (LDS),
+// successive slots `stride` ints apart.  Returns false on stack overflow (cannot
+// happen when STACK > tree depth, which the host guarantees).
+template <int STACK, bool STATS, class Filter>
+__device__ inline bool trace_closest(const Filter& flt, const ColliderView& c, const float3 o, const float3 d,
+                                     float rmin, float rmax, int32_t* stk, int stride, TraceResult& res,
+                                     uint32_t& nvis, uint32_t& ntest)
+{
+    res.tri = -1;
+    float lo, hi;
+    if (!c.nvalid || !ref_box_intersect(c.bmin, c.bmax, o, d, rmin, rmax, lo, hi)) return true;
+    const float3 idir = safe_inv(d);
+    float best = hi, bound = hi;
+    int bi = -1; float bu = 0.f, bv = 0.f; bool bf = false; uint32_t bslot = 0;
+    int32_t ref = c.root;
+    int sp = 0;
+    bool ok = true;
+    for (;;) {
+        if (ref >= 0) {
+            if (STATS) nvis++;
+            const Node nd = c.nodes[ref];
+            float n0, f0, n1, f1;
+            slab2(nd, o, idir, lo, bound, n0, f0, n1, f1);
+            const int32_t r0 = __float_as_int(nd.q3.x), r1 = __float_as_int(nd.q3.y);
+            const bool h0 = n0 <= f0, h1 = n1 <= f1;
+            if (h0 && h1) {
+                const bool swap = n1 < n0;
+                if (sp < STACK) stk[(sp++) * stride] = swap ? r0 : r1; else ok = false;
+                ref = swap ? r1 : r0;
+                continue;
+            }
+            if (h0) { ref = r0; continue; }
+            if (h1) { ref = r1; continue; }
+        } else {
+            const uint32_t st = leaf_start(ref), cnt = leaf_count(ref);
+            bool stop = false;
+            for (uint32_t k = 0; k < cnt; k++) {
+                const uint32_t p = st + k;
+                if (STATS) ntest++;
+                const float4 pl = c.plane[p];
+                float ndd;
+                const float t = plane_distance(pl, o, d, ndd);
+                if (!(t >= lo && t <= bound)) continue;
+                const uint2 mt = c.meta[p];
+                if (!(bi < 0 || t < best || (int)mt.x < bi)) continue;
+                float u, v;
+                if (!srlnp_test(c.sv[p], c.ev[p], (int)mt.y, o, d, t, u, v)) continue;
+                const bool fr = ndd < 0;
+                if (!flt.accept(mt.x, fr, t)) continue;
+                best = t; bi = (int)mt.x; bu = u; bv = v; bf = fr; bslot = p;
+                bound = flt.clamp(best);
+                if (flt.done(t)) { stop = true; break; }
+            }
+            if (stop) break;
+        }
+        if (sp == 0) break;
+        ref = stk[(--sp) * stride];
+    }
+    if (bi >= 0) { res.tri = bi; res.t = best; res.u = bu; res.v = bv; res.front = bf; res.slot = bslot; }
+    return ok;
+}
+
+} // namespace lsg

@@ -1,0 +1,111 @@
+"""ctypes bindings of liblsgpu.so (the C ABI declared in include/lsgpu.h).
+
+The library is the product: there is no Python or CPU fallback.  Loading fails
+loudly when liblsgpu.so has not been built, and every compute entry point raises
+when the library reports an error (including "no gfx950 device")."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "liblsgpu.so")
+
+P = ctypes.c_void_p
+U32 = ctypes.c_uint32
+I32 = ctypes.c_int32
+
+
+class LsgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"liblsgpu error {code}: {msg}")
+        self.code = code
+
+
+class BuildParams(ctypes.Structure):
+    _fields_ = [("leaf_size", U32), ("flags", U32)]
+
+
+class ColliderInfo(ctypes.Structure):
+    _fields_ = [("num_triangles", U32), ("num_valid", U32), ("num_nodes", U32), ("root_ref", I32),
+                ("depth", U32), ("leaf_size", U32), ("box_min", ctypes.c_float * 3), ("box_max", ctypes.c_float * 3),
+                ("node_margin", ctypes.c_float), ("device_bytes", ctypes.c_uint64), ("build_ms", ctypes.c_double)]
+
+
+class Light(ctypes.Structure):
+    _fields_ = [("type", I32), ("position", ctypes.c_float * 3), ("direction", ctypes.c_float * 3),
+                ("color", ctypes.c_float * 3), ("cast_shadows", I32)]
+
+
+class GatherParams(ctypes.Structure):
+    _fields_ = [("quality", U32), ("hemisphere", I32), ("ray_length_min", ctypes.c_float),
+                ("locality", ctypes.c_float), ("rug_distance", ctypes.c_float), ("light_multiplier", ctypes.c_float),
+                ("seed", U32), ("reserved", U32)]
+
+
+# name -> (restype, argtypes); restype None means int status checked by check()
+SIGNATURES = {
+    "lsg_last_error": (ctypes.c_char_p, []),
+    "lsg_version": (ctypes.c_char_p, []),
+    "lsg_device_count": (ctypes.c_int, []),
+    "lsg_set_device": (ctypes.c_int, [ctypes.c_int]),
+    "lsg_malloc": (ctypes.c_int, [ctypes.POINTER(P), ctypes.c_size_t]),
+    "lsg_free": (ctypes.c_int, [P]),
+    "lsg_memcpy_h2d": (ctypes.c_int, [P, P, ctypes.c_size_t]),
+    "lsg_memcpy_d2h": (ctypes.c_int, [P, P, ctypes.c_size_t]),
+    "lsg_memset": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_size_t]),
+    "lsg_synchronize": (ctypes.c_int, []),
+    "lsg_stream_create": (ctypes.c_int, [ctypes.POINTER(P)]),
+    "lsg_stream_destroy": (ctypes.c_int, [P]),
+    "lsg_stream_synchronize": (ctypes.c_int, [P]),
+    "lsg_event_create": (ctypes.c_int, [ctypes.POINTER(P)]),
+    "lsg_event_destroy": (ctypes.c_int, [P]),
+    "lsg_event_record": (ctypes.c_int, [P, P]),
+    "lsg_event_elapsed_ms": (ctypes.c_int, [P, P, ctypes.POINTER(ctypes.c_float)]),
+    "lsg_collider_create": (ctypes.c_int, [P, U32, P, U32, ctypes.POINTER(BuildParams), P, ctypes.POINTER(P)]),
+    "lsg_collider_destroy": (None, [P]),
+    "lsg_collider_info_get": (ctypes.c_int, [P, ctypes.POINTER(ColliderInfo)]),
+    "lsg_collider_download": (ctypes.c_int, [P, P, P]),
+    "lsg_collider_intersect_rrrays": (ctypes.c_int, [P, P, U32, P, ctypes.POINTER(U32)]),
+    "lsg_collider_intersect_device": (ctypes.c_int, [P, P, P, U32, P, P, P]),
+    "lsg_collider_trace_stats": (ctypes.c_int, [P, P, P, U32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+    "lsg_scene_create": (ctypes.c_int, [P, P, P, P, P, U32, ctypes.POINTER(P)]),
+    "lsg_scene_destroy": (None, [P]),
+    "lsg_scene_set_exitance": (ctypes.c_int, [P, P]),
+    "lsg_scene_set_lights": (ctypes.c_int, [P, P, U32]),
+    "lsg_gather_device": (ctypes.c_int, [P, P, U32, P, U32, ctypes.POINTER(GatherParams), P, P]),
+    "lsg_rasterize_texels": (ctypes.c_int, [P, P, U32, U32, U32, U32, P, ctypes.POINTER(U32), P, ctypes.POINTER(U32), U32]),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads liblsgpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                               "(hipcc --offload-arch=gfx950); there is no CPU fallback")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    return (lib().lsg_last_error() or b"").decode(errors="replace")
+
+
+def check(code: int, what: str = "") -> None:
+    if code != 0:
+        raise LsgError(code, f"{what}: {last_error()}" if what else last_error())
+
+
+def device_count() -> int:
+    return int(lib().lsg_device_count())

@@ -1,0 +1,296 @@
+"""Seeded procedural inputs for the ray-cast and lightmap-bake workloads.
+
+Every generator is deterministic for a given seed (numpy PCG64, float64
+arithmetic, cast to float32 at the end), so the GPU path, the CPU oracle and the
+compiled reference collider all see bit-identical meshes and rays.
+
+Workloads (sizes as used by bench.py and the tests):
+  * displaced icosphere: icosahedron subdivided at frequency n (20*n^2 triangles;
+    n=71 gives 100,820), each shared vertex displaced radially by U(-0.1,0.1)*r.
+  * random rays: origins uniform in the 1.2x-scaled mesh box, directions
+    uniform on the unit sphere (normalised Gaussian triples), tmin=0, tmax=inf.
+  * box room: closed 10x10x4 m room + 20 axis-aligned boxes, faces tessellated
+    into grids, one point light and one emissive ceiling quad, per-object
+    albedo U(0.2,0.8), per-object lightmap unwrap (one chart per face, shelf
+    packed with a gutter).
+"""
+from __future__ import annotations
+
+import hashlib
+from dataclasses import dataclass, field
+
+import numpy as np
+
+ICOSPHERE_100K_FREQUENCY = 71   # 20 * 71^2 = 100,820 triangles
+
+
+def _icosahedron():
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0],
+                  [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+                  [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], dtype=np.float64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    f = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11],
+                  [1, 5, 9], [5, 11, 4], [11, 10, 2], [10, 7, 6], [7, 1, 8],
+                  [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9],
+                  [4, 9, 5], [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]], dtype=np.int64)
+    return v, f
+
+
+def icosphere(frequency: int, seed: int = 1, displacement: float = 0.1, radius: float = 1.0):
+    """Displaced geodesic sphere with 20*frequency^2 triangles.
+
+    Returns (verts float32 [nv,3], indices uint32 [nt,3]); triangles face outward."""
+    n = int(frequency)
+    if n < 1:
+        raise ValueError("frequency must be >= 1")
+    base_v, base_f = _icosahedron()
+    # barycentric grid per face: point (i,j) with i+j<=n
+    ii, jj = np.meshgrid(np.arange(n + 1), np.arange(n + 1), indexing="ij")
+    keep = ii + jj <= n
+    ii, jj = ii[keep], jj[keep]
+    local = {(int(a), int(b)): k for k, (a, b) in enumerate(zip(ii, jj))}
+    tris_local = []
+    for i in range(n):
+        for j in range(n - i):
+            tris_local.append((local[(i, j)], local[(i + 1, j)], local[(i, j + 1)]))
+            if i + j < n - 1:
+                tris_local.append((local[(i + 1, j)], local[(i + 1, j + 1)], local[(i, j + 1)]))
+    tris_local = np.array(tris_local, dtype=np.int64)
+    w0 = (n - ii - jj) / n
+    w1 = ii / n
+    w2 = jj / n
+    pts = []
+    faces = []
+    npf = len(ii)
+    for fi, (a, b, c) in enumerate(base_f):
+        p = w0[:, None] * base_v[a] + w1[:, None] * base_v[b] + w2[:, None] * base_v[c]
+        p /= np.linalg.norm(p, axis=1, keepdims=True)
+        pts.append(p)
+        faces.append(tris_local + fi * npf)
+    pts = np.concatenate(pts)
+    faces = np.concatenate(faces)
+    key = np.round(pts * 1e9).astype(np.int64)
+    _, uniq_idx, inverse = np.unique(key, axis=0, return_index=True, return_inverse=True)
+    verts = pts[uniq_idx]
+    inverse = inverse.reshape(-1)
+    faces = inverse[faces]
+    rng = np.random.default_rng(seed)
+    scale = 1.0 + rng.uniform(-displacement, displacement, size=len(verts))
+    verts = verts * (scale * radius)[:, None]
+    # orient outward
+    v0, v1, v2 = verts[faces[:, 0]], verts[faces[:, 1]], verts[faces[:, 2]]
+    nrm = np.cross(v1 - v0, v2 - v0)
+    flip = np.einsum("ij,ij->i", nrm, v0 + v1 + v2) < 0
+    faces[flip] = faces[flip][:, [0, 2, 1]]
+    return verts.astype(np.float32), faces.astype(np.uint32)
+
+
+def random_rays(verts: np.ndarray, n: int, seed: int = 2, box_scale: float = 1.2,
+                tmin: float = 0.0, tmax: float = np.inf):
+    """Rays as float32 [n,8] = (ox,oy,oz,tmin, dx,dy,dz,tmax)."""
+    rng = np.random.default_rng(seed)
+    lo = verts.min(axis=0).astype(np.float64)
+    hi = verts.max(axis=0).astype(np.float64)
+    c = (lo + hi) / 2
+    h = (hi - lo) / 2 * box_scale
+    o = c + rng.uniform(-1.0, 1.0, size=(n, 3)) * h
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = np.empty((n, 8), dtype=np.float32)
+    r[:, 0:3] = o
+    r[:, 3] = tmin
+    r[:, 4:7] = d
+    r[:, 7] = tmax
+    return r
+
+
+def mesh_digest(verts: np.ndarray, idx: np.ndarray) -> str:
+    h = hashlib.sha256()
+    h.update(np.ascontiguousarray(verts, dtype=np.float32).tobytes())
+    h.update(np.ascontiguousarray(idx, dtype=np.uint32).tobytes())
+    return h.hexdigest()
+
+
+# ---------------------------------------------------------------------------------------
+# lightmap scenes
+
+@dataclass
+class LightmapObject:
+    name: str
+    tri_begin: int
+    tri_count: int
+    width: int
+    height: int
+    albedo: tuple
+
+
+@dataclass
+class BakeScene:
+    verts: np.ndarray            # float32 [nv,3]
+    idx: np.ndarray              # uint32 [nt,3]
+    normals: np.ndarray          # float32 [nv,3] per-vertex normals
+    tri_uv: np.ndarray           # float32 [nt,6] lightmap uv per triangle corner
+    side_bits: np.ndarray        # uint8 [nt]
+    albedo: np.ndarray           # float32 [nt,3]
+    emittance: np.ndarray        # float32 [nt,3] diffuse emittance (front side)
+    objects: list = field(default_factory=list)   # LightmapObject for lightmapped objects
+    lights: list = field(default_factory=list)     # dicts: type, position, direction, color, cast_shadows
+
+    @property
+    def ntri(self):
+        return int(self.idx.shape[0])
+
+
+def _grid_face(origin, du, dv, nu, nv_):
+    """Quad origin + s*du + t*dv tessellated into nu x nv cells (2 tris each).
+    Returns local verts [(nu+1)(nv+1),3], tris [2*nu*nv,3], (s,t) per vertex."""
+    s = np.linspace(0.0, 1.0, nu + 1)
+    t = np.linspace(0.0, 1.0, nv_ + 1)
+    S, T = np.meshgrid(s, t, indexing="ij")
+    P = origin[None, None, :] + S[..., None] * du[None, None, :] + T[..., None] * dv[None, None, :]
+    verts = P.reshape(-1, 3)
+    st = np.stack([S.reshape(-1), T.reshape(-1)], axis=1)
+    tris = []
+    for i in range(nu):
+        for j in range(nv_):
+            a = i * (nv_ + 1) + j
+            b = (i + 1) * (nv_ + 1) + j
+            c = (i + 1) * (nv_ + 1) + j + 1
+            d = i * (nv_ + 1) + j + 1
+            tris.append((a, b, c))
+            tris.append((a, c, d))
+    return verts, np.array(tris, dtype=np.int64), st
+
+
+def _box_faces(lo, hi, inward):
+    """Six faces of an axis-aligned box as (origin, du, dv) with du x dv pointing
+    outward (or inward)."""
+    lo = np.asarray(lo, np.float64)
+    hi = np.asarray(hi, np.float64)
+    ex = hi - lo
+    X, Y, Z = np.eye(3)
+    faces = [
+        (np.array([lo[0], lo[1], lo[2]]), Y * ex[1], X * ex[0]),  # z- (normal -z)
+        (np.array([lo[0], lo[1], hi[2]]), X * ex[0], Y * ex[1]),  # z+
+        (np.array([lo[0], lo[1], lo[2]]), X * ex[0], Z * ex[2]),  # y- (normal -y)
+        (np.array([lo[0], hi[1], lo[2]]), Z * ex[2], X * ex[0]),  # y+
+        (np.array([lo[0], lo[1], lo[2]]), Z * ex[2], Y * ex[1]),  # x- (normal -x)
+        (np.array([hi[0], lo[1], lo[2]]), Y * ex[1], Z * ex[2]),  # x+
+    ]
+    if inward:
+        faces = [(o, dv, du) for (o, du, dv) in faces]
+    return faces
+
+
+def _pack_charts(sizes, gutter):
+    """Shelf-pack rectangles (w,h) (in relative units) into the unit square.
+    Returns list of (x,y,scale) so chart k occupies [x, x+w*scale] x [y, y+h*scale]."""
+    total = sum(w * h for w, h in sizes)
+    scale = 1.0 / np.sqrt(total) * 0.75
+    for _ in range(60):
+        x = y = gutter
+        row_h = 0.0
+        placed = []
+        ok = True
+        order = sorted(range(len(sizes)), key=lambda k: -sizes[k][1])
+        pos = [None] * len(sizes)
+        for k in order:
+            w, h = sizes[k][0] * scale, sizes[k][1] * scale
+            if x + w + gutter > 1.0:
+                x = gutter
+                y += row_h + gutter
+                row_h = 0.0
+            if x + w + gutter > 1.0 or y + h + gutter > 1.0:
+                ok = False
+                break
+            pos[k] = (x, y, scale)
+            x += w + gutter
+            row_h = max(row_h, h)
+        if ok:
+            return pos
+        scale *= 0.95
+    raise RuntimeError("chart packing failed")
+
+
+def box_room(seed: int = 3, n_boxes: int = 20, target_tris: int = 50_000, lightmap_size: int = 256,
+             room=(10.0, 10.0, 4.0), box_size=(0.2, 2.0), light_power: float = 40.0,
+             emitter_radiance: float = 8.0) -> BakeScene:
+    """Closed room + axis-aligned boxes, faces tessellated to about target_tris."""
+    rng = np.random.default_rng(seed)
+    RX, RY, RZ = room
+    objs = [("room", (0.0, 0.0, 0.0), (RX, RY, RZ), True)]
+    for b in range(n_boxes):
+        sz = rng.uniform(box_size[0], box_size[1], size=3)
+        sz[2] = min(sz[2], RZ * 0.8)
+        px = rng.uniform(0.2, RX - sz[0] - 0.2)
+        py = rng.uniform(0.2, RY - sz[1] - 0.2)
+        objs.append((f"box{b}", (px, py, 0.0), (px + sz[0], py + sz[1], sz[2]), False))
+    # tessellation density: cells per metre so that total triangles ~ target
+    area = 0.0
+    for _, lo, hi, _inw in objs:
+        e = np.subtract(hi, lo)
+        area += 2 * (e[0] * e[1] + e[1] * e[2] + e[0] * e[2])
+    cells_per_m2 = target_tris / 2.0 / area
+    density = np.sqrt(cells_per_m2)
+    V, I, UV, SB, ALB, EM, N = [], [], [], [], [], [], []
+    objects = []
+    nv = 0
+    for name, lo, hi, inward in objs:
+        faces = _box_faces(lo, hi, inward)
+        albedo = tuple(float(a) for a in rng.uniform(0.2, 0.8, size=3))
+        sizes = [(np.linalg.norm(du), np.linalg.norm(dv)) for _, du, dv in faces]
+        gutter = 3.0 / lightmap_size
+        pos = _pack_charts(sizes, gutter)
+        tri_begin = sum(len(x) for x in I)
+        for (o, du, dv), (cx, cy, sc), (w, h) in zip(faces, pos, sizes):
+            nu = max(1, int(round(w * density)))
+            nvv = max(1, int(round(h * density)))
+            fv, ft, st = _grid_face(o, du, dv, nu, nvv)
+            nrm = np.cross(du, dv)
+            nrm /= np.linalg.norm(nrm)
+            V.append(fv)
+            N.append(np.repeat(nrm[None, :], len(fv), axis=0))
+            I.append(ft + nv)
+            uv_v = np.stack([cx + st[:, 0] * w * sc, cy + st[:, 1] * h * sc], axis=1)
+            UV.append(uv_v[ft].reshape(-1, 6))
+            nv += len(fv)
+        count = sum(len(x) for x in I) - tri_begin
+        SB.append(np.full(count, 1 | 4, np.uint8))
+        ALB.append(np.repeat(np.array(albedo, np.float32)[None, :], count, axis=0))
+        EM.append(np.zeros((count, 3), np.float32))
+        objects.append(LightmapObject(name, tri_begin, count, lightmap_size, lightmap_size, albedo))
+    # emissive ceiling quad (not lightmapped), facing down, 1 cm below the ceiling
+    qx, qy = RX * 0.5, RY * 0.5
+    o = np.array([qx - 0.6, qy - 0.6, RZ - 0.01])
+    fv, ft, st = _grid_face(o, np.array([0.0, 1.2, 0.0]), np.array([1.2, 0.0, 0.0]), 2, 2)
+    V.append(fv)
+    N.append(np.repeat(np.array([[0.0, 0.0, -1.0]]), len(fv), axis=0))
+    I.append(ft + nv)
+    UV.append(np.zeros((len(ft), 6)))
+    SB.append(np.full(len(ft), 1 | 4, np.uint8))
+    ALB.append(np.full((len(ft), 3), 0.5, np.float32))
+    EM.append(np.full((len(ft), 3), emitter_radiance, np.float32))
+    nv += len(fv)
+    lights = [dict(type=0, position=(RX * 0.3, RY * 0.6, RZ * 0.75), direction=(0.0, 0.0, -1.0),
+                   color=(light_power, light_power * 0.9, light_power * 0.8), cast_shadows=1)]
+    return BakeScene(
+        verts=np.concatenate(V).astype(np.float32),
+        idx=np.concatenate(I).astype(np.uint32),
+        normals=np.concatenate(N).astype(np.float32),
+        tri_uv=np.concatenate(UV).astype(np.float32),
+        side_bits=np.concatenate(SB),
+        albedo=np.concatenate(ALB).astype(np.float32),
+        emittance=np.concatenate(EM).astype(np.float32),
+        objects=objects,
+        lights=lights,
+    )
+
+
+def minimal_safe_distance(verts: np.ndarray) -> float:
+    """RRSolver.cpp:333: (maxi-mini).avg()*1e-6 in float32."""
+    lo = verts.min(axis=0).astype(np.float32)
+    hi = verts.max(axis=0).astype(np.float32)
+    e = (hi - lo).astype(np.float32)
+    avg = np.float32((np.float32(e[0] + e[1]) + e[2]) * np.float32(0.33333333333333))
+    return float(np.float32(avg * np.float32(1e-6)))

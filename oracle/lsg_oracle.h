@@ -1,0 +1,98 @@
+/*
+ * lsg_oracle.h -- CPU oracle for the ray-cast + lightmap-gather hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: tests, the smoke
+ * entry point and the CPU-baseline leg of the benchmark may load it; the
+ * product (liblsgpu.so and the lightsprintsdk_amd package) never does.
+ *
+ * Every function restates one piece of LightsprintCore's algorithm in plain C
+ * (compiled with -ffp-contract=off, scalar SSE float, no -ffast-math), citing
+ * the reference file:line it follows.  Where it states pinning, the function is
+ * checked in tests/ against golden vectors produced by the compiled reference
+ * collider (oracle/_ref), closed forms or brute force.
+ *
+ * Pinning status (also recorded in DESIGN.md):
+ *   lsgo_srlnp_setup / lsgo_srlnp_hit  -- pinned: golden vectors from the
+ *       compiled IT_BSP_FASTER + HelloCollider closed-form KAT + FP64 check.
+ *   lsgo_box_*                          -- pinned: slab closed form + goldens.
+ *   lsgo_intersect_linear_srlnp         -- pinned: goldens vs IT_BSP_FASTER.
+ *   lsgo_intersect_mt                   -- pinned: goldens vs IT_LINEAR.
+ *   lsgo_bvh_traverse                   -- pinned: equals brute force.
+ *   lsgo_filler_*                       -- pinned: hand-derived KATs.
+ *   lsgo_gather_*                       -- pinned: closed forms (point light,
+ *       uniform-exitance enclosure, emitter form factor); reference bake is
+ *       unbuildable here (needs boost), so bit-level parity with
+ *       processTexel itself is "parity unpinned" beyond those closed forms.
+ */
+#ifndef LSG_ORACLE_H
+#define LSG_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Per-triangle precompute, field-for-field the TriangleSRLNP record of
+ * RRCollider/IntersectBspFast.h:33-49. byte==10 marks a degenerate triangle. */
+typedef struct lsgo_tri {
+    float n[4];      /* plane: normalised normal + w */
+    float s[3];      /* vertex0 */
+    float ireal;     /* 1/cross in the projection plane */
+    float r[3];      /* vertex1-vertex0 */
+    float ir[3];     /* 1/r */
+    float l[3];      /* vertex2-vertex0 */
+    int32_t byte;    /* 0..8 projection selector, 10 = degenerate */
+} lsgo_tri;
+
+/* One closest-hit result. tri = -1 on miss (other fields undefined). */
+typedef struct lsgo_hit {
+    int32_t tri;
+    float t, u, v;
+    float p[3];
+    float plane[4];
+    int32_t front;
+} lsgo_hit;
+
+/* Traversal statistics of lsgo_bvh_traverse (summed over rays). */
+typedef struct lsgo_stats {
+    double node_visits;   /* internal nodes whose child boxes were tested */
+    double tri_tests;     /* triangle plane+SRLNP tests */
+    double rays;
+} lsgo_stats;
+
+void lsgo_srlnp_setup(const float* v0, const float* v1, const float* v2, lsgo_tri* out);
+int  lsgo_srlnp_hit(const lsgo_tri* t, const float* o, const float* d, float tlo, float thi,
+                    float* t_out, float* u, float* v, int* front);
+void lsgo_mesh_aabb(const float* verts, uint32_t nverts, float* mn, float* mx);
+void lsgo_box_init(float* mn, float* mx);
+int  lsgo_box_intersect(const float* mn, const float* mx, const float* o, const float* d,
+                        float rmin, float rmax, float* lo, float* hi);
+float lsgo_delta_bsp(const float* mn, const float* mx);
+
+/* rays: 8 floats each (ox,oy,oz,tmin, dx,dy,dz,tmax). */
+void lsgo_intersect_linear_srlnp(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_t ntri,
+                                 const float* rays, uint32_t nrays, lsgo_hit* out, int threads);
+void lsgo_intersect_mt(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_t ntri,
+                       const float* rays, uint32_t nrays, lsgo_hit* out, int fp64, int threads);
+
+/* Traverses the BVH2 node layout built by liblsgpu (64-byte nodes, leaf refs
+ * with bit 31 set: start = (ref>>4)&0x7ffffff, count = (ref&15)+1) over the
+ * triangles in leaf order (prim[i] = original triangle index).  mesh gives the
+ * vertices; root_box is the inflated mesh box.  Fills out and stats. */
+void lsgo_bvh_traverse(const float* nodes, uint32_t nnodes, int32_t root_ref,
+                       const uint32_t* prim, uint32_t nprim,
+                       const float* verts, uint32_t nverts, const uint32_t* idx, uint32_t ntri,
+                       const float* rays, uint32_t nrays, lsgo_hit* out, lsgo_stats* stats, int threads);
+
+uint32_t lsgo_lbvh_build(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_t ntri,
+                         uint32_t leafmax, float* nodes_out, uint32_t* prim_out, int32_t* root_ref, int32_t* depth_out);
+
+/* HomogenousFiller2 (RRSolver/gather.cpp:57-111). */
+void  lsgo_filler_triangle_point(uint32_t n, float* a, float* b);
+uint32_t lsgo_filler_circle_point(uint32_t* n, float* a, float* b, float* dist);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,326 @@
+/*
+ * lsg_oracle_gather.c -- CPU oracle, lightmap final-gather part.  TEST
+ * INFRASTRUCTURE ONLY (see lsg_oracle.h): the checker, never the thing measured
+ * or shipped.
+ *
+ * Restates RRSolver/gather.cpp:600-836 processTexel (hemisphere gather
+ * :164-343, light gather :356-586, reliability loop :652-786) for opaque
+ * Lambertian scenes, with the collision-handler rules of
+ * RRStaticSolver/pathtracer.h:117-252 and the exitance lookup of
+ * RRStaticSolver/pathtracer.cpp:139-350 (solver shortcuts at depth 0).
+ *
+ * The reference draws texel jitter and filler seeds from the global C rand()
+ * (gather.h:102, gather.cpp:742-749), which is not reproducible across threads.
+ * This statement replaces exactly those draws with a seeded counter hash
+ * (lowbias32, C. Wellons) and picks the shooter's subtexel by a stratified
+ * closed form instead of the running area walk (gather.cpp:679-688); liblsgpu
+ * uses the identical replacements, so both sides shoot the same rays.
+ *
+ * Geometry (ray origins, directions, hit tests) is evaluated in binary32 in the
+ * reference's operation order, so hit decisions agree bit for bit with the GPU;
+ * radiometric sums are accumulated in binary64.
+ */
+#include "lsg_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef struct lsgo_light {
+    int32_t type;          /* 0 = point (REALISTIC attenuation), 1 = directional */
+    float pos[3];
+    float dir[3];
+    float color[3];
+    int32_t cast_shadows;
+} lsgo_light;
+
+typedef struct lsgo_gparams {
+    uint32_t quality;
+    int32_t hemisphere;        /* gather.cpp:197: env || emittance || useCurrentSolution */
+    float ray_length_min;      /* minimalSafeDistance, RRSolver.cpp:333 */
+    float locality;            /* RRSolver.h:584 default 100000 */
+    float rug_distance;        /* RRSolver.h:582 default 0.001 */
+    float light_multiplier;    /* direct.lightMultiplier */
+    uint32_t seed;
+    uint32_t reserved;
+} lsgo_gparams;
+
+typedef struct lsgo_gstats {
+    double hemi_rays, shadow_rays, series;
+} lsgo_gstats;
+
+static inline uint32_t lowbias32(uint32_t x)
+{
+    x ^= x >> 16; x *= 0x7feb352du; x ^= x >> 15; x *= 0x846ca68bu; x ^= x >> 16;
+    return x;
+}
+
+uint32_t lsgo_texel_key(uint32_t seed, uint32_t key_id) { return lowbias32(seed ^ lowbias32(key_id)); }
+
+/* seeded replacement of gather.cpp:742-749 (u,v in [0,1], u+v<=1) */
+void lsgo_jitter(uint32_t key, uint32_t i, float* u, float* v)
+{
+    uint32_t a = lowbias32(key + 0x9E3779B9u * (2u * i + 1u)) >> 8;
+    uint32_t b = lowbias32(key + 0x9E3779B9u * (2u * i + 2u)) >> 8;
+    const uint32_t M = 0xFFFFFFu;
+    if (a + b > M) { a = M - a; b = M - b; }
+    *u = (float)a / (float)M;
+    *v = (float)b / (float)M;
+}
+
+/* stratified subtexel choice for shooter s of n (replaces gather.cpp:679-688) */
+uint32_t lsgo_pick_subtexel(const float* subs, uint32_t count, float area_max, uint32_t s, uint32_t n)
+{
+    float pos = ((float)s + 0.5f) * area_max / (float)n;
+    uint32_t j = 0;
+    float acc = subs[7];
+    while (j + 1 < count && acc <= pos) { j++; acc += subs[8 * j + 7]; }
+    return j;
+}
+
+/* ---- ray casting with the final-gathering handler rules ------------------------ */
+typedef struct {
+    const float* verts; const uint32_t* idx; uint32_t ntri;
+    const lsgo_tri* tris; const uint8_t* side_mask;
+    const float* nodes; int32_t root_ref; const uint32_t* prim; uint32_t nprim;
+    float bmn[3], bmx[3];
+} scene_t;
+
+/* pathtracer.h:139-166: tri has all three vertices equal to the shooter's */
+static int same_location(const scene_t* sc, uint32_t tri, uint32_t shooter)
+{
+    const float* sv[3];
+    for (int i = 0; i < 3; i++) sv[i] = sc->verts + 3 * (size_t)sc->idx[3 * (size_t)shooter + i];
+    for (int i = 0; i < 3; i++) {
+        const float* hv = sc->verts + 3 * (size_t)sc->idx[3 * (size_t)tri + i];
+        int eq = 0;
+        for (int k = 0; k < 3; k++)
+            if (hv[0] == sv[k][0] && hv[1] == sv[k][1] && hv[2] == sv[k][2]) eq = 1;
+        if (!eq) return 0;
+    }
+    return 1;
+}
+
+/* mode 0 = hemisphere (accept rendered side), 1 = shadow (any shadow caster) */
+static int accept_hit(const scene_t* sc, int mode, uint32_t tri, int front, float t, uint32_t shooter, float tmin)
+{
+    if (tri == shooter) return 0;                                   /* pathtracer.h:133 */
+    if (t < 1000 * tmin && same_location(sc, tri, shooter)) return 0; /* pathtracer.h:139-166 */
+    uint8_t m = sc->side_mask[tri];
+    if (mode == 0) return (m & (front ? 1 : 2)) != 0;                /* TEST_BIT, renderFrom of hit side */
+    return (m & 4) != 0;                                             /* shadow caster, both sides [#45] */
+}
+
+static inline void slab6(const float* b, const float* o, const float* idir, float tlo, float thi, float* tn, float* tf)
+{
+    float a0 = (b[0] - o[0]) * idir[0], b0 = (b[1] - o[0]) * idir[0];
+    float a1 = (b[2] - o[1]) * idir[1], b1 = (b[3] - o[1]) * idir[1];
+    float a2 = (b[4] - o[2]) * idir[2], b2 = (b[5] - o[2]) * idir[2];
+    *tn = fmaxf(fmaxf(fminf(a0, b0), fminf(a1, b1)), fmaxf(fminf(a2, b2), tlo));
+    *tf = fminf(fminf(fmaxf(a0, b0), fmaxf(a1, b1)), fminf(fmaxf(a2, b2), thi));
+}
+
+static void consider(const scene_t* sc, int mode, uint32_t t, const float* o, const float* d, float lo, float tmin,
+                     uint32_t shooter, float* best, int* bi, int* bf)
+{
+    float th, u, v; int f;
+    if (!lsgo_srlnp_hit(&sc->tris[t], o, d, lo, *best, &th, &u, &v, &f)) return;
+    if (!accept_hit(sc, mode, t, f, th, shooter, tmin)) return;
+    if (*bi < 0 || th < *best || (th == *best && (int)t < *bi)) { *best = th; *bi = (int)t; *bf = f; }
+}
+
+/* Closest accepted hit in [tmin,tmax] after the mesh-box pretest
+ * (IntersectBspFast.cpp:816).  Returns tri or -1. */
+static int trace(const scene_t* sc, int mode, const float* o, const float* d, float tmin, float tmax,
+                 uint32_t shooter, float* t_out, int* front_out)
+{
+    float lo, hi;
+    if (!lsgo_box_intersect(sc->bmn, sc->bmx, o, d, tmin, tmax, &lo, &hi)) return -1;
+    float best = hi; int bi = -1, bf = 0;
+    if (!sc->nodes) {
+        for (uint32_t t = 0; t < sc->ntri; t++)
+            if (sc->tris[t].byte <= 8) consider(sc, mode, t, o, d, lo, tmin, shooter, &best, &bi, &bf);
+    } else if (sc->nprim) {
+        float idir[3];
+        for (int j = 0; j < 3; j++) { float dj = d[j]; if (fabsf(dj) < 1e-30f) dj = copysignf(1e-30f, dj); idir[j] = 1.0f / dj; }
+        int32_t stack[256]; int sp = 0; stack[sp++] = sc->root_ref;
+        while (sp) {
+            int32_t ref = stack[--sp];
+            if (ref < 0) {
+                uint32_t st = ((uint32_t)ref >> 4) & 0x7ffffffu, cnt = ((uint32_t)ref & 15u) + 1;
+                for (uint32_t k = 0; k < cnt; k++) consider(sc, mode, sc->prim[st + k], o, d, lo, tmin, shooter, &best, &bi, &bf);
+                continue;
+            }
+            const float* nd = sc->nodes + 16 * (size_t)ref;
+            int32_t r0, r1; memcpy(&r0, nd + 12, 4); memcpy(&r1, nd + 13, 4);
+            float n0, f0, n1, f1;
+            slab6(nd, o, idir, lo, best, &n0, &f0);
+            slab6(nd + 6, o, idir, lo, best, &n1, &f1);
+            int h0 = n0 <= f0, h1 = n1 <= f1;
+            if (h0 && h1) { if (n1 < n0) { stack[sp++] = r0; stack[sp++] = r1; } else { stack[sp++] = r1; stack[sp++] = r0; } }
+            else if (h0) stack[sp++] = r0;
+            else if (h1) stack[sp++] = r1;
+        }
+    }
+    if (bi >= 0) { *t_out = best; *front_out = bf; }
+    return bi;
+}
+
+/* ---- basis: gather.cpp:727-737, RRMesh.cpp:70-75, RRMathPrivate.h:48-58 ------- */
+static void subtexel_basis(const float* vn, const uint32_t* tidx, const float* sub, float* n, float* tg, float* bt)
+{
+    float cx = (sub[1] + sub[3] + sub[5]) * 0.333333333f;
+    float cy = (sub[2] + sub[4] + sub[6]) * 0.333333333f;
+    float w = 1 - cx - cy;
+    const float* n0 = vn + 3 * (size_t)tidx[0];
+    const float* n1 = vn + 3 * (size_t)tidx[1];
+    const float* n2 = vn + 3 * (size_t)tidx[2];
+    float m[3];
+    for (int j = 0; j < 3; j++) m[j] = n0[j] * w + n1[j] * cx + n2[j] * cy;
+    float len = sqrtf(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+    for (int j = 0; j < 3; j++) n[j] = m[j] / len;
+    float o[3];
+    if (fabsf(n[0]) > fabsf(n[2])) { o[0] = -n[1]; o[1] = n[0]; o[2] = 0; }
+    else { o[0] = 0; o[1] = -n[2]; o[2] = n[1]; }
+    float ol = sqrtf(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
+    for (int j = 0; j < 3; j++) tg[j] = o[j] / ol;
+    bt[0] = n[1] * tg[2] - n[2] * tg[1];
+    bt[1] = -n[0] * tg[2] + n[2] * tg[0];
+    bt[2] = n[0] * tg[1] - n[1] * tg[0];
+}
+
+/* Number of light shots among the first s shooters of a series of n shooters
+ * with r rounds, sequential rule of gather.cpp:777-783 (for r < n). */
+uint32_t lsgo_shots_before(uint32_t s, uint32_t n, uint32_t r)
+{
+    uint32_t c = 0;
+    for (uint32_t k = 0; k < s; k++)
+        if ((uint64_t)c * (n - 1) < (uint64_t)r * k) c++;
+    return c;
+}
+
+void lsgo_gather(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_t ntri,
+                 const float* vnormals, const uint8_t* side_mask, const float* exitance,
+                 const lsgo_light* lights, uint32_t nlights, const lsgo_gparams* p,
+                 const uint32_t* texels, uint32_t ntex, const float* subs,
+                 const float* nodes, int32_t root_ref, const uint32_t* prim, uint32_t nprim,
+                 double* out_rgba, lsgo_gstats* stats, int threads)
+{
+    scene_t sc;
+    sc.verts = verts; sc.idx = idx; sc.ntri = ntri; sc.side_mask = side_mask;
+    sc.nodes = nodes; sc.root_ref = root_ref; sc.prim = prim; sc.nprim = nprim;
+    lsgo_tri* tris = (lsgo_tri*)malloc(sizeof(lsgo_tri) * (ntri ? ntri : 1));
+    for (uint32_t t = 0; t < ntri; t++)
+        lsgo_srlnp_setup(verts + 3 * (size_t)idx[3 * t], verts + 3 * (size_t)idx[3 * t + 1], verts + 3 * (size_t)idx[3 * t + 2], &tris[t]);
+    sc.tris = tris;
+    lsgo_mesh_aabb(verts, nverts, sc.bmn, sc.bmx);
+    lsgo_box_init(sc.bmn, sc.bmx);
+
+    const uint32_t q = p->quality;
+    const uint32_t hemi_rays = p->hemisphere ? (q > 1 ? q : 1) : 0;                 /* gather.cpp:197 */
+    const uint32_t rounds = (p->light_multiplier != 0 && nlights) ? q / 10 + 1 : 0;  /* gather.cpp:396 */
+    const uint32_t light_rays = nlights * rounds;
+    double nh = 0, ns = 0, nser = 0;
+    int nt = 1;
+#ifdef _OPENMP
+    nt = threads > 0 ? threads : omp_get_max_threads();
+#endif
+    #pragma omp parallel for schedule(dynamic, 4) num_threads(nt) reduction(+:nh,ns,nser)
+    for (long ti = 0; ti < (long)ntex; ti++) {
+        const uint32_t* tx = texels + 4 * (size_t)ti;
+        uint32_t out_index = tx[0], sb = tx[1], sc_cnt = tx[2] & 0xffffu, flags = (tx[2] >> 16) & 0xffu;
+        const float* ts = subs + 8 * (size_t)sb;
+        double* out = out_rgba + 4 * (size_t)out_index;
+        out[0] = out[1] = out[2] = out[3] = 0;
+        if (!sc_cnt || (!hemi_rays && !light_rays)) continue;
+        uint32_t key = lsgo_texel_key(p->seed, tx[3]);
+        uint32_t filler = key & 0xffffu;
+        float area_max = 0;
+        for (uint32_t j = 0; j < sc_cnt; j++) area_max += ts[8 * j + 7];
+        double sumH[3] = {0, 0, 0}, sumL[3] = {0, 0, 0};
+        uint32_t relH = 0, relL = 0, unrelL = 0, shotRounds = 0, inside = 0;
+        uint32_t shooter_base = 0;
+        for (;;) {
+            int shootH = (relH < hemi_rays || relH * 10 < hemi_rays) && 0 < hemi_rays * 100;
+            int shootL = (relL + unrelL < light_rays || relL * 10 < light_rays) && unrelL < light_rays * 100;
+            if (!shootH && !shootL) break;
+            uint32_t n = (shootH ? hemi_rays : 0) > (shootL ? rounds : 0) ? (shootH ? hemi_rays : 0) : (shootL ? rounds : 0);
+            uint32_t shotH = 0, shotL = 0;
+            nser += 1;
+            for (uint32_t s = 0; s < n; s++) {
+                uint32_t j = lsgo_pick_subtexel(ts, sc_cnt, area_max, s, n);
+                const float* sub = ts + 8 * j;
+                uint32_t tri; memcpy(&tri, sub, 4);
+                const uint32_t* tidx = idx + 3 * (size_t)tri;
+                const float* v0 = verts + 3 * (size_t)tidx[0];
+                const float* v1 = verts + 3 * (size_t)tidx[1];
+                const float* v2 = verts + 3 * (size_t)tidx[2];
+                float bn[3], bt[3], bb[3];
+                subtexel_basis(vnormals, tidx, sub, bn, bt, bb);
+                float ju, jv; lsgo_jitter(key, shooter_base + s, &ju, &jv);
+                float ux = sub[1] + (sub[3] - sub[1]) * ju + (sub[5] - sub[1]) * jv;
+                float uy = sub[2] + (sub[4] - sub[2]) * ju + (sub[6] - sub[2]) * jv;
+                float o[3];
+                for (int c = 0; c < 3; c++) { float s1 = v1[c] - v0[c], s2 = v2[c] - v0[c]; o[c] = v0[c] + s1 * ux + s2 * uy; }
+                /* gather.cpp:759-768 */
+                if (shootH && (!shootL || hemi_rays >= rounds || (uint64_t)shotH * (n - 1) < (uint64_t)hemi_rays * s)) {
+                    shotH++;
+                    float x, y, dist;
+                    lsgo_filler_circle_point(&filler, &x, &y, &dist);
+                    float cosa = sqrtf(1 - dist);
+                    float d[3];
+                    for (int c = 0; c < 3; c++) d[c] = bn[c] * cosa + bt[c] * x + bb[c] * y;
+                    float th; int fr;
+                    int hit = trace(&sc, 0, o, d, p->ray_length_min, p->locality, tri, &th, &fr);
+                    nh += 1;
+                    if (hit >= 0) {
+                        const float* e = exitance + 4 * (2 * (size_t)hit + (fr ? 0 : 1));
+                        for (int c = 0; c < 3; c++) sumH[c] += e[c];
+                    }
+                    relH++;
+                }
+                /* gather.cpp:775-784 */
+                if (shootL && (!shootH || rounds >= hemi_rays || (uint64_t)shotL * (n - 1) < (uint64_t)rounds * s)) {
+                    shotL++;
+                    for (uint32_t li = 0; li < nlights; li++) {
+                        const lsgo_light* L = &lights[li];
+                        float d[3];
+                        if (L->type == 1) { d[0] = -L->dir[0]; d[1] = -L->dir[1]; d[2] = -L->dir[2]; }
+                        else { d[0] = L->pos[0] - o[0]; d[1] = L->pos[1] - o[1]; d[2] = L->pos[2] - o[2]; }
+                        float dsz = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                        d[0] /= dsz; d[1] /= dsz; d[2] /= dsz;
+                        if (L->type == 1) dsz *= p->locality;
+                        float ni = d[0] * bn[0] + d[1] * bn[1] + d[2] * bn[2];
+                        if (ni <= 0 || !isfinite(ni)) { relL++; continue; }    /* gather.cpp:430-437 */
+                        float th = 0; int fr;
+                        int blocked = L->cast_shadows ? (trace(&sc, 1, o, d, p->ray_length_min, dsz, tri, &th, &fr) >= 0) : 0;
+                        ns += L->cast_shadows ? 1 : 0;
+                        if (!blocked) {
+                            /* RRLight.cpp:345-362 getIrradiance */
+                            double att = 1.0;
+                            if (L->type == 0) {
+                                double dx = (double)o[0] - L->pos[0], dy = (double)o[1] - L->pos[1], dz = (double)o[2] - L->pos[2];
+                                att = 1.0 / (0.000000000137289 + (dx * dx + dy * dy + dz * dz));
+                            }
+                            for (int c = 0; c < 3; c++) sumL[c] += (double)L->color[c] * att * p->light_multiplier * ni;
+                            relL++;
+                        } else if (th < p->rug_distance) unrelL++;              /* gather.cpp:500-505 */
+                        else relL++;
+                    }
+                    shotRounds++;
+                }
+            }
+            shooter_base += n;
+        }
+        double rgb[3] = {0, 0, 0};
+        int reliable = 0;
+        if (hemi_rays && relH) { for (int c = 0; c < 3; c++) rgb[c] += sumH[c] / relH; reliable = 1; }
+        if (light_rays && relL && !(inside > light_rays)) { for (int c = 0; c < 3; c++) rgb[c] += sumL[c] / shotRounds; reliable = 1; }
+        out[0] = rgb[0]; out[1] = rgb[1]; out[2] = rgb[2];
+        out[3] = reliable ? flags / 255.0 : 0.0;
+    }
+    if (stats) { stats->hemi_rays += nh; stats->shadow_rays += ns; stats->series += nser; }
+    free(tris);
+}

@@ -1,0 +1,229 @@
+"""ctypes bindings of the CPU oracle (oracle/liblsg_oracle.so) and a runner for
+the compiled reference collider (oracle/_ref/ref_collider).
+
+Test infrastructure only: tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg use this module as the checker; the product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_LIB = os.path.join(ORACLE_DIR, "liblsg_oracle.so")
+REF_BIN = os.path.join(ORACLE_DIR, "_ref", "ref_collider")
+
+HIT_DTYPE = np.dtype([("tri", "<i4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("p", "<f4", 3),
+                      ("plane", "<f4", 4), ("front", "<i4")])
+TRI_DTYPE = np.dtype([("n", "<f4", 4), ("s", "<f4", 3), ("ireal", "<f4"), ("r", "<f4", 3),
+                      ("ir", "<f4", 3), ("l", "<f4", 3), ("byte", "<i4")])
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("node_visits", ctypes.c_double), ("tri_tests", ctypes.c_double), ("rays", ctypes.c_double)]
+
+
+class GParams(ctypes.Structure):
+    _fields_ = [("quality", ctypes.c_uint32), ("hemisphere", ctypes.c_int32), ("ray_length_min", ctypes.c_float),
+                ("locality", ctypes.c_float), ("rug_distance", ctypes.c_float), ("light_multiplier", ctypes.c_float),
+                ("seed", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class GStats(ctypes.Structure):
+    _fields_ = [("hemi_rays", ctypes.c_double), ("shadow_rays", ctypes.c_double), ("series", ctypes.c_double)]
+
+
+LIGHT_DTYPE = np.dtype([("type", "<i4"), ("pos", "<f4", 3), ("dir", "<f4", 3), ("color", "<f4", 3), ("cast_shadows", "<i4")])
+
+_lib = None
+
+
+def have_oracle() -> bool:
+    return os.path.exists(ORACLE_LIB)
+
+
+def build_oracle(ref: bool = True) -> None:
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+    if ref and os.path.isdir("/root/reference"):
+        subprocess.run(["make", "-s", "-j8", "-C", ORACLE_DIR, "ref"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not have_oracle():
+            build_oracle(ref=False)
+        L = ctypes.CDLL(ORACLE_LIB)
+        P = ctypes.c_void_p
+        U = ctypes.c_uint32
+        L.lsgo_srlnp_setup.argtypes = [P, P, P, P]
+        L.lsgo_srlnp_hit.argtypes = [P, P, P, ctypes.c_float, ctypes.c_float, P, P, P, P]
+        L.lsgo_srlnp_hit.restype = ctypes.c_int
+        L.lsgo_mesh_aabb.argtypes = [P, U, P, P]
+        L.lsgo_box_init.argtypes = [P, P]
+        L.lsgo_box_intersect.argtypes = [P, P, P, P, ctypes.c_float, ctypes.c_float, P, P]
+        L.lsgo_box_intersect.restype = ctypes.c_int
+        L.lsgo_delta_bsp.argtypes = [P, P]
+        L.lsgo_delta_bsp.restype = ctypes.c_float
+        L.lsgo_intersect_linear_srlnp.argtypes = [P, U, P, U, P, U, P, ctypes.c_int]
+        L.lsgo_intersect_mt.argtypes = [P, U, P, U, P, U, P, ctypes.c_int, ctypes.c_int]
+        L.lsgo_bvh_traverse.argtypes = [P, U, ctypes.c_int32, P, U, P, U, P, U, P, U, P, P, ctypes.c_int]
+        L.lsgo_lbvh_build.argtypes = [P, U, P, U, U, P, P, P, P]
+        L.lsgo_lbvh_build.restype = U
+        L.lsgo_filler_triangle_point.argtypes = [U, P, P]
+        L.lsgo_filler_circle_point.argtypes = [P, P, P, P]
+        L.lsgo_filler_circle_point.restype = U
+        L.lsgo_gather.argtypes = [P, U, P, U, P, P, P, P, U, P, P, U, P, P, ctypes.c_int32, P, U, P, P, ctypes.c_int]
+        L.lsgo_texel_key.argtypes = [U, U]
+        L.lsgo_texel_key.restype = U
+        L.lsgo_jitter.argtypes = [U, U, P, P]
+        L.lsgo_pick_subtexel.argtypes = [P, U, ctypes.c_float, U, U]
+        L.lsgo_pick_subtexel.restype = U
+        L.lsgo_shots_before.argtypes = [U, U, U]
+        L.lsgo_shots_before.restype = U
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _mesh(verts, idx):
+    v = np.ascontiguousarray(verts, dtype=np.float32)
+    i = np.ascontiguousarray(idx, dtype=np.uint32)
+    return v, i
+
+
+def srlnp_setup(v0, v1, v2):
+    out = np.zeros(1, TRI_DTYPE)
+    a, b, c = (np.ascontiguousarray(x, dtype=np.float32) for x in (v0, v1, v2))
+    lib().lsgo_srlnp_setup(_p(a), _p(b), _p(c), _p(out))
+    return out[0]
+
+
+def mesh_box(verts):
+    mn = np.zeros(3, np.float32)
+    mx = np.zeros(3, np.float32)
+    v = np.ascontiguousarray(verts, dtype=np.float32)
+    lib().lsgo_mesh_aabb(_p(v), len(v), _p(mn), _p(mx))
+    lib().lsgo_box_init(_p(mn), _p(mx))
+    return mn, mx
+
+
+def box_intersect(mn, mx, o, d, rmin, rmax):
+    lo = np.zeros(1, np.float32)
+    hi = np.zeros(1, np.float32)
+    mn, mx, o, d = (np.ascontiguousarray(x, dtype=np.float32) for x in (mn, mx, o, d))
+    r = lib().lsgo_box_intersect(_p(mn), _p(mx), _p(o), _p(d), rmin, rmax, _p(lo), _p(hi))
+    return bool(r), float(lo[0]), float(hi[0])
+
+
+def intersect_linear_srlnp(verts, idx, rays, threads=0):
+    v, i = _mesh(verts, idx)
+    r = np.ascontiguousarray(rays, dtype=np.float32)
+    out = np.zeros(len(r), HIT_DTYPE)
+    lib().lsgo_intersect_linear_srlnp(_p(v), len(v), _p(i), len(i), _p(r), len(r), _p(out), threads)
+    return out
+
+
+def intersect_mt(verts, idx, rays, fp64=False, threads=0):
+    v, i = _mesh(verts, idx)
+    r = np.ascontiguousarray(rays, dtype=np.float32)
+    out = np.zeros(len(r), HIT_DTYPE)
+    lib().lsgo_intersect_mt(_p(v), len(v), _p(i), len(i), _p(r), len(r), _p(out), 1 if fp64 else 0, threads)
+    return out
+
+
+def lbvh_build(verts, idx, leaf_size=4):
+    v, i = _mesh(verts, idx)
+    nt = len(i)
+    nodes = np.zeros((max(nt - 1, 1), 16), np.float32)
+    prim = np.zeros(max(nt, 1), np.uint32)
+    root = ctypes.c_int32(0)
+    depth = ctypes.c_int32(0)
+    n = lib().lsgo_lbvh_build(_p(v), len(v), _p(i), nt, leaf_size, _p(nodes), _p(prim),
+                              ctypes.byref(root), ctypes.byref(depth))
+    return dict(nodes=nodes[:max(n - 1, 0)], prim=prim[:n], root=int(root.value), depth=int(depth.value), nvalid=int(n))
+
+
+def bvh_traverse(tree, verts, idx, rays, threads=0):
+    v, i = _mesh(verts, idx)
+    r = np.ascontiguousarray(rays, dtype=np.float32)
+    nodes = np.ascontiguousarray(tree["nodes"], dtype=np.float32)
+    prim = np.ascontiguousarray(tree["prim"], dtype=np.uint32)
+    out = np.zeros(len(r), HIT_DTYPE)
+    st = Stats()
+    lib().lsgo_bvh_traverse(_p(nodes), len(nodes), tree["root"], _p(prim), len(prim), _p(v), len(v), _p(i), len(i),
+                            _p(r), len(r), _p(out), ctypes.byref(st), threads)
+    return out, st
+
+
+def filler_triangle_point(n):
+    a = np.zeros(1, np.float32)
+    b = np.zeros(1, np.float32)
+    lib().lsgo_filler_triangle_point(n, _p(a), _p(b))
+    return float(a[0]), float(b[0])
+
+
+# ---- compiled reference collider --------------------------------------------------------
+
+def have_reference() -> bool:
+    return os.path.exists(REF_BIN)
+
+
+def reference_intersect(verts, idx, rays, technique=3, threads=1, repeats=1, flags=None):
+    """Runs oracle/_ref/ref_collider; returns (hits, timing dict)."""
+    v, i = _mesh(verts, idx)
+    r = np.ascontiguousarray(rays, dtype=np.float32)
+    with tempfile.TemporaryDirectory() as td:
+        fin = os.path.join(td, "in.bin")
+        fout = os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(np.array([0x3152534C, len(v), len(i), len(r)], np.uint32).tobytes())
+            f.write(v.tobytes())
+            f.write(i.tobytes())
+            f.write(r.tobytes())
+        args = [REF_BIN, str(technique), fin, fout, str(threads), str(repeats)]
+        if flags is not None:
+            args.append(str(flags))
+        env = dict(os.environ)
+        env["OMP_NUM_THREADS"] = str(threads)
+        p = subprocess.run(args, check=True, capture_output=True, text=True, env=env)
+        timing = json.loads(p.stdout.strip().splitlines()[-1])
+        hits = np.fromfile(fout, dtype=HIT_DTYPE)
+    return hits, timing
+
+
+# ---- gather --------------------------------------------------------------------------------
+
+def gather(scene_arrays, params, texels, subs, tree=None, threads=0):
+    """scene_arrays: dict verts, idx, normals, side_bits, exitance[nt,8], lights (LIGHT_DTYPE array)."""
+    v, i = _mesh(scene_arrays["verts"], scene_arrays["idx"])
+    vn = np.ascontiguousarray(scene_arrays["normals"], dtype=np.float32)
+    sb = np.ascontiguousarray(scene_arrays["side_bits"], dtype=np.uint8)
+    ex = np.ascontiguousarray(scene_arrays["exitance"], dtype=np.float32)
+    lights = np.ascontiguousarray(scene_arrays["lights"], dtype=LIGHT_DTYPE)
+    tx = np.ascontiguousarray(texels, dtype=np.uint32)
+    sbs = np.ascontiguousarray(subs).view(np.float32)
+    nout = int(tx[:, 0].max()) + 1 if len(tx) else 0
+    out = np.zeros((max(nout, 1), 4), np.float64)
+    gp = GParams(params["quality"], params["hemisphere"], params["ray_length_min"], params["locality"],
+                 params["rug_distance"], params["light_multiplier"], params["seed"], 0)
+    gs = GStats()
+    if tree is not None:
+        nodes = np.ascontiguousarray(tree["nodes"], dtype=np.float32)
+        prim = np.ascontiguousarray(tree["prim"], dtype=np.uint32)
+        np_, pp_, root = _p(nodes), _p(prim), tree["root"]
+        nprim = len(prim)
+    else:
+        np_, pp_, root, nprim = None, None, 0, 0
+    lib().lsgo_gather(_p(v), len(v), _p(i), len(i), _p(vn), _p(sb), _p(ex), _p(lights), len(lights),
+                      ctypes.byref(gp), _p(tx), len(tx), _p(sbs), np_, root, pp_, nprim, _p(out), ctypes.byref(gs), threads)
+    return out, gs
