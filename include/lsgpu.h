@@ -195,6 +195,13 @@ int  lsg_gather_device(lsg_scene* scene, const lsg_texel* texels, uint32_t ntexe
                        const lsg_subtexel* subs, uint32_t nsubs, const lsg_gather_params* params,
                        float* out_rgba, void* stream);
 
+/* Same gather with traversal work counters (synchronous): stats6 = hemisphere rays,
+ * their internal-node visits, their triangle tests, shadow rays, node visits,
+ * triangle tests -- the inputs of the bytes-per-texel roofline figure. */
+int  lsg_gather_stats(lsg_scene* scene, const lsg_texel* texels, uint32_t ntexels,
+                      const lsg_subtexel* subs, uint32_t nsubs, const lsg_gather_params* params,
+                      float* out_rgba, double* stats6);
+
 /* Texel rasterisation of one object's lightmap, RRSolver/lightmap.cpp:83-335
  * (enumerateTexelsPartial, steps 1-2): clips every texel square against every
  * triangle of the object in its unwrap and emits SubTexels and texelFlags.

@@ -1,0 +1,347 @@
+"""Lightmap bake over liblsgpu: the RRSolver::updateLightmaps gather backend.
+
+Mirrors the reference's call chain for a static scene (RRSolver/lightmap.cpp:943-1163):
+  1. texel rasterisation of every lightmapped object   (lsg_rasterize_texels,
+     enumerateTexelsPartial, lightmap.cpp:83-335)
+  2. first gather of direct light per triangle          (lsg_gather_device with one
+     subtexel covering each triangle, gatherPerTrianglePhysical, gather.cpp:843-952,
+     at quality/2 as updateSolverIndirectIllumination does, gather.cpp:1085-1088)
+  3. exitance table = direct irradiance * albedo + emittance (what
+     PathtracerWorker::getIncidentRadiance reads at a hit, pathtracer.cpp:235-316;
+     the reference's radiosity bounce on top of it is not part of this path)
+  4. final gather of every texel                        (lsg_gather_device,
+     processTexel, gather.cpp:600-836)
+Everything between the host arrays and the RGBA texels runs on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes
+import time
+
+import numpy as np
+
+from . import native as N
+from .collider import RRCollider, DeviceBuffer
+from . import scenes as S
+
+TEXEL_DTYPE = np.dtype([("out_index", "<u4"), ("sub_begin", "<u4"), ("sub_count_flags", "<u4"), ("key", "<u4")])
+SUBTEXEL_DTYPE = np.dtype([("triangle", "<u4"), ("uv", "<f4", 6), ("area", "<f4")])
+LIGHT_DTYPE = np.dtype([("type", "<i4"), ("position", "<f4", 3), ("direction", "<f4", 3), ("color", "<f4", 3),
+                        ("cast_shadows", "<i4")])
+
+
+def rasterize(tri_uv: np.ndarray, tri_ids: np.ndarray, width: int, height: int, object_id: int, out_base: int = 0):
+    """SubTexels of one object's lightmap: (texels TEXEL_DTYPE, subtexels SUBTEXEL_DTYPE)."""
+    uv = np.ascontiguousarray(tri_uv, dtype=np.float32)
+    ids = np.ascontiguousarray(tri_ids, dtype=np.uint32)
+    nt = ctypes.c_uint32(0)
+    ns = ctypes.c_uint32(0)
+    L = N.lib()
+    N.check(L.lsg_rasterize_texels(uv.ctypes.data_as(ctypes.c_void_p), ids.ctypes.data_as(ctypes.c_void_p), len(ids),
+                                   width, height, object_id, None, ctypes.byref(nt), None, ctypes.byref(ns), out_base),
+            "rasterize(count)")
+    tex = np.zeros(nt.value, TEXEL_DTYPE)
+    sub = np.zeros(ns.value, SUBTEXEL_DTYPE)
+    N.check(L.lsg_rasterize_texels(uv.ctypes.data_as(ctypes.c_void_p), ids.ctypes.data_as(ctypes.c_void_p), len(ids),
+                                   width, height, object_id, tex.ctypes.data_as(ctypes.c_void_p), ctypes.byref(nt),
+                                   sub.ctypes.data_as(ctypes.c_void_p), ctypes.byref(ns), out_base), "rasterize")
+    return tex, sub
+
+
+def lights_array(lights) -> np.ndarray:
+    a = np.zeros(len(lights), LIGHT_DTYPE)
+    for k, l in enumerate(lights):
+        a[k]["type"] = l["type"]
+        a[k]["position"] = l.get("position", (0, 0, 0))
+        a[k]["direction"] = l.get("direction", (0, 0, -1))
+        a[k]["color"] = l["color"]
+        a[k]["cast_shadows"] = l.get("cast_shadows", 1)
+    return a
+
+
+class GatherParams:
+    """RRSolver::UpdateParameters(quality) defaults (RRSolver.h:570-592)."""
+
+    def __init__(self, quality=1024, hemisphere=1, ray_length_min=1e-6, locality=100000.0, rug_distance=0.001,
+                 light_multiplier=1.0, seed=1):
+        self.quality = quality
+        self.hemisphere = hemisphere
+        self.ray_length_min = ray_length_min
+        self.locality = locality
+        self.rug_distance = rug_distance
+        self.light_multiplier = light_multiplier
+        self.seed = seed
+
+    def c(self):
+        return N.GatherParams(self.quality, self.hemisphere, self.ray_length_min, self.locality, self.rug_distance,
+                              self.light_multiplier, self.seed, 0)
+
+    def as_dict(self):
+        return dict(quality=self.quality, hemisphere=self.hemisphere, ray_length_min=self.ray_length_min,
+                    locality=self.locality, rug_distance=self.rug_distance, light_multiplier=self.light_multiplier,
+                    seed=self.seed)
+
+
+class GatherScene:
+    """Collider + what the final gather reads at a hit, resident on the GPU."""
+
+    def __init__(self, verts, idx, normals, side_bits, exitance, lights, collider: RRCollider | None = None):
+        self.verts = np.ascontiguousarray(verts, np.float32)
+        self.idx = np.ascontiguousarray(idx, np.uint32)
+        self.col = collider or RRCollider.create(self.verts, self.idx)
+        self.normals = np.ascontiguousarray(normals, np.float32)
+        self.side_bits = np.ascontiguousarray(side_bits, np.uint8)
+        ex = np.ascontiguousarray(exitance, np.float32).reshape(len(self.idx), 8)
+        self.lights = lights_array(lights) if not isinstance(lights, np.ndarray) else np.ascontiguousarray(lights, LIGHT_DTYPE)
+        h = ctypes.c_void_p()
+        N.check(N.lib().lsg_scene_create(self.col._h, self.normals.ctypes.data_as(ctypes.c_void_p),
+                                         self.side_bits.ctypes.data_as(ctypes.c_void_p), ex.ctypes.data_as(ctypes.c_void_p),
+                                         self.lights.ctypes.data_as(ctypes.c_void_p), len(self.lights), ctypes.byref(h)),
+                "lsg_scene_create")
+        self._h = h.value
+
+    def set_exitance(self, exitance):
+        ex = np.ascontiguousarray(exitance, np.float32).reshape(len(self.idx), 8)
+        N.check(N.lib().lsg_scene_set_exitance(self._h, ex.ctypes.data_as(ctypes.c_void_p)), "set_exitance")
+
+    def set_lights(self, lights):
+        self.lights = lights_array(lights) if not isinstance(lights, np.ndarray) else np.ascontiguousarray(lights, LIGHT_DTYPE)
+        N.check(N.lib().lsg_scene_set_lights(self._h, self.lights.ctypes.data_as(ctypes.c_void_p), len(self.lights)),
+                "set_lights")
+
+    def gather_device(self, dtex: DeviceBuffer, ntex: int, dsub: DeviceBuffer, nsub: int, params: GatherParams,
+                      dout: DeviceBuffer, stream=None):
+        p = params.c()
+        N.check(N.lib().lsg_gather_device(self._h, dtex.ptr, ntex, dsub.ptr, nsub, ctypes.byref(p), dout.ptr, stream),
+                "lsg_gather_device")
+
+    def gather(self, texels, subs, params: GatherParams, nout=None):
+        """Host convenience: uploads texels/subtexels, gathers, returns float32 [nout,4]."""
+        if nout is None:
+            nout = int(texels["out_index"].max()) + 1 if len(texels) else 0
+        dt = DeviceBuffer.from_array(texels)
+        ds = DeviceBuffer.from_array(subs)
+        do = DeviceBuffer(16 * max(nout, 1))
+        N.check(N.lib().lsg_memset(do.ptr, 0, 16 * max(nout, 1)), "memset")
+        self.gather_device(dt, len(texels), ds, len(subs), params, do)
+        N.check(N.lib().lsg_synchronize(), "sync")
+        return do.download(np.float32, (nout, 4))
+
+    def oracle_arrays(self, exitance):
+        return dict(verts=self.verts, idx=self.idx, normals=self.normals, side_bits=self.side_bits,
+                    exitance=np.ascontiguousarray(exitance, np.float32).reshape(len(self.idx), 8),
+                    lights=np.array([(l["type"], l["position"], l["direction"], l["color"], l["cast_shadows"])
+                                     for l in self.lights], dtype=[("type", "<i4"), ("pos", "<f4", 3), ("dir", "<f4", 3),
+                                                                   ("color", "<f4", 3), ("cast_shadows", "<i4")]))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            N.lib().lsg_scene_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def triangle_texels(ntri: int, tri_ids=None):
+    """One texel per triangle whose single subtexel is the whole triangle: the
+    first-gather work unit of gatherPerTrianglePhysical (gather.cpp:905-940)."""
+    ids = np.arange(ntri, dtype=np.uint32) if tri_ids is None else np.asarray(tri_ids, np.uint32)
+    tex = np.zeros(len(ids), TEXEL_DTYPE)
+    tex["out_index"] = np.arange(len(ids))
+    tex["sub_begin"] = np.arange(len(ids))
+    tex["sub_count_flags"] = 1 | (255 << 16)
+    tex["key"] = 0x80000000 | ids
+    sub = np.zeros(len(ids), SUBTEXEL_DTYPE)
+    sub["triangle"] = ids
+    sub["uv"] = np.array([0, 0, 1, 0, 0, 1], np.float32)
+    sub["area"] = 1.0
+    return tex, sub
+
+
+class SceneBake:
+    """A whole BakeScene on the GPU: rasterised texels of every lightmapped object,
+    the exitance table and the per-object output slots."""
+
+    def __init__(self, scene: S.BakeScene, params: GatherParams | None = None):
+        self.scene = scene
+        self.params = params or GatherParams(ray_length_min=S.minimal_safe_distance(scene.verts))
+        nt = scene.ntri
+        ex0 = np.zeros((nt, 8), np.float32)
+        ex0[:, 0:3] = scene.emittance
+        self.gs = GatherScene(scene.verts, scene.idx, scene.normals, scene.side_bits, ex0, scene.lights)
+        texs, subs = [], []
+        base = 0
+        sub_base = 0
+        self.slots = []
+        for k, ob in enumerate(scene.objects):
+            ids = np.arange(ob.tri_begin, ob.tri_begin + ob.tri_count, dtype=np.uint32)
+            t, s = rasterize(scene.tri_uv[ob.tri_begin:ob.tri_begin + ob.tri_count], ids, ob.width, ob.height, k, base)
+            t["sub_begin"] += sub_base
+            texs.append(t)
+            subs.append(s)
+            self.slots.append((base, ob.width, ob.height))
+            base += ob.width * ob.height
+            sub_base += len(s)
+        self.texels = np.concatenate(texs) if texs else np.zeros(0, TEXEL_DTYPE)
+        self.subs = np.concatenate(subs) if subs else np.zeros(0, SUBTEXEL_DTYPE)
+        self.nout = base
+        self.d_tex = DeviceBuffer.from_array(self.texels)
+        self.d_sub = DeviceBuffer.from_array(self.subs)
+        self.d_out = DeviceBuffer(16 * max(self.nout, 1))
+        N.check(N.lib().lsg_memset(self.d_out.ptr, 0, 16 * max(self.nout, 1)), "memset")
+        ttex, tsub = triangle_texels(nt)
+        self.d_ttex = DeviceBuffer.from_array(ttex)
+        self.d_tsub = DeviceBuffer.from_array(tsub)
+        self.d_tout = DeviceBuffer(16 * max(nt, 1))
+        self.first_params = GatherParams(quality=max(self.params.quality // 2, 1), hemisphere=0,
+                                         ray_length_min=self.params.ray_length_min, locality=self.params.locality,
+                                         rug_distance=self.params.rug_distance,
+                                         light_multiplier=self.params.light_multiplier, seed=self.params.seed ^ 0x5bd1e995)
+        self.exitance = ex0
+
+    def first_gather(self):
+        """Per-triangle direct irradiance -> exitance table (uploaded to the scene)."""
+        nt = self.scene.ntri
+        self.gs.gather_device(self.d_ttex, nt, self.d_tsub, nt, self.first_params, self.d_tout)
+        N.check(N.lib().lsg_synchronize(), "sync")
+        e = self.d_tout.download(np.float32, (nt, 4))
+        ex = np.zeros((nt, 8), np.float32)
+        ex[:, 0:3] = e[:, 0:3] * self.scene.albedo + self.scene.emittance
+        self.exitance = ex
+        self.gs.set_exitance(ex)
+        return ex
+
+    def final_gather(self, stream=None):
+        self.gs.gather_device(self.d_tex, len(self.texels), self.d_sub, len(self.subs), self.params, self.d_out, stream)
+
+    def lightmaps_rgba(self):
+        N.check(N.lib().lsg_synchronize(), "sync")
+        out = self.d_out.download(np.float32, (max(self.nout, 1), 4))
+        maps = []
+        for base, w, h in self.slots:
+            maps.append(out[base:base + w * h].reshape(h, w, 4))
+        return maps
+
+
+def smoke_check(O):
+    """Tiny bake checked against the CPU oracle (used by __graft_entry__.smoke)."""
+    sc = S.box_room(seed=3, n_boxes=2, target_tris=600, lightmap_size=16)
+    sb = SceneBake(sc, GatherParams(quality=64, ray_length_min=S.minimal_safe_distance(sc.verts), seed=7))
+    sb.first_gather()
+    sb.final_gather()
+    N.check(N.lib().lsg_synchronize(), "sync")
+    gpu = sb.d_out.download(np.float32, (sb.nout, 4))
+    sel = np.arange(0, len(sb.texels), max(1, len(sb.texels) // 64))
+    tex = sb.texels[sel]
+    ref, _ = O.gather(sb.gs.oracle_arrays(sb.exitance), sb.params.as_dict(), tex.view(np.uint32).reshape(-1, 4),
+                      sb.subs, tree=None)
+    g = gpu[tex["out_index"]]
+    r = ref[tex["out_index"]]
+    err = np.abs(g - r) / np.maximum(np.abs(r), 1e-3)
+    assert np.all(err[:, :3] < 1e-4), err.max()
+    assert np.array_equal(g[:, 3], r[:, 3].astype(np.float32))
+    print(f"smoke gather ok: {len(tex)} texels vs oracle, max rel err {err.max():.2e}")
+
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "closest-hit Mrays/s; baked texels/s at \u22642% RMSE, plain vs learned gather"
+
+
+def config2_scene(lightmap_size=256):
+    """Config 2: closed 10x10x4 m room + 20 seeded boxes, ~50k triangles, 21 lightmaps."""
+    return S.box_room(seed=3, n_boxes=20, target_tris=50_000, lightmap_size=lightmap_size)
+
+
+def bench_workload(steps: int, warmup: int, cpu_threads: int):
+    """Plain-gather bake of config 2; one step = the final gather of every covered texel."""
+    sc = config2_scene()
+    t0 = time.perf_counter()
+    sb = SceneBake(sc, GatherParams(quality=1024, ray_length_min=S.minimal_safe_distance(sc.verts), seed=1))
+    setup_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    sb.first_gather()
+    first_s = time.perf_counter() - t0
+    L = N.lib()
+    s = ctypes.c_void_p(); e0 = ctypes.c_void_p(); e1 = ctypes.c_void_p()
+    N.check(L.lsg_stream_create(ctypes.byref(s)), "stream")
+    N.check(L.lsg_event_create(ctypes.byref(e0)), "event")
+    N.check(L.lsg_event_create(ctypes.byref(e1)), "event")
+    for _ in range(warmup):
+        sb.final_gather(s)
+    N.check(L.lsg_synchronize(), "sync")
+    t0 = time.perf_counter()
+    N.check(L.lsg_event_record(e0, s), "ev")
+    for _ in range(steps):
+        sb.final_gather(s)
+    N.check(L.lsg_event_record(e1, s), "ev")
+    N.check(L.lsg_synchronize(), "sync")
+    wall = time.perf_counter() - t0
+    ms = ctypes.c_float()
+    N.check(L.lsg_event_elapsed_ms(e0, e1, ctypes.byref(ms)), "elapsed")
+    kernel_ms = ms.value / steps
+    ntex = len(sb.texels)
+    texels_per_s = ntex / (wall / steps)
+    # traversal work of the same gather (separate counting launch)
+    st = (ctypes.c_double * 6)()
+    p = sb.params.c()
+    N.check(L.lsg_gather_stats(sb.gs._h, sb.d_tex.ptr, ntex, sb.d_sub.ptr, len(sb.subs), ctypes.byref(p), sb.d_out.ptr, st),
+            "gather_stats")
+    hr, hn, ht, sr, sn, stt = list(st)
+    b_ray = 48 + 64 * hn / hr + 48 * ht / hr
+    b_shadow = 48 + 64 * sn / max(sr, 1) + 48 * stt / max(sr, 1)
+    n_sub = len(sb.subs) / ntex
+    R = hr / ntex
+    Lr = sr / ntex
+    b_texel = R * (b_ray + 12) + Lr * b_shadow + 32 * n_sub + 12
+    achieved = b_texel * ntex / (kernel_ms * 1e-3) / 1e9
+    rays_per_s = (hr + sr) / (kernel_ms * 1e-3)
+    maps = sb.lightmaps_rgba()
+    lit = np.concatenate([m.reshape(-1, 4) for m in maps])[sb.texels["out_index"], :3]
+    res = dict(
+        metric=METRIC, value=round(texels_per_s, 1), unit="texels/s", ms_per_step=round(wall / steps * 1e3, 3),
+        dtype="f32",
+        config=dict(workload="plain lightmap bake: box room 10x10x4 m + 20 boxes, 21 lightmaps 256x256 RGBF, "
+                             "1024 hemisphere rays + 103 shadow rounds per texel (point light + emissive quad)",
+                    triangles=int(sc.ntri), lightmaps=len(sc.objects), texels=int(ntex), subtexels=int(len(sb.subs)),
+                    quality=1024, hemisphere_rays=int(hr), shadow_rays=int(sr), rays_per_s=round(rays_per_s, 1),
+                    mrays_per_s=round(rays_per_s / 1e6, 2), first_gather_s=round(first_s, 4), setup_s=round(setup_s, 3),
+                    mean_irradiance=float(lit.mean())),
+        roofline=dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+                      frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel="k_gather", kernel_ms=round(kernel_ms, 4),
+                      bytes_per_texel=round(b_texel, 1), bytes_per_hemisphere_ray=round(b_ray, 2),
+                      bytes_per_shadow_ray=round(b_shadow, 2), n_node_hemi=round(hn / hr, 3), n_tri_hemi=round(ht / hr, 3),
+                      n_node_shadow=round(sn / max(sr, 1), 3), n_tri_shadow=round(stt / max(sr, 1), 3),
+                      subtexels_per_texel=round(n_sub, 3)),
+    )
+    res["cpu_baseline"] = bake_cpu_baseline(sb, cpu_threads)
+    return res
+
+
+def bake_cpu_baseline(sb: "SceneBake", threads: int, target_s: float = 12.0):
+    """The CPU oracle's FP64 statement of processTexel (a port, the reference's
+    RRSolver needs boost and is not built here) on a bounded, evenly spread sample
+    of the same texels, all host threads."""
+    import os
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    sys.path.insert(0, os.path.join(root, "tests"))
+    import oracle_py as O
+    tree = sb.gs.col.download_tree()
+    arrays = sb.gs.oracle_arrays(sb.exitance)
+    n0 = 64 * threads
+    sel = np.linspace(0, len(sb.texels) - 1, n0).astype(np.int64)
+    t0 = time.perf_counter()
+    O.gather(arrays, sb.params.as_dict(), sb.texels[sel].view(np.uint32).reshape(-1, 4), sb.subs, tree=tree, threads=threads)
+    dt = time.perf_counter() - t0
+    n1 = int(min(len(sb.texels), max(n0, n0 * target_s / max(dt, 1e-3))))
+    sel = np.linspace(0, len(sb.texels) - 1, n1).astype(np.int64)
+    t0 = time.perf_counter()
+    O.gather(arrays, sb.params.as_dict(), sb.texels[sel].view(np.uint32).reshape(-1, 4), sb.subs, tree=tree, threads=threads)
+    dt = time.perf_counter() - t0
+    return dict(value=round(n1 / dt, 1), unit="texels/s", cores=threads, kind="port",
+                sample=f"{n1} of {len(sb.texels)} texels evenly spread, oracle C gather (FP64 sums) over the same LBVH, "
+                       f"{dt:.1f} s")

@@ -1,0 +1,456 @@
+// lsgpu_gather.hip -- per-texel lightmap final gather on gfx950.
+//
+// GPU statement of processTexel (RRSolver/gather.cpp:600-836) for opaque
+// Lambertian scenes: hemisphere gather of exitance (GatheredIrradianceHemisphere,
+// :164-343) plus shadow rays to point/directional lights
+// (GatheredIrradianceLights, :356-586) with the reliability loop of :652-786 and
+// the collision-handler rules of RRCollisionHandlerFinalGathering
+// (RRStaticSolver/pathtracer.h:117-252).
+//
+// Work mapping: one 64-lane wavefront per texel.  The texel's shooters (1024 at
+// quality 1024) are dealt to lanes 64 at a time; every lane casts the hemisphere
+// ray of its shooter and, on the shooters the reference's interleave rule picks,
+// one shadow ray per light.  Hemisphere directions come from the reference's
+// HomogenousFiller2 sequence: its rejection sampling is serial, so the wave
+// tests 64 candidate indices at once and hands accepted ones to lanes in order
+// with a ballot + popcount rank.  Sums are reduced across the wave with shuffles.
+//
+// The reference's rand() draws (filler seed gather.h:102, texel jitter
+// gather.cpp:742-749) become a seeded counter hash, and the subtexel walk
+// (:679-688) a stratified closed form; the CPU oracle uses the same rules.
+#include "lsgpu_internal.h"
+#include "lsgpu_trace.h"
+#include <vector>
+#include <cstring>
+
+namespace lsg {
+
+constexpr int kMaxLights = 16;
+
+struct LightsArg { lsg_light l[kMaxLights]; uint32_t n; };
+
+struct SceneView {
+    ColliderView col;
+    const float* verts;          // original vertex order, 3 floats each
+    const uint32_t* idx;         // 3 per triangle
+    const float* vnormal;        // 3 floats per vertex
+    const uint8_t* side;         // LSG_SIDE_* per triangle
+    const float4* exitance;      // 2 per triangle: front, back
+};
+
+__device__ inline uint32_t lowbias32(uint32_t x)
+{
+    x ^= x >> 16; x *= 0x7feb352du; x ^= x >> 15; x *= 0x846ca68bu; x ^= x >> 16;
+    return x;
+}
+
+// HomogenousFiller2::GetTrianglePoint (gather.cpp:66-82)
+__device__ inline void filler_point(uint32_t n, float& x, float& y)
+{
+    x = 0.f; y = 0.f;
+    float dist = 1.f;
+    while (n) {
+        const uint32_t k = n & 3u;
+        const float dx = (k == 2u) ? 0.86602540378444f : ((k == 3u) ? -0.86602540378444f : 0.f);
+        const float dy = (k == 0u) ? 0.f : ((k == 1u) ? 1.f : -0.5f);
+        const float dd = (k == 0u) ? -0.5f : 0.5f;
+        x += dist * dx;
+        y += dist * dy;
+        dist *= dd;
+        n >>= 2;
+    }
+}
+
+// seeded replacement of gather.cpp:742-749
+__device__ inline void jitter(uint32_t key, uint32_t i, float& u, float& v)
+{
+    uint32_t a = lowbias32(key + 0x9E3779B9u * (2u * i + 1u)) >> 8;
+    uint32_t b = lowbias32(key + 0x9E3779B9u * (2u * i + 2u)) >> 8;
+    const uint32_t M = 0xFFFFFFu;
+    if (a + b > M) { a = M - a; b = M - b; }
+    u = (float)a / (float)M;
+    v = (float)b / (float)M;
+}
+
+// shots among the first s shooters of a series of n with r rounds, r < n
+// (closed form of the counter rule in gather.cpp:763,779: shot k happens at
+// shooter floor(k*(n-1)/r)+1)
+__device__ inline uint32_t shots_before(uint32_t s, uint32_t n, uint32_t r)
+{
+    if (s == 0) return 0;
+    uint64_t c = ((uint64_t)(s - 1) * r + (n - 2)) / (n - 1);
+    return (uint32_t)(c < r ? c : r);
+}
+
+__device__ inline uint64_t lanemask_lt(uint32_t lane) { return lane ? (~0ull >> (64 - lane)) : 0ull; }
+
+// position of the k-th (0-based) set bit of m (k < popcount(m))
+__device__ inline uint32_t nth_set_bit(uint64_t m, uint32_t k)
+{
+    uint32_t lo = 0;
+    for (uint32_t step = 32; step; step >>= 1) {
+        uint32_t p = lo + step;
+        uint64_t below = p >= 64 ? m : (m & ((1ull << p) - 1ull));
+        if ((uint32_t)__popcll(below) <= k) lo = p;
+    }
+    return lo;
+}
+
+__device__ inline float wave_sum(float v)
+{
+    for (int o = 32; o; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+__device__ inline uint32_t wave_sum_u(uint32_t v)
+{
+    for (int o = 32; o; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// RRCollisionHandlerFinalGathering::collides (pathtracer.h:117-242) for opaque
+// materials: mode 0 = hemisphere (accept the rendered side), 1 = shadow ray
+// (any shadow caster, both sides [#45]).
+struct GatherFilter {
+    const float* verts; const uint32_t* idx; const uint8_t* side;
+    uint32_t shooter; float near_t; int mode; float rug_below;
+    __device__ bool same_location(uint32_t tri) const
+    {
+        const uint32_t* si = idx + 3 * (size_t)shooter;
+        const uint32_t* hi = idx + 3 * (size_t)tri;
+        for (int i = 0; i < 3; i++) {
+            const float* hv = verts + 3 * (size_t)hi[i];
+            bool eq = false;
+            for (int k = 0; k < 3; k++) {
+                const float* sv = verts + 3 * (size_t)si[k];
+                eq |= (hv[0] == sv[0]) & (hv[1] == sv[1]) & (hv[2] == sv[2]);
+            }
+            if (!eq) return false;
+        }
+        return true;
+    }
+    __device__ bool accept(uint32_t tri, bool front, float t) const
+    {
+        if (tri == shooter) return false;                              // pathtracer.h:133
+        if (t < near_t && same_location(tri)) return false;             // pathtracer.h:139-166
+        const uint32_t m = side[tri];
+        return mode == 0 ? ((m & (front ? 1u : 2u)) != 0) : ((m & 4u) != 0);
+    }
+    // shadow rays only need to know whether a blocker lies below rugDistance
+    __device__ bool done(float t) const { return mode == 1 && t <= rug_below; }
+    __device__ float clamp(float best) const { return mode == 1 ? fminf(best, rug_below) : best; }
+};
+
+struct GatherArgs {
+    SceneView sc;
+    const lsg_texel* texels; uint32_t ntex;
+    const lsg_subtexel* subs;
+    float4* out;
+    unsigned long long* stats;   // STATS builds: hemi rays, nodes, tris, shadow rays, nodes, tris
+    uint32_t quality; uint32_t hemi_rays; uint32_t rounds;
+    float tmin, locality, rug, light_mult; uint32_t seed;
+    LightsArg lights;
+};
+
+template <int STACK, bool STATS>
+__global__ __launch_bounds__(256) void k_gather(GatherArgs A)
+{
+    __shared__ int32_t stk[STACK * 256];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    const uint32_t ti = blockIdx.x * 4u + (tid >> 6);
+    if (ti >= A.ntex) return;                 // whole wavefront leaves together
+    int32_t* mystk = stk + tid;
+    const lsg_texel T = A.texels[ti];
+    const uint32_t m = T.sub_count_flags & 0xffffu, flags = (T.sub_count_flags >> 16) & 0xffu;
+    const uint32_t hemi_rays = A.hemi_rays, rounds = A.rounds, nl = A.lights.n;
+    const uint32_t light_rays = nl * rounds;
+    if (!m || (!hemi_rays && !light_rays)) {
+        if (lane == 0) A.out[T.out_index] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return;
+    }
+    const lsg_subtexel* ts = A.subs + T.sub_begin;
+    const uint32_t key = lowbias32(A.seed ^ lowbias32(T.key));
+    uint32_t filler = key & 0xffffu;           // wave-uniform
+    float area_max = 0.f;
+    for (uint32_t j = 0; j < m; j++) area_max += ts[j].area;
+    const float near_t = 1000 * A.tmin;
+    const float rug_below = __int_as_float(__float_as_int(A.rug) - 1);   // largest float < rugDistance (> 0)
+
+    float hr = 0.f, hg = 0.f, hb = 0.f, lr = 0.f, lg = 0.f, lb = 0.f;
+    uint32_t stH[3] = {0, 0, 0}, stS[3] = {0, 0, 0};
+    uint32_t relH = 0, relL = 0, unrelL = 0, shotRounds = 0;        // wave totals (uniform)
+    uint32_t shooter_base = 0;
+    for (;;) {
+        const bool shootH = (relH < hemi_rays || relH * 10 < hemi_rays) && hemi_rays > 0;
+        const bool shootL = (relL + unrelL < light_rays || relL * 10 < light_rays) && unrelL < light_rays * 100;
+        if (!shootH && !shootL) break;
+        const uint32_t nH = shootH ? hemi_rays : 0, nL = shootL ? rounds : 0;
+        const uint32_t n = nH > nL ? nH : nL;
+        const bool allH = !shootL || hemi_rays >= rounds;
+        const bool allL = !shootH || rounds >= hemi_rays;
+        uint32_t cRelH = 0, cRelL = 0, cUnrelL = 0, cRounds = 0;      // this lane's counts in the series
+        for (uint32_t base = 0; base < n; base += 64) {
+            const uint32_t s = base + lane;
+            const bool active = s < n;
+            bool doH = false, doL = false;
+            if (active && shootH) doH = allH || shots_before(s + 1, n, hemi_rays) > shots_before(s, n, hemi_rays);
+            if (active && shootL) doL = allL || shots_before(s + 1, n, rounds) > shots_before(s, n, rounds);
+            // hand out filler points: hemisphere shooters of this chunk take the next
+            // accepted filler indices in lane order (GetCirclePoint, gather.cpp:85-90)
+            const uint64_t hmask = __ballot(doH);
+            const uint32_t need = (uint32_t)__popcll(hmask);
+            const uint32_t myrank = (uint32_t)__popcll(hmask & lanemask_lt(lane));
+            uint32_t my_index = 0;
+            uint32_t got = 0;
+            while (got < need) {
+                const uint32_t cand = filler + lane;
+                float cx, cy;
+                filler_point(cand, cx, cy);
+                const bool ok = !(cx * cx + cy * cy >= 1.f);
+                const uint64_t am = __ballot(ok);
+                const uint32_t na = (uint32_t)__popcll(am);
+                const uint32_t want = doH ? myrank - got : 64u;
+                const uint32_t src = (doH && myrank >= got && want < na) ? nth_set_bit(am, want) : lane;
+                const uint32_t val = (uint32_t)__shfl((int)cand, (int)src);
+                if (doH && myrank >= got && want < na) my_index = val;
+                if (got + na <= need) { got += na; filler += 64u; }
+                else {
+                    const uint32_t last = nth_set_bit(am, need - got - 1u);
+                    filler = (uint32_t)__shfl((int)cand, (int)last) + 1u;
+                    got = need;
+                }
+            }
+            if (!(doH || doL)) continue;
+            // shooter position (stratified subtexel + seeded jitter) and basis
+            const float pos = ((float)s + 0.5f) * area_max / (float)n;
+            uint32_t j = 0;
+            float acc = ts[0].area;
+            while (j + 1 < m && acc <= pos) { j++; acc += ts[j].area; }
+            const lsg_subtexel st = ts[j];
+            const uint32_t tri = st.triangle;
+            const uint32_t i0 = A.sc.idx[3 * (size_t)tri], i1 = A.sc.idx[3 * (size_t)tri + 1], i2 = A.sc.idx[3 * (size_t)tri + 2];
+            const float* V = A.sc.verts;
+            const float3 v0 = make_float3(V[3 * (size_t)i0], V[3 * (size_t)i0 + 1], V[3 * (size_t)i0 + 2]);
+            const float3 v1 = make_float3(V[3 * (size_t)i1], V[3 * (size_t)i1 + 1], V[3 * (size_t)i1 + 2]);
+            const float3 v2 = make_float3(V[3 * (size_t)i2], V[3 * (size_t)i2 + 1], V[3 * (size_t)i2 + 2]);
+            // gather.cpp:727-737 basis from interpolated vertex normals at the subtexel centroid
+            const float cxs = (st.uv[0] + st.uv[2] + st.uv[4]) * 0.333333333f;
+            const float cys = (st.uv[1] + st.uv[3] + st.uv[5]) * 0.333333333f;
+            const float w = 1 - cxs - cys;
+            const float* NV = A.sc.vnormal;
+            float nx = NV[3 * (size_t)i0] * w + NV[3 * (size_t)i1] * cxs + NV[3 * (size_t)i2] * cys;
+            float ny = NV[3 * (size_t)i0 + 1] * w + NV[3 * (size_t)i1 + 1] * cxs + NV[3 * (size_t)i2 + 1] * cys;
+            float nz = NV[3 * (size_t)i0 + 2] * w + NV[3 * (size_t)i1 + 2] * cxs + NV[3 * (size_t)i2 + 2] * cys;
+            const float nlen = sqrtf(nx * nx + ny * ny + nz * nz);
+            nx = nx / nlen; ny = ny / nlen; nz = nz / nlen;
+            float ox, oy, oz;                                               // RRMathPrivate.h:48-53
+            if (fabsf(nx) > fabsf(nz)) { ox = -ny; oy = nx; oz = 0.f; } else { ox = 0.f; oy = -nz; oz = ny; }
+            const float olen = sqrtf(ox * ox + oy * oy + oz * oz);
+            const float tx = ox / olen, ty = oy / olen, tz = oz / olen;
+            const float bx = ny * tz - nz * ty, by = -nx * tz + nz * tx, bz = nx * ty - ny * tx;
+            float ju, jv;
+            jitter(key, shooter_base + s, ju, jv);
+            const float ux = st.uv[0] + (st.uv[2] - st.uv[0]) * ju + (st.uv[4] - st.uv[0]) * jv;
+            const float uy = st.uv[1] + (st.uv[3] - st.uv[1]) * ju + (st.uv[5] - st.uv[1]) * jv;
+            const float3 o = make_float3(v0.x + (v1.x - v0.x) * ux + (v2.x - v0.x) * uy,
+                                         v0.y + (v1.y - v0.y) * ux + (v2.y - v0.y) * uy,
+                                         v0.z + (v1.z - v0.z) * ux + (v2.z - v0.z) * uy);
+            GatherFilter flt{A.sc.verts, A.sc.idx, A.sc.side, tri, near_t, 0, rug_below};
+            if (doH) {
+                float x, y;
+                filler_point(my_index, x, y);
+                const float dist = x * x + y * y;
+                const float cosa = sqrtf(1 - dist);                          // gather.cpp:125-126
+                const float3 d = make_float3(nx * cosa + tx * x + bx * y, ny * cosa + ty * x + by * y, nz * cosa + tz * x + bz * y);
+                TraceResult r;
+                uint32_t nv_ = 0, nt_ = 0;
+                trace_closest<STACK, STATS>(flt, A.sc.col, o, d, A.tmin, A.locality, mystk, 256, r, nv_, nt_);
+                if (STATS) { stH[0]++; stH[1] += nv_; stH[2] += nt_; }
+                if (r.tri >= 0) {
+                    const float4 e = A.sc.exitance[2 * (size_t)r.tri + (r.front ? 0 : 1)];
+                    hr += e.x; hg += e.y; hb += e.z;
+                }
+                cRelH++;
+            }
+            if (doL) {
+                flt.mode = 1;
+                for (uint32_t li = 0; li < nl; li++) {
+                    const lsg_light L = A.lights.l[li];
+                    float dx, dy, dz;
+                    if (L.type == 1) { dx = -L.direction[0]; dy = -L.direction[1]; dz = -L.direction[2]; }
+                    else { dx = L.position[0] - o.x; dy = L.position[1] - o.y; dz = L.position[2] - o.z; }
+                    float dsz = sqrtf(dx * dx + dy * dy + dz * dz);
+                    dx = dx / dsz; dy = dy / dsz; dz = dz / dsz;
+                    if (L.type == 1) dsz = dsz * A.locality;
+                    const float ni = dx * nx + dy * ny + dz * nz;            // gather.cpp:429
+                    if (ni <= 0 || !isfinite(ni)) { cRelL++; continue; }
+                    bool blocked = false; float bt = 0.f;
+                    if (L.cast_shadows) {
+                        TraceResult r;
+                        uint32_t nv_ = 0, nt_ = 0;
+                        trace_closest<STACK, STATS>(flt, A.sc.col, o, make_float3(dx, dy, dz), A.tmin, dsz, mystk, 256, r, nv_, nt_);
+                        if (STATS) { stS[0]++; stS[1] += nv_; stS[2] += nt_; }
+                        blocked = r.tri >= 0; bt = r.t;
+                    }
+                    if (!blocked) {
+                        // RRLight::getIrradiance (RRLight.cpp:345-362), REALISTIC attenuation
+                        float att = 1.f;
+                        if (L.type == 0) {
+                            const float ex = o.x - L.position[0], ey = o.y - L.position[1], ez = o.z - L.position[2];
+                            att = 1 / (0.000000000137289f + (ex * ex + ey * ey + ez * ez));
+                        }
+                        const float k = att * A.light_mult * ni;
+                        lr += L.color[0] * k; lg += L.color[1] * k; lb += L.color[2] * k;
+                        cRelL++;
+                    } else if (bt < A.rug) cUnrelL++;                        // gather.cpp:500-505
+                    else cRelL++;
+                }
+                cRounds++;
+            }
+        }
+        relH += wave_sum_u(cRelH); relL += wave_sum_u(cRelL); unrelL += wave_sum_u(cUnrelL);
+        shotRounds += wave_sum_u(cRounds);
+        shooter_base += n;
+    }
+    if (STATS) {
+        for (int k = 0; k < 3; k++) {
+            atomicAdd(&A.stats[k], (unsigned long long)stH[k]);
+            atomicAdd(&A.stats[3 + k], (unsigned long long)stS[k]);
+        }
+    }
+    hr = wave_sum(hr); hg = wave_sum(hg); hb = wave_sum(hb);
+    lr = wave_sum(lr); lg = wave_sum(lg); lb = wave_sum(lb);
+    if (lane == 0) {
+        float r = 0.f, g = 0.f, b = 0.f;
+        bool reliable = false;
+        if (hemi_rays && relH) { const float f = 1 / (float)relH; r += hr * f; g += hg * f; b += hb * f; reliable = true; }
+        if (light_rays && relL) { r += lr / (float)shotRounds; g += lg / (float)shotRounds; b += lb / (float)shotRounds; reliable = true; }
+        A.out[T.out_index] = make_float4(r, g, b, reliable ? flags / 255.f : 0.f);
+    }
+}
+
+struct SceneImpl {
+    const lsg_collider* col = nullptr;
+    float* d_vnormal = nullptr;
+    uint8_t* d_side = nullptr;
+    float4* d_exit = nullptr;
+    LightsArg lights{};
+};
+
+static inline void dfree(void* p) { if (p) (void)hipFree(p); }
+
+} // namespace lsg
+
+struct lsg_scene { lsg::SceneImpl impl; };
+
+using namespace lsg;
+
+extern "C" {
+
+int lsg_scene_set_exitance(lsg_scene* s, const float* exitance)
+{
+    if (!s || !exitance) return fail(LSG_ERR_INVALID, "lsg_scene_set_exitance: null argument");
+    const uint32_t nt = s->impl.col->impl.ntri;
+    LSG_HIP(hipMemcpy(s->impl.d_exit, exitance, 32 * (size_t)nt, hipMemcpyHostToDevice));
+    return LSG_OK;
+}
+
+int lsg_scene_set_lights(lsg_scene* s, const lsg_light* lights, uint32_t n)
+{
+    if (!s || (n && !lights)) return fail(LSG_ERR_INVALID, "lsg_scene_set_lights: null argument");
+    if (n > (uint32_t)kMaxLights) return fail(LSG_ERR_LIMIT, "lsg_scene_set_lights: %u lights, at most %d", n, kMaxLights);
+    for (uint32_t i = 0; i < n; i++)
+        if (lights[i].type != 0 && lights[i].type != 1) return fail(LSG_ERR_INVALID, "light %u: type must be 0 (point) or 1 (directional)", i);
+    memset(&s->impl.lights, 0, sizeof s->impl.lights);
+    if (n) memcpy(s->impl.lights.l, lights, sizeof(lsg_light) * n);
+    s->impl.lights.n = n;
+    return LSG_OK;
+}
+
+int lsg_scene_create(const lsg_collider* col, const float* vertex_normals, const uint8_t* side_bits, const float* exitance,
+                     const lsg_light* lights, uint32_t nlights, lsg_scene** out)
+{
+    if (!out) return fail(LSG_ERR_INVALID, "lsg_scene_create: out is null");
+    *out = nullptr;
+    if (!col || !vertex_normals || !side_bits || !exitance) return fail(LSG_ERR_INVALID, "lsg_scene_create: null argument");
+    lsg_scene* s = new (std::nothrow) lsg_scene;
+    if (!s) return fail(LSG_ERR_OOM, "lsg_scene_create: host allocation");
+    SceneImpl& S = s->impl;
+    S.col = col;
+    const uint32_t nv = col->impl.nverts, nt = col->impl.ntri;
+    auto bail = [&](int st) { dfree(S.d_vnormal); dfree(S.d_side); dfree(S.d_exit); delete s; return st; };
+    hipError_t e;
+    if ((e = hipMalloc(&S.d_vnormal, 12 * (size_t)(nv ? nv : 1))) != hipSuccess) return bail(hip_fail(e, "hipMalloc", __LINE__));
+    if ((e = hipMalloc(&S.d_side, (size_t)(nt ? nt : 1))) != hipSuccess) return bail(hip_fail(e, "hipMalloc", __LINE__));
+    if ((e = hipMalloc(&S.d_exit, 32 * (size_t)(nt ? nt : 1))) != hipSuccess) return bail(hip_fail(e, "hipMalloc", __LINE__));
+    if ((e = hipMemcpy(S.d_vnormal, vertex_normals, 12 * (size_t)nv, hipMemcpyHostToDevice)) != hipSuccess) return bail(hip_fail(e, "h2d", __LINE__));
+    if ((e = hipMemcpy(S.d_side, side_bits, (size_t)nt, hipMemcpyHostToDevice)) != hipSuccess) return bail(hip_fail(e, "h2d", __LINE__));
+    if ((e = hipMemcpy(S.d_exit, exitance, 32 * (size_t)nt, hipMemcpyHostToDevice)) != hipSuccess) return bail(hip_fail(e, "h2d", __LINE__));
+    int st = lsg_scene_set_lights(s, lights, nlights);
+    if (st) return bail(st);
+    *out = s;
+    return LSG_OK;
+}
+
+void lsg_scene_destroy(lsg_scene* s)
+{
+    if (!s) return;
+    dfree(s->impl.d_vnormal); dfree(s->impl.d_side); dfree(s->impl.d_exit);
+    delete s;
+}
+
+static int gather_launch(lsg_scene* s, const lsg_texel* texels, uint32_t ntexels, const lsg_subtexel* subs, uint32_t nsubs,
+                         const lsg_gather_params* p, float* out_rgba, void* stream, unsigned long long* stats)
+{
+    if (!s || !p) return fail(LSG_ERR_INVALID, "lsg_gather_device: null argument");
+    if (!ntexels) return LSG_OK;
+    if (!texels || !subs || !out_rgba || !nsubs) return fail(LSG_ERR_INVALID, "lsg_gather_device: null buffer");
+    const ColliderImpl& C = s->impl.col->impl;
+    const int stack = stack_size_for_depth(C.depth);
+    if (!stack) return fail(LSG_ERR_LIMIT, "tree too deep");
+    if (!(p->ray_length_min >= 0) || !(p->rug_distance > 0) || !(p->locality > 0))
+        return fail(LSG_ERR_INVALID, "lsg_gather_device: ray_length_min >= 0, rug_distance > 0 and locality > 0 required");
+    GatherArgs A;
+    A.sc.col = C.view();
+    A.sc.verts = C.d_verts; A.sc.idx = C.d_idx;
+    A.sc.vnormal = s->impl.d_vnormal; A.sc.side = s->impl.d_side; A.sc.exitance = s->impl.d_exit;
+    A.texels = texels; A.ntex = ntexels; A.subs = subs; A.out = (float4*)out_rgba; A.stats = stats;
+    A.quality = p->quality;
+    A.hemi_rays = p->hemisphere ? (p->quality > 1 ? p->quality : 1) : 0;           // gather.cpp:197
+    A.rounds = (p->light_multiplier != 0 && s->impl.lights.n) ? p->quality / 10 + 1 : 0;  // gather.cpp:396
+    A.tmin = p->ray_length_min; A.locality = p->locality; A.rug = p->rug_distance;
+    A.light_mult = p->light_multiplier; A.seed = p->seed;
+    A.lights = s->impl.lights;
+    const unsigned grid = (ntexels + 3) / 4;
+    if (stats) {
+        if (stack <= 32) k_gather<32, true><<<grid, 256, 0, (hipStream_t)stream>>>(A);
+        else k_gather<64, true><<<grid, 256, 0, (hipStream_t)stream>>>(A);
+    } else {
+        if (stack <= 32) k_gather<32, false><<<grid, 256, 0, (hipStream_t)stream>>>(A);
+        else k_gather<64, false><<<grid, 256, 0, (hipStream_t)stream>>>(A);
+    }
+    LSG_HIP(hipGetLastError());
+    return LSG_OK;
+}
+
+int lsg_gather_device(lsg_scene* s, const lsg_texel* texels, uint32_t ntexels, const lsg_subtexel* subs, uint32_t nsubs,
+                      const lsg_gather_params* p, float* out_rgba, void* stream)
+{
+    return gather_launch(s, texels, ntexels, subs, nsubs, p, out_rgba, stream, nullptr);
+}
+
+int lsg_gather_stats(lsg_scene* s, const lsg_texel* texels, uint32_t ntexels, const lsg_subtexel* subs, uint32_t nsubs,
+                     const lsg_gather_params* p, float* out_rgba, double* stats6)
+{
+    if (!stats6) return fail(LSG_ERR_INVALID, "lsg_gather_stats: null stats");
+    unsigned long long* d = nullptr;
+    LSG_HIP(hipMalloc(&d, 6 * sizeof(unsigned long long)));
+    hipError_t e = hipMemset(d, 0, 6 * sizeof(unsigned long long));
+    int st = e == hipSuccess ? gather_launch(s, texels, ntexels, subs, nsubs, p, out_rgba, nullptr, d) : hip_fail(e, "memset", __LINE__);
+    unsigned long long h[6] = {0, 0, 0, 0, 0, 0};
+    if (!st && (e = hipMemcpy(h, d, sizeof h, hipMemcpyDeviceToHost)) != hipSuccess) st = hip_fail(e, "d2h", __LINE__);
+    dfree(d);
+    for (int k = 0; k < 6; k++) stats6[k] = (double)h[k];
+    return st;
+}
+
+} // extern "C"

@@ -74,6 +74,7 @@ SIGNATURES = {
     "lsg_scene_set_exitance": (ctypes.c_int, [P, P]),
     "lsg_scene_set_lights": (ctypes.c_int, [P, P, U32]),
     "lsg_gather_device": (ctypes.c_int, [P, P, U32, P, U32, ctypes.POINTER(GatherParams), P, P]),
+    "lsg_gather_stats": (ctypes.c_int, [P, P, U32, P, U32, ctypes.POINTER(GatherParams), P, ctypes.POINTER(ctypes.c_double)]),
     "lsg_rasterize_texels": (ctypes.c_int, [P, P, U32, U32, U32, U32, P, ctypes.POINTER(U32), P, ctypes.POINTER(U32), U32]),
 }
 

@@ -1,0 +1,217 @@
+"""CPU checks of the bake path: the texel rasteriser (host C++ in liblsgpu,
+restating lightmap.cpp:83-335) against closed-form coverage, and the gather
+oracle (oracle/lsg_oracle_gather.c, restating gather.cpp:600-836) against the
+radiometric closed forms that pin it."""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+from lightsprintsdk_amd import scenes as S
+
+
+def _raster(uv, w, h, tri_ids=None):
+    from lightsprintsdk_amd import bake as B
+    uv = np.asarray(uv, np.float32).reshape(-1, 6)
+    ids = np.arange(len(uv), dtype=np.uint32) if tri_ids is None else tri_ids
+    return B.rasterize(uv, ids, w, h, 0)
+
+
+def heron(u):
+    p = [u[:, 0:2], u[:, 2:4], u[:, 4:6]]
+    a = np.linalg.norm(p[1] - p[0], axis=1)
+    b = np.linalg.norm(p[2] - p[1], axis=1)
+    c = np.linalg.norm(p[0] - p[2], axis=1)
+    s = (a + b + c) / 2
+    return np.sqrt(np.maximum(s * (s - a) * (s - b) * (s - c), 0))
+
+
+def test_raster_full_quad_closed_form():
+    """Two triangles covering the whole unit square of a 4x4 map: 16 texels, each
+    texel split by the diagonal exactly on the 4 diagonal texels; every texel's
+    subtexel areas sum to its share of the triangles."""
+    uv = [[0, 0, 1, 0, 1, 1], [0, 0, 1, 1, 0, 1]]
+    tex, sub = _raster(uv, 4, 4)
+    assert len(tex) == 16
+    # the triangle-space area of a triangle is 0.5: sum over texels = 0.5 * map area (0.5) per triangle
+    for t in range(2):
+        s = sub["area"][sub["triangle"] == t].astype(np.float64).sum()
+        assert abs(s - 0.5 * 0.5) < 1e-6
+    # texel coverage: each texel's area share is 1/16 of the map area (0.5 factor of triangle space)
+    cover = np.zeros(16)
+    for k, tx in enumerate(tex):
+        n = tx["sub_count_flags"] & 0xFFFF
+        cover[k] = sub["area"][tx["sub_begin"]:tx["sub_begin"] + n].astype(np.float64).sum()
+    np.testing.assert_allclose(cover, 0.5 / 16, rtol=1e-5)
+    # interior texels see all four quadrants and no edge bits... full texels have all quadrants
+    flags = (tex["sub_count_flags"] >> 16) & 0xFF
+    assert np.all((flags & 15) == 15)
+    assert np.all(tex["out_index"] == np.arange(16))
+
+
+def test_raster_triangle_area_invariant():
+    """For the config-2 room: per triangle, subtexel areas sum to 0.5 x its map area
+    (all charts are inside the map), and no texel is covered more than once."""
+    sc = S.box_room(seed=3, n_boxes=4, target_tris=4000, lightmap_size=64)
+    for k, ob in enumerate(sc.objects[:3]):
+        ids = np.arange(ob.tri_begin, ob.tri_begin + ob.tri_count, dtype=np.uint32)
+        uv = sc.tri_uv[ob.tri_begin:ob.tri_begin + ob.tri_count]
+        tex, sub = _raster(uv, ob.width, ob.height, ids)
+        a = np.zeros(sc.ntri)
+        np.add.at(a, sub["triangle"], sub["area"].astype(np.float64))
+        ta = heron(uv.astype(np.float64))
+        np.testing.assert_allclose(a[ids], 0.5 * ta, rtol=2e-3, atol=1e-9)
+        # coverage of each texel by the (non-overlapping) unwrap is at most one texel
+        cover = np.add.reduceat(sub["area"].astype(np.float64), tex["sub_begin"]) if len(tex) else np.zeros(0)
+        assert np.all(cover <= 0.5 / (ob.width * ob.height) * (1 + 1e-3))
+
+
+def test_raster_outside_and_degenerate():
+    tex, sub = _raster([[2, 2, 3, 2, 2, 3], [0.1, 0.1, 0.1, 0.1, 0.1, 0.1]], 8, 8)
+    assert len(tex) == 0 and len(sub) == 0
+
+
+# ---- gather oracle closed forms ------------------------------------------------------------
+
+def _single_texel(tri, uv=(0, 0, 1, 0, 0, 1), flags=255, key=1):
+    from lightsprintsdk_amd import bake as B
+    tex = np.zeros(1, B.TEXEL_DTYPE)
+    tex["out_index"] = 0
+    tex["sub_begin"] = 0
+    tex["sub_count_flags"] = 1 | (flags << 16)
+    tex["key"] = key
+    sub = np.zeros(1, B.SUBTEXEL_DTYPE)
+    sub["triangle"] = tri
+    sub["uv"] = np.asarray(uv, np.float32)
+    sub["area"] = 1.0
+    return tex, sub
+
+
+def _lights(lst):
+    import oracle_py as O
+    a = np.zeros(len(lst), O.LIGHT_DTYPE)
+    for k, l in enumerate(lst):
+        a[k] = (l["type"], l.get("position", (0, 0, 0)), l.get("direction", (0, 0, -1)), l["color"], l.get("cast_shadows", 1))
+    return a
+
+
+def _arrays(verts, idx, normals, exitance, lights, side=5):
+    return dict(verts=np.asarray(verts, np.float32), idx=np.asarray(idx, np.uint32), normals=np.asarray(normals, np.float32),
+                side_bits=np.full(len(idx), side, np.uint8), exitance=np.asarray(exitance, np.float32), lights=_lights(lights))
+
+
+def closed_box(size=2.0, n=3, inward=True):
+    sc = S.box_room(seed=0, n_boxes=0, target_tris=200, lightmap_size=16, room=(size, size, size))
+    ob = sc.objects[0]
+    v = sc.verts
+    i = sc.idx[ob.tri_begin:ob.tri_begin + ob.tri_count]
+    return v, i, sc.normals
+
+
+def test_gather_uniform_enclosure(oracle):
+    """Inside a closed box whose walls all have exitance L, every hemisphere ray
+    returns L: the gathered irradiance is exactly L (gather.cpp:250,307-313)."""
+    v, i, nrm = closed_box()
+    L = np.float32([0.7, 1.3, 2.1])
+    ex = np.zeros((len(i), 8), np.float32)
+    ex[:, 0:3] = L
+    arr = _arrays(v, i, nrm, ex, [])
+    p = dict(quality=256, hemisphere=1, ray_length_min=S.minimal_safe_distance(v), locality=1e5, rug_distance=1e-3,
+             light_multiplier=1.0, seed=3)
+    for tri in (0, 17, len(i) - 1):
+        tex, sub = _single_texel(tri, uv=(0.1, 0.1, 0.8, 0.1, 0.1, 0.8))
+        out, st = oracle.gather(arr, p, tex.view(np.uint32).reshape(-1, 4), sub)
+        np.testing.assert_allclose(out[0, :3], L, rtol=1e-12)
+        assert out[0, 3] == 1.0
+        assert st.hemi_rays == 256
+
+
+def test_gather_point_light_closed_form(oracle):
+    """Floor quad, unoccluded point light: each shot adds I*cos/(1.37289e-10+d^2)
+    (RRLight.cpp:21,362; gather.cpp:451-463), averaged over quality/10+1 rounds."""
+    v = np.float32([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]])
+    i = np.uint32([[0, 1, 2], [0, 2, 3]])
+    nrm = np.float32([[0, 0, 1]] * 4)
+    ex = np.zeros((2, 8), np.float32)
+    lp = np.float32([0.3, 0.2, 1.5])
+    col = np.float32([10.0, 5.0, 2.5])
+    arr = _arrays(v, i, nrm, ex, [dict(type=0, position=lp, color=col)])
+    q = 100
+    p = dict(quality=q, hemisphere=0, ray_length_min=1e-6, locality=1e5, rug_distance=1e-3, light_multiplier=1.0, seed=9)
+    tex, sub = _single_texel(0, uv=(0.2, 0.2, 0.3, 0.2, 0.2, 0.3))
+    out, st = oracle.gather(arr, p, tex.view(np.uint32).reshape(-1, 4), sub)
+    rounds = q // 10 + 1
+    assert st.shadow_rays == rounds
+    # recompute the same shooter positions in FP64 from the seeded jitter
+    key = oracle.lib().lsgo_texel_key(9, int(tex["key"][0]))
+    acc = np.zeros(3)
+    for s in range(rounds):
+        u = np.zeros(1, np.float32); w = np.zeros(1, np.float32)
+        oracle.lib().lsgo_jitter(key, s, u.ctypes.data_as(ctypes.c_void_p), w.ctypes.data_as(ctypes.c_void_p))
+        ux = 0.2 + 0.1 * float(u[0]); uy = 0.2 + 0.1 * float(w[0])
+        pnt = np.array([ux, uy, 0.0])          # triangle 0: v0 + (v1-v0)*ux + (v2-v0)*uy
+        pnt = np.array([ux + uy, uy, 0.0])
+        dvec = lp - pnt
+        d2 = dvec @ dvec
+        acc += col * (dvec[2] / math.sqrt(d2)) / (1.37289e-10 + d2)
+    np.testing.assert_allclose(out[0, :3], acc / rounds, rtol=2e-6)
+
+
+def test_gather_emitter_form_factor(oracle):
+    """Differential floor element under a square emitter of radiance L (nothing
+    else in the scene): irradiance = L * F, F the analytic point-to-parallel-
+    rectangle form factor; the cosine-weighted quasi-random directions of
+    HomogenousFiller2 must reproduce it to ~1% at 4096 rays."""
+    h, a = 1.0, 1.0
+    v = np.float32([[-0.001, -0.001, 0], [0.001, -0.001, 0], [0.001, 0.001, 0], [-0.001, 0.001, 0],
+                    [-a, -a, h], [a, -a, h], [a, a, h], [-a, a, h]])
+    i = np.uint32([[0, 1, 2], [0, 2, 3], [4, 6, 5], [4, 7, 6]])   # emitter faces down
+    nrm = np.float32([[0, 0, 1]] * 4 + [[0, 0, -1]] * 4)
+    L = 3.0
+    ex = np.zeros((4, 8), np.float32)
+    ex[2:, 0:3] = L
+    arr = _arrays(v, i, nrm, ex, [])
+    p = dict(quality=4096, hemisphere=1, ray_length_min=1e-6, locality=1e5, rug_distance=1e-3, light_multiplier=1.0, seed=5)
+    tex, sub = _single_texel(0, uv=(0.4, 0.4, 0.6, 0.4, 0.4, 0.6))
+    out, _ = oracle.gather(arr, p, tex.view(np.uint32).reshape(-1, 4), sub)
+
+    def corner(A, B):   # differential area to a parallel rectangle with a corner above it
+        return (A / math.sqrt(1 + A * A) * math.atan(B / math.sqrt(1 + A * A))
+                + B / math.sqrt(1 + B * B) * math.atan(A / math.sqrt(1 + B * B))) / (2 * math.pi)
+    F = 4 * corner(a / h, a / h)
+    np.testing.assert_allclose(out[0, :3], L * F, rtol=1e-2)
+
+
+def test_gather_shadow_and_rug(oracle):
+    """An occluder between texel and light gives reliable black; an occluder
+    closer than rugDistance makes the light rays unreliable (gather.cpp:493-511),
+    so with every ray unreliable the texel is completely unreliable (alpha 0)."""
+    v = np.float32([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
+                    [-1, -1, 0.5], [2, -1, 0.5], [2, 2, 0.5], [-1, 2, 0.5]])
+    i = np.uint32([[0, 1, 2], [0, 2, 3], [4, 5, 6], [4, 6, 7]])
+    nrm = np.float32([[0, 0, 1]] * 8)
+    ex = np.zeros((4, 8), np.float32)
+    lights = [dict(type=0, position=(0.5, 0.5, 2.0), color=(5, 5, 5))]
+    p = dict(quality=50, hemisphere=0, ray_length_min=1e-6, locality=1e5, rug_distance=1e-3, light_multiplier=1.0, seed=2)
+    tex, sub = _single_texel(0, uv=(0.3, 0.3, 0.5, 0.3, 0.3, 0.5), flags=200)
+    out, _ = oracle.gather(_arrays(v, i, nrm, ex, lights), p, tex.view(np.uint32).reshape(-1, 4), sub)
+    assert np.all(out[0, :3] == 0) and out[0, 3] == pytest.approx(200 / 255)
+    v2 = v.copy()
+    v2[4:, 2] = 0.0005            # occluder 0.5 mm above the floor: every shadow ray hits a rug
+    out, st = oracle.gather(_arrays(v2, i, nrm, ex, lights), p, tex.view(np.uint32).reshape(-1, 4), sub)
+    assert np.all(out[0] == 0)
+    assert st.series > 1          # the reliability loop shot more series before giving up
+
+
+def test_shot_schedule_closed_form(oracle):
+    """The GPU deals light shots to shooters with the closed form
+    floor(k*(n-1)/r)+1; it must equal the counter rule of gather.cpp:777-783."""
+    for n, r in [(1024, 103), (100, 11), (64, 7), (513, 52), (2, 1), (1000, 999)]:
+        c = 0
+        for s in range(n + 1):
+            assert oracle.lib().lsgo_shots_before(s, n, r) == c, (n, r, s)
+            want = (0 if s == 0 else min(r, -(-((s - 1) * r) // (n - 1))))
+            assert want == c, (n, r, s)
+            if s < n and c * (n - 1) < r * s:
+                c += 1
