@@ -109,6 +109,13 @@ int  lsg_collider_info_get(const lsg_collider* collider, lsg_collider_info* info
  * triangle index per leaf-order slot (num_valid). Either may be NULL. */
 int  lsg_collider_download(const lsg_collider* collider, float* nodes, uint32_t* prim);
 
+/* copies the per-triangle records in leaf order (num_valid each): plane = float4 n4
+ * (TriangleSRLNP::n4), sv = float4 (s[x], s[y], s[z], intersectReal), ev = float4
+ * (r[x], r[y], l[z], ir[z]) with x,y,z the projection axes of intersectByte, meta =
+ * 2 uint32 (original triangle index, intersectByte).  Used by the host-side
+ * single-ray path so that CPU and GPU test triangles with identical records. */
+int  lsg_collider_download_triangles(const lsg_collider* collider, float* plane, float* sv, float* ev, uint32_t* meta);
+
 /* intersectBatch over the reference's own RRRay records in host memory.
  * Re-adds RRCollider::intersectBatch (doc/doxygen/doc.news:3212, removed :154)
  * with the per-ray contract of RRCollider::intersect (RRCollider.h:197-228):
@@ -201,6 +208,32 @@ int  lsg_gather_device(lsg_scene* scene, const lsg_texel* texels, uint32_t ntexe
 int  lsg_gather_stats(lsg_scene* scene, const lsg_texel* texels, uint32_t ntexels,
                       const lsg_subtexel* subs, uint32_t nsubs, const lsg_gather_params* params,
                       float* out_rgba, double* stats6);
+
+/* ---- lightmap filters and writeback -------------------------------------------------- */
+/* One lightmap inside a batch: its RGBA temp texels start at float4 slot rgba_base
+ * (row-major, width*height), its RGBF output at texel slot rgbf_base. */
+typedef struct lsg_lightmap {
+    uint32_t rgba_base;
+    uint32_t width;
+    uint32_t height;
+    uint32_t rgbf_base;
+} lsg_lightmap;
+
+/* RRSolver::FilteringParameters (RRSolver.h:596-622); defaults 1, 1000, 0, false */
+typedef struct lsg_filter_params {
+    float   smoothing_amount;   /* lightmapSmooth sigma */
+    float   spread_foreground;  /* lightmapGrow steps */
+    float   background[4];      /* lightmapFillBackground colour */
+    int32_t wrap;               /* must be 0 (wrap=true is not supported) */
+} lsg_filter_params;
+
+/* The filter chain updateLightmap runs on every gathered buffer (lightmap.cpp:852-876):
+ * lightmapSmooth -> lightmapGrowForBilinearInterpolation -> lightmapGrow ->
+ * lightmapFillBackground (RRBuffer.cpp:630-975), for a batch of lightmaps at once.
+ * rgba: device float4 texels, updated in place; rgbf (optional): device float3
+ * output (copyElementsTo into a BF_RGBF buffer, RRBuffer.cpp:105).  Synchronous. */
+int  lsg_filter_lightmaps(const lsg_lightmap* maps, uint32_t nmaps, float* rgba, float* rgbf,
+                          const lsg_filter_params* params, void* stream);
 
 /* Texel rasterisation of one object's lightmap, RRSolver/lightmap.cpp:83-335
  * (enumerateTexelsPartial, steps 1-2): clips every texel square against every

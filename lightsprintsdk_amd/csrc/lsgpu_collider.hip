@@ -566,6 +566,19 @@ int lsg_collider_download(const lsg_collider* c, float* nodes, uint32_t* prim)
     return LSG_OK;
 }
 
+int lsg_collider_download_triangles(const lsg_collider* c, float* plane, float* sv, float* ev, uint32_t* meta)
+{
+    if (!c) return fail(LSG_ERR_INVALID, "lsg_collider_download_triangles: null collider");
+    const ColliderImpl& C = c->impl;
+    const size_t n = C.nvalid;
+    if (!n) return LSG_OK;
+    if (plane) LSG_HIP(hipMemcpy(plane, C.d_plane, 16 * n, hipMemcpyDeviceToHost));
+    if (sv) LSG_HIP(hipMemcpy(sv, C.d_sv, 16 * n, hipMemcpyDeviceToHost));
+    if (ev) LSG_HIP(hipMemcpy(ev, C.d_ev, 16 * n, hipMemcpyDeviceToHost));
+    if (meta) LSG_HIP(hipMemcpy(meta, C.d_meta, 8 * n, hipMemcpyDeviceToHost));
+    return LSG_OK;
+}
+
 int lsg_collider_intersect_device(const lsg_collider* c, const void* ro, const void* rd, uint32_t n, void* hits, void* front, void* stream)
 {
     if (!c || (!n)) return c ? LSG_OK : fail(LSG_ERR_INVALID, "lsg_collider_intersect_device: null collider");

@@ -1,0 +1,26 @@
+// lsgpu_filter.h -- internal interface of the lightmap filter chain (lsgpu_filter.hip).
+#pragma once
+#include "lsgpu_internal.h"
+
+namespace lsg {
+
+struct FilterMap {
+    uint32_t first;      // first texel of this map in the batch's flat texel numbering
+    uint32_t base;       // float4 slot of texel (0,0) in the RGBA buffer
+    uint32_t out_base;   // RGBF texel slot of texel (0,0) in the output buffer
+    uint32_t width, height;
+};
+
+struct FilterScratch {
+    float4* a = nullptr; float4* b = nullptr; float4* c = nullptr;
+    uint8_t* flags = nullptr;
+    uint32_t* changed = nullptr;
+};
+
+int maps_upload(const lsg_lightmap* maps, uint32_t nmaps, FilterMap** d_maps, uint32_t* total);
+int scratch_alloc(FilterScratch& S, uint32_t total);
+void scratch_free(FilterScratch& S);
+int filter_run(const FilterMap* d_maps, uint32_t nmaps, uint32_t total, float4* rgba, float* rgbf,
+               const lsg_filter_params& fp, FilterScratch& S, hipStream_t s, bool rgba_out);
+
+} // namespace lsg

@@ -43,6 +43,15 @@ class GatherParams(ctypes.Structure):
                 ("seed", U32), ("reserved", U32)]
 
 
+class Lightmap(ctypes.Structure):
+    _fields_ = [("rgba_base", U32), ("width", U32), ("height", U32), ("rgbf_base", U32)]
+
+
+class FilterParams(ctypes.Structure):
+    _fields_ = [("smoothing_amount", ctypes.c_float), ("spread_foreground", ctypes.c_float),
+                ("background", ctypes.c_float * 4), ("wrap", I32)]
+
+
 # name -> (restype, argtypes); restype None means int status checked by check()
 SIGNATURES = {
     "lsg_last_error": (ctypes.c_char_p, []),
@@ -66,6 +75,7 @@ SIGNATURES = {
     "lsg_collider_destroy": (None, [P]),
     "lsg_collider_info_get": (ctypes.c_int, [P, ctypes.POINTER(ColliderInfo)]),
     "lsg_collider_download": (ctypes.c_int, [P, P, P]),
+    "lsg_collider_download_triangles": (ctypes.c_int, [P, P, P, P, P]),
     "lsg_collider_intersect_rrrays": (ctypes.c_int, [P, P, U32, P, ctypes.POINTER(U32)]),
     "lsg_collider_intersect_device": (ctypes.c_int, [P, P, P, U32, P, P, P]),
     "lsg_collider_trace_stats": (ctypes.c_int, [P, P, P, U32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
@@ -75,6 +85,7 @@ SIGNATURES = {
     "lsg_scene_set_lights": (ctypes.c_int, [P, P, U32]),
     "lsg_gather_device": (ctypes.c_int, [P, P, U32, P, U32, ctypes.POINTER(GatherParams), P, P]),
     "lsg_gather_stats": (ctypes.c_int, [P, P, U32, P, U32, ctypes.POINTER(GatherParams), P, ctypes.POINTER(ctypes.c_double)]),
+    "lsg_filter_lightmaps": (ctypes.c_int, [P, U32, P, P, ctypes.POINTER(FilterParams), P]),
     "lsg_rasterize_texels": (ctypes.c_int, [P, P, U32, U32, U32, U32, P, ctypes.POINTER(U32), P, ctypes.POINTER(U32), U32]),
 }
 

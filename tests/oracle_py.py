@@ -227,3 +227,41 @@ def gather(scene_arrays, params, texels, subs, tree=None, threads=0):
     lib().lsgo_gather(_p(v), len(v), _p(i), len(i), _p(vn), _p(sb), _p(ex), _p(lights), len(lights),
                       ctypes.byref(gp), _p(tx), len(tx), _p(sbs), np_, root, pp_, nprim, _p(out), ctypes.byref(gs), threads)
     return out, gs
+
+
+DROP_IN_BIN = os.path.join(ORACLE_DIR, "_ref", "drop_in_test")
+
+
+def have_drop_in() -> bool:
+    return os.path.exists(DROP_IN_BIN)
+
+
+def drop_in(verts, idx, rays, technique):
+    """Runs oracle/_ref/drop_in_test: RRCollider::create(mesh, nullptr, technique)
+    through the compiled reference with the IT_HIP_LBVH adapter registered.
+    Returns (single-ray hits, batch hits, front-only-handler hits, info dict)."""
+    v, i = _mesh(verts, idx)
+    r = np.ascontiguousarray(rays, dtype=np.float32)
+    with tempfile.TemporaryDirectory() as td:
+        fin = os.path.join(td, "in.bin")
+        fout = os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(np.array([0x3152534C, len(v), len(i), len(r)], np.uint32).tobytes())
+            f.write(v.tobytes()); f.write(i.tobytes()); f.write(r.tobytes())
+        p = subprocess.run([DROP_IN_BIN, str(technique), fin, fout], check=True, capture_output=True, text=True)
+        info = json.loads(p.stdout.strip().splitlines()[-1])
+        out = np.fromfile(fout, dtype=HIT_DTYPE).reshape(3, len(r))
+    return out[0], out[1], out[2], info
+
+
+def filter_lightmap(rgba, sigma=1.0, spread=1000.0, bg=(0, 0, 0, 0)):
+    """RRBuffer filter chain on one RGBA lightmap [h,w,4]; returns (rgba_out, rgbf)."""
+    L = lib()
+    L.lsgo_filter_lightmap.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_float,
+                                       ctypes.c_void_p, ctypes.c_void_p]
+    img = np.ascontiguousarray(rgba, dtype=np.float32).copy()
+    h, w = img.shape[:2]
+    b = np.asarray(bg, np.float32)
+    out = np.zeros((h, w, 3), np.float32)
+    L.lsgo_filter_lightmap(_p(img), w, h, sigma, spread, _p(b), _p(out))
+    return img, out

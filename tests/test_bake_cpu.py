@@ -215,3 +215,47 @@ def test_shot_schedule_closed_form(oracle):
             assert want == c, (n, r, s)
             if s < n and c * (n - 1) < r * s:
                 c += 1
+
+
+# ---- filter oracle invariants (RRBuffer.cpp:630-975) ----------------------------------------
+
+def _flags_image(h, w, value=255):
+    a = np.zeros((h, w, 4), np.float32)
+    a[..., 3] = value / 255.0
+    return a
+
+
+def test_filter_constant_image_invariant(oracle):
+    """A constant, fully covered lightmap is a fixed point of smooth+grow+fill."""
+    img = _flags_image(8, 8)
+    img[..., :3] = [0.3, 0.6, 0.9]
+    out, rgb = oracle.filter_lightmap(img)
+    np.testing.assert_allclose(rgb, np.broadcast_to(np.float32([0.3, 0.6, 0.9]), rgb.shape), rtol=1e-6)
+    np.testing.assert_array_equal(out[..., 3], img[..., 3])
+
+
+def test_filter_grow_and_background(oracle):
+    """Grow fills only empty texels (alpha 0.001 marks them), fill-background
+    leaves foreground; a map without foreground becomes the background colour."""
+    img = np.zeros((6, 6, 4), np.float32)
+    img[2:4, 2:4, :3] = [1.0, 2.0, 3.0]
+    img[2:4, 2:4, 3] = 255 / 255.0      # all flags set
+    out, rgb = oracle.filter_lightmap(img, sigma=0.0)
+    fg = np.zeros((6, 6), bool); fg[2:4, 2:4] = True
+    np.testing.assert_array_equal(rgb[fg], np.broadcast_to(np.float32([1, 2, 3]), (4, 3)))
+    np.testing.assert_allclose(rgb[~fg], np.broadcast_to(np.float32([1, 2, 3]), (32, 3)), rtol=1e-6)
+    assert np.all(out[~fg][:, 3] == np.float32(0.001))
+    empty, rgb0 = oracle.filter_lightmap(np.zeros((4, 4, 4), np.float32), bg=(0.5, 0.25, 0.125, 0))
+    np.testing.assert_array_equal(rgb0, np.broadcast_to(np.float32([0.5, 0.25, 0.125]), (4, 4, 3)))
+
+
+def test_filter_smooth_respects_edge_flags(oracle):
+    """Texels whose edge flags do not face each other are not blurred together
+    (RRBuffer.cpp:683-690): two interior half-planes with EDGE bits only on the
+    outer sides keep their own values."""
+    img = _flags_image(4, 8, value=15)         # quadrant bits only, no EDGE_* bits
+    img[:, :4, :3] = 1.0
+    img[:, 4:, :3] = 5.0
+    out, rgb = oracle.filter_lightmap(img, spread=0.0)
+    np.testing.assert_array_equal(rgb[:, :4], np.ones((4, 4, 3), np.float32))
+    np.testing.assert_array_equal(rgb[:, 4:], np.full((4, 4, 3), 5.0, np.float32))
