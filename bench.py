@@ -123,6 +123,9 @@ def main():
     if workload == "bake":
         from lightsprintsdk_amd import bake
         res = bake.bench_workload(a.steps, a.warmup, threads)
+    elif workload == "batched":
+        from lightsprintsdk_amd import bake
+        res = bake.bench_batched(a.steps, a.warmup, threads)
     else:
         res = collider_workload(a.steps, a.warmup, threads)
     out = dict(metric=res.pop("metric"), value=res.pop("value"), unit=res.pop("unit"), n_gpus=1, steps=a.steps,

@@ -245,6 +245,60 @@ int  lsg_rasterize_texels(const float* tri_uv, const uint32_t* tri_ids, uint32_t
                           lsg_texel* texels, uint32_t* ntexels, lsg_subtexel* subs, uint32_t* nsubs,
                           uint32_t out_base);
 
+/* ---- whole-scene bake plan ------------------------------------------------------------
+ * RRSolver::updateLightmaps (RRSolver/lightmap.cpp:943-1163) for many objects at once:
+ * the plan rasterises every object's lightmap once, keeps texels, SubTexels, the RGBA
+ * temp buffers, the RGBF output and filter scratch in HBM, and runs
+ *   first gather per triangle (gatherPerTrianglePhysical, gather.cpp:843-952)
+ *   -> exitance = direct irradiance * albedo + emittance
+ *   -> final gather of all texels of all objects in one persistent launch
+ *   -> filter chain + RGBF writeback.
+ * With LSG_BAKE_USE_GRAPH the sequence is captured once into a hipGraph and replayed. */
+typedef struct lsg_bake lsg_bake;
+
+typedef struct lsg_bake_object {
+    const float*    tri_uv;    /* host float[ntri*6]: lightmap uv of each triangle */
+    const uint32_t* tri_ids;   /* host: the triangles' indices in the collider's mesh */
+    uint32_t        ntri;
+    uint32_t        width, height;
+} lsg_bake_object;
+
+/* launch shape of the persistent gather; 0 = default (picked by lsg_bake_autotune) */
+typedef struct lsg_bake_tuning {
+    uint32_t tile;             /* texels per work-queue fetch (default 2) */
+    uint32_t waves_per_block;  /* 2, 4 or 8 (default 4) */
+    uint32_t blocks_per_cu;    /* persistent workgroups per compute unit (default 4) */
+    uint32_t xcd_queues;       /* 8 = one work queue per XCD, 1 = one global queue (default 8) */
+} lsg_bake_tuning;
+
+typedef struct lsg_bake_info {
+    uint32_t num_texels, num_subtexels, num_slots, num_lightmaps, num_triangles, compute_units, grow_steps;
+    lsg_bake_tuning tuning;
+    void* rgba;                /* device float4[num_slots]: filtered RGBA temp texels */
+    void* rgbf;                /* device float3[num_slots]: RGBF lightmaps, object after object */
+    void* texels;              /* device lsg_texel[num_texels] */
+    void* subtexels;           /* device lsg_subtexel[num_subtexels] */
+} lsg_bake_info;
+
+#define LSG_BAKE_USE_GRAPH 1u
+
+/* albedo, emittance: host float[ntri*3] per collider triangle (diffuse reflectance and
+ * diffuse emittance, RRMaterial.h).  The scene must outlive the plan. */
+int  lsg_bake_create(lsg_scene* scene, const lsg_bake_object* objects, uint32_t nobjects,
+                     const float* albedo, const float* emittance, lsg_bake** out);
+void lsg_bake_destroy(lsg_bake* bake);
+int  lsg_bake_info_get(const lsg_bake* bake, lsg_bake_info* info);
+int  lsg_bake_set_tuning(lsg_bake* bake, const lsg_bake_tuning* tuning);
+/* first may be NULL (keep the current exitance); filtering may be NULL (no filters).
+ * Without the graph flag the call is asynchronous on stream except for the grow
+ * loop's convergence checks; with it, the replay is ordered after stream's work. */
+int  lsg_bake_run(lsg_bake* bake, const lsg_gather_params* first, const lsg_gather_params* final_params,
+                  const lsg_filter_params* filtering, uint32_t flags, void* stream);
+/* times the persistent final gather over every stride-th texel (autotuner probe) */
+int  lsg_bake_gather_sample(lsg_bake* bake, const lsg_gather_params* params, uint32_t stride, float* ms);
+/* copies object `object`'s RGBF lightmap (width*height*3 floats) to the host */
+int  lsg_bake_download(const lsg_bake* bake, uint32_t object, float* rgbf);
+
 #ifdef __cplusplus
 }
 #endif

@@ -387,3 +387,165 @@ def bake_cpu_baseline(sb: "SceneBake", threads: int, target_s: float = 12.0):
     return dict(value=round(n1 / dt, 1), unit="texels/s", cores=threads, kind="port",
                 sample=f"{n1} of {len(sb.texels)} texels evenly spread, oracle C gather (FP64 sums) over the same LBVH, "
                        f"{dt:.1f} s")
+
+
+# ---------------------------------------------------------------------------------------
+# whole-scene bake plan (lsg_bake_*) and its autotuner
+
+class BakePlan:
+    """All lightmaps of a BakeScene baked by one plan: persistent final gather over
+    every texel of every object, optional hipGraph capture/replay."""
+
+    def __init__(self, scene: S.BakeScene, params: GatherParams | None = None, filtering: "FilterParams | None" = None):
+        self.scene = scene
+        self.params = params or GatherParams(ray_length_min=S.minimal_safe_distance(scene.verts))
+        self.first_params = GatherParams(quality=max(self.params.quality // 2, 1), hemisphere=0,
+                                         ray_length_min=self.params.ray_length_min, locality=self.params.locality,
+                                         rug_distance=self.params.rug_distance,
+                                         light_multiplier=self.params.light_multiplier, seed=self.params.seed ^ 0x5bd1e995)
+        self.filtering = filtering or FilterParams()
+        ex0 = np.zeros((scene.ntri, 8), np.float32)
+        ex0[:, 0:3] = scene.emittance
+        self.gs = GatherScene(scene.verts, scene.idx, scene.normals, scene.side_bits, ex0, scene.lights)
+        self._keep = []
+        objs = (N.BakeObject * max(len(scene.objects), 1))()
+        for k, ob in enumerate(scene.objects):
+            uv = np.ascontiguousarray(scene.tri_uv[ob.tri_begin:ob.tri_begin + ob.tri_count], np.float32)
+            ids = np.arange(ob.tri_begin, ob.tri_begin + ob.tri_count, dtype=np.uint32)
+            self._keep += [uv, ids]
+            objs[k] = N.BakeObject(uv.ctypes.data_as(ctypes.c_void_p), ids.ctypes.data_as(ctypes.c_void_p), ob.tri_count,
+                                   ob.width, ob.height)
+        alb = np.ascontiguousarray(scene.albedo, np.float32)
+        emi = np.ascontiguousarray(scene.emittance, np.float32)
+        h = ctypes.c_void_p()
+        N.check(N.lib().lsg_bake_create(self.gs._h, objs, len(scene.objects), alb.ctypes.data_as(ctypes.c_void_p),
+                                        emi.ctypes.data_as(ctypes.c_void_p), ctypes.byref(h)), "lsg_bake_create")
+        self._h = h.value
+
+    def info(self) -> N.BakeInfo:
+        inf = N.BakeInfo()
+        N.check(N.lib().lsg_bake_info_get(self._h, ctypes.byref(inf)), "bake info")
+        return inf
+
+    def set_tuning(self, tile=0, waves_per_block=0, blocks_per_cu=0, xcd_queues=0):
+        t = N.BakeTuning(tile, waves_per_block, blocks_per_cu, xcd_queues)
+        N.check(N.lib().lsg_bake_set_tuning(self._h, ctypes.byref(t)), "set_tuning")
+
+    def run(self, first=True, graph=False, filtering=True, stream=None):
+        fp = self.first_params.c()
+        p = self.params.c()
+        flt = self.filtering.c()
+        N.check(N.lib().lsg_bake_run(self._h, ctypes.byref(fp) if first else None, ctypes.byref(p),
+                                     ctypes.byref(flt) if filtering else None, N.BAKE_USE_GRAPH if graph else 0, stream),
+                "lsg_bake_run")
+
+    def sample_ms(self, stride):
+        ms = ctypes.c_float()
+        p = self.params.c()
+        N.check(N.lib().lsg_bake_gather_sample(self._h, ctypes.byref(p), stride, ctypes.byref(ms)), "gather_sample")
+        return ms.value
+
+    def lightmap(self, k):
+        ob = self.scene.objects[k]
+        out = np.zeros((ob.height, ob.width, 3), np.float32)
+        N.check(N.lib().lsg_bake_download(self._h, k, out.ctypes.data_as(ctypes.c_void_p)), "bake download")
+        return out
+
+    def rgba(self):
+        inf = self.info()
+        buf = np.empty((max(inf.num_slots, 1), 4), np.float32)
+        N.check(N.lib().lsg_synchronize(), "sync")
+        N.check(N.lib().lsg_memcpy_d2h(buf.ctypes.data_as(ctypes.c_void_p), inf.rgba, buf.nbytes), "d2h")
+        return buf
+
+    def close(self):
+        if getattr(self, "_h", None):
+            N.lib().lsg_bake_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+TUNING_CANDIDATES = [
+    dict(tile=t, waves_per_block=w, blocks_per_cu=b, xcd_queues=x)
+    for t in (1, 4) for w in (4, 8) for b in (2, 4, 8) for x in (1, 8)
+    if w * b <= 32
+]
+
+
+def autotune(plan: BakePlan, sample_texels: int = 60000, candidates=None, cache_path: str | None = None):
+    """Times the persistent final gather on an evenly strided texel sample for each
+    launch shape and keeps the fastest.  The choice is written to cache_path (JSON,
+    keyed by texel count, triangle count and quality) and reused when present."""
+    import json
+    import os
+    info = plan.info()
+    key = f"tex{info.num_texels}_tri{info.num_triangles}_q{plan.params.quality}_cu{info.compute_units}"
+    if cache_path and os.path.exists(cache_path):
+        try:
+            cache = json.load(open(cache_path))
+        except (OSError, ValueError):
+            cache = {}
+        if key in cache:
+            plan.set_tuning(**cache[key]["tuning"])
+            return cache[key]
+    else:
+        cache = {}
+    stride = max(1, info.num_texels // max(sample_texels, 1))
+    results = []
+    for c in candidates or TUNING_CANDIDATES:
+        plan.set_tuning(**c)
+        ms = min(plan.sample_ms(stride) for _ in range(2))
+        results.append((ms, c))
+    results.sort(key=lambda r: r[0])
+    best_ms, best = results[0]
+    plan.set_tuning(**best)
+    entry = dict(tuning=best, sample_ms=round(best_ms, 4), stride=stride,
+                 worst_ms=round(results[-1][0], 4), tried=len(results))
+    if cache_path:
+        cache[key] = entry
+        with open(cache_path, "w") as f:
+            json.dump(cache, f, indent=1, sort_keys=True)
+    return entry
+
+
+def bench_batched(steps: int, warmup: int, cpu_threads: int, n_objects: int = 200, target_tris: int = 2_000_000):
+    """Config 3: 200 objects (~2M triangles), lightmaps 32..1024, 1024 rays per texel,
+    sun + 4 point lights; one step = first gather + persistent final gather + filters
+    replayed from a hipGraph, launch shape from the autotuner."""
+    import os
+    t0 = time.perf_counter()
+    sc = S.object_field(seed=4, n_objects=n_objects, target_tris=target_tris)
+    gen_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    plan = BakePlan(sc, GatherParams(quality=1024, ray_length_min=S.minimal_safe_distance(sc.verts), seed=1))
+    setup_s = time.perf_counter() - t0
+    plan.run(first=True, graph=False)
+    t0 = time.perf_counter()
+    cache = os.environ.get("LSG_TUNING_CACHE", os.path.expanduser("~/.cache/lsgpu_tuning.json"))
+    os.makedirs(os.path.dirname(cache), exist_ok=True)
+    tune = autotune(plan, sample_texels=150_000, cache_path=cache)
+    tune_s = time.perf_counter() - t0
+    for _ in range(warmup):
+        plan.run(first=True, graph=True)
+    N.check(N.lib().lsg_synchronize(), "sync")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.run(first=True, graph=True)
+    N.check(N.lib().lsg_synchronize(), "sync")
+    wall = (time.perf_counter() - t0) / steps
+    inf = plan.info()
+    return dict(metric=METRIC, value=round(inf.num_texels / wall, 1), unit="texels/s", ms_per_step=round(wall * 1e3, 2),
+                dtype="f32",
+                config=dict(workload="batched bake: 200 objects (displaced boxes, cylinders, terrain patches), "
+                                     "lightmaps 32..1024 RGBF, 1024 rays/texel, sun + 4 point lights, hipGraph replay",
+                            triangles=int(sc.ntri), objects=len(sc.objects), texels=int(inf.num_texels),
+                            texel_slots=int(inf.num_slots), subtexels=int(inf.num_subtexels),
+                            tuning=dict(tile=inf.tuning.tile, waves_per_block=inf.tuning.waves_per_block,
+                                        blocks_per_cu=inf.tuning.blocks_per_cu, xcd_queues=inf.tuning.xcd_queues),
+                            autotune=tune, autotune_s=round(tune_s, 2), generate_s=round(gen_s, 2),
+                            setup_s=round(setup_s, 2), grow_steps=int(inf.grow_steps)))

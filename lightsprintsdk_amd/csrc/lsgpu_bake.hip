@@ -1,0 +1,314 @@
+// lsgpu_bake.hip -- whole-scene lightmap bake plan: RRSolver::updateLightmaps
+// (RRSolver/lightmap.cpp:943-1163) for many objects in one go on gfx950.
+//
+// A plan owns, in HBM, every object's texels and SubTexels (rasterised once on the
+// host), one RGBA temp texel array for all lightmaps, the RGBF output, the filter
+// scratch and the per-triangle first-gather work.  One run is
+//   first gather per triangle (gatherPerTrianglePhysical, gather.cpp:843-952)
+//   -> exitance = irradiance * albedo + emittance          (k_exitance)
+//   -> final gather of every texel of every object         (persistent launch,
+//      tiles from XCD-local work queues)
+//   -> filter chain + RGBF writeback                        (lsgpu_filter.hip)
+// and can be captured once into a hipGraph and replayed (lights and exitance live in
+// device memory, so a sun sweep only updates buffers between replays).
+#include "lsgpu_internal.h"
+#include "lsgpu_gather.h"
+#include "lsgpu_filter.h"
+#include <vector>
+#include <cstring>
+#include <chrono>
+
+namespace lsg {
+
+__global__ __launch_bounds__(256) void k_exitance(uint32_t nt, const float4* __restrict__ irr, const float* __restrict__ albedo,
+                                                  const float* __restrict__ emit, float4* __restrict__ ex)
+{
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= nt) return;
+    // PathtracerWorker::getIncidentRadiance at depth 0 with solver shortcuts
+    // (pathtracer.cpp:235-316): direct irradiance * diffuse reflectance + emittance
+    const float4 e = irr[t];
+    ex[2 * (size_t)t] = make_float4(e.x * albedo[3 * (size_t)t] + emit[3 * (size_t)t],
+                                    e.y * albedo[3 * (size_t)t + 1] + emit[3 * (size_t)t + 1],
+                                    e.z * albedo[3 * (size_t)t + 2] + emit[3 * (size_t)t + 2], 0.f);
+    ex[2 * (size_t)t + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ __launch_bounds__(256) void k_zero_f4(float4* p, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+static inline void dfree(void* p) { if (p) (void)hipFree(p); }
+static inline unsigned nblk(uint64_t n) { return (unsigned)((n + 255) / 256); }
+
+struct BakeImpl {
+    lsg_scene* scene = nullptr;
+    uint32_t ntri = 0, ntex = 0, nsub = 0, nmaps = 0, nslots = 0;
+    std::vector<lsg_lightmap> maps;
+    lsg_texel* d_tex = nullptr; lsg_subtexel* d_sub = nullptr;
+    float4* d_rgba = nullptr; float* d_rgbf = nullptr;
+    FilterMap* d_maps = nullptr; uint32_t ftotal = 0;
+    FilterScratch S;
+    lsg_texel* d_ttex = nullptr; lsg_subtexel* d_tsub = nullptr; float4* d_irr = nullptr;
+    float* d_albedo = nullptr; float* d_emit = nullptr;
+    uint32_t* d_queue = nullptr;
+    lsg_bake_tuning tuning{};
+    int cus = 256;
+    hipStream_t own = nullptr;
+    hipGraph_t graph = nullptr; hipGraphExec_t exec = nullptr;
+    std::vector<uint8_t> graph_key;
+    uint32_t grow_steps = 0;
+    ~BakeImpl()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (own) (void)hipStreamDestroy(own);
+        dfree(d_tex); dfree(d_sub); dfree(d_rgba); dfree(d_rgbf); dfree(d_maps); scratch_free(S);
+        dfree(d_ttex); dfree(d_tsub); dfree(d_irr); dfree(d_albedo); dfree(d_emit); dfree(d_queue);
+    }
+};
+
+static void default_tuning(lsg_bake_tuning& t)
+{
+    if (!t.tile) t.tile = 2;
+    if (!t.waves_per_block) t.waves_per_block = 4;
+    if (!t.blocks_per_cu) t.blocks_per_cu = 4;
+    if (!t.xcd_queues) t.xcd_queues = 8;
+}
+
+static int run_gather_final(BakeImpl& B, const lsg_gather_params* p, const lsg_texel* tex, uint32_t ntex, hipStream_t st)
+{
+    GatherArgs A;
+    int stack = 0;
+    int e = gather_args(B.scene, tex, ntex, B.d_sub, p, (float*)B.d_rgba, A, stack);
+    if (e) return e;
+    lsg_bake_tuning t = B.tuning;
+    default_tuning(t);
+    WorkQueue Q;
+    Q.counters = B.d_queue; Q.nqueues = t.xcd_queues > 1 ? 8 : 1; Q.tile = t.tile;
+    Q.ntiles = (ntex + t.tile - 1) / t.tile;
+    if (!Q.ntiles) return LSG_OK;
+    const uint32_t blocks = (uint32_t)B.cus * t.blocks_per_cu;
+    return launch_gather_persistent(A, stack, Q, t.waves_per_block, blocks, st);
+}
+
+// the whole bake on stream st; fixed_grow < 0 = adaptive grow (not capturable)
+static int run_all(BakeImpl& B, const lsg_gather_params* first, const lsg_gather_params* fin, const lsg_filter_params* fp,
+                   hipStream_t st, int fixed_grow, uint32_t* steps_used)
+{
+    int e;
+    if (first) {
+        GatherArgs A;
+        int stack = 0;
+        if ((e = gather_args(B.scene, B.d_ttex, B.ntri, B.d_tsub, first, (float*)B.d_irr, A, stack))) return e;
+        if ((e = launch_gather(A, stack, st))) return e;
+        k_exitance<<<nblk(B.ntri), 256, 0, st>>>(B.ntri, B.d_irr, B.d_albedo, B.d_emit, B.scene->impl.d_exit);
+    }
+    k_zero_f4<<<nblk(B.nslots), 256, 0, st>>>(B.d_rgba, B.nslots);
+    if ((e = run_gather_final(B, fin, B.d_tex, B.ntex, st))) return e;
+    if (fp) {
+        if ((e = filter_run(B.d_maps, B.nmaps, B.ftotal, B.d_rgba, B.d_rgbf, *fp, B.S, st, true, fixed_grow, steps_used))) return e;
+    }
+    LSG_HIP(hipGetLastError());
+    return LSG_OK;
+}
+
+} // namespace lsg
+
+struct lsg_bake { lsg::BakeImpl impl; };
+
+using namespace lsg;
+
+extern "C" {
+
+int lsg_bake_create(lsg_scene* scene, const lsg_bake_object* objects, uint32_t nobjects, const float* albedo,
+                    const float* emittance, lsg_bake** out)
+{
+    if (!out) return fail(LSG_ERR_INVALID, "lsg_bake_create: out is null");
+    *out = nullptr;
+    if (!scene || (nobjects && !objects) || !albedo || !emittance) return fail(LSG_ERR_INVALID, "lsg_bake_create: null argument");
+    lsg_bake* b = new (std::nothrow) lsg_bake;
+    if (!b) return fail(LSG_ERR_OOM, "lsg_bake_create: host allocation");
+    BakeImpl& B = b->impl;
+    B.scene = scene;
+    B.ntri = scene->impl.col->impl.ntri;
+    auto bail = [&](int st) { delete b; return st; };
+    // 1. rasterise every object (lightmap.cpp:83-335) into one texel/SubTexel list
+    std::vector<lsg_texel> tex;
+    std::vector<lsg_subtexel> sub;
+    uint32_t slot = 0;
+    for (uint32_t o = 0; o < nobjects; o++) {
+        const lsg_bake_object& ob = objects[o];
+        uint32_t nt = 0, ns = 0;
+        int st = lsg_rasterize_texels(ob.tri_uv, ob.tri_ids, ob.ntri, ob.width, ob.height, o, nullptr, &nt, nullptr, &ns, slot);
+        if (st) return bail(st);
+        size_t t0 = tex.size(), s0 = sub.size();
+        tex.resize(t0 + nt); sub.resize(s0 + ns);
+        st = lsg_rasterize_texels(ob.tri_uv, ob.tri_ids, ob.ntri, ob.width, ob.height, o, tex.data() + t0, &nt, sub.data() + s0, &ns, slot);
+        if (st) return bail(st);
+        for (size_t k = t0; k < tex.size(); k++) tex[k].sub_begin += (uint32_t)s0;
+        lsg_lightmap m{slot, ob.width, ob.height, slot};
+        B.maps.push_back(m);
+        uint64_t ns64 = (uint64_t)slot + (uint64_t)ob.width * ob.height;
+        if (ns64 >= (1ull << 31)) return bail(fail(LSG_ERR_LIMIT, "lsg_bake_create: more than 2^31 lightmap texels"));
+        slot = (uint32_t)ns64;
+    }
+    B.ntex = (uint32_t)tex.size(); B.nsub = (uint32_t)sub.size(); B.nmaps = nobjects; B.nslots = slot;
+    hipError_t e;
+#define AL(p, n) do { if ((e = hipMalloc(&(p), (n) ? (n) : 16)) != hipSuccess) return bail(hip_fail(e, "hipMalloc", __LINE__)); } while (0)
+#define UP(d, h, n) do { if ((n) && (e = hipMemcpy((d), (h), (n), hipMemcpyHostToDevice)) != hipSuccess) return bail(hip_fail(e, "h2d", __LINE__)); } while (0)
+    AL(B.d_tex, sizeof(lsg_texel) * tex.size()); UP(B.d_tex, tex.data(), sizeof(lsg_texel) * tex.size());
+    AL(B.d_sub, sizeof(lsg_subtexel) * sub.size()); UP(B.d_sub, sub.data(), sizeof(lsg_subtexel) * sub.size());
+    AL(B.d_rgba, 16 * (size_t)slot);
+    AL(B.d_rgbf, 12 * (size_t)slot);
+    AL(B.d_queue, 64);
+    // 2. first-gather work: one texel per triangle covering the whole triangle
+    std::vector<lsg_texel> ttex(B.ntri);
+    std::vector<lsg_subtexel> tsub(B.ntri);
+    for (uint32_t t = 0; t < B.ntri; t++) {
+        ttex[t] = lsg_texel{t, t, 1u | (255u << 16), 0x80000000u | t};
+        tsub[t].triangle = t;
+        const float uv[6] = {0, 0, 1, 0, 0, 1};
+        memcpy(tsub[t].uv, uv, sizeof uv);
+        tsub[t].area = 1.f;
+    }
+    AL(B.d_ttex, sizeof(lsg_texel) * B.ntri); UP(B.d_ttex, ttex.data(), sizeof(lsg_texel) * B.ntri);
+    AL(B.d_tsub, sizeof(lsg_subtexel) * B.ntri); UP(B.d_tsub, tsub.data(), sizeof(lsg_subtexel) * B.ntri);
+    AL(B.d_irr, 16 * (size_t)B.ntri);
+    AL(B.d_albedo, 12 * (size_t)B.ntri); UP(B.d_albedo, albedo, 12 * (size_t)B.ntri);
+    AL(B.d_emit, 12 * (size_t)B.ntri); UP(B.d_emit, emittance, 12 * (size_t)B.ntri);
+#undef AL
+#undef UP
+    int st = maps_upload(B.maps.data(), B.nmaps, &B.d_maps, &B.ftotal);
+    if (st) return bail(st);
+    if ((st = scratch_alloc(B.S, B.ftotal))) return bail(st);
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0) B.cus = prop.multiProcessorCount;
+    default_tuning(B.tuning);
+    *out = b;
+    return LSG_OK;
+}
+
+void lsg_bake_destroy(lsg_bake* b) { delete b; }
+
+int lsg_bake_info_get(const lsg_bake* b, lsg_bake_info* info)
+{
+    if (!b || !info) return fail(LSG_ERR_INVALID, "lsg_bake_info_get: null argument");
+    const BakeImpl& B = b->impl;
+    info->num_texels = B.ntex; info->num_subtexels = B.nsub; info->num_slots = B.nslots;
+    info->num_lightmaps = B.nmaps; info->num_triangles = B.ntri; info->compute_units = (uint32_t)B.cus;
+    info->grow_steps = B.grow_steps;
+    info->tuning = B.tuning;
+    info->rgba = B.d_rgba; info->rgbf = B.d_rgbf; info->texels = B.d_tex; info->subtexels = B.d_sub;
+    return LSG_OK;
+}
+
+int lsg_bake_set_tuning(lsg_bake* b, const lsg_bake_tuning* t)
+{
+    if (!b || !t) return fail(LSG_ERR_INVALID, "lsg_bake_set_tuning: null argument");
+    if (t->waves_per_block && t->waves_per_block != 2 && t->waves_per_block != 4 && t->waves_per_block != 8)
+        return fail(LSG_ERR_INVALID, "waves_per_block must be 2, 4 or 8");
+    if (t->tile > 1024 || t->blocks_per_cu > 64) return fail(LSG_ERR_INVALID, "tuning out of range");
+    b->impl.tuning = *t;
+    default_tuning(b->impl.tuning);
+    b->impl.graph_key.clear();   // invalidate a captured graph
+    return LSG_OK;
+}
+
+int lsg_bake_run(lsg_bake* b, const lsg_gather_params* first, const lsg_gather_params* final_params,
+                 const lsg_filter_params* filtering, uint32_t flags, void* stream)
+{
+    if (!b || !final_params) return fail(LSG_ERR_INVALID, "lsg_bake_run: null argument");
+    if (filtering && filtering->wrap) return fail(LSG_ERR_INVALID, "lsg_bake_run: wrap=true is not supported");
+    BakeImpl& B = b->impl;
+    hipStream_t st = (hipStream_t)stream;
+    if (!(flags & LSG_BAKE_USE_GRAPH)) {
+        uint32_t used = 0;
+        int e = run_all(B, first, final_params, filtering, st, -1, &used);
+        if (!e && filtering) B.grow_steps = used;
+        return e;
+    }
+    // graph mode: capture on the bake's own stream once per parameter set, replay after
+    if (!B.own) LSG_HIP(hipStreamCreate(&B.own));
+    std::vector<uint8_t> key(sizeof(lsg_gather_params) * 2 + sizeof(lsg_filter_params) + 2, 0);
+    if (first) { memcpy(key.data(), first, sizeof *first); key[key.size() - 2] = 1; }
+    memcpy(key.data() + sizeof(lsg_gather_params), final_params, sizeof *final_params);
+    if (filtering) { memcpy(key.data() + 2 * sizeof(lsg_gather_params), filtering, sizeof *filtering); key[key.size() - 1] = 1; }
+    if (key != B.graph_key || !B.exec) {
+        // an eager run fixes the number of grow steps the graph will replay
+        uint32_t used = 0;
+        int e = run_all(B, first, final_params, filtering, B.own, -1, &used);
+        if (e) return e;
+        LSG_HIP(hipStreamSynchronize(B.own));
+        B.grow_steps = used;
+        if (B.exec) { (void)hipGraphExecDestroy(B.exec); B.exec = nullptr; }
+        if (B.graph) { (void)hipGraphDestroy(B.graph); B.graph = nullptr; }
+        LSG_HIP(hipStreamBeginCapture(B.own, hipStreamCaptureModeThreadLocal));
+        e = run_all(B, first, final_params, filtering, B.own, (int)used, nullptr);
+        hipGraph_t g = nullptr;
+        hipError_t ce = hipStreamEndCapture(B.own, &g);
+        if (e) { if (g) (void)hipGraphDestroy(g); return e; }
+        if (ce != hipSuccess) return hip_fail(ce, "hipStreamEndCapture", __LINE__);
+        B.graph = g;
+        LSG_HIP(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
+        B.graph_key = key;
+    }
+    if (st) { hipEvent_t ev; LSG_HIP(hipEventCreate(&ev)); LSG_HIP(hipEventRecord(ev, st)); LSG_HIP(hipStreamWaitEvent(B.own, ev, 0)); (void)hipEventDestroy(ev); }
+    LSG_HIP(hipGraphLaunch(B.exec, B.own));
+    if (filtering && B.grow_steps) {
+        // the replayed grow ran a fixed number of steps; if its last step still changed
+        // texels the coverage changed since capture: finish eagerly like the reference
+        uint32_t last = 0;
+        LSG_HIP(hipMemcpyAsync(&last, B.S.changed, 4, hipMemcpyDeviceToHost, B.own));
+        LSG_HIP(hipStreamSynchronize(B.own));
+        if (last) {
+            uint32_t used = 0;
+            int e = run_all(B, first, final_params, filtering, B.own, -1, &used);
+            if (e) return e;
+            B.grow_steps = used;
+            B.graph_key.clear();
+        }
+    }
+    if (st) { hipEvent_t ev; LSG_HIP(hipEventCreate(&ev)); LSG_HIP(hipEventRecord(ev, B.own)); LSG_HIP(hipStreamWaitEvent(st, ev, 0)); (void)hipEventDestroy(ev); }
+    else LSG_HIP(hipStreamSynchronize(B.own));
+    return LSG_OK;
+}
+
+int lsg_bake_gather_sample(lsg_bake* b, const lsg_gather_params* p, uint32_t stride, float* ms_out)
+{
+    // times the final gather alone over every stride-th texel with the current tuning
+    if (!b || !p || !stride || !ms_out) return fail(LSG_ERR_INVALID, "lsg_bake_gather_sample: bad argument");
+    BakeImpl& B = b->impl;
+    std::vector<lsg_texel> all(B.ntex), pick;
+    LSG_HIP(hipMemcpy(all.data(), B.d_tex, sizeof(lsg_texel) * B.ntex, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < B.ntex; i += stride) pick.push_back(all[i]);
+    lsg_texel* d = nullptr;
+    LSG_HIP(hipMalloc(&d, sizeof(lsg_texel) * (pick.size() ? pick.size() : 1)));
+    hipError_t e = hipMemcpy(d, pick.data(), sizeof(lsg_texel) * pick.size(), hipMemcpyHostToDevice);
+    hipEvent_t a, z;
+    (void)hipEventCreate(&a); (void)hipEventCreate(&z);
+    int st = e == hipSuccess ? LSG_OK : hip_fail(e, "h2d", __LINE__);
+    if (!st) st = run_gather_final(B, p, d, (uint32_t)pick.size(), nullptr);   // warm
+    if (!st) { (void)hipEventRecord(a, nullptr); st = run_gather_final(B, p, d, (uint32_t)pick.size(), nullptr); (void)hipEventRecord(z, nullptr); }
+    if (!st && (e = hipEventSynchronize(z)) != hipSuccess) st = hip_fail(e, "sync", __LINE__);
+    if (!st) (void)hipEventElapsedTime(ms_out, a, z);
+    (void)hipEventDestroy(a); (void)hipEventDestroy(z);
+    dfree(d);
+    return st;
+}
+
+int lsg_bake_download(const lsg_bake* b, uint32_t object, float* rgbf)
+{
+    if (!b || !rgbf) return fail(LSG_ERR_INVALID, "lsg_bake_download: null argument");
+    const BakeImpl& B = b->impl;
+    if (object >= B.nmaps) return fail(LSG_ERR_INVALID, "lsg_bake_download: object %u of %u", object, B.nmaps);
+    const lsg_lightmap& m = B.maps[object];
+    LSG_HIP(hipMemcpy(rgbf, B.d_rgbf + 3 * (size_t)m.rgbf_base, 12 * (size_t)m.width * m.height, hipMemcpyDeviceToHost));
+    return LSG_OK;
+}
+
+} // extern "C"

@@ -20,7 +20,11 @@ struct FilterScratch {
 int maps_upload(const lsg_lightmap* maps, uint32_t nmaps, FilterMap** d_maps, uint32_t* total);
 int scratch_alloc(FilterScratch& S, uint32_t total);
 void scratch_free(FilterScratch& S);
+// fixed_steps < 0: lightmapGrow iterates until a step changes nothing (host checks
+// every 16 steps); fixed_steps >= 0: exactly that many steps, no host sync (graph
+// capture), S.changed[0] = whether the last step still changed a texel.
 int filter_run(const FilterMap* d_maps, uint32_t nmaps, uint32_t total, float4* rgba, float* rgbf,
-               const lsg_filter_params& fp, FilterScratch& S, hipStream_t s, bool rgba_out);
+               const lsg_filter_params& fp, FilterScratch& S, hipStream_t s, bool rgba_out,
+               int fixed_steps, uint32_t* steps_used);
 
 } // namespace lsg

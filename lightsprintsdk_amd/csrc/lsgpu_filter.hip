@@ -196,8 +196,14 @@ __global__ __launch_bounds__(256) void k_finish(const FilterMap* __restrict__ ma
 
 static inline unsigned nblk(uint32_t n) { return (n + 255) / 256; }
 
+__global__ void k_zero_u32(uint32_t* p, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
 int filter_run(const FilterMap* d_maps, uint32_t nmaps, uint32_t total, float4* rgba, float* rgbf, const lsg_filter_params& fp,
-               FilterScratch& S, hipStream_t s, bool rgba_out)
+               FilterScratch& S, hipStream_t s, bool rgba_out, int fixed_steps, uint32_t* steps_used)
 {
     if (!total) return LSG_OK;
     // lightmapSmooth (RRBuffer.cpp:630-802)
@@ -222,7 +228,21 @@ int filter_run(const FilterMap* d_maps, uint32_t nmaps, uint32_t total, float4* 
     // lightmapGrow(spread)
     const float4* g = nullptr;
     uint32_t steps = (uint32_t)(fp.spread_foreground > 0 ? fp.spread_foreground : 0);
-    if (steps) {
+    if (steps_used) *steps_used = 0;
+    if (steps && fixed_steps >= 0) {
+        // graph-capturable form: a fixed number of steps, no host synchronisation, the
+        // last step's change flag left in S.changed[0] for the caller to check
+        k_grow_init<<<nblk(total), 256, 0, s>>>(total, S.c, S.a);
+        float4* src = S.a; float4* dst = S.b;
+        const uint32_t n = std::min<uint32_t>((uint32_t)fixed_steps, steps);
+        for (uint32_t q = 0; q < n; q++) {
+            k_zero_u32<<<1, 256, 0, s>>>(S.changed, 1);
+            k_grow_step<<<nblk(total), 256, 0, s>>>(d_maps, nmaps, total, src, dst, S.changed);
+            std::swap(src, dst);
+        }
+        if (steps_used) *steps_used = n;
+        g = src;
+    } else if (steps) {
         k_grow_init<<<nblk(total), 256, 0, s>>>(total, S.c, S.a);
         float4* src = S.a; float4* dst = S.b;
         uint32_t done = 0;
@@ -236,11 +256,12 @@ int filter_run(const FilterMap* d_maps, uint32_t nmaps, uint32_t total, float4* 
             uint32_t ch[16] = {0};
             LSG_HIP(hipMemcpyAsync(ch, S.changed, 4 * chunk, hipMemcpyDeviceToHost, s));
             LSG_HIP(hipStreamSynchronize(s));
-            done += chunk;
             bool stop = false;
-            for (uint32_t q = 0; q < chunk; q++) if (!ch[q]) { stop = true; break; }
+            for (uint32_t q = 0; q < chunk; q++) if (!ch[q]) { stop = true; done += q + 1; break; }
             if (stop) break;   // a step without change: the reference stops (RRBuffer.cpp:948-949)
+            done += chunk;
         }
+        if (steps_used) *steps_used = done;
         g = src;
     }
     k_finish<<<nblk(total), 256, 0, s>>>(d_maps, nmaps, total, S.c, g, make_float4(fp.background[0], fp.background[1], fp.background[2], fp.background[3]),
@@ -305,7 +326,7 @@ extern "C" int lsg_filter_lightmaps(const lsg_lightmap* maps, uint32_t nmaps, fl
     if (st) { if (d_maps) (void)hipFree(d_maps); return st; }
     FilterScratch S;
     st = scratch_alloc(S, total);
-    if (!st) st = filter_run(d_maps, nmaps, total, (float4*)rgba, rgbf, *params, S, (hipStream_t)stream, true);
+    if (!st) st = filter_run(d_maps, nmaps, total, (float4*)rgba, rgbf, *params, S, (hipStream_t)stream, true, -1, nullptr);
     if (!st) { hipError_t e = hipStreamSynchronize((hipStream_t)stream); if (e != hipSuccess) st = hip_fail(e, "sync", __LINE__); }
     scratch_free(S);
     (void)hipFree(d_maps);

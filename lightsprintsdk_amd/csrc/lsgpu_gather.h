@@ -1,0 +1,56 @@
+// lsgpu_gather.h -- internal interface of the per-texel gather (lsgpu_gather.hip).
+#pragma once
+#include "lsgpu_internal.h"
+
+namespace lsg {
+
+constexpr int kMaxLights = 16;
+
+struct SceneView {
+    ColliderView col;
+    const float* verts;          // original vertex order, 3 floats each
+    const uint32_t* idx;         // 3 per triangle
+    const float* vnormal;        // 3 floats per vertex
+    const uint8_t* side;         // LSG_SIDE_* per triangle
+    const float4* exitance;      // 2 per triangle: front, back
+};
+
+struct GatherArgs {
+    SceneView sc;
+    const lsg_texel* texels; uint32_t ntex;
+    const lsg_subtexel* subs;
+    float4* out;
+    unsigned long long* stats;   // STATS builds: hemi rays, nodes, tris, shadow rays, nodes, tris
+    uint32_t quality; uint32_t hemi_rays; uint32_t rounds;
+    float tmin, locality, rug, light_mult; uint32_t seed;
+    const lsg_light* lights; uint32_t nlights;
+};
+
+// persistent launches: tiles of texels handed out by atomic counters, one queue per
+// XCD (workgroup b runs on XCD b % 8) so that each XCD's L2 sees a contiguous part of
+// the scene; a wave whose queue is empty steals from the next one.
+struct WorkQueue {
+    uint32_t* counters;    // nqueues counters, reset to 0 before each launch
+    uint32_t nqueues;      // 1 or 8
+    uint32_t ntiles;       // tiles of `tile` texels
+    uint32_t tile;
+};
+
+struct SceneImpl {
+    const lsg_collider* col = nullptr;
+    float* d_vnormal = nullptr;
+    uint8_t* d_side = nullptr;
+    float4* d_exit = nullptr;
+    lsg_light* d_lights = nullptr;     // kMaxLights slots
+    uint32_t nlights = 0;
+};
+
+int gather_args(lsg_scene* s, const lsg_texel* texels, uint32_t ntexels, const lsg_subtexel* subs,
+                const lsg_gather_params* p, float* out_rgba, GatherArgs& A, int& stack);
+int launch_gather(const GatherArgs& A, int stack, hipStream_t st);
+int launch_gather_persistent(const GatherArgs& A, int stack, const WorkQueue& Q, uint32_t waves_per_block, uint32_t blocks,
+                             hipStream_t st);
+
+} // namespace lsg
+
+struct lsg_scene { lsg::SceneImpl impl; };

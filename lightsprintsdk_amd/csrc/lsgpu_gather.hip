@@ -20,23 +20,13 @@
 // (:679-688) a stratified closed form; the CPU oracle uses the same rules.
 #include "lsgpu_internal.h"
 #include "lsgpu_trace.h"
+#include "lsgpu_gather.h"
 #include <vector>
 #include <cstring>
 
 namespace lsg {
 
-constexpr int kMaxLights = 16;
 
-struct LightsArg { lsg_light l[kMaxLights]; uint32_t n; };
-
-struct SceneView {
-    ColliderView col;
-    const float* verts;          // original vertex order, 3 floats each
-    const uint32_t* idx;         // 3 per triangle
-    const float* vnormal;        // 3 floats per vertex
-    const uint8_t* side;         // LSG_SIDE_* per triangle
-    const float4* exitance;      // 2 per triangle: front, back
-};
 
 __device__ inline uint32_t lowbias32(uint32_t x)
 {
@@ -140,28 +130,14 @@ struct GatherFilter {
     __device__ float clamp(float best) const { return mode == 1 ? fminf(best, rug_below) : best; }
 };
 
-struct GatherArgs {
-    SceneView sc;
-    const lsg_texel* texels; uint32_t ntex;
-    const lsg_subtexel* subs;
-    float4* out;
-    unsigned long long* stats;   // STATS builds: hemi rays, nodes, tris, shadow rays, nodes, tris
-    uint32_t quality; uint32_t hemi_rays; uint32_t rounds;
-    float tmin, locality, rug, light_mult; uint32_t seed;
-    LightsArg lights;
-};
-
+// processTexel for texel ti by the 64 lanes of one wavefront; mystk = this lane's
+// first LDS stack slot, slots `stride` ints apart
 template <int STACK, bool STATS>
-__global__ __launch_bounds__(256) void k_gather(GatherArgs A)
+__device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t lane, int32_t* mystk, int stride)
 {
-    __shared__ int32_t stk[STACK * 256];
-    const uint32_t tid = threadIdx.x, lane = tid & 63u;
-    const uint32_t ti = blockIdx.x * 4u + (tid >> 6);
-    if (ti >= A.ntex) return;                 // whole wavefront leaves together
-    int32_t* mystk = stk + tid;
     const lsg_texel T = A.texels[ti];
     const uint32_t m = T.sub_count_flags & 0xffffu, flags = (T.sub_count_flags >> 16) & 0xffu;
-    const uint32_t hemi_rays = A.hemi_rays, rounds = A.rounds, nl = A.lights.n;
+    const uint32_t hemi_rays = A.hemi_rays, rounds = A.rounds, nl = A.nlights;
     const uint32_t light_rays = nl * rounds;
     if (!m || (!hemi_rays && !light_rays)) {
         if (lane == 0) A.out[T.out_index] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -263,7 +239,7 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs A)
                 const float3 d = make_float3(nx * cosa + tx * x + bx * y, ny * cosa + ty * x + by * y, nz * cosa + tz * x + bz * y);
                 TraceResult r;
                 uint32_t nv_ = 0, nt_ = 0;
-                trace_closest<STACK, STATS>(flt, A.sc.col, o, d, A.tmin, A.locality, mystk, 256, r, nv_, nt_);
+                trace_closest<STACK, STATS>(flt, A.sc.col, o, d, A.tmin, A.locality, mystk, stride, r, nv_, nt_);
                 if (STATS) { stH[0]++; stH[1] += nv_; stH[2] += nt_; }
                 if (r.tri >= 0) {
                     const float4 e = A.sc.exitance[2 * (size_t)r.tri + (r.front ? 0 : 1)];
@@ -274,7 +250,7 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs A)
             if (doL) {
                 flt.mode = 1;
                 for (uint32_t li = 0; li < nl; li++) {
-                    const lsg_light L = A.lights.l[li];
+                    const lsg_light L = A.lights[li];
                     float dx, dy, dz;
                     if (L.type == 1) { dx = -L.direction[0]; dy = -L.direction[1]; dz = -L.direction[2]; }
                     else { dx = L.position[0] - o.x; dy = L.position[1] - o.y; dz = L.position[2] - o.z; }
@@ -287,7 +263,7 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs A)
                     if (L.cast_shadows) {
                         TraceResult r;
                         uint32_t nv_ = 0, nt_ = 0;
-                        trace_closest<STACK, STATS>(flt, A.sc.col, o, make_float3(dx, dy, dz), A.tmin, dsz, mystk, 256, r, nv_, nt_);
+                        trace_closest<STACK, STATS>(flt, A.sc.col, o, make_float3(dx, dy, dz), A.tmin, dsz, mystk, stride, r, nv_, nt_);
                         if (STATS) { stS[0]++; stS[1] += nv_; stS[2] += nt_; }
                         blocked = r.tri >= 0; bt = r.t;
                     }
@@ -328,19 +304,97 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs A)
     }
 }
 
-struct SceneImpl {
-    const lsg_collider* col = nullptr;
-    float* d_vnormal = nullptr;
-    uint8_t* d_side = nullptr;
-    float4* d_exit = nullptr;
-    LightsArg lights{};
-};
+template <int STACK, bool STATS>
+__global__ __launch_bounds__(256) void k_gather(GatherArgs A)
+{
+    __shared__ int32_t stk[STACK * 256];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    const uint32_t ti = blockIdx.x * 4u + (tid >> 6);
+    if (ti >= A.ntex) return;                 // whole wavefront leaves together
+    gather_texel<STACK, STATS>(A, ti, lane, stk + tid, 256);
+}
+
+__global__ void k_queue_reset(uint32_t* counters, uint32_t n)
+{
+    if (threadIdx.x < n) counters[threadIdx.x] = 0;
+}
+
+template <int STACK, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_gather_persistent(GatherArgs A, WorkQueue Q)
+{
+    __shared__ int32_t stk[STACK * 64 * WAVES];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    const uint32_t home = Q.nqueues > 1 ? blockIdx.x % Q.nqueues : 0;
+    for (uint32_t qi = 0; qi < Q.nqueues; qi++) {
+        const uint32_t q = (home + qi) % Q.nqueues;
+        const uint32_t t0 = (uint32_t)((uint64_t)Q.ntiles * q / Q.nqueues);
+        const uint32_t t1 = (uint32_t)((uint64_t)Q.ntiles * (q + 1) / Q.nqueues);
+        for (;;) {
+            uint32_t k = 0;
+            if (lane == 0) k = atomicAdd(&Q.counters[q], 1u);
+            k = (uint32_t)__shfl((int)k, 0);
+            const uint32_t tile = t0 + k;
+            if (tile >= t1) break;           // this queue is exhausted for every lane of the wave
+            const uint32_t a = tile * Q.tile, b = min(A.ntex, a + Q.tile);
+            for (uint32_t ti = a; ti < b; ti++)
+                gather_texel<STACK, false>(A, ti, lane, stk + tid, 64 * WAVES);
+        }
+    }
+}
 
 static inline void dfree(void* p) { if (p) (void)hipFree(p); }
 
+int gather_args(lsg_scene* s, const lsg_texel* texels, uint32_t ntexels, const lsg_subtexel* subs,
+                const lsg_gather_params* p, float* out_rgba, GatherArgs& A, int& stack)
+{
+    const ColliderImpl& C = s->impl.col->impl;
+    stack = stack_size_for_depth(C.depth);
+    if (!stack) return fail(LSG_ERR_LIMIT, "tree too deep");
+    if (!(p->ray_length_min >= 0) || !(p->rug_distance > 0) || !(p->locality > 0))
+        return fail(LSG_ERR_INVALID, "gather: ray_length_min >= 0, rug_distance > 0 and locality > 0 required");
+    A.sc.col = C.view();
+    A.sc.verts = C.d_verts; A.sc.idx = C.d_idx;
+    A.sc.vnormal = s->impl.d_vnormal; A.sc.side = s->impl.d_side; A.sc.exitance = s->impl.d_exit;
+    A.texels = texels; A.ntex = ntexels; A.subs = subs; A.out = (float4*)out_rgba; A.stats = nullptr;
+    A.quality = p->quality;
+    A.hemi_rays = p->hemisphere ? (p->quality > 1 ? p->quality : 1) : 0;             // gather.cpp:197
+    A.rounds = (p->light_multiplier != 0 && s->impl.nlights) ? p->quality / 10 + 1 : 0; // gather.cpp:396
+    A.tmin = p->ray_length_min; A.locality = p->locality; A.rug = p->rug_distance;
+    A.light_mult = p->light_multiplier; A.seed = p->seed;
+    A.lights = s->impl.d_lights; A.nlights = s->impl.nlights;
+    return LSG_OK;
+}
+
+int launch_gather_persistent(const GatherArgs& A, int stack, const WorkQueue& Q, uint32_t waves_per_block, uint32_t blocks,
+                             hipStream_t st)
+{
+    k_queue_reset<<<1, 64, 0, st>>>(Q.counters, Q.nqueues);
+    if (waves_per_block == 8) {
+        if (stack <= 32) k_gather_persistent<32, 8><<<blocks, 512, 0, st>>>(A, Q);
+        else k_gather_persistent<64, 8><<<blocks, 512, 0, st>>>(A, Q);
+    } else if (waves_per_block == 2) {
+        if (stack <= 32) k_gather_persistent<32, 2><<<blocks, 128, 0, st>>>(A, Q);
+        else k_gather_persistent<64, 2><<<blocks, 128, 0, st>>>(A, Q);
+    } else {
+        if (stack <= 32) k_gather_persistent<32, 4><<<blocks, 256, 0, st>>>(A, Q);
+        else k_gather_persistent<64, 4><<<blocks, 256, 0, st>>>(A, Q);
+    }
+    LSG_HIP(hipGetLastError());
+    return LSG_OK;
+}
+
+int launch_gather(const GatherArgs& A, int stack, hipStream_t st)
+{
+    if (!A.ntex) return LSG_OK;
+    const unsigned grid = (A.ntex + 3) / 4;
+    if (stack <= 32) k_gather<32, false><<<grid, 256, 0, st>>>(A);
+    else k_gather<64, false><<<grid, 256, 0, st>>>(A);
+    LSG_HIP(hipGetLastError());
+    return LSG_OK;
+}
+
 } // namespace lsg
 
-struct lsg_scene { lsg::SceneImpl impl; };
 
 using namespace lsg;
 
@@ -360,9 +414,8 @@ int lsg_scene_set_lights(lsg_scene* s, const lsg_light* lights, uint32_t n)
     if (n > (uint32_t)kMaxLights) return fail(LSG_ERR_LIMIT, "lsg_scene_set_lights: %u lights, at most %d", n, kMaxLights);
     for (uint32_t i = 0; i < n; i++)
         if (lights[i].type != 0 && lights[i].type != 1) return fail(LSG_ERR_INVALID, "light %u: type must be 0 (point) or 1 (directional)", i);
-    memset(&s->impl.lights, 0, sizeof s->impl.lights);
-    if (n) memcpy(s->impl.lights.l, lights, sizeof(lsg_light) * n);
-    s->impl.lights.n = n;
+    if (n) LSG_HIP(hipMemcpy(s->impl.d_lights, lights, sizeof(lsg_light) * n, hipMemcpyHostToDevice));
+    s->impl.nlights = n;
     return LSG_OK;
 }
 
@@ -377,8 +430,9 @@ int lsg_scene_create(const lsg_collider* col, const float* vertex_normals, const
     SceneImpl& S = s->impl;
     S.col = col;
     const uint32_t nv = col->impl.nverts, nt = col->impl.ntri;
-    auto bail = [&](int st) { dfree(S.d_vnormal); dfree(S.d_side); dfree(S.d_exit); delete s; return st; };
+    auto bail = [&](int st) { dfree(S.d_vnormal); dfree(S.d_side); dfree(S.d_exit); dfree(S.d_lights); delete s; return st; };
     hipError_t e;
+    if ((e = hipMalloc(&S.d_lights, sizeof(lsg_light) * kMaxLights)) != hipSuccess) return bail(hip_fail(e, "hipMalloc", __LINE__));
     if ((e = hipMalloc(&S.d_vnormal, 12 * (size_t)(nv ? nv : 1))) != hipSuccess) return bail(hip_fail(e, "hipMalloc", __LINE__));
     if ((e = hipMalloc(&S.d_side, (size_t)(nt ? nt : 1))) != hipSuccess) return bail(hip_fail(e, "hipMalloc", __LINE__));
     if ((e = hipMalloc(&S.d_exit, 32 * (size_t)(nt ? nt : 1))) != hipSuccess) return bail(hip_fail(e, "hipMalloc", __LINE__));
@@ -394,7 +448,7 @@ int lsg_scene_create(const lsg_collider* col, const float* vertex_normals, const
 void lsg_scene_destroy(lsg_scene* s)
 {
     if (!s) return;
-    dfree(s->impl.d_vnormal); dfree(s->impl.d_side); dfree(s->impl.d_exit);
+    dfree(s->impl.d_vnormal); dfree(s->impl.d_side); dfree(s->impl.d_exit); dfree(s->impl.d_lights);
     delete s;
 }
 
@@ -404,22 +458,11 @@ static int gather_launch(lsg_scene* s, const lsg_texel* texels, uint32_t ntexels
     if (!s || !p) return fail(LSG_ERR_INVALID, "lsg_gather_device: null argument");
     if (!ntexels) return LSG_OK;
     if (!texels || !subs || !out_rgba || !nsubs) return fail(LSG_ERR_INVALID, "lsg_gather_device: null buffer");
-    const ColliderImpl& C = s->impl.col->impl;
-    const int stack = stack_size_for_depth(C.depth);
-    if (!stack) return fail(LSG_ERR_LIMIT, "tree too deep");
-    if (!(p->ray_length_min >= 0) || !(p->rug_distance > 0) || !(p->locality > 0))
-        return fail(LSG_ERR_INVALID, "lsg_gather_device: ray_length_min >= 0, rug_distance > 0 and locality > 0 required");
     GatherArgs A;
-    A.sc.col = C.view();
-    A.sc.verts = C.d_verts; A.sc.idx = C.d_idx;
-    A.sc.vnormal = s->impl.d_vnormal; A.sc.side = s->impl.d_side; A.sc.exitance = s->impl.d_exit;
-    A.texels = texels; A.ntex = ntexels; A.subs = subs; A.out = (float4*)out_rgba; A.stats = stats;
-    A.quality = p->quality;
-    A.hemi_rays = p->hemisphere ? (p->quality > 1 ? p->quality : 1) : 0;           // gather.cpp:197
-    A.rounds = (p->light_multiplier != 0 && s->impl.lights.n) ? p->quality / 10 + 1 : 0;  // gather.cpp:396
-    A.tmin = p->ray_length_min; A.locality = p->locality; A.rug = p->rug_distance;
-    A.light_mult = p->light_multiplier; A.seed = p->seed;
-    A.lights = s->impl.lights;
+    int stack = 0;
+    int st = gather_args(s, texels, ntexels, subs, p, out_rgba, A, stack);
+    if (st) return st;
+    A.stats = stats;
     const unsigned grid = (ntexels + 3) / 4;
     if (stats) {
         if (stack <= 32) k_gather<32, true><<<grid, 256, 0, (hipStream_t)stream>>>(A);

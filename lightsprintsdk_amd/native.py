@@ -52,6 +52,22 @@ class FilterParams(ctypes.Structure):
                 ("background", ctypes.c_float * 4), ("wrap", I32)]
 
 
+class BakeObject(ctypes.Structure):
+    _fields_ = [("tri_uv", P), ("tri_ids", P), ("ntri", U32), ("width", U32), ("height", U32)]
+
+
+class BakeTuning(ctypes.Structure):
+    _fields_ = [("tile", U32), ("waves_per_block", U32), ("blocks_per_cu", U32), ("xcd_queues", U32)]
+
+
+class BakeInfo(ctypes.Structure):
+    _fields_ = [("num_texels", U32), ("num_subtexels", U32), ("num_slots", U32), ("num_lightmaps", U32),
+                ("num_triangles", U32), ("compute_units", U32), ("grow_steps", U32), ("tuning", BakeTuning),
+                ("rgba", P), ("rgbf", P), ("texels", P), ("subtexels", P)]
+
+
+BAKE_USE_GRAPH = 1
+
 # name -> (restype, argtypes); restype None means int status checked by check()
 SIGNATURES = {
     "lsg_last_error": (ctypes.c_char_p, []),
@@ -86,6 +102,13 @@ SIGNATURES = {
     "lsg_gather_device": (ctypes.c_int, [P, P, U32, P, U32, ctypes.POINTER(GatherParams), P, P]),
     "lsg_gather_stats": (ctypes.c_int, [P, P, U32, P, U32, ctypes.POINTER(GatherParams), P, ctypes.POINTER(ctypes.c_double)]),
     "lsg_filter_lightmaps": (ctypes.c_int, [P, U32, P, P, ctypes.POINTER(FilterParams), P]),
+    "lsg_bake_create": (ctypes.c_int, [P, P, U32, P, P, ctypes.POINTER(P)]),
+    "lsg_bake_destroy": (None, [P]),
+    "lsg_bake_info_get": (ctypes.c_int, [P, ctypes.POINTER(BakeInfo)]),
+    "lsg_bake_set_tuning": (ctypes.c_int, [P, ctypes.POINTER(BakeTuning)]),
+    "lsg_bake_run": (ctypes.c_int, [P, ctypes.POINTER(GatherParams), ctypes.POINTER(GatherParams), ctypes.POINTER(FilterParams), U32, P]),
+    "lsg_bake_gather_sample": (ctypes.c_int, [P, ctypes.POINTER(GatherParams), U32, ctypes.POINTER(ctypes.c_float)]),
+    "lsg_bake_download": (ctypes.c_int, [P, U32, P]),
     "lsg_rasterize_texels": (ctypes.c_int, [P, P, U32, U32, U32, U32, P, ctypes.POINTER(U32), P, ctypes.POINTER(U32), U32]),
 }
 

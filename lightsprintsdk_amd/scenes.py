@@ -294,3 +294,139 @@ def minimal_safe_distance(verts: np.ndarray) -> float:
     e = (hi - lo).astype(np.float32)
     avg = np.float32((np.float32(e[0] + e[1]) + e[2]) * np.float32(0.33333333333333))
     return float(np.float32(avg * np.float32(1e-6)))
+
+
+# ---------------------------------------------------------------------------------------
+# batched-bake scene: many objects of uneven lightmap sizes
+
+def _grid_normals(verts, tris):
+    n = np.zeros_like(verts)
+    fn = np.cross(verts[tris[:, 1]] - verts[tris[:, 0]], verts[tris[:, 2]] - verts[tris[:, 0]])
+    for k in range(3):
+        np.add.at(n, tris[:, k], fn)
+    ln = np.linalg.norm(n, axis=1, keepdims=True)
+    ln[ln == 0] = 1
+    return n / ln
+
+
+def _chart_object(charts, lightmap_size, gutter_texels=3.0):
+    """charts: list of (verts [n,3], tris [m,3], st [n,2] in [0,1]^2 chart coords, (w,h) chart size).
+    Packs the charts into one unwrap; returns verts, tris, normals, per-triangle uv [m,6]."""
+    sizes = [c[3] for c in charts]
+    pos = _pack_charts(sizes, gutter_texels / lightmap_size)
+    V, I, N, UV = [], [], [], []
+    nv = 0
+    for (v, t, st, (w, h)), (cx, cy, sc) in zip(charts, pos):
+        uv_v = np.stack([cx + st[:, 0] * w * sc, cy + st[:, 1] * h * sc], axis=1)
+        V.append(v)
+        N.append(_grid_normals(v, t))
+        I.append(t + nv)
+        UV.append(uv_v[t].reshape(-1, 6))
+        nv += len(v)
+    return np.concatenate(V), np.concatenate(I), np.concatenate(N), np.concatenate(UV)
+
+
+def _displaced_box(rng, lo, hi, cells_per_m, amp):
+    charts = []
+    for o, du, dv in _box_faces(lo, hi, inward=False):
+        w, h = np.linalg.norm(du), np.linalg.norm(dv)
+        nu = max(2, int(round(w * cells_per_m)))
+        nvv = max(2, int(round(h * cells_per_m)))
+        v, t, st = _grid_face(o, du, dv, nu, nvv)
+        nrm = np.cross(du, dv)
+        nrm /= np.linalg.norm(nrm)
+        interior = (st[:, 0] > 0) & (st[:, 0] < 1) & (st[:, 1] > 0) & (st[:, 1] < 1)
+        v = v + nrm[None, :] * (rng.uniform(-amp, amp, size=len(v)) * interior)[:, None]
+        charts.append((v, t, st, (w, h)))
+    return charts
+
+
+def _cylinder(rng, center, radius, height, cells_per_m):
+    seg = max(8, int(round(2 * np.pi * radius * cells_per_m)))
+    rows = max(2, int(round(height * cells_per_m)))
+    s = np.linspace(0, 1, seg + 1)
+    z = np.linspace(0, 1, rows + 1)
+    S, Z = np.meshgrid(s, z, indexing="ij")
+    ang = S * 2 * np.pi
+    v = np.stack([center[0] + radius * np.cos(ang), center[1] + radius * np.sin(ang), center[2] + Z * height], -1).reshape(-1, 3)
+    st = np.stack([S.reshape(-1), Z.reshape(-1)], 1)
+    t = []
+    for i in range(seg):
+        for j in range(rows):
+            a = i * (rows + 1) + j
+            b = (i + 1) * (rows + 1) + j
+            t.append((a, b, b + 1))
+            t.append((a, b + 1, a + 1))
+    charts = [(v, np.array(t), st, (2 * np.pi * radius, height))]
+    # caps as fans (top faces up, bottom faces down)
+    for top in (True, False):
+        ang2 = np.linspace(0, 2 * np.pi, seg, endpoint=False)
+        ring = np.stack([center[0] + radius * np.cos(ang2), center[1] + radius * np.sin(ang2),
+                         np.full(seg, center[2] + (height if top else 0.0))], 1)
+        cv = np.concatenate([np.array([[center[0], center[1], center[2] + (height if top else 0.0)]]), ring])
+        cst = np.concatenate([np.array([[0.5, 0.5]]), 0.5 + 0.5 * np.stack([np.cos(ang2), np.sin(ang2)], 1)])
+        ct = np.array([(0, 1 + k, 1 + (k + 1) % seg) if top else (0, 1 + (k + 1) % seg, 1 + k) for k in range(seg)])
+        charts.append((cv, ct, cst, (2 * radius, 2 * radius)))
+    return charts
+
+
+def _terrain(rng, origin, size, cells, amp):
+    v, t, st = _grid_face(np.array(origin, np.float64), np.array([size, 0.0, 0.0]), np.array([0.0, size, 0.0]), cells, cells)
+    k = rng.uniform(0.5, 2.0, size=(3, 2)) * 2 * np.pi / size
+    ph = rng.uniform(0, 2 * np.pi, size=3)
+    hgt = sum(np.sin(k[i, 0] * v[:, 0] + k[i, 1] * v[:, 1] + ph[i]) for i in range(3)) * amp / 3
+    v = v.copy()
+    v[:, 2] += hgt
+    return [(v, t, st, (size, size))]
+
+
+def object_field(seed: int = 4, n_objects: int = 200, target_tris: int = 2_000_000, lightmap_range=(32, 1024),
+                 area: float = 200.0) -> BakeScene:
+    """Config 3: displaced boxes, cylinders and terrain patches from one seed, each with its
+    own packed unwrap and a lightmap side log-uniform in lightmap_range; one directional
+    sun and four point lights."""
+    rng = np.random.default_rng(seed)
+    per_obj = target_tris / n_objects
+    V, I, N, UV, SB, ALB, EM = [], [], [], [], [], [], []
+    objects = []
+    nv = 0
+    ntri = 0
+    grid = int(np.ceil(np.sqrt(n_objects)))
+    cell = area / grid
+    for k in range(n_objects):
+        kind = k % 3
+        gx, gy = k % grid, k // grid
+        base = np.array([gx * cell, gy * cell, 0.0])
+        lm = int(round(2 ** rng.uniform(np.log2(lightmap_range[0]), np.log2(lightmap_range[1]))))
+        if kind == 0:
+            sz = rng.uniform(1.0, cell * 0.6, size=3)
+            area_o = 2 * (sz[0] * sz[1] + sz[1] * sz[2] + sz[0] * sz[2])
+            cpm = np.sqrt(per_obj / 2 / area_o)
+            charts = _displaced_box(rng, base + [0.5, 0.5, 0.0], base + [0.5, 0.5, 0.0] + sz, cpm, 0.02 * sz.min())
+        elif kind == 1:
+            r = rng.uniform(0.5, cell * 0.3)
+            hgt = rng.uniform(1.0, cell * 0.8)
+            area_o = 2 * np.pi * r * hgt + 2 * np.pi * r * r
+            cpm = np.sqrt(per_obj / 2 / area_o)
+            charts = _cylinder(rng, base + [cell / 2, cell / 2, 0.0], r, hgt, cpm)
+        else:
+            cells = max(4, int(np.sqrt(per_obj / 2)))
+            charts = _terrain(rng, base + [0.0, 0.0, -0.5], cell * 0.95, cells, cell * 0.05)
+        v, t, n, uv = _chart_object(charts, lm)
+        albedo = rng.uniform(0.2, 0.8, size=3).astype(np.float32)
+        V.append(v); I.append(t + nv); N.append(n); UV.append(uv)
+        SB.append(np.full(len(t), 1 | 4, np.uint8))
+        ALB.append(np.repeat(albedo[None, :], len(t), 0))
+        EM.append(np.zeros((len(t), 3), np.float32))
+        objects.append(LightmapObject(f"obj{k}", ntri, len(t), lm, lm, tuple(float(a) for a in albedo)))
+        nv += len(v)
+        ntri += len(t)
+    lights = [dict(type=1, position=(0, 0, 0), direction=tuple(np.array([0.3, 0.2, -1.0]) / np.linalg.norm([0.3, 0.2, -1.0])),
+                   color=(3.0, 2.9, 2.6), cast_shadows=1)]
+    for j in range(4):
+        p = (rng.uniform(0, area), rng.uniform(0, area), rng.uniform(5, 15))
+        lights.append(dict(type=0, position=p, direction=(0, 0, -1), color=(400.0, 380.0, 350.0), cast_shadows=1))
+    return BakeScene(verts=np.concatenate(V).astype(np.float32), idx=np.concatenate(I).astype(np.uint32),
+                     normals=np.concatenate(N).astype(np.float32), tri_uv=np.concatenate(UV).astype(np.float32),
+                     side_bits=np.concatenate(SB), albedo=np.concatenate(ALB).astype(np.float32),
+                     emittance=np.concatenate(EM).astype(np.float32), objects=objects, lights=lights)
