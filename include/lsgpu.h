@@ -184,7 +184,8 @@ typedef struct lsg_gather_params {
     float    rug_distance;     /* default 0.001 */
     float    light_multiplier; /* direct.lightMultiplier */
     uint32_t seed;             /* seeded replacement of rand() (gather.h:102, gather.cpp:742) */
-    uint32_t reserved;
+    uint32_t light_rounds;     /* 0: quality/10+1 as the reference; else shadow rounds per light
+                                  (the learned path's cheap gather) */
 } lsg_gather_params;
 
 int  lsg_scene_create(const lsg_collider* collider, const float* vertex_normals, const uint8_t* side_bits,
@@ -298,6 +299,56 @@ int  lsg_bake_run(lsg_bake* bake, const lsg_gather_params* first, const lsg_gath
 int  lsg_bake_gather_sample(lsg_bake* bake, const lsg_gather_params* params, uint32_t stride, float* ms);
 /* copies object `object`'s RGBF lightmap (width*height*3 floats) to the host */
 int  lsg_bake_download(const lsg_bake* bake, uint32_t object, float* rgbf);
+
+/* ---- learned gather ----------------------------------------------------------------------
+ * No counterpart in the reference: a surrogate that replaces most of the 1024-ray
+ * final gather.  A cheap gather (few rays) feeds an MLP 32-64-64-64-64-4 (BF16 on
+ * MFMA, FP32 accumulation) that predicts log1p(E/s) per channel plus its own RMS log
+ * error; texels whose predicted error exceeds the bound are re-gathered by the plain
+ * gather (processTexel, gather.cpp:600-836), which stays the judge of the result. */
+typedef struct lsg_mlp lsg_mlp;
+
+uint32_t lsg_mlp_num_params(void);
+/* max_batch: rows per training step / inference chunk, multiple of 16 */
+int  lsg_mlp_create(uint32_t max_batch, lsg_mlp** out);
+void lsg_mlp_destroy(lsg_mlp* mlp);
+/* params: host float[lsg_mlp_num_params()], per layer W[in][out] row-major then b[out]
+ * (layers 32x64, 64x64, 64x64, 64x64, 64x16 with outputs 4..15 held at zero);
+ * setting parameters resets the Adam state */
+int  lsg_mlp_set_params(lsg_mlp* mlp, const float* params);
+int  lsg_mlp_get_params(const lsg_mlp* mlp, float* params);
+int  lsg_mlp_get_grads(const lsg_mlp* mlp, float* grads);
+/* x: device float[n*32]; y: device float[n*4] */
+int  lsg_mlp_forward(lsg_mlp* mlp, const float* x, uint32_t n, float* y, void* stream);
+/* one Adam step on n <= max_batch rows with targets t (device float[n*3]); lr <= 0
+ * only computes gradients (lsg_mlp_get_grads); *loss (host, optional) = mean loss */
+int  lsg_mlp_train_step(lsg_mlp* mlp, const float* x, const float* t, uint32_t n, float lambda, float lr,
+                        float* loss, void* stream);
+/* C[MxN] = A[MxK] B[KxN] on the device through the MLP's MFMA GEMM (bf16 inputs,
+ * row-major; M, N, K multiples of 16) -- exposed to verify the kernel */
+int  lsg_mlp_gemm_test(const uint16_t* a, const uint16_t* b, float* c, uint32_t m, uint32_t n, uint32_t k);
+
+/* the gather with its hemisphere and light parts written separately: parts =
+ * device float4[2*ntexels] per texel list entry (rgb, reliable) */
+int  lsg_gather_parts_device(lsg_scene* scene, const lsg_texel* texels, uint32_t ntexels,
+                             const lsg_subtexel* subs, const lsg_gather_params* params,
+                             float* out_rgba, float* parts, void* stream);
+/* scale s = mean cheap irradiance over reliable texels (synchronous) */
+int  lsg_learned_scale(const float* parts, uint32_t n, float* scale);
+/* the 32 features per texel (device float[n*32]): the cheap parts, their means over
+ * 3x3 / 5x5 / 11x11 windows of same-facing covered texels in the texel's lightmap,
+ * position, normal, albedo, spreads, window coverage, the lights' unshadowed direct term.  tri_albedo: device float[ntri*3];
+ * maps: host array of the lightmaps the texels' out_index slots fall in, sorted by
+ * rgba_base, inside [0, nslots); work: device float4[3*nslots] scratch (synchronous) */
+int  lsg_learned_features(lsg_scene* scene, const lsg_texel* texels, uint32_t n, const lsg_subtexel* subs,
+                          const float* parts, const float* tri_albedo, float scale, const lsg_lightmap* maps,
+                          uint32_t nmaps, uint32_t nslots, float* work, float* x, void* stream);
+/* writes s*expm1(y) into out_rgba for texels whose cheap pass was reliable and whose
+ * predicted absolute error |y3| * (1 + mean expm1(y0..2)) -- in units of s, the
+ * scene's mean irradiance -- is at most tau; appends the others to fallback and counts
+ * them in *nfallback (device counter, caller zeroes) */
+int  lsg_learned_decide(const float* y, const lsg_texel* texels, const float* parts, uint32_t n, float scale,
+                        float tau, float* out_rgba, lsg_texel* fallback, uint32_t* nfallback, void* stream);
 
 #ifdef __cplusplus
 }

@@ -63,7 +63,7 @@ class GatherParams:
     """RRSolver::UpdateParameters(quality) defaults (RRSolver.h:570-592)."""
 
     def __init__(self, quality=1024, hemisphere=1, ray_length_min=1e-6, locality=100000.0, rug_distance=0.001,
-                 light_multiplier=1.0, seed=1):
+                 light_multiplier=1.0, seed=1, light_rounds=0):
         self.quality = quality
         self.hemisphere = hemisphere
         self.ray_length_min = ray_length_min
@@ -71,15 +71,16 @@ class GatherParams:
         self.rug_distance = rug_distance
         self.light_multiplier = light_multiplier
         self.seed = seed
+        self.light_rounds = light_rounds      # 0: quality/10+1 (gather.cpp:396)
 
     def c(self):
         return N.GatherParams(self.quality, self.hemisphere, self.ray_length_min, self.locality, self.rug_distance,
-                              self.light_multiplier, self.seed, 0)
+                              self.light_multiplier, self.seed, self.light_rounds)
 
     def as_dict(self):
         return dict(quality=self.quality, hemisphere=self.hemisphere, ray_length_min=self.ray_length_min,
                     locality=self.locality, rug_distance=self.rug_distance, light_multiplier=self.light_multiplier,
-                    seed=self.seed)
+                    seed=self.seed, light_rounds=self.light_rounds)
 
 
 class FilterParams:
@@ -243,8 +244,14 @@ class SceneBake:
         N.check(N.lib().lsg_filter_lightmaps(self.maps, len(self.slots), self.d_out.ptr, self.d_rgbf.ptr, ctypes.byref(fp),
                                              stream), "lsg_filter_lightmaps")
 
+    def clear(self):
+        """Empty every lightmap slot (the filters grow into uncovered slots in place, and
+        the reference gathers each updateLightmaps pass into a fresh buffer)."""
+        N.check(N.lib().lsg_memset(self.d_out.ptr, 0, 16 * max(self.nout, 1)), "memset")
+
     def bake(self, stream=None):
         """One updateLightmaps pass after the first gather: final gather + filters."""
+        self.clear()
         self.final_gather(stream)
         self.filter(stream)
 

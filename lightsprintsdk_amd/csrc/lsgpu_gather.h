@@ -20,6 +20,7 @@ struct GatherArgs {
     const lsg_texel* texels; uint32_t ntex;
     const lsg_subtexel* subs;
     float4* out;
+    float4* parts;               // optional: per texel list entry, (hemisphere rgb, reliable) and (lights rgb, reliable)
     unsigned long long* stats;   // STATS builds: hemi rays, nodes, tris, shadow rays, nodes, tris
     uint32_t quality; uint32_t hemi_rays; uint32_t rounds;
     float tmin, locality, rug, light_mult; uint32_t seed;

@@ -7,7 +7,8 @@
 // the collision-handler rules of RRCollisionHandlerFinalGathering
 // (RRStaticSolver/pathtracer.h:117-252).
 //
-// Work mapping: one 64-lane wavefront per texel.  The texel's shooters (1024 at
+// Work mapping: one 64-lane wavefront per texel (at most 16 or 32 shooters per
+// series: a group of 16 or 32 lanes per texel).  The texel's shooters (1024 at
 // quality 1024) are dealt to lanes 64 at a time; every lane casts the hemisphere
 // ray of its shooter and, on the shooters the reference's interleave rule picks,
 // one shadow ray per light.  Hemisphere directions come from the reference's
@@ -86,14 +87,17 @@ __device__ inline uint32_t nth_set_bit(uint64_t m, uint32_t k)
     return lo;
 }
 
-__device__ inline float wave_sum(float v)
+// sums over aligned groups of G lanes (G a power of two <= 64; 64 = the whole wave).
+// Lanes with nothing to add hold 0, so for a series of at most 16 shooters the three
+// widths give the same bits.
+template <int G> __device__ inline float group_sum(float v)
 {
-    for (int o = 32; o; o >>= 1) v += __shfl_xor(v, o);
+    for (int o = G / 2; o; o >>= 1) v += __shfl_xor(v, o);
     return v;
 }
-__device__ inline uint32_t wave_sum_u(uint32_t v)
+template <int G> __device__ inline uint32_t group_sum_u(uint32_t v)
 {
-    for (int o = 32; o; o >>= 1) v += __shfl_xor(v, o);
+    for (int o = G / 2; o; o >>= 1) v += __shfl_xor(v, o);
     return v;
 }
 
@@ -130,17 +134,26 @@ struct GatherFilter {
     __device__ float clamp(float best) const { return mode == 1 ? fminf(best, rug_below) : best; }
 };
 
-// processTexel for texel ti by the 64 lanes of one wavefront; mystk = this lane's
-// first LDS stack slot, slots `stride` ints apart
-template <int STACK, bool STATS>
-__device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t lane, int32_t* mystk, int stride)
+// processTexel for texel ti by an aligned group of G lanes of one wavefront (G = 64:
+// the whole wave; 16 or 32 for low ray counts, several texels per wave); lane = the
+// lane in the wave; mystk = this lane's first LDS stack slot, slots `stride` ints
+// apart.  Every group runs its own texel: control flow is uniform within a group.
+template <int STACK, bool STATS, int G = 64>
+__device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, int32_t* mystk, int stride)
 {
+    static_assert(G == 16 || G == 32 || G == 64, "group width");
+    const uint32_t lane = wlane & (uint32_t)(G - 1);               // lane in the group
+    const uint32_t gbase = wlane & ~(uint32_t)(G - 1);             // group's first lane in the wave
+    const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
     const lsg_texel T = A.texels[ti];
     const uint32_t m = T.sub_count_flags & 0xffffu, flags = (T.sub_count_flags >> 16) & 0xffu;
     const uint32_t hemi_rays = A.hemi_rays, rounds = A.rounds, nl = A.nlights;
     const uint32_t light_rays = nl * rounds;
     if (!m || (!hemi_rays && !light_rays)) {
-        if (lane == 0) A.out[T.out_index] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (lane == 0) {
+            A.out[T.out_index] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (A.parts) { A.parts[2 * (size_t)ti] = make_float4(0.f, 0.f, 0.f, 0.f); A.parts[2 * (size_t)ti + 1] = make_float4(0.f, 0.f, 0.f, 0.f); }
+        }
         return;
     }
     const lsg_subtexel* ts = A.subs + T.sub_begin;
@@ -164,7 +177,7 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t lane, in
         const bool allH = !shootL || hemi_rays >= rounds;
         const bool allL = !shootH || rounds >= hemi_rays;
         uint32_t cRelH = 0, cRelL = 0, cUnrelL = 0, cRounds = 0;      // this lane's counts in the series
-        for (uint32_t base = 0; base < n; base += 64) {
+        for (uint32_t base = 0; base < n; base += G) {
             const uint32_t s = base + lane;
             const bool active = s < n;
             bool doH = false, doL = false;
@@ -172,7 +185,7 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t lane, in
             if (active && shootL) doL = allL || shots_before(s + 1, n, rounds) > shots_before(s, n, rounds);
             // hand out filler points: hemisphere shooters of this chunk take the next
             // accepted filler indices in lane order (GetCirclePoint, gather.cpp:85-90)
-            const uint64_t hmask = __ballot(doH);
+            const uint64_t hmask = (__ballot(doH) >> gbase) & gmask;
             const uint32_t need = (uint32_t)__popcll(hmask);
             const uint32_t myrank = (uint32_t)__popcll(hmask & lanemask_lt(lane));
             uint32_t my_index = 0;
@@ -182,16 +195,16 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t lane, in
                 float cx, cy;
                 filler_point(cand, cx, cy);
                 const bool ok = !(cx * cx + cy * cy >= 1.f);
-                const uint64_t am = __ballot(ok);
+                const uint64_t am = (__ballot(ok) >> gbase) & gmask;
                 const uint32_t na = (uint32_t)__popcll(am);
                 const uint32_t want = doH ? myrank - got : 64u;
                 const uint32_t src = (doH && myrank >= got && want < na) ? nth_set_bit(am, want) : lane;
-                const uint32_t val = (uint32_t)__shfl((int)cand, (int)src);
+                const uint32_t val = (uint32_t)__shfl((int)cand, (int)(gbase + src));
                 if (doH && myrank >= got && want < na) my_index = val;
-                if (got + na <= need) { got += na; filler += 64u; }
+                if (got + na <= need) { got += na; filler += (uint32_t)G; }
                 else {
                     const uint32_t last = nth_set_bit(am, need - got - 1u);
-                    filler = (uint32_t)__shfl((int)cand, (int)last) + 1u;
+                    filler = (uint32_t)__shfl((int)cand, (int)(gbase + last)) + 1u;
                     got = need;
                 }
             }
@@ -283,8 +296,8 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t lane, in
                 cRounds++;
             }
         }
-        relH += wave_sum_u(cRelH); relL += wave_sum_u(cRelL); unrelL += wave_sum_u(cUnrelL);
-        shotRounds += wave_sum_u(cRounds);
+        relH += group_sum_u<G>(cRelH); relL += group_sum_u<G>(cRelL); unrelL += group_sum_u<G>(cUnrelL);
+        shotRounds += group_sum_u<G>(cRounds);
         shooter_base += n;
     }
     if (STATS) {
@@ -293,14 +306,23 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t lane, in
             atomicAdd(&A.stats[3 + k], (unsigned long long)stS[k]);
         }
     }
-    hr = wave_sum(hr); hg = wave_sum(hg); hb = wave_sum(hb);
-    lr = wave_sum(lr); lg = wave_sum(lg); lb = wave_sum(lb);
+    hr = group_sum<G>(hr); hg = group_sum<G>(hg); hb = group_sum<G>(hb);
+    lr = group_sum<G>(lr); lg = group_sum<G>(lg); lb = group_sum<G>(lb);
     if (lane == 0) {
         float r = 0.f, g = 0.f, b = 0.f;
+        float4 ph = make_float4(0.f, 0.f, 0.f, 0.f), pl = make_float4(0.f, 0.f, 0.f, 0.f);
         bool reliable = false;
-        if (hemi_rays && relH) { const float f = 1 / (float)relH; r += hr * f; g += hg * f; b += hb * f; reliable = true; }
-        if (light_rays && relL) { r += lr / (float)shotRounds; g += lg / (float)shotRounds; b += lb / (float)shotRounds; reliable = true; }
+        if (hemi_rays && relH) {
+            const float f = 1 / (float)relH;
+            ph = make_float4(hr * f, hg * f, hb * f, 1.f);
+            r += ph.x; g += ph.y; b += ph.z; reliable = true;
+        }
+        if (light_rays && relL) {
+            pl = make_float4(lr / (float)shotRounds, lg / (float)shotRounds, lb / (float)shotRounds, 1.f);
+            r += pl.x; g += pl.y; b += pl.z; reliable = true;
+        }
         A.out[T.out_index] = make_float4(r, g, b, reliable ? flags / 255.f : 0.f);
+        if (A.parts) { A.parts[2 * (size_t)ti] = ph; A.parts[2 * (size_t)ti + 1] = pl; }
     }
 }
 
@@ -312,6 +334,17 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs A)
     const uint32_t ti = blockIdx.x * 4u + (tid >> 6);
     if (ti >= A.ntex) return;                 // whole wavefront leaves together
     gather_texel<STACK, STATS>(A, ti, lane, stk + tid, 256);
+}
+
+// low ray counts (at most G shooters per series): 256/G texels per block, G lanes each
+template <int STACK, int G>
+__global__ __launch_bounds__(256) void k_gather_grouped(GatherArgs A)
+{
+    __shared__ int32_t stk[STACK * 256];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    const uint32_t ti = blockIdx.x * (256u / G) + tid / G;
+    if (ti >= A.ntex) return;                 // the whole group leaves together
+    gather_texel<STACK, false, G>(A, ti, lane, stk + tid, 256);
 }
 
 __global__ void k_queue_reset(uint32_t* counters, uint32_t n)
@@ -355,10 +388,10 @@ int gather_args(lsg_scene* s, const lsg_texel* texels, uint32_t ntexels, const l
     A.sc.col = C.view();
     A.sc.verts = C.d_verts; A.sc.idx = C.d_idx;
     A.sc.vnormal = s->impl.d_vnormal; A.sc.side = s->impl.d_side; A.sc.exitance = s->impl.d_exit;
-    A.texels = texels; A.ntex = ntexels; A.subs = subs; A.out = (float4*)out_rgba; A.stats = nullptr;
+    A.texels = texels; A.ntex = ntexels; A.subs = subs; A.out = (float4*)out_rgba; A.stats = nullptr; A.parts = nullptr;
     A.quality = p->quality;
     A.hemi_rays = p->hemisphere ? (p->quality > 1 ? p->quality : 1) : 0;             // gather.cpp:197
-    A.rounds = (p->light_multiplier != 0 && s->impl.nlights) ? p->quality / 10 + 1 : 0; // gather.cpp:396
+    A.rounds = (p->light_multiplier != 0 && s->impl.nlights) ? (p->light_rounds ? p->light_rounds : p->quality / 10 + 1) : 0; // gather.cpp:396
     A.tmin = p->ray_length_min; A.locality = p->locality; A.rug = p->rug_distance;
     A.light_mult = p->light_multiplier; A.seed = p->seed;
     A.lights = s->impl.d_lights; A.nlights = s->impl.nlights;
@@ -386,6 +419,17 @@ int launch_gather_persistent(const GatherArgs& A, int stack, const WorkQueue& Q,
 int launch_gather(const GatherArgs& A, int stack, hipStream_t st)
 {
     if (!A.ntex) return LSG_OK;
+    // shooters per series = max(hemisphere rays, light rounds); a group of G >= that
+    // many lanes does the same work as a whole wave with idle lanes
+    const uint32_t series = std::max(A.hemi_rays, A.rounds);
+    if (series <= 32) {
+        const uint32_t G = series <= 16 ? 16u : 32u;
+        const unsigned grid = (unsigned)(((uint64_t)A.ntex * G + 255) / 256);
+        if (G == 16) { if (stack <= 32) k_gather_grouped<32, 16><<<grid, 256, 0, st>>>(A); else k_gather_grouped<64, 16><<<grid, 256, 0, st>>>(A); }
+        else { if (stack <= 32) k_gather_grouped<32, 32><<<grid, 256, 0, st>>>(A); else k_gather_grouped<64, 32><<<grid, 256, 0, st>>>(A); }
+        LSG_HIP(hipGetLastError());
+        return LSG_OK;
+    }
     const unsigned grid = (A.ntex + 3) / 4;
     if (stack <= 32) k_gather<32, false><<<grid, 256, 0, st>>>(A);
     else k_gather<64, false><<<grid, 256, 0, st>>>(A);
@@ -462,15 +506,11 @@ static int gather_launch(lsg_scene* s, const lsg_texel* texels, uint32_t ntexels
     int stack = 0;
     int st = gather_args(s, texels, ntexels, subs, p, out_rgba, A, stack);
     if (st) return st;
+    if (!stats) return launch_gather(A, stack, (hipStream_t)stream);
     A.stats = stats;
     const unsigned grid = (ntexels + 3) / 4;
-    if (stats) {
-        if (stack <= 32) k_gather<32, true><<<grid, 256, 0, (hipStream_t)stream>>>(A);
-        else k_gather<64, true><<<grid, 256, 0, (hipStream_t)stream>>>(A);
-    } else {
-        if (stack <= 32) k_gather<32, false><<<grid, 256, 0, (hipStream_t)stream>>>(A);
-        else k_gather<64, false><<<grid, 256, 0, (hipStream_t)stream>>>(A);
-    }
+    if (stack <= 32) k_gather<32, true><<<grid, 256, 0, (hipStream_t)stream>>>(A);
+    else k_gather<64, true><<<grid, 256, 0, (hipStream_t)stream>>>(A);
     LSG_HIP(hipGetLastError());
     return LSG_OK;
 }

@@ -1,0 +1,395 @@
+"""Learned gather (config 4): an MLP surrogate of the 1024-ray final gather.
+
+Pipeline per scene (all on the GPU):
+  cheap gather  -- the plain gather at low quality (16 hemisphere rays, 2 shadow
+                   rounds per light), hemisphere and light parts kept apart
+  features      -- log1p(parts/s) of the texel and window means over its lightmap
+                   neighbours, position in the scene box, normal, albedo, spreads (32)
+  MLP           -- 32-64-64-64-64-4, BF16 weights/activations on MFMA, FP32 accumulate;
+                   predicts log1p(E/s) and its own RMS log error
+  decide        -- keep the prediction where predicted error <= tau, queue the rest
+  fallback      -- the plain 1024-ray gather on the queued texels
+  filters       -- the updateLightmaps filter chain, as after the plain gather
+Training data: texels of seeded config-2-family scenes baked by the plain gather.
+The plain 1024-ray bake stays the judge: RMSE is measured against it, on the
+filtered lightmaps both pipelines write.
+"""
+from __future__ import annotations
+
+import ctypes
+import time
+
+import numpy as np
+
+from . import native as N
+from . import scenes as S
+from .bake import SceneBake, GatherParams, TEXEL_DTYPE
+from .collider import DeviceBuffer
+
+CHEAP_QUALITY = 16
+CHEAP_LIGHT_ROUNDS = 0        # 0: the reference's quality/10+1 rounds (2 at quality 16)
+LAMBDA = 0.5
+NFEAT = 32
+
+
+def rmse_rel(a: np.ndarray, b: np.ndarray) -> float:
+    """Lightmap RMSE relative to the reference: sqrt(mean((a-b)^2)) / sqrt(mean(b^2))
+    over the RGB of every covered texel."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.sqrt(np.mean((a - b) ** 2)) / max(np.sqrt(np.mean(b ** 2)), 1e-30))
+
+
+class Surrogate:
+    """The MLP on the GPU (lsg_mlp_*)."""
+
+    def __init__(self, max_batch: int = 16384, seed: int = 0):
+        self.max_batch = max_batch
+        h = ctypes.c_void_p()
+        N.check(N.lib().lsg_mlp_create(max_batch, ctypes.byref(h)), "lsg_mlp_create")
+        self._h = h.value
+        self.nparams = int(N.lib().lsg_mlp_num_params())
+        self.set_params(self.init_params(seed))
+
+    @staticmethod
+    def layer_shapes():
+        return [(NFEAT, 64), (64, 64), (64, 64), (64, 64), (64, 16)]
+
+    def init_params(self, seed=0):
+        rng = np.random.default_rng(seed)
+        out = []
+        for k, (i, o) in enumerate(self.layer_shapes()):
+            w = rng.normal(size=(i, o)) * np.sqrt(2.0 / i)
+            if k == 4:
+                w *= 0.1
+            out += [w.reshape(-1), np.zeros(o)]
+        return np.concatenate(out).astype(np.float32)
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert len(p) == self.nparams
+        N.check(N.lib().lsg_mlp_set_params(self._h, p.ctypes.data_as(ctypes.c_void_p)), "set_params")
+
+    def get_params(self):
+        p = np.zeros(self.nparams, np.float32)
+        N.check(N.lib().lsg_mlp_get_params(self._h, p.ctypes.data_as(ctypes.c_void_p)), "get_params")
+        return p
+
+    def get_grads(self):
+        g = np.zeros(self.nparams, np.float32)
+        N.check(N.lib().lsg_mlp_get_grads(self._h, g.ctypes.data_as(ctypes.c_void_p)), "get_grads")
+        return g
+
+    def step(self, dx, dt, n, lr, lam=LAMBDA, want_loss=False, offset_rows=0):
+        loss = ctypes.c_float()
+        N.check(N.lib().lsg_mlp_train_step(self._h, dx.ptr + 4 * NFEAT * offset_rows, dt.ptr + 12 * offset_rows, n, lam, lr,
+                                           ctypes.byref(loss) if want_loss else None, None), "train_step")
+        return loss.value if want_loss else None
+
+    def forward(self, dx_ptr, n, dy_ptr, stream=None):
+        N.check(N.lib().lsg_mlp_forward(self._h, dx_ptr, n, dy_ptr, stream), "mlp_forward")
+
+    def train(self, X: np.ndarray, T: np.ndarray, epochs: int = 4, batch: int = 16384, lr: float = 2e-3,
+              seed: int = 1, log=None):
+        """Adam over shuffled mini-batches; returns per-epoch mean loss."""
+        rng = np.random.default_rng(seed)
+        n = len(X)
+        perm = rng.permutation(n)
+        dx = DeviceBuffer.from_array(np.ascontiguousarray(X[perm], np.float32))
+        dt = DeviceBuffer.from_array(np.ascontiguousarray(T[perm], np.float32))
+        batch = min(batch, self.max_batch)
+        nsteps = max(1, n // batch)
+        total = epochs * nsteps
+        hist = []
+        k = 0
+        for ep in range(epochs):
+            losses = []
+            for s in range(nsteps):
+                frac = k / max(total - 1, 1)
+                cur = lr * (0.05 + 0.95 * 0.5 * (1 + np.cos(np.pi * frac)))      # cosine decay
+                want = (s % 32 == 0) or s == nsteps - 1
+                l = self.step(dx, dt, batch, cur, want_loss=want, offset_rows=s * batch)
+                if want:
+                    losses.append(l)
+                k += 1
+            hist.append(float(np.mean(losses)))
+            if log:
+                log(f"epoch {ep}: loss {hist[-1]:.5f}")
+        return hist
+
+    def predict(self, X: np.ndarray) -> np.ndarray:
+        dx = DeviceBuffer.from_array(np.ascontiguousarray(X, np.float32))
+        dy = DeviceBuffer(16 * max(len(X), 1))
+        self.forward(dx.ptr, len(X), dy.ptr)
+        N.check(N.lib().lsg_synchronize(), "sync")
+        return dy.download(np.float32, (len(X), 4))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            N.lib().lsg_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class LearnedScene:
+    """One config-2-family scene prepared for the learned gather: the plain bake plan,
+    its device buffers, the cheap-gather parameters and the albedo table."""
+
+    def __init__(self, scene: S.BakeScene, quality: int = 1024, seed: int = 1):
+        self.scene = scene
+        self.sb = SceneBake(scene, GatherParams(quality=quality, ray_length_min=S.minimal_safe_distance(scene.verts),
+                                                seed=seed))
+        self.sb.first_gather()
+        p = self.sb.params
+        self.cheap = GatherParams(quality=CHEAP_QUALITY, hemisphere=p.hemisphere, ray_length_min=p.ray_length_min,
+                                  locality=p.locality, rug_distance=p.rug_distance, light_multiplier=p.light_multiplier,
+                                  seed=p.seed ^ 0x2545F491, light_rounds=CHEAP_LIGHT_ROUNDS)
+        n = len(self.sb.texels)
+        self.n = n
+        self.d_parts = DeviceBuffer(32 * max(n, 1))
+        self.d_cheap_out = DeviceBuffer(16 * max(self.sb.nout, 1))
+        self.d_x = DeviceBuffer(4 * NFEAT * max(n, 1))
+        self.d_work = DeviceBuffer(48 * max(self.sb.nout, 1))
+        self.d_y = DeviceBuffer(16 * max(n, 1))
+        self.d_fb = DeviceBuffer(16 * max(n, 1))
+        self.d_nfb = DeviceBuffer(16)
+        self.d_albedo = DeviceBuffer.from_array(np.ascontiguousarray(scene.albedo, np.float32))
+        self.scale = 1.0
+
+    def plain(self, stream=None, filtering=True):
+        """The plain 1024-ray gather of every texel into sb.d_out (the judge), then the
+        filter chain into sb.d_rgbf."""
+        self.sb.clear()
+        self.sb.final_gather(stream)
+        if filtering:
+            self.sb.filter(stream)
+
+    def plain_rgb(self):
+        """Gathered RGBA of every texel (before the filters when plain/learned ran with
+        filtering=False; after smoothing otherwise)."""
+        N.check(N.lib().lsg_synchronize(), "sync")
+        out = self.sb.d_out.download(np.float32, (max(self.sb.nout, 1), 4))
+        return out[self.sb.texels["out_index"]]
+
+    def final_rgb(self):
+        """The filtered RGBF lightmap value of every texel (what updateLightmaps writes)."""
+        N.check(N.lib().lsg_synchronize(), "sync")
+        out = self.sb.d_rgbf.download(np.float32, (max(self.sb.nout, 1), 3))
+        return out[self.sb.texels["out_index"]]
+
+    def cheap_and_features(self, stream=None):
+        L = N.lib()
+        p = self.cheap.c()
+        N.check(L.lsg_gather_parts_device(self.sb.gs._h, self.sb.d_tex.ptr, self.n, self.sb.d_sub.ptr, ctypes.byref(p),
+                                          self.d_cheap_out.ptr, self.d_parts.ptr, stream), "gather_parts")
+        sc = ctypes.c_float()
+        N.check(L.lsg_learned_scale(self.d_parts.ptr, self.n, ctypes.byref(sc)), "scale")
+        self.scale = float(sc.value)
+        N.check(L.lsg_learned_features(self.sb.gs._h, self.sb.d_tex.ptr, self.n, self.sb.d_sub.ptr, self.d_parts.ptr,
+                                       self.d_albedo.ptr, self.scale, self.sb.maps, len(self.sb.slots), self.sb.nout,
+                                       self.d_work.ptr, self.d_x.ptr, stream), "features")
+
+    def features(self):
+        N.check(N.lib().lsg_synchronize(), "sync")
+        return self.d_x.download(np.float32, (self.n, NFEAT))
+
+    def targets(self, plain_rgb):
+        return np.log1p(np.maximum(plain_rgb[:, :3], 0) / self.scale).astype(np.float32)
+
+    def close(self):
+        self.sb.gs.close()
+
+    def cheap_only(self, stream=None):
+        """The cheap gather alone written as the lightmap (no network), then the filters:
+        the method the surrogate improves on."""
+        self.sb.clear()
+        p = self.cheap.c()
+        N.check(N.lib().lsg_gather_device(self.sb.gs._h, self.sb.d_tex.ptr, self.n, self.sb.d_sub.ptr, len(self.sb.subs),
+                                          ctypes.byref(p), self.sb.d_out.ptr, stream), "cheap gather")
+        self.sb.filter(stream)
+
+    def learned(self, net: Surrogate, tau: float, stream=None, filtering=True):
+        """Cheap gather + features + MLP + decision + plain gather of the fallback texels,
+        writing every texel into sb.d_out, then the filter chain into sb.d_rgbf.
+        Returns the fallback count."""
+        L = N.lib()
+        self.sb.clear()
+        self.cheap_and_features(stream)
+        net.forward(self.d_x.ptr, self.n, self.d_y.ptr, stream)
+        N.check(L.lsg_memset(self.d_nfb.ptr, 0, 4), "memset")
+        N.check(L.lsg_learned_decide(self.d_y.ptr, self.sb.d_tex.ptr, self.d_parts.ptr, self.n, self.scale, tau,
+                                     self.sb.d_out.ptr, self.d_fb.ptr, self.d_nfb.ptr, stream), "decide")
+        N.check(L.lsg_synchronize(), "sync")
+        nfb = int(self.d_nfb.download(np.uint32, (1,))[0])
+        if nfb:
+            p = self.sb.params.c()
+            N.check(L.lsg_gather_device(self.sb.gs._h, self.d_fb.ptr, nfb, self.sb.d_sub.ptr, len(self.sb.subs),
+                                        ctypes.byref(p), self.sb.d_out.ptr, stream), "fallback gather")
+        if filtering:
+            self.sb.filter(stream)
+        return nfb
+
+
+def config2_family(seed: int, lightmap_size: int = 256, target_tris: int = 50_000) -> S.BakeScene:
+    """A config-2-family scene: the closed 10x10x4 m room with 20 boxes from seed."""
+    return S.box_room(seed=seed, n_boxes=20, target_tris=target_tris, lightmap_size=lightmap_size)
+
+
+def build_dataset(seeds, lightmap_size=256, target_tris=50_000, augment=1, log=None):
+    """(features, targets) of every reliable texel of every scene in seeds: features of
+    `augment` independently seeded cheap gathers, targets log1p(E/s) of the plain
+    1024-ray gather before the filters."""
+    Xs, Ts = [], []
+    for sd in seeds:
+        ls = LearnedScene(config2_family(sd, lightmap_size, target_tris), seed=sd)
+        ls.plain(filtering=False)
+        ref = ls.plain_rgb()
+        ok = ref[:, 3] > 0
+        base = ls.cheap.seed
+        for a in range(augment):
+            ls.cheap.seed = base + 0x632BE5AB * a & 0xFFFFFFFF
+            ls.cheap_and_features()
+            Xs.append(ls.features()[ok])
+            Ts.append(ls.targets(ref)[ok])
+        if log:
+            log(f"scene {sd}: {int(ok.sum())} texels x {augment}, scale {ls.scale:.4f}")
+        ls.close()
+    return np.concatenate(Xs), np.concatenate(Ts)
+
+
+def prepare_eval(seed, lightmap_size=256, target_tris=50_000) -> LearnedScene:
+    """A scene with its plain 1024-ray filtered lightmaps kept as the reference."""
+    ls = LearnedScene(config2_family(seed, lightmap_size, target_tris), seed=seed)
+    ls.plain()
+    ls.ref_final = ls.final_rgb()
+    ls.ref_raw = ls.plain_rgb()
+    return ls
+
+
+def evaluate(ls: LearnedScene, net: Surrogate, tau: float):
+    """(relative RMSE of the filtered learned+fallback lightmaps against the plain bake,
+    fallback fraction)."""
+    nfb = ls.learned(net, tau)
+    return rmse_rel(ls.final_rgb(), ls.ref_final), nfb / max(ls.n, 1)
+
+
+TAU_CANDIDATES = (0.01, 0.015, 0.02, 0.025, 0.03, 0.04, 0.05, 0.06, 0.08, 0.1, 0.13, 0.17, 0.22, 0.3)
+
+
+def calibrate_tau(net: Surrogate, prepared, target_rmse: float = 0.018, candidates=TAU_CANDIDATES):
+    """Largest bound tau on the predicted error whose learned+fallback bake keeps the
+    filtered-lightmap RMSE of every calibration scene at or below target_rmse."""
+    best = candidates[0]
+    for tau in candidates:
+        worst = max(evaluate(ls, net, tau)[0] for ls in prepared)
+        if worst <= target_rmse:
+            best = tau
+        else:
+            break
+    return best
+
+
+# ---- trained surrogate for the config-2 scene family ---------------------------------------
+
+WEIGHTS = __file__.rsplit("/", 1)[0] + "/surrogate_box_room.npz"
+TRAIN_SEEDS = range(1000, 1020)          # 20 training scenes
+CAL_SEEDS = (1500, 1501)                 # tau calibration
+HELDOUT_SEEDS = range(2000, 2005)        # 5 held-out scenes
+
+
+def save_weights(path, net: Surrogate, tau: float, meta: dict):
+    import json
+    np.savez(path, params=net.get_params(), tau=np.float32(tau), nfeat=np.int32(NFEAT), meta=np.array(json.dumps(meta)))
+
+
+def load_weights(path=WEIGHTS, max_batch: int = 65536):
+    """(Surrogate with the stored parameters, tau, meta) or None when absent."""
+    import json
+    import os
+    if not os.path.exists(path):
+        return None
+    z = np.load(path, allow_pickle=False)
+    if int(z["nfeat"]) != NFEAT:
+        return None
+    net = Surrogate(max_batch=max_batch)
+    net.set_params(z["params"])
+    return net, float(z["tau"]), json.loads(str(z["meta"]))
+
+
+def train_box_room(augment=2, epochs=12, batch=8192, lr=2e-3, target_rmse=0.018, log=print):
+    """Train the surrogate on the 20 training scenes, calibrate tau on the calibration
+    scenes; returns (net, tau, meta)."""
+    t0 = time.perf_counter()
+    X, T = build_dataset(TRAIN_SEEDS, augment=augment, log=log)
+    t1 = time.perf_counter()
+    net = Surrogate(max_batch=65536, seed=0)
+    hist = net.train(X, T, epochs=epochs, batch=batch, lr=lr, log=log)
+    t2 = time.perf_counter()
+    n = len(X)
+    del X, T
+    cal = [prepare_eval(s) for s in CAL_SEEDS]
+    tau = calibrate_tau(net, cal, target_rmse)
+    meta = dict(train_seeds=[TRAIN_SEEDS.start, TRAIN_SEEDS.stop], cal_seeds=list(CAL_SEEDS), augment=augment,
+                epochs=epochs, batch=batch, lr=lr, samples=int(n), loss=hist, target_rmse=target_rmse,
+                dataset_s=round(t1 - t0, 2), train_s=round(t2 - t1, 2))
+    log(f"trained on {n} texels in {t2 - t1:.1f}s, tau {tau}")
+    return net, tau, meta
+
+
+def _sync():
+    N.check(N.lib().lsg_synchronize(), "sync")
+
+
+def heldout_report(net: Surrogate, tau: float, seeds=HELDOUT_SEEDS, reps: int = 3, log=print):
+    """Plain vs learned+fallback bake on each held-out scene: RMSE of the filtered
+    lightmaps against the plain bake, fallback fraction and the time of both."""
+    rows = []
+    for sd in seeds:
+        ls = prepare_eval(sd)
+        ls.cheap_only()
+        r_cheap = rmse_rel(ls.final_rgb(), ls.ref_final)
+        r_net, _ = evaluate(ls, net, float("inf"))
+        tp, tl = [], []
+        for _ in range(reps):
+            _sync(); a = time.perf_counter(); ls.plain(); _sync(); b = time.perf_counter()
+            nfb = ls.learned(net, tau); _sync(); c = time.perf_counter()
+            tp.append(b - a); tl.append(c - b)
+        r, f = rmse_rel(ls.final_rgb(), ls.ref_final), nfb / max(ls.n, 1)
+        row = dict(seed=sd, texels=ls.n, rmse=r, rmse_cheap=r_cheap, rmse_network=r_net, fallback=f,
+                   plain_ms=1e3 * min(tp), learned_ms=1e3 * min(tl))
+        rows.append(row)
+        log(f"held-out {sd}: {ls.n} texels, rmse {r:.4f} (cheap {r_cheap:.4f}, network {r_net:.4f}), fallback {f:.3f}, "
+            f"plain {row['plain_ms']:.1f} ms, learned {row['learned_ms']:.1f} ms, x{row['plain_ms'] / row['learned_ms']:.2f}")
+        ls.close()
+    return rows
+
+
+def _main(argv=None):
+    import argparse
+    ap = argparse.ArgumentParser(description="train / evaluate the learned-gather surrogate (config-2 scene family)")
+    ap.add_argument("cmd", choices=["train", "eval"])
+    ap.add_argument("--weights", default=WEIGHTS)
+    ap.add_argument("--augment", type=int, default=2)
+    ap.add_argument("--epochs", type=int, default=12)
+    a = ap.parse_args(argv)
+    t0 = time.perf_counter()
+    log = lambda m: print(f"[{time.perf_counter() - t0:7.1f}s] {m}", flush=True)
+    if a.cmd == "train":
+        net, tau, meta = train_box_room(augment=a.augment, epochs=a.epochs, log=log)
+        save_weights(a.weights, net, tau, meta)
+        log(f"saved {a.weights}")
+    else:
+        got = load_weights(a.weights)
+        if got is None:
+            raise SystemExit(f"no weights at {a.weights}")
+        net, tau, meta = got
+    heldout_report(net, tau, log=log)
+
+
+if __name__ == "__main__":
+    _main()

@@ -40,7 +40,7 @@ class Light(ctypes.Structure):
 class GatherParams(ctypes.Structure):
     _fields_ = [("quality", U32), ("hemisphere", I32), ("ray_length_min", ctypes.c_float),
                 ("locality", ctypes.c_float), ("rug_distance", ctypes.c_float), ("light_multiplier", ctypes.c_float),
-                ("seed", U32), ("reserved", U32)]
+                ("seed", U32), ("light_rounds", U32)]
 
 
 class Lightmap(ctypes.Structure):
@@ -109,6 +109,19 @@ SIGNATURES = {
     "lsg_bake_run": (ctypes.c_int, [P, ctypes.POINTER(GatherParams), ctypes.POINTER(GatherParams), ctypes.POINTER(FilterParams), U32, P]),
     "lsg_bake_gather_sample": (ctypes.c_int, [P, ctypes.POINTER(GatherParams), U32, ctypes.POINTER(ctypes.c_float)]),
     "lsg_bake_download": (ctypes.c_int, [P, U32, P]),
+    "lsg_mlp_num_params": (U32, []),
+    "lsg_mlp_create": (ctypes.c_int, [U32, ctypes.POINTER(P)]),
+    "lsg_mlp_destroy": (None, [P]),
+    "lsg_mlp_set_params": (ctypes.c_int, [P, P]),
+    "lsg_mlp_get_params": (ctypes.c_int, [P, P]),
+    "lsg_mlp_get_grads": (ctypes.c_int, [P, P]),
+    "lsg_mlp_forward": (ctypes.c_int, [P, P, U32, P, P]),
+    "lsg_mlp_train_step": (ctypes.c_int, [P, P, P, U32, ctypes.c_float, ctypes.c_float, ctypes.POINTER(ctypes.c_float), P]),
+    "lsg_mlp_gemm_test": (ctypes.c_int, [P, P, P, U32, U32, U32]),
+    "lsg_gather_parts_device": (ctypes.c_int, [P, P, U32, P, ctypes.POINTER(GatherParams), P, P, P]),
+    "lsg_learned_scale": (ctypes.c_int, [P, U32, ctypes.POINTER(ctypes.c_float)]),
+    "lsg_learned_features": (ctypes.c_int, [P, P, U32, P, P, P, ctypes.c_float, P, U32, U32, P, P, P]),
+    "lsg_learned_decide": (ctypes.c_int, [P, P, P, U32, ctypes.c_float, ctypes.c_float, P, P, P, P]),
     "lsg_rasterize_texels": (ctypes.c_int, [P, P, U32, U32, U32, U32, P, ctypes.POINTER(U32), P, ctypes.POINTER(U32), U32]),
 }
 

@@ -44,7 +44,7 @@ typedef struct lsgo_gparams {
     float rug_distance;        /* RRSolver.h:582 default 0.001 */
     float light_multiplier;    /* direct.lightMultiplier */
     uint32_t seed;
-    uint32_t reserved;
+    uint32_t light_rounds;     /* 0: quality/10+1 (gather.cpp:396); else this many rounds */
 } lsgo_gparams;
 
 typedef struct lsgo_gstats {
@@ -220,7 +220,7 @@ void lsgo_gather(const float* verts, uint32_t nverts, const uint32_t* idx, uint3
 
     const uint32_t q = p->quality;
     const uint32_t hemi_rays = p->hemisphere ? (q > 1 ? q : 1) : 0;                 /* gather.cpp:197 */
-    const uint32_t rounds = (p->light_multiplier != 0 && nlights) ? q / 10 + 1 : 0;  /* gather.cpp:396 */
+    const uint32_t rounds = (p->light_multiplier != 0 && nlights) ? (p->light_rounds ? p->light_rounds : q / 10 + 1) : 0;  /* gather.cpp:396 */
     const uint32_t light_rays = nlights * rounds;
     double nh = 0, ns = 0, nser = 0;
     int nt = 1;

@@ -32,7 +32,7 @@ class Stats(ctypes.Structure):
 class GParams(ctypes.Structure):
     _fields_ = [("quality", ctypes.c_uint32), ("hemisphere", ctypes.c_int32), ("ray_length_min", ctypes.c_float),
                 ("locality", ctypes.c_float), ("rug_distance", ctypes.c_float), ("light_multiplier", ctypes.c_float),
-                ("seed", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("seed", ctypes.c_uint32), ("light_rounds", ctypes.c_uint32)]
 
 
 class GStats(ctypes.Structure):
@@ -215,7 +215,7 @@ def gather(scene_arrays, params, texels, subs, tree=None, threads=0):
     nout = int(tx[:, 0].max()) + 1 if len(tx) else 0
     out = np.zeros((max(nout, 1), 4), np.float64)
     gp = GParams(params["quality"], params["hemisphere"], params["ray_length_min"], params["locality"],
-                 params["rug_distance"], params["light_multiplier"], params["seed"], 0)
+                 params["rug_distance"], params["light_multiplier"], params["seed"], params.get("light_rounds", 0))
     gs = GStats()
     if tree is not None:
         nodes = np.ascontiguousarray(tree["nodes"], dtype=np.float32)
@@ -265,3 +265,24 @@ def filter_lightmap(rgba, sigma=1.0, spread=1000.0, bg=(0, 0, 0, 0)):
     out = np.zeros((h, w, 3), np.float32)
     L.lsgo_filter_lightmap(_p(img), w, h, sigma, spread, _p(b), _p(out))
     return img, out
+
+
+def mlp_num_params():
+    L = lib()
+    L.lsgo_mlp_num_params.restype = ctypes.c_uint32
+    return int(L.lsgo_mlp_num_params())
+
+
+def mlp_loss_grad(params, x, t, lam=0.5, bf16=True, want_grad=True):
+    """(loss, y[n,4], grad[nparams] float64) of the surrogate MLP in binary64."""
+    L = lib()
+    L.lsgo_mlp_loss_grad.restype = ctypes.c_double
+    L.lsgo_mlp_loss_grad.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_double,
+                                     ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    p = np.ascontiguousarray(params, np.float32)
+    xx = np.ascontiguousarray(x, np.float32)
+    tt = np.ascontiguousarray(t, np.float32)
+    y = np.zeros((len(xx), 4), np.float32)
+    g = np.zeros(len(p), np.float64)
+    loss = L.lsgo_mlp_loss_grad(_p(p), _p(xx), _p(tt), len(xx), lam, 1 if bf16 else 0, _p(y), _p(g) if want_grad else None)
+    return loss, y, g

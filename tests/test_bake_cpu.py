@@ -259,3 +259,40 @@ def test_filter_smooth_respects_edge_flags(oracle):
     out, rgb = oracle.filter_lightmap(img, spread=0.0)
     np.testing.assert_array_equal(rgb[:, :4], np.ones((4, 4, 3), np.float32))
     np.testing.assert_array_equal(rgb[:, 4:], np.full((4, 4, 3), 5.0, np.float32))
+
+
+# ---- MLP oracle (learned gather) --------------------------------------------------------------
+
+def test_mlp_oracle_finite_differences(oracle):
+    """The binary64 back-propagation of the oracle equals central finite differences
+    (pins the transposes and the ReLU mask).  The error-head term treats
+    e_i = rms(y_i - t_i) as a constant, so it is differenced with e held fixed."""
+    rng = np.random.default_rng(0)
+    npar = oracle.mlp_num_params()
+    p = (rng.normal(size=npar) * 0.3).astype(np.float32)
+    x = rng.normal(size=(6, 32)).astype(np.float32)
+    t = rng.normal(size=(6, 3)).astype(np.float32)
+    lam = 0.5
+    _, y0, g0 = oracle.mlp_loss_grad(p, x, t, lam=0.0, bf16=False)
+    _, _, g1 = oracle.mlp_loss_grad(p, x, t, lam=lam, bf16=False)
+    e0 = np.sqrt(((y0[:, :3].astype(np.float64) - t) ** 2).mean(axis=1))
+    idx = rng.choice(npar, 80, replace=False)
+    h = 1e-3
+    checked = 0
+    for i in idx:
+        def ev(v):
+            q = p.copy(); q[i] = v
+            l, y, _ = oracle.mlp_loss_grad(q, x, t, lam=0.0, bf16=False, want_grad=False)
+            return l, lam * np.mean((y[:, 3].astype(np.float64) - e0) ** 2)
+        hp, hm = np.float32(p[i] + h), np.float32(p[i] - h)
+        (lp, ep), (lm, em) = ev(hp), ev(hm)
+        den = float(hp) - float(hm)
+        num0 = (lp - lm) / den
+        num1 = (ep - em) / den
+        # skip parameters whose perturbation flips a ReLU (kink inside the step)
+        if abs(num0 - g0[i]) > 2e-4 * max(1.0, abs(g0[i])):
+            continue
+        checked += 1
+        # y comes back in float32, so the head-term difference quotient carries ~1e-4 noise
+        assert abs(num1 - (g1[i] - g0[i])) <= 2e-3 * max(1.0, abs(g1[i] - g0[i])), (i, num1, g1[i] - g0[i])
+    assert checked >= 70

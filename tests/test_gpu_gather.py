@@ -62,6 +62,22 @@ def test_gpu_final_gather_vs_oracle(gpu, oracle):
     assert np.all(flat[~cov] == 0)
 
 
+@pytest.mark.parametrize("quality,rounds", [(16, 0), (20, 0), (32, 0), (16, 16), (16, 40)])
+def test_gpu_low_quality_grouped_vs_oracle(gpu, oracle, quality, rounds):
+    """At most 32 shooters per series run on groups of 16/32 lanes, several texels per
+    wavefront (the cheap gather of the learned path, with its own light-round count);
+    same rays, same results."""
+    sc, sb = _bake(seed=5, quality=quality)
+    sb.params.light_rounds = rounds
+    sb.first_gather()
+    sb.final_gather()
+    flat = np.concatenate([m.reshape(-1, 4) for m in sb.lightmaps_rgba()])
+    tx = sb.texels
+    ref, _ = oracle.gather(sb.gs.oracle_arrays(sb.exitance), sb.params.as_dict(), tx.view(np.uint32).reshape(-1, 4),
+                           sb.subs, tree=sb.gs.col.download_tree(), threads=16)
+    print(f"quality {quality}, rounds {rounds}: {len(tx)} texels, max rel err {_cmp(flat, ref, tx['out_index']):.2e}", flush=True)
+
+
 def test_gpu_config2_full_size_sample(gpu, oracle):
     """Config 2 at full size (box room ~50k triangles, 21 x 256x256, 1024 rays per
     texel); the oracle re-gathers a spread sample of texels."""
