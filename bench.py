@@ -1,20 +1,31 @@
 """Benchmark of the MI355X ray-cast + lightmap-gather path (one GPU).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload bake|collider]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload bake|collider|batched|learned]
 
 Prints ONE JSON line.  Workloads (synthetic, seeded):
-  bake      plain lightmap bake: closed 10x10x4 m box room + 20 boxes (~50k
-            triangles), 21 lightmaps of 256x256, 1024 hemisphere rays per texel
-            plus 103 shadow-ray rounds to the point light; one step = the gather
-            of every covered texel.  value = baked texels/s.
-  collider  closest hit on a 100,820-triangle displaced icosphere, 1M random
-            rays resident in HBM; one step = one batch.  value = Mrays/s.
-Timing: W untimed steps, then K steps between device synchronisations; the
-dominant kernel is also timed with HIP events on its own stream.
+  bake      (default) plain lightmap bake: closed 10x10x4 m box room + 20 boxes
+            (~50k triangles), 21 lightmaps of 256x256, 1024 hemisphere rays per texel
+            plus 103 shadow-ray rounds to the point light; one step = the final gather
+            of every covered texel + the filter chain + RGBF writeback.
+            value = baked texels/s.  After the timed region the line also carries the
+            learned-gather comparison on the 5 held-out scenes of the same family
+            ("learned") and the closest-hit rate on the 100k-triangle icosphere
+            ("collider"); --no-extras leaves them out.
+  collider  closest hit on a 100,820-triangle displaced icosphere, 1M random rays
+            resident in HBM; one step = one batch.  value = Mrays/s.
+  batched   200 objects (~2M triangles), lightmaps 32..1024, sun + 4 point lights:
+            first gather + persistent final gather + filters replayed from a hipGraph.
+  learned   the 5 held-out box-room scenes: one step = the learned+fallback bake of
+            one scene (cycled); value = its texels/s, the plain bake timed beside it.
+Timing: W untimed steps, then K steps between device synchronisations; the dominant
+kernel is also timed with HIP events on the stream it is launched on.  The CPU
+baseline is the oracle (or the compiled reference collider where it exists) on a
+bounded sample, host threads stated.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -29,8 +40,13 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "closest-hit Mrays/s; baked texels/s at ≤2% RMSE, plain vs learned gather"
 
 
+def _oracle():
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_py as O
+    return O
+
+
 def _events():
-    import ctypes
     from lightsprintsdk_amd import native as N
     L = N.lib()
     s = ctypes.c_void_p(); a = ctypes.c_void_p(); b = ctypes.c_void_p()
@@ -40,8 +56,9 @@ def _events():
     return L, s, a, b
 
 
-def collider_workload(steps, warmup, cpu_threads):
-    import ctypes
+# ---- config 1: closest hit ---------------------------------------------------------------
+
+def collider_workload(steps, warmup, cpu_threads, with_cpu=True):
     from lightsprintsdk_amd import native as N, scenes
     from lightsprintsdk_amd.collider import RRCollider, DeviceRays
     V, I = scenes.icosphere(scenes.ICOSPHERE_100K_FREQUENCY, seed=1)
@@ -79,13 +96,13 @@ def collider_workload(steps, warmup, cpu_threads):
                       kernel="k_trace_closest", kernel_ms=round(kernel_ms, 4),
                       bytes_per_ray=round(b_ray, 2), n_node=round(n_node, 3), n_tri=round(n_tri, 3)),
     )
-    res["cpu_baseline"] = collider_cpu_baseline(V, I, R, cpu_threads, col)
+    if with_cpu:
+        res["cpu_baseline"] = collider_cpu_baseline(V, I, R, cpu_threads, col)
     return res
 
 
 def collider_cpu_baseline(V, I, R, threads, col):
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_py as O
+    O = _oracle()
     if O.have_reference():
         _, tm = O.reference_intersect(V, I, R, technique=3, threads=threads, repeats=3, flags=(1 | 4 | 16))
         return dict(value=round(tm["mrays_per_s"], 4), unit="Mrays/s", cores=int(tm["threads"]), kind="reference",
@@ -100,32 +117,221 @@ def collider_cpu_baseline(V, I, R, threads, col):
                 sample=f"oracle C traversal of the same LBVH, {n} of the rays")
 
 
+# ---- config 2: plain bake ----------------------------------------------------------------
+
+def bake_workload(steps: int, warmup: int, cpu_threads: int):
+    """Plain-gather bake of config 2; one step = final gather + filters + RGBF writeback."""
+    from lightsprintsdk_amd import native as N, scenes as S
+    from lightsprintsdk_amd.bake import SceneBake, GatherParams, config2_scene
+    sc = config2_scene()
+    t0 = time.perf_counter()
+    sb = SceneBake(sc, GatherParams(quality=1024, ray_length_min=S.minimal_safe_distance(sc.verts), seed=1))
+    setup_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    sb.first_gather()
+    first_s = time.perf_counter() - t0
+    L, s, e0, e1 = _events()
+    for _ in range(warmup):
+        sb.bake(s)
+    N.check(L.lsg_synchronize(), "sync")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sb.bake(s)                     # clear + final gather + filter chain + RGBF writeback
+    N.check(L.lsg_synchronize(), "sync")
+    wall = time.perf_counter() - t0
+    # the dominant kernel alone, timed with HIP events on the stream it runs on
+    kreps = max(2, min(steps, 5))
+    N.check(L.lsg_event_record(e0, s), "ev")
+    for _ in range(kreps):
+        sb.final_gather(s)
+    N.check(L.lsg_event_record(e1, s), "ev")
+    N.check(L.lsg_synchronize(), "sync")
+    ms = ctypes.c_float()
+    N.check(L.lsg_event_elapsed_ms(e0, e1, ctypes.byref(ms)), "elapsed")
+    kernel_ms = ms.value / kreps
+    ntex = len(sb.texels)
+    texels_per_s = ntex / (wall / steps)
+    # traversal work of the same gather (separate counting launch)
+    st = (ctypes.c_double * 6)()
+    p = sb.params.c()
+    N.check(L.lsg_gather_stats(sb.gs._h, sb.d_tex.ptr, ntex, sb.d_sub.ptr, len(sb.subs), ctypes.byref(p), sb.d_out.ptr, st),
+            "gather_stats")
+    hr, hn, ht, sr, sn, stt = list(st)
+    b_ray = 48 + 64 * hn / hr + 48 * ht / hr
+    b_shadow = 48 + 64 * sn / max(sr, 1) + 48 * stt / max(sr, 1)
+    n_sub = len(sb.subs) / ntex
+    R = hr / ntex
+    Lr = sr / ntex
+    b_texel = R * (b_ray + 12) + Lr * b_shadow + 32 * n_sub + 12
+    achieved = b_texel * ntex / (kernel_ms * 1e-3) / 1e9
+    rays_per_s = (hr + sr) / (kernel_ms * 1e-3)
+    sb.bake(s)
+    maps = sb.lightmaps_rgbf()
+    lit = np.concatenate([m.reshape(-1, 3) for m in maps])[sb.texels["out_index"]]
+    res = dict(
+        metric=METRIC, value=round(texels_per_s, 1), unit="texels/s", ms_per_step=round(wall / steps * 1e3, 3),
+        dtype="f32",
+        config=dict(workload="plain lightmap bake: box room 10x10x4 m + 20 boxes, 21 lightmaps 256x256 RGBF, "
+                             "1024 hemisphere rays + 103 shadow rounds per texel (point light + emissive quad); "
+                             "step = final gather + filter chain + RGBF writeback",
+                    triangles=int(sc.ntri), lightmaps=len(sc.objects), texels=int(ntex), subtexels=int(len(sb.subs)),
+                    quality=1024, hemisphere_rays=int(hr), shadow_rays=int(sr), rays_per_s=round(rays_per_s, 1),
+                    mrays_per_s=round(rays_per_s / 1e6, 2), first_gather_s=round(first_s, 4), setup_s=round(setup_s, 3),
+                    mean_irradiance=float(lit.mean())),
+        roofline=dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+                      frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel="k_gather", kernel_ms=round(kernel_ms, 4),
+                      bytes_per_texel=round(b_texel, 1), bytes_per_hemisphere_ray=round(b_ray, 2),
+                      bytes_per_shadow_ray=round(b_shadow, 2), n_node_hemi=round(hn / hr, 3), n_tri_hemi=round(ht / hr, 3),
+                      n_node_shadow=round(sn / max(sr, 1), 3), n_tri_shadow=round(stt / max(sr, 1), 3),
+                      subtexels_per_texel=round(n_sub, 3)),
+    )
+    res["cpu_baseline"] = bake_cpu_baseline(sb, cpu_threads)
+    return res
+
+
+def bake_cpu_baseline(sb, threads: int, target_s: float = 12.0):
+    """The CPU oracle's FP64 statement of processTexel (a port: the reference's RRSolver
+    needs boost and is not built here) on a bounded, evenly spread sample of the same
+    texels, all host threads."""
+    O = _oracle()
+    tree = sb.gs.col.download_tree()
+    arrays = sb.gs.oracle_arrays(sb.exitance)
+    n0 = 64 * threads
+    sel = np.linspace(0, len(sb.texels) - 1, n0).astype(np.int64)
+    t0 = time.perf_counter()
+    O.gather(arrays, sb.params.as_dict(), sb.texels[sel].view(np.uint32).reshape(-1, 4), sb.subs, tree=tree, threads=threads)
+    dt = time.perf_counter() - t0
+    n1 = int(min(len(sb.texels), max(n0, n0 * target_s / max(dt, 1e-3))))
+    sel = np.linspace(0, len(sb.texels) - 1, n1).astype(np.int64)
+    t0 = time.perf_counter()
+    O.gather(arrays, sb.params.as_dict(), sb.texels[sel].view(np.uint32).reshape(-1, 4), sb.subs, tree=tree, threads=threads)
+    dt = time.perf_counter() - t0
+    return dict(value=round(n1 / dt, 1), unit="texels/s", cores=threads, kind="port",
+                sample=f"{n1} of {len(sb.texels)} texels evenly spread, oracle C gather (FP64 sums) over the same LBVH, "
+                       f"{dt:.1f} s")
+
+
+# ---- config 4: learned + fallback vs plain on the held-out scenes -------------------------
+
+def learned_summary(reps: int = 2, log=None):
+    """The committed surrogate on the 5 held-out box-room scenes (the plain bake of each
+    is the reference and is timed beside it)."""
+    from lightsprintsdk_amd import learned as Lr
+    got = Lr.load_weights()
+    if got is None:
+        return dict(error="no trained surrogate (python -m lightsprintsdk_amd.learned train)")
+    net, tau, meta = got
+    rows = Lr.heldout_report(net, tau, reps=reps, log=log or (lambda m: None))
+    tex = sum(r["texels"] for r in rows)
+    tp = sum(r["plain_ms"] for r in rows) / 1e3
+    tl = sum(r["learned_ms"] for r in rows) / 1e3
+    return dict(workload="learned gather: 5 held-out box-room scenes (seeds 2000-2004), 16-ray cheap gather + "
+                         "MLP 32-64x4-4 (BF16 MFMA, FP32 accumulate) + fallback to the 1024-ray plain gather; "
+                         "RMSE of the filtered lightmaps against the plain bake of the same scene",
+                texels_per_s=round(tex / tl, 1), plain_texels_per_s=round(tex / tp, 1), speedup=round(tp / tl, 3),
+                rmse_max=round(max(r["rmse"] for r in rows), 5), rmse_bound=0.02,
+                fallback_mean=round(float(np.mean([r["fallback"] for r in rows])), 4), tau=tau,
+                training=dict(scenes=20, samples=meta.get("samples"), epochs=meta.get("epochs"),
+                              train_s=meta.get("train_s")),
+                scenes=[{k: (round(v, 5) if isinstance(v, float) else v) for k, v in r.items()} for r in rows])
+
+
+def learned_workload(steps: int, warmup: int):
+    from lightsprintsdk_amd import learned as Lr, native as N
+    got = Lr.load_weights()
+    if got is None:
+        raise SystemExit("no trained surrogate: run python -m lightsprintsdk_amd.learned train")
+    net, tau, meta = got
+    scenes = [Lr.prepare_eval(sd) for sd in Lr.HELDOUT_SEEDS]
+    L = N.lib()
+    for k in range(warmup):
+        scenes[k % len(scenes)].learned(net, tau)
+    N.check(L.lsg_synchronize(), "sync")
+    t0 = time.perf_counter()
+    tex = 0
+    fb = 0
+    for k in range(steps):
+        ls = scenes[k % len(scenes)]
+        fb += ls.learned(net, tau)
+        tex += ls.n
+    N.check(L.lsg_synchronize(), "sync")
+    wall = time.perf_counter() - t0
+    summ = learned_summary()
+    return dict(metric=METRIC, value=round(tex / wall, 1), unit="texels/s", ms_per_step=round(wall / steps * 1e3, 3),
+                dtype="bf16 MLP + f32 gather",
+                config=dict(workload=summ.pop("workload"), steps_cycle_scenes=len(scenes),
+                            fallback_fraction=round(fb / max(tex, 1), 4)),
+                learned=summ)
+
+
+# ---- config 3: batched persistent bake ----------------------------------------------------
+
+def batched_workload(steps: int, warmup: int, n_objects: int = 200, target_tris: int = 2_000_000):
+    """Config 3: 200 objects (~2M triangles), lightmaps 32..1024, 1024 rays per texel,
+    sun + 4 point lights; one step = first gather + persistent final gather + filters
+    replayed from a hipGraph, launch shape from the autotuner."""
+    from lightsprintsdk_amd import native as N, scenes as S
+    from lightsprintsdk_amd.bake import BakePlan, GatherParams, autotune
+    t0 = time.perf_counter()
+    sc = S.object_field(seed=4, n_objects=n_objects, target_tris=target_tris)
+    gen_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    plan = BakePlan(sc, GatherParams(quality=1024, ray_length_min=S.minimal_safe_distance(sc.verts), seed=1))
+    setup_s = time.perf_counter() - t0
+    plan.run(first=True, graph=False)
+    t0 = time.perf_counter()
+    cache = os.environ.get("LSG_TUNING_CACHE", os.path.expanduser("~/.cache/lsgpu_tuning.json"))
+    os.makedirs(os.path.dirname(cache), exist_ok=True)
+    tune = autotune(plan, sample_texels=150_000, cache_path=cache)
+    tune_s = time.perf_counter() - t0
+    for _ in range(warmup):
+        plan.run(first=True, graph=True)
+    N.check(N.lib().lsg_synchronize(), "sync")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.run(first=True, graph=True)
+    N.check(N.lib().lsg_synchronize(), "sync")
+    wall = (time.perf_counter() - t0) / steps
+    inf = plan.info()
+    return dict(metric=METRIC, value=round(inf.num_texels / wall, 1), unit="texels/s", ms_per_step=round(wall * 1e3, 2),
+                dtype="f32",
+                config=dict(workload="batched bake: 200 objects (displaced boxes, cylinders, terrain patches), "
+                                     "lightmaps 32..1024 RGBF, 1024 rays/texel, sun + 4 point lights, hipGraph replay",
+                            triangles=int(sc.ntri), objects=len(sc.objects), texels=int(inf.num_texels),
+                            texel_slots=int(inf.num_slots), subtexels=int(inf.num_subtexels),
+                            tuning=dict(tile=inf.tuning.tile, waves_per_block=inf.tuning.waves_per_block,
+                                        blocks_per_cu=inf.tuning.blocks_per_cu, xcd_queues=inf.tuning.xcd_queues),
+                            autotune=tune, autotune_s=round(tune_s, 2), generate_s=round(gen_s, 2),
+                            setup_s=round(setup_s, 2), grow_steps=int(inf.grow_steps)))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default=os.environ.get("LSG_BENCH_WORKLOAD", "auto"))
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default=os.environ.get("LSG_BENCH_WORKLOAD", "bake"),
+                    choices=["bake", "collider", "batched", "learned"])
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-extras", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 (rays and texels shard "
-                                   "by batch/object across GPUs, see DESIGN.md)", "n_gpus": a.gpus}))
+                                   "by batch/object across GPUs; multi-GPU runs are not part of this build)",
+                          "n_gpus": a.gpus}))
         sys.exit(2)
     threads = a.cpu_threads or min(os.cpu_count() or 1, 16)
-    workload = a.workload
-    if workload == "auto":
-        try:
-            from lightsprintsdk_amd import bake  # noqa: F401
-            workload = "bake"
-        except ImportError:
-            workload = "collider"
-    if workload == "bake":
-        from lightsprintsdk_amd import bake
-        res = bake.bench_workload(a.steps, a.warmup, threads)
-    elif workload == "batched":
-        from lightsprintsdk_amd import bake
-        res = bake.bench_batched(a.steps, a.warmup, threads)
+    if a.workload == "bake":
+        res = bake_workload(a.steps, a.warmup, threads)
+        if not a.no_extras:
+            res["learned"] = learned_summary()
+            c = collider_workload(5, 1, threads, with_cpu=False)
+            res["collider"] = dict(workload=c["config"]["workload"], mrays_per_s=c["value"],
+                                   kernel_ms=c["roofline"]["kernel_ms"], roofline_frac=c["roofline"]["frac"])
+    elif a.workload == "batched":
+        res = batched_workload(a.steps, a.warmup)
+    elif a.workload == "learned":
+        res = learned_workload(a.steps, a.warmup)
     else:
         res = collider_workload(a.steps, a.warmup, threads)
     out = dict(metric=res.pop("metric"), value=res.pop("value"), unit=res.pop("unit"), n_gpus=1, steps=a.steps,

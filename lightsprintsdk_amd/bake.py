@@ -269,131 +269,9 @@ class SceneBake:
         return maps
 
 
-def smoke_check(O):
-    """Tiny bake checked against the CPU oracle (used by __graft_entry__.smoke)."""
-    sc = S.box_room(seed=3, n_boxes=2, target_tris=600, lightmap_size=16)
-    sb = SceneBake(sc, GatherParams(quality=64, ray_length_min=S.minimal_safe_distance(sc.verts), seed=7))
-    sb.first_gather()
-    sb.final_gather()
-    N.check(N.lib().lsg_synchronize(), "sync")
-    gpu = sb.d_out.download(np.float32, (sb.nout, 4))
-    sel = np.arange(0, len(sb.texels), max(1, len(sb.texels) // 64))
-    tex = sb.texels[sel]
-    ref, _ = O.gather(sb.gs.oracle_arrays(sb.exitance), sb.params.as_dict(), tex.view(np.uint32).reshape(-1, 4),
-                      sb.subs, tree=None)
-    g = gpu[tex["out_index"]]
-    r = ref[tex["out_index"]]
-    err = np.abs(g - r) / np.maximum(np.abs(r), 1e-3)
-    assert np.all(err[:, :3] < 1e-4), err.max()
-    assert np.array_equal(g[:, 3], r[:, 3].astype(np.float32))
-    print(f"smoke gather ok: {len(tex)} texels vs oracle, max rel err {err.max():.2e}")
-
-
-HBM_PEAK_GBS = 8000.0
-METRIC = "closest-hit Mrays/s; baked texels/s at \u22642% RMSE, plain vs learned gather"
-
-
 def config2_scene(lightmap_size=256):
     """Config 2: closed 10x10x4 m room + 20 seeded boxes, ~50k triangles, 21 lightmaps."""
     return S.box_room(seed=3, n_boxes=20, target_tris=50_000, lightmap_size=lightmap_size)
-
-
-def bench_workload(steps: int, warmup: int, cpu_threads: int):
-    """Plain-gather bake of config 2; one step = the final gather of every covered texel."""
-    sc = config2_scene()
-    t0 = time.perf_counter()
-    sb = SceneBake(sc, GatherParams(quality=1024, ray_length_min=S.minimal_safe_distance(sc.verts), seed=1))
-    setup_s = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    sb.first_gather()
-    first_s = time.perf_counter() - t0
-    L = N.lib()
-    s = ctypes.c_void_p(); e0 = ctypes.c_void_p(); e1 = ctypes.c_void_p()
-    N.check(L.lsg_stream_create(ctypes.byref(s)), "stream")
-    N.check(L.lsg_event_create(ctypes.byref(e0)), "event")
-    N.check(L.lsg_event_create(ctypes.byref(e1)), "event")
-    for _ in range(warmup):
-        sb.bake(s)
-    N.check(L.lsg_synchronize(), "sync")
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        sb.bake(s)                     # final gather + filter chain + RGBF writeback
-    N.check(L.lsg_synchronize(), "sync")
-    wall = time.perf_counter() - t0
-    # the dominant kernel alone, timed with HIP events on its own stream
-    kreps = max(2, min(steps, 5))
-    N.check(L.lsg_event_record(e0, s), "ev")
-    for _ in range(kreps):
-        sb.final_gather(s)
-    N.check(L.lsg_event_record(e1, s), "ev")
-    N.check(L.lsg_synchronize(), "sync")
-    ms = ctypes.c_float()
-    N.check(L.lsg_event_elapsed_ms(e0, e1, ctypes.byref(ms)), "elapsed")
-    kernel_ms = ms.value / kreps
-    ntex = len(sb.texels)
-    texels_per_s = ntex / (wall / steps)
-    # traversal work of the same gather (separate counting launch)
-    st = (ctypes.c_double * 6)()
-    p = sb.params.c()
-    N.check(L.lsg_gather_stats(sb.gs._h, sb.d_tex.ptr, ntex, sb.d_sub.ptr, len(sb.subs), ctypes.byref(p), sb.d_out.ptr, st),
-            "gather_stats")
-    hr, hn, ht, sr, sn, stt = list(st)
-    b_ray = 48 + 64 * hn / hr + 48 * ht / hr
-    b_shadow = 48 + 64 * sn / max(sr, 1) + 48 * stt / max(sr, 1)
-    n_sub = len(sb.subs) / ntex
-    R = hr / ntex
-    Lr = sr / ntex
-    b_texel = R * (b_ray + 12) + Lr * b_shadow + 32 * n_sub + 12
-    achieved = b_texel * ntex / (kernel_ms * 1e-3) / 1e9
-    rays_per_s = (hr + sr) / (kernel_ms * 1e-3)
-    sb.bake(s)
-    maps = sb.lightmaps_rgbf()
-    lit = np.concatenate([m.reshape(-1, 3) for m in maps])[sb.texels["out_index"]]
-    res = dict(
-        metric=METRIC, value=round(texels_per_s, 1), unit="texels/s", ms_per_step=round(wall / steps * 1e3, 3),
-        dtype="f32",
-        config=dict(workload="plain lightmap bake: box room 10x10x4 m + 20 boxes, 21 lightmaps 256x256 RGBF, "
-                             "1024 hemisphere rays + 103 shadow rounds per texel (point light + emissive quad); "
-                             "step = final gather + filter chain + RGBF writeback",
-                    triangles=int(sc.ntri), lightmaps=len(sc.objects), texels=int(ntex), subtexels=int(len(sb.subs)),
-                    quality=1024, hemisphere_rays=int(hr), shadow_rays=int(sr), rays_per_s=round(rays_per_s, 1),
-                    mrays_per_s=round(rays_per_s / 1e6, 2), first_gather_s=round(first_s, 4), setup_s=round(setup_s, 3),
-                    mean_irradiance=float(lit.mean())),
-        roofline=dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                      frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel="k_gather", kernel_ms=round(kernel_ms, 4),
-                      bytes_per_texel=round(b_texel, 1), bytes_per_hemisphere_ray=round(b_ray, 2),
-                      bytes_per_shadow_ray=round(b_shadow, 2), n_node_hemi=round(hn / hr, 3), n_tri_hemi=round(ht / hr, 3),
-                      n_node_shadow=round(sn / max(sr, 1), 3), n_tri_shadow=round(stt / max(sr, 1), 3),
-                      subtexels_per_texel=round(n_sub, 3)),
-    )
-    res["cpu_baseline"] = bake_cpu_baseline(sb, cpu_threads)
-    return res
-
-
-def bake_cpu_baseline(sb: "SceneBake", threads: int, target_s: float = 12.0):
-    """The CPU oracle's FP64 statement of processTexel (a port, the reference's
-    RRSolver needs boost and is not built here) on a bounded, evenly spread sample
-    of the same texels, all host threads."""
-    import os
-    import sys
-    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-    sys.path.insert(0, os.path.join(root, "tests"))
-    import oracle_py as O
-    tree = sb.gs.col.download_tree()
-    arrays = sb.gs.oracle_arrays(sb.exitance)
-    n0 = 64 * threads
-    sel = np.linspace(0, len(sb.texels) - 1, n0).astype(np.int64)
-    t0 = time.perf_counter()
-    O.gather(arrays, sb.params.as_dict(), sb.texels[sel].view(np.uint32).reshape(-1, 4), sb.subs, tree=tree, threads=threads)
-    dt = time.perf_counter() - t0
-    n1 = int(min(len(sb.texels), max(n0, n0 * target_s / max(dt, 1e-3))))
-    sel = np.linspace(0, len(sb.texels) - 1, n1).astype(np.int64)
-    t0 = time.perf_counter()
-    O.gather(arrays, sb.params.as_dict(), sb.texels[sel].view(np.uint32).reshape(-1, 4), sb.subs, tree=tree, threads=threads)
-    dt = time.perf_counter() - t0
-    return dict(value=round(n1 / dt, 1), unit="texels/s", cores=threads, kind="port",
-                sample=f"{n1} of {len(sb.texels)} texels evenly spread, oracle C gather (FP64 sums) over the same LBVH, "
-                       f"{dt:.1f} s")
 
 
 # ---------------------------------------------------------------------------------------
@@ -518,41 +396,3 @@ def autotune(plan: BakePlan, sample_texels: int = 60000, candidates=None, cache_
         with open(cache_path, "w") as f:
             json.dump(cache, f, indent=1, sort_keys=True)
     return entry
-
-
-def bench_batched(steps: int, warmup: int, cpu_threads: int, n_objects: int = 200, target_tris: int = 2_000_000):
-    """Config 3: 200 objects (~2M triangles), lightmaps 32..1024, 1024 rays per texel,
-    sun + 4 point lights; one step = first gather + persistent final gather + filters
-    replayed from a hipGraph, launch shape from the autotuner."""
-    import os
-    t0 = time.perf_counter()
-    sc = S.object_field(seed=4, n_objects=n_objects, target_tris=target_tris)
-    gen_s = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    plan = BakePlan(sc, GatherParams(quality=1024, ray_length_min=S.minimal_safe_distance(sc.verts), seed=1))
-    setup_s = time.perf_counter() - t0
-    plan.run(first=True, graph=False)
-    t0 = time.perf_counter()
-    cache = os.environ.get("LSG_TUNING_CACHE", os.path.expanduser("~/.cache/lsgpu_tuning.json"))
-    os.makedirs(os.path.dirname(cache), exist_ok=True)
-    tune = autotune(plan, sample_texels=150_000, cache_path=cache)
-    tune_s = time.perf_counter() - t0
-    for _ in range(warmup):
-        plan.run(first=True, graph=True)
-    N.check(N.lib().lsg_synchronize(), "sync")
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        plan.run(first=True, graph=True)
-    N.check(N.lib().lsg_synchronize(), "sync")
-    wall = (time.perf_counter() - t0) / steps
-    inf = plan.info()
-    return dict(metric=METRIC, value=round(inf.num_texels / wall, 1), unit="texels/s", ms_per_step=round(wall * 1e3, 2),
-                dtype="f32",
-                config=dict(workload="batched bake: 200 objects (displaced boxes, cylinders, terrain patches), "
-                                     "lightmaps 32..1024 RGBF, 1024 rays/texel, sun + 4 point lights, hipGraph replay",
-                            triangles=int(sc.ntri), objects=len(sc.objects), texels=int(inf.num_texels),
-                            texel_slots=int(inf.num_slots), subtexels=int(inf.num_subtexels),
-                            tuning=dict(tile=inf.tuning.tile, waves_per_block=inf.tuning.waves_per_block,
-                                        blocks_per_cu=inf.tuning.blocks_per_cu, xcd_queues=inf.tuning.xcd_queues),
-                            autotune=tune, autotune_s=round(tune_s, 2), generate_s=round(gen_s, 2),
-                            setup_s=round(setup_s, 2), grow_steps=int(inf.grow_steps)))
