@@ -1,0 +1,257 @@
+// RRSolverHIP.cpp -- see RRSolverHIP.h.
+#include "RRSolverHIP.h"
+#include "Lightsprint/RRBuffer.h"
+#include "Lightsprint/RRLight.h"
+#include "Lightsprint/RRMaterial.h"
+#include "Lightsprint/RRObject.h"
+#include <vector>
+
+namespace rr_hip {
+
+using namespace rr;
+
+RRSolverHIP::~RRSolverHIP()
+{
+	releaseScene();
+}
+
+void RRSolverHIP::releaseScene()
+{
+	if (scene) lsg_scene_destroy(scene);
+	if (col) lsg_collider_destroy(col);
+	scene = nullptr;
+	col = nullptr;
+}
+
+// The multiObject (RRSolver.h:962) as the GPU scene: world-space triangles with their
+// own three vertices (vertex normals differ per triangle, RRMesh::getTriangleNormals,
+// RRMesh.h:217-231), side bits and colours of the triangle materials
+// (RRObject::getTriangleMaterial, RRObject.h:137) and the exitance of the current
+// solution (getTriangleMeasure with RM_RADIOSITY_LINEAR, RRSolver.h:797: reflected
+// direct + indirect light plus emittance, the value a gather ray reads at a hit,
+// pathtracer.cpp:139-394).
+bool RRSolverHIP::buildScene(const UpdateParameters& params)
+{
+	releaseScene();
+	const RRObject* multi = getMultiObject();
+	if (!multi) return false;
+	const RRMesh* mesh = multi->getCollider()->getMesh();
+	const unsigned nt = mesh->getNumTriangles();
+	std::vector<float> verts(9 * (size_t)nt), normals(9 * (size_t)nt), exitance(8 * (size_t)nt, 0.f);
+	std::vector<uint32_t> idx(3 * (size_t)nt);
+	std::vector<uint8_t> side(nt, 0);
+	albedo.assign(3 * (size_t)nt, 0.f);
+	emittance.assign(3 * (size_t)nt, 0.f);
+	for (unsigned t = 0; t < nt; t++)
+	{
+		RRMesh::Triangle tr;
+		mesh->getTriangle(t, tr);
+		RRMesh::TriangleNormals tn;
+		mesh->getTriangleNormals(t, tn);
+		for (int k = 0; k < 3; k++)
+		{
+			RRMesh::Vertex p;
+			mesh->getVertex(tr[k], p);
+			for (int c = 0; c < 3; c++)
+			{
+				verts[9 * (size_t)t + 3 * k + c] = p[c];
+				normals[9 * (size_t)t + 3 * k + c] = tn.vertex[k].normal[c];
+			}
+			idx[3 * (size_t)t + k] = 3 * t + k;
+		}
+		const RRMaterial* m = multi->getTriangleMaterial(t, nullptr, nullptr);
+		if (m)
+		{
+			// bit0/bit1: front/back side seen by gather rays (pathtracer.h:101-103),
+			// bit2: casts shadows (opaque)
+			side[t] = (m->sideBits[0].renderFrom ? 1 : 0) | (m->sideBits[1].renderFrom ? 2 : 0)
+			        | (m->specularTransmittance.color == RRVec3(0) ? 4 : 0);
+			for (int c = 0; c < 3; c++)
+			{
+				albedo[3 * (size_t)t + c] = m->diffuseReflectance.colorLinear[c];
+				emittance[3 * (size_t)t + c] = m->diffuseEmittance.colorLinear[c] * params.direct.materialEmittanceMultiplier;
+			}
+		}
+		RRVec3 ex(0);
+		if (getTriangleMeasure(t, 3, RM_RADIOSITY_LINEAR, ex))
+			for (int c = 0; c < 3; c++)
+			{
+				exitance[8 * (size_t)t + c] = ex[c];
+				if (m && m->sideBits[1].emitTo) exitance[8 * (size_t)t + 4 + c] = ex[c];
+			}
+	}
+	// lights (RRLight.h:50-153): point with REALISTIC attenuation and directional
+	std::vector<lsg_light> lights;
+	const RRLights& ls = getLights();
+	for (unsigned i = 0; i < ls.size(); i++)
+	{
+		const RRLight* L = ls[i];
+		if (!L || !L->enabled) continue;
+		lsg_light g = {};
+		if (L->type == RRLight::POINT && L->distanceAttenuationType == RRLight::REALISTIC)
+		{
+			g.type = 0;
+			for (int c = 0; c < 3; c++) g.position[c] = L->position[c];
+		}
+		else if (L->type == RRLight::DIRECTIONAL)
+		{
+			g.type = 1;
+			for (int c = 0; c < 3; c++) g.direction[c] = L->direction[c];
+		}
+		else
+		{
+			RRReporter::report(ERRO, "RRSolverHIP: light %d: only point (REALISTIC) and directional lights are supported.\n", i);
+			return false;
+		}
+		for (int c = 0; c < 3; c++) g.color[c] = L->color[c];
+		g.cast_shadows = L->castShadows ? 1 : 0;
+		lights.push_back(g);
+	}
+	lsg_build_params bp = { 4, 0 };
+	if (lsg_collider_create(verts.data(), 3 * nt, idx.data(), nt, &bp, nullptr, &col) != LSG_OK)
+	{
+		RRReporter::report(ERRO, "RRSolverHIP: %s\n", lsg_last_error());
+		return false;
+	}
+	if (lsg_scene_create(col, normals.data(), side.data(), exitance.data(), lights.data(), (uint32_t)lights.size(), &scene) != LSG_OK)
+	{
+		RRReporter::report(ERRO, "RRSolverHIP: %s\n", lsg_last_error());
+		releaseScene();
+		return false;
+	}
+	return true;
+}
+
+// Texels of the targets' objects in multiObject triangle numbering, as
+// enumerateTexelsPartial walks them (lightmap.cpp:114-143): single post-import
+// triangle -> pre-import index -> multiObject post-import triangle, uv from the
+// material's lightmap texcoord channel.
+unsigned RRSolverHIP::bake(const Target* targets, unsigned numTargets, const UpdateParameters& params,
+                           const FilteringParameters* filtering)
+{
+	if (!buildScene(params)) return 0;
+	const RRObjects& objects = getStaticObjects();
+	const RRMesh* multiMesh = getMultiObject()->getCollider()->getMesh();
+	std::vector<std::vector<float> > uvs(numTargets);
+	std::vector<std::vector<uint32_t> > ids(numTargets);
+	std::vector<lsg_bake_object> bo(numTargets);
+	for (unsigned k = 0; k < numTargets; k++)
+	{
+		const RRObject* single = objects[targets[k].object];
+		const RRMesh* singleMesh = single->getCollider()->getMesh();
+		for (unsigned st = 0; st < singleMesh->getNumTriangles(); st++)
+		{
+			RRMesh::PreImportNumber pre(targets[k].object, singleMesh->getPreImportTriangle(st).index);
+			const unsigned t = multiMesh->getPostImportTriangle(pre);
+			if (t == UINT_MAX) continue;
+			const RRMaterial* m = single->getTriangleMaterial(st, nullptr, nullptr);
+			RRMesh::TriangleMapping map;
+			if (!m || !singleMesh->getTriangleMapping(st, map, m->lightmap.texcoord)) continue;
+			for (int v = 0; v < 3; v++) { uvs[k].push_back(map.uv[v][0]); uvs[k].push_back(map.uv[v][1]); }
+			ids[k].push_back(t);
+		}
+		bo[k].tri_uv = uvs[k].data();
+		bo[k].tri_ids = ids[k].data();
+		bo[k].ntri = (uint32_t)ids[k].size();
+		bo[k].width = targets[k].buffer->getWidth();
+		bo[k].height = targets[k].buffer->getHeight();
+	}
+	lsg_bake* plan = nullptr;
+	if (lsg_bake_create(scene, bo.data(), numTargets, albedo.data(), emittance.data(), &plan) != LSG_OK)
+	{
+		RRReporter::report(ERRO, "RRSolverHIP: %s\n", lsg_last_error());
+		return 0;
+	}
+	// UpdateParameters -> gather parameters (gather.cpp:197,396; RRSolver.cpp:333)
+	lsg_gather_params g = {};
+	g.quality = params.quality;
+	g.hemisphere = 1;            // the current solution (or emittance) is gathered
+	{
+		RRVec3 mini, maxi;
+		multiMesh->getAABB(&mini, &maxi, nullptr);
+		g.ray_length_min = (maxi - mini).avg() * 1e-6f;   // minimalSafeDistance, RRSolver.cpp:333
+	}
+	g.locality = params.locality;
+	g.rug_distance = params.rugDistance;
+	g.light_multiplier = params.direct.lightMultiplier;
+	g.seed = 1;
+	lsg_filter_params f = { 1.f, 1000.f, {0, 0, 0, 0}, 0 };
+	if (filtering)
+	{
+		f.smoothing_amount = filtering->smoothingAmount;
+		f.spread_foreground = (float)filtering->spreadForegroundColor;
+		for (int c = 0; c < 4; c++) f.background[c] = filtering->backgroundColor[c];
+		f.wrap = filtering->wrap ? 1 : 0;
+	}
+	unsigned updated = 0;
+	if (lsg_bake_run(plan, nullptr, &g, &f, 0, nullptr) != LSG_OK)
+		RRReporter::report(ERRO, "RRSolverHIP: %s\n", lsg_last_error());
+	else
+	{
+		// writeback: RGBF texels into the caller's buffers (copyElementsTo, RRBuffer.cpp:105;
+		// scaled formats take the solver's colour space)
+		for (unsigned k = 0; k < numTargets; k++)
+		{
+			RRBuffer* b = targets[k].buffer;
+			std::vector<float> rgbf(3 * (size_t)bo[k].width * bo[k].height);
+			if (lsg_bake_download(plan, k, rgbf.data()) != LSG_OK) continue;
+			const RRColorSpace* cs = b->getScaled() ? getColorSpace() : nullptr;
+			for (size_t i = 0; i < (size_t)bo[k].width * bo[k].height; i++)
+				b->setElement((unsigned)i, RRVec4(rgbf[3 * i], rgbf[3 * i + 1], rgbf[3 * i + 2], 1), cs);
+			b->version++;
+			updated++;
+		}
+	}
+	lsg_bake_destroy(plan);
+	return updated;
+}
+
+unsigned RRSolverHIP::updateLightmaps(int layerLightmap, int layerDirectionalLightmap, int layerBentNormals,
+                                      const UpdateParameters* params, const FilteringParameters* filtering)
+{
+	const bool supported = params && params->quality > 0 && layerDirectionalLightmap < 0 && layerBentNormals < 0
+	                    && (!getEnvironment() || params->direct.environmentMultiplier == 0) && lsg_device_count() > 0;
+	if (!supported)
+	{
+		if (referenceFallback) return RRSolver::updateLightmaps(layerLightmap, layerDirectionalLightmap, layerBentNormals, params, filtering);
+		RRReporter::report(ERRO, "RRSolverHIP::updateLightmaps: unsupported request (needs quality>0, lightmap layer only, "
+		                         "no environment, a gfx950 device).\n");
+		return 0;
+	}
+	// indirect solution first, by the reference's own solver (RRSolver.h:731-734)
+	if (!params->useCurrentSolution)
+		RRSolver::updateLightmaps(-1, -1, -1, params, nullptr);
+	if (layerLightmap < 0) return 0;
+	std::vector<Target> targets;
+	const RRObjects& objects = getStaticObjects();
+	for (unsigned i = 0; i < objects.size(); i++)
+	{
+		RRBuffer* b = objects[i]->illumination.getLayer(layerLightmap);
+		if (b && b->getType() == BT_2D_TEXTURE) targets.push_back(Target{i, b});
+		else if (b && !referenceFallback)
+			RRReporter::report(WARN, "RRSolverHIP: object %d: only 2d lightmaps are updated on the GPU.\n", i);
+	}
+	return targets.empty() ? 0 : bake(targets.data(), (unsigned)targets.size(), *params, filtering);
+}
+
+unsigned RRSolverHIP::updateLightmap(int objectNumber, RRBuffer* lightmap, RRBuffer* directionalLightmap[3],
+                                     RRBuffer* bentNormals, const UpdateParameters* params, const FilteringParameters* filtering)
+{
+	const RRObjects& objects = getStaticObjects();
+	const bool dir = directionalLightmap && (directionalLightmap[0] || directionalLightmap[1] || directionalLightmap[2]);
+	const bool supported = params && params->quality > 0 && !dir && !bentNormals && lightmap
+	                    && lightmap->getType() == BT_2D_TEXTURE && objectNumber >= 0 && (unsigned)objectNumber < objects.size()
+	                    && (!getEnvironment() || params->direct.environmentMultiplier == 0) && lsg_device_count() > 0;
+	if (!supported)
+	{
+		if (referenceFallback) return RRSolver::updateLightmap(objectNumber, lightmap, directionalLightmap, bentNormals, params, filtering);
+		RRReporter::report(ERRO, "RRSolverHIP::updateLightmap: unsupported request.\n");
+		return 0;
+	}
+	if (!params->useCurrentSolution)
+		RRSolver::updateLightmaps(-1, -1, -1, params, nullptr);
+	Target t{(unsigned)objectNumber, lightmap};
+	return bake(&t, 1, *params, filtering);
+}
+
+} // namespace rr_hip

@@ -1,0 +1,61 @@
+// RRSolverHIP.h -- RRSolver whose per-pixel lightmap final gather runs on the MI355X.
+//
+// Drop-in through the reference's own extension pattern (subclass RRSolver and
+// override the virtual update functions, as RRSolverGL : RRSolver does,
+// include/Lightsprint/GL/RRSolverGL.h:28):
+//
+//   rr_hip::RRSolverHIP solver;                  // instead of rr::RRSolver
+//   solver.setStaticObjects(objects, nullptr);   // unchanged
+//   solver.setLights(lights);                    // unchanged
+//   solver.updateLightmaps(0, -1, -1, &params, &filtering);
+//
+// updateLightmaps (RRSolver.h:737) keeps the reference's orchestration: unless
+// params->useCurrentSolution, the reference's own solver first computes the indirect
+// solution (first gather + radiosity, RRSolver::updateLightmaps(-1,-1,-1,...)); the
+// per-texel final gather of every lightmap (processTexel, gather.cpp:600-836), the
+// filter chain (RRBuffer.cpp:630-975) and the writeback then run on the GPU through
+// liblsgpu's C ABI (lsg_bake_*), one persistent launch for all objects.
+//
+// Supported: per-pixel lightmaps (BT_2D_TEXTURE) in the lightmap layer, no
+// directional or bent-normal layers, point (REALISTIC attenuation) and directional
+// lights, Lambertian triangle materials, no environment.  Anything else is refused
+// with an RRReporter error and 0 buffers updated; setReferenceFallback(true) hands
+// such calls to the reference RRSolver instead.
+#pragma once
+#include "Lightsprint/RRSolver.h"
+#include "lsgpu.h"
+#include <vector>
+
+namespace rr_hip {
+
+class RRSolverHIP : public rr::RRSolver
+{
+public:
+	RRSolverHIP() {}
+	~RRSolverHIP() override;
+
+	// RRSolver.h:737
+	unsigned updateLightmaps(int layerLightmap, int layerDirectionalLightmap, int layerBentNormals,
+	                         const UpdateParameters* params, const FilteringParameters* filtering) override;
+	// RRSolver.h:675: the same GPU path restricted to one object and one buffer
+	unsigned updateLightmap(int objectNumber, rr::RRBuffer* lightmap, rr::RRBuffer* directionalLightmap[3],
+	                        rr::RRBuffer* bentNormals, const UpdateParameters* params,
+	                        const FilteringParameters* filtering = nullptr) override;
+
+	// unsupported inputs go to the reference implementation instead of failing
+	void setReferenceFallback(bool enable) { referenceFallback = enable; }
+
+private:
+	struct Target { unsigned object; rr::RRBuffer* buffer; };
+	unsigned bake(const Target* targets, unsigned numTargets, const UpdateParameters& params,
+	              const FilteringParameters* filtering);
+	bool buildScene(const UpdateParameters& params);
+	void releaseScene();
+
+	bool referenceFallback = false;
+	lsg_collider* col = nullptr;
+	lsg_scene* scene = nullptr;
+	std::vector<float> albedo, emittance;       // per multiObject triangle
+};
+
+} // namespace rr_hip
