@@ -37,6 +37,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# L2-to-fabric bytes per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by
+# the same counters on a 1 GiB copy (x2.0 and x1.0): profiles/pmc_traffic_k_gather_k_trace_closest.md
+TRAFFIC_BYTES = {"k_gather": 6.308e9, "k_trace_closest": 1.811e9}
 METRIC = "closest-hit Mrays/s; baked texels/s at ≤2% RMSE, plain vs learned gather"
 
 
@@ -92,7 +95,7 @@ def collider_workload(steps, warmup, cpu_threads, with_cpu=True):
                                  bvh_depth=int(info.depth), build_ms=round(info.build_ms, 3),
                                  hit_ratio=round(float((hits["tri"] >= 0).mean()), 4)),
         roofline=dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                      frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                      frac=round(achieved / HBM_PEAK_GBS, 4), traffic=TRAFFIC_BYTES["k_trace_closest"],
                       kernel="k_trace_closest", kernel_ms=round(kernel_ms, 4),
                       bytes_per_ray=round(b_ray, 2), n_node=round(n_node, 3), n_tri=round(n_tri, 3)),
     )
@@ -179,11 +182,12 @@ def bake_workload(steps: int, warmup: int, cpu_threads: int):
                     mrays_per_s=round(rays_per_s / 1e6, 2), first_gather_s=round(first_s, 4), setup_s=round(setup_s, 3),
                     mean_irradiance=float(lit.mean())),
         roofline=dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                      frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel="k_gather", kernel_ms=round(kernel_ms, 4),
+                      frac=round(achieved / HBM_PEAK_GBS, 4), traffic=TRAFFIC_BYTES["k_gather"], kernel="k_gather", kernel_ms=round(kernel_ms, 4),
                       bytes_per_texel=round(b_texel, 1), bytes_per_hemisphere_ray=round(b_ray, 2),
                       bytes_per_shadow_ray=round(b_shadow, 2), n_node_hemi=round(hn / hr, 3), n_tri_hemi=round(ht / hr, 3),
                       n_node_shadow=round(sn / max(sr, 1), 3), n_tri_shadow=round(stt / max(sr, 1), 3),
-                      subtexels_per_texel=round(n_sub, 3)),
+                      subtexels_per_texel=round(n_sub, 3), algorithmic_bytes=round(b_texel * ntex),
+                      traffic_note="L2-to-fabric bytes per launch (PMC); the scene stays in L2/MALL"),
     )
     res["cpu_baseline"] = bake_cpu_baseline(sb, cpu_threads)
     return res
