@@ -224,17 +224,20 @@ def learned_summary(reps: int = 2, log=None):
     got = Lr.load_weights()
     if got is None:
         return dict(error="no trained surrogate (python -m lightsprintsdk_amd.learned train)")
-    net, tau, meta = got
-    rows = Lr.heldout_report(net, tau, reps=reps, log=log or (lambda m: None))
+    net, bound, meta = got
+    rows = Lr.heldout_report(net, bound, reps=reps, log=log or (lambda m: None))
     tex = sum(r["texels"] for r in rows)
     tp = sum(r["plain_ms"] for r in rows) / 1e3
     tl = sum(r["learned_ms"] for r in rows) / 1e3
     return dict(workload="learned gather: 5 held-out box-room scenes (seeds 2000-2004), 16-ray cheap gather + "
-                         "MLP 32-64x4-4 (BF16 MFMA, FP32 accumulate) + fallback to the 1024-ray plain gather; "
+                         "MLP 32-64x4-4 (BF16 MFMA, FP32 accumulate) + per-scene error bound from a second "
+                         "16-ray gather + fallback to the 1024-ray plain gather; "
                          "RMSE of the filtered lightmaps against the plain bake of the same scene",
                 texels_per_s=round(tex / tl, 1), plain_texels_per_s=round(tex / tp, 1), speedup=round(tp / tl, 3),
                 rmse_max=round(max(r["rmse"] for r in rows), 5), rmse_bound=0.02,
-                fallback_mean=round(float(np.mean([r["fallback"] for r in rows])), 4), tau=tau,
+                fallback_mean=round(float(np.mean([r["fallback"] for r in rows])), 4),
+                bound=dict(kind="per-scene: largest tau whose estimated kept-texel RMSE (second 16-ray gather) "
+                                "is within target", target=bound["target"]),
                 training=dict(scenes=20, samples=meta.get("samples"), epochs=meta.get("epochs"),
                               train_s=meta.get("train_s")),
                 scenes=[{k: (round(v, 5) if isinstance(v, float) else v) for k, v in r.items()} for r in rows])
@@ -245,18 +248,18 @@ def learned_workload(steps: int, warmup: int):
     got = Lr.load_weights()
     if got is None:
         raise SystemExit("no trained surrogate: run python -m lightsprintsdk_amd.learned train")
-    net, tau, meta = got
+    net, bound, meta = got
     scenes = [Lr.prepare_eval(sd) for sd in Lr.HELDOUT_SEEDS]
     L = N.lib()
     for k in range(warmup):
-        scenes[k % len(scenes)].learned(net, tau)
+        scenes[k % len(scenes)].learned(net, target=bound["target"])
     N.check(L.lsg_synchronize(), "sync")
     t0 = time.perf_counter()
     tex = 0
     fb = 0
     for k in range(steps):
         ls = scenes[k % len(scenes)]
-        fb += ls.learned(net, tau)
+        fb += ls.learned(net, target=bound["target"])
         tex += ls.n
     N.check(L.lsg_synchronize(), "sync")
     wall = time.perf_counter() - t0

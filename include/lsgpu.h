@@ -343,12 +343,23 @@ int  lsg_learned_scale(const float* parts, uint32_t n, float* scale);
 int  lsg_learned_features(lsg_scene* scene, const lsg_texel* texels, uint32_t n, const lsg_subtexel* subs,
                           const float* parts, const float* tri_albedo, float scale, const lsg_lightmap* maps,
                           uint32_t nmaps, uint32_t nslots, float* work, float* x, void* stream);
-/* writes s*expm1(y) into out_rgba for texels whose cheap pass was reliable and whose
- * predicted absolute error |y3| * (1 + mean expm1(y0..2)) -- in units of s, the
- * scene's mean irradiance -- is at most tau; appends the others to fallback and counts
- * them in *nfallback (device counter, caller zeroes) */
-int  lsg_learned_decide(const float* y, const lsg_texel* texels, const float* parts, uint32_t n, float scale,
-                        float tau, float* out_rgba, lsg_texel* fallback, uint32_t* nfallback, void* stream);
+/* y: network outputs (device float[n*4]): correction r to the window baseline
+ * b = log1p((H5 + L3)/s) of the features x (device float[n*32]) and predicted error.
+ * Writes E = s*expm1(b + r) into out_rgba for texels whose cheap pass was reliable and
+ * whose predicted absolute error |y3| * (1 + mean E/s) -- in units of s, the scene's
+ * mean irradiance -- is at most tau; appends the others to fallback and counts them in
+ * *nfallback (device counter, caller zeroes) */
+int  lsg_learned_decide(const float* y, const float* x, const lsg_texel* texels, const float* parts, uint32_t n,
+                        float scale, float tau, float* out_rgba, lsg_texel* fallback, uint32_t* nfallback, void* stream);
+/* Per-scene error bound.  cheap2_rgba: a second cheap gather of the same texels with
+ * another seed (device float4 per output slot, as lsg_gather_device writes it).  From
+ * (pred - c2)^2 - (c1 - c2)^2 / 2, an unbiased estimate of the network's squared error,
+ * summed per predicted-error bin, picks the largest tau whose kept texels stay within
+ * target_rmse (relative to the RMS prediction).  work: device uint64[513].  *est_rmse
+ * (optional) = estimated relative RMSE with no fallback.  Synchronous. */
+int  lsg_learned_bound(const float* y, const float* x, const lsg_texel* texels, const float* parts,
+                       const float* cheap2_rgba, uint32_t n, float scale, float target_rmse, uint64_t* work,
+                       float* tau, float* est_rmse, void* stream);
 
 #ifdef __cplusplus
 }

@@ -16,9 +16,13 @@
 //   31    1
 // The means run over texels of the same lightmap whose cheap pass was reliable and
 // whose normal is within ~18 degrees of the texel's (same chart face).  Outputs of the
-// network: log1p(E/s) (3) and the predicted RMS log error (1).
+// network: a correction r (3) to the window baseline b = log1p((H5 + L3)/s), H5 the
+// 5x5 hemisphere mean and L3 the 3x3 light mean, so that log1p(E/s) = b + r, and the
+// predicted RMS error of log1p(E/s) (1).
 #include "lsgpu_internal.h"
 #include "lsgpu_gather.h"
+#include <cmath>
+#include <vector>
 
 namespace lsg {
 
@@ -183,9 +187,36 @@ __global__ __launch_bounds__(256) void k_cheap_sum(const float4* __restrict__ pa
     if (threadIdx.x == 0) { atomicAdd(&acc[0], s[0][0]); atomicAdd(&acc[1], s[1][0]); }
 }
 
+// log1p(E/s) of the window baseline, channel c
+__device__ inline float base_log(const float* xi, int c)
+{
+    return log1pf(expm1f(xi[9 + c]) + expm1f(xi[15 + c]));
+}
+
+// network irradiance E/s (3) and the predicted absolute error in units of s: an error
+// d in log1p(E/s) is an irradiance error of about d * (E + s)
+__device__ inline float predict(const float* yi, const float* xi, float e[3])
+{
+    for (int c = 0; c < 3; c++) e[c] = expm1f(base_log(xi, c) + yi[c]);
+    const float em = fmaxf((e[0] + e[1] + e[2]) * (1.f / 3.f), 0.f);
+    return fabsf(yi[3]) * (1.f + em);
+}
+
+// error-score histogram: 8 bins per octave from 2^-24 to 2^8 (256 bins)
+constexpr int kHistBins = 256;
+__host__ __device__ inline int score_bin(float score)
+{
+    if (!(score > 0.f)) return 0;
+    const float b = floorf((log2f(score) + 24.f) * 8.f);
+    return b < 0.f ? 0 : (b > (float)(kHistBins - 1) ? kHistBins - 1 : (int)b);
+}
+// upper score edge of bin b
+__host__ __device__ inline float bin_upper(int b) { return exp2f((float)(b + 1) / 8.f - 24.f); }
+
 // keep the network's answer or queue the texel for the plain gather: a texel falls
 // back when its cheap pass was unreliable or its predicted error exceeds tau * s
-__global__ __launch_bounds__(256) void k_decide(const float* __restrict__ y, const lsg_texel* __restrict__ texels,
+__global__ __launch_bounds__(256) void k_decide(const float* __restrict__ y, const float* __restrict__ x,
+                                                const lsg_texel* __restrict__ texels,
                                                 const float4* __restrict__ parts, uint32_t n, float scale, float tau,
                                                 float4* __restrict__ out, lsg_texel* __restrict__ fallback,
                                                 uint32_t* __restrict__ nfallback)
@@ -198,14 +229,14 @@ __global__ __launch_bounds__(256) void k_decide(const float* __restrict__ y, con
         const float4 a = parts[2 * (size_t)i], b = parts[2 * (size_t)i + 1];
         const bool reliable = a.w > 0 || b.w > 0;
         const float* yi = y + 4 * (size_t)i;
-        // predicted absolute error in units of s: an error d in log1p(E/s) is an
-        // irradiance error of about d * (E + s)
-        const float em = fmaxf((expm1f(yi[0]) + expm1f(yi[1]) + expm1f(yi[2])) * (1.f / 3.f), 0.f);
-        fb = !reliable || !(fabsf(yi[3]) * (1.f + em) <= tau);
+        const float* xi = x + kFeat * (size_t)i;
+        float e[3];
+        const float score = predict(yi, xi, e);
+        fb = !reliable || !(score <= tau);
         if (!fb) {
             const uint32_t flags = (T.sub_count_flags >> 16) & 0xffu;
-            out[T.out_index] = make_float4(fmaxf(scale * expm1f(yi[0]), 0.f), fmaxf(scale * expm1f(yi[1]), 0.f),
-                                           fmaxf(scale * expm1f(yi[2]), 0.f), flags / 255.f);
+            out[T.out_index] = make_float4(fmaxf(scale * e[0], 0.f), fmaxf(scale * e[1], 0.f), fmaxf(scale * e[2], 0.f),
+                                           flags / 255.f);
         }
     }
     // one atomic per wavefront: ballot the fallbacks, rank them, reserve a range
@@ -216,6 +247,48 @@ __global__ __launch_bounds__(256) void k_decide(const float* __restrict__ y, con
     if (lane == 0 && cnt) base = atomicAdd(nfallback, cnt);
     base = (uint32_t)__shfl((int)base, 0);
     if (fb) fallback[base + (uint32_t)__popcll(mask & (lane ? (~0ull >> (64 - lane)) : 0ull))] = T;
+}
+
+// Unbiased per-scene estimate of the surrogate's squared error from a second,
+// independent cheap gather c2 (same rays per texel, another seed): with c1 the cheap
+// gather the features came from and E the expected irradiance,
+//   E[(pred - c2)^2] = (pred - E)^2 + var(c2),   E[(c1 - c2)^2] / 2 = var(c),
+// so (pred - c2)^2 - (c1 - c2)^2 / 2 estimates (pred - E)^2 without the plain gather.
+// Per score bin: the sum of that estimate (fixed point, units s^2 * 2^-24, signed)
+// and the texel count; hist[2*256] = sum of pred^2 over reliable texels (same units).
+__global__ __launch_bounds__(256) void k_error_hist(const float* __restrict__ y, const float* __restrict__ x,
+                                                    const lsg_texel* __restrict__ texels, const float4* __restrict__ parts,
+                                                    const float4* __restrict__ c2img, uint32_t n, float inv_s,
+                                                    unsigned long long* __restrict__ hist)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = parts[2 * (size_t)i], b = parts[2 * (size_t)i + 1];
+    if (!(a.w > 0 || b.w > 0)) return;                   // always re-gathered
+    float e[3];
+    const float score = predict(y + 4 * (size_t)i, x + kFeat * (size_t)i, e);
+    const float4 c2 = c2img[texels[i].out_index];
+    const float c1v[3] = {(a.x + b.x) * inv_s, (a.y + b.y) * inv_s, (a.z + b.z) * inv_s};
+    const float c2v[3] = {c2.x * inv_s, c2.y * inv_s, c2.z * inv_s};
+    float d = 0.f, v = 0.f, p = 0.f;
+    for (int c = 0; c < 3; c++) {
+        const float pc = fmaxf(e[c], 0.f);
+        d += (pc - c2v[c]) * (pc - c2v[c]);
+        v += (c1v[c] - c2v[c]) * (c1v[c] - c2v[c]);
+        p += pc * pc;
+    }
+    const float est = (d - 0.5f * v) * (1.f / 3.f), pw = p * (1.f / 3.f);
+    const int bin = score_bin(score);
+    const double q = 16777216.0;
+    atomicAdd(&hist[bin], (unsigned long long)(long long)llrint((double)est * q));
+    atomicAdd(&hist[kHistBins + bin], 1ull);
+    atomicAdd(&hist[2 * kHistBins], (unsigned long long)(long long)llrint((double)pw * q));
+}
+
+__global__ void k_zero_u64(unsigned long long* p, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0ull;
 }
 
 static inline unsigned nb(uint64_t n) { return (unsigned)((n + 255) / 256); }
@@ -289,13 +362,46 @@ int lsg_learned_features(lsg_scene* s, const lsg_texel* texels, uint32_t n, cons
     return LSG_OK;
 }
 
-int lsg_learned_decide(const float* y, const lsg_texel* texels, const float* parts, uint32_t n, float scale, float tau,
-                       float* out_rgba, lsg_texel* fallback, uint32_t* nfallback, void* stream)
+int lsg_learned_decide(const float* y, const float* x, const lsg_texel* texels, const float* parts, uint32_t n, float scale,
+                       float tau, float* out_rgba, lsg_texel* fallback, uint32_t* nfallback, void* stream)
 {
-    if (n && (!y || !texels || !parts || !out_rgba || !fallback || !nfallback)) return fail(LSG_ERR_INVALID, "lsg_learned_decide: null argument");
+    if (n && (!y || !x || !texels || !parts || !out_rgba || !fallback || !nfallback)) return fail(LSG_ERR_INVALID, "lsg_learned_decide: null argument");
     if (!n) return LSG_OK;
-    k_decide<<<nb(n), 256, 0, (hipStream_t)stream>>>(y, texels, (const float4*)parts, n, scale, tau, (float4*)out_rgba, fallback, nfallback);
+    k_decide<<<nb(n), 256, 0, (hipStream_t)stream>>>(y, x, texels, (const float4*)parts, n, scale, tau, (float4*)out_rgba, fallback, nfallback);
     LSG_HIP(hipGetLastError());
+    return LSG_OK;
+}
+
+// Largest bound tau (a score-bin edge) whose kept texels' estimated squared error
+// stays within target_rmse^2 of the mean squared prediction: the per-scene error bound.
+int lsg_learned_bound(const float* y, const float* x, const lsg_texel* texels, const float* parts, const float* cheap2_rgba,
+                      uint32_t n, float scale, float target_rmse, uint64_t* work, float* tau, float* est_rmse, void* stream)
+{
+    if (!tau || (n && (!y || !x || !texels || !parts || !cheap2_rgba || !work))) return fail(LSG_ERR_INVALID, "lsg_learned_bound: null argument");
+    if (!(scale > 0) || !(target_rmse >= 0)) return fail(LSG_ERR_INVALID, "lsg_learned_bound: scale > 0 and target_rmse >= 0 required");
+    *tau = 0.f;
+    if (est_rmse) *est_rmse = 0.f;
+    if (!n) return LSG_OK;
+    hipStream_t st = (hipStream_t)stream;
+    const uint32_t nh = 2 * kHistBins + 1;
+    k_zero_u64<<<nb(nh), 256, 0, st>>>((unsigned long long*)work, nh);
+    k_error_hist<<<nb(n), 256, 0, st>>>(y, x, texels, (const float4*)parts, (const float4*)cheap2_rgba, n, 1.f / scale,
+                                        (unsigned long long*)work);
+    LSG_HIP(hipGetLastError());
+    std::vector<long long> h(nh);
+    LSG_HIP(hipMemcpyAsync(h.data(), work, 8 * nh, hipMemcpyDeviceToHost, st));
+    LSG_HIP(hipStreamSynchronize(st));
+    const double norm = (double)h[2 * kHistBins];
+    double cum = 0.0, total = 0.0;
+    for (int b = 0; b < kHistBins; b++) total += (double)h[b];
+    int best = -1;
+    for (int b = 0; b < kHistBins; b++) {
+        cum += (double)h[b];
+        if (cum <= (double)target_rmse * target_rmse * norm) best = b;
+        else if (h[kHistBins + b]) break;               // keep a prefix of the score order
+    }
+    *tau = best < 0 ? 0.f : (best == kHistBins - 1 ? INFINITY : bin_upper(best));
+    if (est_rmse) *est_rmse = norm > 0 ? (float)sqrt(fmax(total, 0.0) / norm) : 0.f;
     return LSG_OK;
 }
 

@@ -88,11 +88,10 @@ __global__ void k_loss_grad(const float* __restrict__ y, const float* __restrict
                             float* __restrict__ dy, uint16_t* __restrict__ dyb, float* __restrict__ loss_acc)
 {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
     float l = 0.f;
     float g[16];
     for (int c = 0; c < 16; c++) g[c] = 0.f;
-    if (i < nvalid) {
+    if (i < nvalid && i < n) {
         const float* yi = y + 16 * (size_t)i;
         const float* ti = t + 3 * (size_t)i;
         float se = 0.f;
@@ -102,8 +101,9 @@ __global__ void k_loss_grad(const float* __restrict__ y, const float* __restrict
         g[3] = 2.f * lambda * de / (float)nvalid;
         l = se + lambda * de * de;
     }
-    for (int c = 0; c < 16; c++) { dy[16 * (size_t)i + c] = g[c]; dyb[16 * (size_t)i + c] = f2bf(g[c]); }
-    // block reduction of the loss
+    if (i < n)
+        for (int c = 0; c < 16; c++) { dy[16 * (size_t)i + c] = g[c]; dyb[16 * (size_t)i + c] = f2bf(g[c]); }
+    // block reduction of the loss (every thread of the block takes part)
     __shared__ float s[256];
     s[threadIdx.x] = l;
     __syncthreads();

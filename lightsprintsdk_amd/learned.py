@@ -30,6 +30,14 @@ CHEAP_QUALITY = 16
 CHEAP_LIGHT_ROUNDS = 0        # 0: the reference's quality/10+1 rounds (2 at quality 16)
 LAMBDA = 0.5
 NFEAT = 32
+FEATURE_VERSION = 2      # 32 features, outputs relative to the window baseline
+
+
+def base_log(X: np.ndarray) -> np.ndarray:
+    """log1p(E/s) of the window baseline: 5x5 hemisphere mean + 3x3 light mean
+    (features 9-11 and 15-17, stored as log1p(./s))."""
+    X = np.asarray(X, np.float64)
+    return np.log1p(np.expm1(X[:, 9:12]) + np.expm1(X[:, 15:18]))
 
 
 def rmse_rel(a: np.ndarray, b: np.ndarray) -> float:
@@ -155,6 +163,8 @@ class LearnedScene:
         self.d_cheap_out = DeviceBuffer(16 * max(self.sb.nout, 1))
         self.d_x = DeviceBuffer(4 * NFEAT * max(n, 1))
         self.d_work = DeviceBuffer(48 * max(self.sb.nout, 1))
+        self.d_cheap2 = DeviceBuffer(16 * max(self.sb.nout, 1))
+        self.d_hist = DeviceBuffer(8 * 513)
         self.d_y = DeviceBuffer(16 * max(n, 1))
         self.d_fb = DeviceBuffer(16 * max(n, 1))
         self.d_nfb = DeviceBuffer(16)
@@ -198,8 +208,15 @@ class LearnedScene:
         N.check(N.lib().lsg_synchronize(), "sync")
         return self.d_x.download(np.float32, (self.n, NFEAT))
 
-    def targets(self, plain_rgb):
-        return np.log1p(np.maximum(plain_rgb[:, :3], 0) / self.scale).astype(np.float32)
+    def targets(self, plain_rgb, X):
+        """Training targets: log1p(E/s) of the plain gather minus the window baseline of
+        the features X."""
+        return (np.log1p(np.maximum(plain_rgb[:, :3], 0) / self.scale) - base_log(X)).astype(np.float32)
+
+    def predict_rgb(self, net: "Surrogate"):
+        """The network's irradiance for every texel (no fallback), from the current features."""
+        X = self.features()
+        return self.scale * np.expm1(base_log(X) + net.predict(X)[:, :3])
 
     def close(self):
         self.sb.gs.close()
@@ -213,16 +230,37 @@ class LearnedScene:
                                           ctypes.byref(p), self.sb.d_out.ptr, stream), "cheap gather")
         self.sb.filter(stream)
 
-    def learned(self, net: Surrogate, tau: float, stream=None, filtering=True):
+    def error_bound(self, net: Surrogate, target: float, stream=None):
+        """Per-scene bound tau for a target relative RMSE of the kept texels, estimated
+        from a second, independently seeded cheap gather (lsg_learned_bound).  Needs the
+        features and network outputs of the current pass."""
+        L = N.lib()
+        p = self.cheap.c()
+        p.seed = (self.cheap.seed * 2654435761 + 0x3C6EF372) & 0xFFFFFFFF
+        N.check(L.lsg_gather_device(self.sb.gs._h, self.sb.d_tex.ptr, self.n, self.sb.d_sub.ptr, len(self.sb.subs),
+                                    ctypes.byref(p), self.d_cheap2.ptr, stream), "second cheap gather")
+        tau = ctypes.c_float(); est = ctypes.c_float()
+        N.check(L.lsg_learned_bound(self.d_y.ptr, self.d_x.ptr, self.sb.d_tex.ptr, self.d_parts.ptr, self.d_cheap2.ptr,
+                                    self.n, self.scale, target, self.d_hist.ptr, ctypes.byref(tau), ctypes.byref(est),
+                                    stream), "bound")
+        self.last_est_rmse = float(est.value)
+        return float(tau.value)
+
+    def learned(self, net: Surrogate, tau: float | None = None, target: float | None = None, stream=None,
+                filtering=True):
         """Cheap gather + features + MLP + decision + plain gather of the fallback texels,
-        writing every texel into sb.d_out, then the filter chain into sb.d_rgbf.
+        writing every texel into sb.d_out, then the filter chain into sb.d_rgbf.  The
+        bound is a fixed tau, or with target the per-scene bound of error_bound.
         Returns the fallback count."""
         L = N.lib()
         self.sb.clear()
         self.cheap_and_features(stream)
         net.forward(self.d_x.ptr, self.n, self.d_y.ptr, stream)
+        if target is not None:
+            tau = self.error_bound(net, target, stream)
+        self.last_tau = tau
         N.check(L.lsg_memset(self.d_nfb.ptr, 0, 4), "memset")
-        N.check(L.lsg_learned_decide(self.d_y.ptr, self.sb.d_tex.ptr, self.d_parts.ptr, self.n, self.scale, tau,
+        N.check(L.lsg_learned_decide(self.d_y.ptr, self.d_x.ptr, self.sb.d_tex.ptr, self.d_parts.ptr, self.n, self.scale, tau,
                                      self.sb.d_out.ptr, self.d_fb.ptr, self.d_nfb.ptr, stream), "decide")
         N.check(L.lsg_synchronize(), "sync")
         nfb = int(self.d_nfb.download(np.uint32, (1,))[0])
@@ -254,8 +292,9 @@ def build_dataset(seeds, lightmap_size=256, target_tris=50_000, augment=1, log=N
         for a in range(augment):
             ls.cheap.seed = base + 0x632BE5AB * a & 0xFFFFFFFF
             ls.cheap_and_features()
-            Xs.append(ls.features()[ok])
-            Ts.append(ls.targets(ref)[ok])
+            X = ls.features()
+            Xs.append(X[ok])
+            Ts.append(ls.targets(ref, X)[ok])
         if log:
             log(f"scene {sd}: {int(ok.sum())} texels x {augment}, scale {ls.scale:.4f}")
         ls.close()
@@ -271,24 +310,42 @@ def prepare_eval(seed, lightmap_size=256, target_tris=50_000) -> LearnedScene:
     return ls
 
 
-def evaluate(ls: LearnedScene, net: Surrogate, tau: float):
+def evaluate(ls: LearnedScene, net: Surrogate, tau: float | None = None, target: float | None = None):
     """(relative RMSE of the filtered learned+fallback lightmaps against the plain bake,
     fallback fraction)."""
-    nfb = ls.learned(net, tau)
+    nfb = ls.learned(net, tau=tau, target=target)
     return rmse_rel(ls.final_rgb(), ls.ref_final), nfb / max(ls.n, 1)
 
 
-TAU_CANDIDATES = (0.01, 0.015, 0.02, 0.025, 0.03, 0.04, 0.05, 0.06, 0.08, 0.1, 0.13, 0.17, 0.22, 0.3)
+TAU_CANDIDATES = (0.01, 0.0125, 0.015, 0.0175, 0.02, 0.025, 0.03, 0.035, 0.04, 0.045, 0.05, 0.06, 0.07, 0.08,
+                  0.1, 0.12, 0.15, 0.2, 0.3)
 
 
 def calibrate_tau(net: Surrogate, prepared, target_rmse: float = 0.018, candidates=TAU_CANDIDATES):
-    """Largest bound tau on the predicted error whose learned+fallback bake keeps the
-    filtered-lightmap RMSE of every calibration scene at or below target_rmse."""
+    """Largest fixed bound tau on the predicted error whose learned+fallback bake keeps
+    the filtered-lightmap RMSE of every calibration scene at or below target_rmse."""
     best = candidates[0]
     for tau in candidates:
-        worst = max(evaluate(ls, net, tau)[0] for ls in prepared)
+        worst = max(evaluate(ls, net, tau=tau)[0] for ls in prepared)
         if worst <= target_rmse:
             best = tau
+        else:
+            break
+    return best
+
+
+TARGET_CANDIDATES = (0.012, 0.014, 0.016, 0.018, 0.02, 0.022, 0.024, 0.026, 0.028, 0.03)
+
+
+def calibrate_target(net: Surrogate, prepared, target_rmse: float = 0.018, candidates=TARGET_CANDIDATES):
+    """Largest per-scene estimate target (lsg_learned_bound) whose learned+fallback
+    bake keeps the filtered-lightmap RMSE of every calibration scene at or below
+    target_rmse."""
+    best = candidates[0]
+    for t in candidates:
+        worst = max(evaluate(ls, net, target=t)[0] for ls in prepared)
+        if worst <= target_rmse:
+            best = t
         else:
             break
     return best
@@ -302,28 +359,33 @@ CAL_SEEDS = (1500, 1501)                 # tau calibration
 HELDOUT_SEEDS = range(2000, 2005)        # 5 held-out scenes
 
 
-def save_weights(path, net: Surrogate, tau: float, meta: dict):
+def save_weights(path, net: Surrogate, bound: dict, meta: dict):
+    """bound: {"target": per-scene estimate target, "tau": calibrated fixed bound}."""
     import json
-    np.savez(path, params=net.get_params(), tau=np.float32(tau), nfeat=np.int32(NFEAT), meta=np.array(json.dumps(meta)))
+    np.savez(path, params=net.get_params(), target=np.float32(bound["target"]), tau=np.float32(bound["tau"]),
+             nfeat=np.int32(NFEAT), version=np.int32(FEATURE_VERSION), meta=np.array(json.dumps(meta)))
 
 
 def load_weights(path=WEIGHTS, max_batch: int = 65536):
-    """(Surrogate with the stored parameters, tau, meta) or None when absent."""
+    """(Surrogate with the stored parameters, bound dict, meta) or None when absent or
+    trained for other features."""
     import json
     import os
     if not os.path.exists(path):
         return None
     z = np.load(path, allow_pickle=False)
-    if int(z["nfeat"]) != NFEAT:
+    if int(z["nfeat"]) != NFEAT or "version" not in z.files or int(z["version"]) != FEATURE_VERSION \
+            or "target" not in z.files:
         return None
     net = Surrogate(max_batch=max_batch)
     net.set_params(z["params"])
-    return net, float(z["tau"]), json.loads(str(z["meta"]))
+    return net, dict(target=float(z["target"]), tau=float(z["tau"])), json.loads(str(z["meta"]))
 
 
 def train_box_room(augment=2, epochs=12, batch=8192, lr=2e-3, target_rmse=0.018, log=print):
-    """Train the surrogate on the 20 training scenes, calibrate tau on the calibration
-    scenes; returns (net, tau, meta)."""
+    """Train the surrogate on the 20 training scenes; calibrate on the calibration
+    scenes both the per-scene estimate target and, for comparison, a fixed tau.
+    Returns (net, bound, meta)."""
     t0 = time.perf_counter()
     X, T = build_dataset(TRAIN_SEEDS, augment=augment, log=log)
     t1 = time.perf_counter()
@@ -333,38 +395,41 @@ def train_box_room(augment=2, epochs=12, batch=8192, lr=2e-3, target_rmse=0.018,
     n = len(X)
     del X, T
     cal = [prepare_eval(s) for s in CAL_SEEDS]
-    tau = calibrate_tau(net, cal, target_rmse)
+    bound = dict(target=calibrate_target(net, cal, target_rmse), tau=calibrate_tau(net, cal, target_rmse))
     meta = dict(train_seeds=[TRAIN_SEEDS.start, TRAIN_SEEDS.stop], cal_seeds=list(CAL_SEEDS), augment=augment,
                 epochs=epochs, batch=batch, lr=lr, samples=int(n), loss=hist, target_rmse=target_rmse,
                 dataset_s=round(t1 - t0, 2), train_s=round(t2 - t1, 2))
-    log(f"trained on {n} texels in {t2 - t1:.1f}s, tau {tau}")
-    return net, tau, meta
+    log(f"trained on {n} texels in {t2 - t1:.1f}s, bound {bound}")
+    return net, bound, meta
 
 
 def _sync():
     N.check(N.lib().lsg_synchronize(), "sync")
 
 
-def heldout_report(net: Surrogate, tau: float, seeds=HELDOUT_SEEDS, reps: int = 3, log=print):
+def heldout_report(net: Surrogate, bound: dict, seeds=HELDOUT_SEEDS, reps: int = 3, adaptive: bool = True, log=print):
     """Plain vs learned+fallback bake on each held-out scene: RMSE of the filtered
-    lightmaps against the plain bake, fallback fraction and the time of both."""
+    lightmaps against the plain bake, fallback fraction and the time of both.  The
+    bound is per scene (bound["target"]) or, with adaptive=False, the fixed tau."""
+    kw = dict(target=bound["target"]) if adaptive else dict(tau=bound["tau"])
     rows = []
     for sd in seeds:
         ls = prepare_eval(sd)
         ls.cheap_only()
         r_cheap = rmse_rel(ls.final_rgb(), ls.ref_final)
-        r_net, _ = evaluate(ls, net, float("inf"))
+        r_net, _ = evaluate(ls, net, tau=float("inf"))
         tp, tl = [], []
         for _ in range(reps):
             _sync(); a = time.perf_counter(); ls.plain(); _sync(); b = time.perf_counter()
-            nfb = ls.learned(net, tau); _sync(); c = time.perf_counter()
+            nfb = ls.learned(net, **kw); _sync(); c = time.perf_counter()
             tp.append(b - a); tl.append(c - b)
         r, f = rmse_rel(ls.final_rgb(), ls.ref_final), nfb / max(ls.n, 1)
-        row = dict(seed=sd, texels=ls.n, rmse=r, rmse_cheap=r_cheap, rmse_network=r_net, fallback=f,
+        row = dict(seed=sd, texels=ls.n, rmse=r, rmse_cheap=r_cheap, rmse_network=r_net, fallback=f, tau=ls.last_tau,
                    plain_ms=1e3 * min(tp), learned_ms=1e3 * min(tl))
         rows.append(row)
-        log(f"held-out {sd}: {ls.n} texels, rmse {r:.4f} (cheap {r_cheap:.4f}, network {r_net:.4f}), fallback {f:.3f}, "
-            f"plain {row['plain_ms']:.1f} ms, learned {row['learned_ms']:.1f} ms, x{row['plain_ms'] / row['learned_ms']:.2f}")
+        log(f"held-out {sd}: {ls.n} texels, rmse {r:.4f} (cheap {r_cheap:.4f}, network {r_net:.4f}), fallback {f:.3f} "
+            f"(tau {ls.last_tau:.4g}), plain {row['plain_ms']:.1f} ms, learned {row['learned_ms']:.1f} ms, "
+            f"x{row['plain_ms'] / row['learned_ms']:.2f}")
         ls.close()
     return rows
 
@@ -376,19 +441,20 @@ def _main(argv=None):
     ap.add_argument("--weights", default=WEIGHTS)
     ap.add_argument("--augment", type=int, default=2)
     ap.add_argument("--epochs", type=int, default=12)
+    ap.add_argument("--fixed", action="store_true", help="report with the fixed tau instead of the per-scene bound")
     a = ap.parse_args(argv)
     t0 = time.perf_counter()
     log = lambda m: print(f"[{time.perf_counter() - t0:7.1f}s] {m}", flush=True)
     if a.cmd == "train":
-        net, tau, meta = train_box_room(augment=a.augment, epochs=a.epochs, log=log)
-        save_weights(a.weights, net, tau, meta)
+        net, bound, meta = train_box_room(augment=a.augment, epochs=a.epochs, log=log)
+        save_weights(a.weights, net, bound, meta)
         log(f"saved {a.weights}")
     else:
         got = load_weights(a.weights)
         if got is None:
             raise SystemExit(f"no weights at {a.weights}")
-        net, tau, meta = got
-    heldout_report(net, tau, log=log)
+        net, bound, meta = got
+    heldout_report(net, bound, adaptive=not a.fixed, log=log)
 
 
 if __name__ == "__main__":

@@ -92,13 +92,14 @@ def test_learned_gather_heldout(gpu):
     print("train loss", [round(h, 5) for h in hist], flush=True)
     assert hist[-1] < 0.2 * hist[0]
     cal = [Lr.prepare_eval(3100), Lr.prepare_eval(3101)]
-    tau = Lr.calibrate_tau(net, cal, target_rmse=0.015)
+    target = Lr.calibrate_target(net, cal, target_rmse=0.015)
     ls = Lr.prepare_eval(3200)
     ls.cheap_only()
     r_cheap = Lr.rmse_rel(ls.final_rgb(), ls.ref_final)
-    r_net, f_unrel = Lr.evaluate(ls, net, float("inf"))
-    r_final, frac = Lr.evaluate(ls, net, tau)
-    print(f"held-out: tau {tau} cheap {r_cheap:.4f} network {r_net:.4f} (unreliable {f_unrel:.3f}) "
+    r_net, f_unrel = Lr.evaluate(ls, net, tau=float("inf"))
+    r_final, frac = Lr.evaluate(ls, net, target=target)
+    print(f"held-out: target {target} (tau {ls.last_tau:.4g}, estimated network rmse {ls.last_est_rmse:.4f}) "
+          f"cheap {r_cheap:.4f} network {r_net:.4f} (unreliable {f_unrel:.3f}) "
           f"learned+fallback {r_final:.4f} fallback {frac:.3f}", flush=True)
     assert r_net < 0.8 * r_cheap
     assert r_final <= 0.02
