@@ -50,22 +50,33 @@ __device__ inline float3 safe_inv(float3 d)
 // Closest accepted hit of one ray.  stk = this lane's first stack slot (LDS),
 // successive slots `stride` ints apart.  Returns false on stack overflow (cannot
 // happen when STACK > tree depth, which the host guarantees).
+//
+// Loop shape: "while-while" with speculative traversal (Aila & Laine, HPG 2009).
+// Lanes descend inner nodes; a lane that reaches a leaf parks it and keeps
+// descending, until every lane of the wave holds a parked leaf; then the wave tests
+// triangles together.  Inner-node and triangle work thus run with full lane masks
+// instead of interleaving per iteration.  The result does not depend on the visiting
+// order (see the header), so it is the same as any other order's.
 template <int STACK, bool STATS, class Filter>
 __device__ inline bool trace_closest(const Filter& flt, const ColliderView& c, const float3 o, const float3 d,
                                      float rmin, float rmax, int32_t* stk, int stride, TraceResult& res,
                                      uint32_t& nvis, uint32_t& ntest)
 {
+    constexpr int32_t kDone = 0x7fffffff;          // never a node index
     res.tri = -1;
     float lo, hi;
     if (!c.nvalid || !ref_box_intersect(c.bmin, c.bmax, o, d, rmin, rmax, lo, hi)) return true;
     const float3 idir = safe_inv(d);
     float best = hi, bound = hi;
     int bi = -1; float bu = 0.f, bv = 0.f; bool bf = false; uint32_t bslot = 0;
-    int32_t ref = c.root;
+    int32_t ref = c.root;          // next node: >= 0 inner, < 0 leaf, kDone finished
+    int32_t leaf = 0;              // parked leaf (< 0) or none (0)
     int sp = 0;
     bool ok = true;
-    for (;;) {
-        if (ref >= 0) {
+    auto pop = [&]() -> int32_t { return sp ? stk[(--sp) * stride] : kDone; };
+    while (ref != kDone || leaf < 0) {
+        // inner nodes, parking the first leaf met
+        while (ref >= 0 && ref != kDone) {
             if (STATS) nvis++;
             const Node nd = c.nodes[ref];
             float n0, f0, n1, f1;
@@ -76,12 +87,18 @@ __device__ inline bool trace_closest(const Filter& flt, const ColliderView& c, c
                 const bool swap = n1 < n0;
                 if (sp < STACK) stk[(sp++) * stride] = swap ? r0 : r1; else ok = false;
                 ref = swap ? r1 : r0;
-                continue;
-            }
-            if (h0) { ref = r0; continue; }
-            if (h1) { ref = r1; continue; }
-        } else {
-            const uint32_t st = leaf_start(ref), cnt = leaf_count(ref);
+            } else if (h0) ref = r0;
+            else if (h1) ref = r1;
+            else ref = pop();
+            if (ref < 0 && leaf == 0) { leaf = ref; ref = pop(); }
+            if (!__any(leaf == 0)) break;                // every active lane holds a leaf
+        }
+        // a leaf reached outside the loop above (the root itself, or a leaf next on
+        // the stack while none was parked) is parked here, so every pass progresses
+        if (leaf == 0 && ref < 0) { leaf = ref; ref = pop(); }
+        // triangles of the parked leaf, then of leaves met directly on the stack
+        while (leaf < 0) {
+            const uint32_t st = leaf_start(leaf), cnt = leaf_count(leaf);
             bool stop = false;
             for (uint32_t k = 0; k < cnt; k++) {
                 const uint32_t p = st + k;
@@ -100,10 +117,10 @@ __device__ inline bool trace_closest(const Filter& flt, const ColliderView& c, c
                 bound = flt.clamp(best);
                 if (flt.done(t)) { stop = true; break; }
             }
-            if (stop) break;
+            if (stop) { ref = kDone; leaf = 0; break; }
+            leaf = 0;
+            if (ref < 0) { leaf = ref; ref = pop(); }
         }
-        if (sp == 0) break;
-        ref = stk[(--sp) * stride];
     }
     if (bi >= 0) { res.tri = bi; res.t = best; res.u = bu; res.v = bv; res.front = bf; res.slot = bslot; }
     return ok;
