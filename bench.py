@@ -1,6 +1,6 @@
 """Benchmark of the MI355X ray-cast + lightmap-gather path (one GPU).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload bake|collider|batched|learned]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload bake|collider|batched|learned|scale]
 
 Prints ONE JSON line.  Workloads (synthetic, seeded):
   bake      (default) plain lightmap bake: closed 10x10x4 m box room + 20 boxes
@@ -17,6 +17,9 @@ Prints ONE JSON line.  Workloads (synthetic, seeded):
             first gather + persistent final gather + filters replayed from a hipGraph.
   learned   the 5 held-out box-room scenes: one step = the learned+fallback bake of
             one scene (cycled); value = its texels/s, the plain bake timed beside it.
+  scale     city of 5000 boxes + 2048^2-vertex terrain (~10M triangles), 64 lightmaps
+            of 2048^2; one step = the re-bake for one sun direction of a 60-step sweep
+            (learned+fallback); p50/p99 step latency, plain re-bake timed once.
 Timing: W untimed steps, then K steps between device synchronisations; the dominant
 kernel is also timed with HIP events on the stream it is launched on.  The CPU
 baseline is the oracle (or the compiled reference collider where it exists) on a
@@ -271,6 +274,77 @@ def learned_workload(steps: int, warmup: int):
                 learned=summ)
 
 
+# ---- config 5: large re-bake with a sun sweep ---------------------------------------------
+
+def scale_workload(steps: int, warmup: int, sweep: int = 60, log=None):
+    """Config 5: city of 5000 extruded boxes + 2048^2-vertex terrain (~10M triangles),
+    64 lightmaps of 2048^2.  One step = a full re-bake for the next sun direction of a
+    60-step sweep: first gather (exitance) + learned+fallback final gather + filters.
+    `steps` directions spread over the sweep are timed (p50/p99); the plain re-bake of
+    the first direction is timed once and is the RMSE reference."""
+    from lightsprintsdk_amd import learned as Lr, native as N, scenes as S
+    log = log or (lambda m: print(m, file=sys.stderr, flush=True))
+    L = N.lib()
+    got = Lr.load_weights(Lr.weights_path("city"))
+    if got is None:
+        raise SystemExit("no trained city surrogate: python -m lightsprintsdk_amd.learned train --family city")
+    net, bound, meta = got
+    t0 = time.perf_counter()
+    idx = [int(round(k * (sweep - 1) / max(steps - 1, 1))) for k in range(steps)]
+    sc = S.city_terrain(seed=5, sun_direction=S.sun_sweep(idx[0], sweep))
+    gen_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ls = Lr.LearnedScene(sc, seed=1)
+    setup_s = time.perf_counter() - t0
+    ntex = ls.n
+    log(f"config 5: {sc.ntri} triangles, {len(sc.objects)} lightmaps, {ntex} texels; generate {gen_s:.1f}s, setup {setup_s:.1f}s")
+
+    def set_sun(i):
+        lt = dict(sc.lights[0]); lt["direction"] = S.sun_sweep(i, sweep)
+        ls.sb.gs.set_lights([lt])
+
+    def sync():
+        N.check(L.lsg_synchronize(), "sync")
+    # plain re-bake of the first direction (reference and plain timing)
+    set_sun(idx[0]); sync()
+    t0 = time.perf_counter()
+    ls.sb.first_gather()
+    ls.plain()
+    sync()
+    plain_s = time.perf_counter() - t0
+    ref = ls.final_rgb()
+    log(f"plain re-bake: {plain_s:.2f}s")
+    for _ in range(warmup):
+        ls.learned(net, target=bound["target"])
+    sync()
+    lat, fbs, rmse0 = [], [], None
+    for k, i in enumerate(idx):
+        set_sun(i); sync()
+        t0 = time.perf_counter()
+        ls.sb.first_gather()
+        nfb = ls.learned(net, target=bound["target"])
+        sync()
+        lat.append(time.perf_counter() - t0)
+        fbs.append(nfb / max(ntex, 1))
+        if k == 0:
+            rmse0 = Lr.rmse_rel(ls.final_rgb(), ref)
+        log(f"step {k} (sweep {i}): {lat[-1]:.2f}s, fallback {fbs[-1]:.3f}" + (f", rmse {rmse0:.4f}" if k == 0 else ""))
+    lat = np.array(lat)
+    p50 = float(np.percentile(lat, 50))
+    return dict(metric=METRIC, value=round(ntex / float(lat.mean()), 1), unit="texels/s",
+                ms_per_step=round(float(lat.mean()) * 1e3, 1), dtype="bf16 MLP + f32 gather",
+                config=dict(workload="large re-bake: city of 5000 extruded boxes + 2048^2-vertex terrain, "
+                                     "64 lightmaps 2048^2 RGBF, directional sun swept over 60 steps; step = first "
+                                     "gather + learned+fallback final gather (per-scene error bound) + filters",
+                            triangles=int(sc.ntri), lightmaps=len(sc.objects), texels=int(ntex),
+                            sweep_steps_timed=idx, generate_s=round(gen_s, 1), setup_s=round(setup_s, 1)),
+                scale=dict(step_latency_p50_s=round(p50, 3), step_latency_p99_s=round(float(np.percentile(lat, 99)), 3),
+                           plain_step_s=round(plain_s, 3), plain_texels_per_s=round(ntex / plain_s, 1),
+                           speedup=round(plain_s / float(lat.mean()), 3), rmse_first_step=round(rmse0, 5),
+                           fallback=[round(f, 4) for f in fbs], target=bound["target"],
+                           surrogate=dict(family="city", samples=meta.get("samples"), train_s=meta.get("train_s"))))
+
+
 # ---- config 3: batched persistent bake ----------------------------------------------------
 
 def batched_workload(steps: int, warmup: int, n_objects: int = 200, target_tris: int = 2_000_000):
@@ -318,7 +392,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default=os.environ.get("LSG_BENCH_WORKLOAD", "bake"),
-                    choices=["bake", "collider", "batched", "learned"])
+                    choices=["bake", "collider", "batched", "learned", "scale"])
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-extras", action="store_true")
     a = ap.parse_args()
@@ -339,6 +413,8 @@ def main():
         res = batched_workload(a.steps, a.warmup)
     elif a.workload == "learned":
         res = learned_workload(a.steps, a.warmup)
+    elif a.workload == "scale":
+        res = scale_workload(a.steps, a.warmup)
     else:
         res = collider_workload(a.steps, a.warmup, threads)
     out = dict(metric=res.pop("metric"), value=res.pop("value"), unit=res.pop("unit"), n_gpus=1, steps=a.steps,

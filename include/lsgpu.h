@@ -343,8 +343,9 @@ int  lsg_learned_scale(const float* parts, uint32_t n, float* scale);
 int  lsg_learned_features(lsg_scene* scene, const lsg_texel* texels, uint32_t n, const lsg_subtexel* subs,
                           const float* parts, const float* tri_albedo, float scale, const lsg_lightmap* maps,
                           uint32_t nmaps, uint32_t nslots, float* work, float* x, void* stream);
-/* y: network outputs (device float[n*4]): correction r to the window baseline
- * b = log1p((H5 + L3)/s) of the features x (device float[n*32]) and predicted error.
+/* y: network outputs (device float[n*4]): correction r to the baseline
+ * b = log1p((H5 + L)/s) of the features x (device float[n*32]) -- 5x5 hemisphere mean
+ * and the texel's own light part -- and predicted error.
  * Writes E = s*expm1(b + r) into out_rgba for texels whose cheap pass was reliable and
  * whose predicted absolute error |y3| * (1 + mean E/s) -- in units of s, the scene's
  * mean irradiance -- is at most tau; appends the others to fallback and counts them in

@@ -192,15 +192,28 @@ class SceneBake:
         base = 0
         sub_base = 0
         self.slots = []
-        for k, ob in enumerate(scene.objects):
+        bases = np.cumsum([0] + [ob.width * ob.height for ob in scene.objects])
+        # objects rasterise independently (lsg_rasterize_texels is reentrant and ctypes
+        # releases the GIL during the call); results are joined in object order
+        import concurrent.futures as cf
+        import os
+
+        def one(k):
+            ob = scene.objects[k]
             ids = np.arange(ob.tri_begin, ob.tri_begin + ob.tri_count, dtype=np.uint32)
-            t, s = rasterize(scene.tri_uv[ob.tri_begin:ob.tri_begin + ob.tri_count], ids, ob.width, ob.height, k, base)
+            return rasterize(scene.tri_uv[ob.tri_begin:ob.tri_begin + ob.tri_count], ids, ob.width, ob.height, k,
+                             int(bases[k]))
+        with cf.ThreadPoolExecutor(max_workers=max(1, min(16, os.cpu_count() or 1, len(scene.objects)))) as ex:
+            parts = list(ex.map(one, range(len(scene.objects))))
+        for k, ob in enumerate(scene.objects):
+            t, s = parts[k]
             t["sub_begin"] += sub_base
             texs.append(t)
             subs.append(s)
             self.slots.append((base, ob.width, ob.height))
             base += ob.width * ob.height
             sub_base += len(s)
+        del parts
         self.texels = np.concatenate(texs) if texs else np.zeros(0, TEXEL_DTYPE)
         self.subs = np.concatenate(subs) if subs else np.zeros(0, SUBTEXEL_DTYPE)
         self.nout = base

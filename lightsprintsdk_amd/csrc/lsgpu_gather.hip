@@ -416,7 +416,26 @@ int launch_gather_persistent(const GatherArgs& A, int stack, const WorkQueue& Q,
     return LSG_OK;
 }
 
+static int launch_gather_one(const GatherArgs& A, int stack, hipStream_t st);
+
+// very large texel lists go out in launches of at most kLaunchTexels texels (a few
+// seconds of work each at quality 1024)
+constexpr uint32_t kLaunchTexels = 1u << 22;
+
 int launch_gather(const GatherArgs& A, int stack, hipStream_t st)
+{
+    for (uint32_t base = 0; base < A.ntex; base += kLaunchTexels) {
+        GatherArgs B = A;
+        B.texels = A.texels + base;
+        B.ntex = std::min(kLaunchTexels, A.ntex - base);
+        if (A.parts) B.parts = A.parts + 2 * (size_t)base;
+        const int e = launch_gather_one(B, stack, st);
+        if (e) return e;
+    }
+    return LSG_OK;
+}
+
+static int launch_gather_one(const GatherArgs& A, int stack, hipStream_t st)
 {
     if (!A.ntex) return LSG_OK;
     // shooters per series = max(hemisphere rays, light rounds); a group of G >= that

@@ -16,9 +16,9 @@
 //   31    1
 // The means run over texels of the same lightmap whose cheap pass was reliable and
 // whose normal is within ~18 degrees of the texel's (same chart face).  Outputs of the
-// network: a correction r (3) to the window baseline b = log1p((H5 + L3)/s), H5 the
-// 5x5 hemisphere mean and L3 the 3x3 light mean, so that log1p(E/s) = b + r, and the
-// predicted RMS error of log1p(E/s) (1).
+// network: a correction r (3) to the baseline b = log1p((H5 + L)/s), H5 the 5x5
+// hemisphere mean and L the texel's own light part (unbiased, and sharp at shadow
+// edges), so that log1p(E/s) = b + r, and the predicted RMS error of log1p(E/s) (1).
 #include "lsgpu_internal.h"
 #include "lsgpu_gather.h"
 #include <cmath>
@@ -187,10 +187,10 @@ __global__ __launch_bounds__(256) void k_cheap_sum(const float4* __restrict__ pa
     if (threadIdx.x == 0) { atomicAdd(&acc[0], s[0][0]); atomicAdd(&acc[1], s[1][0]); }
 }
 
-// log1p(E/s) of the window baseline, channel c
+// log1p(E/s) of the baseline, channel c
 __device__ inline float base_log(const float* xi, int c)
 {
-    return log1pf(expm1f(xi[9 + c]) + expm1f(xi[15 + c]));
+    return log1pf(expm1f(xi[9 + c]) + expm1f(xi[3 + c]));
 }
 
 // network irradiance E/s (3) and the predicted absolute error in units of s: an error

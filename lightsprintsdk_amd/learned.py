@@ -30,14 +30,14 @@ CHEAP_QUALITY = 16
 CHEAP_LIGHT_ROUNDS = 0        # 0: the reference's quality/10+1 rounds (2 at quality 16)
 LAMBDA = 0.5
 NFEAT = 32
-FEATURE_VERSION = 2      # 32 features, outputs relative to the window baseline
+FEATURE_VERSION = 3      # 32 features, outputs relative to the baseline below
 
 
 def base_log(X: np.ndarray) -> np.ndarray:
-    """log1p(E/s) of the window baseline: 5x5 hemisphere mean + 3x3 light mean
-    (features 9-11 and 15-17, stored as log1p(./s))."""
+    """log1p(E/s) of the baseline: 5x5 hemisphere mean + the texel's own light part
+    (features 9-11 and 3-5, stored as log1p(./s))."""
     X = np.asarray(X, np.float64)
-    return np.log1p(np.expm1(X[:, 9:12]) + np.expm1(X[:, 15:18]))
+    return np.log1p(np.expm1(X[:, 9:12]) + np.expm1(X[:, 3:6]))
 
 
 def rmse_rel(a: np.ndarray, b: np.ndarray) -> float:
@@ -278,32 +278,56 @@ def config2_family(seed: int, lightmap_size: int = 256, target_tris: int = 50_00
     return S.box_room(seed=seed, n_boxes=20, target_tris=target_tris, lightmap_size=lightmap_size)
 
 
-def build_dataset(seeds, lightmap_size=256, target_tris=50_000, augment=1, log=None):
-    """(features, targets) of every reliable texel of every scene in seeds: features of
+def city_family(seed: int) -> S.BakeScene:
+    """A config-5-family scene at 1/16 of its area, same densities: a 512 m terrain
+    (25 cm texels) and 312 boxes in 3 city lightmaps of 2048^2, sun from the sweep."""
+    step = int(np.random.default_rng(seed).integers(0, 60))      # any direction of the sweep
+    return S.city_terrain(seed=seed, n_boxes=312, grid_vertices=513, terrain_tiles=1, lightmap_size=2048, city_maps=3,
+                          box_tris=100_000, sun_direction=S.sun_sweep(step))
+
+
+FAMILIES = {
+    "box_room": dict(make=config2_family, train=range(1000, 1020), cal=(1500, 1501), heldout=range(2000, 2005),
+                     weights="surrogate_box_room.npz", sample=None),
+    "city": dict(make=city_family, train=range(5000, 5020), cal=(5500, 5501, 5502), heldout=range(6000, 6003),
+                 weights="surrogate_city.npz", sample=1_000_000),
+}
+
+
+def _make(family, seed, **kw):
+    return FAMILIES[family]["make"](seed, **kw)
+
+
+def build_dataset(seeds, augment=1, family="box_room", sample=None, log=None, **kw):
+    """(features, targets) of the reliable texels of every scene in seeds: features of
     `augment` independently seeded cheap gathers, targets log1p(E/s) of the plain
-    1024-ray gather before the filters."""
+    1024-ray gather before the filters (relative to the window baseline).  sample: at
+    most this many texels per scene and cheap gather, drawn uniformly."""
     Xs, Ts = [], []
+    rng = np.random.default_rng(12345)
     for sd in seeds:
-        ls = LearnedScene(config2_family(sd, lightmap_size, target_tris), seed=sd)
+        ls = LearnedScene(_make(family, sd, **kw), seed=sd)
         ls.plain(filtering=False)
         ref = ls.plain_rgb()
-        ok = ref[:, 3] > 0
+        ok = np.nonzero(ref[:, 3] > 0)[0]
         base = ls.cheap.seed
         for a in range(augment):
+            sel = ok if (sample is None or len(ok) <= sample) else np.sort(rng.choice(ok, sample, replace=False))
             ls.cheap.seed = base + 0x632BE5AB * a & 0xFFFFFFFF
             ls.cheap_and_features()
-            X = ls.features()
-            Xs.append(X[ok])
-            Ts.append(ls.targets(ref, X)[ok])
+            X = ls.features()[sel]
+            Xs.append(X)
+            Ts.append(ls.targets(ref[sel], X))
         if log:
-            log(f"scene {sd}: {int(ok.sum())} texels x {augment}, scale {ls.scale:.4f}")
+            log(f"scene {sd}: {len(ok)} texels, {len(sel)} x {augment} taken, scale {ls.scale:.4f}")
         ls.close()
+        del ls
     return np.concatenate(Xs), np.concatenate(Ts)
 
 
-def prepare_eval(seed, lightmap_size=256, target_tris=50_000) -> LearnedScene:
+def prepare_eval(seed, family="box_room", **kw) -> LearnedScene:
     """A scene with its plain 1024-ray filtered lightmaps kept as the reference."""
-    ls = LearnedScene(config2_family(seed, lightmap_size, target_tris), seed=seed)
+    ls = LearnedScene(_make(family, seed, **kw), seed=seed)
     ls.plain()
     ls.ref_final = ls.final_rgb()
     ls.ref_raw = ls.plain_rgb()
@@ -334,7 +358,7 @@ def calibrate_tau(net: Surrogate, prepared, target_rmse: float = 0.018, candidat
     return best
 
 
-TARGET_CANDIDATES = (0.012, 0.014, 0.016, 0.018, 0.02, 0.022, 0.024, 0.026, 0.028, 0.03)
+TARGET_CANDIDATES = (0.012, 0.014, 0.016, 0.018, 0.02, 0.022, 0.024, 0.026, 0.028, 0.03, 0.033, 0.036, 0.04, 0.045, 0.05)
 
 
 def calibrate_target(net: Surrogate, prepared, target_rmse: float = 0.018, candidates=TARGET_CANDIDATES):
@@ -353,10 +377,15 @@ def calibrate_target(net: Surrogate, prepared, target_rmse: float = 0.018, candi
 
 # ---- trained surrogate for the config-2 scene family ---------------------------------------
 
-WEIGHTS = __file__.rsplit("/", 1)[0] + "/surrogate_box_room.npz"
-TRAIN_SEEDS = range(1000, 1020)          # 20 training scenes
-CAL_SEEDS = (1500, 1501)                 # tau calibration
-HELDOUT_SEEDS = range(2000, 2005)        # 5 held-out scenes
+_PKG = __file__.rsplit("/", 1)[0]
+WEIGHTS = _PKG + "/" + FAMILIES["box_room"]["weights"]
+TRAIN_SEEDS = FAMILIES["box_room"]["train"]       # 20 training scenes
+CAL_SEEDS = FAMILIES["box_room"]["cal"]           # bound calibration
+HELDOUT_SEEDS = FAMILIES["box_room"]["heldout"]   # 5 held-out scenes
+
+
+def weights_path(family: str = "box_room") -> str:
+    return _PKG + "/" + FAMILIES[family]["weights"]
 
 
 def save_weights(path, net: Surrogate, bound: dict, meta: dict):
@@ -382,39 +411,47 @@ def load_weights(path=WEIGHTS, max_batch: int = 65536):
     return net, dict(target=float(z["target"]), tau=float(z["tau"])), json.loads(str(z["meta"]))
 
 
-def train_box_room(augment=2, epochs=12, batch=8192, lr=2e-3, target_rmse=0.018, log=print):
-    """Train the surrogate on the 20 training scenes; calibrate on the calibration
-    scenes both the per-scene estimate target and, for comparison, a fixed tau.
-    Returns (net, bound, meta)."""
+def train_family(family="box_room", augment=2, epochs=12, batch=8192, lr=2e-3, target_rmse=0.017, log=print):
+    """Train the surrogate on the family's 20 training scenes; calibrate on its
+    calibration scenes both the per-scene estimate target and, for comparison, a fixed
+    tau.  Returns (net, bound, meta)."""
+    F = FAMILIES[family]
     t0 = time.perf_counter()
-    X, T = build_dataset(TRAIN_SEEDS, augment=augment, log=log)
+    X, T = build_dataset(F["train"], augment=augment, family=family, sample=F["sample"], log=log)
     t1 = time.perf_counter()
     net = Surrogate(max_batch=65536, seed=0)
     hist = net.train(X, T, epochs=epochs, batch=batch, lr=lr, log=log)
     t2 = time.perf_counter()
     n = len(X)
     del X, T
-    cal = [prepare_eval(s) for s in CAL_SEEDS]
+    cal = [prepare_eval(s, family) for s in F["cal"]]
     bound = dict(target=calibrate_target(net, cal, target_rmse), tau=calibrate_tau(net, cal, target_rmse))
-    meta = dict(train_seeds=[TRAIN_SEEDS.start, TRAIN_SEEDS.stop], cal_seeds=list(CAL_SEEDS), augment=augment,
-                epochs=epochs, batch=batch, lr=lr, samples=int(n), loss=hist, target_rmse=target_rmse,
-                dataset_s=round(t1 - t0, 2), train_s=round(t2 - t1, 2))
+    for ls in cal:
+        ls.close()
+    meta = dict(family=family, train_seeds=[F["train"].start, F["train"].stop], cal_seeds=list(F["cal"]),
+                augment=augment, sample=F["sample"], epochs=epochs, batch=batch, lr=lr, samples=int(n), loss=hist,
+                target_rmse=target_rmse, dataset_s=round(t1 - t0, 2), train_s=round(t2 - t1, 2))
     log(f"trained on {n} texels in {t2 - t1:.1f}s, bound {bound}")
     return net, bound, meta
+
+
+def train_box_room(**kw):
+    return train_family("box_room", **kw)
 
 
 def _sync():
     N.check(N.lib().lsg_synchronize(), "sync")
 
 
-def heldout_report(net: Surrogate, bound: dict, seeds=HELDOUT_SEEDS, reps: int = 3, adaptive: bool = True, log=print):
+def heldout_report(net: Surrogate, bound: dict, seeds=HELDOUT_SEEDS, reps: int = 3, adaptive: bool = True, log=print,
+                   family="box_room"):
     """Plain vs learned+fallback bake on each held-out scene: RMSE of the filtered
     lightmaps against the plain bake, fallback fraction and the time of both.  The
     bound is per scene (bound["target"]) or, with adaptive=False, the fixed tau."""
     kw = dict(target=bound["target"]) if adaptive else dict(tau=bound["tau"])
     rows = []
     for sd in seeds:
-        ls = prepare_eval(sd)
+        ls = prepare_eval(sd, family)
         ls.cheap_only()
         r_cheap = rmse_rel(ls.final_rgb(), ls.ref_final)
         r_net, _ = evaluate(ls, net, tau=float("inf"))
@@ -438,15 +475,17 @@ def _main(argv=None):
     import argparse
     ap = argparse.ArgumentParser(description="train / evaluate the learned-gather surrogate (config-2 scene family)")
     ap.add_argument("cmd", choices=["train", "eval"])
-    ap.add_argument("--weights", default=WEIGHTS)
+    ap.add_argument("--family", default="box_room", choices=sorted(FAMILIES))
+    ap.add_argument("--weights", default=None)
     ap.add_argument("--augment", type=int, default=2)
     ap.add_argument("--epochs", type=int, default=12)
     ap.add_argument("--fixed", action="store_true", help="report with the fixed tau instead of the per-scene bound")
     a = ap.parse_args(argv)
     t0 = time.perf_counter()
     log = lambda m: print(f"[{time.perf_counter() - t0:7.1f}s] {m}", flush=True)
+    a.weights = a.weights or weights_path(a.family)
     if a.cmd == "train":
-        net, bound, meta = train_box_room(augment=a.augment, epochs=a.epochs, log=log)
+        net, bound, meta = train_family(a.family, augment=a.augment, epochs=a.epochs, log=log)
         save_weights(a.weights, net, bound, meta)
         log(f"saved {a.weights}")
     else:
@@ -454,7 +493,8 @@ def _main(argv=None):
         if got is None:
             raise SystemExit(f"no weights at {a.weights}")
         net, bound, meta = got
-    heldout_report(net, bound, adaptive=not a.fixed, log=log)
+    heldout_report(net, bound, seeds=FAMILIES[a.family]["heldout"], adaptive=not a.fixed, log=log, family=a.family,
+                   reps=3 if a.family == "box_room" else 1)
 
 
 if __name__ == "__main__":

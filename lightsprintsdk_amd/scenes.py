@@ -151,16 +151,18 @@ def _grid_face(origin, du, dv, nu, nv_):
     P = origin[None, None, :] + S[..., None] * du[None, None, :] + T[..., None] * dv[None, None, :]
     verts = P.reshape(-1, 3)
     st = np.stack([S.reshape(-1), T.reshape(-1)], axis=1)
-    tris = []
-    for i in range(nu):
-        for j in range(nv_):
-            a = i * (nv_ + 1) + j
-            b = (i + 1) * (nv_ + 1) + j
-            c = (i + 1) * (nv_ + 1) + j + 1
-            d = i * (nv_ + 1) + j + 1
-            tris.append((a, b, c))
-            tris.append((a, c, d))
-    return verts, np.array(tris, dtype=np.int64), st
+    return verts, _grid_tris(nu, nv_), st
+
+
+def _grid_tris(nu, nv_):
+    """Two triangles per cell of an (nu+1) x (nv+1) vertex grid (vertex i*(nv+1)+j),
+    cells in i-major order: (a,b,c), (a,c,d)."""
+    i, j = np.meshgrid(np.arange(nu), np.arange(nv_), indexing="ij")
+    a = (i * (nv_ + 1) + j).reshape(-1)
+    b = a + (nv_ + 1)
+    c = b + 1
+    d = a + 1
+    return np.stack([np.stack([a, b, c], 1), np.stack([a, c, d], 1)], 1).reshape(-1, 3).astype(np.int64)
 
 
 def _box_faces(lo, hi, inward):
@@ -430,3 +432,101 @@ def object_field(seed: int = 4, n_objects: int = 200, target_tris: int = 2_000_0
                      normals=np.concatenate(N).astype(np.float32), tri_uv=np.concatenate(UV).astype(np.float32),
                      side_bits=np.concatenate(SB), albedo=np.concatenate(ALB).astype(np.float32),
                      emittance=np.concatenate(EM).astype(np.float32), objects=objects, lights=lights)
+
+
+# ---------------------------------------------------------------------------------------
+# large re-bake scene: city of extruded boxes on a heightfield terrain
+
+def _terrain_height(rng_params, x, y):
+    k, ph, amp = rng_params
+    return sum(np.sin(k[i, 0] * x + k[i, 1] * y + ph[i]) for i in range(len(ph))) * amp / len(ph)
+
+
+def city_terrain(seed: int = 5, n_boxes: int = 5000, grid_vertices: int = 2048, terrain_tiles: int = 4,
+                 lightmap_size: int = 2048, city_maps: int = 48, box_tris: int = 1_600_000,
+                 cell_m: float = 1.0, sun_direction=(0.4, 0.25, -0.88)) -> BakeScene:
+    """Config 5: a grid_vertices^2 heightfield (cell_m metres per cell) split into
+    terrain_tiles^2 lightmapped tiles, and n_boxes extruded boxes (5 faces) spread over
+    city_maps lightmaps; all lightmaps lightmap_size^2.  One directional sun."""
+    rng = np.random.default_rng(seed)
+    cells = grid_vertices - 1
+    size = cells * cell_m
+    tp = (rng.uniform(0.5, 2.0, size=(4, 2)) * 2 * np.pi / size, rng.uniform(0, 2 * np.pi, size=4), size * 0.004)
+    V, I, N, UV, ALB = [], [], [], [], []
+    objects = []
+    nv = 0
+    ntri = 0
+    # terrain tiles: one planar chart each
+    edges = np.linspace(0, cells, terrain_tiles + 1).round().astype(int)
+    gutter = 2.0 / lightmap_size
+    for ty in range(terrain_tiles):
+        for tx in range(terrain_tiles):
+            c0, c1 = edges[tx], edges[tx + 1]
+            r0, r1 = edges[ty], edges[ty + 1]
+            nu, nvv = c1 - c0, r1 - r0
+            xs = np.arange(c0, c1 + 1) * cell_m
+            ys = np.arange(r0, r1 + 1) * cell_m
+            X, Y = np.meshgrid(xs, ys, indexing="ij")
+            Z = _terrain_height(tp, X, Y)
+            v = np.stack([X, Y, Z], -1).reshape(-1, 3)
+            t = _grid_tris(nu, nvv)
+            st = np.stack([(X - xs[0]) / (xs[-1] - xs[0]), (Y - ys[0]) / (ys[-1] - ys[0])], -1).reshape(-1, 2)
+            uvv = gutter + st * (1 - 2 * gutter)
+            albedo = rng.uniform(0.25, 0.45, size=3).astype(np.float32)
+            V.append(v); I.append(t + nv); N.append(_grid_normals(v, t)); UV.append(uvv[t].reshape(-1, 6))
+            ALB.append(np.repeat(albedo[None, :], len(t), 0))
+            objects.append(LightmapObject(f"terrain{ty}_{tx}", ntri, len(t), lightmap_size, lightmap_size,
+                                          tuple(float(a) for a in albedo)))
+            nv += len(v); ntri += len(t)
+    # city: jittered grid of boxes over the central 85% of the terrain
+    side = int(np.ceil(np.sqrt(n_boxes)))
+    pitch = size * 0.85 / side
+    off = size * 0.075
+    fp = rng.uniform(0.3, 0.75, size=(n_boxes, 2)) * pitch
+    hgt = np.exp(rng.uniform(np.log(6.0), np.log(60.0), size=n_boxes))
+    areas = 2 * fp[:, 0] * fp[:, 1] / 2 + 2 * hgt * (fp[:, 0] + fp[:, 1])
+    cpm = np.sqrt(box_tris / 2 / areas.sum())
+    per_map = int(np.ceil(n_boxes / city_maps))
+    for m in range(city_maps):
+        charts = []
+        for b in range(m * per_map, min(n_boxes, (m + 1) * per_map)):
+            gx, gy = b % side, b // side
+            cx = off + (gx + 0.5) * pitch + rng.uniform(-0.1, 0.1) * pitch
+            cy = off + (gy + 0.5) * pitch + rng.uniform(-0.1, 0.1) * pitch
+            lo = np.array([cx - fp[b, 0] / 2, cy - fp[b, 1] / 2, 0.0])
+            hi = np.array([cx + fp[b, 0] / 2, cy + fp[b, 1] / 2, 0.0])
+            corners = _terrain_height(tp, np.array([lo[0], hi[0], lo[0], hi[0]]), np.array([lo[1], lo[1], hi[1], hi[1]]))
+            lo[2] = corners.min() - 0.5
+            hi[2] = corners.max() + hgt[b]
+            for k, (o, du, dv) in enumerate(_box_faces(lo, hi, inward=False)):
+                if k == 0:
+                    continue                       # no floor face
+                w, h = np.linalg.norm(du), np.linalg.norm(dv)
+                v, t, st = _grid_face(o, du, dv, max(1, int(round(w * cpm))), max(1, int(round(h * cpm))))
+                charts.append((v, t, st, (w, h)))
+        if not charts:
+            continue
+        v, t, n, uv = _chart_object(charts, lightmap_size, gutter_texels=2.0)
+        albedo = rng.uniform(0.3, 0.8, size=3).astype(np.float32)
+        V.append(v); I.append(t + nv); N.append(n); UV.append(uv)
+        ALB.append(np.repeat(albedo[None, :], len(t), 0))
+        objects.append(LightmapObject(f"city{m}", ntri, len(t), lightmap_size, lightmap_size,
+                                      tuple(float(a) for a in albedo)))
+        nv += len(v); ntri += len(t)
+    d = np.asarray(sun_direction, np.float64)
+    d /= np.linalg.norm(d)
+    lights = [dict(type=1, position=(0, 0, 0), direction=tuple(d), color=(3.0, 2.9, 2.6), cast_shadows=1)]
+    nt = sum(len(x) for x in I)
+    return BakeScene(verts=np.concatenate(V).astype(np.float32), idx=np.concatenate(I).astype(np.uint32),
+                     normals=np.concatenate(N).astype(np.float32), tri_uv=np.concatenate(UV).astype(np.float32),
+                     side_bits=np.full(nt, 1 | 4, np.uint8), albedo=np.concatenate(ALB).astype(np.float32),
+                     emittance=np.zeros((nt, 3), np.float32), objects=objects, lights=lights)
+
+
+def sun_sweep(step: int, steps: int = 60):
+    """Sun direction of step `step` of a sweep: azimuth once around, elevation 15..75 deg."""
+    f = step / max(steps - 1, 1)
+    az = 2 * np.pi * f
+    el = np.radians(15 + 60 * np.sin(np.pi * f))
+    d = np.array([np.cos(el) * np.cos(az), np.cos(el) * np.sin(az), -np.sin(el)])
+    return tuple(d / np.linalg.norm(d))
