@@ -296,3 +296,21 @@ def test_mlp_oracle_finite_differences(oracle):
         # y comes back in float32, so the head-term difference quotient carries ~1e-4 noise
         assert abs(num1 - (g1[i] - g0[i])) <= 2e-3 * max(1.0, abs(g1[i] - g0[i])), (i, num1, g1[i] - g0[i])
     assert checked >= 70
+
+
+def test_city_terrain_scene_layout():
+    """Config-5 generator at a small size: terrain tiles + city lightmaps, one chart per
+    terrain tile spanning the map, uv inside [0,1], watertight heightfield grid."""
+    from lightsprintsdk_amd import scenes as S
+    sc = S.city_terrain(seed=3, n_boxes=40, grid_vertices=65, terrain_tiles=2, lightmap_size=128, city_maps=2,
+                        box_tris=4000, sun_direction=S.sun_sweep(10))
+    assert len(sc.objects) == 4 + 2
+    assert sum(o.tri_count for o in sc.objects) == sc.ntri
+    assert sc.objects[0].tri_count + sc.objects[1].tri_count + sc.objects[2].tri_count + sc.objects[3].tri_count == 2 * 64 * 64
+    assert np.all(sc.tri_uv >= 0) and np.all(sc.tri_uv <= 1)
+    assert np.all(np.isfinite(sc.verts)) and sc.idx.max() < len(sc.verts)
+    d = np.array(sc.lights[0]["direction"])
+    assert sc.lights[0]["type"] == 1 and abs(np.linalg.norm(d) - 1) < 1e-6 and d[2] < 0
+    # the sweep: unit vectors, elevation 15..75 degrees, one turn of azimuth
+    el = [np.degrees(np.arcsin(-S.sun_sweep(k)[2])) for k in range(60)]
+    assert min(el) >= 15 - 1e-6 and max(el) <= 75 + 1e-6

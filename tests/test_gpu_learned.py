@@ -104,3 +104,24 @@ def test_learned_gather_heldout(gpu):
     assert r_net < 0.8 * r_cheap
     assert r_final <= 0.02
     assert frac < 0.5
+
+
+def test_committed_surrogates_heldout(gpu):
+    """The committed surrogates on a held-out scene of their family each (the box room
+    and a 1/16-area city): learned+fallback within 2% RMSE of the plain bake, and the
+    network alone better than the cheap gather."""
+    from lightsprintsdk_amd import learned as Lr
+    for family, seed in (("box_room", 2004), ("city", 6001)):
+        got = Lr.load_weights(Lr.weights_path(family))
+        assert got is not None, f"no committed surrogate for {family}"
+        net, bound, meta = got
+        ls = Lr.prepare_eval(seed, family)
+        ls.cheap_only()
+        r_cheap = Lr.rmse_rel(ls.final_rgb(), ls.ref_final)
+        r_net, _ = Lr.evaluate(ls, net, tau=float("inf"))
+        r, f = Lr.evaluate(ls, net, target=bound["target"])
+        print(f"{family} {seed}: {ls.n} texels, cheap {r_cheap:.4f} network {r_net:.4f} learned+fallback {r:.4f} "
+              f"fallback {f:.3f}", flush=True)
+        assert r <= 0.02 and r_net < r_cheap
+        ls.close()
+        del ls
