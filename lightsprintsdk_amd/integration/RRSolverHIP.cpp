@@ -107,7 +107,7 @@ bool RRSolverHIP::buildScene(const UpdateParameters& params)
 		g.cast_shadows = L->castShadows ? 1 : 0;
 		lights.push_back(g);
 	}
-	lsg_build_params bp = { 4, 0 };
+	lsg_build_params bp = { 0, 0 };   // library default leaf size
 	if (lsg_collider_create(verts.data(), 3 * nt, idx.data(), nt, &bp, nullptr, &col) != LSG_OK)
 	{
 		RRReporter::report(ERRO, "RRSolverHIP: %s\n", lsg_last_error());
