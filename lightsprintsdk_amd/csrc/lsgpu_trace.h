@@ -25,17 +25,22 @@ struct NoFilter {
     __device__ float clamp(float best) const { return best; }
 };
 
-__device__ inline void slab2(const Node& nd, const float3 o, const float3 idir, float tlo, float thi,
+// Child-box slab tests of one node.  noi = -o * idir per ray, so each plane distance
+// is one fma: (b - o) * idir = b * idir - o * idir.  The fused form differs from the
+// two-step one by about |o * idir| ulp, i.e. |o| * 6e-8 in distance, while every child
+// box is inflated by DELTA_BSP = max|coordinate| * 1e-5; culling stays conservative and
+// the hit, decided only by the triangle test, is unchanged.
+__device__ inline void slab2(const Node& nd, const float3 noi, const float3 idir, float tlo, float thi,
                              float& n0, float& f0, float& n1, float& f1)
 {
     float a, b, ta, tb, tc, td, te, tf;
-    a = (nd.q0.x - o.x) * idir.x; b = (nd.q0.y - o.x) * idir.x; ta = fminf(a, b); tb = fmaxf(a, b);
-    a = (nd.q0.z - o.y) * idir.y; b = (nd.q0.w - o.y) * idir.y; tc = fminf(a, b); td = fmaxf(a, b);
-    a = (nd.q1.x - o.z) * idir.z; b = (nd.q1.y - o.z) * idir.z; te = fminf(a, b); tf = fmaxf(a, b);
+    a = __fmaf_rn(nd.q0.x, idir.x, noi.x); b = __fmaf_rn(nd.q0.y, idir.x, noi.x); ta = fminf(a, b); tb = fmaxf(a, b);
+    a = __fmaf_rn(nd.q0.z, idir.y, noi.y); b = __fmaf_rn(nd.q0.w, idir.y, noi.y); tc = fminf(a, b); td = fmaxf(a, b);
+    a = __fmaf_rn(nd.q1.x, idir.z, noi.z); b = __fmaf_rn(nd.q1.y, idir.z, noi.z); te = fminf(a, b); tf = fmaxf(a, b);
     n0 = fmaxf(fmaxf(ta, tc), fmaxf(te, tlo)); f0 = fminf(fminf(tb, td), fminf(tf, thi));
-    a = (nd.q1.z - o.x) * idir.x; b = (nd.q1.w - o.x) * idir.x; ta = fminf(a, b); tb = fmaxf(a, b);
-    a = (nd.q2.x - o.y) * idir.y; b = (nd.q2.y - o.y) * idir.y; tc = fminf(a, b); td = fmaxf(a, b);
-    a = (nd.q2.z - o.z) * idir.z; b = (nd.q2.w - o.z) * idir.z; te = fminf(a, b); tf = fmaxf(a, b);
+    a = __fmaf_rn(nd.q1.z, idir.x, noi.x); b = __fmaf_rn(nd.q1.w, idir.x, noi.x); ta = fminf(a, b); tb = fmaxf(a, b);
+    a = __fmaf_rn(nd.q2.x, idir.y, noi.y); b = __fmaf_rn(nd.q2.y, idir.y, noi.y); tc = fminf(a, b); td = fmaxf(a, b);
+    a = __fmaf_rn(nd.q2.z, idir.z, noi.z); b = __fmaf_rn(nd.q2.w, idir.z, noi.z); te = fminf(a, b); tf = fmaxf(a, b);
     n1 = fmaxf(fmaxf(ta, tc), fmaxf(te, tlo)); f1 = fminf(fminf(tb, td), fminf(tf, thi));
 }
 
@@ -67,6 +72,7 @@ __device__ inline bool trace_closest(const Filter& flt, const ColliderView& c, c
     float lo, hi;
     if (!c.nvalid || !ref_box_intersect(c.bmin, c.bmax, o, d, rmin, rmax, lo, hi)) return true;
     const float3 idir = safe_inv(d);
+    const float3 noi = make_float3(-o.x * idir.x, -o.y * idir.y, -o.z * idir.z);
     float best = hi, bound = hi;
     int bi = -1; float bu = 0.f, bv = 0.f; bool bf = false; uint32_t bslot = 0;
     int32_t ref = c.root;          // next node: >= 0 inner, < 0 leaf, kDone finished
@@ -80,7 +86,7 @@ __device__ inline bool trace_closest(const Filter& flt, const ColliderView& c, c
             if (STATS) nvis++;
             const Node nd = c.nodes[ref];
             float n0, f0, n1, f1;
-            slab2(nd, o, idir, lo, bound, n0, f0, n1, f1);
+            slab2(nd, noi, idir, lo, bound, n0, f0, n1, f1);
             const int32_t r0 = __float_as_int(nd.q3.x), r1 = __float_as_int(nd.q3.y);
             const bool h0 = n0 <= f0, h1 = n1 <= f1;
             if (h0 && h1) {
