@@ -352,15 +352,16 @@ int  lsg_learned_features(lsg_scene* scene, const lsg_texel* texels, uint32_t n,
  * *nfallback (device counter, caller zeroes) */
 int  lsg_learned_decide(const float* y, const float* x, const lsg_texel* texels, const float* parts, uint32_t n,
                         float scale, float tau, float* out_rgba, lsg_texel* fallback, uint32_t* nfallback, void* stream);
-/* Per-scene error bound.  cheap2_rgba: a second cheap gather of the same texels with
- * another seed (device float4 per output slot, as lsg_gather_device writes it).  From
- * (pred - c2)^2 - (c1 - c2)^2 / 2, an unbiased estimate of the network's squared error,
- * summed per predicted-error bin, picks the largest tau whose kept texels stay within
- * target_rmse (relative to the RMS prediction).  work: device uint64[513].  *est_rmse
- * (optional) = estimated relative RMSE with no fallback.  Synchronous. */
+/* Per-scene error bound.  cheap2_rgba: a second cheap gather with another seed of the
+ * texels 0, sample_stride, 2*sample_stride, ... (device float4 per output slot, as
+ * lsg_gather_device writes it).  From (pred - c2)^2 - (c1 - c2)^2 / 2, an unbiased
+ * estimate of the network's squared error, summed per predicted-error bin over those
+ * texels, picks the largest tau whose kept texels stay within target_rmse (relative to
+ * the RMS prediction).  work: device uint64[513].  *est_rmse (optional) = estimated
+ * relative RMSE with no fallback.  Synchronous. */
 int  lsg_learned_bound(const float* y, const float* x, const lsg_texel* texels, const float* parts,
-                       const float* cheap2_rgba, uint32_t n, float scale, float target_rmse, uint64_t* work,
-                       float* tau, float* est_rmse, void* stream);
+                       const float* cheap2_rgba, uint32_t n, uint32_t sample_stride, float scale, float target_rmse,
+                       uint64_t* work, float* tau, float* est_rmse, void* stream);
 
 #ifdef __cplusplus
 }

@@ -249,20 +249,10 @@ __global__ __launch_bounds__(256) void k_decide(const float* __restrict__ y, con
     if (fb) fallback[base + (uint32_t)__popcll(mask & (lane ? (~0ull >> (64 - lane)) : 0ull))] = T;
 }
 
-// Unbiased per-scene estimate of the surrogate's squared error from a second,
-// independent cheap gather c2 (same rays per texel, another seed): with c1 the cheap
-// gather the features came from and E the expected irradiance,
-//   E[(pred - c2)^2] = (pred - E)^2 + var(c2),   E[(c1 - c2)^2] / 2 = var(c),
-// so (pred - c2)^2 - (c1 - c2)^2 / 2 estimates (pred - E)^2 without the plain gather.
-// Per score bin: the sum of that estimate (fixed point, units s^2 * 2^-24, signed)
-// and the texel count; hist[2*256] = sum of pred^2 over reliable texels (same units).
-__global__ __launch_bounds__(256) void k_error_hist(const float* __restrict__ y, const float* __restrict__ x,
-                                                    const lsg_texel* __restrict__ texels, const float4* __restrict__ parts,
-                                                    const float4* __restrict__ c2img, uint32_t n, float inv_s,
-                                                    unsigned long long* __restrict__ hist)
+__device__ inline void texel_error(const float* __restrict__ y, const float* __restrict__ x,
+                                   const lsg_texel* __restrict__ texels, const float4* __restrict__ parts,
+                                   const float4* __restrict__ c2img, uint64_t i, float inv_s, unsigned long long* hs)
 {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
     const float4 a = parts[2 * (size_t)i], b = parts[2 * (size_t)i + 1];
     if (!(a.w > 0 || b.w > 0)) return;                   // always re-gathered
     float e[3];
@@ -280,9 +270,33 @@ __global__ __launch_bounds__(256) void k_error_hist(const float* __restrict__ y,
     const float est = (d - 0.5f * v) * (1.f / 3.f), pw = p * (1.f / 3.f);
     const int bin = score_bin(score);
     const double q = 16777216.0;
-    atomicAdd(&hist[bin], (unsigned long long)(long long)llrint((double)est * q));
-    atomicAdd(&hist[kHistBins + bin], 1ull);
-    atomicAdd(&hist[2 * kHistBins], (unsigned long long)(long long)llrint((double)pw * q));
+    atomicAdd(&hs[bin], (unsigned long long)(long long)llrint((double)est * q));
+    atomicAdd(&hs[kHistBins + bin], 1ull);
+    atomicAdd(&hs[2 * kHistBins], (unsigned long long)(long long)llrint((double)pw * q));
+}
+
+// Unbiased per-scene estimate of the surrogate's squared error from a second,
+// independent cheap gather c2 (same rays per texel, another seed): with c1 the cheap
+// gather the features came from and E the expected irradiance,
+//   E[(pred - c2)^2] = (pred - E)^2 + var(c2),   E[(c1 - c2)^2] / 2 = var(c),
+// so (pred - c2)^2 - (c1 - c2)^2 / 2 estimates (pred - E)^2 without the plain gather.
+// Per score bin: the sum of that estimate (fixed point, units s^2 * 2^-24, signed)
+// and the texel count; hist[2*256] = sum of pred^2 over reliable texels (same units).
+// Over the texels i = 0, stride, 2*stride, ... (c2 exists for those); per-block sums
+// in LDS, one global atomic per non-empty bin and block.
+__global__ __launch_bounds__(256) void k_error_hist(const float* __restrict__ y, const float* __restrict__ x,
+                                                    const lsg_texel* __restrict__ texels, const float4* __restrict__ parts,
+                                                    const float4* __restrict__ c2img, uint32_t n, uint32_t stride,
+                                                    float inv_s, unsigned long long* __restrict__ hist)
+{
+    __shared__ unsigned long long hs[2 * kHistBins + 1];
+    for (uint32_t k = threadIdx.x; k < 2 * kHistBins + 1; k += 256) hs[k] = 0ull;
+    __syncthreads();
+    const uint64_t i = ((uint64_t)blockIdx.x * 256 + threadIdx.x) * stride;
+    if (i < n) texel_error(y, x, texels, parts, c2img, i, inv_s, hs);
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < 2 * kHistBins + 1; k += 256)
+        if (hs[k]) atomicAdd(&hist[k], hs[k]);
 }
 
 __global__ void k_zero_u64(unsigned long long* p, uint32_t n)
@@ -375,9 +389,11 @@ int lsg_learned_decide(const float* y, const float* x, const lsg_texel* texels, 
 // Largest bound tau (a score-bin edge) whose kept texels' estimated squared error
 // stays within target_rmse^2 of the mean squared prediction: the per-scene error bound.
 int lsg_learned_bound(const float* y, const float* x, const lsg_texel* texels, const float* parts, const float* cheap2_rgba,
-                      uint32_t n, float scale, float target_rmse, uint64_t* work, float* tau, float* est_rmse, void* stream)
+                      uint32_t n, uint32_t sample_stride, float scale, float target_rmse, uint64_t* work, float* tau,
+                      float* est_rmse, void* stream)
 {
     if (!tau || (n && (!y || !x || !texels || !parts || !cheap2_rgba || !work))) return fail(LSG_ERR_INVALID, "lsg_learned_bound: null argument");
+    if (!sample_stride) return fail(LSG_ERR_INVALID, "lsg_learned_bound: sample_stride must be >= 1");
     if (!(scale > 0) || !(target_rmse >= 0)) return fail(LSG_ERR_INVALID, "lsg_learned_bound: scale > 0 and target_rmse >= 0 required");
     *tau = 0.f;
     if (est_rmse) *est_rmse = 0.f;
@@ -385,8 +401,9 @@ int lsg_learned_bound(const float* y, const float* x, const lsg_texel* texels, c
     hipStream_t st = (hipStream_t)stream;
     const uint32_t nh = 2 * kHistBins + 1;
     k_zero_u64<<<nb(nh), 256, 0, st>>>((unsigned long long*)work, nh);
-    k_error_hist<<<nb(n), 256, 0, st>>>(y, x, texels, (const float4*)parts, (const float4*)cheap2_rgba, n, 1.f / scale,
-                                        (unsigned long long*)work);
+    const uint64_t ns = ((uint64_t)n + sample_stride - 1) / sample_stride;
+    k_error_hist<<<nb(ns), 256, 0, st>>>(y, x, texels, (const float4*)parts, (const float4*)cheap2_rgba, n, sample_stride,
+                                         1.f / scale, (unsigned long long*)work);
     LSG_HIP(hipGetLastError());
     std::vector<long long> h(nh);
     LSG_HIP(hipMemcpyAsync(h.data(), work, 8 * nh, hipMemcpyDeviceToHost, st));

@@ -28,6 +28,7 @@ from .collider import DeviceBuffer
 
 CHEAP_QUALITY = 16
 CHEAP_LIGHT_ROUNDS = 0        # 0: the reference's quality/10+1 rounds (2 at quality 16)
+BOUND_SAMPLE = 4              # the error bound uses every 4th texel's second cheap gather
 LAMBDA = 0.5
 NFEAT = 32
 FEATURE_VERSION = 3      # 32 features, outputs relative to the baseline below
@@ -165,6 +166,7 @@ class LearnedScene:
         self.d_work = DeviceBuffer(48 * max(self.sb.nout, 1))
         self.d_cheap2 = DeviceBuffer(16 * max(self.sb.nout, 1))
         self.d_hist = DeviceBuffer(8 * 513)
+        self.d_tex_sample = None
         self.d_y = DeviceBuffer(16 * max(n, 1))
         self.d_fb = DeviceBuffer(16 * max(n, 1))
         self.d_nfb = DeviceBuffer(16)
@@ -237,12 +239,15 @@ class LearnedScene:
         L = N.lib()
         p = self.cheap.c()
         p.seed = (self.cheap.seed * 2654435761 + 0x3C6EF372) & 0xFFFFFFFF
-        N.check(L.lsg_gather_device(self.sb.gs._h, self.sb.d_tex.ptr, self.n, self.sb.d_sub.ptr, len(self.sb.subs),
+        if self.d_tex_sample is None:
+            self.d_tex_sample = DeviceBuffer.from_array(np.ascontiguousarray(self.sb.texels[::BOUND_SAMPLE]))
+        ns = (self.n + BOUND_SAMPLE - 1) // BOUND_SAMPLE
+        N.check(L.lsg_gather_device(self.sb.gs._h, self.d_tex_sample.ptr, ns, self.sb.d_sub.ptr, len(self.sb.subs),
                                     ctypes.byref(p), self.d_cheap2.ptr, stream), "second cheap gather")
         tau = ctypes.c_float(); est = ctypes.c_float()
         N.check(L.lsg_learned_bound(self.d_y.ptr, self.d_x.ptr, self.sb.d_tex.ptr, self.d_parts.ptr, self.d_cheap2.ptr,
-                                    self.n, self.scale, target, self.d_hist.ptr, ctypes.byref(tau), ctypes.byref(est),
-                                    stream), "bound")
+                                    self.n, BOUND_SAMPLE, self.scale, target, self.d_hist.ptr, ctypes.byref(tau),
+                                    ctypes.byref(est), stream), "bound")
         self.last_est_rmse = float(est.value)
         return float(tau.value)
 

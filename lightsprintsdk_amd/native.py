@@ -122,7 +122,7 @@ SIGNATURES = {
     "lsg_learned_scale": (ctypes.c_int, [P, U32, ctypes.POINTER(ctypes.c_float)]),
     "lsg_learned_features": (ctypes.c_int, [P, P, U32, P, P, P, ctypes.c_float, P, U32, U32, P, P, P]),
     "lsg_learned_decide": (ctypes.c_int, [P, P, P, P, U32, ctypes.c_float, ctypes.c_float, P, P, P, P]),
-    "lsg_learned_bound": (ctypes.c_int, [P, P, P, P, P, U32, ctypes.c_float, ctypes.c_float, P,
+    "lsg_learned_bound": (ctypes.c_int, [P, P, P, P, P, U32, U32, ctypes.c_float, ctypes.c_float, P,
                                          ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), P]),
     "lsg_rasterize_texels": (ctypes.c_int, [P, P, U32, U32, U32, U32, P, ctypes.POINTER(U32), P, ctypes.POINTER(U32), U32]),
 }
