@@ -83,7 +83,7 @@ int lsg_event_elapsed_ms(void* start, void* stop, float* ms);
  * Triangles whose SRLNP precompute is degenerate are left out, as the reference's
  * build does (IntersectBsp.h:252-262, IntersectBspFast.cpp:775-780). */
 typedef struct lsg_build_params {
-    uint32_t leaf_size;        /* max triangles per leaf, 1..16 (0 = default 2) */
+    uint32_t leaf_size;        /* max triangles per leaf, 1..16; 0 = 2, or 4/8 when the tree would be deeper than 31 */
     uint32_t flags;            /* reserved, 0 */
 } lsg_build_params;
 

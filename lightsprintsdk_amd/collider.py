@@ -115,7 +115,7 @@ class RRCollider:
 
     @staticmethod
     def create(verts: np.ndarray, idx: np.ndarray, technique: int = IT_HIP_LBVH, aborting=None,
-               cacheLocation=None, buildParams=None, leaf_size: int = 2):
+               cacheLocation=None, buildParams=None, leaf_size: int = 0):
         """RRCollider::create for a mesh given as float32 [nv,3] / uint32 [nt,3]."""
         v = np.ascontiguousarray(verts, dtype=np.float32)
         i = np.ascontiguousarray(idx, dtype=np.uint32)

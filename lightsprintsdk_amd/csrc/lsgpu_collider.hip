@@ -524,12 +524,22 @@ int lsg_collider_create(const float* verts, uint32_t nverts, const uint32_t* ind
     if (ntri >= (1u << 27)) return fail(LSG_ERR_LIMIT, "lsg_collider_create: %u triangles exceeds 2^27", ntri);
     for (uint64_t i = 0; i < 3ull * ntri; i++)
         if (indices[i] >= nverts) return fail(LSG_ERR_INVALID, "lsg_collider_create: index %u out of range at %llu", indices[i], (unsigned long long)i);
-    uint32_t leaf = params && params->leaf_size ? params->leaf_size : 2;   // 2: fastest gather and closest hit measured
+    const bool chosen = params && params->leaf_size;
+    uint32_t leaf = chosen ? params->leaf_size : 2;
     if (leaf < 1 || leaf > 16) return fail(LSG_ERR_INVALID, "lsg_collider_create: leaf_size %u not in 1..16", leaf);
     if (!device_ready()) return fail(LSG_ERR_NO_DEVICE, "lsg_collider_create: no gfx950 device");
     lsg_collider* c = new (std::nothrow) lsg_collider;
     if (!c) return fail(LSG_ERR_OOM, "lsg_collider_create: host allocation failed");
     int st = build(c->impl, verts, nverts, indices, ntri, leaf, aborting);
+    // Default leaf size: 2 triangles per leaf traverses fastest while the tree fits the
+    // 32-entry traversal stack; a deeper tree would need the 64-entry stack, whose LDS
+    // halves the waves per CU, so larger leaves are taken instead (up to 8).
+    while (!st && !chosen && c->impl.depth + 1 > 32 && leaf < 8) {
+        free_impl(c->impl);
+        c->impl = ColliderImpl();
+        leaf *= 2;
+        st = build(c->impl, verts, nverts, indices, ntri, leaf, aborting);
+    }
     if (st) { free_impl(c->impl); delete c; return st; }
     *out = c;
     return LSG_OK;

@@ -26,7 +26,7 @@ class RRColliderHIP : public rr::RRCollider
 public:
 	// Builds the LBVH on the GPU from the mesh; nullptr when the device is
 	// missing or the build fails (the error text is in lsg_last_error()).
-	static RRColliderHIP* create(const rr::RRMesh* mesh, bool& aborting, unsigned leafSize = 2);
+	static RRColliderHIP* create(const rr::RRMesh* mesh, bool& aborting, unsigned leafSize = 0);   // 0: library default
 	~RRColliderHIP() override;
 
 	// RRCollider virtuals (RRCollider.h:190-267)
@@ -49,7 +49,7 @@ private:
 	void fill(rr::RRRay& ray, const lsg_host::Hit& h) const;
 	const rr::RRMesh* mesh;
 	lsg_collider* col = nullptr;
-	unsigned leaf = 2;
+	unsigned leaf = 0;
 	lsg_host::Tree tree;
 };
 
