@@ -148,6 +148,76 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
     p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
+// Inference in one launch: the whole network per 16-row tile, one tile per wave at a
+// time, four waves per workgroup, kInferTiles tiles per wave (the weights are loaded
+// into LDS once per workgroup).  The weights (transposed, BF16) and biases sit in LDS; each
+// wave's activations ping-pong between two LDS tiles.  The products are the same MFMA
+// calls in the same order as k_gemm_bf16's (same lane layout, same K steps), and bias,
+// ReLU and BF16 rounding are k_bias_act's, so the output equals the layered forward
+// pass bit for bit.
+constexpr int kWElems = kFeat * 64 + 3 * 64 * 64 + 64 * 16;
+constexpr int kBElems = 4 * 64 + 16;
+constexpr uint32_t kInferTiles = 8;                       // 16-row tiles per wave -> 512 rows per workgroup
+struct MlpOffsets { uint32_t w[kLayers]; uint32_t b[kLayers]; };
+
+__global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ x, uint32_t n, const uint16_t* __restrict__ wb,
+                                                   const float* __restrict__ p, MlpOffsets off, float* __restrict__ y)
+{
+    constexpr int kInL[kLayers] = {kFeat, 64, 64, 64, 64};
+    constexpr int kOutL[kLayers] = {64, 64, 64, 64, 16};
+    __shared__ uint16_t wt[kWElems];                 // layer l at tofs: W_l transposed, [out][in]
+    __shared__ float bias[kBElems];
+    __shared__ uint16_t act[4][2][16 * 64];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    int tofs = 0, bofs = 0;
+    for (int l = 0; l < kLayers; l++) {
+        const int in = kInL[l], out = kOutL[l];
+        for (int e = tid; e < in * out; e += 256) {
+            const int k = e / out, c = e % out;
+            wt[tofs + c * in + k] = wb[off.w[l] + e];
+        }
+        for (int c = tid; c < out; c += 256) bias[bofs + c] = p[off.b[l] + c];
+        tofs += in * out; bofs += out;
+    }
+    const int r = lane & 15, kq = (lane >> 4) * 4;
+    for (uint32_t it = 0; it < kInferTiles; it++) {
+    const uint64_t row0 = ((uint64_t)(blockIdx.x * kInferTiles + it) * 4u + (uint32_t)wave) * 16u;
+    uint16_t* A = act[wave][0];
+    uint16_t* B = act[wave][1];
+    for (int e = lane; e < 16 * kFeat; e += 64) {
+        const uint32_t rr = (uint32_t)(e / kFeat), k = (uint32_t)(e % kFeat);
+        A[e] = f2bf(row0 + rr < n ? x[(size_t)(row0 + rr) * kFeat + k] : 0.f);
+    }
+    __syncthreads();
+    tofs = 0; bofs = 0;
+    for (int l = 0; l < kLayers; l++) {
+        const int in = kInL[l], out = kOutL[l];
+        const uint16_t* W = wt + tofs;
+        for (int tile = 0; tile < out / 16; tile++) {
+            v4f acc = {0.f, 0.f, 0.f, 0.f};
+            for (int k0 = 0; k0 < in; k0 += 16) {
+                v4s a, b;
+                for (int i = 0; i < 4; i++) {
+                    a[i] = (short)A[r * in + k0 + kq + i];
+                    b[i] = (short)W[(tile * 16 + r) * in + k0 + kq + i];
+                }
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+            }
+            const int col = tile * 16 + r;
+            const float bb = bias[bofs + col];
+            for (int i = 0; i < 4; i++) {
+                const float z = acc[i] + bb;
+                if (l < kLayers - 1) B[(kq + i) * out + col] = f2bf(z > 0.f ? z : 0.f);
+                else if (col < kOutUsed && row0 + kq + i < n) y[(size_t)(row0 + kq + i) * kOutUsed + col] = z;
+            }
+        }
+        __syncthreads();
+        uint16_t* t = A; A = B; B = t;
+        tofs += in * out; bofs += out;
+    }
+    }
+}
+
 static inline unsigned nb(uint64_t n) { return (unsigned)((n + 255) / 256); }
 
 struct MlpImpl {
@@ -267,14 +337,10 @@ int lsg_mlp_forward(lsg_mlp* m, const float* x, uint32_t n, float* y, void* stre
     if (!m || (n && (!x || !y))) return fail(LSG_ERR_INVALID, "lsg_mlp_forward: null argument");
     MlpImpl& M = m->impl;
     hipStream_t s = (hipStream_t)stream;
-    for (uint32_t b0 = 0; b0 < n; b0 += M.max_batch) {
-        const uint32_t rows = std::min(M.max_batch, n - b0);
-        const uint32_t pad = (rows + 15) / 16 * 16;
-        if (pad != rows) LSG_HIP(hipMemsetAsync(M.d_a[0], 0, 2 * (size_t)pad * kFeat, s));
-        k_to_bf16<<<nb((uint64_t)rows * kFeat), 256, 0, s>>>(x + (size_t)b0 * kFeat, M.d_a[0], rows * kFeat);
-        forward(M, pad, s);
-        LSG_HIP(hipMemcpy2DAsync(y + (size_t)b0 * 4, 16, M.d_z[kLayers - 1], 64, 16, rows, hipMemcpyDeviceToDevice, s));
-    }
+    if (!n) return LSG_OK;
+    MlpOffsets off;
+    for (int l = 0; l < kLayers; l++) { off.w[l] = (uint32_t)M.wofs[l]; off.b[l] = (uint32_t)M.bofs[l]; }
+    k_mlp_infer<<<(unsigned)(((uint64_t)n + 64 * kInferTiles - 1) / (64 * kInferTiles)), 256, 0, s>>>(x, n, M.d_wb, M.d_p, off, y);
     LSG_HIP(hipGetLastError());
     return LSG_OK;
 }

@@ -125,3 +125,21 @@ def test_committed_surrogates_heldout(gpu):
         assert r <= 0.02 and r_net < r_cheap
         ls.close()
         del ls
+
+
+@pytest.mark.parametrize("n", [1, 15, 16, 100, 513, 70000])
+def test_fused_inference_matches_oracle(gpu, oracle, n):
+    """The one-launch inference kernel (weights in LDS, 16-row tiles per wave) on ragged
+    row counts against the oracle's forward pass with BF16 rounding at the same points."""
+    from lightsprintsdk_amd.learned import Surrogate
+    rng = np.random.default_rng(n)
+    net = Surrogate(max_batch=256, seed=7)
+    p = net.init_params(7)
+    p[-16 * 65:] += (rng.normal(size=16 * 65) * 0.05).astype(np.float32)
+    net.set_params(p)
+    x = rng.normal(size=(n, 32)).astype(np.float32)
+    t = np.zeros((n, 3), np.float32)
+    _, y_ref, _ = oracle.mlp_loss_grad(net.get_params(), x, t, lam=0.5, bf16=True, want_grad=False)
+    y = net.predict(x)
+    assert y.shape == (n, 4) and np.all(np.isfinite(y))
+    np.testing.assert_allclose(y, y_ref, rtol=2e-2, atol=2e-3)
