@@ -313,22 +313,55 @@ __global__ __launch_bounds__(256) void k_write_tris(uint32_t n, const uint32_t* 
 // ---------------------------------------------------------------------------------------
 // traversal
 
+// Closest hit of a device ray batch.  Each wave owns kRaysPerLane * 64 consecutive rays
+// (lane L: rays base + j*64 + L).  A lane whose ray is finished waits until kRefill
+// lanes are free; then they all load their next rays together while the others keep
+// tracing (trace_resume yields) -- incoherent rays end at very different depths, and
+// without refill most lanes of a wave would idle behind the longest ray.
+constexpr uint32_t kRaysPerLane = 4;
+constexpr uint32_t kRefill = 16;
+
 template <int STACK>
 __global__ __launch_bounds__(256) void k_trace_closest(ColliderView c, const float4* __restrict__ ro, const float4* __restrict__ rd,
                                                        uint32_t n, int4* __restrict__ hits, uint8_t* __restrict__ front,
                                                        uint32_t* __restrict__ overflow)
 {
     __shared__ int32_t stk[STACK * 256];
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const float4 a = ro[i], b = rd[i];
-    TraceResult r; uint32_t nv = 0, nt = 0;
-    bool ok = trace_closest<STACK, false>(NoFilter(), c, make_float3(a.x, a.y, a.z), make_float3(b.x, b.y, b.z), a.w, b.w,
-                                          stk + threadIdx.x, 256, r, nv, nt);
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t base = ((uint64_t)blockIdx.x * 4u + (threadIdx.x >> 6)) * (64u * kRaysPerLane) + lane;
+    uint32_t j = 0;                 // next ray slot of this lane
+    bool busy = false, ok = true;
+    uint64_t ray = 0;
+    TraceState S;
+    for (;;) {
+        const bool can = !busy && j < kRaysPerLane && base + 64ull * j < n;
+        const uint64_t want = __ballot(can), bm = __ballot(busy);
+        if (!want && !bm) break;
+        if ((uint32_t)__popcll(want) >= kRefill || !bm) {
+            // start the next ray; rays that miss the scene box are finished at once
+            while (!busy && j < kRaysPerLane && base + 64ull * j < n) {
+                ray = base + 64ull * (j++);
+                const float4 a = ro[ray], b = rd[ray];
+                if (trace_begin(S, c, make_float3(a.x, a.y, a.z), make_float3(b.x, b.y, b.z), a.w, b.w)) busy = true;
+                else {
+                    hits[ray] = make_int4(-1, 0, 0, 0);
+                    if (front) front[ray] = 0;
+                }
+            }
+        }
+        const uint32_t nwait = (uint32_t)__popcll(__ballot(!busy && j < kRaysPerLane && base + 64ull * j < n));
+        if (!busy) continue;
+        const uint32_t ntr = (uint32_t)__popcll(__ballot(1));
+        const uint32_t need = nwait < kRefill ? kRefill - nwait : 1u;
+        const bool more = want != 0 || nwait != 0;
+        const uint32_t yield_at = (more && ntr > need) ? ntr - need : 0u;
+        if (!trace_resume<STACK>(S, NoFilter(), c, stk + threadIdx.x, 256, yield_at, ok)) continue;
+        busy = false;
+        hits[ray] = S.bi >= 0 ? make_int4(S.bi, __float_as_int(S.best), __float_as_int(S.bu), __float_as_int(S.bv))
+                              : make_int4(-1, 0, 0, 0);
+        if (front) front[ray] = (S.bi >= 0 && S.bf) ? 1 : 0;
+    }
     if (!ok && overflow) atomicOr(overflow, 1u);
-    hits[i] = r.tri >= 0 ? make_int4(r.tri, __float_as_int(r.t), __float_as_int(r.u), __float_as_int(r.v))
-                         : make_int4(-1, 0, 0, 0);
-    if (front) front[i] = (r.tri >= 0 && r.front) ? 1 : 0;
 }
 
 template <int STACK>
@@ -599,9 +632,9 @@ int lsg_collider_intersect_device(const lsg_collider* c, const void* ro, const v
     uint32_t* d_over = nullptr;   // stack >= depth+1 by construction; the kernel guard only prevents overrun
     hipStream_t s = (hipStream_t)stream;
     if (stack <= 32)
-        k_trace_closest<32><<<blocks(n), 256, 0, s>>>(C.view(), (const float4*)ro, (const float4*)rd, n, (int4*)hits, (uint8_t*)front, d_over);
+        k_trace_closest<32><<<blocks(n, 256 * kRaysPerLane), 256, 0, s>>>(C.view(), (const float4*)ro, (const float4*)rd, n, (int4*)hits, (uint8_t*)front, d_over);
     else
-        k_trace_closest<64><<<blocks(n), 256, 0, s>>>(C.view(), (const float4*)ro, (const float4*)rd, n, (int4*)hits, (uint8_t*)front, d_over);
+        k_trace_closest<64><<<blocks(n, 256 * kRaysPerLane), 256, 0, s>>>(C.view(), (const float4*)ro, (const float4*)rd, n, (int4*)hits, (uint8_t*)front, d_over);
     LSG_HIP(hipGetLastError());
     return LSG_OK;
 }

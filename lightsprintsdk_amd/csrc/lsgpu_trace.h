@@ -132,4 +132,85 @@ __device__ inline bool trace_closest(const Filter& flt, const ColliderView& c, c
     return ok;
 }
 
+// ---- resumable form ---------------------------------------------------------------------
+// The same traversal with its state kept by the caller, so a lane can leave the loop
+// before its ray is finished and continue later: trace_begin sets a ray up,
+// trace_resume runs it until it is finished (returns true) or until no more than
+// `yield_at` lanes of the wave are still tracing (returns false, state kept).  Used by
+// the closest-hit kernel to refill lanes whose rays have finished.
+struct TraceState {
+    float3 o, d, idir, noi;
+    float lo, best, bound, bu, bv;
+    int32_t ref, leaf, bi;
+    int sp;
+    bool bf;
+};
+
+// false when the ray misses the scene box (then it is finished, no hit)
+__device__ inline bool trace_begin(TraceState& S, const ColliderView& c, const float3 o, const float3 d, float rmin, float rmax)
+{
+    float lo, hi;
+    S.bi = -1;
+    if (!c.nvalid || !ref_box_intersect(c.bmin, c.bmax, o, d, rmin, rmax, lo, hi)) return false;
+    S.o = o; S.d = d;
+    S.idir = safe_inv(d);
+    S.noi = make_float3(-o.x * S.idir.x, -o.y * S.idir.y, -o.z * S.idir.z);
+    S.lo = lo; S.best = hi; S.bound = hi;
+    S.bu = 0.f; S.bv = 0.f; S.bf = false;
+    S.ref = c.root; S.leaf = 0; S.sp = 0;
+    return true;
+}
+
+template <int STACK, class Filter>
+__device__ inline bool trace_resume(TraceState& S, const Filter& flt, const ColliderView& c, int32_t* stk, int stride,
+                                    uint32_t yield_at, bool& ok)
+{
+    constexpr int32_t kDone = 0x7fffffff;
+    auto pop = [&]() -> int32_t { return S.sp ? stk[(--S.sp) * stride] : kDone; };
+    while (S.ref != kDone || S.leaf < 0) {
+        if ((uint32_t)__popcll(__ballot(1)) <= yield_at) return false;     // enough lanes are free
+        while (S.ref >= 0 && S.ref != kDone) {
+            const Node nd = c.nodes[S.ref];
+            float n0, f0, n1, f1;
+            slab2(nd, S.noi, S.idir, S.lo, S.bound, n0, f0, n1, f1);
+            const int32_t r0 = __float_as_int(nd.q3.x), r1 = __float_as_int(nd.q3.y);
+            const bool h0 = n0 <= f0, h1 = n1 <= f1;
+            if (h0 && h1) {
+                const bool swap = n1 < n0;
+                if (S.sp < STACK) stk[(S.sp++) * stride] = swap ? r0 : r1; else ok = false;
+                S.ref = swap ? r1 : r0;
+            } else if (h0) S.ref = r0;
+            else if (h1) S.ref = r1;
+            else S.ref = pop();
+            if (S.ref < 0 && S.leaf == 0) { S.leaf = S.ref; S.ref = pop(); }
+            if (!__any(S.leaf == 0)) break;
+        }
+        if (S.leaf == 0 && S.ref < 0) { S.leaf = S.ref; S.ref = pop(); }
+        while (S.leaf < 0) {
+            const uint32_t st = leaf_start(S.leaf), cnt = leaf_count(S.leaf);
+            bool stop = false;
+            for (uint32_t k = 0; k < cnt; k++) {
+                const uint32_t p = st + k;
+                const float4 pl = c.plane[p];
+                float ndd;
+                const float t = plane_distance(pl, S.o, S.d, ndd);
+                if (!(t >= S.lo && t <= S.bound)) continue;
+                const uint2 mt = c.meta[p];
+                if (!(S.bi < 0 || t < S.best || (int)mt.x < S.bi)) continue;
+                float u, v;
+                if (!srlnp_test(c.sv[p], c.ev[p], (int)mt.y, S.o, S.d, t, u, v)) continue;
+                const bool fr = ndd < 0;
+                if (!flt.accept(mt.x, fr, t)) continue;
+                S.best = t; S.bi = (int)mt.x; S.bu = u; S.bv = v; S.bf = fr;
+                S.bound = flt.clamp(S.best);
+                if (flt.done(t)) { stop = true; break; }
+            }
+            if (stop) { S.ref = kDone; S.leaf = 0; break; }
+            S.leaf = 0;
+            if (S.ref < 0) { S.leaf = S.ref; S.ref = pop(); }
+        }
+    }
+    return true;
+}
+
 } // namespace lsg
