@@ -134,12 +134,56 @@ struct GatherFilter {
     __device__ float clamp(float best) const { return mode == 1 ? fminf(best, rug_below) : best; }
 };
 
+// Shooter s of a series of n: stratified subtexel pick + seeded jitter -> origin o on
+// the subtexel's triangle tri, and the normalised interpolated vertex normal at the
+// subtexel centroid (gather.cpp:680-752).
+__device__ inline void shooter_at(const GatherArgs& A, const lsg_subtexel* ts, uint32_t m, float area_max, uint32_t n,
+                                  uint32_t s, uint32_t key, uint32_t jitter_index, float3& o, float& nx, float& ny,
+                                  float& nz, uint32_t& tri)
+{
+    const float pos = ((float)s + 0.5f) * area_max / (float)n;
+    uint32_t j = 0;
+    float acc = ts[0].area;
+    while (j + 1 < m && acc <= pos) { j++; acc += ts[j].area; }
+    const lsg_subtexel st = ts[j];
+    tri = st.triangle;
+    const uint32_t i0 = A.sc.idx[3 * (size_t)tri], i1 = A.sc.idx[3 * (size_t)tri + 1], i2 = A.sc.idx[3 * (size_t)tri + 2];
+    const float* V = A.sc.verts;
+    const float3 v0 = make_float3(V[3 * (size_t)i0], V[3 * (size_t)i0 + 1], V[3 * (size_t)i0 + 2]);
+    const float3 v1 = make_float3(V[3 * (size_t)i1], V[3 * (size_t)i1 + 1], V[3 * (size_t)i1 + 2]);
+    const float3 v2 = make_float3(V[3 * (size_t)i2], V[3 * (size_t)i2 + 1], V[3 * (size_t)i2 + 2]);
+    // gather.cpp:727-737 basis from interpolated vertex normals at the subtexel centroid
+    const float cxs = (st.uv[0] + st.uv[2] + st.uv[4]) * 0.333333333f;
+    const float cys = (st.uv[1] + st.uv[3] + st.uv[5]) * 0.333333333f;
+    const float w = 1 - cxs - cys;
+    const float* NV = A.sc.vnormal;
+    nx = NV[3 * (size_t)i0] * w + NV[3 * (size_t)i1] * cxs + NV[3 * (size_t)i2] * cys;
+    ny = NV[3 * (size_t)i0 + 1] * w + NV[3 * (size_t)i1 + 1] * cxs + NV[3 * (size_t)i2 + 1] * cys;
+    nz = NV[3 * (size_t)i0 + 2] * w + NV[3 * (size_t)i1 + 2] * cxs + NV[3 * (size_t)i2 + 2] * cys;
+    const float nlen = sqrtf(nx * nx + ny * ny + nz * nz);
+    nx = nx / nlen; ny = ny / nlen; nz = nz / nlen;
+    float ju, jv;
+    jitter(key, jitter_index, ju, jv);
+    const float ux = st.uv[0] + (st.uv[2] - st.uv[0]) * ju + (st.uv[4] - st.uv[0]) * jv;
+    const float uy = st.uv[1] + (st.uv[3] - st.uv[1]) * ju + (st.uv[5] - st.uv[1]) * jv;
+    o = make_float3(v0.x + (v1.x - v0.x) * ux + (v2.x - v0.x) * uy,
+                    v0.y + (v1.y - v0.y) * ux + (v2.y - v0.y) * uy,
+                    v0.z + (v1.z - v0.z) * ux + (v2.z - v0.z) * uy);
+}
+
 // processTexel for texel ti by an aligned group of G lanes of one wavefront (G = 64:
 // the whole wave; 16 or 32 for low ray counts, several texels per wave); lane = the
 // lane in the wave; mystk = this lane's first LDS stack slot, slots `stride` ints
-// apart.  Every group runs its own texel: control flow is uniform within a group.
+// apart; lq = this lane's slot of the group's light queue (2 slots, `stride` apart).
+// Every group runs its own texel: control flow is uniform within a group.
+//
+// Light shots: the shooters that also cast shadow rays (about one in ten at quality
+// 1024, gather.cpp:763,779) are queued in LDS as their chunks pass and traced G at a
+// time, so shadow rays run with full lane masks instead of a few lanes alongside each
+// chunk's hemisphere rays.  A shooter's shadow rays see the same origin, normal and
+// lights as before; only which lane adds them to the sums changes.
 template <int STACK, bool STATS, int G = 64>
-__device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, int32_t* mystk, int stride)
+__device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, int32_t* mystk, uint32_t* lq, int stride)
 {
     static_assert(G == 16 || G == 32 || G == 64, "group width");
     const uint32_t lane = wlane & (uint32_t)(G - 1);               // lane in the group
@@ -177,6 +221,52 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
         const bool allH = !shootL || hemi_rays >= rounds;
         const bool allL = !shootH || rounds >= hemi_rays;
         uint32_t cRelH = 0, cRelL = 0, cUnrelL = 0, cRounds = 0;      // this lane's counts in the series
+        uint32_t* const lq0 = lq - lane;                                // the group's queue
+        uint32_t qhead = 0, qtail = 0;                                  // group-uniform
+        // the shadow rays of queued light shooters qhead .. qhead+cnt-1, one per lane
+        auto light_batch = [&](uint32_t cnt) {
+            __builtin_amdgcn_wave_barrier();
+            if (lane < cnt) {
+                const uint32_t e = qhead + lane;
+                const uint32_t s = lq0[(e & (uint32_t)(G - 1)) + ((e / (uint32_t)G) & 1u) * stride];
+                float3 o; float nx, ny, nz; uint32_t tri;
+                shooter_at(A, ts, m, area_max, n, s, key, shooter_base + s, o, nx, ny, nz, tri);
+                GatherFilter flt{A.sc.verts, A.sc.idx, A.sc.side, tri, near_t, 1, rug_below};
+                for (uint32_t li = 0; li < nl; li++) {
+                    const lsg_light L = A.lights[li];
+                    float dx, dy, dz;
+                    if (L.type == 1) { dx = -L.direction[0]; dy = -L.direction[1]; dz = -L.direction[2]; }
+                    else { dx = L.position[0] - o.x; dy = L.position[1] - o.y; dz = L.position[2] - o.z; }
+                    float dsz = sqrtf(dx * dx + dy * dy + dz * dz);
+                    dx = dx / dsz; dy = dy / dsz; dz = dz / dsz;
+                    if (L.type == 1) dsz = dsz * A.locality;
+                    const float ni = dx * nx + dy * ny + dz * nz;            // gather.cpp:429
+                    if (ni <= 0 || !isfinite(ni)) { cRelL++; continue; }
+                    bool blocked = false; float bt = 0.f;
+                    if (L.cast_shadows) {
+                        TraceResult r;
+                        uint32_t nv_ = 0, nt_ = 0;
+                        trace_closest<STACK, STATS>(flt, A.sc.col, o, make_float3(dx, dy, dz), A.tmin, dsz, mystk, stride, r, nv_, nt_);
+                        if (STATS) { stS[0]++; stS[1] += nv_; stS[2] += nt_; }
+                        blocked = r.tri >= 0; bt = r.t;
+                    }
+                    if (!blocked) {
+                        // RRLight::getIrradiance (RRLight.cpp:345-362), REALISTIC attenuation
+                        float att = 1.f;
+                        if (L.type == 0) {
+                            const float ex = o.x - L.position[0], ey = o.y - L.position[1], ez = o.z - L.position[2];
+                            att = 1 / (0.000000000137289f + (ex * ex + ey * ey + ez * ez));
+                        }
+                        const float kl = att * A.light_mult * ni;
+                        lr += L.color[0] * kl; lg += L.color[1] * kl; lb += L.color[2] * kl;
+                        cRelL++;
+                    } else if (bt < A.rug) cUnrelL++;                        // gather.cpp:500-505
+                    else cRelL++;
+                }
+                cRounds++;
+            }
+            __builtin_amdgcn_wave_barrier();
+        };
         for (uint32_t base = 0; base < n; base += G) {
             const uint32_t s = base + lane;
             const bool active = s < n;
@@ -208,43 +298,23 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
                     got = need;
                 }
             }
-            if (!(doH || doL)) continue;
-            // shooter position (stratified subtexel + seeded jitter) and basis
-            const float pos = ((float)s + 0.5f) * area_max / (float)n;
-            uint32_t j = 0;
-            float acc = ts[0].area;
-            while (j + 1 < m && acc <= pos) { j++; acc += ts[j].area; }
-            const lsg_subtexel st = ts[j];
-            const uint32_t tri = st.triangle;
-            const uint32_t i0 = A.sc.idx[3 * (size_t)tri], i1 = A.sc.idx[3 * (size_t)tri + 1], i2 = A.sc.idx[3 * (size_t)tri + 2];
-            const float* V = A.sc.verts;
-            const float3 v0 = make_float3(V[3 * (size_t)i0], V[3 * (size_t)i0 + 1], V[3 * (size_t)i0 + 2]);
-            const float3 v1 = make_float3(V[3 * (size_t)i1], V[3 * (size_t)i1 + 1], V[3 * (size_t)i1 + 2]);
-            const float3 v2 = make_float3(V[3 * (size_t)i2], V[3 * (size_t)i2 + 1], V[3 * (size_t)i2 + 2]);
-            // gather.cpp:727-737 basis from interpolated vertex normals at the subtexel centroid
-            const float cxs = (st.uv[0] + st.uv[2] + st.uv[4]) * 0.333333333f;
-            const float cys = (st.uv[1] + st.uv[3] + st.uv[5]) * 0.333333333f;
-            const float w = 1 - cxs - cys;
-            const float* NV = A.sc.vnormal;
-            float nx = NV[3 * (size_t)i0] * w + NV[3 * (size_t)i1] * cxs + NV[3 * (size_t)i2] * cys;
-            float ny = NV[3 * (size_t)i0 + 1] * w + NV[3 * (size_t)i1 + 1] * cxs + NV[3 * (size_t)i2 + 1] * cys;
-            float nz = NV[3 * (size_t)i0 + 2] * w + NV[3 * (size_t)i1 + 2] * cxs + NV[3 * (size_t)i2 + 2] * cys;
-            const float nlen = sqrtf(nx * nx + ny * ny + nz * nz);
-            nx = nx / nlen; ny = ny / nlen; nz = nz / nlen;
-            float ox, oy, oz;                                               // RRMathPrivate.h:48-53
-            if (fabsf(nx) > fabsf(nz)) { ox = -ny; oy = nx; oz = 0.f; } else { ox = 0.f; oy = -nz; oz = ny; }
-            const float olen = sqrtf(ox * ox + oy * oy + oz * oz);
-            const float tx = ox / olen, ty = oy / olen, tz = oz / olen;
-            const float bx = ny * tz - nz * ty, by = -nx * tz + nz * tx, bz = nx * ty - ny * tx;
-            float ju, jv;
-            jitter(key, shooter_base + s, ju, jv);
-            const float ux = st.uv[0] + (st.uv[2] - st.uv[0]) * ju + (st.uv[4] - st.uv[0]) * jv;
-            const float uy = st.uv[1] + (st.uv[3] - st.uv[1]) * ju + (st.uv[5] - st.uv[1]) * jv;
-            const float3 o = make_float3(v0.x + (v1.x - v0.x) * ux + (v2.x - v0.x) * uy,
-                                         v0.y + (v1.y - v0.y) * ux + (v2.y - v0.y) * uy,
-                                         v0.z + (v1.z - v0.z) * ux + (v2.z - v0.z) * uy);
-            GatherFilter flt{A.sc.verts, A.sc.idx, A.sc.side, tri, near_t, 0, rug_below};
+            {
+                const uint64_t lm = (__ballot(doL) >> gbase) & gmask;
+                if (doL) {
+                    const uint32_t e = qtail + (uint32_t)__popcll(lm & lanemask_lt(lane));
+                    lq0[(e & (uint32_t)(G - 1)) + ((e / (uint32_t)G) & 1u) * stride] = s;
+                }
+                qtail += (uint32_t)__popcll(lm);
+            }
             if (doH) {
+                float3 o; float nx, ny, nz; uint32_t tri;
+                shooter_at(A, ts, m, area_max, n, s, key, shooter_base + s, o, nx, ny, nz, tri);
+                float ox, oy, oz;                                               // RRMathPrivate.h:48-53
+                if (fabsf(nx) > fabsf(nz)) { ox = -ny; oy = nx; oz = 0.f; } else { ox = 0.f; oy = -nz; oz = ny; }
+                const float olen = sqrtf(ox * ox + oy * oy + oz * oz);
+                const float tx = ox / olen, ty = oy / olen, tz = oz / olen;
+                const float bx = ny * tz - nz * ty, by = -nx * tz + nz * tx, bz = nx * ty - ny * tx;
+                GatherFilter flt{A.sc.verts, A.sc.idx, A.sc.side, tri, near_t, 0, rug_below};
                 float x, y;
                 filler_point(my_index, x, y);
                 const float dist = x * x + y * y;
@@ -260,41 +330,12 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
                 }
                 cRelH++;
             }
-            if (doL) {
-                flt.mode = 1;
-                for (uint32_t li = 0; li < nl; li++) {
-                    const lsg_light L = A.lights[li];
-                    float dx, dy, dz;
-                    if (L.type == 1) { dx = -L.direction[0]; dy = -L.direction[1]; dz = -L.direction[2]; }
-                    else { dx = L.position[0] - o.x; dy = L.position[1] - o.y; dz = L.position[2] - o.z; }
-                    float dsz = sqrtf(dx * dx + dy * dy + dz * dz);
-                    dx = dx / dsz; dy = dy / dsz; dz = dz / dsz;
-                    if (L.type == 1) dsz = dsz * A.locality;
-                    const float ni = dx * nx + dy * ny + dz * nz;            // gather.cpp:429
-                    if (ni <= 0 || !isfinite(ni)) { cRelL++; continue; }
-                    bool blocked = false; float bt = 0.f;
-                    if (L.cast_shadows) {
-                        TraceResult r;
-                        uint32_t nv_ = 0, nt_ = 0;
-                        trace_closest<STACK, STATS>(flt, A.sc.col, o, make_float3(dx, dy, dz), A.tmin, dsz, mystk, stride, r, nv_, nt_);
-                        if (STATS) { stS[0]++; stS[1] += nv_; stS[2] += nt_; }
-                        blocked = r.tri >= 0; bt = r.t;
-                    }
-                    if (!blocked) {
-                        // RRLight::getIrradiance (RRLight.cpp:345-362), REALISTIC attenuation
-                        float att = 1.f;
-                        if (L.type == 0) {
-                            const float ex = o.x - L.position[0], ey = o.y - L.position[1], ez = o.z - L.position[2];
-                            att = 1 / (0.000000000137289f + (ex * ex + ey * ey + ez * ez));
-                        }
-                        const float k = att * A.light_mult * ni;
-                        lr += L.color[0] * k; lg += L.color[1] * k; lb += L.color[2] * k;
-                        cRelL++;
-                    } else if (bt < A.rug) cUnrelL++;                        // gather.cpp:500-505
-                    else cRelL++;
-                }
-                cRounds++;
-            }
+            while (qtail - qhead >= (uint32_t)G) { light_batch((uint32_t)G); qhead += (uint32_t)G; }
+        }
+        while (qtail != qhead) {
+            const uint32_t cnt = min((uint32_t)G, qtail - qhead);
+            light_batch(cnt);
+            qhead += cnt;
         }
         relH += group_sum_u<G>(cRelH); relL += group_sum_u<G>(cRelL); unrelL += group_sum_u<G>(cUnrelL);
         shotRounds += group_sum_u<G>(cRounds);
@@ -330,10 +371,11 @@ template <int STACK, bool STATS>
 __global__ __launch_bounds__(256) void k_gather(GatherArgs A)
 {
     __shared__ int32_t stk[STACK * 256];
+    __shared__ uint32_t lq[2 * 256];
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const uint32_t ti = blockIdx.x * 4u + (tid >> 6);
     if (ti >= A.ntex) return;                 // whole wavefront leaves together
-    gather_texel<STACK, STATS>(A, ti, lane, stk + tid, 256);
+    gather_texel<STACK, STATS>(A, ti, lane, stk + tid, lq + tid, 256);
 }
 
 // low ray counts (at most G shooters per series): 256/G texels per block, G lanes each
@@ -341,10 +383,11 @@ template <int STACK, int G>
 __global__ __launch_bounds__(256) void k_gather_grouped(GatherArgs A)
 {
     __shared__ int32_t stk[STACK * 256];
+    __shared__ uint32_t lq[2 * 256];
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const uint32_t ti = blockIdx.x * (256u / G) + tid / G;
     if (ti >= A.ntex) return;                 // the whole group leaves together
-    gather_texel<STACK, false, G>(A, ti, lane, stk + tid, 256);
+    gather_texel<STACK, false, G>(A, ti, lane, stk + tid, lq + tid, 256);
 }
 
 __global__ void k_queue_reset(uint32_t* counters, uint32_t n)
@@ -356,6 +399,7 @@ template <int STACK, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void k_gather_persistent(GatherArgs A, WorkQueue Q)
 {
     __shared__ int32_t stk[STACK * 64 * WAVES];
+    __shared__ uint32_t lq[2 * 64 * WAVES];
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const uint32_t home = Q.nqueues > 1 ? blockIdx.x % Q.nqueues : 0;
     for (uint32_t qi = 0; qi < Q.nqueues; qi++) {
@@ -370,7 +414,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_gather_persistent(GatherArgs A, 
             if (tile >= t1) break;           // this queue is exhausted for every lane of the wave
             const uint32_t a = tile * Q.tile, b = min(A.ntex, a + Q.tile);
             for (uint32_t ti = a; ti < b; ti++)
-                gather_texel<STACK, false>(A, ti, lane, stk + tid, 64 * WAVES);
+                gather_texel<STACK, false>(A, ti, lane, stk + tid, lq + tid, 64 * WAVES);
         }
     }
 }
