@@ -203,6 +203,37 @@ int  lsg_gather_device(lsg_scene* scene, const lsg_texel* texels, uint32_t ntexe
                        const lsg_subtexel* subs, uint32_t nsubs, const lsg_gather_params* params,
                        float* out_rgba, void* stream);
 
+/* Output layers of updateLightmap (RRSolver/gather.h:24-31, lightmap.cpp:712-717). */
+#define LSG_LAYER_LIGHTMAP      0   /* LS_LIGHTMAP */
+#define LSG_LAYER_DIRECTION1    1   /* LS_DIRECTION1..3: directional lightmaps */
+#define LSG_LAYER_DIRECTION2    2
+#define LSG_LAYER_DIRECTION3    3
+#define LSG_LAYER_BENT_NORMALS  4   /* LS_BENT_NORMALS */
+#define LSG_NUM_LAYERS          5
+
+/* Per-vertex tangent frames of the scene's mesh, which directional lightmaps are
+ * weighted in (RRMesh::TangentBasis, RRMesh.h:207-215; RRMeshArrays::getTriangleNormals,
+ * RRMeshArrays.cpp:279-306).  tangents, bitangents: host float[nverts*3], in the
+ * collider's vertex order.  Both NULL = build each vertex's frame from its normal
+ * (TangentBasis::buildBasisFromNormal, RRMesh.cpp:70-75), which is also what a scene
+ * that never had frames set uses. */
+int  lsg_scene_set_tangents(lsg_scene* scene, const float* tangents, const float* bitangents);
+
+/* processTexel with every output layer of RRSolver::updateLightmap
+ * (gather.cpp:600-836, lightmap.cpp:806-820).  layers: LSG_NUM_LAYERS device float4
+ * arrays indexed by lsg_texel.out_index, NULL for a layer not wanted.
+ * - When any of LSG_LAYER_DIRECTION1..3 is given, every layer 0..3 is gathered with
+ *   the directional weighting (gatherAllDirections, gather.cpp:284-305, 453-481):
+ *   hemisphere ray k adds exitance * max(0, dir.d_i) / dir.n, a visible light adds
+ *   irradiance * max(0, L.d_i), with d_i = g_lightmapDirections[i] (gather.cpp:43-49)
+ *   in the texel's skewed frame.
+ * - LSG_LAYER_BENT_NORMALS receives the normalised bent normal as (n+1)/2 with the
+ *   texel's alpha (gather.cpp:306, 482, 809-830).
+ * Alphas as lsg_gather_device.  Asynchronous on stream. */
+int  lsg_gather_layers_device(lsg_scene* scene, const lsg_texel* texels, uint32_t ntexels,
+                              const lsg_subtexel* subs, uint32_t nsubs, const lsg_gather_params* params,
+                              float* const* layers, void* stream);
+
 /* Same gather with traversal work counters (synchronous): stats6 = hemisphere rays,
  * their internal-node visits, their triangle tests, shadow rays, node visits,
  * triangle tests -- the inputs of the bytes-per-texel roofline figure. */
@@ -299,6 +330,17 @@ int  lsg_bake_run(lsg_bake* bake, const lsg_gather_params* first, const lsg_gath
 int  lsg_bake_gather_sample(lsg_bake* bake, const lsg_gather_params* params, uint32_t stride, float* ms);
 /* copies object `object`'s RGBF lightmap (width*height*3 floats) to the host */
 int  lsg_bake_download(const lsg_bake* bake, uint32_t object, float* rgbf);
+/* RRSolver::updateLightmaps with directional and bent-normal layers
+ * (lightmap.cpp:943-1163, 806-876): first gather (if first != NULL), the layered final
+ * gather (lsg_gather_layers_device) of every texel, then the filter chain and RGBF
+ * writeback on each layer of layer_mask (bit i = LSG_LAYER_i) separately, as the
+ * reference filters each pixel buffer.  Bent normals are written without a colour
+ * space.  Eager (no graph); asynchronous on stream except for the grow loop's
+ * convergence checks.  Layer 0 is the buffer lsg_bake_download reads. */
+int  lsg_bake_run_layers(lsg_bake* bake, const lsg_gather_params* first, const lsg_gather_params* final_params,
+                         const lsg_filter_params* filtering, uint32_t layer_mask, void* stream);
+/* copies layer `layer` of object `object` (width*height*3 floats) to the host */
+int  lsg_bake_download_layer(const lsg_bake* bake, uint32_t object, uint32_t layer, float* rgbf);
 
 /* ---- learned gather ----------------------------------------------------------------------
  * No counterpart in the reference: a surrogate that replaces most of the 1024-ray

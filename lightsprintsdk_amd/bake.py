@@ -130,6 +130,41 @@ class GatherScene:
         N.check(N.lib().lsg_gather_device(self._h, dtex.ptr, ntex, dsub.ptr, nsub, ctypes.byref(p), dout.ptr, stream),
                 "lsg_gather_device")
 
+    def set_tangents(self, tangents=None, bitangents=None):
+        """Per-vertex tangent frames for directional lightmaps (RRMesh::TangentBasis);
+        None = each vertex's frame built from its normal (buildBasisFromNormal)."""
+        if tangents is None:
+            N.check(N.lib().lsg_scene_set_tangents(self._h, None, None), "set_tangents")
+            self.tangents = self.bitangents = None
+            return
+        self.tangents = np.ascontiguousarray(tangents, np.float32).reshape(len(self.verts), 3)
+        self.bitangents = np.ascontiguousarray(bitangents, np.float32).reshape(len(self.verts), 3)
+        N.check(N.lib().lsg_scene_set_tangents(self._h, self.tangents.ctypes.data_as(ctypes.c_void_p),
+                                               self.bitangents.ctypes.data_as(ctypes.c_void_p)), "set_tangents")
+
+    def gather_layers_device(self, dtex: DeviceBuffer, ntex: int, dsub: DeviceBuffer, nsub: int, params: GatherParams,
+                             douts, stream=None):
+        """processTexel into the output layers of updateLightmap: douts = 5 DeviceBuffers
+        (lightmap, directions 1-3, bent normals) or None for a layer not wanted."""
+        p = params.c()
+        ptrs = (ctypes.c_void_p * 5)(*[(d.ptr if d is not None else None) for d in douts])
+        N.check(N.lib().lsg_gather_layers_device(self._h, dtex.ptr, ntex, dsub.ptr, nsub, ctypes.byref(p), ptrs, stream),
+                "lsg_gather_layers_device")
+
+    def gather_layers(self, texels, subs, params: GatherParams, layers=(0, 1, 2, 3, 4), nout=None):
+        """Host convenience: returns {layer: float32 [nout,4]}."""
+        if nout is None:
+            nout = int(texels["out_index"].max()) + 1 if len(texels) else 0
+        dt = DeviceBuffer.from_array(texels)
+        ds = DeviceBuffer.from_array(subs)
+        outs = [None] * 5
+        for k in layers:
+            outs[k] = DeviceBuffer(16 * max(nout, 1))
+            N.check(N.lib().lsg_memset(outs[k].ptr, 0, 16 * max(nout, 1)), "memset")
+        self.gather_layers_device(dt, len(texels), ds, len(subs), params, outs)
+        N.check(N.lib().lsg_synchronize(), "sync")
+        return {k: outs[k].download(np.float32, (nout, 4)) for k in layers}
+
     def gather(self, texels, subs, params: GatherParams, nout=None):
         """Host convenience: uploads texels/subtexels, gathers, returns float32 [nout,4]."""
         if nout is None:
@@ -144,6 +179,7 @@ class GatherScene:
 
     def oracle_arrays(self, exitance):
         return dict(verts=self.verts, idx=self.idx, normals=self.normals, side_bits=self.side_bits,
+                    tangents=getattr(self, "tangents", None), bitangents=getattr(self, "bitangents", None),
                     exitance=np.ascontiguousarray(exitance, np.float32).reshape(len(self.idx), 8),
                     lights=np.array([(l["type"], l["position"], l["direction"], l["color"], l["cast_shadows"])
                                      for l in self.lights], dtype=[("type", "<i4"), ("pos", "<f4", 3), ("dir", "<f4", 3),
@@ -268,6 +304,28 @@ class SceneBake:
         self.final_gather(stream)
         self.filter(stream)
 
+    def bake_layers(self, layers=(0, 1, 2, 3, 4), stream=None):
+        """updateLightmaps with directional and bent-normal layers: one layered final
+        gather, then the filter chain + RGBF writeback on each layer separately
+        (lightmap.cpp:806-876).  Returns {layer: [per-object RGBF maps]}."""
+        n = max(self.nout, 1)
+        outs, rgbfs = [None] * 5, [None] * 5
+        for k in layers:
+            outs[k] = self.d_out if k == 0 else DeviceBuffer(16 * n)
+            rgbfs[k] = self.d_rgbf if k == 0 else DeviceBuffer(12 * n)
+            N.check(N.lib().lsg_memset(outs[k].ptr, 0, 16 * n), "memset")
+        self.gs.gather_layers_device(self.d_tex, len(self.texels), self.d_sub, len(self.subs), self.params, outs, stream)
+        fp = self.filtering.c()
+        res = {}
+        for k in layers:
+            N.check(N.lib().lsg_filter_lightmaps(self.maps, len(self.slots), outs[k].ptr, rgbfs[k].ptr, ctypes.byref(fp),
+                                                 stream), "lsg_filter_lightmaps")
+        N.check(N.lib().lsg_synchronize(), "sync")
+        for k in layers:
+            out = rgbfs[k].download(np.float32, (n, 3))
+            res[k] = [out[b:b + w * h].reshape(h, w, 3) for b, w, h in self.slots]
+        return res
+
     def lightmaps_rgbf(self):
         N.check(N.lib().lsg_synchronize(), "sync")
         out = self.d_rgbf.download(np.float32, (max(self.nout, 1), 3))
@@ -336,6 +394,23 @@ class BakePlan:
         N.check(N.lib().lsg_bake_run(self._h, ctypes.byref(fp) if first else None, ctypes.byref(p),
                                      ctypes.byref(flt) if filtering else None, N.BAKE_USE_GRAPH if graph else 0, stream),
                 "lsg_bake_run")
+
+    def run_layers(self, layers=(0, 1, 2, 3, 4), first=True, filtering=True, stream=None):
+        """The plan's bake with directional and bent-normal layers (lsg_bake_run_layers)."""
+        fp = self.first_params.c()
+        p = self.params.c()
+        flt = self.filtering.c()
+        mask = 0
+        for k in layers:
+            mask |= 1 << k
+        N.check(N.lib().lsg_bake_run_layers(self._h, ctypes.byref(fp) if first else None, ctypes.byref(p),
+                                            ctypes.byref(flt) if filtering else None, mask, stream), "lsg_bake_run_layers")
+
+    def lightmap_layer(self, k, layer):
+        ob = self.scene.objects[k]
+        out = np.zeros((ob.height, ob.width, 3), np.float32)
+        N.check(N.lib().lsg_bake_download_layer(self._h, k, layer, out.ctypes.data_as(ctypes.c_void_p)), "bake download")
+        return out
 
     def sample_ms(self, stride):
         ms = ctypes.c_float()

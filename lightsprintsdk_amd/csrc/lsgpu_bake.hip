@@ -60,8 +60,11 @@ struct BakeImpl {
     hipGraph_t graph = nullptr; hipGraphExec_t exec = nullptr;
     std::vector<uint8_t> graph_key;
     uint32_t grow_steps = 0;
+    float4* d_lrgba[4] = {nullptr, nullptr, nullptr, nullptr};   // layers 1..4 (layer 0 = d_rgba / d_rgbf)
+    float* d_lrgbf[4] = {nullptr, nullptr, nullptr, nullptr};
     ~BakeImpl()
     {
+        for (int i = 0; i < 4; i++) { dfree(d_lrgba[i]); dfree(d_lrgbf[i]); }
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         if (own) (void)hipStreamDestroy(own);
@@ -94,18 +97,22 @@ static int run_gather_final(BakeImpl& B, const lsg_gather_params* p, const lsg_t
     return launch_gather_persistent(A, stack, Q, t.waves_per_block, blocks, st);
 }
 
+static int run_first(BakeImpl& B, const lsg_gather_params* first, hipStream_t st)
+{
+    GatherArgs A;
+    int stack = 0, e;
+    if ((e = gather_args(B.scene, B.d_ttex, B.ntri, B.d_tsub, first, (float*)B.d_irr, A, stack))) return e;
+    if ((e = launch_gather(A, stack, st))) return e;
+    k_exitance<<<nblk(B.ntri), 256, 0, st>>>(B.ntri, B.d_irr, B.d_albedo, B.d_emit, B.scene->impl.d_exit);
+    return LSG_OK;
+}
+
 // the whole bake on stream st; fixed_grow < 0 = adaptive grow (not capturable)
 static int run_all(BakeImpl& B, const lsg_gather_params* first, const lsg_gather_params* fin, const lsg_filter_params* fp,
                    hipStream_t st, int fixed_grow, uint32_t* steps_used)
 {
     int e;
-    if (first) {
-        GatherArgs A;
-        int stack = 0;
-        if ((e = gather_args(B.scene, B.d_ttex, B.ntri, B.d_tsub, first, (float*)B.d_irr, A, stack))) return e;
-        if ((e = launch_gather(A, stack, st))) return e;
-        k_exitance<<<nblk(B.ntri), 256, 0, st>>>(B.ntri, B.d_irr, B.d_albedo, B.d_emit, B.scene->impl.d_exit);
-    }
+    if (first && (e = run_first(B, first, st))) return e;
     k_zero_f4<<<nblk(B.nslots), 256, 0, st>>>(B.d_rgba, B.nslots);
     if ((e = run_gather_final(B, fin, B.d_tex, B.ntex, st))) return e;
     if (fp) {
@@ -275,6 +282,55 @@ int lsg_bake_run(lsg_bake* b, const lsg_gather_params* first, const lsg_gather_p
     }
     if (st) { hipEvent_t ev; LSG_HIP(hipEventCreate(&ev)); LSG_HIP(hipEventRecord(ev, B.own)); LSG_HIP(hipStreamWaitEvent(st, ev, 0)); (void)hipEventDestroy(ev); }
     else LSG_HIP(hipStreamSynchronize(B.own));
+    return LSG_OK;
+}
+
+int lsg_bake_run_layers(lsg_bake* b, const lsg_gather_params* first, const lsg_gather_params* final_params,
+                        const lsg_filter_params* filtering, uint32_t layer_mask, void* stream)
+{
+    if (!b || !final_params) return fail(LSG_ERR_INVALID, "lsg_bake_run_layers: null argument");
+    if (!layer_mask || layer_mask >= (1u << LSG_NUM_LAYERS)) return fail(LSG_ERR_INVALID, "lsg_bake_run_layers: layer_mask %#x", layer_mask);
+    if (filtering && filtering->wrap) return fail(LSG_ERR_INVALID, "lsg_bake_run_layers: wrap=true is not supported");
+    BakeImpl& B = b->impl;
+    hipStream_t st = (hipStream_t)stream;
+    float4* rgba[LSG_NUM_LAYERS];
+    float* rgbf[LSG_NUM_LAYERS];
+    for (int i = 0; i < LSG_NUM_LAYERS; i++) {
+        rgba[i] = nullptr; rgbf[i] = nullptr;
+        if (!(layer_mask >> i & 1u)) continue;
+        if (i == 0) { rgba[0] = B.d_rgba; rgbf[0] = B.d_rgbf; continue; }
+        if (!B.d_lrgba[i - 1]) LSG_HIP(hipMalloc(&B.d_lrgba[i - 1], 16 * (size_t)(B.nslots ? B.nslots : 1)));
+        if (!B.d_lrgbf[i - 1]) LSG_HIP(hipMalloc(&B.d_lrgbf[i - 1], 12 * (size_t)(B.nslots ? B.nslots : 1)));
+        rgba[i] = B.d_lrgba[i - 1]; rgbf[i] = B.d_lrgbf[i - 1];
+    }
+    int e;
+    if (first && (e = run_first(B, first, st))) return e;
+    for (int i = 0; i < LSG_NUM_LAYERS; i++)
+        if (rgba[i]) k_zero_f4<<<nblk(B.nslots), 256, 0, st>>>(rgba[i], B.nslots);
+    if ((e = lsg_gather_layers_device(B.scene, B.d_tex, B.ntex, B.d_sub, B.nsub, final_params, (float* const*)rgba, st))) return e;
+    if (filtering) {
+        for (int i = 0; i < LSG_NUM_LAYERS; i++) {
+            if (!rgba[i]) continue;
+            uint32_t used = 0;
+            if ((e = filter_run(B.d_maps, B.nmaps, B.ftotal, rgba[i], rgbf[i], *filtering, B.S, st, true, -1, &used))) return e;
+            if (i == 0) B.grow_steps = used;
+        }
+    }
+    LSG_HIP(hipGetLastError());
+    B.graph_key.clear();
+    return LSG_OK;
+}
+
+int lsg_bake_download_layer(const lsg_bake* b, uint32_t object, uint32_t layer, float* rgbf)
+{
+    if (!b || !rgbf) return fail(LSG_ERR_INVALID, "lsg_bake_download_layer: null argument");
+    const BakeImpl& B = b->impl;
+    if (object >= B.nmaps) return fail(LSG_ERR_INVALID, "lsg_bake_download_layer: object %u of %u", object, B.nmaps);
+    if (layer >= LSG_NUM_LAYERS) return fail(LSG_ERR_INVALID, "lsg_bake_download_layer: layer %u", layer);
+    const float* src = layer ? B.d_lrgbf[layer - 1] : B.d_rgbf;
+    if (!src) return fail(LSG_ERR_INVALID, "lsg_bake_download_layer: layer %u was never baked", layer);
+    const lsg_lightmap& m = B.maps[object];
+    LSG_HIP(hipMemcpy(rgbf, src + 3 * (size_t)m.rgbf_base, 12 * (size_t)m.width * m.height, hipMemcpyDeviceToHost));
     return LSG_OK;
 }
 

@@ -13,6 +13,8 @@ struct SceneView {
     const float* vnormal;        // 3 floats per vertex
     const uint8_t* side;         // LSG_SIDE_* per triangle
     const float4* exitance;      // 2 per triangle: front, back
+    const float* vtangent;       // 3 floats per vertex (output layers only)
+    const float* vbitangent;     // 3 floats per vertex (output layers only)
 };
 
 struct GatherArgs {
@@ -25,6 +27,9 @@ struct GatherArgs {
     uint32_t quality; uint32_t hemi_rays; uint32_t rounds;
     float tmin, locality, rug, light_mult; uint32_t seed;
     const lsg_light* lights; uint32_t nlights;
+    // output layers (LS_LIGHTMAP, LS_DIRECTION1..3, LS_BENT_NORMALS; null = not wanted),
+    // used by the layered kernels only; dirs = gatherAllDirections (lightmap.cpp:820)
+    float4* layer[5]; uint32_t dirs; uint32_t layered;
 };
 
 // persistent launches: tiles of texels handed out by atomic counters, one queue per
@@ -44,7 +49,13 @@ struct SceneImpl {
     float4* d_exit = nullptr;
     lsg_light* d_lights = nullptr;     // kMaxLights slots
     uint32_t nlights = 0;
+    float* d_vtan = nullptr;           // per-vertex tangent frames, built on first use
+    float* d_vbit = nullptr;
 };
+
+// per-vertex tangent frames of the scene: the caller's, or built from the vertex
+// normals on stream st when none were set
+int scene_frames(lsg_scene* s, hipStream_t st);
 
 int gather_args(lsg_scene* s, const lsg_texel* texels, uint32_t ntexels, const lsg_subtexel* subs,
                 const lsg_gather_params* p, float* out_rgba, GatherArgs& A, int& stack);

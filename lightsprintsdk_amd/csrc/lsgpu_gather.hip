@@ -136,10 +136,12 @@ struct GatherFilter {
 
 // Shooter s of a series of n: stratified subtexel pick + seeded jitter -> origin o on
 // the subtexel's triangle tri, and the normalised interpolated vertex normal at the
-// subtexel centroid (gather.cpp:680-752).
+// subtexel centroid (gather.cpp:680-752).  FRAME also returns the skewed-normalised
+// tangent and bitangent there (gather.cpp:731-732), which directional layers use.
+template <bool FRAME = false>
 __device__ inline void shooter_at(const GatherArgs& A, const lsg_subtexel* ts, uint32_t m, float area_max, uint32_t n,
                                   uint32_t s, uint32_t key, uint32_t jitter_index, float3& o, float& nx, float& ny,
-                                  float& nz, uint32_t& tri)
+                                  float& nz, uint32_t& tri, float3* tg = nullptr, float3* bt = nullptr)
 {
     const float pos = ((float)s + 0.5f) * area_max / (float)n;
     uint32_t j = 0;
@@ -162,6 +164,18 @@ __device__ inline void shooter_at(const GatherArgs& A, const lsg_subtexel* ts, u
     nz = NV[3 * (size_t)i0 + 2] * w + NV[3 * (size_t)i1 + 2] * cxs + NV[3 * (size_t)i2 + 2] * cys;
     const float nlen = sqrtf(nx * nx + ny * ny + nz * nz);
     nx = nx / nlen; ny = ny / nlen; nz = nz / nlen;
+    if constexpr (FRAME) {
+        const float* F[2] = {A.sc.vtangent, A.sc.vbitangent};
+        float3* out[2] = {tg, bt};
+        for (int k = 0; k < 2; k++) {
+            const float* T = F[k];
+            const float x = T[3 * (size_t)i0] * w + T[3 * (size_t)i1] * cxs + T[3 * (size_t)i2] * cys;
+            const float y = T[3 * (size_t)i0 + 1] * w + T[3 * (size_t)i1 + 1] * cxs + T[3 * (size_t)i2 + 1] * cys;
+            const float z = T[3 * (size_t)i0 + 2] * w + T[3 * (size_t)i1 + 2] * cxs + T[3 * (size_t)i2 + 2] * cys;
+            const float len = sqrtf(x * x + y * y + z * z);
+            *out[k] = make_float3(x / len, y / len, z / len);
+        }
+    }
     float ju, jv;
     jitter(key, jitter_index, ju, jv);
     const float ux = st.uv[0] + (st.uv[2] - st.uv[0]) * ju + (st.uv[4] - st.uv[0]) * jv;
@@ -169,6 +183,20 @@ __device__ inline void shooter_at(const GatherArgs& A, const lsg_subtexel* ts, u
     o = make_float3(v0.x + (v1.x - v0.x) * ux + (v2.x - v0.x) * uy,
                     v0.y + (v1.y - v0.y) * ux + (v2.y - v0.y) * uy,
                     v0.z + (v1.z - v0.z) * ux + (v2.z - v0.z) * uy);
+}
+
+// g_lightmapDirections[i] (gather.cpp:43-49) in the skewed frame (t, b, n), normalised
+// (gather.cpp:292-293, 458-459)
+__device__ inline float3 layer_dir(int i, const float3& t, const float3& b, const float3& n)
+{
+    float g0 = 0.f, g1 = 0.f, g2 = 1.f;
+    if (i == 1) { g1 = sqrtf(6.0f) / 3; g2 = 1 / sqrtf(3.0f); }
+    else if (i >= 2) { g0 = (i == 2 ? -1 : 1) / sqrtf(2.0f); g1 = -1 / sqrtf(6.0f); g2 = 1 / sqrtf(3.0f); }
+    const float x = t.x * g0 + b.x * g1 + n.x * g2;
+    const float y = t.y * g0 + b.y * g1 + n.y * g2;
+    const float z = t.z * g0 + b.z * g1 + n.z * g2;
+    const float len = sqrtf(x * x + y * y + z * z);
+    return make_float3(x / len, y / len, z / len);
 }
 
 // processTexel for texel ti by an aligned group of G lanes of one wavefront (G = 64:
@@ -182,7 +210,12 @@ __device__ inline void shooter_at(const GatherArgs& A, const lsg_subtexel* ts, u
 // time, so shadow rays run with full lane masks instead of a few lanes alongside each
 // chunk's hemisphere rays.  A shooter's shadow rays see the same origin, normal and
 // lights as before; only which lane adds them to the sums changes.
-template <int STACK, bool STATS, int G = 64>
+//
+// LAYERS: every output layer of updateLightmap (lightmap.cpp:806-820) -- the lightmap,
+// with A.dirs the three directional lightmaps weighted per layer direction
+// (gather.cpp:284-305 hemisphere, :453-481 lights), and the bent normal
+// (gather.cpp:306, :482, :809-830), into A.layer[0..4].
+template <int STACK, bool STATS, int G = 64, bool LAYERS = false>
 __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, int32_t* mystk, uint32_t* lq, int stride)
 {
     static_assert(G == 16 || G == 32 || G == 64, "group width");
@@ -195,6 +228,11 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
     const uint32_t light_rays = nl * rounds;
     if (!m || (!hemi_rays && !light_rays)) {
         if (lane == 0) {
+            if constexpr (LAYERS) {
+                for (int i = 0; i < 5; i++)
+                    if (A.layer[i]) A.layer[i][T.out_index] = make_float4(0.f, 0.f, 0.f, 0.f);
+                return;
+            }
             A.out[T.out_index] = make_float4(0.f, 0.f, 0.f, 0.f);
             if (A.parts) { A.parts[2 * (size_t)ti] = make_float4(0.f, 0.f, 0.f, 0.f); A.parts[2 * (size_t)ti + 1] = make_float4(0.f, 0.f, 0.f, 0.f); }
         }
@@ -209,6 +247,12 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
     const float rug_below = __int_as_float(__float_as_int(A.rug) - 1);   // largest float < rugDistance (> 0)
 
     float hr = 0.f, hg = 0.f, hb = 0.f, lr = 0.f, lg = 0.f, lb = 0.f;
+    // LAYERS: hemisphere and light sums of the four lightmap layers, and the bent normal
+    constexpr int NL = LAYERS ? 12 : 1;
+    float hl[NL], ll[NL], bn[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < NL; k++) { hl[k] = 0.f; ll[k] = 0.f; }
+    const bool dirs = LAYERS && A.dirs;
     uint32_t stH[3] = {0, 0, 0}, stS[3] = {0, 0, 0};
     uint32_t relH = 0, relL = 0, unrelL = 0, shotRounds = 0;        // wave totals (uniform)
     uint32_t shooter_base = 0;
@@ -230,7 +274,13 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
                 const uint32_t e = qhead + lane;
                 const uint32_t s = lq0[(e & (uint32_t)(G - 1)) + ((e / (uint32_t)G) & 1u) * stride];
                 float3 o; float nx, ny, nz; uint32_t tri;
-                shooter_at(A, ts, m, area_max, n, s, key, shooter_base + s, o, nx, ny, nz, tri);
+                float3 tg, bt;
+                shooter_at<LAYERS>(A, ts, m, area_max, n, s, key, shooter_base + s, o, nx, ny, nz, tri, &tg, &bt);
+                float3 ld[LAYERS ? 4 : 1];
+                if (dirs) {
+#pragma unroll
+                    for (int i = 0; i < (LAYERS ? 4 : 1); i++) ld[i] = layer_dir(i, tg, bt, make_float3(nx, ny, nz));
+                }
                 GatherFilter flt{A.sc.verts, A.sc.idx, A.sc.side, tri, near_t, 1, rug_below};
                 for (uint32_t li = 0; li < nl; li++) {
                     const lsg_light L = A.lights[li];
@@ -257,11 +307,35 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
                             const float ex = o.x - L.position[0], ey = o.y - L.position[1], ez = o.z - L.position[2];
                             att = 1 / (0.000000000137289f + (ex * ex + ey * ey + ez * ez));
                         }
-                        const float kl = att * A.light_mult * ni;
-                        lr += L.color[0] * kl; lg += L.color[1] * kl; lb += L.color[2] * kl;
-                        cRelL++;
-                    } else if (bt < A.rug) cUnrelL++;                        // gather.cpp:500-505
-                    else cRelL++;
+                        if constexpr (LAYERS) {
+                            const float ka = att * A.light_mult;
+                            if (dirs) {
+#pragma unroll
+                                for (int i = 0; i < 4; i++) {
+                                    const float n2 = dx * ld[i].x + dy * ld[i].y + dz * ld[i].z;   // gather.cpp:460
+                                    if (n2 > 0) {
+                                        const float k = ka * n2;
+                                        ll[3 * i] += L.color[0] * k; ll[3 * i + 1] += L.color[1] * k; ll[3 * i + 2] += L.color[2] * k;
+                                    }
+                                }
+                            } else {
+                                const float kl = ka * ni;      // the lightmap-only sums, as below
+                                ll[0] += L.color[0] * kl; ll[1] += L.color[1] * kl; ll[2] += L.color[2] * kl;
+                            }
+                            const float w = (fabsf(L.color[0] * ka) + fabsf(L.color[1] * ka) + fabsf(L.color[2] * ka))
+                                          * 0.33333333333333f * ni;   // gather.cpp:482
+                            bn[0] += dx * w; bn[1] += dy * w; bn[2] += dz * w;
+                        } else {
+                            const float kl = att * A.light_mult * ni;
+                            lr += L.color[0] * kl; lg += L.color[1] * kl; lb += L.color[2] * kl;
+                        }
+                    }
+                    // a blocker nearer than rugDistance makes the ray unreliable, any other
+                    // outcome is reliable (gather.cpp:483-511); counted arithmetically so
+                    // both counters stay in registers
+                    const uint32_t unrel = (blocked && bt < A.rug) ? 1u : 0u;
+                    cUnrelL += unrel;
+                    cRelL += 1u - unrel;
                 }
                 cRounds++;
             }
@@ -308,7 +382,8 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
             }
             if (doH) {
                 float3 o; float nx, ny, nz; uint32_t tri;
-                shooter_at(A, ts, m, area_max, n, s, key, shooter_base + s, o, nx, ny, nz, tri);
+                float3 tg, bt;
+                shooter_at<LAYERS>(A, ts, m, area_max, n, s, key, shooter_base + s, o, nx, ny, nz, tri, &tg, &bt);
                 float ox, oy, oz;                                               // RRMathPrivate.h:48-53
                 if (fabsf(nx) > fabsf(nz)) { ox = -ny; oy = nx; oz = 0.f; } else { ox = 0.f; oy = -nz; oz = ny; }
                 const float olen = sqrtf(ox * ox + oy * oy + oz * oz);
@@ -326,7 +401,29 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
                 if (STATS) { stH[0]++; stH[1] += nv_; stH[2] += nt_; }
                 if (r.tri >= 0) {
                     const float4 e = A.sc.exitance[2 * (size_t)r.tri + (r.front ? 0 : 1)];
-                    hr += e.x; hg += e.y; hb += e.z;
+                    if constexpr (LAYERS) {
+                        if (dirs) {
+                            // gather.cpp:284-305: dirs near the normal are picked more often,
+                            // so each layer weighs by its own cosine over the normal's
+                            const float ni1 = d.x * nx + d.y * ny + d.z * nz;
+                            if (ni1 > 0) {
+                                const float inv = 1 / ni1;
+#pragma unroll
+                                for (int i = 0; i < 4; i++) {
+                                    const float3 l = layer_dir(i, tg, bt, make_float3(nx, ny, nz));
+                                    const float n2 = d.x * l.x + d.y * l.y + d.z * l.z;
+                                    if (n2 > 0) {
+                                        const float k = n2 * inv;
+                                        hl[3 * i] += e.x * k; hl[3 * i + 1] += e.y * k; hl[3 * i + 2] += e.z * k;
+                                    }
+                                }
+                            }
+                        } else { hl[0] += e.x; hl[1] += e.y; hl[2] += e.z; }
+                        const float w = (fabsf(e.x) + fabsf(e.y) + fabsf(e.z)) * 0.33333333333333f;   // gather.cpp:306
+                        bn[0] += d.x * w; bn[1] += d.y * w; bn[2] += d.z * w;
+                    } else {
+                        hr += e.x; hg += e.y; hb += e.z;
+                    }
                 }
                 cRelH++;
             }
@@ -346,6 +443,37 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
             atomicAdd(&A.stats[k], (unsigned long long)stH[k]);
             atomicAdd(&A.stats[3 + k], (unsigned long long)stS[k]);
         }
+    }
+    if constexpr (LAYERS) {
+#pragma unroll
+        for (int k = 0; k < NL; k++) { hl[k] = group_sum<G>(hl[k]); ll[k] = group_sum<G>(ll[k]); }
+#pragma unroll
+        for (int k = 0; k < 3; k++) bn[k] = group_sum<G>(bn[k]);
+        if (lane == 0) {
+            const bool useH = hemi_rays && relH, useL = light_rays && relL;
+            const float f = useH ? 1 / (float)relH : 0.f;
+            const float alpha = (useH || useL) ? flags / 255.f : 0.f;   // gather.cpp:795-803
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                if (!A.layer[i]) continue;
+                float c[3] = {0.f, 0.f, 0.f};
+                for (int k = 0; k < 3; k++) {
+                    if (useH) c[k] += hl[3 * i + k] * f;
+                    if (useL) c[k] += ll[3 * i + k] / (float)shotRounds;
+                }
+                A.layer[i][T.out_index] = make_float4(c[0], c[1], c[2], alpha);
+            }
+            if (A.layer[4]) {
+                // gather.cpp:809-830: normalised sum, stored as (n+1)/2
+                float x = bn[0], y = bn[1], z = bn[2];
+                if (x != 0.f || y != 0.f || z != 0.f) {
+                    const float len = sqrtf(x * x + y * y + z * z);
+                    x = x / len; y = y / len; z = z / len;
+                }
+                A.layer[4][T.out_index] = make_float4((x + 1) * 0.5f, (y + 1) * 0.5f, (z + 1) * 0.5f, alpha);
+            }
+        }
+        return;
     }
     hr = group_sum<G>(hr); hg = group_sum<G>(hg); hb = group_sum<G>(hb);
     lr = group_sum<G>(lr); lg = group_sum<G>(lg); lb = group_sum<G>(lb);
@@ -367,7 +495,7 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
     }
 }
 
-template <int STACK, bool STATS>
+template <int STACK, bool STATS, bool LAYERS = false>
 __global__ __launch_bounds__(256) void k_gather(GatherArgs A)
 {
     __shared__ int32_t stk[STACK * 256];
@@ -375,11 +503,11 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs A)
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const uint32_t ti = blockIdx.x * 4u + (tid >> 6);
     if (ti >= A.ntex) return;                 // whole wavefront leaves together
-    gather_texel<STACK, STATS>(A, ti, lane, stk + tid, lq + tid, 256);
+    gather_texel<STACK, STATS, 64, LAYERS>(A, ti, lane, stk + tid, lq + tid, 256);
 }
 
 // low ray counts (at most G shooters per series): 256/G texels per block, G lanes each
-template <int STACK, int G>
+template <int STACK, int G, bool LAYERS = false>
 __global__ __launch_bounds__(256) void k_gather_grouped(GatherArgs A)
 {
     __shared__ int32_t stk[STACK * 256];
@@ -387,7 +515,7 @@ __global__ __launch_bounds__(256) void k_gather_grouped(GatherArgs A)
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const uint32_t ti = blockIdx.x * (256u / G) + tid / G;
     if (ti >= A.ntex) return;                 // the whole group leaves together
-    gather_texel<STACK, false, G>(A, ti, lane, stk + tid, lq + tid, 256);
+    gather_texel<STACK, false, G, LAYERS>(A, ti, lane, stk + tid, lq + tid, 256);
 }
 
 __global__ void k_queue_reset(uint32_t* counters, uint32_t n)
@@ -421,6 +549,44 @@ __global__ __launch_bounds__(64 * WAVES) void k_gather_persistent(GatherArgs A, 
 
 static inline void dfree(void* p) { if (p) (void)hipFree(p); }
 
+// TangentBasis::buildBasisFromNormal (RRMesh.cpp:70-75) at every vertex: the frame
+// RRMeshArrays::getTriangleNormals gives a mesh without tangents (RRMeshArrays.cpp:297-305)
+__global__ __launch_bounds__(256) void k_default_frames(const float* __restrict__ vn, uint32_t nv, float* __restrict__ vt,
+                                                        float* __restrict__ vb)
+{
+    const uint32_t v = blockIdx.x * 256 + threadIdx.x;
+    if (v >= nv) return;
+    const float nx = vn[3 * (size_t)v], ny = vn[3 * (size_t)v + 1], nz = vn[3 * (size_t)v + 2];
+    float ox, oy, oz;                                                    // RRMathPrivate.h:48-53
+    if (fabsf(nx) > fabsf(nz)) { ox = -ny; oy = nx; oz = 0.f; } else { ox = 0.f; oy = -nz; oz = ny; }
+    const float len = sqrtf(ox * ox + oy * oy + oz * oz);
+    const float tx = ox / len, ty = oy / len, tz = oz / len;
+    vt[3 * (size_t)v] = tx; vt[3 * (size_t)v + 1] = ty; vt[3 * (size_t)v + 2] = tz;
+    vb[3 * (size_t)v] = ny * tz - nz * ty;                              // RRMathPrivate.h:55-58
+    vb[3 * (size_t)v + 1] = -nx * tz + nz * tx;
+    vb[3 * (size_t)v + 2] = nx * ty - ny * tx;
+}
+
+static int frames_alloc(SceneImpl& S)
+{
+    const uint32_t nv = S.col->impl.nverts;
+    if (!S.d_vtan) LSG_HIP(hipMalloc(&S.d_vtan, 12 * (size_t)(nv ? nv : 1)));
+    if (!S.d_vbit) LSG_HIP(hipMalloc(&S.d_vbit, 12 * (size_t)(nv ? nv : 1)));
+    return LSG_OK;
+}
+
+int scene_frames(lsg_scene* s, hipStream_t st)
+{
+    SceneImpl& S = s->impl;
+    if (S.d_vtan && S.d_vbit) return LSG_OK;
+    const int e = frames_alloc(S);
+    if (e) return e;
+    const uint32_t nv = S.col->impl.nverts;
+    if (nv) k_default_frames<<<(nv + 255) / 256, 256, 0, st>>>(S.d_vnormal, nv, S.d_vtan, S.d_vbit);
+    LSG_HIP(hipGetLastError());
+    return LSG_OK;
+}
+
 int gather_args(lsg_scene* s, const lsg_texel* texels, uint32_t ntexels, const lsg_subtexel* subs,
                 const lsg_gather_params* p, float* out_rgba, GatherArgs& A, int& stack)
 {
@@ -432,6 +598,9 @@ int gather_args(lsg_scene* s, const lsg_texel* texels, uint32_t ntexels, const l
     A.sc.col = C.view();
     A.sc.verts = C.d_verts; A.sc.idx = C.d_idx;
     A.sc.vnormal = s->impl.d_vnormal; A.sc.side = s->impl.d_side; A.sc.exitance = s->impl.d_exit;
+    A.sc.vtangent = s->impl.d_vtan; A.sc.vbitangent = s->impl.d_vbit;
+    for (int i = 0; i < 5; i++) A.layer[i] = nullptr;
+    A.dirs = 0; A.layered = 0;
     A.texels = texels; A.ntex = ntexels; A.subs = subs; A.out = (float4*)out_rgba; A.stats = nullptr; A.parts = nullptr;
     A.quality = p->quality;
     A.hemi_rays = p->hemisphere ? (p->quality > 1 ? p->quality : 1) : 0;             // gather.cpp:197
@@ -479,23 +648,34 @@ int launch_gather(const GatherArgs& A, int stack, hipStream_t st)
     return LSG_OK;
 }
 
-static int launch_gather_one(const GatherArgs& A, int stack, hipStream_t st)
+template <bool LAYERS>
+static void launch_gather_kernels(const GatherArgs& A, int stack, hipStream_t st)
 {
-    if (!A.ntex) return LSG_OK;
     // shooters per series = max(hemisphere rays, light rounds); a group of G >= that
     // many lanes does the same work as a whole wave with idle lanes
     const uint32_t series = std::max(A.hemi_rays, A.rounds);
     if (series <= 32) {
         const uint32_t G = series <= 16 ? 16u : 32u;
         const unsigned grid = (unsigned)(((uint64_t)A.ntex * G + 255) / 256);
-        if (G == 16) { if (stack <= 32) k_gather_grouped<32, 16><<<grid, 256, 0, st>>>(A); else k_gather_grouped<64, 16><<<grid, 256, 0, st>>>(A); }
-        else { if (stack <= 32) k_gather_grouped<32, 32><<<grid, 256, 0, st>>>(A); else k_gather_grouped<64, 32><<<grid, 256, 0, st>>>(A); }
-        LSG_HIP(hipGetLastError());
-        return LSG_OK;
+        if (G == 16) {
+            if (stack <= 32) k_gather_grouped<32, 16, LAYERS><<<grid, 256, 0, st>>>(A);
+            else k_gather_grouped<64, 16, LAYERS><<<grid, 256, 0, st>>>(A);
+        } else {
+            if (stack <= 32) k_gather_grouped<32, 32, LAYERS><<<grid, 256, 0, st>>>(A);
+            else k_gather_grouped<64, 32, LAYERS><<<grid, 256, 0, st>>>(A);
+        }
+        return;
     }
     const unsigned grid = (A.ntex + 3) / 4;
-    if (stack <= 32) k_gather<32, false><<<grid, 256, 0, st>>>(A);
-    else k_gather<64, false><<<grid, 256, 0, st>>>(A);
+    if (stack <= 32) k_gather<32, false, LAYERS><<<grid, 256, 0, st>>>(A);
+    else k_gather<64, false, LAYERS><<<grid, 256, 0, st>>>(A);
+}
+
+static int launch_gather_one(const GatherArgs& A, int stack, hipStream_t st)
+{
+    if (!A.ntex) return LSG_OK;
+    if (A.layered) launch_gather_kernels<true>(A, stack, st);
+    else launch_gather_kernels<false>(A, stack, st);
     LSG_HIP(hipGetLastError());
     return LSG_OK;
 }
@@ -556,7 +736,51 @@ void lsg_scene_destroy(lsg_scene* s)
 {
     if (!s) return;
     dfree(s->impl.d_vnormal); dfree(s->impl.d_side); dfree(s->impl.d_exit); dfree(s->impl.d_lights);
+    dfree(s->impl.d_vtan); dfree(s->impl.d_vbit);
     delete s;
+}
+
+int lsg_scene_set_tangents(lsg_scene* s, const float* tangents, const float* bitangents)
+{
+    if (!s) return fail(LSG_ERR_INVALID, "lsg_scene_set_tangents: null scene");
+    if (!tangents != !bitangents) return fail(LSG_ERR_INVALID, "lsg_scene_set_tangents: give both tangents and bitangents, or neither");
+    SceneImpl& S = s->impl;
+    if (!tangents) {
+        dfree(S.d_vtan); dfree(S.d_vbit); S.d_vtan = S.d_vbit = nullptr;
+        const int e = scene_frames(s, nullptr);
+        if (e) return e;
+        LSG_HIP(hipStreamSynchronize(nullptr));
+        return LSG_OK;
+    }
+    const int e = frames_alloc(S);
+    if (e) return e;
+    const size_t nb = 12 * (size_t)S.col->impl.nverts;
+    if (nb) {
+        LSG_HIP(hipMemcpy(S.d_vtan, tangents, nb, hipMemcpyHostToDevice));
+        LSG_HIP(hipMemcpy(S.d_vbit, bitangents, nb, hipMemcpyHostToDevice));
+    }
+    return LSG_OK;
+}
+
+int lsg_gather_layers_device(lsg_scene* s, const lsg_texel* texels, uint32_t ntexels, const lsg_subtexel* subs, uint32_t nsubs,
+                             const lsg_gather_params* p, float* const* layers, void* stream)
+{
+    if (!s || !p || !layers) return fail(LSG_ERR_INVALID, "lsg_gather_layers_device: null argument");
+    bool any = false;
+    for (int i = 0; i < 5; i++) any |= layers[i] != nullptr;
+    if (!any) return fail(LSG_ERR_INVALID, "lsg_gather_layers_device: no output layer");
+    if (!ntexels) return LSG_OK;
+    if (!texels || !subs || !nsubs) return fail(LSG_ERR_INVALID, "lsg_gather_layers_device: null buffer");
+    hipStream_t st = (hipStream_t)stream;
+    int e = scene_frames(s, st);
+    if (e) return e;
+    GatherArgs A;
+    int stack = 0;
+    if ((e = gather_args(s, texels, ntexels, subs, p, layers[0], A, stack))) return e;
+    for (int i = 0; i < 5; i++) A.layer[i] = (float4*)layers[i];
+    A.dirs = (layers[1] || layers[2] || layers[3]) ? 1u : 0u;       // lightmap.cpp:820
+    A.layered = 1;
+    return launch_gather(A, stack, st);
 }
 
 static int gather_launch(lsg_scene* s, const lsg_texel* texels, uint32_t ntexels, const lsg_subtexel* subs, uint32_t nsubs,

@@ -38,6 +38,7 @@ bool RRSolverHIP::buildScene(const UpdateParameters& params)
 	const RRMesh* mesh = multi->getCollider()->getMesh();
 	const unsigned nt = mesh->getNumTriangles();
 	std::vector<float> verts(9 * (size_t)nt), normals(9 * (size_t)nt), exitance(8 * (size_t)nt, 0.f);
+	std::vector<float> tangents(9 * (size_t)nt), bitangents(9 * (size_t)nt);
 	std::vector<uint32_t> idx(3 * (size_t)nt);
 	std::vector<uint8_t> side(nt, 0);
 	albedo.assign(3 * (size_t)nt, 0.f);
@@ -56,6 +57,8 @@ bool RRSolverHIP::buildScene(const UpdateParameters& params)
 			{
 				verts[9 * (size_t)t + 3 * k + c] = p[c];
 				normals[9 * (size_t)t + 3 * k + c] = tn.vertex[k].normal[c];
+				tangents[9 * (size_t)t + 3 * k + c] = tn.vertex[k].tangent[c];
+				bitangents[9 * (size_t)t + 3 * k + c] = tn.vertex[k].bitangent[c];
 			}
 			idx[3 * (size_t)t + k] = 3 * t + k;
 		}
@@ -119,6 +122,13 @@ bool RRSolverHIP::buildScene(const UpdateParameters& params)
 		releaseScene();
 		return false;
 	}
+	// the mesh's vertex frames (RRMesh::TangentBasis), which directional layers use
+	if (lsg_scene_set_tangents(scene, tangents.data(), bitangents.data()) != LSG_OK)
+	{
+		RRReporter::report(ERRO, "RRSolverHIP: %s\n", lsg_last_error());
+		releaseScene();
+		return false;
+	}
 	return true;
 }
 
@@ -153,9 +163,15 @@ unsigned RRSolverHIP::bake(const Target* targets, unsigned numTargets, const Upd
 		bo[k].tri_uv = uvs[k].data();
 		bo[k].tri_ids = ids[k].data();
 		bo[k].ntri = (uint32_t)ids[k].size();
-		bo[k].width = targets[k].buffer->getWidth();
-		bo[k].height = targets[k].buffer->getHeight();
+		const RRBuffer* first = nullptr;
+		for (int l = 0; l < LSG_NUM_LAYERS && !first; l++) first = targets[k].buffer[l];
+		bo[k].width = first->getWidth();
+		bo[k].height = first->getHeight();
 	}
+	uint32_t mask = 0;
+	for (unsigned k = 0; k < numTargets; k++)
+		for (int l = 0; l < LSG_NUM_LAYERS; l++)
+			if (targets[k].buffer[l]) mask |= 1u << l;
 	lsg_bake* plan = nullptr;
 	if (lsg_bake_create(scene, bo.data(), numTargets, albedo.data(), emittance.data(), &plan) != LSG_OK)
 	{
@@ -184,23 +200,28 @@ unsigned RRSolverHIP::bake(const Target* targets, unsigned numTargets, const Upd
 		f.wrap = filtering->wrap ? 1 : 0;
 	}
 	unsigned updated = 0;
-	if (lsg_bake_run(plan, nullptr, &g, &f, 0, nullptr) != LSG_OK)
+	const int st = mask == 1u ? lsg_bake_run(plan, nullptr, &g, &f, 0, nullptr)
+	                          : lsg_bake_run_layers(plan, nullptr, &g, &f, mask, nullptr);
+	if (st != LSG_OK)
 		RRReporter::report(ERRO, "RRSolverHIP: %s\n", lsg_last_error());
 	else
 	{
 		// writeback: RGBF texels into the caller's buffers (copyElementsTo, RRBuffer.cpp:105;
-		// scaled formats take the solver's colour space)
+		// scaled formats take the solver's colour space, bent normals none, lightmap.cpp:875)
+		std::vector<float> rgbf;
 		for (unsigned k = 0; k < numTargets; k++)
-		{
-			RRBuffer* b = targets[k].buffer;
-			std::vector<float> rgbf(3 * (size_t)bo[k].width * bo[k].height);
-			if (lsg_bake_download(plan, k, rgbf.data()) != LSG_OK) continue;
-			const RRColorSpace* cs = b->getScaled() ? getColorSpace() : nullptr;
-			for (size_t i = 0; i < (size_t)bo[k].width * bo[k].height; i++)
-				b->setElement((unsigned)i, RRVec4(rgbf[3 * i], rgbf[3 * i + 1], rgbf[3 * i + 2], 1), cs);
-			b->version++;
-			updated++;
-		}
+			for (int l = 0; l < LSG_NUM_LAYERS; l++)
+			{
+				RRBuffer* b = targets[k].buffer[l];
+				if (!b) continue;
+				rgbf.resize(3 * (size_t)bo[k].width * bo[k].height);
+				if (lsg_bake_download_layer(plan, k, (uint32_t)l, rgbf.data()) != LSG_OK) continue;
+				const RRColorSpace* cs = (b->getScaled() && l != LSG_LAYER_BENT_NORMALS) ? getColorSpace() : nullptr;
+				for (size_t i = 0; i < (size_t)bo[k].width * bo[k].height; i++)
+					b->setElement((unsigned)i, RRVec4(rgbf[3 * i], rgbf[3 * i + 1], rgbf[3 * i + 2], 1), cs);
+				b->version++;
+				updated++;
+			}
 	}
 	lsg_bake_destroy(plan);
 	return updated;
@@ -209,27 +230,54 @@ unsigned RRSolverHIP::bake(const Target* targets, unsigned numTargets, const Upd
 unsigned RRSolverHIP::updateLightmaps(int layerLightmap, int layerDirectionalLightmap, int layerBentNormals,
                                       const UpdateParameters* params, const FilteringParameters* filtering)
 {
-	const bool supported = params && params->quality > 0 && layerDirectionalLightmap < 0 && layerBentNormals < 0
+	const bool supported = params && params->quality > 0
 	                    && (!getEnvironment() || params->direct.environmentMultiplier == 0) && lsg_device_count() > 0;
 	if (!supported)
 	{
 		if (referenceFallback) return RRSolver::updateLightmaps(layerLightmap, layerDirectionalLightmap, layerBentNormals, params, filtering);
-		RRReporter::report(ERRO, "RRSolverHIP::updateLightmaps: unsupported request (needs quality>0, lightmap layer only, "
+		RRReporter::report(ERRO, "RRSolverHIP::updateLightmaps: unsupported request (needs quality>0, "
 		                         "no environment, a gfx950 device).\n");
 		return 0;
 	}
 	// indirect solution first, by the reference's own solver (RRSolver.h:731-734)
 	if (!params->useCurrentSolution)
 		RRSolver::updateLightmaps(-1, -1, -1, params, nullptr);
-	if (layerLightmap < 0) return 0;
+	// layer numbers of the five outputs (lightmap.cpp:968-973)
+	int layers[LSG_NUM_LAYERS];
+	layers[LSG_LAYER_LIGHTMAP] = layerLightmap;
+	layers[LSG_LAYER_DIRECTION1] = layerDirectionalLightmap;
+	layers[LSG_LAYER_DIRECTION2] = layerDirectionalLightmap + (layerDirectionalLightmap >= 0 ? 1 : 0);
+	layers[LSG_LAYER_DIRECTION3] = layerDirectionalLightmap + (layerDirectionalLightmap >= 0 ? 2 : 0);
+	layers[LSG_LAYER_BENT_NORMALS] = layerBentNormals;
 	std::vector<Target> targets;
 	const RRObjects& objects = getStaticObjects();
 	for (unsigned i = 0; i < objects.size(); i++)
 	{
-		RRBuffer* b = objects[i]->illumination.getLayer(layerLightmap);
-		if (b && b->getType() == BT_2D_TEXTURE) targets.push_back(Target{i, b});
-		else if (b && !referenceFallback)
-			RRReporter::report(WARN, "RRSolverHIP: object %d: only 2d lightmaps are updated on the GPU.\n", i);
+		Target t{i, {nullptr, nullptr, nullptr, nullptr, nullptr}};
+		bool any = false, sizes_ok = true;
+		const RRBuffer* first = nullptr;
+		for (int l = 0; l < LSG_NUM_LAYERS; l++)
+		{
+			RRBuffer* b = layers[l] >= 0 ? objects[i]->illumination.getLayer(layers[l]) : nullptr;
+			if (!b) continue;
+			if (b->getType() != BT_2D_TEXTURE)
+			{
+				if (!referenceFallback)
+					RRReporter::report(WARN, "RRSolverHIP: object %d: only 2d lightmaps are updated on the GPU.\n", i);
+				continue;
+			}
+			if (first && (b->getWidth() != first->getWidth() || b->getHeight() != first->getHeight())) sizes_ok = false;
+			if (!first) first = b;
+			t.buffer[l] = b;
+			any = true;
+		}
+		if (!sizes_ok)
+		{
+			// lightmap.cpp:741-760: every pixel buffer of an object must have one size
+			RRReporter::report(WARN, "RRSolverHIP: object %d: pixel buffer sizes don't match.\n", i);
+			continue;
+		}
+		if (any) targets.push_back(t);
 	}
 	return targets.empty() ? 0 : bake(targets.data(), (unsigned)targets.size(), *params, filtering);
 }
@@ -238,9 +286,22 @@ unsigned RRSolverHIP::updateLightmap(int objectNumber, RRBuffer* lightmap, RRBuf
                                      RRBuffer* bentNormals, const UpdateParameters* params, const FilteringParameters* filtering)
 {
 	const RRObjects& objects = getStaticObjects();
-	const bool dir = directionalLightmap && (directionalLightmap[0] || directionalLightmap[1] || directionalLightmap[2]);
-	const bool supported = params && params->quality > 0 && !dir && !bentNormals && lightmap
-	                    && lightmap->getType() == BT_2D_TEXTURE && objectNumber >= 0 && (unsigned)objectNumber < objects.size()
+	Target t{(unsigned)objectNumber, {lightmap, directionalLightmap ? directionalLightmap[0] : nullptr,
+	                                  directionalLightmap ? directionalLightmap[1] : nullptr,
+	                                  directionalLightmap ? directionalLightmap[2] : nullptr, bentNormals}};
+	bool any = false, flat = true;
+	const RRBuffer* first = nullptr;
+	for (int l = 0; l < LSG_NUM_LAYERS; l++)
+	{
+		const RRBuffer* b = t.buffer[l];
+		if (!b) continue;
+		any = true;
+		if (b->getType() != BT_2D_TEXTURE) flat = false;
+		else if (first && (b->getWidth() != first->getWidth() || b->getHeight() != first->getHeight())) flat = false;
+		if (!first) first = b;
+	}
+	const bool supported = params && params->quality > 0 && any && flat
+	                    && objectNumber >= 0 && (unsigned)objectNumber < objects.size()
 	                    && (!getEnvironment() || params->direct.environmentMultiplier == 0) && lsg_device_count() > 0;
 	if (!supported)
 	{
@@ -250,7 +311,6 @@ unsigned RRSolverHIP::updateLightmap(int objectNumber, RRBuffer* lightmap, RRBuf
 	}
 	if (!params->useCurrentSolution)
 		RRSolver::updateLightmaps(-1, -1, -1, params, nullptr);
-	Target t{(unsigned)objectNumber, lightmap};
 	return bake(&t, 1, *params, filtering);
 }
 

@@ -14,13 +14,15 @@
 // solution (first gather + radiosity, RRSolver::updateLightmaps(-1,-1,-1,...)); the
 // per-texel final gather of every lightmap (processTexel, gather.cpp:600-836), the
 // filter chain (RRBuffer.cpp:630-975) and the writeback then run on the GPU through
-// liblsgpu's C ABI (lsg_bake_*), one persistent launch for all objects.
+// liblsgpu's C ABI (lsg_bake_*), one persistent launch for all objects.  Directional
+// lightmaps and bent normals (layerDirectionalLightmap .. +2, layerBentNormals,
+// lightmap.cpp:968-973) come from the layered gather (lsg_bake_run_layers) in the same
+// way, each layer filtered on its own.
 //
-// Supported: per-pixel lightmaps (BT_2D_TEXTURE) in the lightmap layer, no
-// directional or bent-normal layers, point (REALISTIC attenuation) and directional
-// lights, Lambertian triangle materials, no environment.  Anything else is refused
-// with an RRReporter error and 0 buffers updated; setReferenceFallback(true) hands
-// such calls to the reference RRSolver instead.
+// Supported: per-pixel buffers (BT_2D_TEXTURE) in any of the five layers, point
+// (REALISTIC attenuation) and directional lights, Lambertian triangle materials, no
+// environment.  Anything else is refused with an RRReporter error and 0 buffers
+// updated; setReferenceFallback(true) hands such calls to the reference RRSolver.
 #pragma once
 #include "Lightsprint/RRSolver.h"
 #include "lsgpu.h"
@@ -46,7 +48,8 @@ public:
 	void setReferenceFallback(bool enable) { referenceFallback = enable; }
 
 private:
-	struct Target { unsigned object; rr::RRBuffer* buffer; };
+	// one object's output buffers by layer (LSG_LAYER_*), nullptr = not wanted
+	struct Target { unsigned object; rr::RRBuffer* buffer[LSG_NUM_LAYERS]; };
 	unsigned bake(const Target* targets, unsigned numTargets, const UpdateParameters& params,
 	              const FilteringParameters* filtering);
 	bool buildScene(const UpdateParameters& params);

@@ -101,6 +101,8 @@ SIGNATURES = {
     "lsg_scene_set_lights": (ctypes.c_int, [P, P, U32]),
     "lsg_gather_device": (ctypes.c_int, [P, P, U32, P, U32, ctypes.POINTER(GatherParams), P, P]),
     "lsg_gather_stats": (ctypes.c_int, [P, P, U32, P, U32, ctypes.POINTER(GatherParams), P, ctypes.POINTER(ctypes.c_double)]),
+    "lsg_scene_set_tangents": (ctypes.c_int, [P, P, P]),
+    "lsg_gather_layers_device": (ctypes.c_int, [P, P, U32, P, U32, ctypes.POINTER(GatherParams), P, P]),
     "lsg_filter_lightmaps": (ctypes.c_int, [P, U32, P, P, ctypes.POINTER(FilterParams), P]),
     "lsg_bake_create": (ctypes.c_int, [P, P, U32, P, P, ctypes.POINTER(P)]),
     "lsg_bake_destroy": (None, [P]),
@@ -109,6 +111,8 @@ SIGNATURES = {
     "lsg_bake_run": (ctypes.c_int, [P, ctypes.POINTER(GatherParams), ctypes.POINTER(GatherParams), ctypes.POINTER(FilterParams), U32, P]),
     "lsg_bake_gather_sample": (ctypes.c_int, [P, ctypes.POINTER(GatherParams), U32, ctypes.POINTER(ctypes.c_float)]),
     "lsg_bake_download": (ctypes.c_int, [P, U32, P]),
+    "lsg_bake_run_layers": (ctypes.c_int, [P, ctypes.POINTER(GatherParams), ctypes.POINTER(GatherParams), ctypes.POINTER(FilterParams), U32, P]),
+    "lsg_bake_download_layer": (ctypes.c_int, [P, U32, U32, P]),
     "lsg_mlp_num_params": (U32, []),
     "lsg_mlp_create": (ctypes.c_int, [U32, ctypes.POINTER(P)]),
     "lsg_mlp_destroy": (None, [P]),

@@ -191,6 +191,58 @@ static void subtexel_basis(const float* vn, const uint32_t* tidx, const float* s
     bt[2] = n[0] * tg[1] - n[1] * tg[0];
 }
 
+/* Skewed-normalised tangent and bitangent at the subtexel centroid: the normalised
+ * barycentric blend of the vertex frames (gather.cpp:731-732). */
+static void skewed_frame(const float* vt, const float* vb, const uint32_t* tidx, const float* sub, float* tg, float* bg)
+{
+    float cx = (sub[1] + sub[3] + sub[5]) * 0.333333333f;
+    float cy = (sub[2] + sub[4] + sub[6]) * 0.333333333f;
+    float w = 1 - cx - cy;
+    const float* F[2] = {vt, vb};
+    float* out[2] = {tg, bg};
+    for (int k = 0; k < 2; k++) {
+        const float* a = F[k] + 3 * (size_t)tidx[0];
+        const float* b = F[k] + 3 * (size_t)tidx[1];
+        const float* c = F[k] + 3 * (size_t)tidx[2];
+        float m[3];
+        for (int j = 0; j < 3; j++) m[j] = a[j] * w + b[j] * cx + c[j] * cy;
+        float len = sqrtf(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+        for (int j = 0; j < 3; j++) out[k][j] = m[j] / len;
+    }
+}
+
+/* g_lightmapDirections[i] (gather.cpp:43-49) expressed in the frame (t, b, n) and
+ * normalised (gather.cpp:292-293, 458-459). */
+static void layer_dir(int i, const float* t, const float* b, const float* n, float* out)
+{
+    float g0 = 0.f, g1 = 0.f, g2 = 1.f;
+    if (i == 1) { g1 = sqrtf(6.0f) / 3; g2 = 1 / sqrtf(3.0f); }
+    else if (i >= 2) { g0 = (i == 2 ? -1 : 1) / sqrtf(2.0f); g1 = -1 / sqrtf(6.0f); g2 = 1 / sqrtf(3.0f); }
+    float v[3];
+    for (int j = 0; j < 3; j++) v[j] = t[j] * g0 + b[j] * g1 + n[j] * g2;
+    float len = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    for (int j = 0; j < 3; j++) out[j] = v[j] / len;
+}
+
+/* TangentBasis::buildBasisFromNormal (RRMesh.cpp:70-75) per vertex: the frames of a
+ * mesh without tangents (RRMeshArrays.cpp:297-305). */
+void lsgo_default_frames(const float* vn, uint32_t nv, float* vt, float* vb)
+{
+    for (uint32_t v = 0; v < nv; v++) {
+        const float* n = vn + 3 * (size_t)v;
+        float o[3];
+        if (fabsf(n[0]) > fabsf(n[2])) { o[0] = -n[1]; o[1] = n[0]; o[2] = 0; }
+        else { o[0] = 0; o[1] = -n[2]; o[2] = n[1]; }
+        float len = sqrtf(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
+        float* t = vt + 3 * (size_t)v;
+        float* b = vb + 3 * (size_t)v;
+        for (int j = 0; j < 3; j++) t[j] = o[j] / len;
+        b[0] = n[1] * t[2] - n[2] * t[1];
+        b[1] = -n[0] * t[2] + n[2] * t[0];
+        b[2] = n[0] * t[1] - n[1] * t[0];
+    }
+}
+
 /* Number of light shots among the first s shooters of a series of n shooters
  * with r rounds, sequential rule of gather.cpp:777-783 (for r < n). */
 uint32_t lsgo_shots_before(uint32_t s, uint32_t n, uint32_t r)
@@ -201,13 +253,20 @@ uint32_t lsgo_shots_before(uint32_t s, uint32_t n, uint32_t r)
     return c;
 }
 
-void lsgo_gather(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_t ntri,
-                 const float* vnormals, const uint8_t* side_mask, const float* exitance,
-                 const lsgo_light* lights, uint32_t nlights, const lsgo_gparams* p,
-                 const uint32_t* texels, uint32_t ntex, const float* subs,
-                 const float* nodes, int32_t root_ref, const uint32_t* prim, uint32_t nprim,
-                 double* out_rgba, lsgo_gstats* stats, int threads)
+/* processTexel for every texel.  out_rgba: the lightmap (LS_LIGHTMAP) as the
+ * reference gathers it without directional layers; layers (optional, 5 planes:
+ * LS_LIGHTMAP, LS_DIRECTION1..3, LS_BENT_NORMALS, NULL entries skipped): the output
+ * layers of updateLightmap, with the directional weighting of gather.cpp:284-305 and
+ * :453-481 when any of planes 1..3 is given (gatherAllDirections, lightmap.cpp:820),
+ * and the bent normal of gather.cpp:306, :482, :809-830. */
+static void gather_core(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_t ntri,
+                        const float* vnormals, const float* vtan, const float* vbit, const uint8_t* side_mask,
+                        const float* exitance, const lsgo_light* lights, uint32_t nlights, const lsgo_gparams* p,
+                        const uint32_t* texels, uint32_t ntex, const float* subs,
+                        const float* nodes, int32_t root_ref, const uint32_t* prim, uint32_t nprim,
+                        double* out_rgba, double* const* layers, lsgo_gstats* stats, int threads)
 {
+    const int dirs = layers && (layers[1] || layers[2] || layers[3]);
     scene_t sc;
     sc.verts = verts; sc.idx = idx; sc.ntri = ntri; sc.side_mask = side_mask;
     sc.nodes = nodes; sc.root_ref = root_ref; sc.prim = prim; sc.nprim = nprim;
@@ -232,9 +291,14 @@ void lsgo_gather(const float* verts, uint32_t nverts, const uint32_t* idx, uint3
         const uint32_t* tx = texels + 4 * (size_t)ti;
         uint32_t out_index = tx[0], sb = tx[1], sc_cnt = tx[2] & 0xffffu, flags = (tx[2] >> 16) & 0xffu;
         const float* ts = subs + 8 * (size_t)sb;
-        double* out = out_rgba + 4 * (size_t)out_index;
-        out[0] = out[1] = out[2] = out[3] = 0;
+        double* out = out_rgba ? out_rgba + 4 * (size_t)out_index : NULL;
+        if (out) out[0] = out[1] = out[2] = out[3] = 0;
+        if (layers)
+            for (int i = 0; i < 5; i++)
+                if (layers[i]) for (int c = 0; c < 4; c++) layers[i][4 * (size_t)out_index + c] = 0;
         if (!sc_cnt || (!hemi_rays && !light_rays)) continue;
+        double sumHL[4][3], sumLL[4][3], bent[3] = {0, 0, 0};
+        memset(sumHL, 0, sizeof sumHL); memset(sumLL, 0, sizeof sumLL);
         uint32_t key = lsgo_texel_key(p->seed, tx[3]);
         uint32_t filler = key & 0xffffu;
         float area_max = 0;
@@ -259,6 +323,12 @@ void lsgo_gather(const float* verts, uint32_t nverts, const uint32_t* idx, uint3
                 const float* v2 = verts + 3 * (size_t)tidx[2];
                 float bn[3], bt[3], bb[3];
                 subtexel_basis(vnormals, tidx, sub, bn, bt, bb);
+                float ld[4][3];
+                if (dirs) {
+                    float st[3], sb2[3];
+                    skewed_frame(vtan, vbit, tidx, sub, st, sb2);
+                    for (int i = 0; i < 4; i++) layer_dir(i, st, sb2, bn, ld[i]);
+                }
                 float ju, jv; lsgo_jitter(key, shooter_base + s, &ju, &jv);
                 float ux = sub[1] + (sub[3] - sub[1]) * ju + (sub[5] - sub[1]) * jv;
                 float uy = sub[2] + (sub[4] - sub[2]) * ju + (sub[6] - sub[2]) * jv;
@@ -278,6 +348,20 @@ void lsgo_gather(const float* verts, uint32_t nverts, const uint32_t* idx, uint3
                     if (hit >= 0) {
                         const float* e = exitance + 4 * (2 * (size_t)hit + (fr ? 0 : 1));
                         for (int c = 0; c < 3; c++) sumH[c] += e[c];
+                        if (layers) {
+                            if (dirs) {
+                                float ni1 = d[0] * bn[0] + d[1] * bn[1] + d[2] * bn[2];
+                                if (ni1 > 0) {
+                                    float inv = 1 / ni1;
+                                    for (int i = 0; i < 4; i++) {
+                                        float n2 = d[0] * ld[i][0] + d[1] * ld[i][1] + d[2] * ld[i][2];
+                                        if (n2 > 0) { float k = n2 * inv; for (int c = 0; c < 3; c++) sumHL[i][c] += (double)e[c] * k; }
+                                    }
+                                }
+                            } else for (int c = 0; c < 3; c++) sumHL[0][c] += e[c];
+                            double w = ((double)fabsf(e[0]) + fabsf(e[1]) + fabsf(e[2])) / 3;
+                            for (int c = 0; c < 3; c++) bent[c] += d[c] * w;
+                        }
                     }
                     relH++;
                 }
@@ -305,6 +389,18 @@ void lsgo_gather(const float* verts, uint32_t nverts, const uint32_t* idx, uint3
                                 att = 1.0 / (0.000000000137289 + (dx * dx + dy * dy + dz * dz));
                             }
                             for (int c = 0; c < 3; c++) sumL[c] += (double)L->color[c] * att * p->light_multiplier * ni;
+                            if (layers) {
+                                double ir[3];
+                                for (int c = 0; c < 3; c++) ir[c] = (double)L->color[c] * att * p->light_multiplier;
+                                if (dirs) {
+                                    for (int i = 0; i < 4; i++) {
+                                        float n2 = d[0] * ld[i][0] + d[1] * ld[i][1] + d[2] * ld[i][2];
+                                        if (n2 > 0) for (int c = 0; c < 3; c++) sumLL[i][c] += ir[c] * n2;
+                                    }
+                                } else for (int c = 0; c < 3; c++) sumLL[0][c] += ir[c] * ni;
+                                double w = (fabs(ir[0]) + fabs(ir[1]) + fabs(ir[2])) / 3 * ni;
+                                for (int c = 0; c < 3; c++) bent[c] += d[c] * w;
+                            }
                             relL++;
                         } else if (th < p->rug_distance) unrelL++;              /* gather.cpp:500-505 */
                         else relL++;
@@ -316,11 +412,61 @@ void lsgo_gather(const float* verts, uint32_t nverts, const uint32_t* idx, uint3
         }
         double rgb[3] = {0, 0, 0};
         int reliable = 0;
-        if (hemi_rays && relH) { for (int c = 0; c < 3; c++) rgb[c] += sumH[c] / relH; reliable = 1; }
-        if (light_rays && relL && !(inside > light_rays)) { for (int c = 0; c < 3; c++) rgb[c] += sumL[c] / shotRounds; reliable = 1; }
-        out[0] = rgb[0]; out[1] = rgb[1]; out[2] = rgb[2];
-        out[3] = reliable ? flags / 255.0 : 0.0;
+        const int useH = hemi_rays && relH, useL = light_rays && relL && !(inside > light_rays);
+        if (useH) { for (int c = 0; c < 3; c++) rgb[c] += sumH[c] / relH; reliable = 1; }
+        if (useL) { for (int c = 0; c < 3; c++) rgb[c] += sumL[c] / shotRounds; reliable = 1; }
+        const double alpha = reliable ? flags / 255.0 : 0.0;
+        if (out) { out[0] = rgb[0]; out[1] = rgb[1]; out[2] = rgb[2]; out[3] = alpha; }
+        if (layers) {
+            for (int i = 0; i < 4; i++) {
+                if (!layers[i]) continue;
+                double* o4 = layers[i] + 4 * (size_t)out_index;
+                for (int c = 0; c < 3; c++)
+                    o4[c] = (useH ? sumHL[i][c] / relH : 0.0) + (useL ? sumLL[i][c] / shotRounds : 0.0);
+                o4[3] = alpha;
+            }
+            if (layers[4]) {
+                double x = bent[0], y = bent[1], z = bent[2];
+                if (x != 0 || y != 0 || z != 0) { double len = sqrt(x * x + y * y + z * z); x /= len; y /= len; z /= len; }
+                double* o4 = layers[4] + 4 * (size_t)out_index;
+                o4[0] = (x + 1) * 0.5; o4[1] = (y + 1) * 0.5; o4[2] = (z + 1) * 0.5; o4[3] = alpha;
+            }
+        }
     }
     if (stats) { stats->hemi_rays += nh; stats->shadow_rays += ns; stats->series += nser; }
     free(tris);
+}
+
+void lsgo_gather(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_t ntri,
+                 const float* vnormals, const uint8_t* side_mask, const float* exitance,
+                 const lsgo_light* lights, uint32_t nlights, const lsgo_gparams* p,
+                 const uint32_t* texels, uint32_t ntex, const float* subs,
+                 const float* nodes, int32_t root_ref, const uint32_t* prim, uint32_t nprim,
+                 double* out_rgba, lsgo_gstats* stats, int threads)
+{
+    gather_core(verts, nverts, idx, ntri, vnormals, NULL, NULL, side_mask, exitance, lights, nlights, p, texels, ntex,
+                subs, nodes, root_ref, prim, nprim, out_rgba, NULL, stats, threads);
+}
+
+/* lsgo_gather with the output layers of updateLightmap.  vtan/vbit: per-vertex frames
+ * (NULL = lsgo_default_frames of the normals); layers: 5 planes of float64 RGBA, NULL
+ * for a layer not wanted. */
+void lsgo_gather_layers(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_t ntri,
+                        const float* vnormals, const float* vtan, const float* vbit, const uint8_t* side_mask,
+                        const float* exitance, const lsgo_light* lights, uint32_t nlights, const lsgo_gparams* p,
+                        const uint32_t* texels, uint32_t ntex, const float* subs,
+                        const float* nodes, int32_t root_ref, const uint32_t* prim, uint32_t nprim,
+                        double* const* layers, lsgo_gstats* stats, int threads)
+{
+    float* ft = NULL;
+    float* fb = NULL;
+    if (!vtan || !vbit) {
+        ft = (float*)malloc(12 * (size_t)(nverts ? nverts : 1));
+        fb = (float*)malloc(12 * (size_t)(nverts ? nverts : 1));
+        lsgo_default_frames(vnormals, nverts, ft, fb);
+        vtan = ft; vbit = fb;
+    }
+    gather_core(verts, nverts, idx, ntri, vnormals, vtan, vbit, side_mask, exitance, lights, nlights, p, texels, ntex,
+                subs, nodes, root_ref, prim, nprim, NULL, layers, stats, threads);
+    free(ft); free(fb);
 }

@@ -229,6 +229,60 @@ def gather(scene_arrays, params, texels, subs, tree=None, threads=0):
     return out, gs
 
 
+def gather_layers(scene_arrays, params, texels, subs, layers=(0, 1, 2, 3, 4), tree=None, threads=0):
+    """processTexel with the output layers of updateLightmap (lightmap, directions 1-3,
+    bent normals).  scene_arrays as gather(), plus optional per-vertex tangents and
+    bitangents (None = buildBasisFromNormal of the normals).  Returns {layer: [nout,4]}."""
+    L = lib()
+    if not getattr(L, "_layers_declared", False):
+        P, U = ctypes.c_void_p, ctypes.c_uint32
+        L.lsgo_gather_layers.argtypes = [P, U, P, U, P, P, P, P, P, P, U, P, P, U, P, P, ctypes.c_int32, P, U, P, P,
+                                         ctypes.c_int]
+        L.lsgo_gather_layers.restype = None
+        L.lsgo_default_frames.argtypes = [P, U, P, P]
+        L.lsgo_default_frames.restype = None
+        L._layers_declared = True
+    v, i = _mesh(scene_arrays["verts"], scene_arrays["idx"])
+    vn = np.ascontiguousarray(scene_arrays["normals"], dtype=np.float32)
+    vt, vb = scene_arrays.get("tangents"), scene_arrays.get("bitangents")
+    vt = None if vt is None else np.ascontiguousarray(vt, dtype=np.float32)
+    vb = None if vb is None else np.ascontiguousarray(vb, dtype=np.float32)
+    sb = np.ascontiguousarray(scene_arrays["side_bits"], dtype=np.uint8)
+    ex = np.ascontiguousarray(scene_arrays["exitance"], dtype=np.float32)
+    lights = np.ascontiguousarray(scene_arrays["lights"], dtype=LIGHT_DTYPE)
+    tx = np.ascontiguousarray(texels, dtype=np.uint32)
+    sbs = np.ascontiguousarray(subs).view(np.float32)
+    nout = int(tx[:, 0].max()) + 1 if len(tx) else 0
+    outs = {k: np.zeros((max(nout, 1), 4), np.float64) for k in layers}
+    ptrs = (ctypes.c_void_p * 5)(*[(_p(outs[k]) if k in outs else None) for k in range(5)])
+    gp = GParams(params["quality"], params["hemisphere"], params["ray_length_min"], params["locality"],
+                 params["rug_distance"], params["light_multiplier"], params["seed"], params.get("light_rounds", 0))
+    gs = GStats()
+    if tree is not None:
+        nodes = np.ascontiguousarray(tree["nodes"], dtype=np.float32)
+        prim = np.ascontiguousarray(tree["prim"], dtype=np.uint32)
+        np_, pp_, root, nprim = _p(nodes), _p(prim), tree["root"], len(prim)
+    else:
+        np_, pp_, root, nprim = None, None, 0, 0
+    L.lsgo_gather_layers(_p(v), len(v), _p(i), len(i), _p(vn), _p(vt) if vt is not None else None,
+                         _p(vb) if vb is not None else None, _p(sb), _p(ex), _p(lights), len(lights), ctypes.byref(gp),
+                         _p(tx), len(tx), _p(sbs), np_, root, pp_, nprim, ptrs, ctypes.byref(gs), threads)
+    return outs
+
+
+def default_frames(normals):
+    """Per-vertex (tangent, bitangent) by TangentBasis::buildBasisFromNormal."""
+    L = lib()
+    P, U = ctypes.c_void_p, ctypes.c_uint32
+    L.lsgo_default_frames.argtypes = [P, U, P, P]
+    L.lsgo_default_frames.restype = None
+    n = np.ascontiguousarray(normals, dtype=np.float32).reshape(-1, 3)
+    t = np.zeros_like(n)
+    b = np.zeros_like(n)
+    L.lsgo_default_frames(_p(n), len(n), _p(t), _p(b))
+    return t, b
+
+
 DROP_IN_BIN = os.path.join(ORACLE_DIR, "_ref", "drop_in_test")
 
 
