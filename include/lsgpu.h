@@ -186,6 +186,9 @@ typedef struct lsg_gather_params {
     uint32_t seed;             /* seeded replacement of rand() (gather.h:102, gather.cpp:742) */
     uint32_t light_rounds;     /* 0: quality/10+1 as the reference; else shadow rounds per light
                                   (the learned path's cheap gather) */
+    float    environment_multiplier; /* indirect.environmentMultiplier: hemisphere rays that leave
+                                  the scene return environment * this (pathtracer.cpp:161-172);
+                                  0 = the environment is not sampled (gather.cpp:149) */
 } lsg_gather_params;
 
 int  lsg_scene_create(const lsg_collider* collider, const float* vertex_normals, const uint8_t* side_bits,
@@ -194,6 +197,17 @@ void lsg_scene_destroy(lsg_scene* scene);
 /* exitance: host float[ntri*8] = per triangle (front r,g,b,pad, back r,g,b,pad) */
 int  lsg_scene_set_exitance(lsg_scene* scene, const float* exitance);
 int  lsg_scene_set_lights(lsg_scene* scene, const lsg_light* lights, uint32_t nlights);
+
+/* The solver's environment (RRSolver::setEnvironment), which hemisphere rays that leave
+ * the scene sample (PathtracerWorker::getIncidentRadiance, pathtracer.cpp:161-172), with a
+ * nearest-texel lookup as RRBufferInMemory::getElementAtDirection
+ * (RRBufferInMemory.cpp:410-455).  rgb: host float[texels*3], linear (what
+ * RRBuffer::getElement returns with the solver's colour space).  A sampled environment
+ * also turns the hemisphere gather on (gather.cpp:197).  LSG_ENV_NONE clears it. */
+#define LSG_ENV_NONE      0
+#define LSG_ENV_CUBE      1   /* BT_CUBE_TEXTURE: 6 faces of width x width, +x,-x,+y,-y,+z,-z */
+#define LSG_ENV_PANORAMA  2   /* BT_2D_TEXTURE: 360 x 180 degree panorama, width x height */
+int  lsg_scene_set_environment(lsg_scene* scene, int32_t type, uint32_t width, uint32_t height, const float* rgb);
 
 /* The per-texel final gather, processTexel (RRSolver/gather.cpp:600-836) for every
  * texel at once.  texels/subs are device arrays; out_rgba is a device float4 array

@@ -63,7 +63,7 @@ class GatherParams:
     """RRSolver::UpdateParameters(quality) defaults (RRSolver.h:570-592)."""
 
     def __init__(self, quality=1024, hemisphere=1, ray_length_min=1e-6, locality=100000.0, rug_distance=0.001,
-                 light_multiplier=1.0, seed=1, light_rounds=0):
+                 light_multiplier=1.0, seed=1, light_rounds=0, environment_multiplier=0.0):
         self.quality = quality
         self.hemisphere = hemisphere
         self.ray_length_min = ray_length_min
@@ -72,15 +72,16 @@ class GatherParams:
         self.light_multiplier = light_multiplier
         self.seed = seed
         self.light_rounds = light_rounds      # 0: quality/10+1 (gather.cpp:396)
+        self.environment_multiplier = environment_multiplier   # indirect.environmentMultiplier
 
     def c(self):
         return N.GatherParams(self.quality, self.hemisphere, self.ray_length_min, self.locality, self.rug_distance,
-                              self.light_multiplier, self.seed, self.light_rounds)
+                              self.light_multiplier, self.seed, self.light_rounds, self.environment_multiplier)
 
     def as_dict(self):
         return dict(quality=self.quality, hemisphere=self.hemisphere, ray_length_min=self.ray_length_min,
                     locality=self.locality, rug_distance=self.rug_distance, light_multiplier=self.light_multiplier,
-                    seed=self.seed, light_rounds=self.light_rounds)
+                    seed=self.seed, light_rounds=self.light_rounds, environment_multiplier=self.environment_multiplier)
 
 
 class FilterParams:
@@ -129,6 +130,20 @@ class GatherScene:
         p = params.c()
         N.check(N.lib().lsg_gather_device(self._h, dtex.ptr, ntex, dsub.ptr, nsub, ctypes.byref(p), dout.ptr, stream),
                 "lsg_gather_device")
+
+    ENV_NONE, ENV_CUBE, ENV_PANORAMA = 0, 1, 2
+
+    def set_environment(self, kind=0, width=0, height=0, rgb=None):
+        """RRSolver::setEnvironment: kind ENV_CUBE (6 faces width x width, +x,-x,+y,-y,+z,-z)
+        or ENV_PANORAMA (width x height), rgb linear float texels; ENV_NONE clears it."""
+        if not kind:
+            N.check(N.lib().lsg_scene_set_environment(self._h, 0, 0, 0, None), "set_environment")
+            self.env = None
+            return
+        rgb = np.ascontiguousarray(rgb, np.float32).reshape(-1, 3)
+        N.check(N.lib().lsg_scene_set_environment(self._h, kind, width, height, rgb.ctypes.data_as(ctypes.c_void_p)),
+                "set_environment")
+        self.env = dict(type=kind, width=width, height=height, rgb=rgb)
 
     def set_tangents(self, tangents=None, bitangents=None):
         """Per-vertex tangent frames for directional lightmaps (RRMesh::TangentBasis);
@@ -180,6 +195,7 @@ class GatherScene:
     def oracle_arrays(self, exitance):
         return dict(verts=self.verts, idx=self.idx, normals=self.normals, side_bits=self.side_bits,
                     tangents=getattr(self, "tangents", None), bitangents=getattr(self, "bitangents", None),
+                    env=getattr(self, "env", None),
                     exitance=np.ascontiguousarray(exitance, np.float32).reshape(len(self.idx), 8),
                     lights=np.array([(l["type"], l["position"], l["direction"], l["color"], l["cast_shadows"])
                                      for l in self.lights], dtype=[("type", "<i4"), ("pos", "<f4", 3), ("dir", "<f4", 3),
@@ -273,8 +289,15 @@ class SceneBake:
         self.exitance = ex0
 
     def first_gather(self):
-        """Per-triangle direct irradiance -> exitance table (uploaded to the scene)."""
+        """Per-triangle direct irradiance -> exitance table (uploaded to the scene).  The
+        direct illumination includes the environment (updateSolverDirectIllumination
+        reads indirect.environmentMultiplier, gather.cpp:958-975)."""
         nt = self.scene.ntri
+        self.first_params.environment_multiplier = self.params.environment_multiplier
+        # hemisphere rays of the first gather see emitters only, not an earlier result
+        ex0 = np.zeros((nt, 8), np.float32)
+        ex0[:, 0:3] = self.scene.emittance
+        self.gs.set_exitance(ex0)
         self.gs.gather_device(self.d_ttex, nt, self.d_tsub, nt, self.first_params, self.d_tout)
         N.check(N.lib().lsg_synchronize(), "sync")
         e = self.d_tout.download(np.float32, (nt, 4))
@@ -388,6 +411,7 @@ class BakePlan:
         N.check(N.lib().lsg_bake_set_tuning(self._h, ctypes.byref(t)), "set_tuning")
 
     def run(self, first=True, graph=False, filtering=True, stream=None):
+        self.first_params.environment_multiplier = self.params.environment_multiplier
         fp = self.first_params.c()
         p = self.params.c()
         flt = self.filtering.c()
@@ -397,6 +421,7 @@ class BakePlan:
 
     def run_layers(self, layers=(0, 1, 2, 3, 4), first=True, filtering=True, stream=None):
         """The plan's bake with directional and bent-normal layers (lsg_bake_run_layers)."""
+        self.first_params.environment_multiplier = self.params.environment_multiplier
         fp = self.first_params.c()
         p = self.params.c()
         flt = self.filtering.c()

@@ -34,6 +34,17 @@ __global__ __launch_bounds__(256) void k_exitance(uint32_t nt, const float4* __r
     ex[2 * (size_t)t + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
+// exitance = emittance only, the light the first gather's hemisphere rays may see: the
+// first gather is the direct illumination (lights, environment, emitters), and a table
+// left over from an earlier run would add a bounce per run
+__global__ __launch_bounds__(256) void k_emittance_only(uint32_t nt, const float* __restrict__ emit, float4* __restrict__ ex)
+{
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= nt) return;
+    ex[2 * (size_t)t] = make_float4(emit[3 * (size_t)t], emit[3 * (size_t)t + 1], emit[3 * (size_t)t + 2], 0.f);
+    ex[2 * (size_t)t + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
 __global__ __launch_bounds__(256) void k_zero_f4(float4* p, uint32_t n)
 {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
@@ -102,6 +113,7 @@ static int run_first(BakeImpl& B, const lsg_gather_params* first, hipStream_t st
     GatherArgs A;
     int stack = 0, e;
     if ((e = gather_args(B.scene, B.d_ttex, B.ntri, B.d_tsub, first, (float*)B.d_irr, A, stack))) return e;
+    k_emittance_only<<<nblk(B.ntri), 256, 0, st>>>(B.ntri, B.d_emit, B.scene->impl.d_exit);
     if ((e = launch_gather(A, stack, st))) return e;
     k_exitance<<<nblk(B.ntri), 256, 0, st>>>(B.ntri, B.d_irr, B.d_albedo, B.d_emit, B.scene->impl.d_exit);
     return LSG_OK;
@@ -245,6 +257,13 @@ int lsg_bake_run(lsg_bake* b, const lsg_gather_params* first, const lsg_gather_p
     if (first) { memcpy(key.data(), first, sizeof *first); key[key.size() - 2] = 1; }
     memcpy(key.data() + sizeof(lsg_gather_params), final_params, sizeof *final_params);
     if (filtering) { memcpy(key.data() + 2 * sizeof(lsg_gather_params), filtering, sizeof *filtering); key[key.size() - 1] = 1; }
+    {
+        // the captured kernels hold the environment's buffer and shape as arguments
+        const SceneImpl& S = B.scene->impl;
+        const uintptr_t env[4] = {(uintptr_t)S.d_env, S.env_type, S.env_w, S.env_h};
+        const uint8_t* e8 = (const uint8_t*)env;
+        key.insert(key.end(), e8, e8 + sizeof env);
+    }
     if (key != B.graph_key || !B.exec) {
         // an eager run fixes the number of grow steps the graph will replay
         uint32_t used = 0;

@@ -15,6 +15,8 @@ struct SceneView {
     const float4* exitance;      // 2 per triangle: front, back
     const float* vtangent;       // 3 floats per vertex (output layers only)
     const float* vbitangent;     // 3 floats per vertex (output layers only)
+    const float4* env;           // environment texels, linear rgb, or null
+    uint32_t env_type, env_w, env_h;   // LSG_ENV_*; env_type 0 = rays that leave return 0
 };
 
 struct GatherArgs {
@@ -27,6 +29,7 @@ struct GatherArgs {
     uint32_t quality; uint32_t hemi_rays; uint32_t rounds;
     float tmin, locality, rug, light_mult; uint32_t seed;
     const lsg_light* lights; uint32_t nlights;
+    float env_mult;              // environment multiplier (with sc.env_type != 0)
     // output layers (LS_LIGHTMAP, LS_DIRECTION1..3, LS_BENT_NORMALS; null = not wanted),
     // used by the layered kernels only; dirs = gatherAllDirections (lightmap.cpp:820)
     float4* layer[5]; uint32_t dirs; uint32_t layered;
@@ -51,6 +54,8 @@ struct SceneImpl {
     uint32_t nlights = 0;
     float* d_vtan = nullptr;           // per-vertex tangent frames, built on first use
     float* d_vbit = nullptr;
+    float4* d_env = nullptr;           // environment texels, linear rgb
+    uint32_t env_type = 0, env_w = 0, env_h = 0;
 };
 
 // per-vertex tangent frames of the scene: the caller's, or built from the vertex

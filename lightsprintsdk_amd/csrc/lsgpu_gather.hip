@@ -185,6 +185,75 @@ __device__ inline void shooter_at(const GatherArgs& A, const lsg_subtexel* ts, u
                     v0.z + (v1.z - v0.z) * ux + (v2.z - v0.z) * uy);
 }
 
+// asin in binary32 by Abramowitz & Stegun 4.4.46 (|error| <= 2e-8 in exact arithmetic):
+// asin(x) = pi/2 - sqrt(1-x) * (a0 + a1 x + ... + a7 x^7) for 0 <= x <= 1, odd.  Plain
+// float multiplies and adds in a fixed order, so the GPU and the CPU statement round
+// alike.
+__device__ inline float asin_as(float x)
+{
+    const float ax = fminf(fabsf(x), 1.0f);      // |x| may round past 1 near the poles
+    float p = -0.0012624911f;
+    p = p * ax + 0.0066700901f;
+    p = p * ax - 0.0170881256f;
+    p = p * ax + 0.0308918810f;
+    p = p * ax - 0.0501743046f;
+    p = p * ax + 0.0889789874f;
+    p = p * ax - 0.2145988016f;
+    p = p * ax + 1.5707963050f;
+    const float r = 1.57079632679489662f - sqrtf(1.0f - ax) * p;
+    return x < 0 ? -r : r;
+}
+
+// Nearest environment texel in direction d, RRBufferInMemory::getElementAtDirection
+// (RRBufferInMemory.cpp:410-455).  Cube maps follow the OpenGL face convention: major
+// axis m picks the face (+x,-x,+y,-y,+z,-z; ties go to x, then y), and the face
+// coordinates are (sc/|m|+1)/2, (tc/|m|+1)/2 with
+//   +x: sc=-z tc=-y   -x: sc=+z tc=-y   +y: sc=+x tc=+z
+//   -y: sc=+x tc=-z   +z: sc=+x tc=-y   -z: sc=-x tc=-y.
+// Panoramas map the yaw of (x, z) to columns and asin(y) to rows, in float as the
+// reference, with asin_as for asin.
+__device__ inline float4 env_at(const SceneView& S, const float3& d)
+{
+    uint32_t idx;
+    const uint32_t w = S.env_w, h = S.env_h;
+    if (S.env_type == LSG_ENV_CUBE) {
+        const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+        const uint32_t axis = (ax >= ay && ax >= az) ? 0u : ((ay >= ax && ay >= az) ? 1u : 2u);
+        const float m = axis == 0 ? d.x : (axis == 1 ? d.y : d.z);
+        const uint32_t face = 2 * axis + (m < 0 ? 1u : 0u);
+        const float am = fabsf(m);
+        float sc, tc;
+        switch (face) {
+            case 0: sc = -d.z; tc = -d.y; break;
+            case 1: sc = d.z; tc = -d.y; break;
+            case 2: sc = d.x; tc = d.z; break;
+            case 3: sc = d.x; tc = -d.z; break;
+            case 4: sc = d.x; tc = -d.y; break;
+            default: sc = -d.x; tc = -d.y; break;
+        }
+        int c0 = (int)((sc / am + 1) * (0.5f * (float)w));
+        int c1 = (int)((tc / am + 1) * (0.5f * (float)w));
+        c0 = c0 < 0 ? 0 : (c0 > (int)w - 1 ? (int)w - 1 : c0);
+        c1 = c1 < 0 ? 0 : (c1 > (int)w - 1 ? (int)w - 1 : c1);
+        idx = (uint32_t)c0 + (uint32_t)c1 * w + face * w * w;
+    } else {
+        const float pi = 3.14159265358979323846f;
+        const float d2 = d.x * d.x + d.z * d.z;
+        uint32_t c0 = 0;
+        if (d2 != 0.f) {
+            const float s = d.x / sqrtf(d2);
+            float angle = asin_as(s);
+            if (d.z < 0) angle = pi - angle;
+            c0 = (uint32_t)fmaxf(0.f, (angle * (-0.5f / pi) + 0.75f) * (float)w) % w;   // >= 0 for the cast
+        }
+        const float len = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
+        const float el = asin_as(d.y / len);
+        const uint32_t c1 = (uint32_t)fmaxf(0.f, (el * (1.0f / pi) + 0.5f) * (float)h) % h;
+        idx = c0 + c1 * w;
+    }
+    return S.env[idx];
+}
+
 // g_lightmapDirections[i] (gather.cpp:43-49) in the skewed frame (t, b, n), normalised
 // (gather.cpp:292-293, 458-459)
 __device__ inline float3 layer_dir(int i, const float3& t, const float3& b, const float3& n)
@@ -215,7 +284,10 @@ __device__ inline float3 layer_dir(int i, const float3& t, const float3& b, cons
 // with A.dirs the three directional lightmaps weighted per layer direction
 // (gather.cpp:284-305 hemisphere, :453-481 lights), and the bent normal
 // (gather.cpp:306, :482, :809-830), into A.layer[0..4].
-template <int STACK, bool STATS, int G = 64, bool LAYERS = false>
+//
+// ENV: hemisphere rays that leave the scene sample the environment (a separate
+// instantiation, so that scenes without one keep the smaller register footprint).
+template <int STACK, bool STATS, int G = 64, bool LAYERS = false, bool ENV = false>
 __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, int32_t* mystk, uint32_t* lq, int stride)
 {
     static_assert(G == 16 || G == 32 || G == 64, "group width");
@@ -399,8 +471,9 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
                 uint32_t nv_ = 0, nt_ = 0;
                 trace_closest<STACK, STATS>(flt, A.sc.col, o, d, A.tmin, A.locality, mystk, stride, r, nv_, nt_);
                 if (STATS) { stH[0]++; stH[1] += nv_; stH[2] += nt_; }
-                if (r.tri >= 0) {
-                    const float4 e = A.sc.exitance[2 * (size_t)r.tri + (r.front ? 0 : 1)];
+                // a hit reads the exitance of the side it sees; a ray that leaves the scene
+                // returns environment * multiplier, or nothing (pathtracer.cpp:156-173)
+                auto add = [&](const float4& e) {
                     if constexpr (LAYERS) {
                         if (dirs) {
                             // gather.cpp:284-305: dirs near the normal are picked more often,
@@ -424,6 +497,11 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
                     } else {
                         hr += e.x; hg += e.y; hb += e.z;
                     }
+                };
+                if (r.tri >= 0) add(A.sc.exitance[2 * (size_t)r.tri + (r.front ? 0 : 1)]);
+                else if constexpr (ENV) {
+                    const float4 v = env_at(A.sc, d);
+                    add(make_float4(v.x * A.env_mult, v.y * A.env_mult, v.z * A.env_mult, 0.f));
                 }
                 cRelH++;
             }
@@ -495,7 +573,7 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
     }
 }
 
-template <int STACK, bool STATS, bool LAYERS = false>
+template <int STACK, bool STATS, bool LAYERS = false, bool ENV = false>
 __global__ __launch_bounds__(256) void k_gather(GatherArgs A)
 {
     __shared__ int32_t stk[STACK * 256];
@@ -503,11 +581,11 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs A)
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const uint32_t ti = blockIdx.x * 4u + (tid >> 6);
     if (ti >= A.ntex) return;                 // whole wavefront leaves together
-    gather_texel<STACK, STATS, 64, LAYERS>(A, ti, lane, stk + tid, lq + tid, 256);
+    gather_texel<STACK, STATS, 64, LAYERS, ENV>(A, ti, lane, stk + tid, lq + tid, 256);
 }
 
 // low ray counts (at most G shooters per series): 256/G texels per block, G lanes each
-template <int STACK, int G, bool LAYERS = false>
+template <int STACK, int G, bool LAYERS = false, bool ENV = false>
 __global__ __launch_bounds__(256) void k_gather_grouped(GatherArgs A)
 {
     __shared__ int32_t stk[STACK * 256];
@@ -515,7 +593,7 @@ __global__ __launch_bounds__(256) void k_gather_grouped(GatherArgs A)
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const uint32_t ti = blockIdx.x * (256u / G) + tid / G;
     if (ti >= A.ntex) return;                 // the whole group leaves together
-    gather_texel<STACK, false, G, LAYERS>(A, ti, lane, stk + tid, lq + tid, 256);
+    gather_texel<STACK, false, G, LAYERS, ENV>(A, ti, lane, stk + tid, lq + tid, 256);
 }
 
 __global__ void k_queue_reset(uint32_t* counters, uint32_t n)
@@ -523,7 +601,7 @@ __global__ void k_queue_reset(uint32_t* counters, uint32_t n)
     if (threadIdx.x < n) counters[threadIdx.x] = 0;
 }
 
-template <int STACK, int WAVES>
+template <int STACK, int WAVES, bool ENV = false>
 __global__ __launch_bounds__(64 * WAVES) void k_gather_persistent(GatherArgs A, WorkQueue Q)
 {
     __shared__ int32_t stk[STACK * 64 * WAVES];
@@ -542,7 +620,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_gather_persistent(GatherArgs A, 
             if (tile >= t1) break;           // this queue is exhausted for every lane of the wave
             const uint32_t a = tile * Q.tile, b = min(A.ntex, a + Q.tile);
             for (uint32_t ti = a; ti < b; ti++)
-                gather_texel<STACK, false>(A, ti, lane, stk + tid, lq + tid, 64 * WAVES);
+                gather_texel<STACK, false, 64, false, ENV>(A, ti, lane, stk + tid, lq + tid, 64 * WAVES);
         }
     }
 }
@@ -599,11 +677,16 @@ int gather_args(lsg_scene* s, const lsg_texel* texels, uint32_t ntexels, const l
     A.sc.verts = C.d_verts; A.sc.idx = C.d_idx;
     A.sc.vnormal = s->impl.d_vnormal; A.sc.side = s->impl.d_side; A.sc.exitance = s->impl.d_exit;
     A.sc.vtangent = s->impl.d_vtan; A.sc.vbitangent = s->impl.d_vbit;
+    // the environment is sampled only with a nonzero multiplier (gather.cpp:149)
+    const bool env = s->impl.env_type != LSG_ENV_NONE && p->environment_multiplier != 0;
+    A.sc.env = s->impl.d_env; A.sc.env_type = env ? s->impl.env_type : 0u;
+    A.sc.env_w = s->impl.env_w; A.sc.env_h = s->impl.env_h;
+    A.env_mult = p->environment_multiplier;
     for (int i = 0; i < 5; i++) A.layer[i] = nullptr;
     A.dirs = 0; A.layered = 0;
     A.texels = texels; A.ntex = ntexels; A.subs = subs; A.out = (float4*)out_rgba; A.stats = nullptr; A.parts = nullptr;
     A.quality = p->quality;
-    A.hemi_rays = p->hemisphere ? (p->quality > 1 ? p->quality : 1) : 0;             // gather.cpp:197
+    A.hemi_rays = (p->hemisphere || env) ? (p->quality > 1 ? p->quality : 1) : 0;    // gather.cpp:197
     A.rounds = (p->light_multiplier != 0 && s->impl.nlights) ? (p->light_rounds ? p->light_rounds : p->quality / 10 + 1) : 0; // gather.cpp:396
     A.tmin = p->ray_length_min; A.locality = p->locality; A.rug = p->rug_distance;
     A.light_mult = p->light_multiplier; A.seed = p->seed;
@@ -611,20 +694,28 @@ int gather_args(lsg_scene* s, const lsg_texel* texels, uint32_t ntexels, const l
     return LSG_OK;
 }
 
+template <bool ENV>
+static void launch_persistent(const GatherArgs& A, int stack, const WorkQueue& Q, uint32_t waves_per_block, uint32_t blocks,
+                              hipStream_t st)
+{
+    if (waves_per_block == 8) {
+        if (stack <= 32) k_gather_persistent<32, 8, ENV><<<blocks, 512, 0, st>>>(A, Q);
+        else k_gather_persistent<64, 8, ENV><<<blocks, 512, 0, st>>>(A, Q);
+    } else if (waves_per_block == 2) {
+        if (stack <= 32) k_gather_persistent<32, 2, ENV><<<blocks, 128, 0, st>>>(A, Q);
+        else k_gather_persistent<64, 2, ENV><<<blocks, 128, 0, st>>>(A, Q);
+    } else {
+        if (stack <= 32) k_gather_persistent<32, 4, ENV><<<blocks, 256, 0, st>>>(A, Q);
+        else k_gather_persistent<64, 4, ENV><<<blocks, 256, 0, st>>>(A, Q);
+    }
+}
+
 int launch_gather_persistent(const GatherArgs& A, int stack, const WorkQueue& Q, uint32_t waves_per_block, uint32_t blocks,
                              hipStream_t st)
 {
     k_queue_reset<<<1, 64, 0, st>>>(Q.counters, Q.nqueues);
-    if (waves_per_block == 8) {
-        if (stack <= 32) k_gather_persistent<32, 8><<<blocks, 512, 0, st>>>(A, Q);
-        else k_gather_persistent<64, 8><<<blocks, 512, 0, st>>>(A, Q);
-    } else if (waves_per_block == 2) {
-        if (stack <= 32) k_gather_persistent<32, 2><<<blocks, 128, 0, st>>>(A, Q);
-        else k_gather_persistent<64, 2><<<blocks, 128, 0, st>>>(A, Q);
-    } else {
-        if (stack <= 32) k_gather_persistent<32, 4><<<blocks, 256, 0, st>>>(A, Q);
-        else k_gather_persistent<64, 4><<<blocks, 256, 0, st>>>(A, Q);
-    }
+    if (A.sc.env_type) launch_persistent<true>(A, stack, Q, waves_per_block, blocks, st);
+    else launch_persistent<false>(A, stack, Q, waves_per_block, blocks, st);
     LSG_HIP(hipGetLastError());
     return LSG_OK;
 }
@@ -648,7 +739,7 @@ int launch_gather(const GatherArgs& A, int stack, hipStream_t st)
     return LSG_OK;
 }
 
-template <bool LAYERS>
+template <bool LAYERS, bool ENV>
 static void launch_gather_kernels(const GatherArgs& A, int stack, hipStream_t st)
 {
     // shooters per series = max(hemisphere rays, light rounds); a group of G >= that
@@ -658,24 +749,25 @@ static void launch_gather_kernels(const GatherArgs& A, int stack, hipStream_t st
         const uint32_t G = series <= 16 ? 16u : 32u;
         const unsigned grid = (unsigned)(((uint64_t)A.ntex * G + 255) / 256);
         if (G == 16) {
-            if (stack <= 32) k_gather_grouped<32, 16, LAYERS><<<grid, 256, 0, st>>>(A);
-            else k_gather_grouped<64, 16, LAYERS><<<grid, 256, 0, st>>>(A);
+            if (stack <= 32) k_gather_grouped<32, 16, LAYERS, ENV><<<grid, 256, 0, st>>>(A);
+            else k_gather_grouped<64, 16, LAYERS, ENV><<<grid, 256, 0, st>>>(A);
         } else {
-            if (stack <= 32) k_gather_grouped<32, 32, LAYERS><<<grid, 256, 0, st>>>(A);
-            else k_gather_grouped<64, 32, LAYERS><<<grid, 256, 0, st>>>(A);
+            if (stack <= 32) k_gather_grouped<32, 32, LAYERS, ENV><<<grid, 256, 0, st>>>(A);
+            else k_gather_grouped<64, 32, LAYERS, ENV><<<grid, 256, 0, st>>>(A);
         }
         return;
     }
     const unsigned grid = (A.ntex + 3) / 4;
-    if (stack <= 32) k_gather<32, false, LAYERS><<<grid, 256, 0, st>>>(A);
-    else k_gather<64, false, LAYERS><<<grid, 256, 0, st>>>(A);
+    if (stack <= 32) k_gather<32, false, LAYERS, ENV><<<grid, 256, 0, st>>>(A);
+    else k_gather<64, false, LAYERS, ENV><<<grid, 256, 0, st>>>(A);
 }
 
 static int launch_gather_one(const GatherArgs& A, int stack, hipStream_t st)
 {
     if (!A.ntex) return LSG_OK;
-    if (A.layered) launch_gather_kernels<true>(A, stack, st);
-    else launch_gather_kernels<false>(A, stack, st);
+    const bool env = A.sc.env_type != 0;
+    if (A.layered) { if (env) launch_gather_kernels<true, true>(A, stack, st); else launch_gather_kernels<true, false>(A, stack, st); }
+    else { if (env) launch_gather_kernels<false, true>(A, stack, st); else launch_gather_kernels<false, false>(A, stack, st); }
     LSG_HIP(hipGetLastError());
     return LSG_OK;
 }
@@ -736,8 +828,32 @@ void lsg_scene_destroy(lsg_scene* s)
 {
     if (!s) return;
     dfree(s->impl.d_vnormal); dfree(s->impl.d_side); dfree(s->impl.d_exit); dfree(s->impl.d_lights);
-    dfree(s->impl.d_vtan); dfree(s->impl.d_vbit);
+    dfree(s->impl.d_vtan); dfree(s->impl.d_vbit); dfree(s->impl.d_env);
     delete s;
+}
+
+int lsg_scene_set_environment(lsg_scene* s, int32_t type, uint32_t width, uint32_t height, const float* rgb)
+{
+    if (!s) return fail(LSG_ERR_INVALID, "lsg_scene_set_environment: null scene");
+    SceneImpl& S = s->impl;
+    if (type == LSG_ENV_NONE) {
+        dfree(S.d_env); S.d_env = nullptr; S.env_type = 0; S.env_w = S.env_h = 0;
+        return LSG_OK;
+    }
+    if (type != LSG_ENV_CUBE && type != LSG_ENV_PANORAMA) return fail(LSG_ERR_INVALID, "lsg_scene_set_environment: type %d", type);
+    if (!rgb || !width || !height) return fail(LSG_ERR_INVALID, "lsg_scene_set_environment: empty environment");
+    if (type == LSG_ENV_CUBE && width != height) return fail(LSG_ERR_INVALID, "lsg_scene_set_environment: cube faces must be square");
+    const uint64_t n = (uint64_t)width * height * (type == LSG_ENV_CUBE ? 6u : 1u);
+    if (n > (1ull << 28)) return fail(LSG_ERR_LIMIT, "lsg_scene_set_environment: %llu texels", (unsigned long long)n);
+    std::vector<float4> t((size_t)n);
+    for (size_t i = 0; i < (size_t)n; i++) t[i] = make_float4(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2], 0.f);
+    float4* d = nullptr;
+    LSG_HIP(hipMalloc(&d, 16 * (size_t)n));
+    hipError_t e = hipMemcpy(d, t.data(), 16 * (size_t)n, hipMemcpyHostToDevice);
+    if (e != hipSuccess) { dfree(d); return hip_fail(e, "h2d", __LINE__); }
+    dfree(S.d_env);
+    S.d_env = d; S.env_type = (uint32_t)type; S.env_w = width; S.env_h = height;
+    return LSG_OK;
 }
 
 int lsg_scene_set_tangents(lsg_scene* s, const float* tangents, const float* bitangents)
@@ -796,8 +912,14 @@ static int gather_launch(lsg_scene* s, const lsg_texel* texels, uint32_t ntexels
     if (!stats) return launch_gather(A, stack, (hipStream_t)stream);
     A.stats = stats;
     const unsigned grid = (ntexels + 3) / 4;
-    if (stack <= 32) k_gather<32, true><<<grid, 256, 0, (hipStream_t)stream>>>(A);
-    else k_gather<64, true><<<grid, 256, 0, (hipStream_t)stream>>>(A);
+    hipStream_t hs = (hipStream_t)stream;
+    if (A.sc.env_type) {
+        if (stack <= 32) k_gather<32, true, false, true><<<grid, 256, 0, hs>>>(A);
+        else k_gather<64, true, false, true><<<grid, 256, 0, hs>>>(A);
+    } else {
+        if (stack <= 32) k_gather<32, true><<<grid, 256, 0, hs>>>(A);
+        else k_gather<64, true><<<grid, 256, 0, hs>>>(A);
+    }
     LSG_HIP(hipGetLastError());
     return LSG_OK;
 }

@@ -129,7 +129,39 @@ bool RRSolverHIP::buildScene(const UpdateParameters& params)
 		releaseScene();
 		return false;
 	}
+	// the environment that gather rays leaving the scene sample (gather.cpp:149,
+	// pathtracer.cpp:161-172): its texels in linear colour, as getElementAtDirection
+	// reads them with the solver's colour space
+	const RRBuffer* env = params.indirect.environmentMultiplier ? getEnvironment(0) : nullptr;
+	if (env)
+	{
+		const bool cube = env->getType() == BT_CUBE_TEXTURE;
+		const unsigned w = env->getWidth(), h = env->getHeight();
+		const size_t n = (size_t)w * h * (cube ? 6 : 1);
+		std::vector<float> rgb(3 * n);
+		for (size_t i = 0; i < n; i++)
+		{
+			const RRVec4 c = env->getElement((unsigned)i, getColorSpace());
+			for (int k = 0; k < 3; k++) rgb[3 * i + k] = c[k];
+		}
+		if (lsg_scene_set_environment(scene, cube ? LSG_ENV_CUBE : LSG_ENV_PANORAMA, w, h, rgb.data()) != LSG_OK)
+		{
+			RRReporter::report(ERRO, "RRSolverHIP: %s\n", lsg_last_error());
+			releaseScene();
+			return false;
+		}
+	}
 	return true;
+}
+
+// environments the GPU samples: none, or one cube map or panorama without rotation or
+// blending (RRBuffer::createEnvironmentBlend returns it unchanged then, RRBufferBlend.cpp:21-29)
+bool RRSolverHIP::environmentSupported(const UpdateParameters& params) const
+{
+	RRReal angle0 = 0;
+	const RRBuffer* env = getEnvironment(0, &angle0);
+	if (!env || params.indirect.environmentMultiplier == 0) return true;
+	return !getEnvironment(1) && angle0 == 0 && (env->getType() == BT_CUBE_TEXTURE || env->getType() == BT_2D_TEXTURE);
 }
 
 // Texels of the targets' objects in multiObject triangle numbering, as
@@ -191,6 +223,7 @@ unsigned RRSolverHIP::bake(const Target* targets, unsigned numTargets, const Upd
 	g.rug_distance = params.rugDistance;
 	g.light_multiplier = params.direct.lightMultiplier;
 	g.seed = 1;
+	g.environment_multiplier = params.indirect.environmentMultiplier;
 	lsg_filter_params f = { 1.f, 1000.f, {0, 0, 0, 0}, 0 };
 	if (filtering)
 	{
@@ -230,13 +263,12 @@ unsigned RRSolverHIP::bake(const Target* targets, unsigned numTargets, const Upd
 unsigned RRSolverHIP::updateLightmaps(int layerLightmap, int layerDirectionalLightmap, int layerBentNormals,
                                       const UpdateParameters* params, const FilteringParameters* filtering)
 {
-	const bool supported = params && params->quality > 0
-	                    && (!getEnvironment() || params->direct.environmentMultiplier == 0) && lsg_device_count() > 0;
+	const bool supported = params && params->quality > 0 && environmentSupported(*params) && lsg_device_count() > 0;
 	if (!supported)
 	{
 		if (referenceFallback) return RRSolver::updateLightmaps(layerLightmap, layerDirectionalLightmap, layerBentNormals, params, filtering);
 		RRReporter::report(ERRO, "RRSolverHIP::updateLightmaps: unsupported request (needs quality>0, "
-		                         "no environment, a gfx950 device).\n");
+		                         "no rotated or blended environment, a gfx950 device).\n");
 		return 0;
 	}
 	// indirect solution first, by the reference's own solver (RRSolver.h:731-734)
@@ -302,7 +334,7 @@ unsigned RRSolverHIP::updateLightmap(int objectNumber, RRBuffer* lightmap, RRBuf
 	}
 	const bool supported = params && params->quality > 0 && any && flat
 	                    && objectNumber >= 0 && (unsigned)objectNumber < objects.size()
-	                    && (!getEnvironment() || params->direct.environmentMultiplier == 0) && lsg_device_count() > 0;
+	                    && environmentSupported(*params) && lsg_device_count() > 0;
 	if (!supported)
 	{
 		if (referenceFallback) return RRSolver::updateLightmap(objectNumber, lightmap, directionalLightmap, bentNormals, params, filtering);

@@ -20,9 +20,10 @@
 // way, each layer filtered on its own.
 //
 // Supported: per-pixel buffers (BT_2D_TEXTURE) in any of the five layers, point
-// (REALISTIC attenuation) and directional lights, Lambertian triangle materials, no
-// environment.  Anything else is refused with an RRReporter error and 0 buffers
-// updated; setReferenceFallback(true) hands such calls to the reference RRSolver.
+// (REALISTIC attenuation) and directional lights, Lambertian triangle materials, and
+// an environment that is one cube map or panorama without rotation or blending.
+// Anything else is refused with an RRReporter error and 0 buffers updated;
+// setReferenceFallback(true) hands such calls to the reference RRSolver.
 #pragma once
 #include "Lightsprint/RRSolver.h"
 #include "lsgpu.h"
@@ -53,6 +54,7 @@ private:
 	unsigned bake(const Target* targets, unsigned numTargets, const UpdateParameters& params,
 	              const FilteringParameters* filtering);
 	bool buildScene(const UpdateParameters& params);
+	bool environmentSupported(const UpdateParameters& params) const;
 	void releaseScene();
 
 	bool referenceFallback = false;

@@ -157,7 +157,8 @@ class LearnedScene:
         p = self.sb.params
         self.cheap = GatherParams(quality=CHEAP_QUALITY, hemisphere=p.hemisphere, ray_length_min=p.ray_length_min,
                                   locality=p.locality, rug_distance=p.rug_distance, light_multiplier=p.light_multiplier,
-                                  seed=p.seed ^ 0x2545F491, light_rounds=CHEAP_LIGHT_ROUNDS)
+                                  seed=p.seed ^ 0x2545F491, light_rounds=CHEAP_LIGHT_ROUNDS,
+                                  environment_multiplier=p.environment_multiplier)
         n = len(self.sb.texels)
         self.n = n
         self.d_parts = DeviceBuffer(32 * max(n, 1))

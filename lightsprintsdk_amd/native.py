@@ -40,7 +40,7 @@ class Light(ctypes.Structure):
 class GatherParams(ctypes.Structure):
     _fields_ = [("quality", U32), ("hemisphere", I32), ("ray_length_min", ctypes.c_float),
                 ("locality", ctypes.c_float), ("rug_distance", ctypes.c_float), ("light_multiplier", ctypes.c_float),
-                ("seed", U32), ("light_rounds", U32)]
+                ("seed", U32), ("light_rounds", U32), ("environment_multiplier", ctypes.c_float)]
 
 
 class Lightmap(ctypes.Structure):
@@ -102,6 +102,7 @@ SIGNATURES = {
     "lsg_gather_device": (ctypes.c_int, [P, P, U32, P, U32, ctypes.POINTER(GatherParams), P, P]),
     "lsg_gather_stats": (ctypes.c_int, [P, P, U32, P, U32, ctypes.POINTER(GatherParams), P, ctypes.POINTER(ctypes.c_double)]),
     "lsg_scene_set_tangents": (ctypes.c_int, [P, P, P]),
+    "lsg_scene_set_environment": (ctypes.c_int, [P, I32, U32, U32, P]),
     "lsg_gather_layers_device": (ctypes.c_int, [P, P, U32, P, U32, ctypes.POINTER(GatherParams), P, P]),
     "lsg_filter_lightmaps": (ctypes.c_int, [P, U32, P, P, ctypes.POINTER(FilterParams), P]),
     "lsg_bake_create": (ctypes.c_int, [P, P, U32, P, P, ctypes.POINTER(P)]),

@@ -45,7 +45,16 @@ typedef struct lsgo_gparams {
     float light_multiplier;    /* direct.lightMultiplier */
     uint32_t seed;
     uint32_t light_rounds;     /* 0: quality/10+1 (gather.cpp:396); else this many rounds */
+    float environment_multiplier; /* indirect.environmentMultiplier; 0 = environment not sampled */
 } lsgo_gparams;
+
+/* RRSolver's environment: 1 = cube map (6 faces w x w: +x,-x,+y,-y,+z,-z), 2 = 360x180
+ * panorama (w x h); rgb: linear float[texels*3]. */
+typedef struct lsgo_env {
+    int32_t type;
+    uint32_t w, h;
+    const float* rgb;
+} lsgo_env;
 
 typedef struct lsgo_gstats {
     double hemi_rays, shadow_rays, series;
@@ -191,6 +200,64 @@ static void subtexel_basis(const float* vn, const uint32_t* tidx, const float* s
     bt[2] = n[0] * tg[1] - n[1] * tg[0];
 }
 
+/* asin in binary32 by Abramowitz & Stegun 4.4.46 (|error| <= 2e-8 in exact arithmetic):
+ * asin(x) = pi/2 - sqrt(1-x) * (a0 + a1 x + ... + a7 x^7) for 0 <= x <= 1, odd.  Plain
+ * float multiplies and adds in a fixed order, so the GPU and the CPU statement round
+ * alike. */
+static float asin_as(float x)
+{
+    const float ax = fminf(fabsf(x), 1.0f);      // |x| may round past 1 near the poles
+    float p = -0.0012624911f;
+    p = p * ax + 0.0066700901f;
+    p = p * ax - 0.0170881256f;
+    p = p * ax + 0.0308918810f;
+    p = p * ax - 0.0501743046f;
+    p = p * ax + 0.0889789874f;
+    p = p * ax - 0.2145988016f;
+    p = p * ax + 1.5707963050f;
+    const float r = 1.57079632679489662f - sqrtf(1.0f - ax) * p;
+    return x < 0 ? -r : r;
+}
+
+/* RRBufferInMemory::getElementAtDirection (RRBufferInMemory.cpp:410-455): index of the
+ * nearest texel in direction d.  Cube faces per the OpenGL cube-map table (major axis m,
+ * face coordinates (sc/|m|+1)/2 and (tc/|m|+1)/2); panoramas by yaw of (x, z) and
+ * asin(y), in float as the reference, with asin_as for asin. */
+uint32_t lsgo_env_index(const lsgo_env* env, const float* d)
+{
+    const uint32_t w = env->w, h = env->h;
+    if (env->type == 1) {
+        float ax = fabsf(d[0]), ay = fabsf(d[1]), az = fabsf(d[2]);
+        uint32_t axis = (ax >= ay && ax >= az) ? 0 : ((ay >= ax && ay >= az) ? 1 : 2);
+        float m = d[axis];
+        uint32_t face = 2 * axis + (m < 0 ? 1 : 0);
+        float am = fabsf(m);
+        /* (sc, tc) per face: +x (-z,-y), -x (+z,-y), +y (+x,+z), -y (+x,-z), +z (+x,-y), -z (-x,-y) */
+        static const int sc_axis[6] = {2, 2, 0, 0, 0, 0}, tc_axis[6] = {1, 1, 2, 2, 1, 1};
+        static const float sc_sign[6] = {-1, 1, 1, 1, 1, -1}, tc_sign[6] = {-1, -1, 1, -1, -1, -1};
+        float sc = sc_sign[face] * d[sc_axis[face]];
+        float tc = tc_sign[face] * d[tc_axis[face]];
+        int c0 = (int)((sc / am + 1) * (0.5f * (float)w));
+        int c1 = (int)((tc / am + 1) * (0.5f * (float)w));
+        c0 = c0 < 0 ? 0 : (c0 > (int)w - 1 ? (int)w - 1 : c0);
+        c1 = c1 < 0 ? 0 : (c1 > (int)w - 1 ? (int)w - 1 : c1);
+        return (uint32_t)c0 + (uint32_t)c1 * w + face * w * w;
+    }
+    const float pi = 3.14159265358979323846f;
+    float d2 = d[0] * d[0] + d[2] * d[2];
+    uint32_t c0 = 0;
+    if (d2 != 0.f) {
+        float s = d[0] / sqrtf(d2);
+        float angle = asin_as(s);
+        if (d[2] < 0) angle = pi - angle;
+        c0 = (uint32_t)fmaxf(0.f, (angle * (-0.5f / pi) + 0.75f) * (float)w) % w;   /* >= 0 for the cast */
+    }
+    float len = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    float el = asin_as(d[1] / len);
+    uint32_t c1 = (uint32_t)fmaxf(0.f, (el * (1.0f / pi) + 0.5f) * (float)h) % h;
+    return c0 + c1 * w;
+}
+
 /* Skewed-normalised tangent and bitangent at the subtexel centroid: the normalised
  * barycentric blend of the vertex frames (gather.cpp:731-732). */
 static void skewed_frame(const float* vt, const float* vb, const uint32_t* tidx, const float* sub, float* tg, float* bg)
@@ -264,9 +331,11 @@ static void gather_core(const float* verts, uint32_t nverts, const uint32_t* idx
                         const float* exitance, const lsgo_light* lights, uint32_t nlights, const lsgo_gparams* p,
                         const uint32_t* texels, uint32_t ntex, const float* subs,
                         const float* nodes, int32_t root_ref, const uint32_t* prim, uint32_t nprim,
-                        double* out_rgba, double* const* layers, lsgo_gstats* stats, int threads)
+                        const lsgo_env* env_in, double* out_rgba, double* const* layers, lsgo_gstats* stats, int threads)
 {
     const int dirs = layers && (layers[1] || layers[2] || layers[3]);
+    /* the environment is sampled only with a nonzero multiplier (gather.cpp:149) */
+    const lsgo_env* env = (env_in && env_in->type && p->environment_multiplier != 0) ? env_in : NULL;
     scene_t sc;
     sc.verts = verts; sc.idx = idx; sc.ntri = ntri; sc.side_mask = side_mask;
     sc.nodes = nodes; sc.root_ref = root_ref; sc.prim = prim; sc.nprim = nprim;
@@ -278,7 +347,7 @@ static void gather_core(const float* verts, uint32_t nverts, const uint32_t* idx
     lsgo_box_init(sc.bmn, sc.bmx);
 
     const uint32_t q = p->quality;
-    const uint32_t hemi_rays = p->hemisphere ? (q > 1 ? q : 1) : 0;                 /* gather.cpp:197 */
+    const uint32_t hemi_rays = (p->hemisphere || env) ? (q > 1 ? q : 1) : 0;        /* gather.cpp:197 */
     const uint32_t rounds = (p->light_multiplier != 0 && nlights) ? (p->light_rounds ? p->light_rounds : q / 10 + 1) : 0;  /* gather.cpp:396 */
     const uint32_t light_rays = nlights * rounds;
     double nh = 0, ns = 0, nser = 0;
@@ -345,8 +414,17 @@ static void gather_core(const float* verts, uint32_t nverts, const uint32_t* idx
                     float th; int fr;
                     int hit = trace(&sc, 0, o, d, p->ray_length_min, p->locality, tri, &th, &fr);
                     nh += 1;
-                    if (hit >= 0) {
-                        const float* e = exitance + 4 * (2 * (size_t)hit + (fr ? 0 : 1));
+                    /* a ray that leaves the scene returns environment * multiplier
+                     * (pathtracer.cpp:156-173) */
+                    float envc[3];
+                    const float* e = NULL;
+                    if (hit >= 0) e = exitance + 4 * (2 * (size_t)hit + (fr ? 0 : 1));
+                    else if (env) {
+                        const float* t = env->rgb + 3 * (size_t)lsgo_env_index(env, d);
+                        for (int c = 0; c < 3; c++) envc[c] = t[c] * p->environment_multiplier;
+                        e = envc;
+                    }
+                    if (e) {
                         for (int c = 0; c < 3; c++) sumH[c] += e[c];
                         if (layers) {
                             if (dirs) {
@@ -445,7 +523,19 @@ void lsgo_gather(const float* verts, uint32_t nverts, const uint32_t* idx, uint3
                  double* out_rgba, lsgo_gstats* stats, int threads)
 {
     gather_core(verts, nverts, idx, ntri, vnormals, NULL, NULL, side_mask, exitance, lights, nlights, p, texels, ntex,
-                subs, nodes, root_ref, prim, nprim, out_rgba, NULL, stats, threads);
+                subs, nodes, root_ref, prim, nprim, NULL, out_rgba, NULL, stats, threads);
+}
+
+/* lsgo_gather with an environment (NULL = none). */
+void lsgo_gather_env(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_t ntri,
+                     const float* vnormals, const uint8_t* side_mask, const float* exitance,
+                     const lsgo_light* lights, uint32_t nlights, const lsgo_gparams* p,
+                     const uint32_t* texels, uint32_t ntex, const float* subs,
+                     const float* nodes, int32_t root_ref, const uint32_t* prim, uint32_t nprim,
+                     const lsgo_env* env, double* out_rgba, lsgo_gstats* stats, int threads)
+{
+    gather_core(verts, nverts, idx, ntri, vnormals, NULL, NULL, side_mask, exitance, lights, nlights, p, texels, ntex,
+                subs, nodes, root_ref, prim, nprim, env, out_rgba, NULL, stats, threads);
 }
 
 /* lsgo_gather with the output layers of updateLightmap.  vtan/vbit: per-vertex frames
@@ -456,7 +546,7 @@ void lsgo_gather_layers(const float* verts, uint32_t nverts, const uint32_t* idx
                         const float* exitance, const lsgo_light* lights, uint32_t nlights, const lsgo_gparams* p,
                         const uint32_t* texels, uint32_t ntex, const float* subs,
                         const float* nodes, int32_t root_ref, const uint32_t* prim, uint32_t nprim,
-                        double* const* layers, lsgo_gstats* stats, int threads)
+                        const lsgo_env* env, double* const* layers, lsgo_gstats* stats, int threads)
 {
     float* ft = NULL;
     float* fb = NULL;
@@ -467,6 +557,6 @@ void lsgo_gather_layers(const float* verts, uint32_t nverts, const uint32_t* idx
         vtan = ft; vbit = fb;
     }
     gather_core(verts, nverts, idx, ntri, vnormals, vtan, vbit, side_mask, exitance, lights, nlights, p, texels, ntex,
-                subs, nodes, root_ref, prim, nprim, NULL, layers, stats, threads);
+                subs, nodes, root_ref, prim, nprim, env, NULL, layers, stats, threads);
     free(ft); free(fb);
 }

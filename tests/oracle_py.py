@@ -32,7 +32,33 @@ class Stats(ctypes.Structure):
 class GParams(ctypes.Structure):
     _fields_ = [("quality", ctypes.c_uint32), ("hemisphere", ctypes.c_int32), ("ray_length_min", ctypes.c_float),
                 ("locality", ctypes.c_float), ("rug_distance", ctypes.c_float), ("light_multiplier", ctypes.c_float),
-                ("seed", ctypes.c_uint32), ("light_rounds", ctypes.c_uint32)]
+                ("seed", ctypes.c_uint32), ("light_rounds", ctypes.c_uint32), ("environment_multiplier", ctypes.c_float)]
+
+
+class Env(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("w", ctypes.c_uint32), ("h", ctypes.c_uint32), ("rgb", ctypes.c_void_p)]
+
+
+ENV_CUBE, ENV_PANORAMA = 1, 2
+
+
+def _env(scene_arrays):
+    """scene_arrays['env'] = dict(type, width, height, rgb) or absent -> (Env or None, keep-alive)."""
+    e = scene_arrays.get("env")
+    if not e:
+        return None, None
+    rgb = np.ascontiguousarray(e["rgb"], dtype=np.float32)
+    return Env(e["type"], e["width"], e["height"], rgb.ctypes.data_as(ctypes.c_void_p)), rgb
+
+
+def env_index(env, d):
+    """Texel index the oracle's environment lookup picks for direction d."""
+    L = lib()
+    L.lsgo_env_index.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    L.lsgo_env_index.restype = ctypes.c_uint32
+    ev, keep = _env(dict(env=env))
+    dd = np.ascontiguousarray(d, dtype=np.float32)
+    return int(L.lsgo_env_index(ctypes.byref(ev), _p(dd)))
 
 
 class GStats(ctypes.Structure):
@@ -215,7 +241,8 @@ def gather(scene_arrays, params, texels, subs, tree=None, threads=0):
     nout = int(tx[:, 0].max()) + 1 if len(tx) else 0
     out = np.zeros((max(nout, 1), 4), np.float64)
     gp = GParams(params["quality"], params["hemisphere"], params["ray_length_min"], params["locality"],
-                 params["rug_distance"], params["light_multiplier"], params["seed"], params.get("light_rounds", 0))
+                 params["rug_distance"], params["light_multiplier"], params["seed"], params.get("light_rounds", 0),
+                 params.get("environment_multiplier", 0.0))
     gs = GStats()
     if tree is not None:
         nodes = np.ascontiguousarray(tree["nodes"], dtype=np.float32)
@@ -224,8 +251,14 @@ def gather(scene_arrays, params, texels, subs, tree=None, threads=0):
         nprim = len(prim)
     else:
         np_, pp_, root, nprim = None, None, 0, 0
-    lib().lsgo_gather(_p(v), len(v), _p(i), len(i), _p(vn), _p(sb), _p(ex), _p(lights), len(lights),
-                      ctypes.byref(gp), _p(tx), len(tx), _p(sbs), np_, root, pp_, nprim, _p(out), ctypes.byref(gs), threads)
+    ev, keep = _env(scene_arrays)
+    L = lib()
+    P, U = ctypes.c_void_p, ctypes.c_uint32
+    L.lsgo_gather_env.argtypes = [P, U, P, U, P, P, P, P, U, P, P, U, P, P, ctypes.c_int32, P, U, P, P, P, ctypes.c_int]
+    L.lsgo_gather_env.restype = None
+    L.lsgo_gather_env(_p(v), len(v), _p(i), len(i), _p(vn), _p(sb), _p(ex), _p(lights), len(lights),
+                      ctypes.byref(gp), _p(tx), len(tx), _p(sbs), np_, root, pp_, nprim,
+                      ctypes.byref(ev) if ev is not None else None, _p(out), ctypes.byref(gs), threads)
     return out, gs
 
 
@@ -236,7 +269,7 @@ def gather_layers(scene_arrays, params, texels, subs, layers=(0, 1, 2, 3, 4), tr
     L = lib()
     if not getattr(L, "_layers_declared", False):
         P, U = ctypes.c_void_p, ctypes.c_uint32
-        L.lsgo_gather_layers.argtypes = [P, U, P, U, P, P, P, P, P, P, U, P, P, U, P, P, ctypes.c_int32, P, U, P, P,
+        L.lsgo_gather_layers.argtypes = [P, U, P, U, P, P, P, P, P, P, U, P, P, U, P, P, ctypes.c_int32, P, U, P, P, P,
                                          ctypes.c_int]
         L.lsgo_gather_layers.restype = None
         L.lsgo_default_frames.argtypes = [P, U, P, P]
@@ -256,7 +289,8 @@ def gather_layers(scene_arrays, params, texels, subs, layers=(0, 1, 2, 3, 4), tr
     outs = {k: np.zeros((max(nout, 1), 4), np.float64) for k in layers}
     ptrs = (ctypes.c_void_p * 5)(*[(_p(outs[k]) if k in outs else None) for k in range(5)])
     gp = GParams(params["quality"], params["hemisphere"], params["ray_length_min"], params["locality"],
-                 params["rug_distance"], params["light_multiplier"], params["seed"], params.get("light_rounds", 0))
+                 params["rug_distance"], params["light_multiplier"], params["seed"], params.get("light_rounds", 0),
+                 params.get("environment_multiplier", 0.0))
     gs = GStats()
     if tree is not None:
         nodes = np.ascontiguousarray(tree["nodes"], dtype=np.float32)
@@ -264,9 +298,11 @@ def gather_layers(scene_arrays, params, texels, subs, layers=(0, 1, 2, 3, 4), tr
         np_, pp_, root, nprim = _p(nodes), _p(prim), tree["root"], len(prim)
     else:
         np_, pp_, root, nprim = None, None, 0, 0
+    ev, keep = _env(scene_arrays)
     L.lsgo_gather_layers(_p(v), len(v), _p(i), len(i), _p(vn), _p(vt) if vt is not None else None,
                          _p(vb) if vb is not None else None, _p(sb), _p(ex), _p(lights), len(lights), ctypes.byref(gp),
-                         _p(tx), len(tx), _p(sbs), np_, root, pp_, nprim, ptrs, ctypes.byref(gs), threads)
+                         _p(tx), len(tx), _p(sbs), np_, root, pp_, nprim, ctypes.byref(ev) if ev is not None else None,
+                         ptrs, ctypes.byref(gs), threads)
     return outs
 
 
