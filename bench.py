@@ -40,9 +40,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+NODE_BYTES = 128          # one 4-wide node per traversal step (n_node counts these)
 # L2-to-fabric bytes per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by
 # the same counters on a 1 GiB copy (x2.0 and x1.0): profiles/pmc_traffic_k_gather_k_trace_closest.md
-TRAFFIC_BYTES = {"k_gather": 17.376e9, "k_trace_closest": 2.306e9}
+TRAFFIC_BYTES = {"k_gather": 4.671e9, "k_trace_closest": 1.519e9}
 METRIC = "closest-hit Mrays/s; baked texels/s at ≤2% RMSE, plain vs learned gather"
 
 
@@ -87,7 +88,7 @@ def collider_workload(steps, warmup, cpu_threads, with_cpu=True):
     N.check(L.lsg_event_elapsed_ms(e0, e1, ctypes.byref(ms)), "elapsed")
     kernel_ms = ms.value / steps
     n_node, n_tri = col.trace_stats(rays)
-    b_ray = 48 + 64 * n_node + 48 * n_tri
+    b_ray = 48 + NODE_BYTES * n_node + 48 * n_tri
     mrays = len(R) / (wall / steps) / 1e6
     achieved = b_ray * len(R) / (kernel_ms * 1e-3) / 1e9
     hits, _ = rays.download_hits()
@@ -95,7 +96,8 @@ def collider_workload(steps, warmup, cpu_threads, with_cpu=True):
         metric=METRIC, value=round(mrays, 3), unit="Mrays/s", ms_per_step=round(wall / steps * 1e3, 4),
         dtype="f32", config=dict(workload="collider: displaced icosphere 100,820 tris, 1M random rays, closest hit",
                                  triangles=int(len(I)), rays=int(len(R)), leaf_size=int(info.leaf_size),
-                                 bvh_depth=int(info.depth), build_ms=round(info.build_ms, 3),
+                                 bvh_depth=int(info.depth), bvh4_nodes=int(info.num_nodes4),
+                                 stack_need=int(info.stack_need), build_ms=round(info.build_ms, 3),
                                  hit_ratio=round(float((hits["tri"] >= 0).mean()), 4)),
         roofline=dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
                       frac=round(achieved / HBM_PEAK_GBS, 4), traffic=TRAFFIC_BYTES["k_trace_closest"],
@@ -163,8 +165,8 @@ def bake_workload(steps: int, warmup: int, cpu_threads: int):
     N.check(L.lsg_gather_stats(sb.gs._h, sb.d_tex.ptr, ntex, sb.d_sub.ptr, len(sb.subs), ctypes.byref(p), sb.d_out.ptr, st),
             "gather_stats")
     hr, hn, ht, sr, sn, stt = list(st)
-    b_ray = 48 + 64 * hn / hr + 48 * ht / hr
-    b_shadow = 48 + 64 * sn / max(sr, 1) + 48 * stt / max(sr, 1)
+    b_ray = 48 + NODE_BYTES * hn / hr + 48 * ht / hr
+    b_shadow = 48 + NODE_BYTES * sn / max(sr, 1) + 48 * stt / max(sr, 1)
     n_sub = len(sb.subs) / ntex
     R = hr / ntex
     Lr = sr / ntex

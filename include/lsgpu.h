@@ -99,6 +99,8 @@ typedef struct lsg_collider_info {
     float    node_margin;      /* absolute inflation of node boxes */
     uint64_t device_bytes;     /* RRCollider::getMemoryOccupied (RRCollider.h:265) analogue */
     double   build_ms;
+    uint32_t num_nodes4;       /* 4-wide nodes the GPU traverses (collapsed from the BVH2) */
+    uint32_t stack_need;       /* worst-case traversal stack entries of the 4-wide tree */
 } lsg_collider_info;
 
 int  lsg_collider_create(const float* verts, uint32_t nverts, const uint32_t* indices, uint32_t ntri,

@@ -573,8 +573,11 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
     }
 }
 
-template <int STACK, bool STATS, bool LAYERS = false, bool ENV = false>
-__global__ __launch_bounds__(256) void k_gather(GatherArgs A)
+// One texel per wave, 4 per block.  The plain and environment kernels are capped at 128
+// VGPRs (4 waves per SIMD, measured faster than 3 with more registers); the layered ones
+// hold 12 more sums per lane and are left uncapped rather than spill.
+template <int STACK, bool STATS, bool LAYERS, bool ENV>
+__device__ inline void gather_block(const GatherArgs& A)
 {
     __shared__ int32_t stk[STACK * 256];
     __shared__ uint32_t lq[2 * 256];
@@ -584,9 +587,21 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs A)
     gather_texel<STACK, STATS, 64, LAYERS, ENV>(A, ti, lane, stk + tid, lq + tid, 256);
 }
 
+template <int STACK, bool STATS, bool ENV = false>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_gather(GatherArgs A)
+{
+    gather_block<STACK, STATS, false, ENV>(A);
+}
+
+template <int STACK, bool ENV>
+__global__ __launch_bounds__(256) void k_gather_layers(GatherArgs A)
+{
+    gather_block<STACK, false, true, ENV>(A);
+}
+
 // low ray counts (at most G shooters per series): 256/G texels per block, G lanes each
-template <int STACK, int G, bool LAYERS = false, bool ENV = false>
-__global__ __launch_bounds__(256) void k_gather_grouped(GatherArgs A)
+template <int STACK, int G, bool LAYERS, bool ENV>
+__device__ inline void gather_groups(const GatherArgs& A)
 {
     __shared__ int32_t stk[STACK * 256];
     __shared__ uint32_t lq[2 * 256];
@@ -596,13 +611,27 @@ __global__ __launch_bounds__(256) void k_gather_grouped(GatherArgs A)
     gather_texel<STACK, false, G, LAYERS, ENV>(A, ti, lane, stk + tid, lq + tid, 256);
 }
 
+template <int STACK, int G, bool ENV = false>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_gather_grouped(GatherArgs A)
+{
+    gather_groups<STACK, G, false, ENV>(A);
+}
+
+template <int STACK, int G, bool ENV>
+__global__ __launch_bounds__(256) void k_gather_grouped_layers(GatherArgs A)
+{
+    gather_groups<STACK, G, true, ENV>(A);
+}
+
 __global__ void k_queue_reset(uint32_t* counters, uint32_t n)
 {
     if (threadIdx.x < n) counters[threadIdx.x] = 0;
 }
 
+// At 141 VGPRs this kernel would run 3 waves per SIMD; capped at 128 (12-16 registers
+// spilled to scratch) it runs 4, which measured faster for the batched bake.
 template <int STACK, int WAVES, bool ENV = false>
-__global__ __launch_bounds__(64 * WAVES) void k_gather_persistent(GatherArgs A, WorkQueue Q)
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4))) void k_gather_persistent(GatherArgs A, WorkQueue Q)
 {
     __shared__ int32_t stk[STACK * 64 * WAVES];
     __shared__ uint32_t lq[2 * 64 * WAVES];
@@ -748,18 +777,33 @@ static void launch_gather_kernels(const GatherArgs& A, int stack, hipStream_t st
     if (series <= 32) {
         const uint32_t G = series <= 16 ? 16u : 32u;
         const unsigned grid = (unsigned)(((uint64_t)A.ntex * G + 255) / 256);
-        if (G == 16) {
-            if (stack <= 32) k_gather_grouped<32, 16, LAYERS, ENV><<<grid, 256, 0, st>>>(A);
-            else k_gather_grouped<64, 16, LAYERS, ENV><<<grid, 256, 0, st>>>(A);
+        if constexpr (LAYERS) {
+            if (G == 16) {
+                if (stack <= 32) k_gather_grouped_layers<32, 16, ENV><<<grid, 256, 0, st>>>(A);
+                else k_gather_grouped_layers<64, 16, ENV><<<grid, 256, 0, st>>>(A);
+            } else {
+                if (stack <= 32) k_gather_grouped_layers<32, 32, ENV><<<grid, 256, 0, st>>>(A);
+                else k_gather_grouped_layers<64, 32, ENV><<<grid, 256, 0, st>>>(A);
+            }
         } else {
-            if (stack <= 32) k_gather_grouped<32, 32, LAYERS, ENV><<<grid, 256, 0, st>>>(A);
-            else k_gather_grouped<64, 32, LAYERS, ENV><<<grid, 256, 0, st>>>(A);
+            if (G == 16) {
+                if (stack <= 32) k_gather_grouped<32, 16, ENV><<<grid, 256, 0, st>>>(A);
+                else k_gather_grouped<64, 16, ENV><<<grid, 256, 0, st>>>(A);
+            } else {
+                if (stack <= 32) k_gather_grouped<32, 32, ENV><<<grid, 256, 0, st>>>(A);
+                else k_gather_grouped<64, 32, ENV><<<grid, 256, 0, st>>>(A);
+            }
         }
         return;
     }
     const unsigned grid = (A.ntex + 3) / 4;
-    if (stack <= 32) k_gather<32, false, LAYERS, ENV><<<grid, 256, 0, st>>>(A);
-    else k_gather<64, false, LAYERS, ENV><<<grid, 256, 0, st>>>(A);
+    if constexpr (LAYERS) {
+        if (stack <= 32) k_gather_layers<32, ENV><<<grid, 256, 0, st>>>(A);
+        else k_gather_layers<64, ENV><<<grid, 256, 0, st>>>(A);
+    } else {
+        if (stack <= 32) k_gather<32, false, ENV><<<grid, 256, 0, st>>>(A);
+        else k_gather<64, false, ENV><<<grid, 256, 0, st>>>(A);
+    }
 }
 
 static int launch_gather_one(const GatherArgs& A, int stack, hipStream_t st)
@@ -914,8 +958,8 @@ static int gather_launch(lsg_scene* s, const lsg_texel* texels, uint32_t ntexels
     const unsigned grid = (ntexels + 3) / 4;
     hipStream_t hs = (hipStream_t)stream;
     if (A.sc.env_type) {
-        if (stack <= 32) k_gather<32, true, false, true><<<grid, 256, 0, hs>>>(A);
-        else k_gather<64, true, false, true><<<grid, 256, 0, hs>>>(A);
+        if (stack <= 32) k_gather<32, true, true><<<grid, 256, 0, hs>>>(A);
+        else k_gather<64, true, true><<<grid, 256, 0, hs>>>(A);
     } else {
         if (stack <= 32) k_gather<32, true><<<grid, 256, 0, hs>>>(A);
         else k_gather<64, true><<<grid, 256, 0, hs>>>(A);

@@ -29,6 +29,14 @@ bool device_ready();   // a gfx950 device is present and selected
 // count = (ref&15)+1 triangles in leaf order.
 struct alignas(16) Node { float4 q0, q1, q2, q3; };
 
+// BVH4 node, 128 bytes (one cache line), collapsed from the BVH2 above and used by
+// every GPU traversal:
+//   q[0] = lo.x of children 0..3   q[1] = hi.x   q[2] = lo.y   q[3] = hi.y
+//   q[4] = lo.z                    q[5] = hi.z   q[6] = child refs (int bits)
+// Refs as in the BVH2 (>= 0 inner BVH4 node, < 0 leaf); kNoChild marks an unused slot.
+struct alignas(128) Node4 { float4 q[8]; };
+constexpr int32_t kNoChild = 0x7ffffffe;
+
 // Triangle in leaf order, 48 bytes + 8 bytes meta (TriangleSRLNP,
 // IntersectBspFast.h:33-49, with the projection-selected components pre-picked):
 //   plane = n4;  sv = (s[x], s[y], s[z], intersectReal);  ev = (r[x], r[y], l[z], ir[z])
@@ -95,6 +103,8 @@ __device__ inline bool srlnp_test(const float4 sv, const float4 ev, int byte, co
 // ---- device-side collider view ---------------------------------------------------------
 struct ColliderView {
     const Node* nodes;
+    const Node4* nodes4;
+    int32_t root4;
     const float4* plane; const float4* sv; const float4* ev; const uint2* meta;
     int32_t root;
     uint32_t nvalid;
@@ -103,6 +113,9 @@ struct ColliderView {
 
 struct ColliderImpl {
     uint32_t ntri = 0, nverts = 0, nvalid = 0, nnodes = 0, depth = 0, leaf = 4;
+    uint32_t nnodes4 = 0, stack_need = 0;   // BVH4 nodes; worst-case traversal stack entries
+    int32_t root4 = 0;
+    Node4* d_nodes4 = nullptr;
     int32_t root = 0;
     float bmin[3], bmax[3], margin = 0;
     float* d_verts = nullptr; uint32_t* d_idx = nullptr;
@@ -114,6 +127,7 @@ struct ColliderImpl {
     {
         ColliderView v;
         v.nodes = d_nodes; v.plane = d_plane; v.sv = d_sv; v.ev = d_ev; v.meta = d_meta;
+        v.nodes4 = d_nodes4; v.root4 = root4;
         v.root = root; v.nvalid = nvalid;
         v.bmin = make_float3(bmin[0], bmin[1], bmin[2]);
         v.bmax = make_float3(bmax[0], bmax[1], bmax[2]);

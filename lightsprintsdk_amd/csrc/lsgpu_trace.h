@@ -1,6 +1,6 @@
-// lsgpu_trace.h -- closest-hit BVH2 traversal for one ray per lane (gfx950).
+// lsgpu_trace.h -- closest-hit BVH4 traversal for one ray per lane (gfx950).
 //
-// Near-child-first stack traversal over 64-byte nodes whose child boxes are
+// Near-child-first stack traversal over 128-byte BVH4 nodes whose child boxes are
 // inflated by a DELTA_BSP-sized margin, so node culling is conservative and the
 // result depends only on the per-triangle SRLNP test (IntersectBspFast.cpp:189-252)
 // and the plane distance (:349-352): among accepted triangles with distance in
@@ -44,6 +44,63 @@ __device__ inline void slab2(const Node& nd, const float3 noi, const float3 idir
     n1 = fmaxf(fmaxf(ta, tc), fmaxf(te, tlo)); f1 = fminf(fminf(tb, td), fminf(tf, thi));
 }
 
+// One BVH4 node: the four child-box slab tests (one fma per plane, as slab2), then the
+// children that are hit, nearest first: the nearest is returned as the next ref, the
+// others go on the stack farthest first.  No child hit: the next ref is popped.  The
+// tree is collapsed so that the stack never holds more than STACK-1 entries.
+__device__ inline void cswap(float& ka, int32_t& ra, float& kb, int32_t& rb)
+{
+    const bool sw = kb < ka;
+    const float k = sw ? kb : ka; kb = sw ? ka : kb; ka = k;
+    const int32_t r = sw ? rb : ra; rb = sw ? ra : rb; ra = r;
+}
+
+template <int STACK, class Pop>
+__device__ inline int32_t visit4(const Node4* __restrict__ nodes, int32_t ref, const float3 noi, const float3 idir, float tlo,
+                                 float thi, int32_t* stk, int stride, int& sp, bool& ok, Pop pop)
+{
+    const Node4* np = nodes + ref;
+    float4 tn, tf;
+    {
+        const float4 l = np->q[0], h = np->q[1];
+        float a, b;
+        a = __fmaf_rn(l.x, idir.x, noi.x); b = __fmaf_rn(h.x, idir.x, noi.x); tn.x = fminf(a, b); tf.x = fmaxf(a, b);
+        a = __fmaf_rn(l.y, idir.x, noi.x); b = __fmaf_rn(h.y, idir.x, noi.x); tn.y = fminf(a, b); tf.y = fmaxf(a, b);
+        a = __fmaf_rn(l.z, idir.x, noi.x); b = __fmaf_rn(h.z, idir.x, noi.x); tn.z = fminf(a, b); tf.z = fmaxf(a, b);
+        a = __fmaf_rn(l.w, idir.x, noi.x); b = __fmaf_rn(h.w, idir.x, noi.x); tn.w = fminf(a, b); tf.w = fmaxf(a, b);
+    }
+    {
+        const float4 l = np->q[2], h = np->q[3];
+        float a, b;
+        a = __fmaf_rn(l.x, idir.y, noi.y); b = __fmaf_rn(h.x, idir.y, noi.y); tn.x = fmaxf(tn.x, fminf(a, b)); tf.x = fminf(tf.x, fmaxf(a, b));
+        a = __fmaf_rn(l.y, idir.y, noi.y); b = __fmaf_rn(h.y, idir.y, noi.y); tn.y = fmaxf(tn.y, fminf(a, b)); tf.y = fminf(tf.y, fmaxf(a, b));
+        a = __fmaf_rn(l.z, idir.y, noi.y); b = __fmaf_rn(h.z, idir.y, noi.y); tn.z = fmaxf(tn.z, fminf(a, b)); tf.z = fminf(tf.z, fmaxf(a, b));
+        a = __fmaf_rn(l.w, idir.y, noi.y); b = __fmaf_rn(h.w, idir.y, noi.y); tn.w = fmaxf(tn.w, fminf(a, b)); tf.w = fminf(tf.w, fmaxf(a, b));
+    }
+    {
+        const float4 l = np->q[4], h = np->q[5];
+        float a, b;
+        a = __fmaf_rn(l.x, idir.z, noi.z); b = __fmaf_rn(h.x, idir.z, noi.z); tn.x = fmaxf(tn.x, fminf(a, b)); tf.x = fminf(tf.x, fmaxf(a, b));
+        a = __fmaf_rn(l.y, idir.z, noi.z); b = __fmaf_rn(h.y, idir.z, noi.z); tn.y = fmaxf(tn.y, fminf(a, b)); tf.y = fminf(tf.y, fmaxf(a, b));
+        a = __fmaf_rn(l.z, idir.z, noi.z); b = __fmaf_rn(h.z, idir.z, noi.z); tn.z = fmaxf(tn.z, fminf(a, b)); tf.z = fminf(tf.z, fmaxf(a, b));
+        a = __fmaf_rn(l.w, idir.z, noi.z); b = __fmaf_rn(h.w, idir.z, noi.z); tn.w = fmaxf(tn.w, fminf(a, b)); tf.w = fminf(tf.w, fmaxf(a, b));
+    }
+    const float4 rq = np->q[6];
+    int32_t r0 = __float_as_int(rq.x), r1 = __float_as_int(rq.y), r2 = __float_as_int(rq.z), r3 = __float_as_int(rq.w);
+    constexpr float kInf = __builtin_huge_valf();
+    float k0 = fmaxf(tn.x, tlo), k1 = fmaxf(tn.y, tlo), k2 = fmaxf(tn.z, tlo), k3 = fmaxf(tn.w, tlo);
+    k0 = (r0 != kNoChild && k0 <= fminf(tf.x, thi)) ? k0 : kInf;
+    k1 = (r1 != kNoChild && k1 <= fminf(tf.y, thi)) ? k1 : kInf;
+    k2 = (r2 != kNoChild && k2 <= fminf(tf.z, thi)) ? k2 : kInf;
+    k3 = (r3 != kNoChild && k3 <= fminf(tf.w, thi)) ? k3 : kInf;
+    cswap(k0, r0, k1, r1); cswap(k2, r2, k3, r3); cswap(k0, r0, k2, r2); cswap(k1, r1, k3, r3); cswap(k1, r1, k2, r2);
+    auto push = [&](int32_t r) { if (sp < STACK) stk[(sp++) * stride] = r; else ok = false; };
+    if (k3 < kInf) push(r3);
+    if (k2 < kInf) push(r2);
+    if (k1 < kInf) push(r1);
+    return k0 < kInf ? r0 : pop();
+}
+
 __device__ inline float3 safe_inv(float3 d)
 {
     float x = fabsf(d.x) < 1e-30f ? copysignf(1e-30f, d.x) : d.x;
@@ -75,7 +132,7 @@ __device__ inline bool trace_closest(const Filter& flt, const ColliderView& c, c
     const float3 noi = make_float3(-o.x * idir.x, -o.y * idir.y, -o.z * idir.z);
     float best = hi, bound = hi;
     int bi = -1; float bu = 0.f, bv = 0.f; bool bf = false; uint32_t bslot = 0;
-    int32_t ref = c.root;          // next node: >= 0 inner, < 0 leaf, kDone finished
+    int32_t ref = c.root4;         // next node: >= 0 inner, < 0 leaf, kDone finished
     int32_t leaf = 0;              // parked leaf (< 0) or none (0)
     int sp = 0;
     bool ok = true;
@@ -84,18 +141,7 @@ __device__ inline bool trace_closest(const Filter& flt, const ColliderView& c, c
         // inner nodes, parking the first leaf met
         while (ref >= 0 && ref != kDone) {
             if (STATS) nvis++;
-            const Node nd = c.nodes[ref];
-            float n0, f0, n1, f1;
-            slab2(nd, noi, idir, lo, bound, n0, f0, n1, f1);
-            const int32_t r0 = __float_as_int(nd.q3.x), r1 = __float_as_int(nd.q3.y);
-            const bool h0 = n0 <= f0, h1 = n1 <= f1;
-            if (h0 && h1) {
-                const bool swap = n1 < n0;
-                if (sp < STACK) stk[(sp++) * stride] = swap ? r0 : r1; else ok = false;
-                ref = swap ? r1 : r0;
-            } else if (h0) ref = r0;
-            else if (h1) ref = r1;
-            else ref = pop();
+            ref = visit4<STACK>(c.nodes4, ref, noi, idir, lo, bound, stk, stride, sp, ok, pop);
             if (ref < 0 && leaf == 0) { leaf = ref; ref = pop(); }
             if (!__any(leaf == 0)) break;                // every active lane holds a leaf
         }
@@ -157,7 +203,7 @@ __device__ inline bool trace_begin(TraceState& S, const ColliderView& c, const f
     S.noi = make_float3(-o.x * S.idir.x, -o.y * S.idir.y, -o.z * S.idir.z);
     S.lo = lo; S.best = hi; S.bound = hi;
     S.bu = 0.f; S.bv = 0.f; S.bf = false;
-    S.ref = c.root; S.leaf = 0; S.sp = 0;
+    S.ref = c.root4; S.leaf = 0; S.sp = 0;
     return true;
 }
 
@@ -170,18 +216,7 @@ __device__ inline bool trace_resume(TraceState& S, const Filter& flt, const Coll
     while (S.ref != kDone || S.leaf < 0) {
         if ((uint32_t)__popcll(__ballot(1)) <= yield_at) return false;     // enough lanes are free
         while (S.ref >= 0 && S.ref != kDone) {
-            const Node nd = c.nodes[S.ref];
-            float n0, f0, n1, f1;
-            slab2(nd, S.noi, S.idir, S.lo, S.bound, n0, f0, n1, f1);
-            const int32_t r0 = __float_as_int(nd.q3.x), r1 = __float_as_int(nd.q3.y);
-            const bool h0 = n0 <= f0, h1 = n1 <= f1;
-            if (h0 && h1) {
-                const bool swap = n1 < n0;
-                if (S.sp < STACK) stk[(S.sp++) * stride] = swap ? r0 : r1; else ok = false;
-                S.ref = swap ? r1 : r0;
-            } else if (h0) S.ref = r0;
-            else if (h1) S.ref = r1;
-            else S.ref = pop();
+            S.ref = visit4<STACK>(c.nodes4, S.ref, S.noi, S.idir, S.lo, S.bound, stk, stride, S.sp, ok, pop);
             if (S.ref < 0 && S.leaf == 0) { S.leaf = S.ref; S.ref = pop(); }
             if (!__any(S.leaf == 0)) break;
         }

@@ -29,7 +29,8 @@ class BuildParams(ctypes.Structure):
 class ColliderInfo(ctypes.Structure):
     _fields_ = [("num_triangles", U32), ("num_valid", U32), ("num_nodes", U32), ("root_ref", I32),
                 ("depth", U32), ("leaf_size", U32), ("box_min", ctypes.c_float * 3), ("box_max", ctypes.c_float * 3),
-                ("node_margin", ctypes.c_float), ("device_bytes", ctypes.c_uint64), ("build_ms", ctypes.c_double)]
+                ("node_margin", ctypes.c_float), ("device_bytes", ctypes.c_uint64), ("build_ms", ctypes.c_double),
+                ("num_nodes4", U32), ("stack_need", U32)]
 
 
 class Light(ctypes.Structure):

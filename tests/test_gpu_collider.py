@@ -169,3 +169,37 @@ def test_gpu_deterministic(gpu):
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
     c2 = _col(V, I)
     np.testing.assert_array_equal(c.download_tree()["nodes"], c2.download_tree()["nodes"])
+
+
+@pytest.mark.parametrize("scene", ["ico100k", "room", "deep"])
+def test_gpu_bvh4_stack_bound(gpu, oracle, scene):
+    """The GPU traverses a 4-wide tree collapsed from the LBVH under a stack budget: the
+    worst-case stack need stays below the 32 (or 64) LDS entries, and hits equal the
+    oracle's."""
+    from lightsprintsdk_amd.collider import RRCollider
+    if scene == "ico100k":
+        V, I = scenes.icosphere(scenes.ICOSPHERE_100K_FREQUENCY, seed=1)
+    elif scene == "room":
+        sc = scenes.box_room(seed=3, n_boxes=20, target_tris=50_000, lightmap_size=256)
+        V, I = sc.verts, sc.idx
+    else:
+        # a fan of slivers crowded towards one end: Morton codes collide, the tree is
+        # deep (26 levels at one triangle per leaf)
+        n = 3000
+        a = np.linspace(0, 1, n + 1, dtype=np.float64) ** 6
+        V = np.zeros((n + 2, 3), np.float32)
+        V[1:, 0] = a
+        V[1:, 1] = a ** 2
+        V[1:, 2] = 0.01 * np.sin(np.arange(n + 1))
+        I = np.stack([np.zeros(n, np.uint32), np.arange(1, n + 1, dtype=np.uint32),
+                      np.arange(2, n + 2, dtype=np.uint32)], 1)
+    c = RRCollider.create(V, I, leaf_size=1 if scene == "deep" else 0)
+    inf = c.info()
+    stack = 32 if inf.depth + 1 <= 32 else 64
+    print(f"{scene}: depth {inf.depth}, BVH4 nodes {inf.num_nodes4}, stack need {inf.stack_need} of {stack}", flush=True)
+    assert inf.stack_need <= stack - 1
+    assert inf.num_nodes4 >= 1
+    R = scenes.random_rays(V, 20000, seed=5)
+    g, gf = c.intersect_batch(R)
+    o = oracle.intersect_linear_srlnp(V, I, R)
+    np.testing.assert_array_equal(g["tri"], o["tri"])
