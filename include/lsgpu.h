@@ -198,6 +198,8 @@ int  lsg_scene_create(const lsg_collider* collider, const float* vertex_normals,
 void lsg_scene_destroy(lsg_scene* scene);
 /* exitance: host float[ntri*8] = per triangle (front r,g,b,pad, back r,g,b,pad) */
 int  lsg_scene_set_exitance(lsg_scene* scene, const float* exitance);
+/* copies the current exitance table (same layout) to the host; synchronous */
+int  lsg_scene_get_exitance(const lsg_scene* scene, float* exitance);
 int  lsg_scene_set_lights(lsg_scene* scene, const lsg_light* lights, uint32_t nlights);
 
 /* The solver's environment (RRSolver::setEnvironment), which hemisphere rays that leave
@@ -342,6 +344,15 @@ int  lsg_bake_set_tuning(lsg_bake* bake, const lsg_bake_tuning* tuning);
  * loop's convergence checks; with it, the replay is ordered after stream's work. */
 int  lsg_bake_run(lsg_bake* bake, const lsg_gather_params* first, const lsg_gather_params* final_params,
                   const lsg_filter_params* filtering, uint32_t flags, void* stream);
+/* Radiosity on the GPU: after the first gather (direct light), `bounces` Jacobi
+ * iterations of the per-triangle gather spread the light between triangles, which the
+ * reference leaves to its CPU solver (RRSolver::updateSolverIndirectIllumination,
+ * gather.cpp:1038-1120).  Each iteration is a hemisphere-only gather of `quality` rays per
+ * triangle (0 = the first gather's quality) over the current exitance table, and
+ *   exitance = emittance + albedo * (light part of the first gather + hemisphere part).
+ * Takes effect on the next lsg_bake_run with first != NULL (graph replays included);
+ * 0 bounces (the default) = direct light only. */
+int  lsg_bake_set_bounces(lsg_bake* bake, uint32_t bounces, uint32_t quality);
 /* times the persistent final gather over every stride-th texel (autotuner probe) */
 int  lsg_bake_gather_sample(lsg_bake* bake, const lsg_gather_params* params, uint32_t stride, float* ms);
 /* copies object `object`'s RGBF lightmap (width*height*3 floats) to the host */

@@ -120,6 +120,11 @@ class GatherScene:
         ex = np.ascontiguousarray(exitance, np.float32).reshape(len(self.idx), 8)
         N.check(N.lib().lsg_scene_set_exitance(self._h, ex.ctypes.data_as(ctypes.c_void_p)), "set_exitance")
 
+    def get_exitance(self):
+        out = np.zeros((len(self.idx), 8), np.float32)
+        N.check(N.lib().lsg_scene_get_exitance(self._h, out.ctypes.data_as(ctypes.c_void_p)), "get_exitance")
+        return out
+
     def set_lights(self, lights):
         self.lights = lights_array(lights) if not isinstance(lights, np.ndarray) else np.ascontiguousarray(lights, LIGHT_DTYPE)
         N.check(N.lib().lsg_scene_set_lights(self._h, self.lights.ctypes.data_as(ctypes.c_void_p), len(self.lights)),
@@ -418,6 +423,17 @@ class BakePlan:
         N.check(N.lib().lsg_bake_run(self._h, ctypes.byref(fp) if first else None, ctypes.byref(p),
                                      ctypes.byref(flt) if filtering else None, N.BAKE_USE_GRAPH if graph else 0, stream),
                 "lsg_bake_run")
+
+    def set_bounces(self, bounces=0, quality=0):
+        """Radiosity on the GPU (lsg_bake_set_bounces): after the first gather, `bounces`
+        hemisphere-only per-triangle gathers of `quality` rays (0 = the first gather's)
+        spread the light between triangles before the final gather."""
+        N.check(N.lib().lsg_bake_set_bounces(self._h, bounces, quality), "set_bounces")
+        self.bounces, self.bounce_quality = bounces, quality
+
+    def exitance(self):
+        """The scene's current exitance table, float32 [ntri, 8] (front rgb+pad, back rgb+pad)."""
+        return self.gs.get_exitance()
 
     def run_layers(self, layers=(0, 1, 2, 3, 4), first=True, filtering=True, stream=None):
         """The plan's bake with directional and bent-normal layers (lsg_bake_run_layers)."""

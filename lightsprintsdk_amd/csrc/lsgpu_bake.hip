@@ -45,6 +45,23 @@ __global__ __launch_bounds__(256) void k_emittance_only(uint32_t nt, const float
     ex[2 * (size_t)t + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
+// exitance after a bounce: emittance + albedo * (light part of the first gather +
+// hemisphere part gathered over the previous table); hemi == nullptr: the first gather's
+// own hemisphere part (parts[2t])
+__global__ __launch_bounds__(256) void k_exitance_bounce(uint32_t nt, const float4* __restrict__ parts, const float4* __restrict__ hemi,
+                                                         const float* __restrict__ albedo, const float* __restrict__ emit,
+                                                         float4* __restrict__ ex)
+{
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= nt) return;
+    const float4 h = hemi ? hemi[t] : parts[2 * (size_t)t];
+    const float4 l = parts[2 * (size_t)t + 1];
+    ex[2 * (size_t)t] = make_float4((h.x + l.x) * albedo[3 * (size_t)t] + emit[3 * (size_t)t],
+                                    (h.y + l.y) * albedo[3 * (size_t)t + 1] + emit[3 * (size_t)t + 1],
+                                    (h.z + l.z) * albedo[3 * (size_t)t + 2] + emit[3 * (size_t)t + 2], 0.f);
+    ex[2 * (size_t)t + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
 __global__ __launch_bounds__(256) void k_zero_f4(float4* p, uint32_t n)
 {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
@@ -71,11 +88,15 @@ struct BakeImpl {
     hipGraph_t graph = nullptr; hipGraphExec_t exec = nullptr;
     std::vector<uint8_t> graph_key;
     uint32_t grow_steps = 0;
+    uint32_t bounces = 0, bounce_quality = 0;
+    float4* d_parts = nullptr;         // first gather's (hemisphere, light) parts per triangle, with bounces
+    float4* d_hemi = nullptr;          // a bounce's hemisphere gather per triangle
     float4* d_lrgba[4] = {nullptr, nullptr, nullptr, nullptr};   // layers 1..4 (layer 0 = d_rgba / d_rgbf)
     float* d_lrgbf[4] = {nullptr, nullptr, nullptr, nullptr};
     ~BakeImpl()
     {
         for (int i = 0; i < 4; i++) { dfree(d_lrgba[i]); dfree(d_lrgbf[i]); }
+        dfree(d_parts); dfree(d_hemi);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         if (own) (void)hipStreamDestroy(own);
@@ -114,8 +135,27 @@ static int run_first(BakeImpl& B, const lsg_gather_params* first, hipStream_t st
     int stack = 0, e;
     if ((e = gather_args(B.scene, B.d_ttex, B.ntri, B.d_tsub, first, (float*)B.d_irr, A, stack))) return e;
     k_emittance_only<<<nblk(B.ntri), 256, 0, st>>>(B.ntri, B.d_emit, B.scene->impl.d_exit);
+    if (!B.bounces) {
+        if ((e = launch_gather(A, stack, st))) return e;
+        k_exitance<<<nblk(B.ntri), 256, 0, st>>>(B.ntri, B.d_irr, B.d_albedo, B.d_emit, B.scene->impl.d_exit);
+        return LSG_OK;
+    }
+    // radiosity: keep the first gather's light part, regather the hemisphere each bounce
+    A.parts = B.d_parts;
     if ((e = launch_gather(A, stack, st))) return e;
-    k_exitance<<<nblk(B.ntri), 256, 0, st>>>(B.ntri, B.d_irr, B.d_albedo, B.d_emit, B.scene->impl.d_exit);
+    k_exitance_bounce<<<nblk(B.ntri), 256, 0, st>>>(B.ntri, B.d_parts, nullptr, B.d_albedo, B.d_emit, B.scene->impl.d_exit);
+    lsg_gather_params bp = *first;
+    bp.hemisphere = 1;
+    bp.light_multiplier = 0.f;
+    bp.light_rounds = 0;
+    if (B.bounce_quality) bp.quality = B.bounce_quality;
+    for (uint32_t b = 0; b < B.bounces; b++) {
+        bp.seed = first->seed ^ (0x9E3779B9u * (b + 1));
+        GatherArgs H;
+        if ((e = gather_args(B.scene, B.d_ttex, B.ntri, B.d_tsub, &bp, (float*)B.d_hemi, H, stack))) return e;
+        if ((e = launch_gather(H, stack, st))) return e;
+        k_exitance_bounce<<<nblk(B.ntri), 256, 0, st>>>(B.ntri, B.d_parts, B.d_hemi, B.d_albedo, B.d_emit, B.scene->impl.d_exit);
+    }
     return LSG_OK;
 }
 
@@ -350,6 +390,21 @@ int lsg_bake_download_layer(const lsg_bake* b, uint32_t object, uint32_t layer, 
     if (!src) return fail(LSG_ERR_INVALID, "lsg_bake_download_layer: layer %u was never baked", layer);
     const lsg_lightmap& m = B.maps[object];
     LSG_HIP(hipMemcpy(rgbf, src + 3 * (size_t)m.rgbf_base, 12 * (size_t)m.width * m.height, hipMemcpyDeviceToHost));
+    return LSG_OK;
+}
+
+int lsg_bake_set_bounces(lsg_bake* b, uint32_t bounces, uint32_t quality)
+{
+    if (!b) return fail(LSG_ERR_INVALID, "lsg_bake_set_bounces: null plan");
+    if (bounces > 64) return fail(LSG_ERR_INVALID, "lsg_bake_set_bounces: %u bounces, at most 64", bounces);
+    BakeImpl& B = b->impl;
+    if (bounces && !B.d_parts) {
+        LSG_HIP(hipMalloc(&B.d_parts, 32 * (size_t)(B.ntri ? B.ntri : 1)));
+        LSG_HIP(hipMalloc(&B.d_hemi, 16 * (size_t)(B.ntri ? B.ntri : 1)));
+    }
+    B.bounces = bounces;
+    B.bounce_quality = quality;
+    B.graph_key.clear();        // the captured sequence changes
     return LSG_OK;
 }
 

@@ -831,6 +831,15 @@ int lsg_scene_set_exitance(lsg_scene* s, const float* exitance)
     return LSG_OK;
 }
 
+int lsg_scene_get_exitance(const lsg_scene* s, float* exitance)
+{
+    if (!s || !exitance) return fail(LSG_ERR_INVALID, "lsg_scene_get_exitance: null argument");
+    const uint32_t nt = s->impl.col->impl.ntri;
+    LSG_HIP(hipDeviceSynchronize());
+    LSG_HIP(hipMemcpy(exitance, s->impl.d_exit, 32 * (size_t)nt, hipMemcpyDeviceToHost));
+    return LSG_OK;
+}
+
 int lsg_scene_set_lights(lsg_scene* s, const lsg_light* lights, uint32_t n)
 {
     if (!s || (n && !lights)) return fail(LSG_ERR_INVALID, "lsg_scene_set_lights: null argument");

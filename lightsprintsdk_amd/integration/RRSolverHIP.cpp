@@ -76,7 +76,12 @@ bool RRSolverHIP::buildScene(const UpdateParameters& params)
 			}
 		}
 		RRVec3 ex(0);
-		if (getTriangleMeasure(t, 3, RM_RADIOSITY_LINEAR, ex))
+		if (gpuBounces)
+		{
+			// the GPU solution starts from the emitters (the plan's first gather resets to them)
+			for (int c = 0; c < 3; c++) exitance[8 * (size_t)t + c] = emittance[3 * (size_t)t + c];
+		}
+		else if (getTriangleMeasure(t, 3, RM_RADIOSITY_LINEAR, ex))
 			for (int c = 0; c < 3; c++)
 			{
 				exitance[8 * (size_t)t + c] = ex[c];
@@ -232,9 +237,27 @@ unsigned RRSolverHIP::bake(const Target* targets, unsigned numTargets, const Upd
 		for (int c = 0; c < 4; c++) f.background[c] = filtering->backgroundColor[c];
 		f.wrap = filtering->wrap ? 1 : 0;
 	}
+	// GPU radiosity: first gather at half quality (gather.cpp:1083-1088), lights and the
+	// environment only, then the bounces; otherwise the reference solver's exitance is used
+	lsg_gather_params first = g;
+	first.quality = params.quality / 2 ? params.quality / 2 : 1;
+	first.hemisphere = 0;
+	first.light_multiplier = params.indirect.lightMultiplier;
+	first.seed = g.seed ^ 0x5bd1e995u;
+	const lsg_gather_params* firstp = nullptr;
+	if (gpuBounces)
+	{
+		if (lsg_bake_set_bounces(plan, gpuBounces, gpuBounceQuality) != LSG_OK)
+		{
+			RRReporter::report(ERRO, "RRSolverHIP: %s\n", lsg_last_error());
+			lsg_bake_destroy(plan);
+			return 0;
+		}
+		firstp = &first;
+	}
 	unsigned updated = 0;
-	const int st = mask == 1u ? lsg_bake_run(plan, nullptr, &g, &f, 0, nullptr)
-	                          : lsg_bake_run_layers(plan, nullptr, &g, &f, mask, nullptr);
+	const int st = mask == 1u ? lsg_bake_run(plan, firstp, &g, &f, 0, nullptr)
+	                          : lsg_bake_run_layers(plan, firstp, &g, &f, mask, nullptr);
 	if (st != LSG_OK)
 		RRReporter::report(ERRO, "RRSolverHIP: %s\n", lsg_last_error());
 	else
@@ -271,8 +294,9 @@ unsigned RRSolverHIP::updateLightmaps(int layerLightmap, int layerDirectionalLig
 		                         "no rotated or blended environment, a gfx950 device).\n");
 		return 0;
 	}
-	// indirect solution first, by the reference's own solver (RRSolver.h:731-734)
-	if (!params->useCurrentSolution)
+	// indirect solution first, by the reference's own solver (RRSolver.h:731-734), unless
+	// the GPU computes it (setGpuRadiosity)
+	if (!params->useCurrentSolution && !gpuBounces)
 		RRSolver::updateLightmaps(-1, -1, -1, params, nullptr);
 	// layer numbers of the five outputs (lightmap.cpp:968-973)
 	int layers[LSG_NUM_LAYERS];
@@ -341,7 +365,7 @@ unsigned RRSolverHIP::updateLightmap(int objectNumber, RRBuffer* lightmap, RRBuf
 		RRReporter::report(ERRO, "RRSolverHIP::updateLightmap: unsupported request.\n");
 		return 0;
 	}
-	if (!params->useCurrentSolution)
+	if (!params->useCurrentSolution && !gpuBounces)
 		RRSolver::updateLightmaps(-1, -1, -1, params, nullptr);
 	return bake(&t, 1, *params, filtering);
 }

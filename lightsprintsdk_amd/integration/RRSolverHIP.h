@@ -48,6 +48,11 @@ public:
 	// unsupported inputs go to the reference implementation instead of failing
 	void setReferenceFallback(bool enable) { referenceFallback = enable; }
 
+	// bounces > 0: compute the indirect solution on the GPU (first gather at half quality,
+	// as gather.cpp:1083-1088, then that many Jacobi bounces of `quality` rays per triangle,
+	// 0 = the first gather's) instead of with the reference's CPU solver; 0 = reference solver
+	void setGpuRadiosity(unsigned bounces, unsigned quality = 0) { gpuBounces = bounces; gpuBounceQuality = quality; }
+
 private:
 	// one object's output buffers by layer (LSG_LAYER_*), nullptr = not wanted
 	struct Target { unsigned object; rr::RRBuffer* buffer[LSG_NUM_LAYERS]; };
@@ -58,6 +63,7 @@ private:
 	void releaseScene();
 
 	bool referenceFallback = false;
+	unsigned gpuBounces = 0, gpuBounceQuality = 0;
 	lsg_collider* col = nullptr;
 	lsg_scene* scene = nullptr;
 	std::vector<float> albedo, emittance;       // per multiObject triangle

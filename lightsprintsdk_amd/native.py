@@ -99,6 +99,7 @@ SIGNATURES = {
     "lsg_scene_create": (ctypes.c_int, [P, P, P, P, P, U32, ctypes.POINTER(P)]),
     "lsg_scene_destroy": (None, [P]),
     "lsg_scene_set_exitance": (ctypes.c_int, [P, P]),
+    "lsg_scene_get_exitance": (ctypes.c_int, [P, P]),
     "lsg_scene_set_lights": (ctypes.c_int, [P, P, U32]),
     "lsg_gather_device": (ctypes.c_int, [P, P, U32, P, U32, ctypes.POINTER(GatherParams), P, P]),
     "lsg_gather_stats": (ctypes.c_int, [P, P, U32, P, U32, ctypes.POINTER(GatherParams), P, ctypes.POINTER(ctypes.c_double)]),
@@ -113,6 +114,7 @@ SIGNATURES = {
     "lsg_bake_run": (ctypes.c_int, [P, ctypes.POINTER(GatherParams), ctypes.POINTER(GatherParams), ctypes.POINTER(FilterParams), U32, P]),
     "lsg_bake_gather_sample": (ctypes.c_int, [P, ctypes.POINTER(GatherParams), U32, ctypes.POINTER(ctypes.c_float)]),
     "lsg_bake_download": (ctypes.c_int, [P, U32, P]),
+    "lsg_bake_set_bounces": (ctypes.c_int, [P, U32, U32]),
     "lsg_bake_run_layers": (ctypes.c_int, [P, ctypes.POINTER(GatherParams), ctypes.POINTER(GatherParams), ctypes.POINTER(FilterParams), U32, P]),
     "lsg_bake_download_layer": (ctypes.c_int, [P, U32, U32, P]),
     "lsg_mlp_num_params": (U32, []),
