@@ -198,6 +198,13 @@ int  lsg_scene_create(const lsg_collider* collider, const float* vertex_normals,
 void lsg_scene_destroy(lsg_scene* scene);
 /* exitance: host float[ntri*8] = per triangle (front r,g,b,pad, back r,g,b,pad) */
 int  lsg_scene_set_exitance(lsg_scene* scene, const float* exitance);
+/* exitance on the device: front = irradiance * albedo + emittance, back = 0 per triangle
+ * (PathtracerWorker::getIncidentRadiance at depth 0 with solver shortcuts,
+ * pathtracer.cpp:235-316), the table a first gather leaves.  irradiance: device
+ * float4[ntri], or NULL for emittance only; albedo, emittance: device float[ntri*3].
+ * Asynchronous on stream. */
+int  lsg_scene_exitance_device(lsg_scene* scene, const float* irradiance, const float* albedo, const float* emittance,
+                               void* stream);
 /* copies the current exitance table (same layout) to the host; synchronous */
 int  lsg_scene_get_exitance(const lsg_scene* scene, float* exitance);
 int  lsg_scene_set_lights(lsg_scene* scene, const lsg_light* lights, uint32_t nlights);
@@ -284,6 +291,14 @@ typedef struct lsg_filter_params {
  * output (copyElementsTo into a BF_RGBF buffer, RRBuffer.cpp:105).  Synchronous. */
 int  lsg_filter_lightmaps(const lsg_lightmap* maps, uint32_t nmaps, float* rgba, float* rgbf,
                           const lsg_filter_params* params, void* stream);
+
+/* The same filter chain with the batch and its scratch kept in HBM between runs, so
+ * repeated filtering (a sweep of re-bakes) allocates nothing.  run: as
+ * lsg_filter_lightmaps on the plan's lightmaps. */
+typedef struct lsg_filter_plan lsg_filter_plan;
+int  lsg_filter_plan_create(const lsg_lightmap* maps, uint32_t nmaps, lsg_filter_plan** out);
+void lsg_filter_plan_destroy(lsg_filter_plan* plan);
+int  lsg_filter_plan_run(lsg_filter_plan* plan, float* rgba, float* rgbf, const lsg_filter_params* params, void* stream);
 
 /* Texel rasterisation of one object's lightmap, RRSolver/lightmap.cpp:83-335
  * (enumerateTexelsPartial, steps 1-2): clips every texel square against every

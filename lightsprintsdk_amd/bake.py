@@ -291,35 +291,51 @@ class SceneBake:
                                          ray_length_min=self.params.ray_length_min, locality=self.params.locality,
                                          rug_distance=self.params.rug_distance,
                                          light_multiplier=self.params.light_multiplier, seed=self.params.seed ^ 0x5bd1e995)
-        self.exitance = ex0
+        self.d_albedo = DeviceBuffer.from_array(np.ascontiguousarray(scene.albedo, np.float32))
+        self.d_emit = DeviceBuffer.from_array(np.ascontiguousarray(scene.emittance, np.float32))
 
-    def first_gather(self):
-        """Per-triangle direct irradiance -> exitance table (uploaded to the scene).  The
-        direct illumination includes the environment (updateSolverDirectIllumination
-        reads indirect.environmentMultiplier, gather.cpp:958-975)."""
+    @property
+    def exitance(self):
+        """The scene's current exitance table, float32 [ntri, 8], read back from the GPU."""
+        return self.gs.get_exitance()
+
+    def first_gather(self, stream=None):
+        """Per-triangle direct irradiance -> exitance table, on the GPU.  The direct
+        illumination includes the environment (updateSolverDirectIllumination reads
+        indirect.environmentMultiplier, gather.cpp:958-975)."""
         nt = self.scene.ntri
         self.first_params.environment_multiplier = self.params.environment_multiplier
+        L = N.lib()
         # hemisphere rays of the first gather see emitters only, not an earlier result
-        ex0 = np.zeros((nt, 8), np.float32)
-        ex0[:, 0:3] = self.scene.emittance
-        self.gs.set_exitance(ex0)
-        self.gs.gather_device(self.d_ttex, nt, self.d_tsub, nt, self.first_params, self.d_tout)
-        N.check(N.lib().lsg_synchronize(), "sync")
-        e = self.d_tout.download(np.float32, (nt, 4))
-        ex = np.zeros((nt, 8), np.float32)
-        ex[:, 0:3] = e[:, 0:3] * self.scene.albedo + self.scene.emittance
-        self.exitance = ex
-        self.gs.set_exitance(ex)
-        return ex
+        N.check(L.lsg_scene_exitance_device(self.gs._h, None, self.d_albedo.ptr, self.d_emit.ptr, stream), "exitance")
+        self.gs.gather_device(self.d_ttex, nt, self.d_tsub, nt, self.first_params, self.d_tout, stream)
+        N.check(L.lsg_scene_exitance_device(self.gs._h, self.d_tout.ptr, self.d_albedo.ptr, self.d_emit.ptr, stream),
+                "exitance")
 
     def final_gather(self, stream=None):
         self.gs.gather_device(self.d_tex, len(self.texels), self.d_sub, len(self.subs), self.params, self.d_out, stream)
 
-    def filter(self, stream=None):
+    def _filter_plan(self):
+        """The lightmap batch and its filter scratch, kept in HBM between bakes."""
+        if getattr(self, "_fplan", None) is None:
+            h = ctypes.c_void_p()
+            N.check(N.lib().lsg_filter_plan_create(self.maps, len(self.slots), ctypes.byref(h)), "lsg_filter_plan_create")
+            self._fplan = h.value
+        return self._fplan
+
+    def filter(self, stream=None, rgba=None, rgbf=None):
         """updateLightmap's filter chain + RGBF writeback on every lightmap (GPU)."""
         fp = self.filtering.c()
-        N.check(N.lib().lsg_filter_lightmaps(self.maps, len(self.slots), self.d_out.ptr, self.d_rgbf.ptr, ctypes.byref(fp),
-                                             stream), "lsg_filter_lightmaps")
+        N.check(N.lib().lsg_filter_plan_run(self._filter_plan(), (rgba or self.d_out).ptr, (rgbf or self.d_rgbf).ptr,
+                                            ctypes.byref(fp), stream), "lsg_filter_plan_run")
+
+    def __del__(self):
+        try:
+            if getattr(self, "_fplan", None):
+                N.lib().lsg_filter_plan_destroy(self._fplan)
+                self._fplan = None
+        except Exception:
+            pass
 
     def clear(self):
         """Empty every lightmap slot (the filters grow into uncovered slots in place, and
@@ -343,11 +359,9 @@ class SceneBake:
             rgbfs[k] = self.d_rgbf if k == 0 else DeviceBuffer(12 * n)
             N.check(N.lib().lsg_memset(outs[k].ptr, 0, 16 * n), "memset")
         self.gs.gather_layers_device(self.d_tex, len(self.texels), self.d_sub, len(self.subs), self.params, outs, stream)
-        fp = self.filtering.c()
         res = {}
         for k in layers:
-            N.check(N.lib().lsg_filter_lightmaps(self.maps, len(self.slots), outs[k].ptr, rgbfs[k].ptr, ctypes.byref(fp),
-                                                 stream), "lsg_filter_lightmaps")
+            self.filter(stream, rgba=outs[k], rgbf=rgbfs[k])
         N.check(N.lib().lsg_synchronize(), "sync")
         for k in layers:
             out = rgbfs[k].download(np.float32, (n, 3))

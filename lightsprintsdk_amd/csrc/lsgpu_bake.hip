@@ -182,6 +182,18 @@ using namespace lsg;
 
 extern "C" {
 
+int lsg_scene_exitance_device(lsg_scene* scene, const float* irr, const float* albedo, const float* emit, void* stream)
+{
+    if (!scene || !albedo || !emit) return fail(LSG_ERR_INVALID, "lsg_scene_exitance_device: null argument");
+    const uint32_t nt = scene->impl.col->impl.ntri;
+    if (!nt) return LSG_OK;
+    hipStream_t st = (hipStream_t)stream;
+    if (irr) k_exitance<<<nblk(nt), 256, 0, st>>>(nt, (const float4*)irr, albedo, emit, scene->impl.d_exit);
+    else k_emittance_only<<<nblk(nt), 256, 0, st>>>(nt, emit, scene->impl.d_exit);
+    LSG_HIP(hipGetLastError());
+    return LSG_OK;
+}
+
 int lsg_bake_create(lsg_scene* scene, const lsg_bake_object* objects, uint32_t nobjects, const float* albedo,
                     const float* emittance, lsg_bake** out)
 {

@@ -14,7 +14,9 @@ struct FilterMap {
 struct FilterScratch {
     float4* a = nullptr; float4* b = nullptr; float4* c = nullptr;
     uint8_t* flags = nullptr;
-    uint32_t* changed = nullptr;
+    uint32_t* changed = nullptr;   // [0] last step changed; [1],[2] frontier sizes
+    uint32_t* front[2] = {nullptr, nullptr};   // lightmapGrow frontiers (texel indices)
+    uint32_t* mark = nullptr;      // texel already queued in a frontier
 };
 
 int maps_upload(const lsg_lightmap* maps, uint32_t nmaps, FilterMap** d_maps, uint32_t* total);

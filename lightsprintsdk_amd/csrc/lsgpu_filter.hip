@@ -148,30 +148,125 @@ __global__ __launch_bounds__(256) void k_grow_init(uint32_t total, const float4*
 }
 
 // one lightmapGrow step without wrap (RRBuffer.cpp:925-947)
-__global__ __launch_bounds__(256) void k_grow_step(const FilterMap* __restrict__ maps, uint32_t nmaps, uint32_t total,
-                                                   const float4* __restrict__ src, float4* __restrict__ dst, uint32_t* __restrict__ changed)
+
+// lightmapGrow by frontiers.  The reference's step (RRBuffer.cpp:912-947) changes exactly
+// the empty texels that have a filled neighbour in the previous state, and those are the
+// still-empty neighbours of the texels filled one step earlier.  So each step visits only
+// its frontier: every frontier texel's value is computed from the previous state
+// (1.6 x the 4 edge neighbours + the 4 corner neighbours, clamped at the map border,
+// divided by its alpha), then all are committed, then their empty neighbours are
+// queued once (mark) as the next frontier.  Grids are fixed and sizes live in device
+// memory, so a step captures into a graph; a step with an empty frontier does nothing,
+// as the reference stops there.
+constexpr unsigned kFrontBlocks = 2048;
+
+__device__ inline void grow_neighbours(const FilterMap& M, uint32_t k, int& x, int& y, int& xp, int& xm, int& yp, int& ym)
+{
+    const int w = (int)M.width, h = (int)M.height;
+    x = (int)(k % M.width); y = (int)(k / M.width);
+    xp = x + 1 < w - 1 ? x + 1 : w - 1; xm = x - 1 > 0 ? x - 1 : 0;
+    yp = y + 1 < h - 1 ? y + 1 : h - 1; ym = y - 1 > 0 ? y - 1 : 0;
+}
+
+// wave-aggregated append of t to list (count at *cnt) where pred holds
+__device__ inline void append(bool pred, uint32_t t, uint32_t* list, uint32_t* cnt)
+{
+    const uint64_t m = __ballot(pred);
+    if (!m) return;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(cnt, (uint32_t)__popcll(m));
+    base = (uint32_t)__shfl((int)base, (int)leader);
+    if (pred) list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = t;
+}
+
+__global__ __launch_bounds__(256) void k_front_seed(const FilterMap* __restrict__ maps, uint32_t nmaps, uint32_t total,
+                                                    const float4* __restrict__ g, uint32_t* __restrict__ mark,
+                                                    uint32_t* __restrict__ front, uint32_t* __restrict__ cnt)
 {
     const uint32_t t = blockIdx.x * 256 + threadIdx.x;
-    if (t >= total) return;
-    const FilterMap M = maps[find_map(maps, nmaps, t)];
-    const uint32_t k = t - M.first;
-    const int w = (int)M.width, h = (int)M.height;
-    const int x = (int)(k % M.width), y = (int)(k / M.width);
-    const float4* s = src + M.first;
-    float4 c = s[k];
-    if (c.w == 0) {
-        const int xp = x + 1 < w - 1 ? x + 1 : w - 1, xm = x - 1 > 0 ? x - 1 : 0;
-        const int yp = y + 1 < h - 1 ? y + 1 : h - 1, ym = y - 1 > 0 ? y - 1 : 0;
+    bool seed = false;
+    if (t < total) {
+        mark[t] = 0u;
+        if (g[t].w == 0) {
+            const FilterMap M = maps[find_map(maps, nmaps, t)];
+            int x, y, xp, xm, yp, ym;
+            grow_neighbours(M, t - M.first, x, y, xp, xm, yp, ym);
+            const float4* s = g + M.first;
+            const int w = (int)M.width;
+            seed = s[xp + y * w].w != 0 || s[xm + y * w].w != 0 || s[x + yp * w].w != 0 || s[x + ym * w].w != 0 ||
+                   s[xp + yp * w].w != 0 || s[xp + ym * w].w != 0 || s[xm + yp * w].w != 0 || s[xm + ym * w].w != 0;
+            if (seed) mark[t] = 1u;
+        }
+    }
+    append(seed, t, front, cnt);
+}
+
+__global__ __launch_bounds__(256) void k_front_compute(const FilterMap* __restrict__ maps, uint32_t nmaps,
+                                                       const float4* __restrict__ g, const uint32_t* __restrict__ front,
+                                                       const uint32_t* __restrict__ cnt, float4* __restrict__ v)
+{
+    const uint32_t n = *cnt;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const uint32_t t = front[i];
+        const FilterMap M = maps[find_map(maps, nmaps, t)];
+        int x, y, xp, xm, yp, ym;
+        grow_neighbours(M, t - M.first, x, y, xp, xm, yp, ym);
+        const float4* s = g + M.first;
+        const int w = (int)M.width;
         float4 e = f4add(f4add(f4add(s[xp + y * w], s[xm + y * w]), s[x + yp * w]), s[x + ym * w]);
         float4 dg = f4add(f4add(f4add(s[xp + yp * w], s[xp + ym * w]), s[xm + yp * w]), s[xm + ym * w]);
         float4 d = f4add(f4mul(e, 1.6f), dg);
-        if (d.w != 0) {
-            const float a = d.w;
-            c = make_float4(d.x / a, d.y / a, d.z / a, d.w / a);
-            *changed = 1u;
+        const float a = d.w;
+        v[i] = make_float4(d.x / a, d.y / a, d.z / a, d.w / a);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_front_commit(const uint32_t* __restrict__ front, const uint32_t* __restrict__ cnt,
+                                                      const float4* __restrict__ v, float4* __restrict__ g,
+                                                      uint32_t* __restrict__ next_cnt, uint32_t* __restrict__ log)
+{
+    const uint32_t n = *cnt;
+    if (blockIdx.x == 0 && threadIdx.x == 0) { *next_cnt = 0u; *log = n; }
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) g[front[i]] = v[i];
+}
+
+__global__ __launch_bounds__(256) void k_front_expand(const FilterMap* __restrict__ maps, uint32_t nmaps,
+                                                      const float4* __restrict__ g, const uint32_t* __restrict__ front,
+                                                      const uint32_t* __restrict__ cnt, uint32_t* __restrict__ mark,
+                                                      uint32_t* __restrict__ next, uint32_t* __restrict__ next_cnt)
+{
+    const uint32_t n = *cnt;
+    const uint32_t stride = gridDim.x * 256;
+    // whole waves walk the loop together so that append's ballot sees every lane
+    const uint32_t rounds = (n + stride - 1) / stride;
+    for (uint32_t r = 0; r < rounds; r++) {
+        const uint32_t i = r * stride + blockIdx.x * 256 + threadIdx.x;
+        const bool live = i < n;
+        uint32_t t = 0, first = 0;
+        int x = 0, y = 0, xp = 0, xm = 0, yp = 0, ym = 0, w = 1;
+        if (live) {
+            t = front[i];
+            const FilterMap M = maps[find_map(maps, nmaps, t)];
+            grow_neighbours(M, t - M.first, x, y, xp, xm, yp, ym);
+            first = M.first; w = (int)M.width;
+        }
+        const int nx[8] = {xp, xm, x, x, xp, xp, xm, xm};
+        const int ny[8] = {y, y, yp, ym, yp, ym, yp, ym};
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const uint32_t q = first + (uint32_t)(nx[j] + ny[j] * w);
+            const bool add = live && g[q].w == 0 && atomicCAS(&mark[q], 0u, 1u) == 0u;
+            append(add, q, next, next_cnt);
         }
     }
-    dst[M.first + k] = c;
+}
+
+__global__ void k_front_changed(const uint32_t* __restrict__ cnt, uint32_t* __restrict__ changed)
+{
+    // after a fixed number of steps: did the last one still leave a frontier to fill?
+    if (threadIdx.x == 0) changed[0] = *cnt != 0u ? 1u : 0u;
 }
 
 // lightmapGrow copy back (RRBuffer.cpp:952-956) + lightmapFillBackground (:964-975) +
@@ -229,40 +324,46 @@ int filter_run(const FilterMap* d_maps, uint32_t nmaps, uint32_t total, float4* 
     const float4* g = nullptr;
     uint32_t steps = (uint32_t)(fp.spread_foreground > 0 ? fp.spread_foreground : 0);
     if (steps_used) *steps_used = 0;
-    if (steps && fixed_steps >= 0) {
-        // graph-capturable form: a fixed number of steps, no host synchronisation, the
-        // last step's change flag left in S.changed[0] for the caller to check
+    if (steps) {
+        // frontier sizes live in S.changed[1] and [2]; step q reads front[q & 1]
         k_grow_init<<<nblk(total), 256, 0, s>>>(total, S.c, S.a);
-        float4* src = S.a; float4* dst = S.b;
-        const uint32_t n = std::min<uint32_t>((uint32_t)fixed_steps, steps);
-        for (uint32_t q = 0; q < n; q++) {
-            k_zero_u32<<<1, 256, 0, s>>>(S.changed, 1);
-            k_grow_step<<<nblk(total), 256, 0, s>>>(d_maps, nmaps, total, src, dst, S.changed);
-            std::swap(src, dst);
-        }
-        if (steps_used) *steps_used = n;
-        g = src;
-    } else if (steps) {
-        k_grow_init<<<nblk(total), 256, 0, s>>>(total, S.c, S.a);
-        float4* src = S.a; float4* dst = S.b;
-        uint32_t done = 0;
-        while (done < steps) {
-            const uint32_t chunk = std::min<uint32_t>(16, steps - done);
-            LSG_HIP(hipMemsetAsync(S.changed, 0, 4 * chunk, s));
-            for (uint32_t q = 0; q < chunk; q++) {
-                k_grow_step<<<nblk(total), 256, 0, s>>>(d_maps, nmaps, total, src, dst, S.changed + q);
-                std::swap(src, dst);
+        k_zero_u32<<<1, 256, 0, s>>>(S.changed, 20);
+        k_front_seed<<<nblk(total), 256, 0, s>>>(d_maps, nmaps, total, S.a, S.mark, S.front[0], S.changed + 1);
+        // S.changed[4 + q % 16] = frontier size of step q (the host reads a chunk at a time)
+        auto step = [&](uint32_t q) {
+            const uint32_t c = q & 1u;
+            uint32_t* cur = S.changed + 1 + c;
+            uint32_t* nxt = S.changed + 2 - c;
+            k_front_compute<<<kFrontBlocks, 256, 0, s>>>(d_maps, nmaps, S.a, S.front[c], cur, S.b);
+            k_front_commit<<<kFrontBlocks, 256, 0, s>>>(S.front[c], cur, S.b, S.a, nxt, S.changed + 4 + (q & 15u));
+            k_front_expand<<<kFrontBlocks, 256, 0, s>>>(d_maps, nmaps, S.a, S.front[c], cur, S.mark, S.front[1 - c], nxt);
+        };
+        if (fixed_steps >= 0) {
+            // graph-capturable form: a fixed number of steps, no host synchronisation,
+            // S.changed[0] = whether a frontier is still left after the last one
+            const uint32_t n = std::min<uint32_t>((uint32_t)fixed_steps, steps);
+            for (uint32_t q = 0; q < n; q++) step(q);
+            k_front_changed<<<1, 64, 0, s>>>(S.changed + 1 + (n & 1u), S.changed);
+            if (steps_used) *steps_used = n;
+        } else {
+            // a step with an empty frontier changes nothing: the reference stops there
+            // (RRBuffer.cpp:948-949); checked every 16 steps
+            uint32_t done = 0;
+            while (done < steps) {
+                const uint32_t chunk = std::min<uint32_t>(16, steps - done);
+                for (uint32_t q = 0; q < chunk; q++) step(done + q);
+                uint32_t sizes[16];
+                LSG_HIP(hipMemcpyAsync(sizes, S.changed + 4, 4 * 16, hipMemcpyDeviceToHost, s));
+                LSG_HIP(hipStreamSynchronize(s));
+                bool stop = false;
+                for (uint32_t q = 0; q < chunk; q++)
+                    if (!sizes[(done + q) & 15u]) { done += q + 1; stop = true; break; }
+                if (stop) break;
+                done += chunk;
             }
-            uint32_t ch[16] = {0};
-            LSG_HIP(hipMemcpyAsync(ch, S.changed, 4 * chunk, hipMemcpyDeviceToHost, s));
-            LSG_HIP(hipStreamSynchronize(s));
-            bool stop = false;
-            for (uint32_t q = 0; q < chunk; q++) if (!ch[q]) { stop = true; done += q + 1; break; }
-            if (stop) break;   // a step without change: the reference stops (RRBuffer.cpp:948-949)
-            done += chunk;
+            if (steps_used) *steps_used = done;
         }
-        if (steps_used) *steps_used = done;
-        g = src;
+        g = S.a;
     }
     k_finish<<<nblk(total), 256, 0, s>>>(d_maps, nmaps, total, S.c, g, make_float4(fp.background[0], fp.background[1], fp.background[2], fp.background[3]),
                                          rgbf, rgba_out ? rgba : nullptr);
@@ -278,7 +379,10 @@ int scratch_alloc(FilterScratch& S, uint32_t total)
     if ((e = hipMalloc(&S.b, 16 * n)) != hipSuccess) return hip_fail(e, "hipMalloc", __LINE__);
     if ((e = hipMalloc(&S.c, 16 * n)) != hipSuccess) return hip_fail(e, "hipMalloc", __LINE__);
     if ((e = hipMalloc(&S.flags, n)) != hipSuccess) return hip_fail(e, "hipMalloc", __LINE__);
-    if ((e = hipMalloc(&S.changed, 64)) != hipSuccess) return hip_fail(e, "hipMalloc", __LINE__);
+    if ((e = hipMalloc(&S.changed, 4 * 32)) != hipSuccess) return hip_fail(e, "hipMalloc", __LINE__);
+    if ((e = hipMalloc(&S.front[0], 4 * n)) != hipSuccess) return hip_fail(e, "hipMalloc", __LINE__);
+    if ((e = hipMalloc(&S.front[1], 4 * n)) != hipSuccess) return hip_fail(e, "hipMalloc", __LINE__);
+    if ((e = hipMalloc(&S.mark, 4 * n)) != hipSuccess) return hip_fail(e, "hipMalloc", __LINE__);
     return LSG_OK;
 }
 
@@ -289,6 +393,8 @@ void scratch_free(FilterScratch& S)
     if (S.c) (void)hipFree(S.c);
     if (S.flags) (void)hipFree(S.flags);
     if (S.changed) (void)hipFree(S.changed);
+    for (int i = 0; i < 2; i++) if (S.front[i]) (void)hipFree(S.front[i]);
+    if (S.mark) (void)hipFree(S.mark);
     S = FilterScratch();
 }
 
@@ -313,7 +419,45 @@ int maps_upload(const lsg_lightmap* maps, uint32_t nmaps, FilterMap** d_maps, ui
 
 } // namespace lsg
 
+struct lsg_filter_plan {
+    lsg::FilterMap* d_maps = nullptr;
+    uint32_t nmaps = 0, total = 0;
+    lsg::FilterScratch S;
+};
+
 using namespace lsg;
+
+extern "C" int lsg_filter_plan_create(const lsg_lightmap* maps, uint32_t nmaps, lsg_filter_plan** out)
+{
+    if (!out) return fail(LSG_ERR_INVALID, "lsg_filter_plan_create: out is null");
+    *out = nullptr;
+    if (!maps && nmaps) return fail(LSG_ERR_INVALID, "lsg_filter_plan_create: null maps");
+    lsg_filter_plan* f = new (std::nothrow) lsg_filter_plan;
+    if (!f) return fail(LSG_ERR_OOM, "lsg_filter_plan_create: host allocation");
+    int st = maps_upload(maps, nmaps, &f->d_maps, &f->total);
+    if (!st) st = scratch_alloc(f->S, f->total);
+    if (st) { scratch_free(f->S); if (f->d_maps) (void)hipFree(f->d_maps); delete f; return st; }
+    f->nmaps = nmaps;
+    *out = f;
+    return LSG_OK;
+}
+
+extern "C" void lsg_filter_plan_destroy(lsg_filter_plan* f)
+{
+    if (!f) return;
+    scratch_free(f->S);
+    if (f->d_maps) (void)hipFree(f->d_maps);
+    delete f;
+}
+
+extern "C" int lsg_filter_plan_run(lsg_filter_plan* f, float* rgba, float* rgbf, const lsg_filter_params* params, void* stream)
+{
+    if (!f || !rgba || !params) return fail(LSG_ERR_INVALID, "lsg_filter_plan_run: null argument");
+    if (params->wrap) return fail(LSG_ERR_INVALID, "lsg_filter_plan_run: wrap=true is not supported");
+    int st = filter_run(f->d_maps, f->nmaps, f->total, (float4*)rgba, rgbf, *params, f->S, (hipStream_t)stream, true, -1, nullptr);
+    if (!st) { hipError_t e = hipStreamSynchronize((hipStream_t)stream); if (e != hipSuccess) st = hip_fail(e, "sync", __LINE__); }
+    return st;
+}
 
 extern "C" int lsg_filter_lightmaps(const lsg_lightmap* maps, uint32_t nmaps, float* rgba, float* rgbf,
                                     const lsg_filter_params* params, void* stream)

@@ -100,6 +100,7 @@ SIGNATURES = {
     "lsg_scene_destroy": (None, [P]),
     "lsg_scene_set_exitance": (ctypes.c_int, [P, P]),
     "lsg_scene_get_exitance": (ctypes.c_int, [P, P]),
+    "lsg_scene_exitance_device": (ctypes.c_int, [P, P, P, P, P]),
     "lsg_scene_set_lights": (ctypes.c_int, [P, P, U32]),
     "lsg_gather_device": (ctypes.c_int, [P, P, U32, P, U32, ctypes.POINTER(GatherParams), P, P]),
     "lsg_gather_stats": (ctypes.c_int, [P, P, U32, P, U32, ctypes.POINTER(GatherParams), P, ctypes.POINTER(ctypes.c_double)]),
@@ -107,6 +108,9 @@ SIGNATURES = {
     "lsg_scene_set_environment": (ctypes.c_int, [P, I32, U32, U32, P]),
     "lsg_gather_layers_device": (ctypes.c_int, [P, P, U32, P, U32, ctypes.POINTER(GatherParams), P, P]),
     "lsg_filter_lightmaps": (ctypes.c_int, [P, U32, P, P, ctypes.POINTER(FilterParams), P]),
+    "lsg_filter_plan_create": (ctypes.c_int, [P, U32, ctypes.POINTER(P)]),
+    "lsg_filter_plan_destroy": (None, [P]),
+    "lsg_filter_plan_run": (ctypes.c_int, [P, P, P, ctypes.POINTER(FilterParams), P]),
     "lsg_bake_create": (ctypes.c_int, [P, P, U32, P, P, ctypes.POINTER(P)]),
     "lsg_bake_destroy": (None, [P]),
     "lsg_bake_info_get": (ctypes.c_int, [P, ctypes.POINTER(BakeInfo)]),
