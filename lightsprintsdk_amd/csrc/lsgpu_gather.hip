@@ -22,6 +22,7 @@
 #include "lsgpu_internal.h"
 #include "lsgpu_trace.h"
 #include "lsgpu_gather.h"
+#include <atomic>
 #include <vector>
 #include <cstring>
 
@@ -573,6 +574,142 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
     }
 }
 
+// processTexel for texel ti by one lane: the low ray counts of the cheap gather (at most
+// 32 shooters per series).  Texels of one quality walk the same shooter schedule, so the
+// lanes of a wave trace their hemisphere rays and their few shadow rays in step, with no
+// lanes idle while a texel's shadow rays run; nothing is shared between lanes.  The rays
+// are those of gather_texel (the same filler indices in shooter order, the same origins
+// and shots), summed per lane instead of across a group.
+template <int STACK, bool ENV>
+__device__ void gather_texel_lane(const GatherArgs& A, uint32_t ti, int32_t* stk, int stride)
+{
+    const lsg_texel T = A.texels[ti];
+    const uint32_t m = T.sub_count_flags & 0xffffu, flags = (T.sub_count_flags >> 16) & 0xffu;
+    const uint32_t hemi_rays = A.hemi_rays, rounds = A.rounds, nl = A.nlights;
+    const uint32_t light_rays = nl * rounds;
+    if (!m || (!hemi_rays && !light_rays)) {
+        A.out[T.out_index] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (A.parts) { A.parts[2 * (size_t)ti] = make_float4(0.f, 0.f, 0.f, 0.f); A.parts[2 * (size_t)ti + 1] = make_float4(0.f, 0.f, 0.f, 0.f); }
+        return;
+    }
+    const lsg_subtexel* ts = A.subs + T.sub_begin;
+    const uint32_t key = lowbias32(A.seed ^ lowbias32(T.key));
+    uint32_t filler = key & 0xffffu;
+    float area_max = 0.f;
+    for (uint32_t j = 0; j < m; j++) area_max += ts[j].area;
+    const float near_t = 1000 * A.tmin;
+    const float rug_below = __int_as_float(__float_as_int(A.rug) - 1);   // largest float < rugDistance (> 0)
+    float hr = 0.f, hg = 0.f, hb = 0.f, lr = 0.f, lg = 0.f, lb = 0.f;
+    uint32_t relH = 0, relL = 0, unrelL = 0, shotRounds = 0, shooter_base = 0;
+    for (;;) {
+        const bool shootH = (relH < hemi_rays || relH * 10 < hemi_rays) && hemi_rays > 0;
+        const bool shootL = (relL + unrelL < light_rays || relL * 10 < light_rays) && unrelL < light_rays * 100;
+        if (!shootH && !shootL) break;
+        const uint32_t nH = shootH ? hemi_rays : 0, nL = shootL ? rounds : 0;
+        const uint32_t n = nH > nL ? nH : nL;
+        const bool allH = !shootL || hemi_rays >= rounds;
+        const bool allL = !shootH || rounds >= hemi_rays;
+        for (uint32_t s = 0; s < n; s++) {
+            const bool doH = shootH && (allH || shots_before(s + 1, n, hemi_rays) > shots_before(s, n, hemi_rays));
+            const bool doL = shootL && (allL || shots_before(s + 1, n, rounds) > shots_before(s, n, rounds));
+            if (!doH && !doL) continue;
+            float3 o; float nx, ny, nz; uint32_t tri;
+            shooter_at(A, ts, m, area_max, n, s, key, shooter_base + s, o, nx, ny, nz, tri);
+            if (doH) {
+                float ox, oy, oz;                                               // RRMathPrivate.h:48-53
+                if (fabsf(nx) > fabsf(nz)) { ox = -ny; oy = nx; oz = 0.f; } else { ox = 0.f; oy = -nz; oz = ny; }
+                const float olen = sqrtf(ox * ox + oy * oy + oz * oz);
+                const float tx = ox / olen, ty = oy / olen, tz = oz / olen;
+                const float bx = ny * tz - nz * ty, by = -nx * tz + nz * tx, bz = nx * ty - ny * tx;
+                float x, y;
+                for (;;) {                                                      // GetCirclePoint, gather.cpp:85-90
+                    filler_point(filler++, x, y);
+                    if (!(x * x + y * y >= 1.f)) break;
+                }
+                const float dist = x * x + y * y;
+                const float cosa = sqrtf(1 - dist);                          // gather.cpp:125-126
+                const float3 d = make_float3(nx * cosa + tx * x + bx * y, ny * cosa + ty * x + by * y, nz * cosa + tz * x + bz * y);
+                GatherFilter flt{A.sc.verts, A.sc.idx, A.sc.side, tri, near_t, 0, rug_below};
+                TraceResult r;
+                uint32_t nv_ = 0, nt_ = 0;
+                trace_closest<STACK, false>(flt, A.sc.col, o, d, A.tmin, A.locality, stk, stride, r, nv_, nt_);
+                if (r.tri >= 0) {
+                    const float4 e = A.sc.exitance[2 * (size_t)r.tri + (r.front ? 0 : 1)];
+                    hr += e.x; hg += e.y; hb += e.z;
+                } else if constexpr (ENV) {
+                    const float4 v = env_at(A.sc, d);
+                    hr += v.x * A.env_mult; hg += v.y * A.env_mult; hb += v.z * A.env_mult;
+                }
+                relH++;
+            }
+            if (doL) {
+                GatherFilter flt{A.sc.verts, A.sc.idx, A.sc.side, tri, near_t, 1, rug_below};
+                for (uint32_t li = 0; li < nl; li++) {
+                    const lsg_light L = A.lights[li];
+                    float dx, dy, dz;
+                    if (L.type == 1) { dx = -L.direction[0]; dy = -L.direction[1]; dz = -L.direction[2]; }
+                    else { dx = L.position[0] - o.x; dy = L.position[1] - o.y; dz = L.position[2] - o.z; }
+                    float dsz = sqrtf(dx * dx + dy * dy + dz * dz);
+                    dx = dx / dsz; dy = dy / dsz; dz = dz / dsz;
+                    if (L.type == 1) dsz = dsz * A.locality;
+                    const float ni = dx * nx + dy * ny + dz * nz;            // gather.cpp:429
+                    if (ni <= 0 || !isfinite(ni)) { relL++; continue; }
+                    bool blocked = false; float bt = 0.f;
+                    if (L.cast_shadows) {
+                        TraceResult r;
+                        uint32_t nv_ = 0, nt_ = 0;
+                        trace_closest<STACK, false>(flt, A.sc.col, o, make_float3(dx, dy, dz), A.tmin, dsz, stk, stride, r, nv_, nt_);
+                        blocked = r.tri >= 0; bt = r.t;
+                    }
+                    if (!blocked) {
+                        // RRLight::getIrradiance (RRLight.cpp:345-362), REALISTIC attenuation
+                        float att = 1.f;
+                        if (L.type == 0) {
+                            const float ex = o.x - L.position[0], ey = o.y - L.position[1], ez = o.z - L.position[2];
+                            att = 1 / (0.000000000137289f + (ex * ex + ey * ey + ez * ez));
+                        }
+                        const float kl = att * A.light_mult * ni;
+                        lr += L.color[0] * kl; lg += L.color[1] * kl; lb += L.color[2] * kl;
+                    }
+                    const uint32_t unrel = (blocked && bt < A.rug) ? 1u : 0u;    // gather.cpp:483-511
+                    unrelL += unrel;
+                    relL += 1u - unrel;
+                }
+                shotRounds++;
+            }
+        }
+        shooter_base += n;
+    }
+    float r = 0.f, g = 0.f, b = 0.f;
+    float4 ph = make_float4(0.f, 0.f, 0.f, 0.f), pl = make_float4(0.f, 0.f, 0.f, 0.f);
+    bool reliable = false;
+    if (hemi_rays && relH) {
+        const float f = 1 / (float)relH;
+        ph = make_float4(hr * f, hg * f, hb * f, 1.f);
+        r += ph.x; g += ph.y; b += ph.z; reliable = true;
+    }
+    if (light_rays && relL) {
+        pl = make_float4(lr / (float)shotRounds, lg / (float)shotRounds, lb / (float)shotRounds, 1.f);
+        r += pl.x; g += pl.y; b += pl.z; reliable = true;
+    }
+    A.out[T.out_index] = make_float4(r, g, b, reliable ? flags / 255.f : 0.f);
+    if (A.parts) { A.parts[2 * (size_t)ti] = ph; A.parts[2 * (size_t)ti + 1] = pl; }
+}
+
+// launches of at least this many rounds of 64-texel waves per wave slot use k_gather_lanes
+constexpr uint32_t kLaneRounds = 16;
+
+// low ray counts without output layers: one texel per lane, 256 per block
+template <int STACK, bool ENV = false>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_gather_lanes(GatherArgs A)
+{
+    __shared__ int32_t stk[STACK * 256];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t ti = blockIdx.x * 256 + tid;
+    if (ti >= A.ntex) return;
+    gather_texel_lane<STACK, ENV>(A, ti, stk + tid, 256);
+}
+
 // One texel per wave, 4 per block.  The plain and environment kernels are capped at 128
 // VGPRs (4 waves per SIMD, measured faster than 3 with more registers); the layered ones
 // hold 12 more sums per lane and are left uncapped rather than spill.
@@ -768,6 +905,22 @@ int launch_gather(const GatherArgs& A, int stack, hipStream_t st)
     return LSG_OK;
 }
 
+// compute units of the current device (cached per device)
+static uint32_t device_cus()
+{
+    static std::atomic<uint32_t> cached[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    uint32_t c = cached[dev].load(std::memory_order_relaxed);
+    if (!c) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        c = (uint32_t)n;
+        cached[dev].store(c, std::memory_order_relaxed);
+    }
+    return c;
+}
+
 template <bool LAYERS, bool ENV>
 static void launch_gather_kernels(const GatherArgs& A, int stack, hipStream_t st)
 {
@@ -777,6 +930,11 @@ static void launch_gather_kernels(const GatherArgs& A, int stack, hipStream_t st
     if (series <= 32) {
         const uint32_t G = series <= 16 ? 16u : 32u;
         const unsigned grid = (unsigned)(((uint64_t)A.ntex * G + 255) / 256);
+        // one texel per lane makes each wave G times longer: worth it only when the
+        // launch fills every wave slot (4 per SIMD, 4 SIMDs per CU) many times over,
+        // else the last waves run on an almost idle chip (733k texels: 20% slower,
+        // 11.5M texels: 13% faster than groups)
+        const bool lanes = !LAYERS && (uint64_t)A.ntex >= (uint64_t)kLaneRounds * 64u * 16u * device_cus();
         if constexpr (LAYERS) {
             if (G == 16) {
                 if (stack <= 32) k_gather_grouped_layers<32, 16, ENV><<<grid, 256, 0, st>>>(A);
@@ -785,14 +943,16 @@ static void launch_gather_kernels(const GatherArgs& A, int stack, hipStream_t st
                 if (stack <= 32) k_gather_grouped_layers<32, 32, ENV><<<grid, 256, 0, st>>>(A);
                 else k_gather_grouped_layers<64, 32, ENV><<<grid, 256, 0, st>>>(A);
             }
+        } else if (lanes) {
+            const unsigned lgrid = (A.ntex + 255) / 256;
+            if (stack <= 32) k_gather_lanes<32, ENV><<<lgrid, 256, 0, st>>>(A);
+            else k_gather_lanes<64, ENV><<<lgrid, 256, 0, st>>>(A);
+        } else if (G == 16) {
+            if (stack <= 32) k_gather_grouped<32, 16, ENV><<<grid, 256, 0, st>>>(A);
+            else k_gather_grouped<64, 16, ENV><<<grid, 256, 0, st>>>(A);
         } else {
-            if (G == 16) {
-                if (stack <= 32) k_gather_grouped<32, 16, ENV><<<grid, 256, 0, st>>>(A);
-                else k_gather_grouped<64, 16, ENV><<<grid, 256, 0, st>>>(A);
-            } else {
-                if (stack <= 32) k_gather_grouped<32, 32, ENV><<<grid, 256, 0, st>>>(A);
-                else k_gather_grouped<64, 32, ENV><<<grid, 256, 0, st>>>(A);
-            }
+            if (stack <= 32) k_gather_grouped<32, 32, ENV><<<grid, 256, 0, st>>>(A);
+            else k_gather_grouped<64, 32, ENV><<<grid, 256, 0, st>>>(A);
         }
         return;
     }

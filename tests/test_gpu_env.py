@@ -109,3 +109,30 @@ def test_gpu_environment_plan_matches_scene_bake(gpu):
     plan.run(first=True, graph=True)
     assert not np.array_equal(plan.rgba(), want)
     plan.close()
+
+
+@pytest.mark.parametrize("kind,quality,rounds", [(1, 16, 0), (2, 20, 8)])
+def test_gpu_environment_lanes_vs_oracle(gpu, oracle, kind, quality, rounds):
+    """The same open block with 5.3M texels: low-quality environment gathers run one texel
+    per lane; the oracle re-gathers a spread and a random sample of texels."""
+    from lightsprintsdk_amd import bake as B
+    sc = S.city_terrain(seed=7, n_boxes=30, grid_vertices=40, terrain_tiles=2, lightmap_size=1024, city_maps=2,
+                        box_tris=3000, cell_m=1.0)
+    sb = B.SceneBake(sc, B.GatherParams(quality=quality, ray_length_min=S.minimal_safe_distance(sc.verts), seed=13,
+                                        environment_multiplier=0.7, light_rounds=rounds))
+    n = len(sb.texels)
+    assert n >= 16 * 64 * 16 * 256, n      # the size from which the one-texel-per-lane kernel runs
+    e = _env(kind, seed=kind + quality)
+    sb.gs.set_environment(e["kind"], e["width"], e["height"], e["rgb"])
+    sb.first_gather()
+    sb.final_gather()
+    flat = sb.d_out.download(np.float32, (sb.nout, 4))
+    rng = np.random.default_rng(kind)
+    sel = np.unique(np.concatenate([np.linspace(0, n - 1, 2000).astype(np.int64), rng.integers(0, n, 2000)]))
+    tx = sb.texels[sel]
+    ref, _ = oracle.gather(sb.gs.oracle_arrays(sb.exitance), sb.params.as_dict(), tx.view(np.uint32).reshape(-1, 4),
+                           sb.subs, tree=sb.gs.col.download_tree(), threads=16)
+    err = _cmp(flat, ref, tx["out_index"], "lanes")
+    print(f"env lanes kind {kind} quality {quality} rounds {rounds}: {n} texels, {len(sel)} sampled, "
+          f"max rel err {err:.2e}", flush=True)
+    sb.gs.close()

@@ -64,9 +64,9 @@ def test_gpu_final_gather_vs_oracle(gpu, oracle):
 
 @pytest.mark.parametrize("quality,rounds", [(16, 0), (20, 0), (32, 0), (16, 16), (16, 40)])
 def test_gpu_low_quality_grouped_vs_oracle(gpu, oracle, quality, rounds):
-    """At most 32 shooters per series run on groups of 16/32 lanes, several texels per
-    wavefront (the cheap gather of the learned path, with its own light-round count);
-    same rays, same results."""
+    """At most 32 shooters per series: launches this small run on groups of 16/32 lanes,
+    several texels per wavefront (the cheap gather of the learned path, with its own
+    light-round count); same rays, same results."""
     sc, sb = _bake(seed=5, quality=quality)
     sb.params.light_rounds = rounds
     sb.first_gather()
@@ -76,6 +76,41 @@ def test_gpu_low_quality_grouped_vs_oracle(gpu, oracle, quality, rounds):
     ref, _ = oracle.gather(sb.gs.oracle_arrays(sb.exitance), sb.params.as_dict(), tx.view(np.uint32).reshape(-1, 4),
                            sb.subs, tree=sb.gs.col.download_tree(), threads=16)
     print(f"quality {quality}, rounds {rounds}: {len(tx)} texels, max rel err {_cmp(flat, ref, tx['out_index']):.2e}", flush=True)
+
+
+# launches of at least this many texels run the cheap gather one texel per lane (16
+# rounds of 64-texel waves on every wave slot of a 256-CU MI355X); smaller ones use groups
+LANE_TEXELS = 16 * 64 * 16 * 256
+
+
+def _sample(n, k, seed):
+    rng = np.random.default_rng(seed)
+    return np.unique(np.concatenate([np.linspace(0, n - 1, k // 2).astype(np.int64), rng.integers(0, n, k // 2)]))
+
+
+def test_gpu_low_quality_lanes_vs_oracle(gpu, oracle):
+    """A box room with 4.8M texels: low-quality gathers run one texel per lane; the
+    oracle re-gathers a spread and a random sample of texels on the same rays."""
+    from lightsprintsdk_amd import bake as B
+    sc = S.box_room(seed=3, n_boxes=20, target_tris=50_000, lightmap_size=640)
+    sb = B.SceneBake(sc, B.GatherParams(quality=16, ray_length_min=S.minimal_safe_distance(sc.verts), seed=21))
+    n = len(sb.texels)
+    assert n >= LANE_TEXELS, n
+    sb.first_gather()
+    tree = sb.gs.col.download_tree()
+    sel = _sample(n, 4000, 5)
+    tx = sb.texels[sel]
+    for quality, rounds in [(16, 0), (16, 16), (20, 0), (12, 24)]:
+        sb.params.quality = quality
+        sb.params.light_rounds = rounds
+        sb.clear()
+        sb.final_gather()
+        flat = sb.d_out.download(np.float32, (sb.nout, 4))
+        ref, _ = oracle.gather(sb.gs.oracle_arrays(sb.exitance), sb.params.as_dict(), tx.view(np.uint32).reshape(-1, 4),
+                               sb.subs, tree=tree, threads=16)
+        print(f"lanes quality {quality}, rounds {rounds}: {n} texels, {len(sel)} sampled, "
+              f"max rel err {_cmp(flat, ref, tx['out_index']):.2e}", flush=True)
+    sb.gs.close()
 
 
 def test_gpu_config2_full_size_sample(gpu, oracle):
