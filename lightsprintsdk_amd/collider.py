@@ -49,6 +49,18 @@ def make_rrrays(rays: np.ndarray, flags: int = 0x3F) -> np.ndarray:
     return out
 
 
+def rebuild_sah(tree: dict) -> dict:
+    """lsg_bvh_rebuild_sah on a tree dict of download_tree's form (host only, no GPU):
+    the inner nodes rebuilt over the same leaves by binned SAH."""
+    nodes = np.ascontiguousarray(tree["nodes"], dtype=np.float32).reshape(-1, 16)
+    out = np.zeros((max(len(nodes), 1), 16), np.float32)
+    nn, root, depth = ctypes.c_uint32(), ctypes.c_int32(), ctypes.c_uint32()
+    N.check(N.lib().lsg_bvh_rebuild_sah(nodes.ctypes.data_as(ctypes.c_void_p), len(nodes), int(tree["root"]),
+                                        out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(nn), ctypes.byref(root),
+                                        ctypes.byref(depth)), "lsg_bvh_rebuild_sah")
+    return dict(tree, nodes=out[:nn.value], root=int(root.value), depth=int(depth.value))
+
+
 class DeviceBuffer:
     """Device allocation owned by Python (freed on close/GC)."""
 
@@ -115,12 +127,13 @@ class RRCollider:
 
     @staticmethod
     def create(verts: np.ndarray, idx: np.ndarray, technique: int = IT_HIP_LBVH, aborting=None,
-               cacheLocation=None, buildParams=None, leaf_size: int = 0):
-        """RRCollider::create for a mesh given as float32 [nv,3] / uint32 [nt,3]."""
+               cacheLocation=None, buildParams=None, leaf_size: int = 0, lbvh_only: bool = False):
+        """RRCollider::create for a mesh given as float32 [nv,3] / uint32 [nt,3].  lbvh_only keeps
+        the LBVH's inner nodes instead of rebuilding them by SAH."""
         v = np.ascontiguousarray(verts, dtype=np.float32)
         i = np.ascontiguousarray(idx, dtype=np.uint32)
         h = ctypes.c_void_p()
-        bp = N.BuildParams(leaf_size, 0)
+        bp = N.BuildParams(leaf_size, 1 if lbvh_only else 0)
         ab = None
         if aborting is not None:
             ab = ctypes.cast(ctypes.pointer(aborting), ctypes.c_void_p)

@@ -427,15 +427,13 @@ static inline unsigned blocks(uint64_t n, unsigned per = 256) { return (unsigned
 // at most H stack entries, so with P entries already on the stack a node may take the
 // children ch when P + |ch| - 1 + max H(inner child) <= budget.  With budget =
 // STACK - 1 >= BVH2 depth this holds at the root, and by construction below it.
-static int collapse4(ColliderImpl& C, uint32_t budget)
+static int collapse4(ColliderImpl& C, uint32_t budget, const std::vector<Node>& n2)
 {
     C.stack_need = 0;
     if (C.root < 0 || !C.nnodes) {
         C.root4 = C.root; C.nnodes4 = 0;
         return dalloc(&C.d_nodes4, 1, &C.bytes);
     }
-    std::vector<Node> n2(C.nnodes);
-    LSG_HIP(hipMemcpy(n2.data(), C.d_nodes, sizeof(Node) * n2.size(), hipMemcpyDeviceToHost));
     struct Ch { int32_t ref; float b[6]; };
     auto bits = [](float x) { int32_t r; memcpy(&r, &x, 4); return r; };
     auto kids = [&](int32_t i, Ch* out) {
@@ -507,7 +505,7 @@ static int collapse4(ColliderImpl& C, uint32_t budget)
 }
 
 static int build(ColliderImpl& C, const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt, uint32_t leafmax,
-                 const volatile int* aborting)
+                 bool sah, const volatile int* aborting)
 {
     auto t0 = std::chrono::steady_clock::now();
     int st;
@@ -621,9 +619,27 @@ static int build(ColliderImpl& C, const float* verts, uint32_t nv, const uint32_
     cleanup();
 #undef CK
 #undef CKH
+    std::vector<Node> n2(C.nnodes);
+    if (C.nnodes) LSG_HIP(hipMemcpy(n2.data(), C.d_nodes, sizeof(Node) * n2.size(), hipMemcpyDeviceToHost));
+    if (sah && C.root >= 0 && C.nnodes) {
+        // inner nodes rebuilt over the same leaves by SAH; kept unless the new tree would
+        // need the larger traversal stack
+        std::vector<Node> s2(C.nnodes);
+        uint32_t nn = 0, d = 0;
+        int32_t r = 0;
+        if (sah_rebuild((const float*)n2.data(), C.nnodes, C.root, (float*)s2.data(), &nn, &r, &d) == 0) {
+            const int have = stack_size_for_depth(C.depth), want = stack_size_for_depth(d);
+            if (want && (!have || want <= have)) {
+                s2.resize(nn);
+                LSG_HIP(hipMemcpy(C.d_nodes, s2.data(), sizeof(Node) * nn, hipMemcpyHostToDevice));
+                n2.swap(s2);
+                C.nnodes = nn; C.root = r; C.depth = d;
+            }
+        }
+    }
     if (!stack_size_for_depth(C.depth))
-        return fail(LSG_ERR_LIMIT, "LBVH depth %u exceeds the traversal stack (63)", C.depth);
-    if ((st = collapse4(C, (uint32_t)stack_size_for_depth(C.depth) - 1))) return st;
+        return fail(LSG_ERR_LIMIT, "BVH depth %u exceeds the traversal stack (63)", C.depth);
+    if ((st = collapse4(C, (uint32_t)stack_size_for_depth(C.depth) - 1, n2))) return st;
     C.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return LSG_OK;
 }
@@ -649,7 +665,8 @@ int lsg_collider_create(const float* verts, uint32_t nverts, const uint32_t* ind
     if (!device_ready()) return fail(LSG_ERR_NO_DEVICE, "lsg_collider_create: no gfx950 device");
     lsg_collider* c = new (std::nothrow) lsg_collider;
     if (!c) return fail(LSG_ERR_OOM, "lsg_collider_create: host allocation failed");
-    int st = build(c->impl, verts, nverts, indices, ntri, leaf, aborting);
+    const bool sah = !(params && (params->flags & LSG_BUILD_LBVH_ONLY));
+    int st = build(c->impl, verts, nverts, indices, ntri, leaf, sah, aborting);
     // Default leaf size: 2 triangles per leaf traverses fastest while the tree fits the
     // 32-entry traversal stack; a deeper tree would need the 64-entry stack, whose LDS
     // halves the waves per CU, so larger leaves are taken instead (up to 8).
@@ -657,7 +674,7 @@ int lsg_collider_create(const float* verts, uint32_t nverts, const uint32_t* ind
         free_impl(c->impl);
         c->impl = ColliderImpl();
         leaf *= 2;
-        st = build(c->impl, verts, nverts, indices, ntri, leaf, aborting);
+        st = build(c->impl, verts, nverts, indices, ntri, leaf, sah, aborting);
     }
     if (st) { free_impl(c->impl); delete c; return st; }
     *out = c;

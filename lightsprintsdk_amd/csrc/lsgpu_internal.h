@@ -136,6 +136,10 @@ struct ColliderImpl {
 };
 
 int stack_size_for_depth(uint32_t depth);   // 0 when too deep
+// SAH rebuild of a BVH2's inner nodes over its leaves (lsgpu_sah.cpp): 0 = rebuilt,
+// 1 = fewer than two leaves, 2 = malformed tree
+int sah_rebuild(const float* nodes, uint32_t nnodes, int32_t root, float* out, uint32_t* out_nodes, int32_t* out_root,
+                uint32_t* out_depth);
 
 } // namespace lsg
 

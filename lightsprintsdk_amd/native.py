@@ -92,6 +92,7 @@ SIGNATURES = {
     "lsg_collider_destroy": (None, [P]),
     "lsg_collider_info_get": (ctypes.c_int, [P, ctypes.POINTER(ColliderInfo)]),
     "lsg_collider_download": (ctypes.c_int, [P, P, P]),
+    "lsg_bvh_rebuild_sah": (ctypes.c_int, [P, U32, I32, P, P, P, P]),
     "lsg_collider_download_triangles": (ctypes.c_int, [P, P, P, P, P]),
     "lsg_collider_intersect_rrrays": (ctypes.c_int, [P, P, U32, P, ctypes.POINTER(U32)]),
     "lsg_collider_intersect_device": (ctypes.c_int, [P, P, P, U32, P, P, P]),

@@ -18,9 +18,9 @@ def _load(name):
     return np.load(os.path.join(GOLDEN, name + ".npz"))
 
 
-def _col(V, I, leaf=4):
+def _col(V, I, leaf=4, lbvh_only=False):
     from lightsprintsdk_amd.collider import RRCollider
-    return RRCollider.create(V, I, leaf_size=leaf)
+    return RRCollider.create(V, I, leaf_size=leaf, lbvh_only=lbvh_only)
 
 
 def _same_hits(g, gf, o, h):
@@ -34,14 +34,22 @@ def _same_hits(g, gf, o, h):
 
 @pytest.mark.parametrize("leaf", [1, 4, 8])
 def test_gpu_tree_equals_oracle_tree(gpu, oracle, leaf):
+    """The GPU LBVH equals the oracle's node for node; the default build's tree equals
+    the host SAH rebuild of that LBVH (lsg_bvh_rebuild_sah, checked on the CPU)."""
+    from lightsprintsdk_amd.collider import rebuild_sah
     V, I = scenes.icosphere(30, seed=11)
-    c = _col(V, I, leaf)
+    c = _col(V, I, leaf, lbvh_only=True)
     t_gpu = c.download_tree()
     t_cpu = oracle.lbvh_build(V, I, leaf_size=leaf)
     assert t_gpu["nvalid"] == t_cpu["nvalid"] and t_gpu["root"] == t_cpu["root"]
     assert t_gpu["depth"] == t_cpu["depth"]
     np.testing.assert_array_equal(t_gpu["prim"], t_cpu["prim"])
     np.testing.assert_array_equal(t_gpu["nodes"].view(np.uint32), t_cpu["nodes"].view(np.uint32))
+    t_sah = _col(V, I, leaf).download_tree()
+    want = rebuild_sah(t_cpu)
+    assert t_sah["root"] == want["root"] and t_sah["depth"] == want["depth"] <= t_cpu["depth"]
+    np.testing.assert_array_equal(t_sah["prim"], t_cpu["prim"])
+    np.testing.assert_array_equal(t_sah["nodes"].view(np.uint32), want["nodes"].view(np.uint32))
 
 
 def test_gpu_vs_oracle_small(gpu, oracle):
