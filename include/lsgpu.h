@@ -81,7 +81,8 @@ int lsg_event_elapsed_ms(void* start, void* stop, float* ms);
  * technique IT_HIP_LBVH: builds a linear BVH on the GPU (Morton codes, LSD radix sort,
  * Karras radix tree, bottom-up refit) over the mesh given as host arrays, then
  * rebuilds its inner nodes over the same leaves with a binned surface-area heuristic
- * (lsg_bvh_rebuild_sah; the reference's BSP build is SAH-driven too, IntersectBsp.h).
+ * (lsg_bvh_rebuild_sah; the reference's kd splits weigh box surface areas too,
+ * RRCollider/bsp.cpp:388-574).
  * Triangles whose SRLNP precompute is degenerate are left out, as the reference's
  * build does (IntersectBsp.h:252-262, IntersectBspFast.cpp:775-780). */
 #define LSG_BUILD_LBVH_ONLY 1u   /* keep the LBVH's inner nodes (no SAH rebuild) */
@@ -115,13 +116,15 @@ int  lsg_collider_info_get(const lsg_collider* collider, lsg_collider_info* info
 int  lsg_collider_download(const lsg_collider* collider, float* nodes, uint32_t* prim);
 
 /* Host-only (no GPU call): rebuilds the inner nodes of a BVH2 in the download layout
- * over its leaves with a top-down binned SAH (32 bins per axis on leaf-box centroids),
+ * over its leaves with a top-down binned SAH (32 bins per axis on leaf-box centroids,
+ * as many as leaves below 32),
  * keeping every leaf's triangle range and inflated box.  out_nodes holds num_nodes * 16
  * floats; the result is numbered in preorder from root 0 and depends only on the input.
  * out_depth = inner nodes on the longest root path.  A tree with fewer than two leaves is
  * copied unchanged (out_depth 0).  The step of lsg_collider_create after the LBVH build;
- * exported so that it can be checked on the host.  The reference builds its BSP with its
- * own SAH-driven builder (IntersectBsp.h createAndSaveBsp); this has no direct
+ * exported so that it can be checked on the host.  The reference builds its BSP/kd tree
+ * with its own builder (createAndSaveBsp, RRCollider/bsp.cpp:1192; kd split planes weighed
+ * by box surface areas, find_best_root_kd, bsp.cpp:388-574); this has no direct
  * counterpart. */
 int  lsg_bvh_rebuild_sah(const float* nodes, uint32_t num_nodes, int32_t root, float* out_nodes,
                          uint32_t* out_num_nodes, int32_t* out_root, uint32_t* out_depth);
