@@ -43,7 +43,7 @@ HBM_PEAK_GBS = 8000.0
 NODE_BYTES = 128          # one 4-wide node per traversal step (n_node counts these)
 # L2-to-fabric bytes per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by
 # the same counters on a 1 GiB copy (x2.0 and x1.0): profiles/pmc_traffic_k_gather_k_trace_closest.md
-TRAFFIC_BYTES = {"k_gather": 4.671e9, "k_trace_closest": 1.519e9}
+TRAFFIC_BYTES = {"k_gather": 4.620e9, "k_trace_closest": 1.480e9}
 METRIC = "closest-hit Mrays/s; baked texels/s at ≤2% RMSE, plain vs learned gather"
 
 
@@ -122,7 +122,7 @@ def collider_cpu_baseline(V, I, R, threads, col):
     O.bvh_traverse(tree, V, I, R[:n], threads=threads)
     dt = time.perf_counter() - t0
     return dict(value=round(n / dt / 1e6, 4), unit="Mrays/s", cores=threads, kind="port",
-                sample=f"oracle C traversal of the same LBVH, {n} of the rays")
+                sample=f"oracle C traversal of the same tree, {n} of the rays")
 
 
 # ---- config 2: plain bake ----------------------------------------------------------------
@@ -216,7 +216,7 @@ def bake_cpu_baseline(sb, threads: int, target_s: float = 12.0):
     O.gather(arrays, sb.params.as_dict(), sb.texels[sel].view(np.uint32).reshape(-1, 4), sb.subs, tree=tree, threads=threads)
     dt = time.perf_counter() - t0
     return dict(value=round(n1 / dt, 1), unit="texels/s", cores=threads, kind="port",
-                sample=f"{n1} of {len(sb.texels)} texels evenly spread, oracle C gather (FP64 sums) over the same LBVH, "
+                sample=f"{n1} of {len(sb.texels)} texels evenly spread, oracle C gather (FP64 sums) over the same tree, "
                        f"{dt:.1f} s")
 
 
