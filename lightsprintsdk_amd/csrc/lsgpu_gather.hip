@@ -719,7 +719,8 @@ __device__ inline void gather_block(const GatherArgs& A)
     __shared__ int32_t stk[STACK * 256];
     __shared__ uint32_t lq[2 * 256];
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
-    const uint32_t ti = blockIdx.x * 4u + (tid >> 6);
+    // wave-uniform: kept in a scalar register, the texel record is read by scalar loads
+    const uint32_t ti = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4u + (tid >> 6)));
     if (ti >= A.ntex) return;                 // whole wavefront leaves together
     gather_texel<STACK, STATS, 64, LAYERS, ENV>(A, ti, lane, stk + tid, lq + tid, 256);
 }
@@ -781,7 +782,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4)))
         for (;;) {
             uint32_t k = 0;
             if (lane == 0) k = atomicAdd(&Q.counters[q], 1u);
-            k = (uint32_t)__shfl((int)k, 0);
+            k = (uint32_t)__builtin_amdgcn_readfirstlane((int)k);   // lane 0's tile, scalar
             const uint32_t tile = t0 + k;
             if (tile >= t1) break;           // this queue is exhausted for every lane of the wave
             const uint32_t a = tile * Q.tile, b = min(A.ntex, a + Q.tile);
