@@ -889,16 +889,20 @@ int launch_gather_persistent(const GatherArgs& A, int stack, const WorkQueue& Q,
 
 static int launch_gather_one(const GatherArgs& A, int stack, hipStream_t st);
 
-// very large texel lists go out in launches of at most kLaunchTexels texels (a few
-// seconds of work each at quality 1024)
+// very large texel lists go out in launches of at most kLaunchTexels texels at 1024
+// shooters per series (a few seconds of work each), proportionally more at fewer
+// shooters (fewer launch drains for low-quality gathers), at most 2^25 texels (64
+// threads per texel stay below 2^31 threads per launch)
 constexpr uint32_t kLaunchTexels = 1u << 22;
 
 int launch_gather(const GatherArgs& A, int stack, hipStream_t st)
 {
-    for (uint32_t base = 0; base < A.ntex; base += kLaunchTexels) {
+    const uint32_t series = std::max(std::max(A.hemi_rays, A.rounds), 1u);
+    const uint32_t chunk = (uint32_t)std::min<uint64_t>((uint64_t)kLaunchTexels * std::max(1u, 1024u / series), 1u << 25);
+    for (uint64_t base = 0; base < A.ntex; base += chunk) {
         GatherArgs B = A;
         B.texels = A.texels + base;
-        B.ntex = std::min(kLaunchTexels, A.ntex - base);
+        B.ntex = (uint32_t)std::min<uint64_t>(chunk, A.ntex - base);
         if (A.parts) B.parts = A.parts + 2 * (size_t)base;
         const int e = launch_gather_one(B, stack, st);
         if (e) return e;
