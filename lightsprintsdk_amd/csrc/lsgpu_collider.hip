@@ -299,14 +299,17 @@ __global__ __launch_bounds__(256) void k_write_nodes(int n, uint32_t leafmax, fl
 __global__ __launch_bounds__(256) void k_write_tris(uint32_t n, const uint32_t* __restrict__ order, const float4* __restrict__ plane_u,
                                                     const float4* __restrict__ sv_u, const float4* __restrict__ ev_u,
                                                     const uint32_t* __restrict__ bytes, const float4* __restrict__ blo_u,
-                                                    const float4* __restrict__ bhi_u, float4* __restrict__ plane, float4* __restrict__ sv,
-                                                    float4* __restrict__ ev, uint2* __restrict__ meta, float4* __restrict__ plo, float4* __restrict__ phi)
+                                                    const float4* __restrict__ bhi_u, float4* __restrict__ plane, TriRest* __restrict__ rest,
+                                                    float4* __restrict__ plo, float4* __restrict__ phi)
 {
     uint32_t p = blockIdx.x * 256 + threadIdx.x;
     if (p >= n) return;
     uint32_t t = order[p];
-    plane[p] = plane_u[t]; sv[p] = sv_u[t]; ev[p] = ev_u[t];
-    meta[p] = make_uint2(t, bytes[t]);
+    plane[p] = plane_u[t];
+    TriRest r;
+    r.sv = sv_u[t]; r.ev = ev_u[t];
+    r.meta = make_uint2(t, bytes[t]); r.pad = make_uint2(0u, 0u);
+    rest[p] = r;
     plo[p] = blo_u[t]; phi[p] = bhi_u[t];
 }
 
@@ -415,7 +418,7 @@ template <class T> static int dalloc(T** p, size_t n, uint64_t* acc = nullptr)
 static void free_impl(ColliderImpl& c)
 {
     dfree(c.d_verts); dfree(c.d_idx); dfree(c.d_nodes); dfree(c.d_nodes4);
-    dfree(c.d_plane); dfree(c.d_sv); dfree(c.d_ev); dfree(c.d_meta);
+    dfree(c.d_plane); dfree(c.d_rest);
     c = ColliderImpl();
 }
 
@@ -591,10 +594,10 @@ static int build(ColliderImpl& C, const float* verts, uint32_t nv, const uint32_
         // after 4 passes results are back in d_keys/d_vals
     }
     const uint32_t n = nvalid;
-    CK(dalloc(&C.d_plane, n, &C.bytes)); CK(dalloc(&C.d_sv, n, &C.bytes)); CK(dalloc(&C.d_ev, n, &C.bytes)); CK(dalloc(&C.d_meta, n, &C.bytes));
+    CK(dalloc(&C.d_plane, n, &C.bytes)); CK(dalloc(&C.d_rest, n, &C.bytes));
     CK(dalloc(&d_plo, n)); CK(dalloc(&d_phi, n));
     if (n) k_write_tris<<<blocks(n), 256>>>(n, d_vals, d_plane_u, d_sv_u, d_ev_u, d_bytes, d_blo_u, d_bhi_u,
-                                            C.d_plane, C.d_sv, C.d_ev, C.d_meta, d_plo, d_phi);
+                                            C.d_plane, C.d_rest, d_plo, d_phi);
     CKH(hipGetLastError());
     C.nnodes = n > 1 ? n - 1 : 0;
     CK(dalloc(&C.d_nodes, C.nnodes ? C.nnodes : 1, &C.bytes));
@@ -706,9 +709,9 @@ int lsg_collider_download(const lsg_collider* c, float* nodes, uint32_t* prim)
     const ColliderImpl& C = c->impl;
     if (nodes && C.nnodes) LSG_HIP(hipMemcpy(nodes, C.d_nodes, sizeof(Node) * (size_t)C.nnodes, hipMemcpyDeviceToHost));
     if (prim && C.nvalid) {
-        std::vector<uint2> m(C.nvalid);
-        LSG_HIP(hipMemcpy(m.data(), C.d_meta, sizeof(uint2) * (size_t)C.nvalid, hipMemcpyDeviceToHost));
-        for (uint32_t i = 0; i < C.nvalid; i++) prim[i] = m[i].x;
+        std::vector<TriRest> m(C.nvalid);
+        LSG_HIP(hipMemcpy(m.data(), C.d_rest, sizeof(TriRest) * (size_t)C.nvalid, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < C.nvalid; i++) prim[i] = m[i].meta.x;
     }
     return LSG_OK;
 }
@@ -720,9 +723,15 @@ int lsg_collider_download_triangles(const lsg_collider* c, float* plane, float* 
     const size_t n = C.nvalid;
     if (!n) return LSG_OK;
     if (plane) LSG_HIP(hipMemcpy(plane, C.d_plane, 16 * n, hipMemcpyDeviceToHost));
-    if (sv) LSG_HIP(hipMemcpy(sv, C.d_sv, 16 * n, hipMemcpyDeviceToHost));
-    if (ev) LSG_HIP(hipMemcpy(ev, C.d_ev, 16 * n, hipMemcpyDeviceToHost));
-    if (meta) LSG_HIP(hipMemcpy(meta, C.d_meta, 8 * n, hipMemcpyDeviceToHost));
+    if (sv || ev || meta) {
+        std::vector<TriRest> m(n);
+        LSG_HIP(hipMemcpy(m.data(), C.d_rest, sizeof(TriRest) * n, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; i++) {
+            if (sv) memcpy(sv + 4 * i, &m[i].sv, 16);
+            if (ev) memcpy(ev + 4 * i, &m[i].ev, 16);
+            if (meta) { meta[2 * i] = m[i].meta.x; meta[2 * i + 1] = m[i].meta.y; }
+        }
+    }
     return LSG_OK;
 }
 

@@ -101,11 +101,15 @@ __device__ inline bool srlnp_test(const float4 sv, const float4 ev, int byte, co
 }
 
 // ---- device-side collider view ---------------------------------------------------------
+// The rest of a triangle's test record in leaf order, 48 bytes: projected edge data and
+// (original index, intersectByte), read together once a ray passes the plane test.
+struct alignas(16) TriRest { float4 sv, ev; uint2 meta; uint2 pad; };
+
 struct ColliderView {
     const Node* nodes;
     const Node4* nodes4;
     int32_t root4;
-    const float4* plane; const float4* sv; const float4* ev; const uint2* meta;
+    const float4* plane; const TriRest* rest;
     int32_t root;
     uint32_t nvalid;
     float3 bmin, bmax;     // inflated mesh box
@@ -120,13 +124,13 @@ struct ColliderImpl {
     float bmin[3], bmax[3], margin = 0;
     float* d_verts = nullptr; uint32_t* d_idx = nullptr;
     Node* d_nodes = nullptr;
-    float4* d_plane = nullptr; float4* d_sv = nullptr; float4* d_ev = nullptr; uint2* d_meta = nullptr;
+    float4* d_plane = nullptr; TriRest* d_rest = nullptr;
     uint64_t bytes = 0;
     double build_ms = 0;
     ColliderView view() const
     {
         ColliderView v;
-        v.nodes = d_nodes; v.plane = d_plane; v.sv = d_sv; v.ev = d_ev; v.meta = d_meta;
+        v.nodes = d_nodes; v.plane = d_plane; v.rest = d_rest;
         v.nodes4 = d_nodes4; v.root4 = root4;
         v.root = root; v.nvalid = nvalid;
         v.bmin = make_float3(bmin[0], bmin[1], bmin[2]);

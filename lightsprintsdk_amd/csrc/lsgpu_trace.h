@@ -164,10 +164,11 @@ __device__ inline bool trace_closest(const Filter& flt, const ColliderView& c, c
                 float ndd;
                 const float t = plane_distance(pl, o, d, ndd);
                 if (!(t >= lo && t <= bound)) continue;
-                const uint2 mt = c.meta[p];
+                const TriRest* tr = c.rest + p;
+                const uint2 mt = tr->meta;
                 if (!(bi < 0 || t < best || (int)mt.x < bi)) continue;
                 float u, v;
-                if (!srlnp_test(c.sv[p], c.ev[p], (int)mt.y, o, d, t, u, v)) continue;
+                if (!srlnp_test(tr->sv, tr->ev, (int)mt.y, o, d, t, u, v)) continue;
                 const bool fr = ndd < 0;
                 if (!flt.accept(mt.x, fr, t)) continue;
                 best = t; bi = (int)mt.x; bu = u; bv = v; bf = fr; bslot = p;
@@ -235,10 +236,11 @@ __device__ inline bool trace_resume(TraceState& S, const Filter& flt, const Coll
                 float ndd;
                 const float t = plane_distance(pl, S.o, S.d, ndd);
                 if (!(t >= S.lo && t <= S.bound)) continue;
-                const uint2 mt = c.meta[p];
+                const TriRest* tr = c.rest + p;
+                const uint2 mt = tr->meta;
                 if (!(S.bi < 0 || t < S.best || (int)mt.x < S.bi)) continue;
                 float u, v;
-                if (!srlnp_test(c.sv[p], c.ev[p], (int)mt.y, S.o, S.d, t, u, v)) continue;
+                if (!srlnp_test(tr->sv, tr->ev, (int)mt.y, S.o, S.d, t, u, v)) continue;
                 const bool fr = ndd < 0;
                 if (!flt.accept(mt.x, fr, t)) continue;
                 S.best = t; S.bi = (int)mt.x; S.bu = u; S.bv = v; S.bf = fr;
