@@ -149,15 +149,18 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
 }
 
 // Inference in one launch: the whole network per 16-row tile, one tile per wave at a
-// time, four waves per workgroup, kInferTiles tiles per wave (the weights are loaded
-// into LDS once per workgroup).  The weights (transposed, BF16) and biases sit in LDS; each
+// time, four waves per workgroup, workgroups striding over 64-row groups (the weights are
+// loaded into LDS once per workgroup).  The weights (transposed, BF16) and biases sit in LDS; each
 // wave's activations ping-pong between two LDS tiles.  The products are the same MFMA
 // calls in the same order as k_gemm_bf16's (same lane layout, same K steps), and bias,
 // ReLU and BF16 rounding are k_bias_act's, so the output equals the layered forward
 // pass bit for bit.
-constexpr int kWElems = kFeat * 64 + 3 * 64 * 64 + 64 * 16;
+// LDS rows are padded by 4 elements (8 bytes): 16 lanes reading 16 rows at one column
+// then hit 16 different bank pairs instead of one bank (row pitches of 64/128 bytes)
+constexpr int kPad = 4;
+constexpr int kWElems = (kFeat + kPad) * 64 + 3 * (64 + kPad) * 64 + (64 + kPad) * 16;
 constexpr int kBElems = 4 * 64 + 16;
-constexpr uint32_t kInferTiles = 8;                       // 16-row tiles per wave -> 512 rows per workgroup
+constexpr uint32_t kInferBlocks = 2048;                   // workgroups of the grid-stride inference launch
 struct MlpOffsets { uint32_t w[kLayers]; uint32_t b[kLayers]; };
 
 __global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ x, uint32_t n, const uint16_t* __restrict__ wb,
@@ -165,30 +168,40 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ x, 
 {
     constexpr int kInL[kLayers] = {kFeat, 64, 64, 64, 64};
     constexpr int kOutL[kLayers] = {64, 64, 64, 64, 16};
-    __shared__ uint16_t wt[kWElems];                 // layer l at tofs: W_l transposed, [out][in]
+    __shared__ __attribute__((aligned(16))) uint16_t wt[kWElems];   // layer l at tofs: W_l transposed, [out][in]
     __shared__ float bias[kBElems];
-    __shared__ uint16_t act[4][2][16 * 64];
+    __shared__ __attribute__((aligned(16))) uint16_t act[4][2][16 * (64 + kPad)];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     int tofs = 0, bofs = 0;
     for (int l = 0; l < kLayers; l++) {
         const int in = kInL[l], out = kOutL[l];
         for (int e = tid; e < in * out; e += 256) {
             const int k = e / out, c = e % out;
-            wt[tofs + c * in + k] = wb[off.w[l] + e];
+            wt[tofs + c * (in + kPad) + k] = wb[off.w[l] + e];
         }
         for (int c = tid; c < out; c += 256) bias[bofs + c] = p[off.b[l] + c];
-        tofs += in * out; bofs += out;
+        tofs += (in + kPad) * out; bofs += out;
     }
+    __syncthreads();                                 // weights and biases in place
     const int r = lane & 15, kq = (lane >> 4) * 4;
-    for (uint32_t it = 0; it < kInferTiles; it++) {
-    const uint64_t row0 = ((uint64_t)(blockIdx.x * kInferTiles + it) * 4u + (uint32_t)wave) * 16u;
+    // each wave owns its activation tiles: within the tile loop only the wave's own LDS
+    // writes must be visible to its lanes (no workgroup barrier)
+    auto wave_sync = []() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    // grid-stride over 64-row groups: the weights are loaded into LDS once per workgroup
+    const uint32_t groups = (uint32_t)(((uint64_t)n + 63) / 64);
+    for (uint32_t it = blockIdx.x; it < groups; it += gridDim.x) {
+    const uint64_t row0 = ((uint64_t)it * 4u + (uint32_t)wave) * 16u;
     uint16_t* A = act[wave][0];
     uint16_t* B = act[wave][1];
     for (int e = lane; e < 16 * kFeat; e += 64) {
         const uint32_t rr = (uint32_t)(e / kFeat), k = (uint32_t)(e % kFeat);
-        A[e] = f2bf(row0 + rr < n ? x[(size_t)(row0 + rr) * kFeat + k] : 0.f);
+        A[rr * (kFeat + kPad) + k] = f2bf(row0 + rr < n ? x[(size_t)(row0 + rr) * kFeat + k] : 0.f);
     }
-    __syncthreads();
+    wave_sync();
     tofs = 0; bofs = 0;
     for (int l = 0; l < kLayers; l++) {
         const int in = kInL[l], out = kOutL[l];
@@ -196,24 +209,23 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ x, 
         for (int tile = 0; tile < out / 16; tile++) {
             v4f acc = {0.f, 0.f, 0.f, 0.f};
             for (int k0 = 0; k0 < in; k0 += 16) {
-                v4s a, b;
-                for (int i = 0; i < 4; i++) {
-                    a[i] = (short)A[r * in + k0 + kq + i];
-                    b[i] = (short)W[(tile * 16 + r) * in + k0 + kq + i];
-                }
+                // four consecutive BF16 values per operand: one 8-byte LDS read each
+                // (offsets are multiples of 4 elements: pitches, k0 and kq are)
+                const v4s a = *(const v4s*)(A + r * (in + kPad) + k0 + kq);
+                const v4s b = *(const v4s*)(W + (tile * 16 + r) * (in + kPad) + k0 + kq);
                 acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
             }
             const int col = tile * 16 + r;
             const float bb = bias[bofs + col];
             for (int i = 0; i < 4; i++) {
                 const float z = acc[i] + bb;
-                if (l < kLayers - 1) B[(kq + i) * out + col] = f2bf(z > 0.f ? z : 0.f);
+                if (l < kLayers - 1) B[(kq + i) * (out + kPad) + col] = f2bf(z > 0.f ? z : 0.f);
                 else if (col < kOutUsed && row0 + kq + i < n) y[(size_t)(row0 + kq + i) * kOutUsed + col] = z;
             }
         }
-        __syncthreads();
+        wave_sync();
         uint16_t* t = A; A = B; B = t;
-        tofs += in * out; bofs += out;
+        tofs += (in + kPad) * out; bofs += out;
     }
     }
 }
@@ -340,7 +352,8 @@ int lsg_mlp_forward(lsg_mlp* m, const float* x, uint32_t n, float* y, void* stre
     if (!n) return LSG_OK;
     MlpOffsets off;
     for (int l = 0; l < kLayers; l++) { off.w[l] = (uint32_t)M.wofs[l]; off.b[l] = (uint32_t)M.bofs[l]; }
-    k_mlp_infer<<<(unsigned)(((uint64_t)n + 64 * kInferTiles - 1) / (64 * kInferTiles)), 256, 0, s>>>(x, n, M.d_wb, M.d_p, off, y);
+    const uint64_t groups = ((uint64_t)n + 63) / 64;
+    k_mlp_infer<<<(unsigned)std::min<uint64_t>(groups, kInferBlocks), 256, 0, s>>>(x, n, M.d_wb, M.d_p, off, y);
     LSG_HIP(hipGetLastError());
     return LSG_OK;
 }
