@@ -154,10 +154,9 @@ __global__ __launch_bounds__(256) void k_grow_init(uint32_t total, const float4*
 // still-empty neighbours of the texels filled one step earlier.  So each step visits only
 // its frontier: every frontier texel's value is computed from the previous state
 // (1.6 x the 4 edge neighbours + the 4 corner neighbours, clamped at the map border,
-// divided by its alpha), then all are committed, then their empty neighbours are
-// queued once (mark) as the next frontier.  Grids are fixed and sizes live in device
-// memory, so a step captures into a graph; a step with an empty frontier does nothing,
-// as the reference stops there.
+// divided by its alpha) and written, and its empty neighbours are queued once (mark) as
+// the next frontier.  Grids are fixed and sizes live in device memory, so a step captures
+// into a graph; a step with an empty frontier does nothing, as the reference stops there.
 constexpr unsigned kFrontBlocks = 2048;
 
 __device__ inline void grow_neighbours(const FilterMap& M, uint32_t k, int& x, int& y, int& xp, int& xm, int& yp, int& ym)
@@ -197,67 +196,58 @@ __global__ __launch_bounds__(256) void k_front_seed(const FilterMap* __restrict_
             const int w = (int)M.width;
             seed = s[xp + y * w].w != 0 || s[xm + y * w].w != 0 || s[x + yp * w].w != 0 || s[x + ym * w].w != 0 ||
                    s[xp + yp * w].w != 0 || s[xp + ym * w].w != 0 || s[xm + yp * w].w != 0 || s[xm + ym * w].w != 0;
-            if (seed) mark[t] = 1u;
+            if (seed) mark[t] = 1u;                 // step 0's frontier
         }
     }
     append(seed, t, front, cnt);
 }
 
-__global__ __launch_bounds__(256) void k_front_compute(const FilterMap* __restrict__ maps, uint32_t nmaps,
-                                                       const float4* __restrict__ g, const uint32_t* __restrict__ front,
-                                                       const uint32_t* __restrict__ cnt, float4* __restrict__ v)
+// One lightmapGrow step over its frontier in one launch.  Step q's frontier texels carry
+// mark q + 1 (the seeds 1), texels queued for step q + 1 get q + 2, and 0 means never
+// queued.  A frontier texel's value is computed from the previous state: its neighbours
+// in the same frontier count as empty (their previous state, zero) whether or not another
+// thread has written them yet, every other neighbour is unchanged during the step.  The
+// value is written at once; then the texel's still-empty, never-queued neighbours are
+// queued for the next step (the mark decides, so texels written in this step are never
+// queued).  Counts rotate over three words: this step reads cur, appends to next and
+// clears the one the step after will append to.
+__global__ __launch_bounds__(256) void k_front_step(const FilterMap* __restrict__ maps, uint32_t nmaps, float4* __restrict__ g,
+                                                    const uint32_t* __restrict__ front, const uint32_t* __restrict__ cnt,
+                                                    uint32_t* __restrict__ mark, uint32_t cur_mark, uint32_t* __restrict__ next,
+                                                    uint32_t* __restrict__ next_cnt, uint32_t* __restrict__ clear_cnt,
+                                                    uint32_t* __restrict__ log)
 {
     const uint32_t n = *cnt;
-    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-        const uint32_t t = front[i];
-        const FilterMap M = maps[find_map(maps, nmaps, t)];
-        int x, y, xp, xm, yp, ym;
-        grow_neighbours(M, t - M.first, x, y, xp, xm, yp, ym);
-        const float4* s = g + M.first;
-        const int w = (int)M.width;
-        float4 e = f4add(f4add(f4add(s[xp + y * w], s[xm + y * w]), s[x + yp * w]), s[x + ym * w]);
-        float4 dg = f4add(f4add(f4add(s[xp + yp * w], s[xp + ym * w]), s[xm + yp * w]), s[xm + ym * w]);
-        float4 d = f4add(f4mul(e, 1.6f), dg);
-        const float a = d.w;
-        v[i] = make_float4(d.x / a, d.y / a, d.z / a, d.w / a);
-    }
-}
-
-__global__ __launch_bounds__(256) void k_front_commit(const uint32_t* __restrict__ front, const uint32_t* __restrict__ cnt,
-                                                      const float4* __restrict__ v, float4* __restrict__ g,
-                                                      uint32_t* __restrict__ next_cnt, uint32_t* __restrict__ log)
-{
-    const uint32_t n = *cnt;
-    if (blockIdx.x == 0 && threadIdx.x == 0) { *next_cnt = 0u; *log = n; }
-    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) g[front[i]] = v[i];
-}
-
-__global__ __launch_bounds__(256) void k_front_expand(const FilterMap* __restrict__ maps, uint32_t nmaps,
-                                                      const float4* __restrict__ g, const uint32_t* __restrict__ front,
-                                                      const uint32_t* __restrict__ cnt, uint32_t* __restrict__ mark,
-                                                      uint32_t* __restrict__ next, uint32_t* __restrict__ next_cnt)
-{
-    const uint32_t n = *cnt;
+    if (blockIdx.x == 0 && threadIdx.x == 0) { *clear_cnt = 0u; *log = n; }
     const uint32_t stride = gridDim.x * 256;
     // whole waves walk the loop together so that append's ballot sees every lane
     const uint32_t rounds = (n + stride - 1) / stride;
     for (uint32_t r = 0; r < rounds; r++) {
         const uint32_t i = r * stride + blockIdx.x * 256 + threadIdx.x;
         const bool live = i < n;
-        uint32_t t = 0, first = 0;
+        uint32_t first = 0;
         int x = 0, y = 0, xp = 0, xm = 0, yp = 0, ym = 0, w = 1;
         if (live) {
-            t = front[i];
+            const uint32_t t = front[i];
             const FilterMap M = maps[find_map(maps, nmaps, t)];
             grow_neighbours(M, t - M.first, x, y, xp, xm, yp, ym);
             first = M.first; w = (int)M.width;
+            auto at = [&](int xx, int yy) {
+                const uint32_t q = first + (uint32_t)(xx + yy * w);
+                return mark[q] == cur_mark ? make_float4(0.f, 0.f, 0.f, 0.f) : g[q];
+            };
+            const float4 e = f4add(f4add(f4add(at(xp, y), at(xm, y)), at(x, yp)), at(x, ym));
+            const float4 dg = f4add(f4add(f4add(at(xp, yp), at(xp, ym)), at(xm, yp)), at(xm, ym));
+            const float4 d = f4add(f4mul(e, 1.6f), dg);
+            const float a = d.w;
+            g[t] = make_float4(d.x / a, d.y / a, d.z / a, d.w / a);
         }
         const int nx[8] = {xp, xm, x, x, xp, xp, xm, xm};
         const int ny[8] = {y, y, yp, ym, yp, ym, yp, ym};
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const uint32_t q = first + (uint32_t)(nx[j] + ny[j] * w);
-            const bool add = live && g[q].w == 0 && atomicCAS(&mark[q], 0u, 1u) == 0u;
+            const bool add = live && g[q].w == 0 && atomicCAS(&mark[q], 0u, cur_mark + 1u) == 0u;
             append(add, q, next, next_cnt);
         }
     }
@@ -325,42 +315,53 @@ int filter_run(const FilterMap* d_maps, uint32_t nmaps, uint32_t total, float4* 
     uint32_t steps = (uint32_t)(fp.spread_foreground > 0 ? fp.spread_foreground : 0);
     if (steps_used) *steps_used = 0;
     if (steps) {
-        // frontier sizes live in S.changed[1] and [2]; step q reads front[q & 1]
+        // frontier sizes rotate over S.changed[1..3] (step q reads [1 + q % 3]); step q reads
+        // front[q & 1] and queues into the other list
         k_grow_init<<<nblk(total), 256, 0, s>>>(total, S.c, S.a);
         k_zero_u32<<<1, 256, 0, s>>>(S.changed, 20);
         k_front_seed<<<nblk(total), 256, 0, s>>>(d_maps, nmaps, total, S.a, S.mark, S.front[0], S.changed + 1);
         // S.changed[4 + q % 16] = frontier size of step q (the host reads a chunk at a time)
         auto step = [&](uint32_t q) {
             const uint32_t c = q & 1u;
-            uint32_t* cur = S.changed + 1 + c;
-            uint32_t* nxt = S.changed + 2 - c;
-            k_front_compute<<<kFrontBlocks, 256, 0, s>>>(d_maps, nmaps, S.a, S.front[c], cur, S.b);
-            k_front_commit<<<kFrontBlocks, 256, 0, s>>>(S.front[c], cur, S.b, S.a, nxt, S.changed + 4 + (q & 15u));
-            k_front_expand<<<kFrontBlocks, 256, 0, s>>>(d_maps, nmaps, S.a, S.front[c], cur, S.mark, S.front[1 - c], nxt);
+            k_front_step<<<kFrontBlocks, 256, 0, s>>>(d_maps, nmaps, S.a, S.front[c], S.changed + 1 + q % 3u, S.mark, q + 1u,
+                                                      S.front[1 - c], S.changed + 1 + (q + 1u) % 3u,
+                                                      S.changed + 1 + (q + 2u) % 3u, S.changed + 4 + (q & 15u));
         };
         if (fixed_steps >= 0) {
             // graph-capturable form: a fixed number of steps, no host synchronisation,
             // S.changed[0] = whether a frontier is still left after the last one
             const uint32_t n = std::min<uint32_t>((uint32_t)fixed_steps, steps);
             for (uint32_t q = 0; q < n; q++) step(q);
-            k_front_changed<<<1, 64, 0, s>>>(S.changed + 1 + (n & 1u), S.changed);
+            k_front_changed<<<1, 64, 0, s>>>(S.changed + 1 + n % 3u, S.changed);
             if (steps_used) *steps_used = n;
         } else {
             // a step with an empty frontier changes nothing: the reference stops there
-            // (RRBuffer.cpp:948-949); checked every 16 steps
-            uint32_t done = 0;
-            while (done < steps) {
-                const uint32_t chunk = std::min<uint32_t>(16, steps - done);
-                for (uint32_t q = 0; q < chunk; q++) step(done + q);
-                uint32_t sizes[16];
-                LSG_HIP(hipMemcpyAsync(sizes, S.changed + 4, 4 * 16, hipMemcpyDeviceToHost, s));
-                LSG_HIP(hipStreamSynchronize(s));
+            // (RRBuffer.cpp:948-949).  Steps go out in chunks of 16, each followed by a copy
+            // of its frontier sizes into pinned host memory and an event; the host checks
+            // chunk k while chunk k + 1 already runs, so the GPU does not drain between
+            // chunks (steps after an empty one do nothing)
+            constexpr uint32_t kChunk = 16;
+            const uint32_t nchunks = (steps + kChunk - 1) / kChunk;
+            auto launch = [&](uint32_t k) -> hipError_t {
+                const uint32_t b = k * kChunk, e = std::min(steps, b + kChunk);
+                for (uint32_t q = b; q < e; q++) step(q);
+                hipError_t r = hipMemcpyAsync(S.host_log + (k & 1u) * kChunk, S.changed + 4, 4 * kChunk, hipMemcpyDeviceToHost, s);
+                if (r == hipSuccess) r = hipEventRecord(S.ev[k & 1u], s);
+                return r;
+            };
+            uint32_t done = steps, k = 0;
+            LSG_HIP(launch(0));
+            for (; k < nchunks; k++) {
+                if (k + 1 < nchunks) LSG_HIP(launch(k + 1));
+                LSG_HIP(hipEventSynchronize(S.ev[k & 1u]));
+                const uint32_t b = k * kChunk, e = std::min(steps, b + kChunk);
                 bool stop = false;
-                for (uint32_t q = 0; q < chunk; q++)
-                    if (!sizes[(done + q) & 15u]) { done += q + 1; stop = true; break; }
+                for (uint32_t q = b; q < e; q++)
+                    if (!S.host_log[(k & 1u) * kChunk + (q & 15u)]) { done = q + 1; stop = true; break; }
                 if (stop) break;
-                done += chunk;
             }
+            // the chunk launched ahead must finish its copy before the buffers are reused
+            if (k + 1 < nchunks) LSG_HIP(hipEventSynchronize(S.ev[(k + 1) & 1u]));
             if (steps_used) *steps_used = done;
         }
         g = S.a;
@@ -383,6 +384,9 @@ int scratch_alloc(FilterScratch& S, uint32_t total)
     if ((e = hipMalloc(&S.front[0], 4 * n)) != hipSuccess) return hip_fail(e, "hipMalloc", __LINE__);
     if ((e = hipMalloc(&S.front[1], 4 * n)) != hipSuccess) return hip_fail(e, "hipMalloc", __LINE__);
     if ((e = hipMalloc(&S.mark, 4 * n)) != hipSuccess) return hip_fail(e, "hipMalloc", __LINE__);
+    if ((e = hipHostMalloc((void**)&S.host_log, 4 * 32, hipHostMallocDefault)) != hipSuccess) return hip_fail(e, "hipHostMalloc", __LINE__);
+    for (int i = 0; i < 2; i++)
+        if ((e = hipEventCreateWithFlags(&S.ev[i], hipEventDisableTiming)) != hipSuccess) return hip_fail(e, "hipEventCreate", __LINE__);
     return LSG_OK;
 }
 
@@ -395,6 +399,8 @@ void scratch_free(FilterScratch& S)
     if (S.changed) (void)hipFree(S.changed);
     for (int i = 0; i < 2; i++) if (S.front[i]) (void)hipFree(S.front[i]);
     if (S.mark) (void)hipFree(S.mark);
+    if (S.host_log) (void)hipHostFree(S.host_log);
+    for (int i = 0; i < 2; i++) if (S.ev[i]) (void)hipEventDestroy(S.ev[i]);
     S = FilterScratch();
 }
 
