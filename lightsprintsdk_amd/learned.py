@@ -167,7 +167,9 @@ class LearnedScene:
         self.d_work = DeviceBuffer(48 * max(self.sb.nout, 1))
         self.d_cheap2 = DeviceBuffer(16 * max(self.sb.nout, 1))
         self.d_hist = DeviceBuffer(8 * 513)
-        self.d_tex_sample = None
+        # the bound's texel sample and the filter plan are set up here, not on first use
+        self.d_tex_sample = DeviceBuffer.from_array(np.ascontiguousarray(self.sb.texels[::BOUND_SAMPLE]))
+        self.sb._filter_plan()
         self.d_y = DeviceBuffer(16 * max(n, 1))
         self.d_fb = DeviceBuffer(16 * max(n, 1))
         self.d_nfb = DeviceBuffer(16)
@@ -240,8 +242,6 @@ class LearnedScene:
         L = N.lib()
         p = self.cheap.c()
         p.seed = (self.cheap.seed * 2654435761 + 0x3C6EF372) & 0xFFFFFFFF
-        if self.d_tex_sample is None:
-            self.d_tex_sample = DeviceBuffer.from_array(np.ascontiguousarray(self.sb.texels[::BOUND_SAMPLE]))
         ns = (self.n + BOUND_SAMPLE - 1) // BOUND_SAMPLE
         N.check(L.lsg_gather_device(self.sb.gs._h, self.d_tex_sample.ptr, ns, self.sb.d_sub.ptr, len(self.sb.subs),
                                     ctypes.byref(p), self.d_cheap2.ptr, stream), "second cheap gather")
