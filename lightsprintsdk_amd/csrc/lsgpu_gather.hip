@@ -369,7 +369,7 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
                     if (L.cast_shadows) {
                         TraceResult r;
                         uint32_t nv_ = 0, nt_ = 0;
-                        trace_closest<STACK, STATS>(flt, A.sc.col, o, make_float3(dx, dy, dz), A.tmin, dsz, mystk, stride, r, nv_, nt_);
+                        trace_closest<STACK, STATS, GatherFilter, false>(flt, A.sc.col, o, make_float3(dx, dy, dz), A.tmin, dsz, mystk, stride, r, nv_, nt_);
                         if (STATS) { stS[0]++; stS[1] += nv_; stS[2] += nt_; }
                         blocked = r.tri >= 0; bt = r.t;
                     }
@@ -658,7 +658,7 @@ __device__ void gather_texel_lane(const GatherArgs& A, uint32_t ti, int32_t* stk
                     if (L.cast_shadows) {
                         TraceResult r;
                         uint32_t nv_ = 0, nt_ = 0;
-                        trace_closest<STACK, false>(flt, A.sc.col, o, make_float3(dx, dy, dz), A.tmin, dsz, stk, stride, r, nv_, nt_);
+                        trace_closest<STACK, false, GatherFilter, false>(flt, A.sc.col, o, make_float3(dx, dy, dz), A.tmin, dsz, stk, stride, r, nv_, nt_);
                         blocked = r.tri >= 0; bt = r.t;
                     }
                     if (!blocked) {

@@ -55,7 +55,7 @@ __device__ inline void cswap(float& ka, int32_t& ra, float& kb, int32_t& rb)
     const int32_t r = sw ? rb : ra; rb = sw ? ra : rb; ra = r;
 }
 
-template <int STACK, class Pop>
+template <int STACK, class Pop, bool ORDERED = true>
 __device__ inline int32_t visit4(const Node4* __restrict__ nodes, int32_t ref, const float3 noi, const float3 idir, float tlo,
                                  float thi, int32_t* stk, int stride, int& sp, bool& ok, Pop pop)
 {
@@ -98,8 +98,16 @@ __device__ inline int32_t visit4(const Node4* __restrict__ nodes, int32_t ref, c
     k1 = (r1 != kNoChild && k1 <= fminf(tf.y, thi)) ? k1 : kInf;
     k2 = (r2 != kNoChild && k2 <= fminf(tf.z, thi)) ? k2 : kInf;
     k3 = (r3 != kNoChild && k3 <= fminf(tf.w, thi)) ? k3 : kInf;
-    cswap(k0, r0, k1, r1); cswap(k2, r2, k3, r3); cswap(k0, r0, k2, r2); cswap(k1, r1, k3, r3); cswap(k1, r1, k2, r2);
     auto push = [&](int32_t r) { if (sp < STACK) stk[(sp++) * stride] = r; else ok = false; };
+    if constexpr (!ORDERED) {
+        // any order: continue with the last hit child in slot order, push the others
+        int32_t nxt = 0;
+        bool have = false;
+        auto take = [&](float k, int32_t r) { if (k < kInf) { if (have) push(nxt); nxt = r; have = true; } };
+        take(k0, r0); take(k1, r1); take(k2, r2); take(k3, r3);
+        return have ? nxt : pop();
+    }
+    cswap(k0, r0, k1, r1); cswap(k2, r2, k3, r3); cswap(k0, r0, k2, r2); cswap(k1, r1, k3, r3); cswap(k1, r1, k2, r2);
     if (k3 < kInf) push(r3);
     if (k2 < kInf) push(r2);
     if (k1 < kInf) push(r1);
@@ -124,7 +132,11 @@ __device__ inline float3 safe_inv(float3 d)
 // triangles together.  Inner-node and triangle work thus run with full lane masks
 // instead of interleaving per iteration.  The result does not depend on the visiting
 // order (see the header), so it is the same as any other order's.
-template <int STACK, bool STATS, class Filter>
+// ORDERED = false (shadow rays): children are visited in any order.  A shadow ray only
+// asks whether a blocker lies below rugDistance and, if not, whether any blocker lies in
+// range; after the first hit its range shrinks below rugDistance and the search there is
+// exhaustive, so the answer does not depend on the order, and the sort is saved.
+template <int STACK, bool STATS, class Filter, bool ORDERED = true>
 __device__ inline bool trace_closest(const Filter& flt, const ColliderView& c, const float3 o, const float3 d,
                                      float rmin, float rmax, int32_t* stk, int stride, TraceResult& res,
                                      uint32_t& nvis, uint32_t& ntest)
@@ -146,7 +158,7 @@ __device__ inline bool trace_closest(const Filter& flt, const ColliderView& c, c
         // inner nodes, parking the first leaf met
         while (ref >= 0 && ref != kDone) {
             if (STATS) nvis++;
-            ref = visit4<STACK>(c.nodes4, ref, noi, idir, lo, bound, stk, stride, sp, ok, pop);
+            ref = visit4<STACK, decltype(pop), ORDERED>(c.nodes4, ref, noi, idir, lo, bound, stk, stride, sp, ok, pop);
             if (ref < 0 && leaf == 0) { leaf = ref; ref = pop(); }
             if (!__any(leaf == 0)) break;                // every active lane holds a leaf
         }
