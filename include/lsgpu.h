@@ -352,6 +352,8 @@ typedef struct lsg_bake_tuning {
     uint32_t waves_per_block;  /* 2, 4 or 8 (default 4) */
     uint32_t blocks_per_cu;    /* persistent workgroups per compute unit (default 4) */
     uint32_t xcd_queues;       /* 8 = one work queue per XCD, 1 = one global queue (default 8) */
+    uint32_t lane_refill;      /* above 32 shooters per series: 1 = a texel's lanes take 64 shooters at a time,
+                                  2 = each lane takes the next shooter as soon as 16 lanes are free (default 2) */
 } lsg_bake_tuning;
 
 typedef struct lsg_bake_info {

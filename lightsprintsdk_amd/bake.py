@@ -425,8 +425,8 @@ class BakePlan:
         N.check(N.lib().lsg_bake_info_get(self._h, ctypes.byref(inf)), "bake info")
         return inf
 
-    def set_tuning(self, tile=0, waves_per_block=0, blocks_per_cu=0, xcd_queues=0):
-        t = N.BakeTuning(tile, waves_per_block, blocks_per_cu, xcd_queues)
+    def set_tuning(self, tile=0, waves_per_block=0, blocks_per_cu=0, xcd_queues=0, lane_refill=0):
+        t = N.BakeTuning(tile, waves_per_block, blocks_per_cu, xcd_queues, lane_refill)
         N.check(N.lib().lsg_bake_set_tuning(self._h, ctypes.byref(t)), "set_tuning")
 
     def run(self, first=True, graph=False, filtering=True, stream=None):
@@ -531,6 +531,16 @@ def autotune(plan: BakePlan, sample_texels: int = 60000, candidates=None, cache_
         results.append((ms, c))
     results.sort(key=lambda r: r[0])
     best_ms, best = results[0]
+    # then the lane schedule on the best launch shape: 64 shooters at a time (1) or
+    # lanes taking shooters as they free up (2, the default the shapes were timed with)
+    if candidates is None:
+        alt = dict(best, lane_refill=1)
+        plan.set_tuning(**alt)
+        ms = min(plan.sample_ms(stride) for _ in range(2))
+        results.append((ms, alt))
+        if ms < best_ms:
+            best_ms, best = ms, alt
+        results.sort(key=lambda r: r[0])
     plan.set_tuning(**best)
     entry = dict(tuning=best, sample_ms=round(best_ms, 4), stride=stride,
                  worst_ms=round(results[-1][0], 4), tried=len(results))

@@ -111,6 +111,7 @@ static void default_tuning(lsg_bake_tuning& t)
     if (!t.waves_per_block) t.waves_per_block = 4;
     if (!t.blocks_per_cu) t.blocks_per_cu = 4;
     if (!t.xcd_queues) t.xcd_queues = 8;
+    if (!t.lane_refill) t.lane_refill = 2;
 }
 
 static int run_gather_final(BakeImpl& B, const lsg_gather_params* p, const lsg_texel* tex, uint32_t ntex, hipStream_t st)
@@ -126,7 +127,7 @@ static int run_gather_final(BakeImpl& B, const lsg_gather_params* p, const lsg_t
     Q.ntiles = (ntex + t.tile - 1) / t.tile;
     if (!Q.ntiles) return LSG_OK;
     const uint32_t blocks = (uint32_t)B.cus * t.blocks_per_cu;
-    return launch_gather_persistent(A, stack, Q, t.waves_per_block, blocks, st);
+    return launch_gather_persistent(A, stack, Q, t.waves_per_block, blocks, t.lane_refill == 2, st);
 }
 
 static int run_first(BakeImpl& B, const lsg_gather_params* first, hipStream_t st)
@@ -283,7 +284,7 @@ int lsg_bake_set_tuning(lsg_bake* b, const lsg_bake_tuning* t)
     if (!b || !t) return fail(LSG_ERR_INVALID, "lsg_bake_set_tuning: null argument");
     if (t->waves_per_block && t->waves_per_block != 2 && t->waves_per_block != 4 && t->waves_per_block != 8)
         return fail(LSG_ERR_INVALID, "waves_per_block must be 2, 4 or 8");
-    if (t->tile > 1024 || t->blocks_per_cu > 64) return fail(LSG_ERR_INVALID, "tuning out of range");
+    if (t->tile > 1024 || t->blocks_per_cu > 64 || t->lane_refill > 2) return fail(LSG_ERR_INVALID, "tuning out of range");
     b->impl.tuning = *t;
     default_tuning(b->impl.tuning);
     b->impl.graph_key.clear();   // invalidate a captured graph

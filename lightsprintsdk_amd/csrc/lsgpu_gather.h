@@ -66,6 +66,7 @@ int gather_args(lsg_scene* s, const lsg_texel* texels, uint32_t ntexels, const l
                 const lsg_gather_params* p, float* out_rgba, GatherArgs& A, int& stack);
 int launch_gather(const GatherArgs& A, int stack, hipStream_t st);
 int launch_gather_persistent(const GatherArgs& A, int stack, const WorkQueue& Q, uint32_t waves_per_block, uint32_t blocks,
+                             bool refill,
                              hipStream_t st);
 
 } // namespace lsg

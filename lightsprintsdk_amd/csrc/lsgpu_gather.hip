@@ -574,6 +574,220 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
     }
 }
 
+// processTexel by one wave with refill (at most 64 lanes per series chunk replaced by a
+// pool): lanes take the series' shooters in order as they become free, trace their
+// hemisphere rays with the resumable traversal, and the wave hands out the next shooters
+// whenever kGatherRefill lanes are free, instead of waiting for the slowest ray of every
+// 64.  Shooter s still gets the filler point that sequential order gives it (free lanes
+// take consecutive shooters in lane order, and the hand-out ranks them in that order).
+// Light shooters are queued as before; their shadow rays run only while no hemisphere
+// ray is in flight (the lanes' LDS stacks are free then): when the queue holds a full
+// batch, refills pause until every lane is done, and the rest at the series' end.
+constexpr uint32_t kGatherRefill = 16;
+
+template <int STACK, bool ENV>
+__device__ void gather_texel_refill(const GatherArgs& A, uint32_t ti, uint32_t lane, int32_t* mystk, uint32_t* lq, int stride)
+{
+    const lsg_texel T = A.texels[ti];
+    const uint32_t m = T.sub_count_flags & 0xffffu, flags = (T.sub_count_flags >> 16) & 0xffu;
+    const uint32_t hemi_rays = A.hemi_rays, rounds = A.rounds, nl = A.nlights;
+    const uint32_t light_rays = nl * rounds;
+    if (!m || (!hemi_rays && !light_rays)) {
+        if (lane == 0) {
+            A.out[T.out_index] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (A.parts) { A.parts[2 * (size_t)ti] = make_float4(0.f, 0.f, 0.f, 0.f); A.parts[2 * (size_t)ti + 1] = make_float4(0.f, 0.f, 0.f, 0.f); }
+        }
+        return;
+    }
+    const lsg_subtexel* ts = A.subs + T.sub_begin;
+    const uint32_t key = lowbias32(A.seed ^ lowbias32(T.key));
+    uint32_t filler = key & 0xffffu;           // wave-uniform
+    float area_max = 0.f;
+    for (uint32_t j = 0; j < m; j++) area_max += ts[j].area;
+    const float near_t = 1000 * A.tmin;
+    const float rug_below = __int_as_float(__float_as_int(A.rug) - 1);   // largest float < rugDistance (> 0)
+    float hr = 0.f, hg = 0.f, hb = 0.f, lr = 0.f, lg = 0.f, lb = 0.f;
+    uint32_t relH = 0, relL = 0, unrelL = 0, shotRounds = 0, shooter_base = 0;
+    for (;;) {
+        const bool shootH = (relH < hemi_rays || relH * 10 < hemi_rays) && hemi_rays > 0;
+        const bool shootL = (relL + unrelL < light_rays || relL * 10 < light_rays) && unrelL < light_rays * 100;
+        if (!shootH && !shootL) break;
+        const uint32_t nH = shootH ? hemi_rays : 0, nL = shootL ? rounds : 0;
+        const uint32_t n = nH > nL ? nH : nL;
+        const bool allH = !shootL || hemi_rays >= rounds;
+        const bool allL = !shootH || rounds >= hemi_rays;
+        uint32_t cRelH = 0, cRelL = 0, cUnrelL = 0, cRounds = 0;
+        uint32_t* const lq0 = lq - lane;
+        uint32_t qhead = 0, qtail = 0;             // wave-uniform
+        auto light_batch = [&](uint32_t cnt) {
+            __builtin_amdgcn_wave_barrier();
+            if (lane < cnt) {
+                const uint32_t e = qhead + lane;
+                const uint32_t s = lq0[(e & 63u) + ((e / 64u) & 1u) * stride];
+                float3 o; float nx, ny, nz; uint32_t tri;
+                shooter_at(A, ts, m, area_max, n, s, key, shooter_base + s, o, nx, ny, nz, tri);
+                GatherFilter flt{A.sc.verts, A.sc.idx, A.sc.side, tri, near_t, 1, rug_below};
+                for (uint32_t li = 0; li < nl; li++) {
+                    const lsg_light L = A.lights[li];
+                    float dx, dy, dz;
+                    if (L.type == 1) { dx = -L.direction[0]; dy = -L.direction[1]; dz = -L.direction[2]; }
+                    else { dx = L.position[0] - o.x; dy = L.position[1] - o.y; dz = L.position[2] - o.z; }
+                    float dsz = sqrtf(dx * dx + dy * dy + dz * dz);
+                    dx = dx / dsz; dy = dy / dsz; dz = dz / dsz;
+                    if (L.type == 1) dsz = dsz * A.locality;
+                    const float ni = dx * nx + dy * ny + dz * nz;            // gather.cpp:429
+                    if (ni <= 0 || !isfinite(ni)) { cRelL++; continue; }
+                    bool blocked = false; float bt = 0.f;
+                    if (L.cast_shadows) {
+                        TraceResult r;
+                        uint32_t nv_ = 0, nt_ = 0;
+                        trace_closest<STACK, false, GatherFilter, false>(flt, A.sc.col, o, make_float3(dx, dy, dz), A.tmin, dsz, mystk, stride, r, nv_, nt_);
+                        blocked = r.tri >= 0; bt = r.t;
+                    }
+                    if (!blocked) {
+                        // RRLight::getIrradiance (RRLight.cpp:345-362), REALISTIC attenuation
+                        float att = 1.f;
+                        if (L.type == 0) {
+                            const float ex = o.x - L.position[0], ey = o.y - L.position[1], ez = o.z - L.position[2];
+                            att = 1 / (0.000000000137289f + (ex * ex + ey * ey + ez * ez));
+                        }
+                        const float kl = att * A.light_mult * ni;
+                        lr += L.color[0] * kl; lg += L.color[1] * kl; lb += L.color[2] * kl;
+                    }
+                    const uint32_t unrel = (blocked && bt < A.rug) ? 1u : 0u;    // gather.cpp:483-511
+                    cUnrelL += unrel;
+                    cRelL += 1u - unrel;
+                }
+                cRounds++;
+            }
+            __builtin_amdgcn_wave_barrier();
+        };
+        uint32_t next = 0;                         // next shooter to hand out (wave-uniform)
+        bool busy = false, ok = true;
+        uint32_t stri = 0;                         // the lane's shooter triangle
+        TraceState S;
+        for (;;) {
+            const uint64_t bm = __ballot(busy);
+            const bool qfull = qtail - qhead >= 64u;
+            if (next >= n && !bm) break;
+            if (next < n && !qfull && (64u - (uint32_t)__popcll(bm) >= kGatherRefill || !bm)) {
+                // free lanes take the next shooters, in lane order
+                const uint64_t fm = ~bm;
+                const uint32_t rank = (uint32_t)__popcll(fm & lanemask_lt(lane));
+                const uint32_t take = min((uint32_t)__popcll(fm), n - next);
+                const bool mine = !busy && rank < take;
+                const uint32_t s = next + rank;
+                bool doH = false, doL = false;
+                if (mine && shootH) doH = allH || shots_before(s + 1, n, hemi_rays) > shots_before(s, n, hemi_rays);
+                if (mine && shootL) doL = allL || shots_before(s + 1, n, rounds) > shots_before(s, n, rounds);
+                // filler points in shooter order (GetCirclePoint, gather.cpp:85-90)
+                const uint64_t hmask = __ballot(doH);
+                const uint32_t need = (uint32_t)__popcll(hmask);
+                const uint32_t myrank = (uint32_t)__popcll(hmask & lanemask_lt(lane));
+                uint32_t my_index = 0, got = 0;
+                while (got < need) {
+                    const uint32_t cand = filler + lane;
+                    float cx, cy;
+                    filler_point(cand, cx, cy);
+                    const bool acc = !(cx * cx + cy * cy >= 1.f);
+                    const uint64_t am = __ballot(acc);
+                    const uint32_t na = (uint32_t)__popcll(am);
+                    const uint32_t want = doH ? myrank - got : 64u;
+                    const uint32_t src = (doH && myrank >= got && want < na) ? nth_set_bit(am, want) : lane;
+                    const uint32_t val = (uint32_t)__shfl((int)cand, (int)src);
+                    if (doH && myrank >= got && want < na) my_index = val;
+                    if (got + na <= need) { got += na; filler += 64u; }
+                    else {
+                        const uint32_t last = nth_set_bit(am, need - got - 1u);
+                        filler = (uint32_t)__shfl((int)cand, (int)last) + 1u;
+                        got = need;
+                    }
+                }
+                {
+                    const uint64_t lm = __ballot(doL);
+                    if (doL) {
+                        const uint32_t e = qtail + (uint32_t)__popcll(lm & lanemask_lt(lane));
+                        lq0[(e & 63u) + ((e / 64u) & 1u) * stride] = s;
+                    }
+                    qtail += (uint32_t)__popcll(lm);
+                }
+                next += take;
+                if (doH) {
+                    float3 o; float nx, ny, nz; uint32_t tri;
+                    shooter_at(A, ts, m, area_max, n, s, key, shooter_base + s, o, nx, ny, nz, tri);
+                    float ox, oy, oz;                                               // RRMathPrivate.h:48-53
+                    if (fabsf(nx) > fabsf(nz)) { ox = -ny; oy = nx; oz = 0.f; } else { ox = 0.f; oy = -nz; oz = ny; }
+                    const float olen = sqrtf(ox * ox + oy * oy + oz * oz);
+                    const float tx = ox / olen, ty = oy / olen, tz = oz / olen;
+                    const float bx = ny * tz - nz * ty, by = -nx * tz + nz * tx, bz = nx * ty - ny * tx;
+                    float x, y;
+                    filler_point(my_index, x, y);
+                    const float dist = x * x + y * y;
+                    const float cosa = sqrtf(1 - dist);                          // gather.cpp:125-126
+                    const float3 d = make_float3(nx * cosa + tx * x + bx * y, ny * cosa + ty * x + by * y, nz * cosa + tz * x + bz * y);
+                    stri = tri;
+                    if (trace_begin(S, A.sc.col, o, d, A.tmin, A.locality)) busy = true;
+                    else {
+                        // the ray leaves the scene box: environment or nothing (pathtracer.cpp:156-173)
+                        if constexpr (ENV) {
+                            const float4 v = env_at(A.sc, d);
+                            hr += v.x * A.env_mult; hg += v.y * A.env_mult; hb += v.z * A.env_mult;
+                        }
+                        cRelH++;
+                    }
+                }
+            }
+            if (busy) {
+                // trace until the ray ends or enough lanes are free to hand out more shooters
+                const bool refill = next < n && qtail - qhead < 64u;
+                const uint32_t yield_at = refill ? 64u - kGatherRefill : 0u;
+                GatherFilter flt{A.sc.verts, A.sc.idx, A.sc.side, stri, near_t, 0, rug_below};
+                if (trace_resume<STACK>(S, flt, A.sc.col, mystk, stride, yield_at, ok)) {
+                    busy = false;
+                    // a hit reads the exitance of the side it sees; a ray that leaves the
+                    // scene returns environment * multiplier, or nothing
+                    if (S.bi >= 0) {
+                        const float4 e = A.sc.exitance[2 * (size_t)S.bi + (S.bf ? 0 : 1)];
+                        hr += e.x; hg += e.y; hb += e.z;
+                    } else if constexpr (ENV) {
+                        const float4 v = env_at(A.sc, S.d);
+                        hr += v.x * A.env_mult; hg += v.y * A.env_mult; hb += v.z * A.env_mult;
+                    }
+                    cRelH++;
+                }
+            }
+            // a full light batch runs once no hemisphere ray is in flight
+            if (!__any(busy) && qtail - qhead >= 64u) { light_batch(64u); qhead += 64u; }
+        }
+        while (qtail != qhead) {
+            const uint32_t cnt = min(64u, qtail - qhead);
+            light_batch(cnt);
+            qhead += cnt;
+        }
+        relH += group_sum_u<64>(cRelH); relL += group_sum_u<64>(cRelL); unrelL += group_sum_u<64>(cUnrelL);
+        shotRounds += group_sum_u<64>(cRounds);
+        shooter_base += n;
+    }
+    hr = group_sum<64>(hr); hg = group_sum<64>(hg); hb = group_sum<64>(hb);
+    lr = group_sum<64>(lr); lg = group_sum<64>(lg); lb = group_sum<64>(lb);
+    if (lane == 0) {
+        float r = 0.f, g = 0.f, b = 0.f;
+        float4 ph = make_float4(0.f, 0.f, 0.f, 0.f), pl = make_float4(0.f, 0.f, 0.f, 0.f);
+        bool reliable = false;
+        if (hemi_rays && relH) {
+            const float f = 1 / (float)relH;
+            ph = make_float4(hr * f, hg * f, hb * f, 1.f);
+            r += ph.x; g += ph.y; b += ph.z; reliable = true;
+        }
+        if (light_rays && relL) {
+            pl = make_float4(lr / (float)shotRounds, lg / (float)shotRounds, lb / (float)shotRounds, 1.f);
+            r += pl.x; g += pl.y; b += pl.z; reliable = true;
+        }
+        A.out[T.out_index] = make_float4(r, g, b, reliable ? flags / 255.f : 0.f);
+        if (A.parts) { A.parts[2 * (size_t)ti] = ph; A.parts[2 * (size_t)ti + 1] = pl; }
+    }
+}
+
 // processTexel for texel ti by one lane: the low ray counts of the cheap gather (at most
 // 32 shooters per series).  Texels of one quality walk the same shooter schedule, so the
 // lanes of a wave trace their hemisphere rays and their few shadow rays in step, with no
@@ -766,9 +980,9 @@ __global__ void k_queue_reset(uint32_t* counters, uint32_t n)
     if (threadIdx.x < n) counters[threadIdx.x] = 0;
 }
 
-// At 141 VGPRs this kernel would run 3 waves per SIMD; capped at 128 (12-16 registers
-// spilled to scratch) it runs 4, which measured faster for the batched bake.
-template <int STACK, int WAVES, bool ENV = false>
+// Capped at 128 VGPRs it runs 4 waves per SIMD, measured faster than 3 with more
+// registers; the chunked form fits, the refill form (REFILL) spills 56 bytes.
+template <int STACK, int WAVES, bool ENV = false, bool REFILL = false>
 __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4))) void k_gather_persistent(GatherArgs A, WorkQueue Q)
 {
     __shared__ int32_t stk[STACK * 64 * WAVES];
@@ -787,7 +1001,8 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4)))
             if (tile >= t1) break;           // this queue is exhausted for every lane of the wave
             const uint32_t a = tile * Q.tile, b = min(A.ntex, a + Q.tile);
             for (uint32_t ti = a; ti < b; ti++)
-                gather_texel<STACK, false, 64, false, ENV>(A, ti, lane, stk + tid, lq + tid, 64 * WAVES);
+                if constexpr (REFILL) gather_texel_refill<STACK, ENV>(A, ti, lane, stk + tid, lq + tid, 64 * WAVES);
+                else gather_texel<STACK, false, 64, false, ENV>(A, ti, lane, stk + tid, lq + tid, 64 * WAVES);
         }
     }
 }
@@ -861,28 +1076,35 @@ int gather_args(lsg_scene* s, const lsg_texel* texels, uint32_t ntexels, const l
     return LSG_OK;
 }
 
-template <bool ENV>
+template <bool ENV, bool REFILL>
 static void launch_persistent(const GatherArgs& A, int stack, const WorkQueue& Q, uint32_t waves_per_block, uint32_t blocks,
                               hipStream_t st)
 {
     if (waves_per_block == 8) {
-        if (stack <= 32) k_gather_persistent<32, 8, ENV><<<blocks, 512, 0, st>>>(A, Q);
-        else k_gather_persistent<64, 8, ENV><<<blocks, 512, 0, st>>>(A, Q);
+        if (stack <= 32) k_gather_persistent<32, 8, ENV, REFILL><<<blocks, 512, 0, st>>>(A, Q);
+        else k_gather_persistent<64, 8, ENV, REFILL><<<blocks, 512, 0, st>>>(A, Q);
     } else if (waves_per_block == 2) {
-        if (stack <= 32) k_gather_persistent<32, 2, ENV><<<blocks, 128, 0, st>>>(A, Q);
-        else k_gather_persistent<64, 2, ENV><<<blocks, 128, 0, st>>>(A, Q);
+        if (stack <= 32) k_gather_persistent<32, 2, ENV, REFILL><<<blocks, 128, 0, st>>>(A, Q);
+        else k_gather_persistent<64, 2, ENV, REFILL><<<blocks, 128, 0, st>>>(A, Q);
     } else {
-        if (stack <= 32) k_gather_persistent<32, 4, ENV><<<blocks, 256, 0, st>>>(A, Q);
-        else k_gather_persistent<64, 4, ENV><<<blocks, 256, 0, st>>>(A, Q);
+        if (stack <= 32) k_gather_persistent<32, 4, ENV, REFILL><<<blocks, 256, 0, st>>>(A, Q);
+        else k_gather_persistent<64, 4, ENV, REFILL><<<blocks, 256, 0, st>>>(A, Q);
     }
 }
 
 int launch_gather_persistent(const GatherArgs& A, int stack, const WorkQueue& Q, uint32_t waves_per_block, uint32_t blocks,
-                             hipStream_t st)
+                             bool refill, hipStream_t st)
 {
     k_queue_reset<<<1, 64, 0, st>>>(Q.counters, Q.nqueues);
-    if (A.sc.env_type) launch_persistent<true>(A, stack, Q, waves_per_block, blocks, st);
-    else launch_persistent<false>(A, stack, Q, waves_per_block, blocks, st);
+    // shooters per series above 32: the texel's lanes may take shooters as they free up
+    refill = refill && std::max(A.hemi_rays, A.rounds) > 32u;
+    if (A.sc.env_type) {
+        if (refill) launch_persistent<true, true>(A, stack, Q, waves_per_block, blocks, st);
+        else launch_persistent<true, false>(A, stack, Q, waves_per_block, blocks, st);
+    } else {
+        if (refill) launch_persistent<false, true>(A, stack, Q, waves_per_block, blocks, st);
+        else launch_persistent<false, false>(A, stack, Q, waves_per_block, blocks, st);
+    }
     LSG_HIP(hipGetLastError());
     return LSG_OK;
 }

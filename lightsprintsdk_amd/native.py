@@ -58,7 +58,7 @@ class BakeObject(ctypes.Structure):
 
 
 class BakeTuning(ctypes.Structure):
-    _fields_ = [("tile", U32), ("waves_per_block", U32), ("blocks_per_cu", U32), ("xcd_queues", U32)]
+    _fields_ = [("tile", U32), ("waves_per_block", U32), ("blocks_per_cu", U32), ("xcd_queues", U32), ("lane_refill", U32)]
 
 
 class BakeInfo(ctypes.Structure):

@@ -97,6 +97,7 @@ def test_gpu_environment_plan_matches_scene_bake(gpu):
     ref.bake()
     want = ref.d_out.download(np.float32, (max(ref.nout, 1), 4))
     plan = B.BakePlan(sc, params)
+    plan.set_tuning(lane_refill=1)       # 64 shooters at a time, as the plain gather
     plan.gs.set_environment(e["kind"], e["width"], e["height"], e["rgb"])
     plan.run(first=True, graph=False)
     np.testing.assert_array_equal(plan.rgba().view(np.uint32), want.view(np.uint32))
