@@ -443,7 +443,8 @@ int  lsg_learned_scale(const float* parts, uint32_t n, float* scale);
  * 3x3 / 5x5 / 11x11 windows of same-facing covered texels in the texel's lightmap,
  * position, normal, albedo, spreads, window coverage, the lights' unshadowed direct term.  tri_albedo: device float[ntri*3];
  * maps: host array of the lightmaps the texels' out_index slots fall in, sorted by
- * rgba_base, inside [0, nslots); work: device float4[3*nslots] scratch (synchronous) */
+ * rgba_base, inside [0, nslots); work: device float4[3*nslots] scratch.  Asynchronous on
+ * stream; the lightmap list is copied to the device (and kept there) when it changes */
 int  lsg_learned_features(lsg_scene* scene, const lsg_texel* texels, uint32_t n, const lsg_subtexel* subs,
                           const float* parts, const float* tri_albedo, float scale, const lsg_lightmap* maps,
                           uint32_t nmaps, uint32_t nslots, float* work, float* x, void* stream);

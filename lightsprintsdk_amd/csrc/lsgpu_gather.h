@@ -1,6 +1,7 @@
 // lsgpu_gather.h -- internal interface of the per-texel gather (lsgpu_gather.hip).
 #pragma once
 #include "lsgpu_internal.h"
+#include <vector>
 
 namespace lsg {
 
@@ -56,6 +57,9 @@ struct SceneImpl {
     float* d_vbit = nullptr;
     float4* d_env = nullptr;           // environment texels, linear rgb
     uint32_t env_type = 0, env_w = 0, env_h = 0;
+    lsg_lightmap* d_lmaps = nullptr;   // lightmap list of the last learned-feature pass
+    uint32_t lmaps_cap = 0;
+    std::vector<lsg_lightmap> h_lmaps; // host copy it was uploaded from
 };
 
 // per-vertex tangent frames of the scene: the caller's, or built from the vertex

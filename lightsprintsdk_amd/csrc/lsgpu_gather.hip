@@ -1268,7 +1268,7 @@ void lsg_scene_destroy(lsg_scene* s)
 {
     if (!s) return;
     dfree(s->impl.d_vnormal); dfree(s->impl.d_side); dfree(s->impl.d_exit); dfree(s->impl.d_lights);
-    dfree(s->impl.d_vtan); dfree(s->impl.d_vbit); dfree(s->impl.d_env);
+    dfree(s->impl.d_vtan); dfree(s->impl.d_vbit); dfree(s->impl.d_env); dfree(s->impl.d_lmaps);
     delete s;
 }
 

@@ -22,6 +22,7 @@
 #include "lsgpu_internal.h"
 #include "lsgpu_gather.h"
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 namespace lsg {
@@ -360,19 +361,28 @@ int lsg_learned_features(lsg_scene* s, const lsg_texel* texels, uint32_t n, cons
     sv.col = C.view(); sv.verts = C.d_verts; sv.idx = C.d_idx; sv.vnormal = s->impl.d_vnormal;
     sv.side = s->impl.d_side; sv.exitance = s->impl.d_exit;
     hipStream_t st = (hipStream_t)stream;
-    lsg_lightmap* dmaps = nullptr;
-    LSG_HIP(hipMalloc(&dmaps, sizeof(lsg_lightmap) * nmaps));
-    hipError_t e = hipMemcpyAsync(dmaps, maps, sizeof(lsg_lightmap) * nmaps, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) {
-        k_zero4<<<nb(3ull * nslots), 256, 0, st>>>((float4*)work, 3ull * nslots);
-        k_features_point<<<nb(n), 256, 0, st>>>(sv, texels, n, subs, (const float4*)parts, tri_albedo, s->impl.d_lights,
-                                                  s->impl.nlights, 1.f / scale, x, (float4*)work);
-        k_features_window<<<nb(n), 256, 0, st>>>(texels, n, dmaps, nmaps, (const float4*)work, 1.f / scale, x);
-        e = hipGetLastError();
+    // the lightmap list stays on the device between passes; uploaded again only when it
+    // changes (the upload waits for earlier work on st that may still read it)
+    SceneImpl& S = s->impl;
+    const bool same = S.d_lmaps && S.h_lmaps.size() == nmaps &&
+                      !memcmp(S.h_lmaps.data(), maps, sizeof(lsg_lightmap) * nmaps);
+    if (!same) {
+        if (nmaps > S.lmaps_cap) {
+            LSG_HIP(hipStreamSynchronize(st));
+            if (S.d_lmaps) (void)hipFree(S.d_lmaps);
+            S.d_lmaps = nullptr; S.lmaps_cap = 0;
+            LSG_HIP(hipMalloc(&S.d_lmaps, sizeof(lsg_lightmap) * nmaps));
+            S.lmaps_cap = nmaps;
+        }
+        S.h_lmaps.assign(maps, maps + nmaps);
+        LSG_HIP(hipStreamSynchronize(st));
+        LSG_HIP(hipMemcpy(S.d_lmaps, S.h_lmaps.data(), sizeof(lsg_lightmap) * nmaps, hipMemcpyHostToDevice));
     }
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    (void)hipFree(dmaps);
-    if (e != hipSuccess) return hip_fail(e, "lsg_learned_features", __LINE__);
+    k_zero4<<<nb(3ull * nslots), 256, 0, st>>>((float4*)work, 3ull * nslots);
+    k_features_point<<<nb(n), 256, 0, st>>>(sv, texels, n, subs, (const float4*)parts, tri_albedo, S.d_lights,
+                                              S.nlights, 1.f / scale, x, (float4*)work);
+    k_features_window<<<nb(n), 256, 0, st>>>(texels, n, S.d_lmaps, nmaps, (const float4*)work, 1.f / scale, x);
+    LSG_HIP(hipGetLastError());
     return LSG_OK;
 }
 
