@@ -21,6 +21,7 @@
 // edges), so that log1p(E/s) = b + r, and the predicted RMS error of log1p(E/s) (1).
 #include "lsgpu_internal.h"
 #include "lsgpu_gather.h"
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -169,23 +170,25 @@ __global__ __launch_bounds__(256) void k_features_window(const lsg_texel* __rest
     xi[29] = c5 * (1.f / ((2 * kWin + 1) * (2 * kWin + 1)));
 }
 
-// sum of the cheap estimate's mean channel over reliable texels -> scale
-__global__ __launch_bounds__(256) void k_cheap_sum(const float4* __restrict__ parts, uint32_t n, float* __restrict__ acc)
+// sum of the cheap estimate's mean channel over reliable texels -> scale.  A fixed grid
+// strides over the texels; each workgroup writes its partial sums (in double) to its
+// own slot, and the host adds the slots in order: no atomics, the same result every run.
+constexpr uint32_t kSumBlocks = 1024;
+__global__ __launch_bounds__(256) void k_cheap_sum(const float4* __restrict__ parts, uint32_t n, double* __restrict__ partial)
 {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    float v = 0.f, c = 0.f;
-    if (i < n) {
+    double v = 0.0, c = 0.0;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
         const float4 a = parts[2 * (size_t)i], b = parts[2 * (size_t)i + 1];
-        if (a.w > 0 || b.w > 0) { v = (a.x + a.y + a.z + b.x + b.y + b.z) / 3.f; c = 1.f; }
+        if (a.w > 0 || b.w > 0) { v += (a.x + a.y + a.z + b.x + b.y + b.z) / 3.f; c += 1.0; }
     }
-    __shared__ float s[2][256];
+    __shared__ double s[2][256];
     s[0][threadIdx.x] = v; s[1][threadIdx.x] = c;
     __syncthreads();
     for (int w = 128; w; w >>= 1) {
         if ((int)threadIdx.x < w) { s[0][threadIdx.x] += s[0][threadIdx.x + w]; s[1][threadIdx.x] += s[1][threadIdx.x + w]; }
         __syncthreads();
     }
-    if (threadIdx.x == 0) { atomicAdd(&acc[0], s[0][0]); atomicAdd(&acc[1], s[1][0]); }
+    if (threadIdx.x == 0) { partial[2 * blockIdx.x] = s[0][0]; partial[2 * blockIdx.x + 1] = s[1][0]; }
 }
 
 // log1p(E/s) of the baseline, channel c
@@ -330,15 +333,19 @@ int lsg_gather_parts_device(lsg_scene* s, const lsg_texel* texels, uint32_t ntex
 int lsg_learned_scale(const float* parts, uint32_t n, float* scale)
 {
     if (!scale || (n && !parts)) return fail(LSG_ERR_INVALID, "lsg_learned_scale: null argument");
-    float* d = nullptr;
-    LSG_HIP(hipMalloc(&d, 8));
-    hipError_t e = hipMemset(d, 0, 8);
-    if (e == hipSuccess && n) k_cheap_sum<<<nb(n), 256>>>((const float4*)parts, n, d);
-    float h[2] = {0, 0};
-    if (e == hipSuccess) e = hipMemcpy(h, d, 8, hipMemcpyDeviceToHost);
+    if (!n) { *scale = 1.f; return LSG_OK; }
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>(nb(n), kSumBlocks);
+    double* d = nullptr;
+    LSG_HIP(hipMalloc(&d, 16 * (size_t)blocks));
+    k_cheap_sum<<<blocks, 256>>>((const float4*)parts, n, d);
+    std::vector<double> h(2 * (size_t)blocks);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpy(h.data(), d, 16 * (size_t)blocks, hipMemcpyDeviceToHost);
     (void)hipFree(d);
     if (e != hipSuccess) return hip_fail(e, "scale", __LINE__);
-    *scale = (h[1] > 0 && h[0] > 0) ? h[0] / h[1] : 1.f;
+    double v = 0.0, c = 0.0;
+    for (uint32_t b = 0; b < blocks; b++) { v += h[2 * b]; c += h[2 * b + 1]; }
+    *scale = (c > 0 && v > 0) ? (float)(v / c) : 1.f;
     return LSG_OK;
 }
 
