@@ -641,7 +641,8 @@ __device__ void gather_texel_refill(const GatherArgs& A, uint32_t ti, uint32_t l
                     if (L.cast_shadows) {
                         TraceResult r;
                         uint32_t nv_ = 0, nt_ = 0;
-                        trace_closest<STACK, false, GatherFilter, false>(flt, A.sc.col, o, make_float3(dx, dy, dz), A.tmin, dsz, mystk, stride, r, nv_, nt_);
+                        // min/max visit: this kernel keeps a hemisphere ray's state alongside
+                        trace_closest<STACK, false, GatherFilter, false, false>(flt, A.sc.col, o, make_float3(dx, dy, dz), A.tmin, dsz, mystk, stride, r, nv_, nt_);
                         blocked = r.tri >= 0; bt = r.t;
                     }
                     if (!blocked) {
@@ -846,7 +847,9 @@ __device__ void gather_texel_lane(const GatherArgs& A, uint32_t ti, int32_t* stk
                 GatherFilter flt{A.sc.verts, A.sc.idx, A.sc.side, tri, near_t, 0, rug_below};
                 TraceResult r;
                 uint32_t nv_ = 0, nt_ = 0;
-                trace_closest<STACK, false>(flt, A.sc.col, o, d, A.tmin, A.locality, stk, stride, r, nv_, nt_);
+                // min/max visit: measured faster here (the near/far form raised this kernel
+                // from 109 to 122 VGPRs and the 10M-triangle city's cheap gathers slowed)
+                trace_closest<STACK, false, GatherFilter, true, false>(flt, A.sc.col, o, d, A.tmin, A.locality, stk, stride, r, nv_, nt_);
                 if (r.tri >= 0) {
                     const float4 e = A.sc.exitance[2 * (size_t)r.tri + (r.front ? 0 : 1)];
                     hr += e.x; hg += e.y; hb += e.z;
@@ -872,7 +875,7 @@ __device__ void gather_texel_lane(const GatherArgs& A, uint32_t ti, int32_t* stk
                     if (L.cast_shadows) {
                         TraceResult r;
                         uint32_t nv_ = 0, nt_ = 0;
-                        trace_closest<STACK, false, GatherFilter, false>(flt, A.sc.col, o, make_float3(dx, dy, dz), A.tmin, dsz, stk, stride, r, nv_, nt_);
+                        trace_closest<STACK, false, GatherFilter, false, false>(flt, A.sc.col, o, make_float3(dx, dy, dz), A.tmin, dsz, stk, stride, r, nv_, nt_);
                         blocked = r.tri >= 0; bt = r.t;
                     }
                     if (!blocked) {
