@@ -25,27 +25,12 @@ struct NoFilter {
     __device__ float clamp(float best) const { return best; }
 };
 
-// Child-box slab tests of one node.  noi = -o * idir per ray, so each plane distance
-// is one fma: (b - o) * idir = b * idir - o * idir.  The fused form differs from the
-// two-step one by about |o * idir| ulp, i.e. |o| * 6e-8 in distance, while every child
-// box is inflated by DELTA_BSP = max|coordinate| * 1e-5; culling stays conservative and
-// the hit, decided only by the triangle test, is unchanged.
-__device__ inline void slab2(const Node& nd, const float3 noi, const float3 idir, float tlo, float thi,
-                             float& n0, float& f0, float& n1, float& f1)
-{
-    float a, b, ta, tb, tc, td, te, tf;
-    a = __fmaf_rn(nd.q0.x, idir.x, noi.x); b = __fmaf_rn(nd.q0.y, idir.x, noi.x); ta = fminf(a, b); tb = fmaxf(a, b);
-    a = __fmaf_rn(nd.q0.z, idir.y, noi.y); b = __fmaf_rn(nd.q0.w, idir.y, noi.y); tc = fminf(a, b); td = fmaxf(a, b);
-    a = __fmaf_rn(nd.q1.x, idir.z, noi.z); b = __fmaf_rn(nd.q1.y, idir.z, noi.z); te = fminf(a, b); tf = fmaxf(a, b);
-    n0 = fmaxf(fmaxf(ta, tc), fmaxf(te, tlo)); f0 = fminf(fminf(tb, td), fminf(tf, thi));
-    a = __fmaf_rn(nd.q1.z, idir.x, noi.x); b = __fmaf_rn(nd.q1.w, idir.x, noi.x); ta = fminf(a, b); tb = fmaxf(a, b);
-    a = __fmaf_rn(nd.q2.x, idir.y, noi.y); b = __fmaf_rn(nd.q2.y, idir.y, noi.y); tc = fminf(a, b); td = fmaxf(a, b);
-    a = __fmaf_rn(nd.q2.z, idir.z, noi.z); b = __fmaf_rn(nd.q2.w, idir.z, noi.z); te = fminf(a, b); tf = fmaxf(a, b);
-    n1 = fmaxf(fmaxf(ta, tc), fmaxf(te, tlo)); f1 = fminf(fminf(tb, td), fminf(tf, thi));
-}
-
-// One BVH4 node: the four child-box slab tests (one fma per plane, as slab2), then the
-// children that are hit, nearest first: the nearest is returned as the next ref, the
+// One BVH4 node: the four child-box slab tests, then the children that are hit, nearest
+// first.  noi = -o * idir per ray, so each plane distance is one fma: (b - o) * idir =
+// b * idir - o * idir.  The fused form differs from the two-step one by about |o * idir|
+// ulp, i.e. |o| * 6e-8 in distance, while every child box is inflated by DELTA_BSP =
+// max|coordinate| * 1e-5; culling stays conservative and the hit, decided only by the
+// triangle test, is unchanged.  The nearest child is returned as the next ref, the
 // others go on the stack farthest first.  No child hit: the next ref is popped.  The
 // tree is collapsed so that the stack never holds more than STACK-1 entries.
 __device__ inline void cswap(float& ka, int32_t& ra, float& kb, int32_t& rb)
