@@ -425,7 +425,7 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
             const uint64_t hmask = (__ballot(doH) >> gbase) & gmask;
             const uint32_t need = (uint32_t)__popcll(hmask);
             const uint32_t myrank = (uint32_t)__popcll(hmask & lanemask_lt(lane));
-            uint32_t my_index = 0;
+            float my_x = 0.f, my_y = 0.f;   // the lane's accepted filler point (hand-out below)
             uint32_t got = 0;
             while (got < need) {
                 const uint32_t cand = filler + lane;
@@ -436,8 +436,9 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
                 const uint32_t na = (uint32_t)__popcll(am);
                 const uint32_t want = doH ? myrank - got : 64u;
                 const uint32_t src = (doH && myrank >= got && want < na) ? nth_set_bit(am, want) : lane;
-                const uint32_t val = (uint32_t)__shfl((int)cand, (int)(gbase + src));
-                if (doH && myrank >= got && want < na) my_index = val;
+                // the accepted point itself travels with it (no second filler_point call)
+                const float vx = __shfl(cx, (int)(gbase + src)), vy = __shfl(cy, (int)(gbase + src));
+                if (doH && myrank >= got && want < na) { my_x = vx; my_y = vy; }
                 if (got + na <= need) { got += na; filler += (uint32_t)G; }
                 else {
                     const uint32_t last = nth_set_bit(am, need - got - 1u);
@@ -463,8 +464,7 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
                 const float tx = ox / olen, ty = oy / olen, tz = oz / olen;
                 const float bx = ny * tz - nz * ty, by = -nx * tz + nz * tx, bz = nx * ty - ny * tx;
                 GatherFilter flt{A.sc.verts, A.sc.idx, A.sc.side, tri, near_t, 0, rug_below};
-                float x, y;
-                filler_point(my_index, x, y);
+                const float x = my_x, y = my_y;
                 const float dist = x * x + y * y;
                 const float cosa = sqrtf(1 - dist);                          // gather.cpp:125-126
                 const float3 d = make_float3(nx * cosa + tx * x + bx * y, ny * cosa + ty * x + by * y, nz * cosa + tz * x + bz * y);
@@ -685,7 +685,8 @@ __device__ void gather_texel_refill(const GatherArgs& A, uint32_t ti, uint32_t l
                 const uint64_t hmask = __ballot(doH);
                 const uint32_t need = (uint32_t)__popcll(hmask);
                 const uint32_t myrank = (uint32_t)__popcll(hmask & lanemask_lt(lane));
-                uint32_t my_index = 0, got = 0;
+                float my_x = 0.f, my_y = 0.f;
+                uint32_t got = 0;
                 while (got < need) {
                     const uint32_t cand = filler + lane;
                     float cx, cy;
@@ -695,8 +696,8 @@ __device__ void gather_texel_refill(const GatherArgs& A, uint32_t ti, uint32_t l
                     const uint32_t na = (uint32_t)__popcll(am);
                     const uint32_t want = doH ? myrank - got : 64u;
                     const uint32_t src = (doH && myrank >= got && want < na) ? nth_set_bit(am, want) : lane;
-                    const uint32_t val = (uint32_t)__shfl((int)cand, (int)src);
-                    if (doH && myrank >= got && want < na) my_index = val;
+                    const float vx = __shfl(cx, (int)src), vy = __shfl(cy, (int)src);
+                    if (doH && myrank >= got && want < na) { my_x = vx; my_y = vy; }
                     if (got + na <= need) { got += na; filler += 64u; }
                     else {
                         const uint32_t last = nth_set_bit(am, need - got - 1u);
@@ -721,8 +722,7 @@ __device__ void gather_texel_refill(const GatherArgs& A, uint32_t ti, uint32_t l
                     const float olen = sqrtf(ox * ox + oy * oy + oz * oz);
                     const float tx = ox / olen, ty = oy / olen, tz = oz / olen;
                     const float bx = ny * tz - nz * ty, by = -nx * tz + nz * tx, bz = nx * ty - ny * tx;
-                    float x, y;
-                    filler_point(my_index, x, y);
+                    const float x = my_x, y = my_y;
                     const float dist = x * x + y * y;
                     const float cosa = sqrtf(1 - dist);                          // gather.cpp:125-126
                     const float3 d = make_float3(nx * cosa + tx * x + bx * y, ny * cosa + ty * x + by * y, nz * cosa + tz * x + bz * y);
