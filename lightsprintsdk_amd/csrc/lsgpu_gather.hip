@@ -186,6 +186,66 @@ __device__ inline void shooter_at(const GatherArgs& A, const lsg_subtexel* ts, u
                     v0.z + (v1.z - v0.z) * ux + (v2.z - v0.z) * uy);
 }
 
+// The normal at subtexel st's centroid (gather.cpp:727-737) and the hemisphere basis built
+// from it (RRMathPrivate.h:48-53): nine floats, the same for every ray of the subtexel.
+__device__ inline void subtexel_frame(const GatherArgs& A, const lsg_subtexel& st, float* f)
+{
+    const uint32_t tri = st.triangle;
+    const uint32_t i0 = A.sc.idx[3 * (size_t)tri], i1 = A.sc.idx[3 * (size_t)tri + 1], i2 = A.sc.idx[3 * (size_t)tri + 2];
+    const float cxs = (st.uv[0] + st.uv[2] + st.uv[4]) * 0.333333333f;
+    const float cys = (st.uv[1] + st.uv[3] + st.uv[5]) * 0.333333333f;
+    const float w = 1 - cxs - cys;
+    const float* NV = A.sc.vnormal;
+    float nx = NV[3 * (size_t)i0] * w + NV[3 * (size_t)i1] * cxs + NV[3 * (size_t)i2] * cys;
+    float ny = NV[3 * (size_t)i0 + 1] * w + NV[3 * (size_t)i1 + 1] * cxs + NV[3 * (size_t)i2 + 1] * cys;
+    float nz = NV[3 * (size_t)i0 + 2] * w + NV[3 * (size_t)i1 + 2] * cxs + NV[3 * (size_t)i2 + 2] * cys;
+    const float nlen = sqrtf(nx * nx + ny * ny + nz * nz);
+    nx = nx / nlen; ny = ny / nlen; nz = nz / nlen;
+    float ox, oy, oz;
+    if (fabsf(nx) > fabsf(nz)) { ox = -ny; oy = nx; oz = 0.f; } else { ox = 0.f; oy = -nz; oz = ny; }
+    const float olen = sqrtf(ox * ox + oy * oy + oz * oz);
+    const float tx = ox / olen, ty = oy / olen, tz = oz / olen;
+    f[0] = nx; f[1] = ny; f[2] = nz;
+    f[3] = tx; f[4] = ty; f[5] = tz;
+    f[6] = ny * tz - nz * ty; f[7] = -nx * tz + nz * tx; f[8] = nx * ty - ny * tx;
+}
+
+// subtexels per texel whose frames a wave keeps in LDS (later ones are computed per ray)
+constexpr uint32_t kSubCache = 8;
+
+// shooter_at with the subtexel's frame read from the wave's LDS cache (kSubCache frames of
+// nine floats, frame k at cache[k * 9]) when the subtexel is one of the first kSubCache;
+// returns the subtexel index.  Origin and normal equal shooter_at's bit for bit.
+__device__ inline uint32_t shooter_cached(const GatherArgs& A, const lsg_subtexel* ts, uint32_t m, float area_max, uint32_t n,
+                                          uint32_t s, uint32_t key, uint32_t jitter_index, const float* cache, float3& o,
+                                          float& nx, float& ny, float& nz, uint32_t& tri)
+{
+    const float pos = ((float)s + 0.5f) * area_max / (float)n;
+    uint32_t j = 0;
+    float acc = ts[0].area;
+    while (j + 1 < m && acc <= pos) { j++; acc += ts[j].area; }
+    if (j >= kSubCache) {
+        shooter_at(A, ts, m, area_max, n, s, key, jitter_index, o, nx, ny, nz, tri);
+        return j;
+    }
+    const lsg_subtexel st = ts[j];
+    tri = st.triangle;
+    const uint32_t i0 = A.sc.idx[3 * (size_t)tri], i1 = A.sc.idx[3 * (size_t)tri + 1], i2 = A.sc.idx[3 * (size_t)tri + 2];
+    const float* V = A.sc.verts;
+    const float3 v0 = make_float3(V[3 * (size_t)i0], V[3 * (size_t)i0 + 1], V[3 * (size_t)i0 + 2]);
+    const float3 v1 = make_float3(V[3 * (size_t)i1], V[3 * (size_t)i1 + 1], V[3 * (size_t)i1 + 2]);
+    const float3 v2 = make_float3(V[3 * (size_t)i2], V[3 * (size_t)i2 + 1], V[3 * (size_t)i2 + 2]);
+    nx = cache[9 * j]; ny = cache[9 * j + 1]; nz = cache[9 * j + 2];
+    float ju, jv;
+    jitter(key, jitter_index, ju, jv);
+    const float ux = st.uv[0] + (st.uv[2] - st.uv[0]) * ju + (st.uv[4] - st.uv[0]) * jv;
+    const float uy = st.uv[1] + (st.uv[3] - st.uv[1]) * ju + (st.uv[5] - st.uv[1]) * jv;
+    o = make_float3(v0.x + (v1.x - v0.x) * ux + (v2.x - v0.x) * uy,
+                    v0.y + (v1.y - v0.y) * ux + (v2.y - v0.y) * uy,
+                    v0.z + (v1.z - v0.z) * ux + (v2.z - v0.z) * uy);
+    return j;
+}
+
 // asin in binary32 by Abramowitz & Stegun 4.4.46 (|error| <= 2e-8 in exact arithmetic):
 // asin(x) = pi/2 - sqrt(1-x) * (a0 + a1 x + ... + a7 x^7) for 0 <= x <= 1, odd.  Plain
 // float multiplies and adds in a fixed order, so the GPU and the CPU statement round
@@ -289,8 +349,12 @@ __device__ inline float3 layer_dir(int i, const float3& t, const float3& b, cons
 // ENV: hemisphere rays that leave the scene sample the environment (a separate
 // instantiation, so that scenes without one keep the smaller register footprint).
 template <int STACK, bool STATS, int G = 64, bool LAYERS = false, bool ENV = false>
-__device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, int32_t* mystk, uint32_t* lq, int stride)
+__device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, int32_t* mystk, uint32_t* lq, int stride,
+                             float* subc = nullptr)
 {
+    // one texel per wave: the frames of its first subtexels are kept in the wave's LDS
+    // (when the caller gives one)
+    const bool kCache = G == 64 && !LAYERS && subc != nullptr;
     static_assert(G == 16 || G == 32 || G == 64, "group width");
     const uint32_t lane = wlane & (uint32_t)(G - 1);               // lane in the group
     const uint32_t gbase = wlane & ~(uint32_t)(G - 1);             // group's first lane in the wave
@@ -318,6 +382,13 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
     for (uint32_t j = 0; j < m; j++) area_max += ts[j].area;
     const float near_t = 1000 * A.tmin;
     const float rug_below = __int_as_float(__float_as_int(A.rug) - 1);   // largest float < rugDistance (> 0)
+    if (kCache) {
+        __builtin_amdgcn_wave_barrier();
+        if (lane < min(m, kSubCache)) subtexel_frame(A, ts[lane], subc + 9 * lane);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
 
     float hr = 0.f, hg = 0.f, hb = 0.f, lr = 0.f, lg = 0.f, lb = 0.f;
     // LAYERS: hemisphere and light sums of the four lightmap layers, and the bent normal
@@ -348,7 +419,8 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
                 const uint32_t s = lq0[(e & (uint32_t)(G - 1)) + ((e / (uint32_t)G) & 1u) * stride];
                 float3 o; float nx, ny, nz; uint32_t tri;
                 float3 tg, bt;
-                shooter_at<LAYERS>(A, ts, m, area_max, n, s, key, shooter_base + s, o, nx, ny, nz, tri, &tg, &bt);
+                if (kCache) shooter_cached(A, ts, m, area_max, n, s, key, shooter_base + s, subc, o, nx, ny, nz, tri);
+                else shooter_at<LAYERS>(A, ts, m, area_max, n, s, key, shooter_base + s, o, nx, ny, nz, tri, &tg, &bt);
                 float3 ld[LAYERS ? 4 : 1];
                 if (dirs) {
 #pragma unroll
@@ -457,12 +529,20 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
             if (doH) {
                 float3 o; float nx, ny, nz; uint32_t tri;
                 float3 tg, bt;
-                shooter_at<LAYERS>(A, ts, m, area_max, n, s, key, shooter_base + s, o, nx, ny, nz, tri, &tg, &bt);
-                float ox, oy, oz;                                               // RRMathPrivate.h:48-53
-                if (fabsf(nx) > fabsf(nz)) { ox = -ny; oy = nx; oz = 0.f; } else { ox = 0.f; oy = -nz; oz = ny; }
-                const float olen = sqrtf(ox * ox + oy * oy + oz * oz);
-                const float tx = ox / olen, ty = oy / olen, tz = oz / olen;
-                const float bx = ny * tz - nz * ty, by = -nx * tz + nz * tx, bz = nx * ty - ny * tx;
+                float tx, ty, tz, bx, by, bz;
+                uint32_t sj = kSubCache;
+                if (kCache) sj = shooter_cached(A, ts, m, area_max, n, s, key, shooter_base + s, subc, o, nx, ny, nz, tri);
+                else shooter_at<LAYERS>(A, ts, m, area_max, n, s, key, shooter_base + s, o, nx, ny, nz, tri, &tg, &bt);
+                if (kCache && sj < kSubCache) {
+                    const float* f = subc + 9 * sj;
+                    tx = f[3]; ty = f[4]; tz = f[5]; bx = f[6]; by = f[7]; bz = f[8];
+                } else {
+                    float ox, oy, oz;                                           // RRMathPrivate.h:48-53
+                    if (fabsf(nx) > fabsf(nz)) { ox = -ny; oy = nx; oz = 0.f; } else { ox = 0.f; oy = -nz; oz = ny; }
+                    const float olen = sqrtf(ox * ox + oy * oy + oz * oz);
+                    tx = ox / olen; ty = oy / olen; tz = oz / olen;
+                    bx = ny * tz - nz * ty; by = -nx * tz + nz * tx; bz = nx * ty - ny * tx;
+                }
                 GatherFilter flt{A.sc.verts, A.sc.idx, A.sc.side, tri, near_t, 0, rug_below};
                 const float x = my_x, y = my_y;
                 const float dist = x * x + y * y;
@@ -935,11 +1015,12 @@ __device__ inline void gather_block(const GatherArgs& A)
 {
     __shared__ int32_t stk[STACK * 256];
     __shared__ uint32_t lq[2 * 256];
+    __shared__ float subc[4 * 9 * kSubCache];
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     // wave-uniform: kept in a scalar register, the texel record is read by scalar loads
     const uint32_t ti = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4u + (tid >> 6)));
     if (ti >= A.ntex) return;                 // whole wavefront leaves together
-    gather_texel<STACK, STATS, 64, LAYERS, ENV>(A, ti, lane, stk + tid, lq + tid, 256);
+    gather_texel<STACK, STATS, 64, LAYERS, ENV>(A, ti, lane, stk + tid, lq + tid, 256, subc + (tid >> 6) * 9 * kSubCache);
 }
 
 template <int STACK, bool STATS, bool ENV = false>
@@ -990,6 +1071,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4)))
 {
     __shared__ int32_t stk[STACK * 64 * WAVES];
     __shared__ uint32_t lq[2 * 64 * WAVES];
+    __shared__ float subc[WAVES * 9 * kSubCache];
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const uint32_t home = Q.nqueues > 1 ? blockIdx.x % Q.nqueues : 0;
     for (uint32_t qi = 0; qi < Q.nqueues; qi++) {
@@ -1005,7 +1087,8 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4)))
             const uint32_t a = tile * Q.tile, b = min(A.ntex, a + Q.tile);
             for (uint32_t ti = a; ti < b; ti++)
                 if constexpr (REFILL) gather_texel_refill<STACK, ENV>(A, ti, lane, stk + tid, lq + tid, 64 * WAVES);
-                else gather_texel<STACK, false, 64, false, ENV>(A, ti, lane, stk + tid, lq + tid, 64 * WAVES);
+                else gather_texel<STACK, false, 64, false, ENV>(A, ti, lane, stk + tid, lq + tid, 64 * WAVES,
+                                                                subc + (tid >> 6) * 9 * kSubCache);
         }
     }
 }
