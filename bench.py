@@ -43,7 +43,7 @@ HBM_PEAK_GBS = 8000.0
 NODE_BYTES = 128          # one 4-wide node per traversal step (n_node counts these)
 # L2-to-fabric bytes per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by
 # the same counters on a 1 GiB copy (x2.0 and x1.0): profiles/pmc_traffic_k_gather_k_trace_closest.md
-TRAFFIC_BYTES = {"k_gather": 4.742e9, "k_trace_closest": 1.194e9}
+TRAFFIC_BYTES = {"k_gather": 4.691e9, "k_trace_closest": 1.194e9}
 METRIC = "closest-hit Mrays/s; baked texels/s at ≤2% RMSE, plain vs learned gather"
 
 
