@@ -666,7 +666,7 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
 constexpr uint32_t kGatherRefill = 16;
 
 template <int STACK, bool ENV>
-__device__ void gather_texel_refill(const GatherArgs& A, uint32_t ti, uint32_t lane, int32_t* mystk, uint32_t* lq, int stride)
+__device__ void gather_texel_refill(const GatherArgs& A, uint32_t ti, uint32_t lane, int32_t* mystk, uint32_t* lq, int stride, float* subc)
 {
     const lsg_texel T = A.texels[ti];
     const uint32_t m = T.sub_count_flags & 0xffffu, flags = (T.sub_count_flags >> 16) & 0xffu;
@@ -686,6 +686,12 @@ __device__ void gather_texel_refill(const GatherArgs& A, uint32_t ti, uint32_t l
     for (uint32_t j = 0; j < m; j++) area_max += ts[j].area;
     const float near_t = 1000 * A.tmin;
     const float rug_below = __int_as_float(__float_as_int(A.rug) - 1);   // largest float < rugDistance (> 0)
+    // the frames of the texel's first subtexels, once per texel, in the wave's LDS
+    __builtin_amdgcn_wave_barrier();
+    if (lane < min(m, kSubCache)) subtexel_frame(A, ts[lane], subc + 9 * lane);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     float hr = 0.f, hg = 0.f, hb = 0.f, lr = 0.f, lg = 0.f, lb = 0.f;
     uint32_t relH = 0, relL = 0, unrelL = 0, shotRounds = 0, shooter_base = 0;
     for (;;) {
@@ -705,7 +711,7 @@ __device__ void gather_texel_refill(const GatherArgs& A, uint32_t ti, uint32_t l
                 const uint32_t e = qhead + lane;
                 const uint32_t s = lq0[(e & 63u) + ((e / 64u) & 1u) * stride];
                 float3 o; float nx, ny, nz; uint32_t tri;
-                shooter_at(A, ts, m, area_max, n, s, key, shooter_base + s, o, nx, ny, nz, tri);
+                shooter_cached(A, ts, m, area_max, n, s, key, shooter_base + s, subc, o, nx, ny, nz, tri);
                 GatherFilter flt{A.sc.verts, A.sc.idx, A.sc.side, tri, near_t, 1, rug_below};
                 for (uint32_t li = 0; li < nl; li++) {
                     const lsg_light L = A.lights[li];
@@ -796,12 +802,18 @@ __device__ void gather_texel_refill(const GatherArgs& A, uint32_t ti, uint32_t l
                 next += take;
                 if (doH) {
                     float3 o; float nx, ny, nz; uint32_t tri;
-                    shooter_at(A, ts, m, area_max, n, s, key, shooter_base + s, o, nx, ny, nz, tri);
-                    float ox, oy, oz;                                               // RRMathPrivate.h:48-53
-                    if (fabsf(nx) > fabsf(nz)) { ox = -ny; oy = nx; oz = 0.f; } else { ox = 0.f; oy = -nz; oz = ny; }
-                    const float olen = sqrtf(ox * ox + oy * oy + oz * oz);
-                    const float tx = ox / olen, ty = oy / olen, tz = oz / olen;
-                    const float bx = ny * tz - nz * ty, by = -nx * tz + nz * tx, bz = nx * ty - ny * tx;
+                    const uint32_t sj = shooter_cached(A, ts, m, area_max, n, s, key, shooter_base + s, subc, o, nx, ny, nz, tri);
+                    float tx, ty, tz, bx, by, bz;
+                    if (sj < kSubCache) {
+                        const float* fr = subc + 9 * sj;
+                        tx = fr[3]; ty = fr[4]; tz = fr[5]; bx = fr[6]; by = fr[7]; bz = fr[8];
+                    } else {
+                        float ox, oy, oz;                                           // RRMathPrivate.h:48-53
+                        if (fabsf(nx) > fabsf(nz)) { ox = -ny; oy = nx; oz = 0.f; } else { ox = 0.f; oy = -nz; oz = ny; }
+                        const float olen = sqrtf(ox * ox + oy * oy + oz * oz);
+                        tx = ox / olen; ty = oy / olen; tz = oz / olen;
+                        bx = ny * tz - nz * ty; by = -nx * tz + nz * tx; bz = nx * ty - ny * tx;
+                    }
                     const float x = my_x, y = my_y;
                     const float dist = x * x + y * y;
                     const float cosa = sqrtf(1 - dist);                          // gather.cpp:125-126
@@ -1065,7 +1077,7 @@ __global__ void k_queue_reset(uint32_t* counters, uint32_t n)
 }
 
 // Capped at 128 VGPRs it runs 4 waves per SIMD, measured faster than 3 with more
-// registers; the chunked form fits, the refill form (REFILL) spills 56 bytes.
+// registers; the chunked form fits, the refill form (REFILL) spills 44 bytes.
 template <int STACK, int WAVES, bool ENV = false, bool REFILL = false>
 __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4))) void k_gather_persistent(GatherArgs A, WorkQueue Q)
 {
@@ -1086,7 +1098,8 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4)))
             if (tile >= t1) break;           // this queue is exhausted for every lane of the wave
             const uint32_t a = tile * Q.tile, b = min(A.ntex, a + Q.tile);
             for (uint32_t ti = a; ti < b; ti++)
-                if constexpr (REFILL) gather_texel_refill<STACK, ENV>(A, ti, lane, stk + tid, lq + tid, 64 * WAVES);
+                if constexpr (REFILL) gather_texel_refill<STACK, ENV>(A, ti, lane, stk + tid, lq + tid, 64 * WAVES,
+                                                                     subc + (tid >> 6) * 9 * kSubCache);
                 else gather_texel<STACK, false, 64, false, ENV>(A, ti, lane, stk + tid, lq + tid, 64 * WAVES,
                                                                 subc + (tid >> 6) * 9 * kSubCache);
         }
