@@ -187,7 +187,8 @@ __device__ inline void shooter_at(const GatherArgs& A, const lsg_subtexel* ts, u
 }
 
 // The normal at subtexel st's centroid (gather.cpp:727-737) and the hemisphere basis built
-// from it (RRMathPrivate.h:48-53): nine floats, the same for every ray of the subtexel.
+// from it (RRMathPrivate.h:48-53), then the triangle's three vertices: kFrame floats, the
+// same for every ray of the subtexel.
 __device__ inline void subtexel_frame(const GatherArgs& A, const lsg_subtexel& st, float* f)
 {
     const uint32_t tri = st.triangle;
@@ -208,13 +209,19 @@ __device__ inline void subtexel_frame(const GatherArgs& A, const lsg_subtexel& s
     f[0] = nx; f[1] = ny; f[2] = nz;
     f[3] = tx; f[4] = ty; f[5] = tz;
     f[6] = ny * tz - nz * ty; f[7] = -nx * tz + nz * tx; f[8] = nx * ty - ny * tx;
+    const float* V = A.sc.verts;
+    for (int k = 0; k < 3; k++) {
+        f[9 + k] = V[3 * (size_t)i0 + k]; f[12 + k] = V[3 * (size_t)i1 + k]; f[15 + k] = V[3 * (size_t)i2 + k];
+    }
 }
 
 // subtexels per texel whose frames a wave keeps in LDS (later ones are computed per ray)
 constexpr uint32_t kSubCache = 8;
+// floats per cached subtexel: normal, tangent, bitangent, three vertices
+constexpr uint32_t kFrame = 18;
 
 // shooter_at with the subtexel's frame read from the wave's LDS cache (kSubCache frames of
-// nine floats, frame k at cache[k * 9]) when the subtexel is one of the first kSubCache;
+// kFrame floats, frame k at cache[k * kFrame]) when the subtexel is one of the first kSubCache;
 // returns the subtexel index.  Origin and normal equal shooter_at's bit for bit.
 __device__ inline uint32_t shooter_cached(const GatherArgs& A, const lsg_subtexel* ts, uint32_t m, float area_max, uint32_t n,
                                           uint32_t s, uint32_t key, uint32_t jitter_index, const float* cache, float3& o,
@@ -230,12 +237,11 @@ __device__ inline uint32_t shooter_cached(const GatherArgs& A, const lsg_subtexe
     }
     const lsg_subtexel st = ts[j];
     tri = st.triangle;
-    const uint32_t i0 = A.sc.idx[3 * (size_t)tri], i1 = A.sc.idx[3 * (size_t)tri + 1], i2 = A.sc.idx[3 * (size_t)tri + 2];
-    const float* V = A.sc.verts;
-    const float3 v0 = make_float3(V[3 * (size_t)i0], V[3 * (size_t)i0 + 1], V[3 * (size_t)i0 + 2]);
-    const float3 v1 = make_float3(V[3 * (size_t)i1], V[3 * (size_t)i1 + 1], V[3 * (size_t)i1 + 2]);
-    const float3 v2 = make_float3(V[3 * (size_t)i2], V[3 * (size_t)i2 + 1], V[3 * (size_t)i2 + 2]);
-    nx = cache[9 * j]; ny = cache[9 * j + 1]; nz = cache[9 * j + 2];
+    const float* c = cache + kFrame * j;
+    const float3 v0 = make_float3(c[9], c[10], c[11]);
+    const float3 v1 = make_float3(c[12], c[13], c[14]);
+    const float3 v2 = make_float3(c[15], c[16], c[17]);
+    nx = c[0]; ny = c[1]; nz = c[2];
     float ju, jv;
     jitter(key, jitter_index, ju, jv);
     const float ux = st.uv[0] + (st.uv[2] - st.uv[0]) * ju + (st.uv[4] - st.uv[0]) * jv;
@@ -384,7 +390,7 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
     const float rug_below = __int_as_float(__float_as_int(A.rug) - 1);   // largest float < rugDistance (> 0)
     if (kCache) {
         __builtin_amdgcn_wave_barrier();
-        if (lane < min(m, kSubCache)) subtexel_frame(A, ts[lane], subc + 9 * lane);
+        if (lane < min(m, kSubCache)) subtexel_frame(A, ts[lane], subc + kFrame * lane);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -534,7 +540,7 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
                 if (kCache) sj = shooter_cached(A, ts, m, area_max, n, s, key, shooter_base + s, subc, o, nx, ny, nz, tri);
                 else shooter_at<LAYERS>(A, ts, m, area_max, n, s, key, shooter_base + s, o, nx, ny, nz, tri, &tg, &bt);
                 if (kCache && sj < kSubCache) {
-                    const float* f = subc + 9 * sj;
+                    const float* f = subc + kFrame * sj;
                     tx = f[3]; ty = f[4]; tz = f[5]; bx = f[6]; by = f[7]; bz = f[8];
                 } else {
                     float ox, oy, oz;                                           // RRMathPrivate.h:48-53
@@ -688,7 +694,7 @@ __device__ void gather_texel_refill(const GatherArgs& A, uint32_t ti, uint32_t l
     const float rug_below = __int_as_float(__float_as_int(A.rug) - 1);   // largest float < rugDistance (> 0)
     // the frames of the texel's first subtexels, once per texel, in the wave's LDS
     __builtin_amdgcn_wave_barrier();
-    if (lane < min(m, kSubCache)) subtexel_frame(A, ts[lane], subc + 9 * lane);
+    if (lane < min(m, kSubCache)) subtexel_frame(A, ts[lane], subc + kFrame * lane);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -805,7 +811,7 @@ __device__ void gather_texel_refill(const GatherArgs& A, uint32_t ti, uint32_t l
                     const uint32_t sj = shooter_cached(A, ts, m, area_max, n, s, key, shooter_base + s, subc, o, nx, ny, nz, tri);
                     float tx, ty, tz, bx, by, bz;
                     if (sj < kSubCache) {
-                        const float* fr = subc + 9 * sj;
+                        const float* fr = subc + kFrame * sj;
                         tx = fr[3]; ty = fr[4]; tz = fr[5]; bx = fr[6]; by = fr[7]; bz = fr[8];
                     } else {
                         float ox, oy, oz;                                           // RRMathPrivate.h:48-53
@@ -1027,12 +1033,12 @@ __device__ inline void gather_block(const GatherArgs& A)
 {
     __shared__ int32_t stk[STACK * 256];
     __shared__ uint32_t lq[2 * 256];
-    __shared__ float subc[4 * 9 * kSubCache];
+    __shared__ float subc[4 * kFrame * kSubCache];
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     // wave-uniform: kept in a scalar register, the texel record is read by scalar loads
     const uint32_t ti = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4u + (tid >> 6)));
     if (ti >= A.ntex) return;                 // whole wavefront leaves together
-    gather_texel<STACK, STATS, 64, LAYERS, ENV>(A, ti, lane, stk + tid, lq + tid, 256, subc + (tid >> 6) * 9 * kSubCache);
+    gather_texel<STACK, STATS, 64, LAYERS, ENV>(A, ti, lane, stk + tid, lq + tid, 256, subc + (tid >> 6) * kFrame * kSubCache);
 }
 
 template <int STACK, bool STATS, bool ENV = false>
@@ -1083,7 +1089,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4)))
 {
     __shared__ int32_t stk[STACK * 64 * WAVES];
     __shared__ uint32_t lq[2 * 64 * WAVES];
-    __shared__ float subc[WAVES * 9 * kSubCache];
+    __shared__ float subc[WAVES * kFrame * kSubCache];
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const uint32_t home = Q.nqueues > 1 ? blockIdx.x % Q.nqueues : 0;
     for (uint32_t qi = 0; qi < Q.nqueues; qi++) {
@@ -1099,9 +1105,9 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4)))
             const uint32_t a = tile * Q.tile, b = min(A.ntex, a + Q.tile);
             for (uint32_t ti = a; ti < b; ti++)
                 if constexpr (REFILL) gather_texel_refill<STACK, ENV>(A, ti, lane, stk + tid, lq + tid, 64 * WAVES,
-                                                                     subc + (tid >> 6) * 9 * kSubCache);
+                                                                     subc + (tid >> 6) * kFrame * kSubCache);
                 else gather_texel<STACK, false, 64, false, ENV>(A, ti, lane, stk + tid, lq + tid, 64 * WAVES,
-                                                                subc + (tid >> 6) * 9 * kSubCache);
+                                                                subc + (tid >> 6) * kFrame * kSubCache);
         }
     }
 }
