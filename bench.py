@@ -63,7 +63,7 @@ def _events():
     return L, s, a, b
 
 
-# ---- config 1: closest hit ---------------------------------------------------------------
+# ---- closest hit: 100,820-triangle displaced icosphere, 1M rays ---------------------------------------------------------------
 
 def collider_workload(steps, warmup, cpu_threads, with_cpu=True):
     from lightsprintsdk_amd import native as N, scenes
@@ -125,13 +125,13 @@ def collider_cpu_baseline(V, I, R, threads, col):
                 sample=f"oracle C traversal of the same tree, {n} of the rays")
 
 
-# ---- config 2: plain bake ----------------------------------------------------------------
+# ---- plain bake: box room, ~50k triangles, 21 lightmaps of 256x256 ----------------------------------------------------------------
 
 def bake_workload(steps: int, warmup: int, cpu_threads: int):
-    """Plain-gather bake of config 2; one step = final gather + filters + RGBF writeback."""
+    """Plain-gather bake of the box room; one step = final gather + filters + RGBF writeback."""
     from lightsprintsdk_amd import native as N, scenes as S
-    from lightsprintsdk_amd.bake import SceneBake, GatherParams, config2_scene
-    sc = config2_scene()
+    from lightsprintsdk_amd.bake import SceneBake, GatherParams, box_room_scene
+    sc = box_room_scene()
     t0 = time.perf_counter()
     sb = SceneBake(sc, GatherParams(quality=1024, ray_length_min=S.minimal_safe_distance(sc.verts), seed=1))
     setup_s = time.perf_counter() - t0
@@ -220,7 +220,7 @@ def bake_cpu_baseline(sb, threads: int, target_s: float = 12.0):
                        f"{dt:.1f} s")
 
 
-# ---- config 4: learned + fallback vs plain on the held-out scenes -------------------------
+# ---- learned + fallback vs plain on the held-out box-room scenes -------------------------
 
 def learned_summary(reps: int = 2, log=None):
     """The committed surrogate on the 5 held-out box-room scenes (the plain bake of each
@@ -276,10 +276,10 @@ def learned_workload(steps: int, warmup: int):
                 learned=summ)
 
 
-# ---- config 5: large re-bake with a sun sweep ---------------------------------------------
+# ---- large re-bake with a sun sweep (10M-triangle city and terrain) ---------------------------------------------
 
 def scale_workload(steps: int, warmup: int, sweep: int = 60, log=None):
-    """Config 5: city of 5000 extruded boxes + 2048^2-vertex terrain (~10M triangles),
+    """Large re-bake: city of 5000 extruded boxes + 2048^2-vertex terrain (~10M triangles),
     64 lightmaps of 2048^2.  One step = a full re-bake for the next sun direction of a
     60-step sweep: first gather (exitance) + learned+fallback final gather + filters.
     `steps` directions spread over the sweep are timed (p50/p99); the plain re-bake of
@@ -299,7 +299,7 @@ def scale_workload(steps: int, warmup: int, sweep: int = 60, log=None):
     ls = Lr.LearnedScene(sc, seed=1)
     setup_s = time.perf_counter() - t0
     ntex = ls.n
-    log(f"config 5: {sc.ntri} triangles, {len(sc.objects)} lightmaps, {ntex} texels; generate {gen_s:.1f}s, setup {setup_s:.1f}s")
+    log(f"city re-bake: {sc.ntri} triangles, {len(sc.objects)} lightmaps, {ntex} texels; generate {gen_s:.1f}s, setup {setup_s:.1f}s")
 
     def set_sun(i):
         lt = dict(sc.lights[0]); lt["direction"] = S.sun_sweep(i, sweep)
@@ -347,10 +347,10 @@ def scale_workload(steps: int, warmup: int, sweep: int = 60, log=None):
                            surrogate=dict(family="city", samples=meta.get("samples"), train_s=meta.get("train_s"))))
 
 
-# ---- config 3: batched persistent bake ----------------------------------------------------
+# ---- batched persistent bake: 200 objects, ~2M triangles ----------------------------------------------------
 
 def batched_workload(steps: int, warmup: int, n_objects: int = 200, target_tris: int = 2_000_000):
-    """Config 3: 200 objects (~2M triangles), lightmaps 32..1024, 1024 rays per texel,
+    """Batched bake: 200 objects (~2M triangles), lightmaps 32..1024, 1024 rays per texel,
     sun + 4 point lights; one step = first gather + persistent final gather + filters
     replayed from a hipGraph, launch shape from the autotuner."""
     from lightsprintsdk_amd import native as N, scenes as S
@@ -399,8 +399,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 (rays and texels shard "
-                                   "by batch/object across GPUs; multi-GPU runs are not part of this build)",
+        print(json.dumps({"error": "liblsgpu runs on one GPU per process; --gpus must be 1 (rays and texels "
+                                   "would shard by batch/object across GPUs; no multi-GPU path is implemented)",
                           "n_gpus": a.gpus}))
         sys.exit(2)
     threads = a.cpu_threads or min(os.cpu_count() or 1, 16)

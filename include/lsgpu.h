@@ -409,7 +409,7 @@ int  lsg_bake_download_layer(const lsg_bake* bake, uint32_t object, uint32_t lay
  * final gather.  A cheap gather (few rays) feeds an MLP 32-64-64-64-64-4 (BF16 on
  * MFMA, FP32 accumulation) that predicts log1p(E/s) per channel plus its own RMS log
  * error; texels whose predicted error exceeds the bound are re-gathered by the plain
- * gather (processTexel, gather.cpp:600-836), which stays the judge of the result. */
+ * gather (processTexel, gather.cpp:600-836), which stays the measure of the result. */
 typedef struct lsg_mlp lsg_mlp;
 
 uint32_t lsg_mlp_num_params(void);

@@ -382,8 +382,8 @@ class SceneBake:
         return maps
 
 
-def config2_scene(lightmap_size=256):
-    """Config 2: closed 10x10x4 m room + 20 seeded boxes, ~50k triangles, 21 lightmaps."""
+def box_room_scene(lightmap_size=256):
+    """The box room: closed 10x10x4 m room + 20 seeded boxes, ~50k triangles, 21 lightmaps."""
     return S.box_room(seed=3, n_boxes=20, target_tris=50_000, lightmap_size=lightmap_size)
 
 

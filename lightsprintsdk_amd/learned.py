@@ -1,4 +1,4 @@
-"""Learned gather (config 4): an MLP surrogate of the 1024-ray final gather.
+"""Learned gather: an MLP surrogate of the 1024-ray final gather.
 
 Pipeline per scene (all on the GPU):
   cheap gather  -- the plain gather at low quality (16 hemisphere rays, 2 shadow
@@ -10,8 +10,9 @@ Pipeline per scene (all on the GPU):
   decide        -- keep the prediction where predicted error <= tau, queue the rest
   fallback      -- the plain 1024-ray gather on the queued texels
   filters       -- the updateLightmaps filter chain, as after the plain gather
-Training data: texels of seeded config-2-family scenes baked by the plain gather.
-The plain 1024-ray bake stays the judge: RMSE is measured against it, on the
+Training data: texels of seeded box-room scenes (10x10x4 m room, 20 boxes, ~50k
+triangles, 21 lightmaps of 256x256) baked by the plain gather.
+The plain 1024-ray bake stays the ground truth: RMSE is measured against it, on the
 filtered lightmaps both pipelines write.
 """
 from __future__ import annotations
@@ -146,7 +147,7 @@ class Surrogate:
 
 
 class LearnedScene:
-    """One config-2-family scene prepared for the learned gather: the plain bake plan,
+    """One box-room scene prepared for the learned gather: the plain bake plan,
     its device buffers, the cheap-gather parameters and the albedo table."""
 
     def __init__(self, scene: S.BakeScene, quality: int = 1024, seed: int = 1):
@@ -177,7 +178,7 @@ class LearnedScene:
         self.scale = 1.0
 
     def plain(self, stream=None, filtering=True):
-        """The plain 1024-ray gather of every texel into sb.d_out (the judge), then the
+        """The plain 1024-ray gather of every texel into sb.d_out (the ground truth), then the
         filter chain into sb.d_rgbf."""
         self.sb.clear()
         self.sb.final_gather(stream)
@@ -279,13 +280,13 @@ class LearnedScene:
         return nfb
 
 
-def config2_family(seed: int, lightmap_size: int = 256, target_tris: int = 50_000) -> S.BakeScene:
-    """A config-2-family scene: the closed 10x10x4 m room with 20 boxes from seed."""
+def box_room_family(seed: int, lightmap_size: int = 256, target_tris: int = 50_000) -> S.BakeScene:
+    """A box-room scene: the closed 10x10x4 m room with 20 boxes from seed."""
     return S.box_room(seed=seed, n_boxes=20, target_tris=target_tris, lightmap_size=lightmap_size)
 
 
 def city_family(seed: int) -> S.BakeScene:
-    """A config-5-family scene at 1/16 of its area, same densities: a 512 m terrain
+    """A city-and-terrain scene at 1/16 of the 10M-triangle one's area, same densities: a 512 m terrain
     (25 cm texels) and 312 boxes in 3 city lightmaps of 2048^2, sun from the sweep."""
     step = int(np.random.default_rng(seed).integers(0, 60))      # any direction of the sweep
     return S.city_terrain(seed=seed, n_boxes=312, grid_vertices=513, terrain_tiles=1, lightmap_size=2048, city_maps=3,
@@ -293,7 +294,7 @@ def city_family(seed: int) -> S.BakeScene:
 
 
 FAMILIES = {
-    "box_room": dict(make=config2_family, train=range(1000, 1020), cal=(1500, 1501), heldout=range(2000, 2005),
+    "box_room": dict(make=box_room_family, train=range(1000, 1020), cal=(1500, 1501), heldout=range(2000, 2005),
                      weights="surrogate_box_room.npz", sample=None),
     "city": dict(make=city_family, train=range(5000, 5020), cal=(5500, 5501, 5502), heldout=range(6000, 6003),
                  weights="surrogate_city.npz", sample=1_000_000),
@@ -381,7 +382,7 @@ def calibrate_target(net: Surrogate, prepared, target_rmse: float = 0.018, candi
     return best
 
 
-# ---- trained surrogate for the config-2 scene family ---------------------------------------
+# ---- trained surrogate for the box-room scene family ---------------------------------------
 
 _PKG = __file__.rsplit("/", 1)[0]
 WEIGHTS = _PKG + "/" + FAMILIES["box_room"]["weights"]
@@ -479,7 +480,7 @@ def heldout_report(net: Surrogate, bound: dict, seeds=HELDOUT_SEEDS, reps: int =
 
 def _main(argv=None):
     import argparse
-    ap = argparse.ArgumentParser(description="train / evaluate the learned-gather surrogate (config-2 scene family)")
+    ap = argparse.ArgumentParser(description="train / evaluate the learned-gather surrogate (box-room scene family)")
     ap.add_argument("cmd", choices=["train", "eval"])
     ap.add_argument("--family", default="box_room", choices=sorted(FAMILIES))
     ap.add_argument("--weights", default=None)

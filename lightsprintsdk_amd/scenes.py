@@ -384,7 +384,7 @@ def _terrain(rng, origin, size, cells, amp):
 
 def object_field(seed: int = 4, n_objects: int = 200, target_tris: int = 2_000_000, lightmap_range=(32, 1024),
                  area: float = 200.0) -> BakeScene:
-    """Config 3: displaced boxes, cylinders and terrain patches from one seed, each with its
+    """Batched-bake scene: displaced boxes, cylinders and terrain patches from one seed, each with its
     own packed unwrap and a lightmap side log-uniform in lightmap_range; one directional
     sun and four point lights."""
     rng = np.random.default_rng(seed)
@@ -445,7 +445,7 @@ def _terrain_height(rng_params, x, y):
 def city_terrain(seed: int = 5, n_boxes: int = 5000, grid_vertices: int = 2048, terrain_tiles: int = 4,
                  lightmap_size: int = 2048, city_maps: int = 48, box_tris: int = 1_600_000,
                  cell_m: float = 1.0, sun_direction=(0.4, 0.25, -0.88)) -> BakeScene:
-    """Config 5: a grid_vertices^2 heightfield (cell_m metres per cell) split into
+    """City and terrain: a grid_vertices^2 heightfield (cell_m metres per cell) split into
     terrain_tiles^2 lightmapped tiles, and n_boxes extruded boxes (5 faces) spread over
     city_maps lightmaps; all lightmaps lightmap_size^2.  One directional sun."""
     rng = np.random.default_rng(seed)

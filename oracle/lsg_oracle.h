@@ -11,7 +11,7 @@
  * checked in tests/ against golden vectors produced by the compiled reference
  * collider (oracle/_ref), closed forms or brute force.
  *
- * Pinning status (also recorded in DESIGN.md):
+ * Pinning status:
  *   lsgo_srlnp_setup / lsgo_srlnp_hit  -- pinned: golden vectors from the
  *       compiled IT_BSP_FASTER + HelloCollider closed-form KAT + FP64 check.
  *   lsgo_box_*                          -- pinned: slab closed form + goldens.

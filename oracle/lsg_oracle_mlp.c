@@ -2,7 +2,7 @@
  * lsg_oracle_mlp.c -- CPU oracle of the learned-gather MLP.  TEST INFRASTRUCTURE
  * ONLY (see lsg_oracle.h).
  *
- * The surrogate network has no counterpart in the reference (it is this build's
+ * The surrogate network has no counterpart in the reference (it is liblsgpu's
  * addition beside the plain gather), so this file restates liblsgpu's own
  * definition: layers 32-64-64-64-64-16 (4 outputs used), ReLU between layers,
  * loss L = mean_i [ sum_{c<3} (y_ic - t_ic)^2 + lambda (y_i3 - e_i)^2 ] with

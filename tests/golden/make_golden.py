@@ -6,7 +6,7 @@ oracle/_ref/ref_collider from the reference's own sources):
     python tests/golden/make_golden.py
 
 Fixtures (inputs + the reference's outputs, nothing else):
-  collider_ico100k.npz  config-1 mesh (icosphere frequency 71, seed 1; only its
+  collider_ico100k.npz  closest-hit benchmark mesh (icosphere frequency 71, seed 1; only its
                         sha256 is stored, the generator rebuilds it) with 2048
                         seeded rays; IT_BSP_FASTER and IT_LINEAR hit records.
   collider_edge.npz     small icosphere (frequency 6, 720 triangles) stored in
@@ -82,7 +82,7 @@ def plane_mesh():
 def main():
     if not O.have_reference():
         O.build_oracle(ref=True)
-    # config-1 mesh, seeded rays
+    # the closest-hit benchmark mesh, seeded rays
     V, I = scenes.icosphere(scenes.ICOSPHERE_100K_FREQUENCY, seed=1)
     R = scenes.random_rays(V, 2048, seed=2)
     np.savez_compressed(os.path.join(HERE, "collider_ico100k.npz"),

@@ -61,14 +61,14 @@ def test_gpu_vs_oracle_small(gpu, oracle):
     _same_hits(g, gf, o, None)
 
 
-def test_gpu_vs_oracle_config1_full(gpu, oracle):
-    """Config 1 at full size: 100,820-triangle displaced icosphere, 1M rays,
+def test_gpu_vs_oracle_icosphere_full(gpu, oracle):
+    """Full size: 100,820-triangle displaced icosphere, 1M rays,
     bit-exact against the oracle's traversal of the same tree (itself equal to
     brute force, pinned on CPU)."""
     V, I = scenes.icosphere(scenes.ICOSPHERE_100K_FREQUENCY, seed=1)
     R = scenes.random_rays(V, 1_000_000, seed=2)
     c = _col(V, I)
-    print(f"config1: tris={len(I)} rays={len(R)}", flush=True)
+    print(f"icosphere: tris={len(I)} rays={len(R)}", flush=True)
     g, gf = c.intersect_batch(R)
     tree = c.download_tree()
     o, st = oracle.bvh_traverse(tree, V, I, R, threads=16)
@@ -81,7 +81,7 @@ def test_gpu_vs_oracle_config1_full(gpu, oracle):
 
 
 def test_gpu_vs_reference_bsp_faster_golden(gpu, oracle):
-    """Against the compiled reference IT_BSP_FASTER on the config-1 mesh: the
+    """Against the compiled reference IT_BSP_FASTER on the 100,820-triangle icosphere: the
     same triangle for every golden ray, distance bit-exact except the counted
     plane-node cases."""
     z = _load("collider_ico100k")

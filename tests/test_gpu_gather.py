@@ -113,11 +113,11 @@ def test_gpu_low_quality_lanes_vs_oracle(gpu, oracle):
     sb.gs.close()
 
 
-def test_gpu_config2_full_size_sample(gpu, oracle):
-    """Config 2 at full size (box room ~50k triangles, 21 x 256x256, 1024 rays per
+def test_gpu_box_room_full_size_sample(gpu, oracle):
+    """The box room at full size ( ~50k triangles, 21 x 256x256, 1024 rays per
     texel); the oracle re-gathers a spread sample of texels."""
     sc, sb = _bake(boxes=20, tris=50_000, size=256, quality=1024)
-    print(f"config2: tris={sc.ntri} texels={len(sb.texels)} subtexels={len(sb.subs)}", flush=True)
+    print(f"box room: tris={sc.ntri} texels={len(sb.texels)} subtexels={len(sb.subs)}", flush=True)
     sb.first_gather()
     sb.final_gather()
     flat = np.concatenate([m.reshape(-1, 4) for m in sb.lightmaps_rgba()])
@@ -126,7 +126,7 @@ def test_gpu_config2_full_size_sample(gpu, oracle):
     tx = sb.texels[sel]
     ref, _ = oracle.gather(sb.gs.oracle_arrays(sb.exitance), sb.params.as_dict(), tx.view(np.uint32).reshape(-1, 4),
                            sb.subs, tree=tree, threads=16)
-    print("config2 sample max rel err", _cmp(flat, ref, tx["out_index"]), flush=True)
+    print("box room sample max rel err", _cmp(flat, ref, tx["out_index"]), flush=True)
     lit = flat[sb.texels["out_index"], :3]
     # box bottoms lie on the floor and face it: their texels are black; the rest is lit
     assert np.isfinite(lit).all() and (lit.max(axis=1) > 0).mean() > 0.8

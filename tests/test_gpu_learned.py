@@ -79,7 +79,7 @@ def test_mlp_training_converges(gpu):
 
 
 def test_learned_gather_heldout(gpu):
-    """Train on seeded config-2-family scenes, pick the error bound on one more scene,
+    """Train on seeded box-room scenes, pick the error bound on one more scene,
     then on a held-out scene the learned+fallback bake stays within 2% RMSE of the
     plain 1024-ray bake (filtered lightmaps, as updateLightmaps writes them) while
     re-gathering only part of the texels, and the network alone does better than the
