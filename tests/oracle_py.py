@@ -344,9 +344,11 @@ def drop_in(verts, idx, rays, technique):
     return out[0], out[1], out[2], info
 
 
-def filter_lightmap(rgba, sigma=1.0, spread=1000.0, bg=(0, 0, 0, 0)):
-    """RRBuffer filter chain on one RGBA lightmap [h,w,4]; returns (rgba_out, rgbf)."""
-    L = lib()
+def filter_lightmap(rgba, sigma=1.0, spread=1000.0, bg=(0, 0, 0, 0), library=None):
+    """RRBuffer filter chain on one RGBA lightmap [h,w,4]; returns (rgba_out, rgbf).
+    `library` is another ctypes library exporting lsgo_filter_lightmap (a planted
+    mutation of the oracle, in the mutation tests); default: the oracle."""
+    L = lib() if library is None else library
     L.lsgo_filter_lightmap.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_float,
                                        ctypes.c_void_p, ctypes.c_void_p]
     img = np.ascontiguousarray(rgba, dtype=np.float32).copy()
