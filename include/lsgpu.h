@@ -168,6 +168,8 @@ int  lsg_collider_trace_stats(const lsg_collider* collider, const void* ray_orig
 #define LSG_SIDE_FRONT_RENDERS  1u   /* sideBits[0].renderFrom */
 #define LSG_SIDE_BACK_RENDERS   2u   /* sideBits[1].renderFrom */
 #define LSG_SIDE_CASTS_SHADOW   4u   /* getTriangleMaterial(t, light, ...) != nullptr */
+#define LSG_SIDE_FRONT_INDIRECT 8u   /* set by lsg_scene_set_indirect: front hits add the corner indirect term */
+#define LSG_SIDE_BACK_INDIRECT 16u   /* ... back hits */
 
 typedef struct lsg_light {
     int32_t type;              /* 0 = point (REALISTIC attenuation), 1 = directional */
@@ -225,6 +227,18 @@ int  lsg_scene_exitance_device(lsg_scene* scene, const float* irradiance, const 
                                void* stream);
 /* copies the current exitance table (same layout) to the host; synchronous */
 int  lsg_scene_get_exitance(const lsg_scene* scene, float* exitance);
+/* Per-vertex indirect light at hemisphere hits.  The reference adds, on top of the flat
+ * direct exitance, the albedo times the indirect irradiance of the hit triangle's three
+ * corners interpolated at the hit's hitPoint2d (PathtracerWorker::getIncidentRadiance,
+ * pathtracer.cpp:326-348; Triangle::getPointMeasure, rrcore.cpp:262-273):
+ *   exitance(hit) = table[side] + albedo * (I0 (1-u-v) + I1 u + I2 v).
+ * corners: host float[ntri*9], RGB of corners 0, 1, 2 per triangle (what
+ * RRSolver::getTriangleMeasure(t, 0..2, RM_IRRADIANCE_LINEAR_INDIRECT) returns); albedo:
+ * host float[ntri*3]; sides: host uint8[ntri], bit 0 = front hits take the term, bit 1 =
+ * back hits (the side is legal and the albedo not black, pathtracer.cpp:191, 325-331);
+ * NULL = both.  The exitance table then holds only the flat part (direct irradiance times
+ * albedo plus emittance).  corners == NULL removes the term. */
+int  lsg_scene_set_indirect(lsg_scene* scene, const float* corners, const float* albedo, const uint8_t* sides);
 int  lsg_scene_set_lights(lsg_scene* scene, const lsg_light* lights, uint32_t nlights);
 
 /* The solver's environment (RRSolver::setEnvironment), which hemisphere rays that leave

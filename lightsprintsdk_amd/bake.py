@@ -162,6 +162,31 @@ class GatherScene:
         N.check(N.lib().lsg_scene_set_tangents(self._h, self.tangents.ctypes.data_as(ctypes.c_void_p),
                                                self.bitangents.ctypes.data_as(ctypes.c_void_p)), "set_tangents")
 
+    def set_indirect(self, corners=None, albedo=None, sides=None):
+        """Per-corner indirect irradiance that hemisphere hits interpolate at their
+        hitPoint2d and multiply by the albedo (pathtracer.cpp:326-348, rrcore.cpp:262-273):
+        corners [ntri, 3 corners, 3] linear RGB, albedo [ntri, 3], sides [ntri] (bit 0 front,
+        bit 1 back take the term; None = both).  corners None removes the term."""
+        L = N.lib()
+        if corners is None:
+            N.check(L.lsg_scene_set_indirect(self._h, None, None, None), "set_indirect")
+            self.ind = None
+            self.side_bits = self.side_bits & np.uint8(~(8 | 16) & 0xFF)
+            return
+        c = np.ascontiguousarray(corners, np.float32).reshape(len(self.idx), 9)
+        a = np.ascontiguousarray(albedo, np.float32).reshape(len(self.idx), 3)
+        sd = None if sides is None else np.ascontiguousarray(sides, np.uint8).reshape(len(self.idx))
+        N.check(L.lsg_scene_set_indirect(self._h, c.ctypes.data_as(ctypes.c_void_p), a.ctypes.data_as(ctypes.c_void_p),
+                                         sd.ctypes.data_as(ctypes.c_void_p) if sd is not None else None), "set_indirect")
+        # the records as the library lays them out: corner k rgb, then albedo channel k
+        rec = np.zeros((len(self.idx), 3, 4), np.float32)
+        rec[:, :, :3] = c.reshape(-1, 3, 3)
+        rec[:, :, 3] = a
+        self.ind = rec.reshape(-1, 12)
+        b = np.full(len(self.idx), 3, np.uint8) if sd is None else sd
+        self.side_bits = (self.side_bits & np.uint8(~(8 | 16) & 0xFF)) | np.where(b & 1, 8, 0).astype(np.uint8) \
+            | np.where(b & 2, 16, 0).astype(np.uint8)
+
     def gather_layers_device(self, dtex: DeviceBuffer, ntex: int, dsub: DeviceBuffer, nsub: int, params: GatherParams,
                              douts, stream=None):
         """processTexel into the output layers of updateLightmap: douts = 5 DeviceBuffers
@@ -200,7 +225,7 @@ class GatherScene:
     def oracle_arrays(self, exitance):
         return dict(verts=self.verts, idx=self.idx, normals=self.normals, side_bits=self.side_bits,
                     tangents=getattr(self, "tangents", None), bitangents=getattr(self, "bitangents", None),
-                    env=getattr(self, "env", None),
+                    env=getattr(self, "env", None), ind=getattr(self, "ind", None),
                     exitance=np.ascontiguousarray(exitance, np.float32).reshape(len(self.idx), 8),
                     lights=np.array([(l["type"], l["position"], l["direction"], l["color"], l["cast_shadows"])
                                      for l in self.lights], dtype=[("type", "<i4"), ("pos", "<f4", 3), ("dir", "<f4", 3),

@@ -82,6 +82,7 @@ struct BakeImpl {
     lsg_texel* d_ttex = nullptr; lsg_subtexel* d_tsub = nullptr; float4* d_irr = nullptr;
     float* d_albedo = nullptr; float* d_emit = nullptr;
     uint32_t* d_queue = nullptr;
+    void* d_walk = nullptr;            // subtexel walk records of the final gather's texels (32 B each)
     lsg_bake_tuning tuning{};
     int cus = 256;
     hipStream_t own = nullptr;
@@ -101,7 +102,7 @@ struct BakeImpl {
         if (graph) (void)hipGraphDestroy(graph);
         if (own) (void)hipStreamDestroy(own);
         dfree(d_tex); dfree(d_sub); dfree(d_rgba); dfree(d_rgbf); dfree(d_maps); scratch_free(S);
-        dfree(d_ttex); dfree(d_tsub); dfree(d_irr); dfree(d_albedo); dfree(d_emit); dfree(d_queue);
+        dfree(d_ttex); dfree(d_tsub); dfree(d_irr); dfree(d_albedo); dfree(d_emit); dfree(d_queue); dfree(d_walk);
     }
 };
 
@@ -120,6 +121,7 @@ static int run_gather_final(BakeImpl& B, const lsg_gather_params* p, const lsg_t
     int stack = 0;
     int e = gather_args(B.scene, tex, ntex, B.d_sub, p, (float*)B.d_rgba, A, stack);
     if (e) return e;
+    A.walk_buf = B.d_walk; A.walk_cap = B.ntex;   // the plan's own records: nothing allocated under capture
     lsg_bake_tuning t = B.tuning;
     default_tuning(t);
     WorkQueue Q;
@@ -236,6 +238,7 @@ int lsg_bake_create(lsg_scene* scene, const lsg_bake_object* objects, uint32_t n
     AL(B.d_rgba, 16 * (size_t)slot);
     AL(B.d_rgbf, 12 * (size_t)slot);
     AL(B.d_queue, 64);
+    AL(B.d_walk, 32 * tex.size());
     // 2. first-gather work: one texel per triangle covering the whole triangle
     std::vector<lsg_texel> ttex(B.ntri);
     std::vector<lsg_subtexel> tsub(B.ntri);

@@ -300,11 +300,12 @@ __global__ __launch_bounds__(256) void k_write_tris(uint32_t n, const uint32_t* 
                                                     const float4* __restrict__ sv_u, const float4* __restrict__ ev_u,
                                                     const uint32_t* __restrict__ bytes, const float4* __restrict__ blo_u,
                                                     const float4* __restrict__ bhi_u, float4* __restrict__ plane, TriRest* __restrict__ rest,
-                                                    float4* __restrict__ plo, float4* __restrict__ phi)
+                                                    float4* __restrict__ plo, float4* __restrict__ phi, uint32_t* __restrict__ slot)
 {
     uint32_t p = blockIdx.x * 256 + threadIdx.x;
     if (p >= n) return;
     uint32_t t = order[p];
+    slot[t] = p;
     plane[p] = plane_u[t];
     TriRest r;
     r.sv = sv_u[t]; r.ev = ev_u[t];
@@ -418,7 +419,7 @@ template <class T> static int dalloc(T** p, size_t n, uint64_t* acc = nullptr)
 static void free_impl(ColliderImpl& c)
 {
     dfree(c.d_verts); dfree(c.d_idx); dfree(c.d_nodes); dfree(c.d_nodes4);
-    dfree(c.d_plane); dfree(c.d_rest);
+    dfree(c.d_plane); dfree(c.d_rest); dfree(c.d_slot);
     c = ColliderImpl();
 }
 
@@ -595,9 +596,11 @@ static int build(ColliderImpl& C, const float* verts, uint32_t nv, const uint32_
     }
     const uint32_t n = nvalid;
     CK(dalloc(&C.d_plane, n, &C.bytes)); CK(dalloc(&C.d_rest, n, &C.bytes));
+    CK(dalloc(&C.d_slot, nt, &C.bytes));
+    CKH(hipMemset(C.d_slot, 0, sizeof(uint32_t) * (size_t)(nt ? nt : 1)));   // degenerate triangles: never hit
     CK(dalloc(&d_plo, n)); CK(dalloc(&d_phi, n));
     if (n) k_write_tris<<<blocks(n), 256>>>(n, d_vals, d_plane_u, d_sv_u, d_ev_u, d_bytes, d_blo_u, d_bhi_u,
-                                            C.d_plane, C.d_rest, d_plo, d_phi);
+                                            C.d_plane, C.d_rest, d_plo, d_phi, C.d_slot);
     CKH(hipGetLastError());
     C.nnodes = n > 1 ? n - 1 : 0;
     CK(dalloc(&C.d_nodes, C.nnodes ? C.nnodes : 1, &C.bytes));

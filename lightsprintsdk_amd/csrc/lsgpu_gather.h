@@ -18,6 +18,8 @@ struct SceneView {
     const float* vbitangent;     // 3 floats per vertex (output layers only)
     const float4* env;           // environment texels, linear rgb, or null
     uint32_t env_type, env_w, env_h;   // LSG_ENV_*; env_type 0 = rays that leave return 0
+    const float4* ind;           // 3 per triangle: corner k = (indirect irradiance rgb, albedo channel k), or null
+    const uint32_t* slot;        // leaf-order slot of each triangle (hit (u, v) recomputed from its record)
 };
 
 struct GatherArgs {
@@ -34,6 +36,11 @@ struct GatherArgs {
     // output layers (LS_LIGHTMAP, LS_DIRECTION1..3, LS_BENT_NORMALS; null = not wanted),
     // used by the layered kernels only; dirs = gatherAllDirections (lightmap.cpp:820)
     float4* layer[5]; uint32_t dirs; uint32_t layered;
+    // subtexel walk records of the first series per texel list entry (k_subtexel_walk, 32
+    // bytes each), or null: the kernels then replay the walk themselves.  walk_buf
+    // (walk_cap records) is where the launch functions write them; null = no records.
+    const struct WalkRec* walk;
+    void* walk_buf; uint32_t walk_cap;
 };
 
 // persistent launches: tiles of texels handed out by atomic counters, one queue per
@@ -60,7 +67,21 @@ struct SceneImpl {
     lsg_lightmap* d_lmaps = nullptr;   // lightmap list of the last learned-feature pass
     uint32_t lmaps_cap = 0;
     std::vector<lsg_lightmap> h_lmaps; // host copy it was uploaded from
+    float4* d_ind = nullptr;           // corner indirect records (3 float4 per triangle), or null
+    std::vector<uint8_t> h_side;       // side bits as given (the indirect bits are merged in)
+    void* d_walk = nullptr;            // subtexel walk records of the last gather launch (32 B each)
+    uint32_t walk_cap = 0;
 };
+
+// walk-record scratch of the scene for a launch of n texels (grown on demand; not while
+// a stream is captured)
+int scene_walk_scratch(lsg_scene* s, uint32_t n, GatherArgs& A);
+// the first series' shooters per texel: max(hemisphere rays, light rounds when lights shoot)
+__host__ __device__ inline uint32_t first_series(const GatherArgs& A)
+{
+    const uint32_t l = (A.nlights && A.rounds) ? A.rounds : 0u;
+    return A.hemi_rays > l ? A.hemi_rays : l;
+}
 
 // per-vertex tangent frames of the scene: the caller's, or built from the vertex
 // normals on stream st when none were set
@@ -69,6 +90,10 @@ int scene_frames(lsg_scene* s, hipStream_t st);
 int gather_args(lsg_scene* s, const lsg_texel* texels, uint32_t ntexels, const lsg_subtexel* subs,
                 const lsg_gather_params* p, float* out_rgba, GatherArgs& A, int& stack);
 int launch_gather(const GatherArgs& A, int stack, hipStream_t st);
+// texels per launch of launch_gather's chunks
+uint32_t gather_chunk(const GatherArgs& A);
+// points A at the scene's walk-record scratch when the launch's kernels read records
+int walk_records(lsg_scene* s, GatherArgs& A);
 int launch_gather_persistent(const GatherArgs& A, int stack, const WorkQueue& Q, uint32_t waves_per_block, uint32_t blocks,
                              bool refill,
                              hipStream_t st);

@@ -17,8 +17,16 @@
 // with a ballot + popcount rank.  Sums are reduced across the wave with shuffles.
 //
 // The reference's rand() draws (filler seed gather.h:102, texel jitter
-// gather.cpp:742-749) become a seeded counter hash, and the subtexel walk
-// (:679-688) a stratified closed form; the CPU oracle uses the same rules.
+// gather.cpp:742-749) become a seeded counter hash; the CPU oracle uses the same
+// rule.  Each shooter's subtexel follows the reference's own area walk
+// (gather.cpp:645-647, 669, 680-688), which is sequential over a texel's shooters:
+// a pre-pass (k_subtexel_walk, one lane per texel) records where the first series
+// enters each subtexel, and later series (or texels the record cannot describe)
+// step the walk in the kernel, group-uniform, as the shooters are handed out.
+//
+// A hemisphere hit reads the flat exitance of the side it sees and, where the scene
+// has per-corner indirect irradiance, adds its interpolation at the hit's hitPoint2d
+// times the albedo (pathtracer.cpp:326-348, rrcore.cpp:262-273).
 #include "lsgpu_internal.h"
 #include "lsgpu_trace.h"
 #include "lsgpu_gather.h"
@@ -135,19 +143,15 @@ struct GatherFilter {
     __device__ float clamp(float best) const { return mode == 1 ? fminf(best, rug_below) : best; }
 };
 
-// Shooter s of a series of n: stratified subtexel pick + seeded jitter -> origin o on
-// the subtexel's triangle tri, and the normalised interpolated vertex normal at the
-// subtexel centroid (gather.cpp:680-752).  FRAME also returns the skewed-normalised
-// tangent and bitangent there (gather.cpp:731-732), which directional layers use.
+// The shooter of subtexel j: seeded jitter -> origin o on the subtexel's triangle tri,
+// and the normalised interpolated vertex normal at the subtexel centroid
+// (gather.cpp:690-752).  FRAME also returns the skewed-normalised tangent and bitangent
+// there (gather.cpp:731-732), which directional layers use.
 template <bool FRAME = false>
-__device__ inline void shooter_at(const GatherArgs& A, const lsg_subtexel* ts, uint32_t m, float area_max, uint32_t n,
-                                  uint32_t s, uint32_t key, uint32_t jitter_index, float3& o, float& nx, float& ny,
-                                  float& nz, uint32_t& tri, float3* tg = nullptr, float3* bt = nullptr)
+__device__ inline void shooter_at(const GatherArgs& A, const lsg_subtexel* ts, uint32_t j, uint32_t key, uint32_t jitter_index,
+                                  float3& o, float& nx, float& ny, float& nz, uint32_t& tri, float3* tg = nullptr,
+                                  float3* bt = nullptr)
 {
-    const float pos = ((float)s + 0.5f) * area_max / (float)n;
-    uint32_t j = 0;
-    float acc = ts[0].area;
-    while (j + 1 < m && acc <= pos) { j++; acc += ts[j].area; }
     const lsg_subtexel st = ts[j];
     tri = st.triangle;
     const uint32_t i0 = A.sc.idx[3 * (size_t)tri], i1 = A.sc.idx[3 * (size_t)tri + 1], i2 = A.sc.idx[3 * (size_t)tri + 2];
@@ -221,19 +225,15 @@ constexpr uint32_t kSubCache = 8;
 constexpr uint32_t kFrame = 18;
 
 // shooter_at with the subtexel's frame read from the wave's LDS cache (kSubCache frames of
-// kFrame floats, frame k at cache[k * kFrame]) when the subtexel is one of the first kSubCache;
-// returns the subtexel index.  Origin and normal equal shooter_at's bit for bit.
-__device__ inline uint32_t shooter_cached(const GatherArgs& A, const lsg_subtexel* ts, uint32_t m, float area_max, uint32_t n,
-                                          uint32_t s, uint32_t key, uint32_t jitter_index, const float* cache, float3& o,
-                                          float& nx, float& ny, float& nz, uint32_t& tri)
+// kFrame floats, frame k at cache[k * kFrame]) when j is one of the first kSubCache
+// subtexels.  Origin and normal equal shooter_at's bit for bit.
+__device__ inline void shooter_cached(const GatherArgs& A, const lsg_subtexel* ts, uint32_t j, uint32_t key,
+                                      uint32_t jitter_index, const float* cache, float3& o, float& nx, float& ny, float& nz,
+                                      uint32_t& tri)
 {
-    const float pos = ((float)s + 0.5f) * area_max / (float)n;
-    uint32_t j = 0;
-    float acc = ts[0].area;
-    while (j + 1 < m && acc <= pos) { j++; acc += ts[j].area; }
     if (j >= kSubCache) {
-        shooter_at(A, ts, m, area_max, n, s, key, jitter_index, o, nx, ny, nz, tri);
-        return j;
+        shooter_at(A, ts, j, key, jitter_index, o, nx, ny, nz, tri);
+        return;
     }
     const lsg_subtexel st = ts[j];
     tri = st.triangle;
@@ -249,7 +249,134 @@ __device__ inline uint32_t shooter_cached(const GatherArgs& A, const lsg_subtexe
     o = make_float3(v0.x + (v1.x - v0.x) * ux + (v2.x - v0.x) * uy,
                     v0.y + (v1.y - v0.y) * ux + (v2.y - v0.y) * uy,
                     v0.z + (v1.z - v0.z) * ux + (v2.z - v0.z) * uy);
+}
+
+// ---- subtexel selection (gather.cpp:645-647, 669, 680-688) -------------------------------
+// The reference keeps an iterator over the texel's subtexels, starting at the first with an
+// area accumulator of minus its area.  A series of n shooters steps by areaMax/(n+0.91); each
+// shooter adds the step, then moves the iterator on (cyclically) and subtracts the new
+// subtexel's area while the accumulator is positive.  Both carry from series to series.
+struct SubWalk {
+    float accu;
+    uint32_t it;
+    __device__ void init(const lsg_subtexel* ts) { it = 0; accu = -ts[0].area; }
+    __device__ uint32_t next(const lsg_subtexel* ts, uint32_t m, float step)
+    {
+        accu += step;
+        while (accu > 0) {
+            it = it + 1 == m ? 0u : it + 1;
+            accu -= ts[it].area;
+        }
+        return it;
+    }
+};
+__host__ __device__ inline float walk_step(float area_max, uint32_t n) { return area_max / ((float)n + 0.91f); }
+
+// Record of a texel's first series (k_subtexel_walk), 32 bytes: sixteen 16-bit fields,
+// field 0 = 1 when the record holds the series, fields 1..15 = the shooter at which the walk
+// enters subtexels 1..15 (0xffff: never in the first series).  The first series walks
+// forward from subtexel 0 and does not wrap (n steps of areaMax/(n+0.91) cover less than
+// the texel), so its shooter s sits in subtexel #{i >= 1 : enter_i <= s}.
+struct WalkRec { uint4 a, b; };
+constexpr uint32_t kWalkRecMax = 16;   // subtexels a record describes
+
+__device__ inline uint32_t walk_field(const WalkRec& R, uint32_t i)
+{
+    const uint4& q = i < 8 ? R.a : R.b;
+    const uint32_t k = i & 7u;
+    const uint32_t w = k < 2 ? q.x : (k < 4 ? q.y : (k < 6 ? q.z : q.w));
+    return (k & 1u) ? (w >> 16) : (w & 0xffffu);
+}
+__device__ inline bool walk_valid(const WalkRec& R) { return (R.a.x & 0xffffu) == 1u; }
+// shooter s of the first series from the record in memory (fields read one by one, so the
+// 32-byte record is not held in registers)
+__device__ inline uint32_t walk_pick(const WalkRec* R, uint32_t m, uint32_t s)
+{
+    const uint16_t* f = (const uint16_t*)R;
+    uint32_t j = 0;
+    for (uint32_t i = 1; i < m; i++) j += f[i] <= s ? 1u : 0u;
     return j;
+}
+__device__ inline bool walk_has(const WalkRec* R) { return ((const uint16_t*)R)[0] == 1u; }
+
+// The subtexel of shooter g0 + k of a texel (k < cnt), by stepping the walk from the
+// texel's first shooter: the first series has n1 shooters, every later one nrest (after
+// the first series only light rounds are shot again, gather.cpp:652-661, and every such
+// series has the same size).  The loop is the same for every lane of the group; lane k
+// keeps shooter g0 + k.  Used where no record applies: later series and texels of more
+// than kWalkRecMax subtexels.
+__device__ inline uint32_t walk_replay(const lsg_subtexel* ts, uint32_t m, float area_max, uint32_t n1, uint32_t nrest,
+                                       uint32_t g0, uint32_t cnt, uint32_t k)
+{
+    SubWalk w;
+    w.init(ts);
+    const float s1 = walk_step(area_max, n1), s2 = walk_step(area_max, nrest);
+    uint32_t g = 0, mine = 0;
+    for (; g < g0; g++) w.next(ts, m, g < n1 ? s1 : s2);
+    for (uint32_t i = 0; i < cnt; i++, g++) {
+        const uint32_t j = w.next(ts, m, g < n1 ? s1 : s2);
+        if (i == k) mine = j;
+    }
+    return mine;
+}
+
+// One lane per texel: the reference's walk over the first series of n1 shooters.  Texels
+// of more than kWalkRecMax subtexels, series beyond 65535 shooters, or a walk that wraps
+// (float rounding at very high n) get no record and replay the walk in the gather kernel.
+__global__ __launch_bounds__(256) void k_subtexel_walk(const lsg_texel* __restrict__ texels, uint32_t ntex,
+                                                       const lsg_subtexel* __restrict__ subs, uint32_t n1,
+                                                       WalkRec* __restrict__ W)
+{
+    const uint32_t ti = blockIdx.x * 256 + threadIdx.x;
+    if (ti >= ntex) return;
+    const lsg_texel T = texels[ti];
+    const uint32_t m = T.sub_count_flags & 0xffffu;
+    WalkRec out = {make_uint4(0u, 0u, 0u, 0u), make_uint4(0u, 0u, 0u, 0u)};
+    if (m >= 2 && m <= kWalkRecMax && n1 >= 1 && n1 <= 65535u) {
+        const lsg_subtexel* ts = subs + T.sub_begin;
+        float area_max = 0.f;
+        for (uint32_t j = 0; j < m; j++) area_max += ts[j].area;
+        const float step = walk_step(area_max, n1);
+        uint32_t f[kWalkRecMax];
+        for (uint32_t i = 0; i < kWalkRecMax; i++) f[i] = 0xffffu;
+        f[0] = 1u;
+        SubWalk w;
+        w.init(ts);
+        uint32_t cur = 0;
+        bool ok = true;
+        for (uint32_t s = 0; s < n1; s++) {
+            const uint32_t j = w.next(ts, m, step);
+            if (j < cur) { ok = false; break; }
+            while (cur < j) f[++cur] = s;
+        }
+        if (ok) {
+            out.a = make_uint4(f[0] | (f[1] << 16), f[2] | (f[3] << 16), f[4] | (f[5] << 16), f[6] | (f[7] << 16));
+            out.b = make_uint4(f[8] | (f[9] << 16), f[10] | (f[11] << 16), f[12] | (f[13] << 16), f[14] | (f[15] << 16));
+        }
+    }
+    W[ti] = out;
+}
+
+// Exitance a hemisphere ray reads at its hit: the flat value of the side it sees (direct
+// light times albedo plus emittance, pathtracer.cpp:235-262, 306-316) plus, on sides that
+// take it, albedo * (I0 (1-u-v) + I1 u + I2 v) of the triangle's corner indirect irradiance
+// at the hit's hitPoint2d (pathtracer.cpp:326-348, Triangle::getPointMeasure
+// rrcore.cpp:262-273).  (u, v) is recomputed by the hit's own triangle test on the
+// recorded distance, bit-identical to the traversal's.
+__device__ inline float4 hit_exitance(const SceneView& S, int tri, bool front, const float3 o, const float3 d, float t)
+{
+    float4 e = S.exitance[2 * (size_t)tri + (front ? 0 : 1)];
+    if (S.ind && (S.side[tri] & (front ? LSG_SIDE_FRONT_INDIRECT : LSG_SIDE_BACK_INDIRECT))) {
+        const TriRest* tr = S.col.rest + S.slot[tri];
+        float u = 0.f, v = 0.f;
+        srlnp_test(tr->sv, tr->ev, (int)tr->meta.y, o, d, t, u, v);
+        const float4 c0 = S.ind[3 * (size_t)tri], c1 = S.ind[3 * (size_t)tri + 1], c2 = S.ind[3 * (size_t)tri + 2];
+        const float w = 1 - u - v;
+        e.x = e.x + (c0.x * w + c1.x * u + c2.x * v) * c0.w;
+        e.y = e.y + (c0.y * w + c1.y * u + c2.y * v) * c1.w;
+        e.z = e.z + (c0.z * w + c1.z * u + c2.z * v) * c2.w;
+    }
+    return e;
 }
 
 // asin in binary32 by Abramowitz & Stegun 4.4.46 (|error| <= 2e-8 in exact arithmetic):
@@ -355,8 +482,8 @@ __device__ inline float3 layer_dir(int i, const float3& t, const float3& b, cons
 // ENV: hemisphere rays that leave the scene sample the environment (a separate
 // instantiation, so that scenes without one keep the smaller register footprint).
 template <int STACK, bool STATS, int G = 64, bool LAYERS = false, bool ENV = false>
-__device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, int32_t* mystk, uint32_t* lq, int stride,
-                             float* subc = nullptr)
+__device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, int32_t* mystk, uint32_t* lq, uint16_t* lqj,
+                             int stride, float* subc = nullptr)
 {
     // one texel per wave: the frames of its first subtexels are kept in the wave's LDS
     // (when the caller gives one)
@@ -388,6 +515,7 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
     for (uint32_t j = 0; j < m; j++) area_max += ts[j].area;
     const float near_t = 1000 * A.tmin;
     const float rug_below = __int_as_float(__float_as_int(A.rug) - 1);   // largest float < rugDistance (> 0)
+    const uint32_t n1 = first_series(A);
     if (kCache) {
         __builtin_amdgcn_wave_barrier();
         if (lane < min(m, kSubCache)) subtexel_frame(A, ts[lane], subc + kFrame * lane);
@@ -416,17 +544,19 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
         const bool allL = !shootH || rounds >= hemi_rays;
         uint32_t cRelH = 0, cRelL = 0, cUnrelL = 0, cRounds = 0;      // this lane's counts in the series
         uint32_t* const lq0 = lq - lane;                                // the group's queue
+        uint16_t* const lqj0 = lqj - lane;                              // and the queued shooters' subtexels
         uint32_t qhead = 0, qtail = 0;                                  // group-uniform
         // the shadow rays of queued light shooters qhead .. qhead+cnt-1, one per lane
         auto light_batch = [&](uint32_t cnt) {
             __builtin_amdgcn_wave_barrier();
             if (lane < cnt) {
                 const uint32_t e = qhead + lane;
-                const uint32_t s = lq0[(e & (uint32_t)(G - 1)) + ((e / (uint32_t)G) & 1u) * stride];
+                const uint32_t qi = (e & (uint32_t)(G - 1)) + ((e / (uint32_t)G) & 1u) * stride;
+                const uint32_t s = lq0[qi], sj = lqj0[qi];
                 float3 o; float nx, ny, nz; uint32_t tri;
                 float3 tg, bt;
-                if (kCache) shooter_cached(A, ts, m, area_max, n, s, key, shooter_base + s, subc, o, nx, ny, nz, tri);
-                else shooter_at<LAYERS>(A, ts, m, area_max, n, s, key, shooter_base + s, o, nx, ny, nz, tri, &tg, &bt);
+                if (kCache) shooter_cached(A, ts, sj, key, shooter_base + s, subc, o, nx, ny, nz, tri);
+                else shooter_at<LAYERS>(A, ts, sj, key, shooter_base + s, o, nx, ny, nz, tri, &tg, &bt);
                 float3 ld[LAYERS ? 4 : 1];
                 if (dirs) {
 #pragma unroll
@@ -524,11 +654,20 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
                     got = need;
                 }
             }
+            // the chunk's subtexels (every shooter steps the walk, shooting or not): from
+            // the first series' record, else by replaying the walk
+            uint32_t myj = 0;
+            if (m > 1) {
+                if (A.walk && shooter_base == 0 && walk_has(A.walk + ti)) myj = walk_pick(A.walk + ti, m, s);
+                else myj = walk_replay(ts, m, area_max, n1, n, shooter_base + base, min((uint32_t)G, n - base), lane);
+            }
             {
                 const uint64_t lm = (__ballot(doL) >> gbase) & gmask;
                 if (doL) {
                     const uint32_t e = qtail + (uint32_t)__popcll(lm & lanemask_lt(lane));
-                    lq0[(e & (uint32_t)(G - 1)) + ((e / (uint32_t)G) & 1u) * stride] = s;
+                    const uint32_t qi = (e & (uint32_t)(G - 1)) + ((e / (uint32_t)G) & 1u) * stride;
+                    lq0[qi] = s;
+                    lqj0[qi] = (uint16_t)myj;
                 }
                 qtail += (uint32_t)__popcll(lm);
             }
@@ -536,9 +675,9 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
                 float3 o; float nx, ny, nz; uint32_t tri;
                 float3 tg, bt;
                 float tx, ty, tz, bx, by, bz;
-                uint32_t sj = kSubCache;
-                if (kCache) sj = shooter_cached(A, ts, m, area_max, n, s, key, shooter_base + s, subc, o, nx, ny, nz, tri);
-                else shooter_at<LAYERS>(A, ts, m, area_max, n, s, key, shooter_base + s, o, nx, ny, nz, tri, &tg, &bt);
+                if (kCache) shooter_cached(A, ts, myj, key, shooter_base + s, subc, o, nx, ny, nz, tri);
+                else shooter_at<LAYERS>(A, ts, myj, key, shooter_base + s, o, nx, ny, nz, tri, &tg, &bt);
+                const uint32_t sj = kCache ? myj : kSubCache;
                 if (kCache && sj < kSubCache) {
                     const float* f = subc + kFrame * sj;
                     tx = f[3]; ty = f[4]; tz = f[5]; bx = f[6]; by = f[7]; bz = f[8];
@@ -585,7 +724,7 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
                         hr += e.x; hg += e.y; hb += e.z;
                     }
                 };
-                if (r.tri >= 0) add(A.sc.exitance[2 * (size_t)r.tri + (r.front ? 0 : 1)]);
+                if (r.tri >= 0) add(hit_exitance(A.sc, r.tri, r.front, o, d, r.t));
                 else if constexpr (ENV) {
                     const float4 v = env_at(A.sc, d);
                     add(make_float4(v.x * A.env_mult, v.y * A.env_mult, v.z * A.env_mult, 0.f));
@@ -672,7 +811,8 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
 constexpr uint32_t kGatherRefill = 16;
 
 template <int STACK, bool ENV>
-__device__ void gather_texel_refill(const GatherArgs& A, uint32_t ti, uint32_t lane, int32_t* mystk, uint32_t* lq, int stride, float* subc)
+__device__ void gather_texel_refill(const GatherArgs& A, uint32_t ti, uint32_t lane, int32_t* mystk, uint32_t* lq, uint16_t* lqj,
+                                    int stride, float* subc)
 {
     const lsg_texel T = A.texels[ti];
     const uint32_t m = T.sub_count_flags & 0xffffu, flags = (T.sub_count_flags >> 16) & 0xffu;
@@ -692,6 +832,7 @@ __device__ void gather_texel_refill(const GatherArgs& A, uint32_t ti, uint32_t l
     for (uint32_t j = 0; j < m; j++) area_max += ts[j].area;
     const float near_t = 1000 * A.tmin;
     const float rug_below = __int_as_float(__float_as_int(A.rug) - 1);   // largest float < rugDistance (> 0)
+    const uint32_t n1 = first_series(A);
     // the frames of the texel's first subtexels, once per texel, in the wave's LDS
     __builtin_amdgcn_wave_barrier();
     if (lane < min(m, kSubCache)) subtexel_frame(A, ts[lane], subc + kFrame * lane);
@@ -710,14 +851,16 @@ __device__ void gather_texel_refill(const GatherArgs& A, uint32_t ti, uint32_t l
         const bool allL = !shootH || rounds >= hemi_rays;
         uint32_t cRelH = 0, cRelL = 0, cUnrelL = 0, cRounds = 0;
         uint32_t* const lq0 = lq - lane;
+        uint16_t* const lqj0 = lqj - lane;
         uint32_t qhead = 0, qtail = 0;             // wave-uniform
         auto light_batch = [&](uint32_t cnt) {
             __builtin_amdgcn_wave_barrier();
             if (lane < cnt) {
                 const uint32_t e = qhead + lane;
-                const uint32_t s = lq0[(e & 63u) + ((e / 64u) & 1u) * stride];
+                const uint32_t qi = (e & 63u) + ((e / 64u) & 1u) * stride;
+                const uint32_t s = lq0[qi], sj = lqj0[qi];
                 float3 o; float nx, ny, nz; uint32_t tri;
-                shooter_cached(A, ts, m, area_max, n, s, key, shooter_base + s, subc, o, nx, ny, nz, tri);
+                shooter_cached(A, ts, sj, key, shooter_base + s, subc, o, nx, ny, nz, tri);
                 GatherFilter flt{A.sc.verts, A.sc.idx, A.sc.side, tri, near_t, 1, rug_below};
                 for (uint32_t li = 0; li < nl; li++) {
                     const lsg_light L = A.lights[li];
@@ -797,18 +940,28 @@ __device__ void gather_texel_refill(const GatherArgs& A, uint32_t ti, uint32_t l
                         got = need;
                     }
                 }
+                // the handed-out shooters' subtexels: from the first series' record, else
+                // by replaying the walk
+                uint32_t myj = 0;
+                if (m > 1) {
+                    if (A.walk && shooter_base == 0 && walk_has(A.walk + ti)) myj = walk_pick(A.walk + ti, m, s);
+                    else myj = walk_replay(ts, m, area_max, n1, n, shooter_base + next, take, mine ? rank : 64u);
+                }
                 {
                     const uint64_t lm = __ballot(doL);
                     if (doL) {
                         const uint32_t e = qtail + (uint32_t)__popcll(lm & lanemask_lt(lane));
-                        lq0[(e & 63u) + ((e / 64u) & 1u) * stride] = s;
+                        const uint32_t qi = (e & 63u) + ((e / 64u) & 1u) * stride;
+                        lq0[qi] = s;
+                        lqj0[qi] = (uint16_t)myj;
                     }
                     qtail += (uint32_t)__popcll(lm);
                 }
                 next += take;
                 if (doH) {
                     float3 o; float nx, ny, nz; uint32_t tri;
-                    const uint32_t sj = shooter_cached(A, ts, m, area_max, n, s, key, shooter_base + s, subc, o, nx, ny, nz, tri);
+                    shooter_cached(A, ts, myj, key, shooter_base + s, subc, o, nx, ny, nz, tri);
+                    const uint32_t sj = myj;
                     float tx, ty, tz, bx, by, bz;
                     if (sj < kSubCache) {
                         const float* fr = subc + kFrame * sj;
@@ -846,7 +999,7 @@ __device__ void gather_texel_refill(const GatherArgs& A, uint32_t ti, uint32_t l
                     // a hit reads the exitance of the side it sees; a ray that leaves the
                     // scene returns environment * multiplier, or nothing
                     if (S.bi >= 0) {
-                        const float4 e = A.sc.exitance[2 * (size_t)S.bi + (S.bf ? 0 : 1)];
+                        const float4 e = hit_exitance(A.sc, S.bi, S.bf, S.o, S.d, S.best);
                         hr += e.x; hg += e.y; hb += e.z;
                     } else if constexpr (ENV) {
                         const float4 v = env_at(A.sc, S.d);
@@ -914,6 +1067,8 @@ __device__ void gather_texel_lane(const GatherArgs& A, uint32_t ti, int32_t* stk
     const float rug_below = __int_as_float(__float_as_int(A.rug) - 1);   // largest float < rugDistance (> 0)
     float hr = 0.f, hg = 0.f, hb = 0.f, lr = 0.f, lg = 0.f, lb = 0.f;
     uint32_t relH = 0, relL = 0, unrelL = 0, shotRounds = 0, shooter_base = 0;
+    SubWalk wk;                                // the lane walks its own texel's shooters in order
+    wk.init(ts);
     for (;;) {
         const bool shootH = (relH < hemi_rays || relH * 10 < hemi_rays) && hemi_rays > 0;
         const bool shootL = (relL + unrelL < light_rays || relL * 10 < light_rays) && unrelL < light_rays * 100;
@@ -922,12 +1077,14 @@ __device__ void gather_texel_lane(const GatherArgs& A, uint32_t ti, int32_t* stk
         const uint32_t n = nH > nL ? nH : nL;
         const bool allH = !shootL || hemi_rays >= rounds;
         const bool allL = !shootH || rounds >= hemi_rays;
+        const float astep = walk_step(area_max, n);
         for (uint32_t s = 0; s < n; s++) {
+            const uint32_t j = m > 1 ? wk.next(ts, m, astep) : 0u;      // every shooter steps the walk
             const bool doH = shootH && (allH || shots_before(s + 1, n, hemi_rays) > shots_before(s, n, hemi_rays));
             const bool doL = shootL && (allL || shots_before(s + 1, n, rounds) > shots_before(s, n, rounds));
             if (!doH && !doL) continue;
             float3 o; float nx, ny, nz; uint32_t tri;
-            shooter_at(A, ts, m, area_max, n, s, key, shooter_base + s, o, nx, ny, nz, tri);
+            shooter_at(A, ts, j, key, shooter_base + s, o, nx, ny, nz, tri);
             if (doH) {
                 float ox, oy, oz;                                               // RRMathPrivate.h:48-53
                 if (fabsf(nx) > fabsf(nz)) { ox = -ny; oy = nx; oz = 0.f; } else { ox = 0.f; oy = -nz; oz = ny; }
@@ -949,7 +1106,7 @@ __device__ void gather_texel_lane(const GatherArgs& A, uint32_t ti, int32_t* stk
                 // from 109 to 122 VGPRs and the 10M-triangle city's cheap gathers slowed)
                 trace_closest<STACK, false, GatherFilter, true, false>(flt, A.sc.col, o, d, A.tmin, A.locality, stk, stride, r, nv_, nt_);
                 if (r.tri >= 0) {
-                    const float4 e = A.sc.exitance[2 * (size_t)r.tri + (r.front ? 0 : 1)];
+                    const float4 e = hit_exitance(A.sc, r.tri, r.front, o, d, r.t);
                     hr += e.x; hg += e.y; hb += e.z;
                 } else if constexpr (ENV) {
                     const float4 v = env_at(A.sc, d);
@@ -1033,12 +1190,14 @@ __device__ inline void gather_block(const GatherArgs& A)
 {
     __shared__ int32_t stk[STACK * 256];
     __shared__ uint32_t lq[2 * 256];
+    __shared__ uint16_t lqj[2 * 256];
     __shared__ float subc[4 * kFrame * kSubCache];
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     // wave-uniform: kept in a scalar register, the texel record is read by scalar loads
     const uint32_t ti = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4u + (tid >> 6)));
     if (ti >= A.ntex) return;                 // whole wavefront leaves together
-    gather_texel<STACK, STATS, 64, LAYERS, ENV>(A, ti, lane, stk + tid, lq + tid, 256, subc + (tid >> 6) * kFrame * kSubCache);
+    gather_texel<STACK, STATS, 64, LAYERS, ENV>(A, ti, lane, stk + tid, lq + tid, lqj + tid, 256,
+                                                subc + (tid >> 6) * kFrame * kSubCache);
 }
 
 template <int STACK, bool STATS, bool ENV = false>
@@ -1059,10 +1218,11 @@ __device__ inline void gather_groups(const GatherArgs& A)
 {
     __shared__ int32_t stk[STACK * 256];
     __shared__ uint32_t lq[2 * 256];
+    __shared__ uint16_t lqj[2 * 256];
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const uint32_t ti = blockIdx.x * (256u / G) + tid / G;
     if (ti >= A.ntex) return;                 // the whole group leaves together
-    gather_texel<STACK, false, G, LAYERS, ENV>(A, ti, lane, stk + tid, lq + tid, 256);
+    gather_texel<STACK, false, G, LAYERS, ENV>(A, ti, lane, stk + tid, lq + tid, lqj + tid, 256);
 }
 
 template <int STACK, int G, bool ENV = false>
@@ -1089,6 +1249,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4)))
 {
     __shared__ int32_t stk[STACK * 64 * WAVES];
     __shared__ uint32_t lq[2 * 64 * WAVES];
+    __shared__ uint16_t lqj[2 * 64 * WAVES];
     __shared__ float subc[WAVES * kFrame * kSubCache];
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const uint32_t home = Q.nqueues > 1 ? blockIdx.x % Q.nqueues : 0;
@@ -1104,9 +1265,9 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4)))
             if (tile >= t1) break;           // this queue is exhausted for every lane of the wave
             const uint32_t a = tile * Q.tile, b = min(A.ntex, a + Q.tile);
             for (uint32_t ti = a; ti < b; ti++)
-                if constexpr (REFILL) gather_texel_refill<STACK, ENV>(A, ti, lane, stk + tid, lq + tid, 64 * WAVES,
+                if constexpr (REFILL) gather_texel_refill<STACK, ENV>(A, ti, lane, stk + tid, lq + tid, lqj + tid, 64 * WAVES,
                                                                      subc + (tid >> 6) * kFrame * kSubCache);
-                else gather_texel<STACK, false, 64, false, ENV>(A, ti, lane, stk + tid, lq + tid, 64 * WAVES,
+                else gather_texel<STACK, false, 64, false, ENV>(A, ti, lane, stk + tid, lq + tid, lqj + tid, 64 * WAVES,
                                                                 subc + (tid >> 6) * kFrame * kSubCache);
         }
     }
@@ -1178,6 +1339,21 @@ int gather_args(lsg_scene* s, const lsg_texel* texels, uint32_t ntexels, const l
     A.tmin = p->ray_length_min; A.locality = p->locality; A.rug = p->rug_distance;
     A.light_mult = p->light_multiplier; A.seed = p->seed;
     A.lights = s->impl.d_lights; A.nlights = s->impl.nlights;
+    A.sc.ind = s->impl.d_ind; A.sc.slot = C.d_slot;
+    A.walk = nullptr; A.walk_buf = nullptr; A.walk_cap = 0;
+    return LSG_OK;
+}
+
+int scene_walk_scratch(lsg_scene* s, uint32_t n, GatherArgs& A)
+{
+    SceneImpl& S = s->impl;
+    if (n > S.walk_cap) {
+        dfree(S.d_walk);
+        S.d_walk = nullptr; S.walk_cap = 0;
+        LSG_HIP(hipMalloc(&S.d_walk, 2 * sizeof(uint4) * (size_t)n));
+        S.walk_cap = n;
+    }
+    A.walk_buf = S.d_walk; A.walk_cap = S.walk_cap;
     return LSG_OK;
 }
 
@@ -1201,14 +1377,19 @@ int launch_gather_persistent(const GatherArgs& A, int stack, const WorkQueue& Q,
                              bool refill, hipStream_t st)
 {
     k_queue_reset<<<1, 64, 0, st>>>(Q.counters, Q.nqueues);
+    GatherArgs B = A;
+    if (B.walk_buf && B.ntex <= B.walk_cap && B.ntex) {
+        k_subtexel_walk<<<(B.ntex + 255) / 256, 256, 0, st>>>(B.texels, B.ntex, B.subs, first_series(B), (WalkRec*)B.walk_buf);
+        B.walk = (const WalkRec*)B.walk_buf;
+    }
     // shooters per series above 32: the texel's lanes may take shooters as they free up
-    refill = refill && std::max(A.hemi_rays, A.rounds) > 32u;
-    if (A.sc.env_type) {
-        if (refill) launch_persistent<true, true>(A, stack, Q, waves_per_block, blocks, st);
-        else launch_persistent<true, false>(A, stack, Q, waves_per_block, blocks, st);
+    refill = refill && std::max(B.hemi_rays, B.rounds) > 32u;
+    if (B.sc.env_type) {
+        if (refill) launch_persistent<true, true>(B, stack, Q, waves_per_block, blocks, st);
+        else launch_persistent<true, false>(B, stack, Q, waves_per_block, blocks, st);
     } else {
-        if (refill) launch_persistent<false, true>(A, stack, Q, waves_per_block, blocks, st);
-        else launch_persistent<false, false>(A, stack, Q, waves_per_block, blocks, st);
+        if (refill) launch_persistent<false, true>(B, stack, Q, waves_per_block, blocks, st);
+        else launch_persistent<false, false>(B, stack, Q, waves_per_block, blocks, st);
     }
     LSG_HIP(hipGetLastError());
     return LSG_OK;
@@ -1222,10 +1403,15 @@ static int launch_gather_one(const GatherArgs& A, int stack, hipStream_t st);
 // threads per texel stay below 2^31 threads per launch)
 constexpr uint32_t kLaunchTexels = 1u << 22;
 
-int launch_gather(const GatherArgs& A, int stack, hipStream_t st)
+uint32_t gather_chunk(const GatherArgs& A)
 {
     const uint32_t series = std::max(std::max(A.hemi_rays, A.rounds), 1u);
-    const uint32_t chunk = (uint32_t)std::min<uint64_t>((uint64_t)kLaunchTexels * std::max(1u, 1024u / series), 1u << 25);
+    return (uint32_t)std::min<uint64_t>((uint64_t)kLaunchTexels * std::max(1u, 1024u / series), 1u << 25);
+}
+
+int launch_gather(const GatherArgs& A, int stack, hipStream_t st)
+{
+    const uint32_t chunk = gather_chunk(A);
     for (uint64_t base = 0; base < A.ntex; base += chunk) {
         GatherArgs B = A;
         B.texels = A.texels + base;
@@ -1253,20 +1439,38 @@ static uint32_t device_cus()
     return c;
 }
 
+// one texel per lane makes each wave G times longer: worth it only when the launch
+// fills every wave slot (4 per SIMD, 4 SIMDs per CU) many times over, else the last waves
+// run on an almost idle chip (733k texels: 20% slower, 11.5M texels: 13% faster than groups)
+static bool use_lanes(const GatherArgs& A, uint32_t ntex)
+{
+    const uint32_t series = std::max(A.hemi_rays, A.rounds);
+    return !A.layered && series <= 32 && (uint64_t)ntex >= (uint64_t)kLaneRounds * 64u * 16u * device_cus();
+}
+
+// the walk records the kernels of one launch read (none for the one-texel-per-lane kernel,
+// whose lanes walk their own texels)
+int walk_records(lsg_scene* s, GatherArgs& A)
+{
+    const uint32_t n = std::min(A.ntex, gather_chunk(A));
+    if (!n || use_lanes(A, n)) return LSG_OK;
+    return scene_walk_scratch(s, n, A);
+}
+
 template <bool LAYERS, bool ENV>
-static void launch_gather_kernels(const GatherArgs& A, int stack, hipStream_t st)
+static void launch_gather_kernels(GatherArgs A, int stack, hipStream_t st)
 {
     // shooters per series = max(hemisphere rays, light rounds); a group of G >= that
     // many lanes does the same work as a whole wave with idle lanes
     const uint32_t series = std::max(A.hemi_rays, A.rounds);
+    const bool lanes = !LAYERS && use_lanes(A, A.ntex);
+    if (!lanes && A.walk_buf && A.ntex <= A.walk_cap) {
+        k_subtexel_walk<<<(A.ntex + 255) / 256, 256, 0, st>>>(A.texels, A.ntex, A.subs, first_series(A), (WalkRec*)A.walk_buf);
+        A.walk = (const WalkRec*)A.walk_buf;
+    }
     if (series <= 32) {
         const uint32_t G = series <= 16 ? 16u : 32u;
         const unsigned grid = (unsigned)(((uint64_t)A.ntex * G + 255) / 256);
-        // one texel per lane makes each wave G times longer: worth it only when the
-        // launch fills every wave slot (4 per SIMD, 4 SIMDs per CU) many times over,
-        // else the last waves run on an almost idle chip (733k texels: 20% slower,
-        // 11.5M texels: 13% faster than groups)
-        const bool lanes = !LAYERS && (uint64_t)A.ntex >= (uint64_t)kLaneRounds * 64u * 16u * device_cus();
         if constexpr (LAYERS) {
             if (G == 16) {
                 if (stack <= 32) k_gather_grouped_layers<32, 16, ENV><<<grid, 256, 0, st>>>(A);
@@ -1332,6 +1536,37 @@ int lsg_scene_get_exitance(const lsg_scene* s, float* exitance)
     return LSG_OK;
 }
 
+int lsg_scene_set_indirect(lsg_scene* s, const float* corners, const float* albedo, const uint8_t* sides)
+{
+    if (!s) return fail(LSG_ERR_INVALID, "lsg_scene_set_indirect: null scene");
+    SceneImpl& S = s->impl;
+    const uint32_t nt = S.col->impl.ntri;
+    std::vector<uint8_t> side(S.h_side);
+    for (uint32_t t = 0; t < nt; t++) side[t] &= (uint8_t)~(LSG_SIDE_FRONT_INDIRECT | LSG_SIDE_BACK_INDIRECT);
+    if (!corners) {
+        dfree(S.d_ind); S.d_ind = nullptr;
+        if (nt) LSG_HIP(hipMemcpy(S.d_side, side.data(), nt, hipMemcpyHostToDevice));
+        S.h_side = side;
+        return LSG_OK;
+    }
+    if (!albedo) return fail(LSG_ERR_INVALID, "lsg_scene_set_indirect: corners without albedo");
+    std::vector<float4> rec(3 * (size_t)nt);
+    for (uint32_t t = 0; t < nt; t++) {
+        for (int k = 0; k < 3; k++)
+            rec[3 * (size_t)t + k] = make_float4(corners[9 * (size_t)t + 3 * k], corners[9 * (size_t)t + 3 * k + 1],
+                                                 corners[9 * (size_t)t + 3 * k + 2], albedo[3 * (size_t)t + k]);
+        const uint8_t b = sides ? sides[t] : 3u;
+        side[t] |= (uint8_t)(((b & 1u) ? LSG_SIDE_FRONT_INDIRECT : 0u) | ((b & 2u) ? LSG_SIDE_BACK_INDIRECT : 0u));
+    }
+    if (!S.d_ind) LSG_HIP(hipMalloc(&S.d_ind, sizeof(float4) * 3 * (size_t)(nt ? nt : 1)));
+    if (nt) {
+        LSG_HIP(hipMemcpy(S.d_ind, rec.data(), sizeof(float4) * 3 * (size_t)nt, hipMemcpyHostToDevice));
+        LSG_HIP(hipMemcpy(S.d_side, side.data(), nt, hipMemcpyHostToDevice));
+    }
+    S.h_side = side;
+    return LSG_OK;
+}
+
 int lsg_scene_set_lights(lsg_scene* s, const lsg_light* lights, uint32_t n)
 {
     if (!s || (n && !lights)) return fail(LSG_ERR_INVALID, "lsg_scene_set_lights: null argument");
@@ -1362,6 +1597,7 @@ int lsg_scene_create(const lsg_collider* col, const float* vertex_normals, const
     if ((e = hipMalloc(&S.d_exit, 32 * (size_t)(nt ? nt : 1))) != hipSuccess) return bail(hip_fail(e, "hipMalloc", __LINE__));
     if ((e = hipMemcpy(S.d_vnormal, vertex_normals, 12 * (size_t)nv, hipMemcpyHostToDevice)) != hipSuccess) return bail(hip_fail(e, "h2d", __LINE__));
     if ((e = hipMemcpy(S.d_side, side_bits, (size_t)nt, hipMemcpyHostToDevice)) != hipSuccess) return bail(hip_fail(e, "h2d", __LINE__));
+    S.h_side.assign(side_bits, side_bits + nt);
     if ((e = hipMemcpy(S.d_exit, exitance, 32 * (size_t)nt, hipMemcpyHostToDevice)) != hipSuccess) return bail(hip_fail(e, "h2d", __LINE__));
     int st = lsg_scene_set_lights(s, lights, nlights);
     if (st) return bail(st);
@@ -1374,6 +1610,7 @@ void lsg_scene_destroy(lsg_scene* s)
     if (!s) return;
     dfree(s->impl.d_vnormal); dfree(s->impl.d_side); dfree(s->impl.d_exit); dfree(s->impl.d_lights);
     dfree(s->impl.d_vtan); dfree(s->impl.d_vbit); dfree(s->impl.d_env); dfree(s->impl.d_lmaps);
+    dfree(s->impl.d_ind); dfree(s->impl.d_walk);
     delete s;
 }
 
@@ -1441,6 +1678,7 @@ int lsg_gather_layers_device(lsg_scene* s, const lsg_texel* texels, uint32_t nte
     for (int i = 0; i < 5; i++) A.layer[i] = (float4*)layers[i];
     A.dirs = (layers[1] || layers[2] || layers[3]) ? 1u : 0u;       // lightmap.cpp:820
     A.layered = 1;
+    if ((e = walk_records(s, A))) return e;
     return launch_gather(A, stack, st);
 }
 
@@ -1454,10 +1692,16 @@ static int gather_launch(lsg_scene* s, const lsg_texel* texels, uint32_t ntexels
     int stack = 0;
     int st = gather_args(s, texels, ntexels, subs, p, out_rgba, A, stack);
     if (st) return st;
-    if (!stats) return launch_gather(A, stack, (hipStream_t)stream);
+    if (!stats) {
+        if ((st = walk_records(s, A))) return st;
+        return launch_gather(A, stack, (hipStream_t)stream);
+    }
     A.stats = stats;
     const unsigned grid = (ntexels + 3) / 4;
     hipStream_t hs = (hipStream_t)stream;
+    if ((st = scene_walk_scratch(s, ntexels, A))) return st;
+    k_subtexel_walk<<<(ntexels + 255) / 256, 256, 0, hs>>>(texels, ntexels, subs, first_series(A), (WalkRec*)A.walk_buf);
+    A.walk = (const WalkRec*)A.walk_buf;
     if (A.sc.env_type) {
         if (stack <= 32) k_gather<32, true, true><<<grid, 256, 0, hs>>>(A);
         else k_gather<64, true, true><<<grid, 256, 0, hs>>>(A);

@@ -125,6 +125,7 @@ struct ColliderImpl {
     float* d_verts = nullptr; uint32_t* d_idx = nullptr;
     Node* d_nodes = nullptr;
     float4* d_plane = nullptr; TriRest* d_rest = nullptr;
+    uint32_t* d_slot = nullptr;   // leaf-order slot of each original triangle
     uint64_t bytes = 0;
     double build_ms = 0;
     ColliderView view() const

@@ -327,6 +327,7 @@ int lsg_gather_parts_device(lsg_scene* s, const lsg_texel* texels, uint32_t ntex
     int st = gather_args(s, texels, ntexels, subs, p, out_rgba, A, stack);
     if (st) return st;
     A.parts = (float4*)parts;
+    if ((st = walk_records(s, A))) return st;
     return launch_gather(A, stack, (hipStream_t)stream);
 }
 

@@ -103,6 +103,7 @@ SIGNATURES = {
     "lsg_scene_get_exitance": (ctypes.c_int, [P, P]),
     "lsg_scene_exitance_device": (ctypes.c_int, [P, P, P, P, P]),
     "lsg_scene_set_lights": (ctypes.c_int, [P, P, U32]),
+    "lsg_scene_set_indirect": (ctypes.c_int, [P, P, P, P]),
     "lsg_gather_device": (ctypes.c_int, [P, P, U32, P, U32, ctypes.POINTER(GatherParams), P, P]),
     "lsg_gather_stats": (ctypes.c_int, [P, P, U32, P, U32, ctypes.POINTER(GatherParams), P, ctypes.POINTER(ctypes.c_double)]),
     "lsg_scene_set_tangents": (ctypes.c_int, [P, P, P]),

@@ -12,9 +12,14 @@
  * The reference draws texel jitter and filler seeds from the global C rand()
  * (gather.h:102, gather.cpp:742-749), which is not reproducible across threads.
  * This statement replaces exactly those draws with a seeded counter hash
- * (lowbias32, C. Wellons) and picks the shooter's subtexel by a stratified
- * closed form instead of the running area walk (gather.cpp:679-688); liblsgpu
- * uses the identical replacements, so both sides shoot the same rays.
+ * (lowbias32, C. Wellons); liblsgpu uses the identical replacement, so both sides
+ * shoot the same rays.  The shooter's subtexel follows the reference's own area walk
+ * (gather.cpp:645-647, 669, 680-688), carried from series to series.
+ *
+ * A hemisphere hit reads the flat exitance of the side it sees (direct light times
+ * albedo plus emittance, pathtracer.cpp:235-262, 306-316) and, where the scene gives
+ * per-corner indirect irradiance, adds the albedo times its interpolation at the hit's
+ * hitPoint2d (pathtracer.cpp:326-348, Triangle::getPointMeasure rrcore.cpp:262-273).
  *
  * Geometry (ray origins, directions, hit tests) is evaluated in binary32 in the
  * reference's operation order, so hit decisions agree bit for bit with the GPU;
@@ -79,14 +84,42 @@ void lsgo_jitter(uint32_t key, uint32_t i, float* u, float* v)
     *v = (float)b / (float)M;
 }
 
-/* stratified subtexel choice for shooter s of n (replaces gather.cpp:679-688) */
-uint32_t lsgo_pick_subtexel(const float* subs, uint32_t count, float area_max, uint32_t s, uint32_t n)
+/* The subtexel selector of processTexel (gather.cpp:645-647, 669, 680-688): an iterator
+ * over the texel's subtexels starting at the first with an area accumulator of minus its
+ * area; each series of n shooters steps by areaMax/(n+0.91), and each shooter adds the
+ * step, then moves the iterator on (cyclically) and subtracts the new subtexel's area
+ * while the accumulator is positive.  Both are carried from series to series.
+ * areas: the subtexels' areaInMapSpace, `stride` floats apart. */
+typedef struct { float accu; uint32_t it; } walk_t;
+
+static void walk_init(walk_t* w, const float* areas) { w->it = 0; w->accu = -areas[0]; }
+
+static uint32_t walk_next(walk_t* w, const float* areas, uint32_t stride, uint32_t m, float step)
 {
-    float pos = ((float)s + 0.5f) * area_max / (float)n;
-    uint32_t j = 0;
-    float acc = subs[7];
-    while (j + 1 < count && acc <= pos) { j++; acc += subs[8 * j + 7]; }
-    return j;
+    w->accu += step;
+    while (w->accu > 0) {
+        w->it = w->it + 1 == m ? 0 : w->it + 1;
+        w->accu -= areas[stride * w->it];
+    }
+    return w->it;
+}
+
+static float walk_step(float area_max, uint32_t n) { return area_max / ((float)n + 0.91f); }
+
+/* the subtexel of every shooter of nseries series of series_n[k] shooters (out: their
+ * sum of entries); areas: m floats */
+void lsgo_subtexel_walk(const float* areas, uint32_t m, const uint32_t* series_n, uint32_t nseries, uint32_t* out)
+{
+    if (!m) return;
+    float area_max = 0;
+    for (uint32_t j = 0; j < m; j++) area_max += areas[j];
+    walk_t w;
+    walk_init(&w, areas);
+    size_t o = 0;
+    for (uint32_t k = 0; k < nseries; k++) {
+        const float step = walk_step(area_max, series_n[k]);
+        for (uint32_t s = 0; s < series_n[k]; s++) out[o++] = walk_next(&w, areas, 1, m, step);
+    }
 }
 
 /* ---- ray casting with the final-gathering handler rules ------------------------ */
@@ -132,25 +165,26 @@ static inline void slab6(const float* b, const float* o, const float* idir, floa
 }
 
 static void consider(const scene_t* sc, int mode, uint32_t t, const float* o, const float* d, float lo, float tmin,
-                     uint32_t shooter, float* best, int* bi, int* bf)
+                     uint32_t shooter, float* best, int* bi, int* bf, float* buv)
 {
     float th, u, v; int f;
     if (!lsgo_srlnp_hit(&sc->tris[t], o, d, lo, *best, &th, &u, &v, &f)) return;
     if (!accept_hit(sc, mode, t, f, th, shooter, tmin)) return;
-    if (*bi < 0 || th < *best || (th == *best && (int)t < *bi)) { *best = th; *bi = (int)t; *bf = f; }
+    if (*bi < 0 || th < *best || (th == *best && (int)t < *bi)) { *best = th; *bi = (int)t; *bf = f; buv[0] = u; buv[1] = v; }
 }
 
 /* Closest accepted hit in [tmin,tmax] after the mesh-box pretest
  * (IntersectBspFast.cpp:816).  Returns tri or -1. */
 static int trace(const scene_t* sc, int mode, const float* o, const float* d, float tmin, float tmax,
-                 uint32_t shooter, float* t_out, int* front_out)
+                 uint32_t shooter, float* t_out, int* front_out, float* uv_out)
 {
     float lo, hi;
     if (!lsgo_box_intersect(sc->bmn, sc->bmx, o, d, tmin, tmax, &lo, &hi)) return -1;
     float best = hi; int bi = -1, bf = 0;
+    float buv[2] = {0, 0};
     if (!sc->nodes) {
         for (uint32_t t = 0; t < sc->ntri; t++)
-            if (sc->tris[t].byte <= 8) consider(sc, mode, t, o, d, lo, tmin, shooter, &best, &bi, &bf);
+            if (sc->tris[t].byte <= 8) consider(sc, mode, t, o, d, lo, tmin, shooter, &best, &bi, &bf, buv);
     } else if (sc->nprim) {
         float idir[3];
         for (int j = 0; j < 3; j++) { float dj = d[j]; if (fabsf(dj) < 1e-30f) dj = copysignf(1e-30f, dj); idir[j] = 1.0f / dj; }
@@ -159,7 +193,7 @@ static int trace(const scene_t* sc, int mode, const float* o, const float* d, fl
             int32_t ref = stack[--sp];
             if (ref < 0) {
                 uint32_t st = ((uint32_t)ref >> 4) & 0x7ffffffu, cnt = ((uint32_t)ref & 15u) + 1;
-                for (uint32_t k = 0; k < cnt; k++) consider(sc, mode, sc->prim[st + k], o, d, lo, tmin, shooter, &best, &bi, &bf);
+                for (uint32_t k = 0; k < cnt; k++) consider(sc, mode, sc->prim[st + k], o, d, lo, tmin, shooter, &best, &bi, &bf, buv);
                 continue;
             }
             const float* nd = sc->nodes + 16 * (size_t)ref;
@@ -173,7 +207,7 @@ static int trace(const scene_t* sc, int mode, const float* o, const float* d, fl
             else if (h1) stack[sp++] = r1;
         }
     }
-    if (bi >= 0) { *t_out = best; *front_out = bf; }
+    if (bi >= 0) { *t_out = best; *front_out = bf; if (uv_out) { uv_out[0] = buv[0]; uv_out[1] = buv[1]; } }
     return bi;
 }
 
@@ -328,7 +362,7 @@ uint32_t lsgo_shots_before(uint32_t s, uint32_t n, uint32_t r)
  * and the bent normal of gather.cpp:306, :482, :809-830. */
 static void gather_core(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_t ntri,
                         const float* vnormals, const float* vtan, const float* vbit, const uint8_t* side_mask,
-                        const float* exitance, const lsgo_light* lights, uint32_t nlights, const lsgo_gparams* p,
+                        const float* exitance, const float* ind, const lsgo_light* lights, uint32_t nlights, const lsgo_gparams* p,
                         const uint32_t* texels, uint32_t ntex, const float* subs,
                         const float* nodes, int32_t root_ref, const uint32_t* prim, uint32_t nprim,
                         const lsgo_env* env_in, double* out_rgba, double* const* layers, lsgo_gstats* stats, int threads)
@@ -375,6 +409,8 @@ static void gather_core(const float* verts, uint32_t nverts, const uint32_t* idx
         double sumH[3] = {0, 0, 0}, sumL[3] = {0, 0, 0};
         uint32_t relH = 0, relL = 0, unrelL = 0, shotRounds = 0, inside = 0;
         uint32_t shooter_base = 0;
+        walk_t wk;
+        walk_init(&wk, ts + 7);
         for (;;) {
             int shootH = (relH < hemi_rays || relH * 10 < hemi_rays) && 0 < hemi_rays * 100;
             int shootL = (relL + unrelL < light_rays || relL * 10 < light_rays) && unrelL < light_rays * 100;
@@ -382,8 +418,9 @@ static void gather_core(const float* verts, uint32_t nverts, const uint32_t* idx
             uint32_t n = (shootH ? hemi_rays : 0) > (shootL ? rounds : 0) ? (shootH ? hemi_rays : 0) : (shootL ? rounds : 0);
             uint32_t shotH = 0, shotL = 0;
             nser += 1;
+            const float astep = walk_step(area_max, n);
             for (uint32_t s = 0; s < n; s++) {
-                uint32_t j = lsgo_pick_subtexel(ts, sc_cnt, area_max, s, n);
+                uint32_t j = walk_next(&wk, ts + 7, 8, sc_cnt, astep);
                 const float* sub = ts + 8 * j;
                 uint32_t tri; memcpy(&tri, sub, 4);
                 const uint32_t* tidx = idx + 3 * (size_t)tri;
@@ -411,15 +448,23 @@ static void gather_core(const float* verts, uint32_t nverts, const uint32_t* idx
                     float cosa = sqrtf(1 - dist);
                     float d[3];
                     for (int c = 0; c < 3; c++) d[c] = bn[c] * cosa + bt[c] * x + bb[c] * y;
-                    float th; int fr;
-                    int hit = trace(&sc, 0, o, d, p->ray_length_min, p->locality, tri, &th, &fr);
+                    float th, huv[2]; int fr;
+                    int hit = trace(&sc, 0, o, d, p->ray_length_min, p->locality, tri, &th, &fr, huv);
                     nh += 1;
                     /* a ray that leaves the scene returns environment * multiplier
                      * (pathtracer.cpp:156-173) */
-                    float envc[3];
+                    float envc[3], hitc[3];
                     const float* e = NULL;
-                    if (hit >= 0) e = exitance + 4 * (2 * (size_t)hit + (fr ? 0 : 1));
-                    else if (env) {
+                    if (hit >= 0) {
+                        e = exitance + 4 * (2 * (size_t)hit + (fr ? 0 : 1));
+                        if (ind && (side_mask[hit] & (fr ? 8 : 16))) {
+                            /* + albedo * (I0 (1-u-v) + I1 u + I2 v), pathtracer.cpp:346, rrcore.cpp:266-268 */
+                            const float* c = ind + 12 * (size_t)hit;
+                            const float u = huv[0], v = huv[1], w = 1 - u - v;
+                            for (int k = 0; k < 3; k++) hitc[k] = e[k] + (c[k] * w + c[4 + k] * u + c[8 + k] * v) * c[4 * k + 3];
+                            e = hitc;
+                        }
+                    } else if (env) {
                         const float* t = env->rgb + 3 * (size_t)lsgo_env_index(env, d);
                         for (int c = 0; c < 3; c++) envc[c] = t[c] * p->environment_multiplier;
                         e = envc;
@@ -457,7 +502,7 @@ static void gather_core(const float* verts, uint32_t nverts, const uint32_t* idx
                         float ni = d[0] * bn[0] + d[1] * bn[1] + d[2] * bn[2];
                         if (ni <= 0 || !isfinite(ni)) { relL++; continue; }    /* gather.cpp:430-437 */
                         float th = 0; int fr;
-                        int blocked = L->cast_shadows ? (trace(&sc, 1, o, d, p->ray_length_min, dsz, tri, &th, &fr) >= 0) : 0;
+                        int blocked = L->cast_shadows ? (trace(&sc, 1, o, d, p->ray_length_min, dsz, tri, &th, &fr, NULL) >= 0) : 0;
                         ns += L->cast_shadows ? 1 : 0;
                         if (!blocked) {
                             /* RRLight.cpp:345-362 getIrradiance */
@@ -522,19 +567,21 @@ void lsgo_gather(const float* verts, uint32_t nverts, const uint32_t* idx, uint3
                  const float* nodes, int32_t root_ref, const uint32_t* prim, uint32_t nprim,
                  double* out_rgba, lsgo_gstats* stats, int threads)
 {
-    gather_core(verts, nverts, idx, ntri, vnormals, NULL, NULL, side_mask, exitance, lights, nlights, p, texels, ntex,
+    gather_core(verts, nverts, idx, ntri, vnormals, NULL, NULL, side_mask, exitance, NULL, lights, nlights, p, texels, ntex,
                 subs, nodes, root_ref, prim, nprim, NULL, out_rgba, NULL, stats, threads);
 }
 
-/* lsgo_gather with an environment (NULL = none). */
+/* lsgo_gather with an environment (NULL = none) and per-corner indirect irradiance (NULL =
+ * none): ind = 12 floats per triangle, corner k's rgb at 4k..4k+2 and albedo channel k at
+ * 4k+3, added on the sides whose side_mask bit 3 (front) / 4 (back) is set. */
 void lsgo_gather_env(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_t ntri,
-                     const float* vnormals, const uint8_t* side_mask, const float* exitance,
+                     const float* vnormals, const uint8_t* side_mask, const float* exitance, const float* ind,
                      const lsgo_light* lights, uint32_t nlights, const lsgo_gparams* p,
                      const uint32_t* texels, uint32_t ntex, const float* subs,
                      const float* nodes, int32_t root_ref, const uint32_t* prim, uint32_t nprim,
                      const lsgo_env* env, double* out_rgba, lsgo_gstats* stats, int threads)
 {
-    gather_core(verts, nverts, idx, ntri, vnormals, NULL, NULL, side_mask, exitance, lights, nlights, p, texels, ntex,
+    gather_core(verts, nverts, idx, ntri, vnormals, NULL, NULL, side_mask, exitance, ind, lights, nlights, p, texels, ntex,
                 subs, nodes, root_ref, prim, nprim, env, out_rgba, NULL, stats, threads);
 }
 
@@ -543,8 +590,8 @@ void lsgo_gather_env(const float* verts, uint32_t nverts, const uint32_t* idx, u
  * for a layer not wanted. */
 void lsgo_gather_layers(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_t ntri,
                         const float* vnormals, const float* vtan, const float* vbit, const uint8_t* side_mask,
-                        const float* exitance, const lsgo_light* lights, uint32_t nlights, const lsgo_gparams* p,
-                        const uint32_t* texels, uint32_t ntex, const float* subs,
+                        const float* exitance, const float* ind, const lsgo_light* lights, uint32_t nlights,
+                        const lsgo_gparams* p, const uint32_t* texels, uint32_t ntex, const float* subs,
                         const float* nodes, int32_t root_ref, const uint32_t* prim, uint32_t nprim,
                         const lsgo_env* env, double* const* layers, lsgo_gstats* stats, int threads)
 {
@@ -556,7 +603,7 @@ void lsgo_gather_layers(const float* verts, uint32_t nverts, const uint32_t* idx
         lsgo_default_frames(vnormals, nverts, ft, fb);
         vtan = ft; vbit = fb;
     }
-    gather_core(verts, nverts, idx, ntri, vnormals, vtan, vbit, side_mask, exitance, lights, nlights, p, texels, ntex,
+    gather_core(verts, nverts, idx, ntri, vnormals, vtan, vbit, side_mask, exitance, ind, lights, nlights, p, texels, ntex,
                 subs, nodes, root_ref, prim, nprim, env, NULL, layers, stats, threads);
     free(ft); free(fb);
 }

@@ -109,8 +109,8 @@ def lib():
         L.lsgo_texel_key.argtypes = [U, U]
         L.lsgo_texel_key.restype = U
         L.lsgo_jitter.argtypes = [U, U, P, P]
-        L.lsgo_pick_subtexel.argtypes = [P, U, ctypes.c_float, U, U]
-        L.lsgo_pick_subtexel.restype = U
+        L.lsgo_subtexel_walk.argtypes = [P, U, P, U, P]
+        L.lsgo_subtexel_walk.restype = None
         L.lsgo_shots_before.argtypes = [U, U, U]
         L.lsgo_shots_before.restype = U
         _lib = L
@@ -252,14 +252,31 @@ def gather(scene_arrays, params, texels, subs, tree=None, threads=0):
     else:
         np_, pp_, root, nprim = None, None, 0, 0
     ev, keep = _env(scene_arrays)
+    ind = _ind(scene_arrays)
     L = lib()
     P, U = ctypes.c_void_p, ctypes.c_uint32
-    L.lsgo_gather_env.argtypes = [P, U, P, U, P, P, P, P, U, P, P, U, P, P, ctypes.c_int32, P, U, P, P, P, ctypes.c_int]
+    L.lsgo_gather_env.argtypes = [P, U, P, U, P, P, P, P, P, U, P, P, U, P, P, ctypes.c_int32, P, U, P, P, P, ctypes.c_int]
     L.lsgo_gather_env.restype = None
-    L.lsgo_gather_env(_p(v), len(v), _p(i), len(i), _p(vn), _p(sb), _p(ex), _p(lights), len(lights),
-                      ctypes.byref(gp), _p(tx), len(tx), _p(sbs), np_, root, pp_, nprim,
+    L.lsgo_gather_env(_p(v), len(v), _p(i), len(i), _p(vn), _p(sb), _p(ex), _p(ind) if ind is not None else None,
+                      _p(lights), len(lights), ctypes.byref(gp), _p(tx), len(tx), _p(sbs), np_, root, pp_, nprim,
                       ctypes.byref(ev) if ev is not None else None, _p(out), ctypes.byref(gs), threads)
     return out, gs
+
+
+def _ind(scene_arrays):
+    """Per-corner indirect records [nt, 12] (corner k rgb at 4k..4k+2, albedo channel k at
+    4k+3), or None when the scene has none."""
+    ind = scene_arrays.get("ind")
+    return None if ind is None else np.ascontiguousarray(ind, dtype=np.float32).reshape(-1, 12)
+
+
+def subtexel_walk(areas, series):
+    """The subtexel of every shooter of the given series (list of shooter counts)."""
+    a = np.ascontiguousarray(areas, dtype=np.float32)
+    n = np.ascontiguousarray(series, dtype=np.uint32)
+    out = np.zeros(int(n.sum()), np.uint32)
+    lib().lsgo_subtexel_walk(_p(a), len(a), _p(n), len(n), _p(out))
+    return out
 
 
 def gather_layers(scene_arrays, params, texels, subs, layers=(0, 1, 2, 3, 4), tree=None, threads=0):
@@ -269,7 +286,7 @@ def gather_layers(scene_arrays, params, texels, subs, layers=(0, 1, 2, 3, 4), tr
     L = lib()
     if not getattr(L, "_layers_declared", False):
         P, U = ctypes.c_void_p, ctypes.c_uint32
-        L.lsgo_gather_layers.argtypes = [P, U, P, U, P, P, P, P, P, P, U, P, P, U, P, P, ctypes.c_int32, P, U, P, P, P,
+        L.lsgo_gather_layers.argtypes = [P, U, P, U, P, P, P, P, P, P, P, U, P, P, U, P, P, ctypes.c_int32, P, U, P, P, P,
                                          ctypes.c_int]
         L.lsgo_gather_layers.restype = None
         L.lsgo_default_frames.argtypes = [P, U, P, P]
@@ -299,8 +316,10 @@ def gather_layers(scene_arrays, params, texels, subs, layers=(0, 1, 2, 3, 4), tr
     else:
         np_, pp_, root, nprim = None, None, 0, 0
     ev, keep = _env(scene_arrays)
+    ind = _ind(scene_arrays)
     L.lsgo_gather_layers(_p(v), len(v), _p(i), len(i), _p(vn), _p(vt) if vt is not None else None,
-                         _p(vb) if vb is not None else None, _p(sb), _p(ex), _p(lights), len(lights), ctypes.byref(gp),
+                         _p(vb) if vb is not None else None, _p(sb), _p(ex), _p(ind) if ind is not None else None,
+                         _p(lights), len(lights), ctypes.byref(gp),
                          _p(tx), len(tx), _p(sbs), np_, root, pp_, nprim, ctypes.byref(ev) if ev is not None else None,
                          ptrs, ctypes.byref(gs), threads)
     return outs
