@@ -332,6 +332,19 @@ int  lsg_filter_plan_create(const lsg_lightmap* maps, uint32_t nmaps, lsg_filter
 void lsg_filter_plan_destroy(lsg_filter_plan* plan);
 int  lsg_filter_plan_run(lsg_filter_plan* plan, float* rgba, float* rgbf, const lsg_filter_params* params, void* stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory
+ * (tight serving loops, harnesses that forbid hipMalloc): lsg_filter_lightmaps with the
+ * same kernels and results, its scratch held in a static device array of the library
+ * (about 61 MiB on each device the library runs on, held until the process ends), so
+ * the call allocates nothing.  At most LSG_FILTER_NOALLOC_MAX_TEXELS texels over all maps
+ * and LSG_FILTER_NOALLOC_MAX_MAPS maps; larger batches fail with LSG_ERR_LIMIT.  Every
+ * call on one device shares that device's scratch and clears what must start at zero:
+ * calls that overlap on one device (on different streams or threads) race.  Synchronous. */
+#define LSG_FILTER_NOALLOC_MAX_TEXELS (1u << 20)
+#define LSG_FILTER_NOALLOC_MAX_MAPS   4096u
+int  lsg_filter_lightmaps_noalloc(const lsg_lightmap* maps, uint32_t nmaps, float* rgba, float* rgbf,
+                                  const lsg_filter_params* params, void* stream);
+
 /* Texel rasterisation of one object's lightmap, RRSolver/lightmap.cpp:83-335
  * (enumerateTexelsPartial, steps 1-2): clips every texel square against every
  * triangle of the object in its unwrap and emits SubTexels and texelFlags.

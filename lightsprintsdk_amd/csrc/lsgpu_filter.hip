@@ -11,8 +11,10 @@
 // and is not part of this kernel set; wrap=true is rejected.
 #include "lsgpu_internal.h"
 #include "lsgpu_filter.h"
-#include <vector>
+#include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <vector>
 
 namespace lsg {
 
@@ -334,6 +336,24 @@ int filter_run(const FilterMap* d_maps, uint32_t nmaps, uint32_t total, float4* 
             for (uint32_t q = 0; q < n; q++) step(q);
             k_front_changed<<<1, 64, 0, s>>>(S.changed + 1 + n % 3u, S.changed);
             if (steps_used) *steps_used = n;
+        } else if (!S.host_log) {
+            // scratch without pinned memory or events (the allocation-free entry point):
+            // each chunk of 16 steps is followed by a copy of its frontier sizes into a
+            // stack array and a stream synchronisation
+            constexpr uint32_t kChunk = 16;
+            uint32_t done = steps;
+            for (uint32_t b = 0; b < steps; b += kChunk) {
+                const uint32_t e = std::min(steps, b + kChunk);
+                for (uint32_t q = b; q < e; q++) step(q);
+                uint32_t log[kChunk];
+                LSG_HIP(hipMemcpyAsync(log, S.changed + 4, sizeof(log), hipMemcpyDeviceToHost, s));
+                LSG_HIP(hipStreamSynchronize(s));
+                bool stop = false;
+                for (uint32_t q = b; q < e; q++)
+                    if (!log[q & 15u]) { done = q + 1; stop = true; break; }
+                if (stop) break;
+            }
+            if (steps_used) *steps_used = done;
         } else {
             // a step with an empty frontier changes nothing: the reference stops there
             // (RRBuffer.cpp:948-949).  Steps go out in chunks of 16, each followed by a copy
@@ -480,5 +500,115 @@ extern "C" int lsg_filter_lightmaps(const lsg_lightmap* maps, uint32_t nmaps, fl
     if (!st) { hipError_t e = hipStreamSynchronize((hipStream_t)stream); if (e != hipSuccess) st = hip_fail(e, "sync", __LINE__); }
     scratch_free(S);
     (void)hipFree(d_maps);
+    return st;
+}
+
+// ---- allocation-free filter chain --------------------------------------------------------
+// Scratch of lsg_filter_lightmaps_noalloc: one static device array per device, laid out as
+// three float4 images, two frontier lists and the mark array (kNoAllocTexels entries each),
+// the blur flags, the step counters, then the FilterMap table and the caller's lightmap list
+// staged for the kernel that builds it.  On the host the array's name is not a device
+// address: every use goes through the per-device pointer hipGetSymbolAddress returns.
+namespace lsg {
+constexpr size_t kNoAllocTexels = LSG_FILTER_NOALLOC_MAX_TEXELS;
+constexpr size_t kNoAllocMaps = LSG_FILTER_NOALLOC_MAX_MAPS;
+constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr size_t kOffA = 0;
+constexpr size_t kOffB = kOffA + 16 * kNoAllocTexels;
+constexpr size_t kOffC = kOffB + 16 * kNoAllocTexels;
+constexpr size_t kOffFront0 = kOffC + 16 * kNoAllocTexels;
+constexpr size_t kOffFront1 = kOffFront0 + 4 * kNoAllocTexels;
+constexpr size_t kOffMark = kOffFront1 + 4 * kNoAllocTexels;
+constexpr size_t kOffFlags = kOffMark + 4 * kNoAllocTexels;
+constexpr size_t kOffChanged = align256(kOffFlags + kNoAllocTexels);
+constexpr size_t kOffMaps = align256(kOffChanged + 4 * 32);
+constexpr size_t kOffStage = align256(kOffMaps + sizeof(FilterMap) * kNoAllocMaps);
+constexpr size_t kNoAllocBytes = align256(kOffStage + sizeof(lsg_lightmap) * kNoAllocMaps);
+static_assert(kNoAllocBytes <= 72u * 1024u * 1024u, "the static filter scratch must stay modest");
+}  // namespace lsg
+
+static __device__ __align__(256) unsigned char scratch[lsg::kNoAllocBytes];
+
+namespace lsg {
+constexpr int kNoAllocDevices = 64;
+static void* g_noalloc_ptr[kNoAllocDevices];
+static std::mutex g_noalloc_mu;
+
+// the current device's copy of `scratch`, looked up once per device with it current
+static int noalloc_scratch(unsigned char** out)
+{
+    int dev = 0;
+    LSG_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocDevices) return fail(LSG_ERR_LIMIT, "device %d beyond the %d the library tracks", dev, kNoAllocDevices);
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)", __LINE__);
+        if (!p) return fail(LSG_ERR_DEVICE, "hipGetSymbolAddress(scratch) returned null");
+        g_noalloc_ptr[dev] = p;
+    }
+    *out = (unsigned char*)g_noalloc_ptr[dev];
+    return LSG_OK;
+}
+
+// FilterMap table from the caller's lightmap list: first = exclusive prefix sum of the
+// texel counts (one workgroup; each thread sums a run of 16 maps, then a scan of the runs)
+__global__ __launch_bounds__(256) void k_maps_from_lightmaps(const lsg_lightmap* __restrict__ lm, uint32_t nmaps,
+                                                             FilterMap* __restrict__ fm)
+{
+    __shared__ uint32_t run[256];
+    constexpr uint32_t kRun = (uint32_t)(kNoAllocMaps / 256);
+    const uint32_t t = threadIdx.x, b = t * kRun, e = min(nmaps, b + kRun);
+    uint32_t s = 0;
+    for (uint32_t m = b; m < e; m++) s += lm[m].width * lm[m].height;
+    run[t] = s;
+    __syncthreads();
+    for (uint32_t d = 1; d < 256; d *= 2) {
+        const uint32_t v = t >= d ? run[t - d] : 0u;
+        __syncthreads();
+        run[t] += v;
+        __syncthreads();
+    }
+    uint32_t first = run[t] - s;
+    for (uint32_t m = b; m < e; m++) {
+        const lsg_lightmap L = lm[m];
+        fm[m].first = first; fm[m].base = L.rgba_base; fm[m].out_base = L.rgbf_base;
+        fm[m].width = L.width; fm[m].height = L.height;
+        first += L.width * L.height;
+    }
+}
+static_assert(kNoAllocMaps % 256 == 0, "maps are summed in 256 runs");
+}  // namespace lsg
+
+extern "C" int lsg_filter_lightmaps_noalloc(const lsg_lightmap* maps, uint32_t nmaps, float* rgba, float* rgbf,
+                                            const lsg_filter_params* params, void* stream)
+{
+    if (!maps || !rgba || !params) return fail(LSG_ERR_INVALID, "lsg_filter_lightmaps_noalloc: null argument");
+    if (params->wrap) return fail(LSG_ERR_INVALID, "lsg_filter_lightmaps_noalloc: wrap=true is not supported");
+    if (nmaps > kNoAllocMaps) return fail(LSG_ERR_LIMIT, "lsg_filter_lightmaps_noalloc: %u lightmaps, at most %u", nmaps, (unsigned)kNoAllocMaps);
+    uint64_t total = 0;
+    for (uint32_t m = 0; m < nmaps; m++) {
+        if (!maps[m].width || !maps[m].height) return fail(LSG_ERR_INVALID, "lightmap %u has zero size", m);
+        total += (uint64_t)maps[m].width * maps[m].height;
+    }
+    if (total > kNoAllocTexels)
+        return fail(LSG_ERR_LIMIT, "lsg_filter_lightmaps_noalloc: %llu texels, at most %u", (unsigned long long)total, (unsigned)kNoAllocTexels);
+    if (!total) return LSG_OK;
+    unsigned char* base = nullptr;
+    int st = noalloc_scratch(&base);
+    if (st) return st;
+    const hipStream_t s = (hipStream_t)stream;
+    lsg_lightmap* stage = (lsg_lightmap*)(base + kOffStage);
+    FilterMap* d_maps = (FilterMap*)(base + kOffMaps);
+    LSG_HIP(hipMemcpyAsync(stage, maps, sizeof(lsg_lightmap) * nmaps, hipMemcpyHostToDevice, s));
+    k_maps_from_lightmaps<<<1, 256, 0, s>>>(stage, nmaps, d_maps);
+    FilterScratch S;   // no pinned log and no events: the grow loop synchronises per chunk
+    S.a = (float4*)(base + kOffA); S.b = (float4*)(base + kOffB); S.c = (float4*)(base + kOffC);
+    S.front[0] = (uint32_t*)(base + kOffFront0); S.front[1] = (uint32_t*)(base + kOffFront1);
+    S.mark = (uint32_t*)(base + kOffMark); S.flags = (uint8_t*)(base + kOffFlags);
+    S.changed = (uint32_t*)(base + kOffChanged);
+    st = filter_run(d_maps, nmaps, (uint32_t)total, (float4*)rgba, rgbf, *params, S, s, true, -1, nullptr);
+    if (!st) { hipError_t e = hipStreamSynchronize(s); if (e != hipSuccess) st = hip_fail(e, "sync", __LINE__); }
     return st;
 }

@@ -113,6 +113,7 @@ SIGNATURES = {
     "lsg_filter_plan_create": (ctypes.c_int, [P, U32, ctypes.POINTER(P)]),
     "lsg_filter_plan_destroy": (None, [P]),
     "lsg_filter_plan_run": (ctypes.c_int, [P, P, P, ctypes.POINTER(FilterParams), P]),
+    "lsg_filter_lightmaps_noalloc": (ctypes.c_int, [P, U32, P, P, ctypes.POINTER(FilterParams), P]),
     "lsg_bake_create": (ctypes.c_int, [P, P, U32, P, P, ctypes.POINTER(P)]),
     "lsg_bake_destroy": (None, [P]),
     "lsg_bake_info_get": (ctypes.c_int, [P, ctypes.POINTER(BakeInfo)]),

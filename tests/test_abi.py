@@ -44,6 +44,13 @@ def test_invalid_arguments_without_device():
                                  None, None, ctypes.byref(out))
     assert code == -3 and "out of range" in native.last_error()
     assert L.lsg_collider_info_get(None, None) == -3
+    # the allocation-free filter chain checks its size limit on the host, before any device call
+    params = native.FilterParams(1.0, 1000.0, (ctypes.c_float * 4)(0, 0, 0, 0), 0)
+    big = (native.Lightmap * 1)(native.Lightmap(0, 1025, 1024, 0))
+    code = L.lsg_filter_lightmaps_noalloc(big, 1, ctypes.c_void_p(16), ctypes.c_void_p(16), ctypes.byref(params), None)
+    assert code == -6 and "at most 1048576" in native.last_error()
+    many = (native.Lightmap * 4097)()
+    assert L.lsg_filter_lightmaps_noalloc(many, 4097, ctypes.c_void_p(16), None, ctypes.byref(params), None) == -6
 
 
 def test_device_count_is_safe():
