@@ -275,18 +275,13 @@ class SceneBake:
         sub_base = 0
         self.slots = []
         bases = np.cumsum([0] + [ob.width * ob.height for ob in scene.objects])
-        # objects rasterise independently (lsg_rasterize_texels is reentrant and ctypes
-        # releases the GIL during the call); results are joined in object order
-        import concurrent.futures as cf
-        import os
-
+        # every object's lightmap is rasterised on the GPU (lsg_rasterize_texels), in object order
         def one(k):
             ob = scene.objects[k]
             ids = np.arange(ob.tri_begin, ob.tri_begin + ob.tri_count, dtype=np.uint32)
             return rasterize(scene.tri_uv[ob.tri_begin:ob.tri_begin + ob.tri_count], ids, ob.width, ob.height, k,
                              int(bases[k]))
-        with cf.ThreadPoolExecutor(max_workers=max(1, min(16, os.cpu_count() or 1, len(scene.objects)))) as ex:
-            parts = list(ex.map(one, range(len(scene.objects))))
+        parts = [one(k) for k in range(len(scene.objects))]
         for k, ob in enumerate(scene.objects):
             t, s = parts[k]
             t["sub_begin"] += sub_base

@@ -141,6 +141,17 @@ struct ColliderImpl {
 };
 
 int stack_size_for_depth(uint32_t depth);   // 0 when too deep
+
+// ---- device-wide primitives (lsgpu_prim.hip) --------------------------------------------
+// words of scratch scan_exclusive_u32 needs for n values
+size_t scan_tmp_words(uint32_t n);
+// out[i] = in[0] + ... + in[i-1] (in != out); *total (device, optional) = the sum of all
+int scan_exclusive_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total, uint32_t* tmp, hipStream_t st);
+size_t sort_tmp_words(uint32_t n);
+// stable sort of n (key, value) pairs by the low `bits` bits of the keys; the result is
+// in keys/vals, keys_tmp/vals_tmp are n-word scratch
+int sort_pairs_u32(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t n, int bits,
+                   uint32_t* tmp, hipStream_t st);
 // SAH rebuild of a BVH2's inner nodes over its leaves (lsgpu_sah.cpp): 0 = rebuilt,
 // 1 = fewer than two leaves, 2 = malformed tree
 int sah_rebuild(const float* nodes, uint32_t nnodes, int32_t root, float* out, uint32_t* out_nodes, int32_t* out_root,

@@ -270,6 +270,26 @@ def _ind(scene_arrays):
     return None if ind is None else np.ascontiguousarray(ind, dtype=np.float32).reshape(-1, 12)
 
 
+def rasterize(tri_uv, tri_ids, width, height, object_id=0, out_base=0):
+    """The oracle's texels (bake.TEXEL_DTYPE) and pieces (bake.SUBTEXEL_DTYPE) of one lightmap."""
+    from lightsprintsdk_amd import bake as B
+    L = lib()
+    P, U = ctypes.c_void_p, ctypes.c_uint32
+    L.lsgo_rasterize.argtypes = [P, P, U, U, U, U, U, P, U, P, U, ctypes.POINTER(U), ctypes.POINTER(U)]
+    L.lsgo_rasterize.restype = ctypes.c_int
+    uv = np.ascontiguousarray(tri_uv, np.float32).reshape(-1, 6)
+    ids = np.ascontiguousarray(tri_ids, np.uint32)
+    nt, ns = U(0), U(0)
+    L.lsgo_rasterize(_p(uv), _p(ids), len(ids), width, height, object_id, out_base, None, 0, None, 0,
+                     ctypes.byref(nt), ctypes.byref(ns))
+    tex = np.zeros(nt.value, B.TEXEL_DTYPE)
+    sub = np.zeros(ns.value, B.SUBTEXEL_DTYPE)
+    r = L.lsgo_rasterize(_p(uv), _p(ids), len(ids), width, height, object_id, out_base, _p(tex), len(tex), _p(sub), len(sub),
+                         ctypes.byref(nt), ctypes.byref(ns))
+    assert r == 0
+    return tex, sub
+
+
 def subtexel_walk(areas, series):
     """The subtexel of every shooter of the given series (list of shooter counts)."""
     a = np.ascontiguousarray(areas, dtype=np.float32)

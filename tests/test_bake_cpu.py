@@ -1,4 +1,4 @@
-"""CPU checks of the bake path: the texel rasteriser (host C++ in liblsgpu,
+"""CPU checks of the bake path: the oracle's texel rasteriser (oracle/lsg_oracle_raster.c,
 restating lightmap.cpp:83-335) against closed-form coverage, and the gather
 oracle (oracle/lsg_oracle_gather.c, restating gather.cpp:600-836) against the
 radiometric closed forms that pin it."""
@@ -12,10 +12,10 @@ from lightsprintsdk_amd import scenes as S
 
 
 def _raster(uv, w, h, tri_ids=None):
-    from lightsprintsdk_amd import bake as B
+    import oracle_py as O
     uv = np.asarray(uv, np.float32).reshape(-1, 6)
     ids = np.arange(len(uv), dtype=np.uint32) if tri_ids is None else tri_ids
-    return B.rasterize(uv, ids, w, h, 0)
+    return O.rasterize(uv, ids, w, h, 0)
 
 
 def heron(u):

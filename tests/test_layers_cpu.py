@@ -130,8 +130,8 @@ def test_layers_without_directions_match_plain_gather(oracle):
     arr = _arrays(sc.verts, sc.idx, sc.normals, ex,
                   [dict(type=0, position=(5, 5, 3.5), color=(20, 20, 20)), dict(type=1, direction=(0.2, 0.1, -1), color=(1, 1, 1))])
     ob = sc.objects[0]
-    tex, sub = B.rasterize(sc.tri_uv[ob.tri_begin:ob.tri_begin + ob.tri_count],
-                           np.arange(ob.tri_begin, ob.tri_begin + ob.tri_count, dtype=np.uint32), ob.width, ob.height, 0)
+    tex, sub = oracle.rasterize(sc.tri_uv[ob.tri_begin:ob.tri_begin + ob.tri_count],
+                                np.arange(ob.tri_begin, ob.tri_begin + ob.tri_count, dtype=np.uint32), ob.width, ob.height, 0)
     tex = np.ascontiguousarray(tex[::3])
     p = dict(quality=64, hemisphere=1, ray_length_min=S.minimal_safe_distance(sc.verts), locality=1e5, rug_distance=1e-3,
              light_multiplier=1.0, seed=3)

@@ -1,0 +1,84 @@
+"""The oracle's texel rasteriser (oracle/lsg_oracle_raster.c, restating the method of
+lightmap.cpp:147-331) against brute force: every (texel, triangle) intersection computed in
+binary64 by clipping the triangle with the texel's square.  Covered texels, piece areas
+(half the map-space area, within the binary32 Heron error) and the texel order must agree;
+copies of the oracle with a wrong clipping side, a biased area or a transposed map scale,
+built here on the CPU, must fail."""
+import ctypes
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from lightsprintsdk_amd import scenes as S
+from test_gpu_raster import _truth, _pieces
+
+
+def check_oracle(uv, w, h, oracle, sliver=1e-6):
+    uv = np.asarray(uv, np.float32).reshape(-1, 6)
+    ot, os_ = oracle.rasterize(uv, np.arange(len(uv), dtype=np.uint32), w, h)
+    assert np.all(np.diff(ot["out_index"].astype(np.int64)) > 0)
+    unit = 0.5 / (w * h)
+    T, Oo = _truth(uv, w, h), _pieces(ot, os_)
+    bad = 0
+    for key in set(T) | set(Oo):
+        at = T.get(key, (0.0, []))[0]
+        ao = Oo.get(key, (0.0, []))[0]
+        if key not in Oo:
+            bad += at > sliver * unit
+        elif abs(ao - at) > 2e-3 * max(at, 1e-3 * unit) and (key in T or ao > sliver * unit):
+            bad += 1
+    return bad, len(T)
+
+
+def test_oracle_raster_random_vs_brute_force(oracle):
+    rng = np.random.default_rng(7)
+    for w, h, n, size in [(16, 16, 40, 0.3), (64, 48, 300, 0.08), (256, 256, 500, 0.02), (37, 91, 40, 0.5)]:
+        c = rng.uniform(-0.1, 1.1, (n, 1, 2))
+        bad, n_int = check_oracle((c + rng.uniform(-size, size, (n, 3, 2))).reshape(n, 6), w, h, oracle)
+        assert bad == 0 and n_int > 100
+
+
+def test_oracle_raster_grid_aligned_and_scenes_vs_brute_force(oracle):
+    rng = np.random.default_rng(8)
+    q = rng.integers(0, 65, (300, 3, 2)) / 64.0
+    assert check_oracle(q.reshape(-1, 6), 32, 32, oracle, sliver=1e-5)[0] == 0
+    sc = S.box_room(seed=3, n_boxes=6, target_tris=8000, lightmap_size=96)
+    for ob in sc.objects[:3]:
+        assert check_oracle(sc.tri_uv[ob.tri_begin:ob.tri_begin + ob.tri_count], ob.width, ob.height, oracle)[0] == 0
+
+
+ORACLE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle")
+MUTATIONS = [
+    ("    return k == 0 ? q.y : (k == 1 ? 1 - q.x - q.y : q.x);", "    return k == 0 ? q.y : (k == 1 ? 1 - q.x : q.x);"),
+    ("    float s = (ab + bc + ca) * 0.5f;", "    float s = (ab + bc + ca) * 0.5f + 1e-4f;"),
+    ("                float dx = cx[k] / W - u0.x, dy = cy[k] / H - u0.y;", "                float dx = cx[k] / W - u0.x, dy = cy[k] / W - u0.y;"),
+]
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="needs gcc to build the planted copies")
+@pytest.mark.parametrize("k", range(len(MUTATIONS)))
+def test_planted_raster_mistakes_fail(oracle, tmp_path, k):
+    old, new = MUTATIONS[k]
+    srcs = ["lsg_oracle.c", "lsg_oracle_gather.c", "lsg_oracle_filters.c", "lsg_oracle_mlp.c", "lsg_oracle_raster.c"]
+    for f in srcs:
+        txt = open(os.path.join(ORACLE, f)).read()
+        if f == "lsg_oracle_raster.c":
+            assert txt.count(old) == 1, "the planted site moved: " + old
+            txt = txt.replace(old, new)
+        (tmp_path / f).write_text(txt)
+    shutil.copy(os.path.join(ORACLE, "lsg_oracle.h"), tmp_path / "lsg_oracle.h")
+    so = tmp_path / "mutant.so"
+    subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-std=c11", "-fopenmp", "-o", str(so)]
+                   + [str(tmp_path / f) for f in srcs] + ["-lm"], check=True)
+    saved, orig = oracle._lib, oracle.ORACLE_LIB
+    try:
+        oracle._lib, oracle.ORACLE_LIB = None, str(so)
+        rng = np.random.default_rng(7)
+        c = rng.uniform(-0.1, 1.1, (300, 1, 2))
+        bad, _ = check_oracle((c + rng.uniform(-0.08, 0.08, (300, 3, 2))).reshape(300, 6), 64, 48, oracle)
+    finally:
+        oracle._lib, oracle.ORACLE_LIB = saved, orig
+    assert bad > 0, "planted mistake survived: " + new
