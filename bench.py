@@ -43,7 +43,16 @@ HBM_PEAK_GBS = 8000.0
 NODE_BYTES = 128          # one 4-wide node per traversal step (n_node counts these)
 # L2-to-fabric bytes per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by
 # the same counters on a 1 GiB copy (x2.0 and x1.0): profiles/pmc_traffic_k_gather_k_trace_closest.md
-TRAFFIC_BYTES = {"k_gather": 4.691e9, "k_trace_closest": 1.194e9}
+TRAFFIC_BYTES = {"k_gather": 4.668e9, "k_trace_closest": 1.194e9}
+# VALU issue: a SIMD issues one wave64 VALU instruction per 4 cycles (16 lanes wide; the
+# SQ counters give one quad-cycle of SQ_ACTIVE_INST_VALU per SQ_INSTS_VALU), so MI355X
+# issues 256 CUs x 4 SIMDs x 2.4 GHz / 4 = 614.4 G wave64 VALU instructions per second.
+VALU_PEAK_G = 256 * 4 * 2.4 / 4
+# wave64 VALU instructions per launch and active lanes per instruction, from rocprofv3
+# --pmc SQ_INSTS_VALU SQ_ACTIVE_INST_VALU SQ_THREAD_CYCLES_VALU:
+# profiles/pmc_sq_valu_k_gather_k_trace_closest.md
+VALU_INSTS = {"k_gather": 52.0965e9, "k_trace_closest": 289.757e6}
+LANE_UTIL = {"k_gather": 0.6285, "k_trace_closest": 0.1342}
 METRIC = "closest-hit Mrays/s; baked texels/s at ≤2% RMSE, plain vs learned gather"
 
 
@@ -102,11 +111,46 @@ def collider_workload(steps, warmup, cpu_threads, with_cpu=True):
         roofline=dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
                       frac=round(achieved / HBM_PEAK_GBS, 4), traffic=TRAFFIC_BYTES["k_trace_closest"],
                       kernel="k_trace_closest", kernel_ms=round(kernel_ms, 4),
-                      bytes_per_ray=round(b_ray, 2), n_node=round(n_node, 3), n_tri=round(n_tri, 3)),
+                      bytes_per_ray=round(b_ray, 2), n_node=round(n_node, 3), n_tri=round(n_tri, 3),
+                      hbm_counters_gbs=round(TRAFFIC_BYTES["k_trace_closest"] / (kernel_ms * 1e-3) / 1e9, 1),
+                      valu=_valu("k_trace_closest", kernel_ms)),
     )
     if with_cpu:
         res["cpu_baseline"] = collider_cpu_baseline(V, I, R, cpu_threads, col)
     return res
+
+
+def _valu(kernel, kernel_ms):
+    """VALU issue of a kernel: its profiled instruction count per launch over the live
+    launch time, against the chip's issue peak."""
+    g = VALU_INSTS[kernel] / (kernel_ms * 1e-3) / 1e9
+    return dict(achieved=round(g, 1), peak=VALU_PEAK_G, unit="G VALU instr/s", frac=round(g / VALU_PEAK_G, 4),
+                insts_per_launch=VALU_INSTS[kernel], lanes_active=LANE_UTIL[kernel])
+
+
+def cpu_calibration(sample: int = 200_000):
+    """rho = the oracle's CPU traversal over the compiled reference collider (IT_BSP_FASTER),
+    one core each, on the same 100,820-triangle icosphere and rays: the ratio that turns a
+    time of this project's CPU statement into a reference-equivalent one where the
+    reference cannot run (its RRSolver needs boost)."""
+    O = _oracle()
+    if not O.have_reference():
+        return None
+    from lightsprintsdk_amd import scenes
+    from lightsprintsdk_amd.collider import RRCollider
+    V, I = scenes.icosphere(scenes.ICOSPHERE_100K_FREQUENCY, seed=1)
+    R = scenes.random_rays(V, sample, seed=2)
+    _, tm = O.reference_intersect(V, I, R, technique=3, threads=1, repeats=3, flags=(1 | 4 | 16))
+    tree = RRCollider.create(V, I).download_tree()
+    O.bvh_traverse(tree, V, I, R[:10_000], threads=1)
+    best = 0.0
+    for _ in range(3):
+        t0 = time.perf_counter()
+        O.bvh_traverse(tree, V, I, R, threads=1)
+        best = max(best, len(R) / (time.perf_counter() - t0) / 1e6)
+    return dict(rho=round(best / tm["mrays_per_s"], 4), reference_mrays_per_core=round(tm["mrays_per_s"], 4),
+                port_mrays_per_core=round(best, 4), rays=sample,
+                note="one host core each, closest hit on the same icosphere and rays")
 
 
 def collider_cpu_baseline(V, I, R, threads, col):
@@ -114,8 +158,9 @@ def collider_cpu_baseline(V, I, R, threads, col):
     if O.have_reference():
         _, tm = O.reference_intersect(V, I, R, technique=3, threads=threads, repeats=3, flags=(1 | 4 | 16))
         return dict(value=round(tm["mrays_per_s"], 4), unit="Mrays/s", cores=int(tm["threads"]), kind="reference",
-                    sample=f"IT_BSP_FASTER (compiled reference collider) on the same mesh and all {len(R)} rays, "
-                           f"FILL_DISTANCE|FILL_POINT2D|FILL_TRIANGLE, best of 3; build {tm['build_s']:.2f} s not included")
+                    sample=f"IT_BSP_FASTER (the reference collider compiled from its sources) on the same mesh and "
+                           f"all {len(R)} rays, all {threads} host cores, FILL_DISTANCE|FILL_POINT2D|FILL_TRIANGLE, "
+                           f"best of 3; its BSP build ({tm['build_s']:.2f} s) not included")
     tree = col.download_tree()
     n = min(len(R), 500_000)
     t0 = time.perf_counter()
@@ -186,13 +231,18 @@ def bake_workload(steps: int, warmup: int, cpu_threads: int):
                     quality=1024, hemisphere_rays=int(hr), shadow_rays=int(sr), rays_per_s=round(rays_per_s, 1),
                     mrays_per_s=round(rays_per_s / 1e6, 2), first_gather_s=round(first_s, 4), setup_s=round(setup_s, 3),
                     mean_irradiance=float(lit.mean())),
-        roofline=dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                      frac=round(achieved / HBM_PEAK_GBS, 4), traffic=TRAFFIC_BYTES["k_gather"], kernel="k_gather", kernel_ms=round(kernel_ms, 4),
-                      bytes_per_texel=round(b_texel, 1), bytes_per_hemisphere_ray=round(b_ray, 2),
-                      bytes_per_shadow_ray=round(b_shadow, 2), n_node_hemi=round(hn / hr, 3), n_tri_hemi=round(ht / hr, 3),
-                      n_node_shadow=round(sn / max(sr, 1), 3), n_tri_shadow=round(stt / max(sr, 1), 3),
-                      subtexels_per_texel=round(n_sub, 3), algorithmic_bytes=round(b_texel * ntex),
-                      traffic_note="L2-to-fabric bytes per launch (PMC); the scene stays in L2/MALL"),
+        roofline=dict(bound="valu", **{k: v for k, v in _valu("k_gather", kernel_ms).items()},
+                      traffic=TRAFFIC_BYTES["k_gather"], kernel="k_gather", kernel_ms=round(kernel_ms, 4),
+                      hbm=dict(achieved_gbs=round(TRAFFIC_BYTES["k_gather"] / (kernel_ms * 1e-3) / 1e9, 1), peak_gbs=HBM_PEAK_GBS,
+                               frac=round(TRAFFIC_BYTES["k_gather"] / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+                               note="L2-to-fabric bytes per launch (FETCH_SIZE/WRITE_SIZE); the scene stays in L2/MALL"),
+                      model=dict(bytes_per_texel=round(b_texel, 1), bytes_per_hemisphere_ray=round(b_ray, 2),
+                                 bytes_per_shadow_ray=round(b_shadow, 2), n_node_hemi=round(hn / hr, 3),
+                                 n_tri_hemi=round(ht / hr, 3), n_node_shadow=round(sn / max(sr, 1), 3),
+                                 n_tri_shadow=round(stt / max(sr, 1), 3), subtexels_per_texel=round(n_sub, 3),
+                                 algorithmic_bytes=round(b_texel * ntex), model_gbs=round(achieved, 1),
+                                 note="node and triangle bytes the traversal reads, most of them cache hits: "
+                                      "a work measure, not HBM traffic")),
     )
     res["cpu_baseline"] = bake_cpu_baseline(sb, cpu_threads)
     return res
@@ -201,7 +251,9 @@ def bake_workload(steps: int, warmup: int, cpu_threads: int):
 def bake_cpu_baseline(sb, threads: int, target_s: float = 12.0):
     """The CPU oracle's FP64 statement of processTexel (a port: the reference's RRSolver
     needs boost and is not built here) on a bounded, evenly spread sample of the same
-    texels, all host threads."""
+    texels, all host cores; with the per-core rate and, where the compiled reference
+    collider exists, the reference-equivalent rate port / rho (rho measured on the collider,
+    which is most of a gather's work)."""
     O = _oracle()
     tree = sb.gs.col.download_tree()
     arrays = sb.gs.oracle_arrays(sb.exitance)
@@ -215,9 +267,14 @@ def bake_cpu_baseline(sb, threads: int, target_s: float = 12.0):
     t0 = time.perf_counter()
     O.gather(arrays, sb.params.as_dict(), sb.texels[sel].view(np.uint32).reshape(-1, 4), sb.subs, tree=tree, threads=threads)
     dt = time.perf_counter() - t0
-    return dict(value=round(n1 / dt, 1), unit="texels/s", cores=threads, kind="port",
-                sample=f"{n1} of {len(sb.texels)} texels evenly spread, oracle C gather (FP64 sums) over the same tree, "
-                       f"{dt:.1f} s")
+    out = dict(value=round(n1 / dt, 1), unit="texels/s", cores=threads, kind="port",
+               sample=f"{n1} of {len(sb.texels)} texels evenly spread, oracle C gather (FP64 sums) over the same tree, "
+                      f"{dt:.1f} s on all {threads} host cores", per_core=round(n1 / dt / threads, 1))
+    cal = cpu_calibration()
+    if cal:
+        out["calibration"] = cal
+        out["reference_equivalent"] = round(n1 / dt / cal["rho"], 1)
+    return out
 
 
 # ---- learned + fallback vs plain on the held-out box-room scenes -------------------------
@@ -388,13 +445,33 @@ def batched_workload(steps: int, warmup: int, n_objects: int = 200, target_tris:
                             setup_s=round(setup_s, 2), grow_steps=int(inf.grow_steps)))
 
 
+def copy_workload(steps: int, warmup: int):
+    """A plain 1 GiB device-to-device copy: the known-size kernel whose FETCH_SIZE and
+    WRITE_SIZE readings scale the same counters of the bake and collider kernels."""
+    import torch
+    n = 1 << 28
+    a = torch.empty(n, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(warmup):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / max(steps, 1)
+    return dict(metric="device copy GB/s (counter calibration)", value=round(2 * 4 * n / dt / 1e9, 1), unit="GB/s",
+                ms_per_step=round(dt * 1e3, 4), dtype="f32",
+                config=dict(workload="1 GiB device-to-device copy (reads 1 GiB, writes 1 GiB)", bytes=4 * n))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default=os.environ.get("LSG_BENCH_WORKLOAD", "bake"),
-                    choices=["bake", "collider", "batched", "learned", "scale"])
+                    choices=["bake", "collider", "batched", "learned", "scale", "copy"])
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-extras", action="store_true")
     a = ap.parse_args()
@@ -403,20 +480,24 @@ def main():
                                    "would shard by batch/object across GPUs; no multi-GPU path is implemented)",
                           "n_gpus": a.gpus}))
         sys.exit(2)
-    threads = a.cpu_threads or min(os.cpu_count() or 1, 16)
+    threads = a.cpu_threads or len(os.sched_getaffinity(0))      # every host core this process may use
     if a.workload == "bake":
         res = bake_workload(a.steps, a.warmup, threads)
         if not a.no_extras:
             res["learned"] = learned_summary()
-            c = collider_workload(5, 1, threads, with_cpu=False)
+            c = collider_workload(5, 1, threads, with_cpu=True)
+            cb = c["cpu_baseline"]
             res["collider"] = dict(workload=c["config"]["workload"], mrays_per_s=c["value"],
-                                   kernel_ms=c["roofline"]["kernel_ms"], roofline_frac=c["roofline"]["frac"])
+                                   kernel_ms=c["roofline"]["kernel_ms"], roofline=c["roofline"], cpu_baseline=cb,
+                                   vs_cpu=round(c["value"] / cb["value"], 2))
     elif a.workload == "batched":
         res = batched_workload(a.steps, a.warmup)
     elif a.workload == "learned":
         res = learned_workload(a.steps, a.warmup)
     elif a.workload == "scale":
         res = scale_workload(a.steps, a.warmup)
+    elif a.workload == "copy":
+        res = copy_workload(a.steps, a.warmup)
     else:
         res = collider_workload(a.steps, a.warmup, threads)
     out = dict(metric=res.pop("metric"), value=res.pop("value"), unit=res.pop("unit"), n_gpus=1, steps=a.steps,
