@@ -430,6 +430,12 @@ int  lsg_bake_run_layers(lsg_bake* bake, const lsg_gather_params* first, const l
                          const lsg_filter_params* filtering, uint32_t layer_mask, void* stream);
 /* copies layer `layer` of object `object` (width*height*3 floats) to the host */
 int  lsg_bake_download_layer(const lsg_bake* bake, uint32_t object, uint32_t layer, float* rgbf);
+/* the same layer as the filters left it, linear RGBA (width*height*4 floats): the
+ * reference's per-object RGBAF work buffer after lightmapGrow and lightmapFillBackground
+ * (lightmap.cpp:809, 866-873), which copyElementsTo then writes into the caller's
+ * buffer with alpha (RRBuffer.cpp:105-131); background texels carry the filter
+ * parameters' background colour, alpha included */
+int  lsg_bake_download_layer_rgba(const lsg_bake* bake, uint32_t object, uint32_t layer, float* rgba);
 
 /* ---- learned gather ----------------------------------------------------------------------
  * No counterpart in the reference: a surrogate that replaces most of the 1024-ray

@@ -487,6 +487,14 @@ class BakePlan:
         N.check(N.lib().lsg_bake_download_layer(self._h, k, layer, out.ctypes.data_as(ctypes.c_void_p)), "bake download")
         return out
 
+    def lightmap_layer_rgba(self, k, layer=0):
+        """Object k's layer as the filters left it, linear RGBA with the background's alpha
+        (lsg_bake_download_layer_rgba)."""
+        ob = self.scene.objects[k]
+        out = np.zeros((ob.height, ob.width, 4), np.float32)
+        N.check(N.lib().lsg_bake_download_layer_rgba(self._h, k, layer, out.ctypes.data_as(ctypes.c_void_p)), "bake download")
+        return out
+
     def sample_ms(self, stride):
         ms = ctypes.c_float()
         p = self.params.c()

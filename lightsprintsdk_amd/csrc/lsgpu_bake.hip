@@ -409,6 +409,19 @@ int lsg_bake_download_layer(const lsg_bake* b, uint32_t object, uint32_t layer, 
     return LSG_OK;
 }
 
+int lsg_bake_download_layer_rgba(const lsg_bake* b, uint32_t object, uint32_t layer, float* rgba)
+{
+    if (!b || !rgba) return fail(LSG_ERR_INVALID, "lsg_bake_download_layer_rgba: null argument");
+    const BakeImpl& B = b->impl;
+    if (object >= B.nmaps) return fail(LSG_ERR_INVALID, "lsg_bake_download_layer_rgba: object %u of %u", object, B.nmaps);
+    if (layer >= LSG_NUM_LAYERS) return fail(LSG_ERR_INVALID, "lsg_bake_download_layer_rgba: layer %u", layer);
+    const float4* src = layer ? B.d_lrgba[layer - 1] : B.d_rgba;
+    if (!src || (layer && !B.d_lrgbf[layer - 1])) return fail(LSG_ERR_INVALID, "lsg_bake_download_layer_rgba: layer %u was never baked", layer);
+    const lsg_lightmap& m = B.maps[object];
+    LSG_HIP(hipMemcpy(rgba, src + m.rgba_base, 16 * (size_t)m.width * m.height, hipMemcpyDeviceToHost));
+    return LSG_OK;
+}
+
 int lsg_bake_set_bounces(lsg_bake* b, uint32_t bounces, uint32_t quality)
 {
     if (!b) return fail(LSG_ERR_INVALID, "lsg_bake_set_bounces: null plan");

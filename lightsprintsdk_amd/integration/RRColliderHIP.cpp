@@ -1,5 +1,6 @@
 // RRColliderHIP.cpp -- see RRColliderHIP.h.
 #include "RRColliderHIP.h"
+#include "Lightsprint/RRDebug.h"
 #include <cstring>
 #include <vector>
 
@@ -41,9 +42,19 @@ bool RRColliderHIP::build(bool& aborting)
 	volatile int abortFlag = aborting ? 1 : 0;
 	lsg_collider* c = nullptr;
 	if (lsg_collider_create(v.data(), (uint32_t)(v.size() / 3), idx.data(), (uint32_t)(idx.size() / 3), &bp, &abortFlag, &c) != LSG_OK)
+	{
+		// RRCollider::create reports a failed builder and falls back to IT_LINEAR
+		// (RRCollider.cpp:216-239); the device's reason goes to the same log
+		RRReporter::report(ERRO, "RRColliderHIP: %s\n", lsg_last_error());
 		return false;
+	}
 	lsg_host::Tree t;
-	if (!t.load(c)) { lsg_collider_destroy(c); return false; }
+	if (!t.load(c))
+	{
+		RRReporter::report(ERRO, "RRColliderHIP: tree download failed: %s\n", lsg_last_error());
+		lsg_collider_destroy(c);
+		return false;
+	}
 	if (col) lsg_collider_destroy(col);
 	col = c;
 	tree = std::move(t);
@@ -134,11 +145,16 @@ unsigned RRColliderHIP::intersectBatch(RRRay* rays, unsigned numRays) const
 		else gpu.push_back(i);
 	if (gpu.empty()) return hits;
 	static_assert(sizeof(RRRay) == LSG_RRRAY_SIZE, "RRRay layout differs from the 128-byte LP64 record");
+	// a device error is reported (RRReporter, as the reference reports a failed collider,
+	// RRCollider.cpp:216-239) and the batch is answered by the host copy of the same tree,
+	// which gives the same hits
 	uint32_t n = 0;
 	if (gpu.size() == numRays)
 	{
 		if (lsg_collider_intersect_rrrays(col, rays, numRays, nullptr, &n) != LSG_OK)
 		{
+			RRReporter::report(ERRO, "RRColliderHIP::intersectBatch: %s; %u rays traced on the host.\n", lsg_last_error(), numRays);
+			n = 0;
 			for (unsigned i = 0; i < numRays; i++) n += intersect(rays[i]) ? 1 : 0;
 		}
 		return hits + n;
@@ -147,6 +163,7 @@ unsigned RRColliderHIP::intersectBatch(RRRay* rays, unsigned numRays) const
 	for (size_t k = 0; k < gpu.size(); k++) memcpy((void*)&tmp[k], (const void*)&rays[gpu[k]], sizeof(RRRay));
 	if (lsg_collider_intersect_rrrays(col, tmp.data(), (uint32_t)tmp.size(), nullptr, &n) != LSG_OK)
 	{
+		RRReporter::report(ERRO, "RRColliderHIP::intersectBatch: %s; %u rays traced on the host.\n", lsg_last_error(), (unsigned)tmp.size());
 		n = 0;
 		for (auto& r : tmp) n += intersect(r) ? 1 : 0;
 	}

@@ -1,5 +1,6 @@
 // RRSolverHIP.cpp -- see RRSolverHIP.h.
 #include "RRSolverHIP.h"
+#include "lsg_writeback.h"
 #include "Lightsprint/RRBuffer.h"
 #include "Lightsprint/RRLight.h"
 #include "Lightsprint/RRMaterial.h"
@@ -26,10 +27,16 @@ void RRSolverHIP::releaseScene()
 // The multiObject (RRSolver.h:962) as the GPU scene: world-space triangles with their
 // own three vertices (vertex normals differ per triangle, RRMesh::getTriangleNormals,
 // RRMesh.h:217-231), side bits and colours of the triangle materials
-// (RRObject::getTriangleMaterial, RRObject.h:137) and the exitance of the current
-// solution (getTriangleMeasure with RM_RADIOSITY_LINEAR, RRSolver.h:797: reflected
-// direct + indirect light plus emittance, the value a gather ray reads at a hit,
-// pathtracer.cpp:139-394).
+// (RRObject::getTriangleMaterial, RRObject.h:137) and what a gather ray reads where it
+// hits a legal side of a static triangle (PathtracerWorker::getIncidentRadiance with the
+// solver shortcuts the final gather turns on, gather.cpp:181-189, pathtracer.cpp:189-340):
+//   flat per side:  direct irradiance (getTriangleMeasure(t, 3, direct only)) x albedo
+//                   + emittance x indirect.materialEmittanceMultiplier where the side emits
+//   per corner:     albedo x the indirect irradiance of corners 0..2
+//                   (getTriangleMeasure(t, 0..2, RM_IRRADIANCE_LINEAR_INDIRECT)),
+//                   interpolated at the hit's hitPoint2d on the GPU, on legal sides of a
+//                   triangle whose albedo is not black (the diffuse shortcut's condition).
+// The final gather takes both multipliers from params.indirect (gather.cpp:181).
 bool RRSolverHIP::buildScene(const UpdateParameters& params)
 {
 	releaseScene();
@@ -40,9 +47,11 @@ bool RRSolverHIP::buildScene(const UpdateParameters& params)
 	std::vector<float> verts(9 * (size_t)nt), normals(9 * (size_t)nt), exitance(8 * (size_t)nt, 0.f);
 	std::vector<float> tangents(9 * (size_t)nt), bitangents(9 * (size_t)nt);
 	std::vector<uint32_t> idx(3 * (size_t)nt);
-	std::vector<uint8_t> side(nt, 0);
+	std::vector<uint8_t> side(nt, 0), indirectSides(nt, 0);
+	std::vector<float> corners(9 * (size_t)nt, 0.f);
 	albedo.assign(3 * (size_t)nt, 0.f);
 	emittance.assign(3 * (size_t)nt, 0.f);
+	const float emitMultiplier = params.indirect.materialEmittanceMultiplier;
 	for (unsigned t = 0; t < nt; t++)
 	{
 		RRMesh::Triangle tr;
@@ -72,20 +81,34 @@ bool RRSolverHIP::buildScene(const UpdateParameters& params)
 			for (int c = 0; c < 3; c++)
 			{
 				albedo[3 * (size_t)t + c] = m->diffuseReflectance.colorLinear[c];
-				emittance[3 * (size_t)t + c] = m->diffuseEmittance.colorLinear[c] * params.direct.materialEmittanceMultiplier;
+				emittance[3 * (size_t)t + c] = m->diffuseEmittance.colorLinear[c] * emitMultiplier;
 			}
 		}
-		RRVec3 ex(0);
+		if (!m) continue;
 		if (gpuBounces)
 		{
 			// the GPU solution starts from the emitters (the plan's first gather resets to them)
 			for (int c = 0; c < 3; c++) exitance[8 * (size_t)t + c] = emittance[3 * (size_t)t + c];
+			continue;
 		}
-		else if (getTriangleMeasure(t, 3, RM_RADIOSITY_LINEAR, ex))
+		RRVec3 direct(0);
+		getTriangleMeasure(t, 3, RRRadiometricMeasure(0, 0, 0, 1, 0), direct);
+		const bool diffuse = m->diffuseReflectance.colorLinear.avg() > 0;
+		bool anyIndirect = false;
+		for (int s = 0; s < 2; s++)
+		{
+			if (!m->sideBits[s].legal) continue;
 			for (int c = 0; c < 3; c++)
+				exitance[8 * (size_t)t + 4 * s + c] = direct[c] * albedo[3 * (size_t)t + c]
+				                                    + (m->sideBits[s].emitTo ? emittance[3 * (size_t)t + c] : 0.f);
+			if (diffuse) { indirectSides[t] |= (uint8_t)(1u << s); anyIndirect = true; }
+		}
+		if (anyIndirect)
+			for (unsigned k = 0; k < 3; k++)
 			{
-				exitance[8 * (size_t)t + c] = ex[c];
-				if (m && m->sideBits[1].emitTo) exitance[8 * (size_t)t + 4 + c] = ex[c];
+				RRVec3 in(0);
+				getTriangleMeasure(t, k, RM_IRRADIANCE_LINEAR_INDIRECT, in);
+				for (int c = 0; c < 3; c++) corners[9 * (size_t)t + 3 * k + c] = in[c];
 			}
 	}
 	// lights (RRLight.h:50-153): point with REALISTIC attenuation and directional
@@ -122,6 +145,13 @@ bool RRSolverHIP::buildScene(const UpdateParameters& params)
 		return false;
 	}
 	if (lsg_scene_create(col, normals.data(), side.data(), exitance.data(), lights.data(), (uint32_t)lights.size(), &scene) != LSG_OK)
+	{
+		RRReporter::report(ERRO, "RRSolverHIP: %s\n", lsg_last_error());
+		releaseScene();
+		return false;
+	}
+	// the interpolated indirect term of the reference solver's solution
+	if (!gpuBounces && lsg_scene_set_indirect(scene, corners.data(), albedo.data(), indirectSides.data()) != LSG_OK)
 	{
 		RRReporter::report(ERRO, "RRSolverHIP: %s\n", lsg_last_error());
 		releaseScene();
@@ -229,14 +259,13 @@ unsigned RRSolverHIP::bake(const Target* targets, unsigned numTargets, const Upd
 	g.light_multiplier = params.direct.lightMultiplier;
 	g.seed = 1;
 	g.environment_multiplier = params.indirect.environmentMultiplier;
-	lsg_filter_params f = { 1.f, 1000.f, {0, 0, 0, 0}, 0 };
-	if (filtering)
-	{
-		f.smoothing_amount = filtering->smoothingAmount;
-		f.spread_foreground = (float)filtering->spreadForegroundColor;
-		for (int c = 0; c < 4; c++) f.background[c] = filtering->backgroundColor[c];
-		f.wrap = filtering->wrap ? 1 : 0;
-	}
+	// filters with the defaults of FilteringParameters when none are given (lightmap.cpp:852-854);
+	// the GPU fills the background with alpha 0 and the writeback puts the caller's
+	// background there, converted per target (lsg_writeback.h)
+	const FilteringParameters defaults;
+	if (!filtering) filtering = &defaults;
+	lsg_filter_params f = { filtering->smoothingAmount, (float)filtering->spreadForegroundColor, {0, 0, 0, 0},
+	                        filtering->wrap ? 1 : 0 };
 	// GPU radiosity: first gather at half quality (gather.cpp:1083-1088), lights and the
 	// environment only, then the bounces; otherwise the reference solver's exitance is used
 	lsg_gather_params first = g;
@@ -262,20 +291,25 @@ unsigned RRSolverHIP::bake(const Target* targets, unsigned numTargets, const Upd
 		RRReporter::report(ERRO, "RRSolverHIP: %s\n", lsg_last_error());
 	else
 	{
-		// writeback: RGBF texels into the caller's buffers (copyElementsTo, RRBuffer.cpp:105;
-		// scaled formats take the solver's colour space, bent normals none, lightmap.cpp:875)
-		std::vector<float> rgbf;
+		// writeback of every layer into the caller's buffers (lightmap.cpp:866-876)
+		std::vector<float> rgba;
+		const unsigned version = getSolutionVersion();
 		for (unsigned k = 0; k < numTargets; k++)
 			for (int l = 0; l < LSG_NUM_LAYERS; l++)
 			{
 				RRBuffer* b = targets[k].buffer[l];
 				if (!b) continue;
-				rgbf.resize(3 * (size_t)bo[k].width * bo[k].height);
-				if (lsg_bake_download_layer(plan, k, (uint32_t)l, rgbf.data()) != LSG_OK) continue;
-				const RRColorSpace* cs = (b->getScaled() && l != LSG_LAYER_BENT_NORMALS) ? getColorSpace() : nullptr;
-				for (size_t i = 0; i < (size_t)bo[k].width * bo[k].height; i++)
-					b->setElement((unsigned)i, RRVec4(rgbf[3 * i], rgbf[3 * i + 1], rgbf[3 * i + 2], 1), cs);
-				b->version++;
+				const size_t n = (size_t)bo[k].width * bo[k].height;
+				rgba.resize(4 * n);
+				if (lsg_bake_download_layer_rgba(plan, k, (uint32_t)l, rgba.data()) != LSG_OK)
+				{
+					RRReporter::report(ERRO, "RRSolverHIP: %s\n", lsg_last_error());
+					continue;
+				}
+				writebackLayer(rgba.data(), n, filtering->backgroundColor, b->getScaled(), getColorSpace(),
+				               l == LSG_LAYER_BENT_NORMALS ? nullptr : getColorSpace(),
+				               [b](unsigned i, const RRVec4& c) { b->setElement(i, c, nullptr); });
+				b->version = version;
 				updated++;
 			}
 	}

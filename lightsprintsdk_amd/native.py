@@ -124,6 +124,7 @@ SIGNATURES = {
     "lsg_bake_set_bounces": (ctypes.c_int, [P, U32, U32]),
     "lsg_bake_run_layers": (ctypes.c_int, [P, ctypes.POINTER(GatherParams), ctypes.POINTER(GatherParams), ctypes.POINTER(FilterParams), U32, P]),
     "lsg_bake_download_layer": (ctypes.c_int, [P, U32, U32, P]),
+    "lsg_bake_download_layer_rgba": (ctypes.c_int, [P, U32, U32, P]),
     "lsg_mlp_num_params": (U32, []),
     "lsg_mlp_create": (ctypes.c_int, [U32, ctypes.POINTER(P)]),
     "lsg_mlp_destroy": (None, [P]),

@@ -103,6 +103,11 @@ def test_gpu_bake_layers_plan_equals_scene_bake(gpu, oracle):
     # filter chain of one layer against the oracle's
     raw = sb.gs.gather_layers(sb.texels, sb.subs, sb.params, layers=(2,), nout=sb.nout)[2]
     base, w, h = sb.slots[0]
-    _, rgbf = oracle.filter_lightmap(raw[base:base + w * h].reshape(h, w, 4))
+    rgba, rgbf = oracle.filter_lightmap(raw[base:base + w * h].reshape(h, w, 4))
     np.testing.assert_array_equal(got[2][0], rgbf.astype(np.float32))
+    # the RGBA work buffer the drop-in solver writes back: the oracle's, alpha included
+    np.testing.assert_array_equal(plan.lightmap_layer_rgba(0, 2), rgba.astype(np.float32))
+    for k in layers:
+        for o in range(len(sc.objects)):
+            np.testing.assert_array_equal(plan.lightmap_layer_rgba(o, k)[..., :3], got[k][o])
     plan.close()
