@@ -9,6 +9,9 @@ Fixtures (inputs + the reference's outputs, nothing else):
   collider_ico100k.npz  closest-hit benchmark mesh (icosphere frequency 71, seed 1; only its
                         sha256 is stored, the generator rebuilds it) with 2048
                         seeded rays; IT_BSP_FASTER and IT_LINEAR hit records.
+  collider_ico100k_wide.npz  the same mesh with 100,000 seeded rays (seed 3; only the
+                        rays' sha256 is stored); the IT_BSP_FASTER triangle and distance
+                        of each (the reference's other fields are in the 2048-ray file).
   collider_edge.npz     small icosphere (frequency 6, 720 triangles) stored in
                         full, with edge-case rays: axis-aligned directions
                         (zero components), rays starting on vertices and
@@ -17,6 +20,7 @@ Fixtures (inputs + the reference's outputs, nothing else):
                         faces) and rays hitting them; exercises the reference's
                         plane-node distance rule.
 """
+import hashlib
 import os
 import sys
 
@@ -89,6 +93,12 @@ def main():
                         frequency=scenes.ICOSPHERE_100K_FREQUENCY, mesh_seed=1, ray_seed=2,
                         mesh_sha256=scenes.mesh_digest(V, I), rays=R,
                         bsp_faster=ref(V, I, R, IT_BSP_FASTER), linear=ref(V, I, R, IT_LINEAR))
+    R = scenes.random_rays(V, 100_000, seed=3)
+    h, _ = O.reference_intersect(V, I, R, technique=IT_BSP_FASTER, threads=8)
+    np.savez_compressed(os.path.join(HERE, "collider_ico100k_wide.npz"),
+                        frequency=scenes.ICOSPHERE_100K_FREQUENCY, mesh_seed=1, ray_seed=3, nrays=len(R),
+                        mesh_sha256=scenes.mesh_digest(V, I), rays_sha256=hashlib.sha256(R.tobytes()).hexdigest(),
+                        tri=h["tri"].astype(np.int32), t=h["t"].astype(np.float32))
     # small mesh with edge-case rays
     v, i = scenes.icosphere(6, seed=7)
     r = edge_rays(v, 1024, seed=8)

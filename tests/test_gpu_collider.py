@@ -102,6 +102,27 @@ def test_gpu_vs_reference_bsp_faster_golden(gpu, oracle):
     np.testing.assert_array_equal(gf[m].astype(np.int32), ref["front"][m])
 
 
+def test_gpu_vs_reference_bsp_faster_100k_rays(gpu, oracle):
+    """100,000 rays of the 100,820-triangle icosphere against the compiled IT_BSP_FASTER:
+    the GPU's hits equal the oracle's bit for bit, and against the reference every ray is
+    identical or in a counted class: shared-edge tie, surface start, plane-node distance
+    (the reference's t is another triangle's plane distance), plane-node miss (the
+    reference returned a farther triangle past a nearly coplanar one)."""
+    V, I, R, ref = parity.wide_golden(GOLDEN)
+    g, gf = _col(V, I).intersect_batch(R)
+    o, _ = oracle.bvh_traverse(oracle.lbvh_build(V, I, leaf_size=2), V, I, R, threads=16)
+    np.testing.assert_array_equal(g["tri"], o["tri"])
+    h = g["tri"] >= 0
+    np.testing.assert_array_equal(g["t"][h], o["t"][h])
+    tab = parity.srlnp_table(V, I)
+    delta = float(oracle.lib().lsgo_delta_bsp(*[np.ascontiguousarray(b).ctypes.data_as(ctypes.c_void_p) for b in tab["box"]]))
+    c = parity.count_classes(ref, g, R, V, I, 10 * delta, tab)
+    print("GPU vs IT_BSP_FASTER, 100k rays:", c, flush=True)
+    assert c["unexplained"] == 0
+    assert c["exact"] + c["shared_edge_ties"] + c["surface_start"] + c["plane_node_distance"] + c["plane_node_miss"] == c["rays"]
+    assert c["exact"] >= 0.999 * c["rays"]
+
+
 @pytest.mark.parametrize("name", ["collider_edge", "collider_planes"])
 def test_gpu_edge_cases(gpu, oracle, name):
     """Axis-aligned rays (zero direction components), rays from vertices and

@@ -113,6 +113,25 @@ def test_srlnp_matches_bsp_faster_ico100k(oracle):
     assert nplane <= 4
 
 
+def test_oracle_traversal_vs_bsp_faster_100k_rays(oracle):
+    """The oracle's tree traversal (equal to its brute force on every other fixture) against
+    the compiled IT_BSP_FASTER on 100,000 rays of the 100,820-triangle icosphere: every ray
+    identical, or in a counted class (shared-edge tie, surface start, a distance the
+    reference took from a plane node's first triangle, a nearly coplanar triangle the
+    reference's plane node missed), none unexplained."""
+    import ctypes
+    V, I, R, ref = parity.wide_golden(GOLDEN)
+    got, _ = oracle.bvh_traverse(oracle.lbvh_build(V, I, leaf_size=2), V, I, R, threads=8)
+    tab = parity.srlnp_table(V, I)
+    delta = float(oracle.lib().lsgo_delta_bsp(*[np.ascontiguousarray(b).ctypes.data_as(ctypes.c_void_p) for b in tab["box"]]))
+    c = parity.count_classes(ref, got, R, V, I, 10 * delta, tab)
+    print("oracle vs IT_BSP_FASTER, 100k rays:", c, flush=True)
+    assert c["unexplained"] == 0
+    assert c["exact"] + c["shared_edge_ties"] + c["surface_start"] + c["plane_node_distance"] + c["plane_node_miss"] == c["rays"]
+    assert c["exact"] >= 0.999 * c["rays"]
+    assert c["plane_node_miss"] <= 5
+
+
 def test_srlnp_vs_bsp_faster_edge_cases(oracle):
     """Rays from vertices/edges, axis-aligned rays, tmin>0, short segments: every
     triangle mismatch is a tie (both triangles are SRLNP hits within DELTA_BSP)
