@@ -437,6 +437,43 @@ int  lsg_bake_download_layer(const lsg_bake* bake, uint32_t object, uint32_t lay
  * parameters' background colour, alpha included */
 int  lsg_bake_download_layer_rgba(const lsg_bake* bake, uint32_t object, uint32_t layer, float* rgba);
 
+/* ---- per-vertex outputs ------------------------------------------------------------------
+ * updateLightmap / updateLightmaps into vertex buffers (lightmap.cpp:766-800, 1095-1125): a
+ * final gather of one texel per triangle whose subtexel is the whole triangle
+ * (gatherPerTrianglePhysical, gather.cpp:843-952), smoothed into the solver's stitched
+ * vertices (updateVertexBufferFromPerTriangleDataPhysical, vertexBuffer.cpp:262-290). */
+typedef struct lsg_vertex_plan lsg_vertex_plan;
+/* Groups mesh vertices as RRMesh::createOptimizedVertices(weld_distance, max_smooth_angle,
+ * 0, NULL) does (RRMeshLessVertices.h:38-230; the solver calls it with
+ * SmoothingParameters::vertexWeldDistance and |maxSmoothAngle|, interpol.cpp:140).  Host
+ * function.  verts: float[nverts*3]; idx: uint32[ntri*3]; corner_normals: float[ntri*9]
+ * (RRMesh::getTriangleNormals); vertex_group: uint32[nverts] out, the group of each vertex,
+ * 0xFFFFFFFF for vertices no triangle uses; group_rep: uint32[nverts] out or NULL, the
+ * vertex whose position stands for each group; *ngroups out. */
+int  lsg_stitch_vertices(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_t ntri, const float* corner_normals,
+                         float weld_distance, float max_smooth_angle, uint32_t* vertex_group, uint32_t* group_rep, uint32_t* ngroups);
+/* The smoothing of per-triangle values into groups for the scene's triangles
+ * (Object::buildTopIVertices, interpol.cpp:125-172, and IVertex::insert, interpol.cpp:51-73).
+ * corner_group: host uint32[ntri*3], the group of each triangle corner (0xFFFFFFFF = none);
+ * group_positions: host float[ngroups*3]. */
+int  lsg_vertex_plan_create(const lsg_scene* scene, const uint32_t* corner_group, const float* group_positions, uint32_t ngroups,
+                            lsg_vertex_plan** out);
+void lsg_vertex_plan_destroy(lsg_vertex_plan* plan);
+/* host copies of the group each corner counts for (0xFFFFFFFF: the solver dropped the
+ * triangle or stitching joined two of its corners) and of its power (corner angle x
+ * triangle area); either may be NULL */
+int  lsg_vertex_plan_corners(const lsg_vertex_plan* plan, uint32_t* corner_group, float* power);
+/* group value = sum(power x value of the corner's triangle) / sum(power), 0 when the sum is 0
+ * (IVertex::getVertexDataFromTriangleData, RRStaticSolver.cpp:130-145).  tri_data: device
+ * float4[ntri] (rgb used); group_out: device float4[ngroups].  Asynchronous on stream. */
+int  lsg_vertex_plan_smooth(const lsg_vertex_plan* plan, const float* tri_data, float* group_out, void* stream);
+/* the per-triangle final gather of the layers in layer_mask (bit i = LSG_LAYER_i) with
+ * params, each smoothed into group_out[i] (device float4[ngroups]).  Asynchronous on stream. */
+int  lsg_vertex_plan_gather(lsg_vertex_plan* plan, const lsg_gather_params* params, uint32_t layer_mask, float* const* group_out,
+                            void* stream);
+/* host copy of the last per-triangle gather of a layer, float4[ntri] (synchronous) */
+int  lsg_vertex_plan_triangle_data(const lsg_vertex_plan* plan, uint32_t layer, float* rgba);
+
 /* ---- learned gather ----------------------------------------------------------------------
  * No counterpart in the reference: a surrogate that replaces most of the 1024-ray
  * final gather.  A cheap gather (few rays) feeds an MLP 32-64-64-64-64-4 (BF16 on

@@ -402,6 +402,115 @@ class SceneBake:
         return maps
 
 
+# ---------------------------------------------------------------------------------------
+# per-vertex outputs (updateLightmap into vertex buffers, lightmap.cpp:766-800)
+
+def stitch_vertices(verts, idx, corner_normals, weld_distance=0.0, max_smooth_angle=0.01):
+    """The solver's vertex stitching (lsg_stitch_vertices, RRMesh::createOptimizedVertices
+    with SmoothingParameters' vertexWeldDistance and |maxSmoothAngle|, defaults 0 and 0.01):
+    (group of every vertex, 0xFFFFFFFF when unused; representative vertex of every group)."""
+    v = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
+    i = np.ascontiguousarray(idx, np.uint32).reshape(-1, 3)
+    n = np.ascontiguousarray(corner_normals, np.float32).reshape(-1, 9)
+    assert len(n) == len(i)
+    group = np.empty(len(v), np.uint32)
+    rep = np.empty(max(len(v), 1), np.uint32)
+    ng = ctypes.c_uint32()
+    N.check(N.lib().lsg_stitch_vertices(v.ctypes.data_as(ctypes.c_void_p), len(v), i.ctypes.data_as(ctypes.c_void_p), len(i),
+                                        n.ctypes.data_as(ctypes.c_void_p), float(weld_distance), float(abs(max_smooth_angle)),
+                                        group.ctypes.data_as(ctypes.c_void_p), rep.ctypes.data_as(ctypes.c_void_p),
+                                        ctypes.byref(ng)), "lsg_stitch_vertices")
+    return group, rep[:ng.value].copy()
+
+
+class VertexPlan:
+    """Per-vertex lighting of a scene (lsg_vertex_plan_*): the final gather of one texel per
+    triangle (gatherPerTrianglePhysical, gather.cpp:843-952) smoothed into the stitched
+    vertices (updateVertexBufferFromPerTriangleDataPhysical, vertexBuffer.cpp:262-290)."""
+
+    def __init__(self, gs: GatherScene, scene: S.BakeScene, weld_distance=0.0, max_smooth_angle=0.01):
+        self.gs, self.scene = gs, scene
+        idx = np.ascontiguousarray(scene.idx, np.uint32)
+        self.vertex_group, self.group_rep = stitch_vertices(scene.verts, idx, scene.normals[idx].reshape(-1, 9),
+                                                            weld_distance, max_smooth_angle)
+        self.ngroups = len(self.group_rep)
+        self.corner_group = np.ascontiguousarray(self.vertex_group[idx].reshape(-1), np.uint32)
+        self.group_pos = np.ascontiguousarray(scene.verts[self.group_rep], np.float32).reshape(-1, 3)
+        h = ctypes.c_void_p()
+        N.check(N.lib().lsg_vertex_plan_create(gs._h, self.corner_group.ctypes.data_as(ctypes.c_void_p),
+                                               self.group_pos.ctypes.data_as(ctypes.c_void_p), self.ngroups,
+                                               ctypes.byref(h)), "lsg_vertex_plan_create")
+        self._h = h.value
+        self._out = {}
+
+    def corners(self):
+        """(group each corner counts for, 0xFFFFFFFF = none; its power = angle x area)"""
+        nc = 3 * self.scene.ntri
+        key = np.empty(max(nc, 1), np.uint32)
+        power = np.empty(max(nc, 1), np.float32)
+        N.check(N.lib().lsg_vertex_plan_corners(self._h, key.ctypes.data_as(ctypes.c_void_p),
+                                                power.ctypes.data_as(ctypes.c_void_p)), "corners")
+        return key[:nc], power[:nc]
+
+    def _buf(self, k):
+        if k not in self._out:
+            self._out[k] = DeviceBuffer(16 * max(self.ngroups, 1))
+        return self._out[k]
+
+    def gather(self, params: GatherParams, layers=(0,), stream=None):
+        """The per-triangle final gather of each layer smoothed into the groups: {layer: [ngroups, 4]}."""
+        outs = (ctypes.c_void_p * 5)()           # one slot per LSG_LAYER_*
+        mask = 0
+        for k in layers:
+            mask |= 1 << k
+            outs[k] = self._buf(k).ptr
+        p = params.c()
+        N.check(N.lib().lsg_vertex_plan_gather(self._h, ctypes.byref(p), mask, outs, stream), "lsg_vertex_plan_gather")
+        N.check(N.lib().lsg_synchronize(), "sync")
+        return {k: self._buf(k).download(np.float32, (self.ngroups, 4)) for k in layers}
+
+    def triangle_data(self, layer=0):
+        out = np.empty((max(self.scene.ntri, 1), 4), np.float32)
+        N.check(N.lib().lsg_vertex_plan_triangle_data(self._h, layer, out.ctypes.data_as(ctypes.c_void_p)), "triangle data")
+        return out[:self.scene.ntri]
+
+    def smooth(self, tri_data):
+        """Smooths host per-triangle values [ntri, 4] into the groups: [ngroups, 4]."""
+        d = DeviceBuffer.from_array(np.ascontiguousarray(tri_data, np.float32).reshape(-1, 4))
+        o = DeviceBuffer(16 * max(self.ngroups, 1))
+        N.check(N.lib().lsg_vertex_plan_smooth(self._h, d.ptr, o.ptr, None), "lsg_vertex_plan_smooth")
+        N.check(N.lib().lsg_synchronize(), "sync")
+        return o.download(np.float32, (self.ngroups, 4))
+
+    def object_vertices(self, k, group_values):
+        """Object k's vertex buffer: (its vertex indices in ascending order, [n, 3] values).  A
+        vertex takes its group's value when a triangle the smoothing counts uses it, else 0
+        (the solver's vertex lookup, vertexBuffer.cpp:21-76)."""
+        ob = self.scene.objects[k]
+        tri = np.arange(ob.tri_begin, ob.tri_begin + ob.tri_count)
+        key, _ = self.corners()
+        corners = (3 * tri[:, None] + np.arange(3)).reshape(-1)
+        verts = self.scene.idx[tri].reshape(-1)
+        ids = np.unique(verts)
+        counted = np.zeros(len(self.scene.verts), bool)
+        counted[verts[key[corners] != 0xFFFFFFFF]] = True
+        vals = np.zeros((len(ids), 3), np.float32)
+        m = counted[ids]
+        vals[m] = group_values[self.vertex_group[ids[m]], :3]
+        return ids, vals
+
+    def close(self):
+        if getattr(self, "_h", None):
+            N.lib().lsg_vertex_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def box_room_scene(lightmap_size=256):
     """The box room: closed 10x10x4 m room + 20 seeded boxes, ~50k triangles, 21 lightmaps."""
     return S.box_room(seed=3, n_boxes=20, target_tris=50_000, lightmap_size=lightmap_size)

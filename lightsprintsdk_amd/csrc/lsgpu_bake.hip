@@ -168,7 +168,7 @@ static int run_all(BakeImpl& B, const lsg_gather_params* first, const lsg_gather
 {
     int e;
     if (first && (e = run_first(B, first, st))) return e;
-    k_zero_f4<<<nblk(B.nslots), 256, 0, st>>>(B.d_rgba, B.nslots);
+    if (B.nslots) k_zero_f4<<<nblk(B.nslots), 256, 0, st>>>(B.d_rgba, B.nslots);   // a plan without lightmaps only gathers the solution
     if ((e = run_gather_final(B, fin, B.d_tex, B.ntex, st))) return e;
     if (fp) {
         if ((e = filter_run(B.d_maps, B.nmaps, B.ftotal, B.d_rgba, B.d_rgbf, *fp, B.S, st, true, fixed_grow, steps_used))) return e;
@@ -381,7 +381,7 @@ int lsg_bake_run_layers(lsg_bake* b, const lsg_gather_params* first, const lsg_g
     int e;
     if (first && (e = run_first(B, first, st))) return e;
     for (int i = 0; i < LSG_NUM_LAYERS; i++)
-        if (rgba[i]) k_zero_f4<<<nblk(B.nslots), 256, 0, st>>>(rgba[i], B.nslots);
+        if (rgba[i] && B.nslots) k_zero_f4<<<nblk(B.nslots), 256, 0, st>>>(rgba[i], B.nslots);
     if ((e = lsg_gather_layers_device(B.scene, B.d_tex, B.ntex, B.d_sub, B.nsub, final_params, (float* const*)rgba, st))) return e;
     if (filtering) {
         for (int i = 0; i < LSG_NUM_LAYERS; i++) {

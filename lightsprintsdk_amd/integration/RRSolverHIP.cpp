@@ -199,26 +199,33 @@ bool RRSolverHIP::environmentSupported(const UpdateParameters& params) const
 	return !getEnvironment(1) && angle0 == 0 && (env->getType() == BT_CUBE_TEXTURE || env->getType() == BT_2D_TEXTURE);
 }
 
-// Texels of the targets' objects in multiObject triangle numbering, as
-// enumerateTexelsPartial walks them (lightmap.cpp:114-143): single post-import
-// triangle -> pre-import index -> multiObject post-import triangle, uv from the
-// material's lightmap texcoord channel.
-unsigned RRSolverHIP::bake(const Target* targets, unsigned numTargets, const UpdateParameters& params,
+void RRSolverHIP::setStaticObjects(const RRObjects& objects, const SmoothingParameters* smoothing, const char* cacheLocation,
+                                   RRCollider::IntersectTechnique intersectTechnique, RRSolver* copyFrom)
+{
+	smoothingParams = smoothing ? *smoothing : SmoothingParameters();
+	RRSolver::setStaticObjects(objects, smoothing, cacheLocation, intersectTechnique, copyFrom);
+}
+
+// Pixel targets: texels of the targets' objects in multiObject triangle numbering, as
+// enumerateTexelsPartial walks them (lightmap.cpp:114-143): single post-import triangle ->
+// pre-import index -> multiObject post-import triangle, uv from the material's lightmap
+// texcoord channel.  Vertex targets: bakeVertices.  The scene is built by the caller.
+unsigned RRSolverHIP::bake(const std::vector<Target>& pixel, const std::vector<Target>& vertex, const UpdateParameters& params,
                            const FilteringParameters* filtering)
 {
-	if (!buildScene(params)) return 0;
 	const RRObjects& objects = getStaticObjects();
 	const RRMesh* multiMesh = getMultiObject()->getCollider()->getMesh();
+	const unsigned numTargets = (unsigned)pixel.size();
 	std::vector<std::vector<float> > uvs(numTargets);
 	std::vector<std::vector<uint32_t> > ids(numTargets);
 	std::vector<lsg_bake_object> bo(numTargets);
 	for (unsigned k = 0; k < numTargets; k++)
 	{
-		const RRObject* single = objects[targets[k].object];
+		const RRObject* single = objects[pixel[k].object];
 		const RRMesh* singleMesh = single->getCollider()->getMesh();
 		for (unsigned st = 0; st < singleMesh->getNumTriangles(); st++)
 		{
-			RRMesh::PreImportNumber pre(targets[k].object, singleMesh->getPreImportTriangle(st).index);
+			RRMesh::PreImportNumber pre(pixel[k].object, singleMesh->getPreImportTriangle(st).index);
 			const unsigned t = multiMesh->getPostImportTriangle(pre);
 			if (t == UINT_MAX) continue;
 			const RRMaterial* m = single->getTriangleMaterial(st, nullptr, nullptr);
@@ -231,20 +238,14 @@ unsigned RRSolverHIP::bake(const Target* targets, unsigned numTargets, const Upd
 		bo[k].tri_ids = ids[k].data();
 		bo[k].ntri = (uint32_t)ids[k].size();
 		const RRBuffer* first = nullptr;
-		for (int l = 0; l < LSG_NUM_LAYERS && !first; l++) first = targets[k].buffer[l];
+		for (int l = 0; l < LSG_NUM_LAYERS && !first; l++) first = pixel[k].buffer[l];
 		bo[k].width = first->getWidth();
 		bo[k].height = first->getHeight();
 	}
 	uint32_t mask = 0;
 	for (unsigned k = 0; k < numTargets; k++)
 		for (int l = 0; l < LSG_NUM_LAYERS; l++)
-			if (targets[k].buffer[l]) mask |= 1u << l;
-	lsg_bake* plan = nullptr;
-	if (lsg_bake_create(scene, bo.data(), numTargets, albedo.data(), emittance.data(), &plan) != LSG_OK)
-	{
-		RRReporter::report(ERRO, "RRSolverHIP: %s\n", lsg_last_error());
-		return 0;
-	}
+			if (pixel[k].buffer[l]) mask |= 1u << l;
 	// UpdateParameters -> gather parameters (gather.cpp:197,396; RRSolver.cpp:333)
 	lsg_gather_params g = {};
 	g.quality = params.quality;
@@ -259,45 +260,57 @@ unsigned RRSolverHIP::bake(const Target* targets, unsigned numTargets, const Upd
 	g.light_multiplier = params.direct.lightMultiplier;
 	g.seed = 1;
 	g.environment_multiplier = params.indirect.environmentMultiplier;
-	// filters with the defaults of FilteringParameters when none are given (lightmap.cpp:852-854);
-	// the GPU fills the background with alpha 0 and the writeback puts the caller's
-	// background there, converted per target (lsg_writeback.h)
-	const FilteringParameters defaults;
-	if (!filtering) filtering = &defaults;
-	lsg_filter_params f = { filtering->smoothingAmount, (float)filtering->spreadForegroundColor, {0, 0, 0, 0},
-	                        filtering->wrap ? 1 : 0 };
-	// GPU radiosity: first gather at half quality (gather.cpp:1083-1088), lights and the
-	// environment only, then the bounces; otherwise the reference solver's exitance is used
-	lsg_gather_params first = g;
-	first.quality = params.quality / 2 ? params.quality / 2 : 1;
-	first.hemisphere = 0;
-	first.light_multiplier = params.indirect.lightMultiplier;
-	first.seed = g.seed ^ 0x5bd1e995u;
-	const lsg_gather_params* firstp = nullptr;
-	if (gpuBounces)
+	unsigned updated = 0;
+	// GPU radiosity, or pixel targets: one bake plan (a plan without lightmaps only computes
+	// the solution the vertex gather reads)
+	if (numTargets || gpuBounces)
 	{
-		if (lsg_bake_set_bounces(plan, gpuBounces, gpuBounceQuality) != LSG_OK)
+		lsg_bake* plan = nullptr;
+		if (lsg_bake_create(scene, bo.data(), numTargets, albedo.data(), emittance.data(), &plan) != LSG_OK)
+		{
+			RRReporter::report(ERRO, "RRSolverHIP: %s\n", lsg_last_error());
+			return 0;
+		}
+		// filters with the defaults of FilteringParameters when none are given (lightmap.cpp:852-854);
+		// the GPU fills the background with alpha 0 and the writeback puts the caller's
+		// background there, converted per target (lsg_writeback.h)
+		const FilteringParameters defaults;
+		if (!filtering) filtering = &defaults;
+		lsg_filter_params f = { filtering->smoothingAmount, (float)filtering->spreadForegroundColor, {0, 0, 0, 0},
+		                        filtering->wrap ? 1 : 0 };
+		// GPU radiosity: first gather at half quality (gather.cpp:1083-1088), lights and the
+		// environment only, then the bounces; otherwise the reference solver's exitance is used
+		lsg_gather_params first = g;
+		first.quality = params.quality / 2 ? params.quality / 2 : 1;
+		first.hemisphere = 0;
+		first.light_multiplier = params.indirect.lightMultiplier;
+		first.seed = g.seed ^ 0x5bd1e995u;
+		const lsg_gather_params* firstp = nullptr;
+		if (gpuBounces)
+		{
+			if (lsg_bake_set_bounces(plan, gpuBounces, gpuBounceQuality) != LSG_OK)
+			{
+				RRReporter::report(ERRO, "RRSolverHIP: %s\n", lsg_last_error());
+				lsg_bake_destroy(plan);
+				return 0;
+			}
+			firstp = &first;
+		}
+		const int st = mask <= 1u ? lsg_bake_run(plan, firstp, &g, numTargets ? &f : nullptr, 0, nullptr)
+		                          : lsg_bake_run_layers(plan, firstp, &g, &f, mask, nullptr);
+		if (st != LSG_OK)
 		{
 			RRReporter::report(ERRO, "RRSolverHIP: %s\n", lsg_last_error());
 			lsg_bake_destroy(plan);
 			return 0;
 		}
-		firstp = &first;
-	}
-	unsigned updated = 0;
-	const int st = mask == 1u ? lsg_bake_run(plan, firstp, &g, &f, 0, nullptr)
-	                          : lsg_bake_run_layers(plan, firstp, &g, &f, mask, nullptr);
-	if (st != LSG_OK)
-		RRReporter::report(ERRO, "RRSolverHIP: %s\n", lsg_last_error());
-	else
-	{
 		// writeback of every layer into the caller's buffers (lightmap.cpp:866-876)
 		std::vector<float> rgba;
 		const unsigned version = getSolutionVersion();
 		for (unsigned k = 0; k < numTargets; k++)
 			for (int l = 0; l < LSG_NUM_LAYERS; l++)
 			{
-				RRBuffer* b = targets[k].buffer[l];
+				RRBuffer* b = pixel[k].buffer[l];
 				if (!b) continue;
 				const size_t n = (size_t)bo[k].width * bo[k].height;
 				rgba.resize(4 * n);
@@ -312,8 +325,109 @@ unsigned RRSolverHIP::bake(const Target* targets, unsigned numTargets, const Upd
 				b->version = version;
 				updated++;
 			}
+		lsg_bake_destroy(plan);
 	}
-	lsg_bake_destroy(plan);
+	if (!vertex.empty()) updated += bakeVertices(vertex, g);
+	return updated;
+}
+
+// Vertex targets: the final gather of one texel per triangle, smoothed on the GPU into the
+// vertices of the reference's own stitching of the multiObject mesh (createOptimizedVertices
+// with the solver's SmoothingParameters, interpol.cpp:140), then written per object vertex
+// through the solver's lookup (each object vertex takes the last counted triangle corner
+// that maps to it, vertexBuffer.cpp:21-76) with the target's colour space, none for bent
+// normals (vertexBuffer.cpp:262-290).
+unsigned RRSolverHIP::bakeVertices(const std::vector<Target>& vertex, const lsg_gather_params& g)
+{
+	const RRObjects& objects = getStaticObjects();
+	const RRMesh* multiMesh = getMultiObject()->getCollider()->getMesh();
+	const unsigned nt = multiMesh->getNumTriangles();
+	const RRMesh* stitched = multiMesh->createOptimizedVertices(smoothingParams.vertexWeldDistance, fabs(smoothingParams.maxSmoothAngle), 0, nullptr);
+	const unsigned ng = stitched->getNumVertices();
+	std::vector<uint32_t> group(3 * (size_t)nt), key(3 * (size_t)nt);
+	std::vector<float> pos(3 * (size_t)ng);
+	for (unsigned t = 0; t < nt; t++)
+	{
+		RRMesh::Triangle tr;
+		stitched->getTriangle(t, tr);
+		for (int k = 0; k < 3; k++) group[3 * (size_t)t + k] = tr[k] < ng ? tr[k] : 0xffffffffu;
+	}
+	for (unsigned u = 0; u < ng; u++)
+	{
+		RRMesh::Vertex p;
+		stitched->getVertex(u, p);
+		for (int c = 0; c < 3; c++) pos[3 * (size_t)u + c] = p[c];
+	}
+	if (stitched != multiMesh) delete stitched;
+	lsg_vertex_plan* plan = nullptr;
+	if (lsg_vertex_plan_create(scene, group.data(), pos.data(), ng, &plan) != LSG_OK)
+	{
+		RRReporter::report(ERRO, "RRSolverHIP: %s\n", lsg_last_error());
+		return 0;
+	}
+	uint32_t mask = 0;
+	for (const Target& t : vertex)
+		for (int l = 0; l < LSG_NUM_LAYERS; l++)
+			if (t.buffer[l]) mask |= 1u << l;
+	float* out[LSG_NUM_LAYERS] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+	std::vector<std::vector<float> > values(LSG_NUM_LAYERS);
+	bool ok = lsg_vertex_plan_corners(plan, key.data(), nullptr) == LSG_OK;
+	for (int l = 0; ok && l < LSG_NUM_LAYERS; l++)
+		if (mask >> l & 1u) ok = lsg_malloc((void**)&out[l], 16 * (size_t)(ng ? ng : 1)) == LSG_OK;
+	ok = ok && lsg_vertex_plan_gather(plan, &g, mask, out, nullptr) == LSG_OK && lsg_synchronize() == LSG_OK;
+	for (int l = 0; ok && l < LSG_NUM_LAYERS; l++)
+		if (out[l])
+		{
+			values[l].resize(4 * (size_t)ng);
+			ok = lsg_memcpy_d2h(values[l].data(), out[l], 16 * (size_t)ng) == LSG_OK;
+		}
+	if (!ok) RRReporter::report(ERRO, "RRSolverHIP: %s\n", lsg_last_error());
+	for (float* p : out) if (p) lsg_free(p);
+	lsg_vertex_plan_destroy(plan);
+	if (!ok) return 0;
+	// object vertex -> counted corner (updateVertexLookupTableDynamicSolver, vertexBuffer.cpp:21-76)
+	std::vector<std::vector<uint32_t> > lookup(objects.size());
+	for (const Target& t : vertex)
+		lookup[t.object].assign(objects[t.object]->getCollider()->getMesh()->getNumVertices(), 0xffffffffu);
+	for (unsigned t = 0; t < nt; t++)
+	{
+		RRMesh::Triangle tr;
+		multiMesh->getTriangle(t, tr);
+		for (unsigned v = 0; v < 3; v++)
+		{
+			if (key[3 * (size_t)t + v] == 0xffffffffu) continue;
+			const RRMesh::PreImportNumber pre = multiMesh->getPreImportVertex(tr[v], t);
+			if (pre.object < lookup.size() && pre.index < lookup[pre.object].size()) lookup[pre.object][pre.index] = 3 * t + v;
+		}
+	}
+	unsigned updated = 0;
+	const unsigned version = getSolutionVersion();
+	for (const Target& t : vertex)
+		for (int l = 0; l < LSG_NUM_LAYERS; l++)
+		{
+			RRBuffer* b = t.buffer[l];
+			if (!b) continue;
+			const std::vector<uint32_t>& lk = lookup[t.object];
+			if (b->getWidth() < lk.size())
+			{
+				RRReporter::report(WARN, "RRSolverHIP: object %d: vertex buffer smaller than the mesh.\n", t.object);
+				continue;
+			}
+			const RRColorSpace* cs = (b->getScaled() && l != LSG_LAYER_BENT_NORMALS) ? getColorSpace() : nullptr;
+			for (unsigned i = 0; i < lk.size(); i++)
+			{
+				RRVec4 c(0);
+				if (lk[i] != 0xffffffffu)
+				{
+					const float* gv = values[l].data() + 4 * (size_t)key[lk[i]];
+					c = RRVec4(gv[0], gv[1], gv[2], 0);
+					if (cs) cs->fromLinear(c);
+				}
+				b->setElement(i, c, nullptr);
+			}
+			b->version = version;
+			updated++;
+		}
 	return updated;
 }
 
@@ -339,37 +453,40 @@ unsigned RRSolverHIP::updateLightmaps(int layerLightmap, int layerDirectionalLig
 	layers[LSG_LAYER_DIRECTION2] = layerDirectionalLightmap + (layerDirectionalLightmap >= 0 ? 1 : 0);
 	layers[LSG_LAYER_DIRECTION3] = layerDirectionalLightmap + (layerDirectionalLightmap >= 0 ? 2 : 0);
 	layers[LSG_LAYER_BENT_NORMALS] = layerBentNormals;
-	std::vector<Target> targets;
+	std::vector<Target> pixel, vertex;
 	const RRObjects& objects = getStaticObjects();
 	for (unsigned i = 0; i < objects.size(); i++)
 	{
-		Target t{i, {nullptr, nullptr, nullptr, nullptr, nullptr}};
-		bool any = false, sizes_ok = true;
+		Target p{i, {nullptr, nullptr, nullptr, nullptr, nullptr}}, v = p;
+		bool anyPixel = false, anyVertex = false, sizes_ok = true;
 		const RRBuffer* first = nullptr;
 		for (int l = 0; l < LSG_NUM_LAYERS; l++)
 		{
 			RRBuffer* b = layers[l] >= 0 ? objects[i]->illumination.getLayer(layers[l]) : nullptr;
 			if (!b) continue;
+			if (b->getType() == BT_VERTEX_BUFFER) { v.buffer[l] = b; anyVertex = true; continue; }
 			if (b->getType() != BT_2D_TEXTURE)
 			{
-				if (!referenceFallback)
-					RRReporter::report(WARN, "RRSolverHIP: object %d: only 2d lightmaps are updated on the GPU.\n", i);
+				RRReporter::report(WARN, "RRSolverHIP: object %d: layer %d is neither a 2d lightmap nor a vertex buffer.\n", i, layers[l]);
 				continue;
 			}
 			if (first && (b->getWidth() != first->getWidth() || b->getHeight() != first->getHeight())) sizes_ok = false;
 			if (!first) first = b;
-			t.buffer[l] = b;
-			any = true;
+			p.buffer[l] = b;
+			anyPixel = true;
 		}
 		if (!sizes_ok)
 		{
 			// lightmap.cpp:741-760: every pixel buffer of an object must have one size
 			RRReporter::report(WARN, "RRSolverHIP: object %d: pixel buffer sizes don't match.\n", i);
-			continue;
+			anyPixel = false;
 		}
-		if (any) targets.push_back(t);
+		if (anyPixel) pixel.push_back(p);
+		if (anyVertex) vertex.push_back(v);
 	}
-	return targets.empty() ? 0 : bake(targets.data(), (unsigned)targets.size(), *params, filtering);
+	if (pixel.empty() && vertex.empty()) return 0;
+	if (!buildScene(*params)) return 0;
+	return bake(pixel, vertex, *params, filtering);
 }
 
 unsigned RRSolverHIP::updateLightmap(int objectNumber, RRBuffer* lightmap, RRBuffer* directionalLightmap[3],
@@ -379,18 +496,21 @@ unsigned RRSolverHIP::updateLightmap(int objectNumber, RRBuffer* lightmap, RRBuf
 	Target t{(unsigned)objectNumber, {lightmap, directionalLightmap ? directionalLightmap[0] : nullptr,
 	                                  directionalLightmap ? directionalLightmap[1] : nullptr,
 	                                  directionalLightmap ? directionalLightmap[2] : nullptr, bentNormals}};
-	bool any = false, flat = true;
+	// updateLightmap takes either pixel buffers of one size or vertex buffers (lightmap.cpp:712-760)
+	bool any = false, pixels = false, vertices = false, sizes_ok = true;
 	const RRBuffer* first = nullptr;
 	for (int l = 0; l < LSG_NUM_LAYERS; l++)
 	{
 		const RRBuffer* b = t.buffer[l];
 		if (!b) continue;
 		any = true;
-		if (b->getType() != BT_2D_TEXTURE) flat = false;
-		else if (first && (b->getWidth() != first->getWidth() || b->getHeight() != first->getHeight())) flat = false;
+		if (b->getType() == BT_VERTEX_BUFFER) { vertices = true; continue; }
+		if (b->getType() != BT_2D_TEXTURE) sizes_ok = false;
+		else if (first && (b->getWidth() != first->getWidth() || b->getHeight() != first->getHeight())) sizes_ok = false;
 		if (!first) first = b;
+		pixels = true;
 	}
-	const bool supported = params && params->quality > 0 && any && flat
+	const bool supported = params && params->quality > 0 && any && sizes_ok && !(pixels && vertices)
 	                    && objectNumber >= 0 && (unsigned)objectNumber < objects.size()
 	                    && environmentSupported(*params) && lsg_device_count() > 0;
 	if (!supported)
@@ -401,7 +521,9 @@ unsigned RRSolverHIP::updateLightmap(int objectNumber, RRBuffer* lightmap, RRBuf
 	}
 	if (!params->useCurrentSolution && !gpuBounces)
 		RRSolver::updateLightmaps(-1, -1, -1, params, nullptr);
-	return bake(&t, 1, *params, filtering);
+	if (!buildScene(*params)) return 0;
+	const std::vector<Target> none, one{t};
+	return vertices ? bake(none, one, *params, filtering) : bake(one, none, *params, filtering);
 }
 
 } // namespace rr_hip

@@ -19,7 +19,16 @@
 // lightmap.cpp:968-973) come from the layered gather (lsg_bake_run_layers) in the same
 // way, each layer filtered on its own.
 //
-// Supported: per-pixel buffers (BT_2D_TEXTURE) in any of the five layers, point
+// Per-vertex buffers (BT_VERTEX_BUFFER) in any of the five layers come from the same
+// final gather with one texel per triangle (gatherPerTrianglePhysical, gather.cpp:843-952),
+// smoothed on the GPU into the vertices the reference's own stitching gives
+// (createOptimizedVertices with the SmoothingParameters of setStaticObjects, as the solver's
+// buildTopIVertices, interpol.cpp:125-172) and written per object vertex
+// (updateVertexBufferFromPerTriangleDataPhysical, vertexBuffer.cpp:262-290).  Requests of
+// quality 0 for vertex buffers read the solver directly (updateVertexBufferFromSolver) and
+// stay with the reference.
+//
+// Supported: per-pixel and per-vertex buffers in any of the five layers, point
 // (REALISTIC attenuation) and directional lights, Lambertian triangle materials, and
 // an environment that is one cube map or panorama without rotation or blending.
 // Anything else is refused with an RRReporter error and 0 buffers updated;
@@ -45,6 +54,11 @@ public:
 	                        rr::RRBuffer* bentNormals, const UpdateParameters* params,
 	                        const FilteringParameters* filtering = nullptr) override;
 
+	// RRSolver.h:239: also keeps the smoothing parameters, which per-vertex buffers use
+	void setStaticObjects(const rr::RRObjects& objects, const SmoothingParameters* smoothing, const char* cacheLocation = nullptr,
+	                      rr::RRCollider::IntersectTechnique intersectTechnique = rr::RRCollider::IT_BVH_FAST,
+	                      rr::RRSolver* copyFrom = nullptr) override;
+
 	// unsupported inputs go to the reference implementation instead of failing
 	void setReferenceFallback(bool enable) { referenceFallback = enable; }
 
@@ -56,13 +70,16 @@ public:
 private:
 	// one object's output buffers by layer (LSG_LAYER_*), nullptr = not wanted
 	struct Target { unsigned object; rr::RRBuffer* buffer[LSG_NUM_LAYERS]; };
-	unsigned bake(const Target* targets, unsigned numTargets, const UpdateParameters& params,
+	// pixel targets through one bake plan, vertex targets through one vertex plan
+	unsigned bake(const std::vector<Target>& pixel, const std::vector<Target>& vertex, const UpdateParameters& params,
 	              const FilteringParameters* filtering);
+	unsigned bakeVertices(const std::vector<Target>& vertex, const lsg_gather_params& g);
 	bool buildScene(const UpdateParameters& params);
 	bool environmentSupported(const UpdateParameters& params) const;
 	void releaseScene();
 
 	bool referenceFallback = false;
+	SmoothingParameters smoothingParams;
 	unsigned gpuBounces = 0, gpuBounceQuality = 0;
 	lsg_collider* col = nullptr;
 	lsg_scene* scene = nullptr;

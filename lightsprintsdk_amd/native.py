@@ -125,6 +125,13 @@ SIGNATURES = {
     "lsg_bake_run_layers": (ctypes.c_int, [P, ctypes.POINTER(GatherParams), ctypes.POINTER(GatherParams), ctypes.POINTER(FilterParams), U32, P]),
     "lsg_bake_download_layer": (ctypes.c_int, [P, U32, U32, P]),
     "lsg_bake_download_layer_rgba": (ctypes.c_int, [P, U32, U32, P]),
+    "lsg_stitch_vertices": (ctypes.c_int, [P, U32, P, U32, P, ctypes.c_float, ctypes.c_float, P, P, P]),
+    "lsg_vertex_plan_create": (ctypes.c_int, [P, P, P, U32, P]),
+    "lsg_vertex_plan_destroy": (None, [P]),
+    "lsg_vertex_plan_corners": (ctypes.c_int, [P, P, P]),
+    "lsg_vertex_plan_smooth": (ctypes.c_int, [P, P, P, P]),
+    "lsg_vertex_plan_gather": (ctypes.c_int, [P, P, U32, P, P]),
+    "lsg_vertex_plan_triangle_data": (ctypes.c_int, [P, U32, P]),
     "lsg_mlp_num_params": (U32, []),
     "lsg_mlp_create": (ctypes.c_int, [U32, ctypes.POINTER(P)]),
     "lsg_mlp_destroy": (None, [P]),

@@ -23,6 +23,10 @@
  *       uniform-exitance enclosure, emitter form factor); reference bake is
  *       unbuildable here (needs boost), so bit-level parity with
  *       processTexel itself is "parity unpinned" beyond those closed forms.
+ *   lsgo_vertex_corners / lsgo_smooth_vertices -- pinned: closed forms of
+ *       the angle x area weights (equal fans, a right-angle corner, constant and
+ *       linear fields) with planted weight mistakes; the grouping they take is
+ *       the compiled reference's createOptimizedVertices (oracle/_ref/stitch_ref).
  */
 #ifndef LSG_ORACLE_H
 #define LSG_ORACLE_H

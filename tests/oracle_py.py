@@ -19,6 +19,19 @@ ORACLE_DIR = os.path.join(ROOT, "oracle")
 ORACLE_LIB = os.path.join(ORACLE_DIR, "liblsg_oracle.so")
 REF_BIN = os.path.join(ORACLE_DIR, "_ref", "ref_collider")
 
+
+def _sources():
+    """The oracle's C files, as oracle/Makefile lists them (planted-mistake tests rebuild
+    all of them with one file changed)."""
+    with open(os.path.join(ORACLE_DIR, "Makefile")) as f:
+        for line in f:
+            if line.startswith("ORACLE_SRC"):
+                return line.split(":=", 1)[1].split()
+    raise RuntimeError("ORACLE_SRC not found in oracle/Makefile")
+
+
+SOURCES = _sources()
+
 HIT_DTYPE = np.dtype([("tri", "<i4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("p", "<f4", 3),
                       ("plane", "<f4", 4), ("front", "<i4")])
 TRI_DTYPE = np.dtype([("n", "<f4", 4), ("s", "<f4", 3), ("ireal", "<f4"), ("r", "<f4", 3),
@@ -113,6 +126,10 @@ def lib():
         L.lsgo_subtexel_walk.restype = None
         L.lsgo_shots_before.argtypes = [U, U, U]
         L.lsgo_shots_before.restype = U
+        L.lsgo_vertex_corners.argtypes = [P, P, U, P, P, P, P]
+        L.lsgo_vertex_corners.restype = None
+        L.lsgo_smooth_vertices.argtypes = [P, P, U, P, U, P]
+        L.lsgo_smooth_vertices.restype = None
         _lib = L
     return _lib
 
@@ -288,6 +305,61 @@ def rasterize(tri_uv, tri_ids, width, height, object_id=0, out_base=0):
                          ctypes.byref(nt), ctypes.byref(ns))
     assert r == 0
     return tex, sub
+
+
+def vertex_corners(verts, idx, corner_group, group_pos, library=None):
+    """(key, power) of every triangle corner: the group it counts for (0xFFFFFFFF = none)
+    and its power = corner angle x triangle area.  `library`: a planted mutation."""
+    L = lib() if library is None else library
+    if library is not None:
+        L.lsgo_vertex_corners.argtypes = [ctypes.c_void_p] * 2 + [ctypes.c_uint32] + [ctypes.c_void_p] * 4
+    v, i = _mesh(verts, idx)
+    g = np.ascontiguousarray(corner_group, np.uint32).reshape(-1)
+    gp = np.ascontiguousarray(group_pos, np.float32).reshape(-1, 3)
+    key = np.zeros(3 * len(i), np.uint32)
+    power = np.zeros(3 * len(i), np.float32)
+    L.lsgo_vertex_corners(_p(v), _p(i), len(i), _p(g), _p(gp), _p(key), _p(power))
+    return key, power
+
+
+def smooth_vertices(key, power, tri_data, ngroups, library=None):
+    """[ngroups, 4] smoothed values of per-triangle data [ntri, 4]."""
+    L = lib() if library is None else library
+    if library is not None:
+        L.lsgo_smooth_vertices.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
+                                           ctypes.c_uint32, ctypes.c_void_p]
+    k = np.ascontiguousarray(key, np.uint32)
+    pw = np.ascontiguousarray(power, np.float32)
+    d = np.ascontiguousarray(tri_data, np.float32).reshape(-1, 4)
+    out = np.zeros((max(ngroups, 1), 4), np.float32)
+    L.lsgo_smooth_vertices(_p(k), _p(pw), len(d), _p(d), ngroups, _p(out))
+    return out[:ngroups]
+
+
+STITCH_BIN = os.path.join(ORACLE_DIR, "_ref", "stitch_ref")
+
+
+def have_stitch_ref() -> bool:
+    return os.path.exists(STITCH_BIN)
+
+
+def reference_stitch(verts, idx, corner_normals, weld, angle):
+    """The compiled reference's createOptimizedVertices (oracle/_ref/stitch_ref): (stitched
+    vertex of every corner [ntri*3], stitched vertex positions [n, 3])."""
+    v, i = _mesh(verts, idx)
+    n = np.ascontiguousarray(corner_normals, np.float32).reshape(-1, 9)
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(np.array([0x48435453, len(v), len(i)], np.uint32).tobytes())
+            f.write(np.array([weld, angle], np.float32).tobytes())
+            f.write(v.tobytes()); f.write(i.tobytes()); f.write(n.tobytes())
+        subprocess.run([STITCH_BIN, fin, fout], check=True)
+        raw = np.fromfile(fout, np.uint8)
+    ns = int(raw[:4].view(np.uint32)[0])
+    corners = raw[4:4 + 12 * len(i)].view(np.uint32).copy()
+    pos = raw[4 + 12 * len(i):].view(np.float32).reshape(ns, 3).copy()
+    return corners, pos
 
 
 def subtexel_walk(areas, series):

@@ -62,7 +62,7 @@ MUTATIONS = [
 @pytest.mark.parametrize("k", range(len(MUTATIONS)))
 def test_planted_raster_mistakes_fail(oracle, tmp_path, k):
     old, new = MUTATIONS[k]
-    srcs = ["lsg_oracle.c", "lsg_oracle_gather.c", "lsg_oracle_filters.c", "lsg_oracle_mlp.c", "lsg_oracle_raster.c"]
+    srcs = list(oracle.SOURCES)
     for f in srcs:
         txt = open(os.path.join(ORACLE, f)).read()
         if f == "lsg_oracle_raster.c":

@@ -125,9 +125,9 @@ MUTATIONS = [
 
 @pytest.mark.skipif(shutil.which("gcc") is None, reason="needs gcc to build the planted copies")
 @pytest.mark.parametrize("k", range(len(MUTATIONS)))
-def test_planted_walk_mistakes_fail(tmp_path, k):
+def test_planted_walk_mistakes_fail(oracle, tmp_path, k):
     old, new = MUTATIONS[k]
-    srcs = ["lsg_oracle.c", "lsg_oracle_gather.c", "lsg_oracle_filters.c", "lsg_oracle_mlp.c"]
+    srcs = list(oracle.SOURCES)
     for f in srcs:
         txt = open(os.path.join(ORACLE, f)).read()
         if f == "lsg_oracle_gather.c":
