@@ -111,6 +111,35 @@ int  lsg_collider_create(const float* verts, uint32_t nverts, const uint32_t* in
                          const lsg_build_params* params, const volatile int* aborting, lsg_collider** out);
 void lsg_collider_destroy(lsg_collider* collider);
 int  lsg_collider_info_get(const lsg_collider* collider, lsg_collider_info* info);
+
+/* ---- collider cache ------------------------------------------------------------------------
+ * The reference caches built trees on disk under a name made from the mesh's hash
+ * (IntersectBsp.h:213-291, TREE_VERSION 4, files of MIN_BYTES_FOR_SAVE = 1024 bytes or
+ * more).  This library's tree files hold the BVH2, the 4-wide tree and the triangles' leaf
+ * order, under the same naming scheme with its own format version and extension. */
+#define LSG_TREE_VERSION 1u
+/* SHA-1 of the mesh as RRMesh::getHash computes it (RRHash.cpp:48-77): every vertex as
+ * three binary32 coordinates, then every triangle as three uint32 indices; hash: 20 bytes */
+int  lsg_mesh_hash(const float* verts, uint32_t nverts, const uint32_t* indices, uint32_t ntri, uint8_t* hash);
+/* RRHash::getFileName (RRHash.cpp:198-226): location (NULL = the system temp directory's
+ * "Lightsprint/", created) + 32 letters of the hash with version added to its first
+ * little-endian word + extension; LSG_ERR_LIMIT when it does not fit capacity bytes */
+int  lsg_cache_file_name(const uint8_t* hash, uint32_t version, const char* location, const char* extension, char* out,
+                         uint32_t capacity);
+/* writes a built collider to path (synchronous; the file appears complete or not at all) */
+int  lsg_collider_save(const lsg_collider* collider, const char* path);
+/* a collider from a tree file of this mesh: LSG_ERR_INVALID when the file is missing, of
+ * another mesh or format version, or damaged */
+int  lsg_collider_load(const float* verts, uint32_t nverts, const uint32_t* indices, uint32_t ntri, const char* path,
+                       lsg_collider** out);
+/* RRCollider::create with a cacheLocation (RRCollider.h:179-183): loads the tree file
+ * cache_location + name + ".lsgbvh" when it holds this mesh built with the same leaf size
+ * (any, when params ask for the default) and flags, else builds and writes it (trees of
+ * at least 1024 bytes; a failed write is ignored).  cache_location NULL = the temp
+ * directory, "*" = no caching.  *from_cache (may be NULL) = 1 when loaded. */
+int  lsg_collider_create_cached(const float* verts, uint32_t nverts, const uint32_t* indices, uint32_t ntri,
+                                const lsg_build_params* params, const volatile int* aborting, const char* cache_location,
+                                lsg_collider** out, int32_t* from_cache);
 /* copies the tree to the host: nodes = 16 floats per node slot, prim = original
  * triangle index per leaf-order slot (num_valid). Either may be NULL. */
 int  lsg_collider_download(const lsg_collider* collider, float* nodes, uint32_t* prim);
@@ -436,6 +465,30 @@ int  lsg_bake_download_layer(const lsg_bake* bake, uint32_t object, uint32_t lay
  * buffer with alpha (RRBuffer.cpp:105-131); background texels carry the filter
  * parameters' background colour, alpha included */
 int  lsg_bake_download_layer_rgba(const lsg_bake* bake, uint32_t object, uint32_t layer, float* rgba);
+
+/* ---- consumer formats -----------------------------------------------------------------------
+ * A filtered RGBA lightmap (lsg_bake_download_layer_rgba's texels, on the device) written in
+ * a consumer buffer's format, as updateLightmap's writeback does through the buffer interface
+ * (lightmap.cpp:866-876: lightmapFillBackground, copyElementsTo, RRBufferInMemory::setElement,
+ * RRBufferInMemory.cpp:229-285).  Format numbers are RRBufferFormat's (RRBuffer.h:89-102). */
+#define LSG_FORMAT_RGB         0   /* BF_RGB: 3 bytes, RR_FLOAT2BYTE per channel */
+#define LSG_FORMAT_BGR         1   /* BF_BGR */
+#define LSG_FORMAT_RGBA        2   /* BF_RGBA: 4 bytes */
+#define LSG_FORMAT_RGBF        3   /* BF_RGBF: 3 floats */
+#define LSG_FORMAT_RGBAF       4   /* BF_RGBAF: 4 floats */
+#define LSG_FORMAT_LUMINANCE   9   /* BF_LUMINANCE: 1 byte of the rgb average */
+#define LSG_FORMAT_LUMINANCEF 10   /* BF_LUMINANCEF: 1 float */
+/* bytes per texel of a format, 0 for formats without a per-texel encoding */
+uint32_t lsg_format_bytes(int32_t format);
+/* rgba: device float4[ntexels], linear, background texels with alpha <= 0; out: device,
+ * ntexels * lsg_format_bytes(format) bytes, 4-byte aligned for the byte formats.
+ * scaled: the target buffer is scaled (RRBuffer::getScaled); solver_gamma: the power of the
+ * solver's sRGB colour space (RRColorSpace::create_sRGB, 0.45 by default; 0 = no colour
+ * space), which makes the background linear for a scaled target; copy_gamma: the colour
+ * space the copy converts to (the solver's, or 0 for bent normals).  background: 4 floats.
+ * Asynchronous on stream. */
+int  lsg_lightmap_encode(const float* rgba, uint32_t ntexels, int32_t format, int32_t scaled, float solver_gamma, float copy_gamma,
+                         const float* background, void* out, void* stream);
 
 /* ---- per-vertex outputs ------------------------------------------------------------------
  * updateLightmap / updateLightmaps into vertex buffers (lightmap.cpp:766-800, 1095-1125): a

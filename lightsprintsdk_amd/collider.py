@@ -117,6 +117,26 @@ class DeviceRays:
         return h, f
 
 
+def mesh_hash(verts: np.ndarray, idx: np.ndarray) -> bytes:
+    """RRMesh::getHash's SHA-1 of a mesh (lsg_mesh_hash)."""
+    v = np.ascontiguousarray(verts, dtype=np.float32)
+    i = np.ascontiguousarray(idx, dtype=np.uint32)
+    out = (ctypes.c_uint8 * 20)()
+    N.check(N.lib().lsg_mesh_hash(v.ctypes.data_as(ctypes.c_void_p), len(v), i.ctypes.data_as(ctypes.c_void_p), len(i), out),
+            "lsg_mesh_hash")
+    return bytes(out)
+
+
+def cache_file_name(hash20: bytes, version: int, location=None, extension=None) -> str:
+    """RRHash::getFileName of a hash (lsg_cache_file_name)."""
+    h = (ctypes.c_uint8 * 20).from_buffer_copy(hash20)
+    out = ctypes.create_string_buffer(4096)
+    N.check(N.lib().lsg_cache_file_name(h, version, None if location is None else str(location).encode(),
+                                        None if extension is None else str(extension).encode(), out, 4096),
+            "lsg_cache_file_name")
+    return out.value.decode()
+
+
 class RRCollider:
     """Collider built by liblsgpu (technique IT_HIP_LBVH)."""
 
@@ -127,9 +147,12 @@ class RRCollider:
 
     @staticmethod
     def create(verts: np.ndarray, idx: np.ndarray, technique: int = IT_HIP_LBVH, aborting=None,
-               cacheLocation=None, buildParams=None, leaf_size: int = 0, lbvh_only: bool = False):
+               cacheLocation="*", buildParams=None, leaf_size: int = 0, lbvh_only: bool = False):
         """RRCollider::create for a mesh given as float32 [nv,3] / uint32 [nt,3].  lbvh_only keeps
-        the LBVH's inner nodes instead of rebuilding them by SAH."""
+        the LBVH's inner nodes instead of rebuilding them by SAH.  cacheLocation: a directory
+        prefix for tree files (lsg_collider_create_cached), None for the temp directory's
+        "Lightsprint/" as the reference's default; "*" (this binding's default) builds without
+        a cache.  from_cache tells whether the tree came from a file."""
         v = np.ascontiguousarray(verts, dtype=np.float32)
         i = np.ascontiguousarray(idx, dtype=np.uint32)
         h = ctypes.c_void_p()
@@ -137,10 +160,33 @@ class RRCollider:
         ab = None
         if aborting is not None:
             ab = ctypes.cast(ctypes.pointer(aborting), ctypes.c_void_p)
-        code = N.lib().lsg_collider_create(v.ctypes.data_as(ctypes.c_void_p), len(v), i.ctypes.data_as(ctypes.c_void_p),
-                                           len(i), ctypes.byref(bp), ab, ctypes.byref(h))
+        fc = ctypes.c_int32(0)
+        if cacheLocation == "*":
+            code = N.lib().lsg_collider_create(v.ctypes.data_as(ctypes.c_void_p), len(v), i.ctypes.data_as(ctypes.c_void_p),
+                                               len(i), ctypes.byref(bp), ab, ctypes.byref(h))
+        else:
+            loc = None if cacheLocation is None else str(cacheLocation).encode()
+            code = N.lib().lsg_collider_create_cached(v.ctypes.data_as(ctypes.c_void_p), len(v), i.ctypes.data_as(ctypes.c_void_p),
+                                                      len(i), ctypes.byref(bp), ab, loc, ctypes.byref(h), ctypes.byref(fc))
         N.check(code, "lsg_collider_create")
-        return RRCollider(h.value, v, i)
+        col = RRCollider(h.value, v, i)
+        col.from_cache = bool(fc.value)
+        return col
+
+    @staticmethod
+    def load(verts: np.ndarray, idx: np.ndarray, path: str):
+        """A collider from a tree file written by save() for the same mesh (lsg_collider_load)."""
+        v = np.ascontiguousarray(verts, dtype=np.float32)
+        i = np.ascontiguousarray(idx, dtype=np.uint32)
+        h = ctypes.c_void_p()
+        N.check(N.lib().lsg_collider_load(v.ctypes.data_as(ctypes.c_void_p), len(v), i.ctypes.data_as(ctypes.c_void_p), len(i),
+                                          str(path).encode(), ctypes.byref(h)), "lsg_collider_load")
+        col = RRCollider(h.value, v, i)
+        col.from_cache = True
+        return col
+
+    def save(self, path: str):
+        N.check(N.lib().lsg_collider_save(self._h, str(path).encode()), "lsg_collider_save")
 
     def info(self) -> N.ColliderInfo:
         inf = N.ColliderInfo()

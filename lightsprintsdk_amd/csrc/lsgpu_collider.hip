@@ -650,11 +650,243 @@ static int build(ColliderImpl& C, const float* verts, uint32_t nv, const uint32_
     return LSG_OK;
 }
 
+// ---- tree files --------------------------------------------------------------------------
+// A built collider on disk: header, then the leaf order of the valid triangles (uint32 each),
+// the BVH2 nodes (64 B each) and the BVH4 nodes (128 B each).  Loading uploads the mesh,
+// repeats the per-triangle setup and writes the triangles in the stored leaf order, and
+// uploads both trees: the Morton sort, Karras tree, refit, SAH rebuild and collapse are
+// skipped.  The header holds the mesh's SHA-1 (lsg_mesh_hash) and a 64-bit checksum of the
+// payload, so a file of another mesh, an older format or a damaged file is refused.
+struct TreeFileHeader {
+    char magic[8];
+    uint32_t version, header_bytes;
+    uint8_t mesh_hash[20];
+    uint64_t payload_sum;
+    uint32_t nverts, ntri, leaf, flags, nvalid, nnodes, nnodes4, depth, stack_need;
+    int32_t root, root4;
+    float bmin[3], bmax[3], margin;
+    uint64_t payload_bytes;
+};
+static const char kTreeMagic[8] = {'L', 'S', 'G', 'B', 'V', 'H', '4', 0};
+
+// checksum of the payload: four interleaved FNV-1a style lanes over 64-bit words, folded
+// (a damaged or truncated file is caught without hashing hundreds of MB cryptographically)
+static uint64_t payload_checksum(const uint8_t* p, size_t n)
+{
+    const uint64_t prime = 0x100000001b3ull;
+    uint64_t h[4] = {0xcbf29ce484222325ull, 0x84222325cbf29ce4ull, 0x9e3779b97f4a7c15ull, 0xc2b2ae3d27d4eb4full};
+    size_t i = 0;
+    for (; i + 32 <= n; i += 32)
+        for (int k = 0; k < 4; k++) {
+            uint64_t w;
+            memcpy(&w, p + i + 8 * k, 8);
+            h[k] = (h[k] ^ w) * prime;
+        }
+    uint64_t r = h[0] ^ (h[1] * 3) ^ (h[2] * 5) ^ (h[3] * 7);
+    for (; i < n; i++) r = (r ^ p[i]) * prime;
+    return (r ^ (uint64_t)n) * prime;
+}
+
+static int load_tree(ColliderImpl& C, const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt, const TreeFileHeader& h,
+                     const uint32_t* order, const Node* nodes, const Node4* nodes4)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    int st;
+    C.ntri = nt; C.nverts = nv; C.leaf = h.leaf;
+    memcpy(C.bmin, h.bmin, sizeof C.bmin); memcpy(C.bmax, h.bmax, sizeof C.bmax); C.margin = h.margin;
+    if ((st = dalloc(&C.d_verts, 3 * (size_t)nv, &C.bytes))) return st;
+    if ((st = dalloc(&C.d_idx, 3 * (size_t)nt, &C.bytes))) return st;
+    LSG_HIP(hipMemcpy(C.d_verts, verts, 12 * (size_t)nv, hipMemcpyHostToDevice));
+    LSG_HIP(hipMemcpy(C.d_idx, idx, 12 * (size_t)nt, hipMemcpyHostToDevice));
+    uint32_t *d_keys = nullptr, *d_vals = nullptr, *d_bytes = nullptr, *d_nvalid = nullptr, *d_order = nullptr;
+    float4 *d_plane_u = nullptr, *d_sv_u = nullptr, *d_ev_u = nullptr, *d_blo_u = nullptr, *d_bhi_u = nullptr, *d_plo = nullptr, *d_phi = nullptr;
+    auto cleanup = [&]() {
+        dfree(d_keys); dfree(d_vals); dfree(d_bytes); dfree(d_nvalid); dfree(d_order);
+        dfree(d_plane_u); dfree(d_sv_u); dfree(d_ev_u); dfree(d_blo_u); dfree(d_bhi_u); dfree(d_plo); dfree(d_phi);
+    };
+#define CK(x) do { if ((st = (x))) { cleanup(); return st; } } while (0)
+#define CKH(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { cleanup(); return hip_fail(e_, #x, __LINE__); } } while (0)
+    const size_t N = nt ? nt : 1;
+    CK(dalloc(&d_keys, N)); CK(dalloc(&d_vals, N)); CK(dalloc(&d_bytes, N)); CK(dalloc(&d_nvalid, 1));
+    CK(dalloc(&d_plane_u, N)); CK(dalloc(&d_sv_u, N)); CK(dalloc(&d_ev_u, N)); CK(dalloc(&d_blo_u, N)); CK(dalloc(&d_bhi_u, N));
+    CKH(hipMemset(d_nvalid, 0, 4));
+    if (nt)
+        k_tri_setup<<<blocks(nt), 256>>>(C.d_verts, C.d_idx, nt, make_float3(h.bmin[0], h.bmin[1], h.bmin[2]), make_float3(0.f, 0.f, 0.f),
+                                         d_keys, d_vals, d_plane_u, d_sv_u, d_ev_u, d_bytes, d_blo_u, d_bhi_u, d_nvalid);
+    CKH(hipGetLastError());
+    uint32_t nvalid = 0;
+    CKH(hipMemcpy(&nvalid, d_nvalid, 4, hipMemcpyDeviceToHost));
+    if (nvalid != h.nvalid) { cleanup(); return fail(LSG_ERR_INVALID, "tree file: %u valid triangles stored, %u in the mesh", h.nvalid, nvalid); }
+    const uint32_t n = nvalid;
+    C.nvalid = n;
+    CK(dalloc(&d_order, n)); CK(dalloc(&d_plo, n)); CK(dalloc(&d_phi, n));
+    if (n) CKH(hipMemcpy(d_order, order, 4 * (size_t)n, hipMemcpyHostToDevice));
+    CK(dalloc(&C.d_plane, n, &C.bytes)); CK(dalloc(&C.d_rest, n, &C.bytes));
+    CK(dalloc(&C.d_slot, nt, &C.bytes));
+    CKH(hipMemset(C.d_slot, 0, sizeof(uint32_t) * N));
+    if (n) k_write_tris<<<blocks(n), 256>>>(n, d_order, d_plane_u, d_sv_u, d_ev_u, d_bytes, d_blo_u, d_bhi_u,
+                                            C.d_plane, C.d_rest, d_plo, d_phi, C.d_slot);
+    CKH(hipGetLastError());
+    C.nnodes = h.nnodes; C.root = h.root; C.depth = h.depth;
+    CK(dalloc(&C.d_nodes, C.nnodes ? C.nnodes : 1, &C.bytes));
+    if (C.nnodes) CKH(hipMemcpy(C.d_nodes, nodes, sizeof(Node) * (size_t)C.nnodes, hipMemcpyHostToDevice));
+    C.nnodes4 = h.nnodes4; C.root4 = h.root4; C.stack_need = h.stack_need;
+    CK(dalloc(&C.d_nodes4, C.nnodes4 ? C.nnodes4 : 1, &C.bytes));
+    if (C.nnodes4) CKH(hipMemcpy(C.d_nodes4, nodes4, sizeof(Node4) * (size_t)C.nnodes4, hipMemcpyHostToDevice));
+    CKH(hipDeviceSynchronize());
+    cleanup();
+#undef CK
+#undef CKH
+    C.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return LSG_OK;
+}
+
+// reads and checks a tree file for the mesh; 0 = header and payload ready
+static int read_tree_file(const char* path, const uint8_t mesh_hash[20], uint32_t nv, uint32_t nt, TreeFileHeader& h,
+                          std::vector<uint8_t>& payload)
+{
+    FILE* f = fopen(path, "rb");
+    if (!f) return fail(LSG_ERR_INVALID, "tree file %s: cannot open", path);
+    const bool head = fread(&h, sizeof h, 1, f) == 1;
+    bool ok = head && !memcmp(h.magic, kTreeMagic, 8) && h.version == LSG_TREE_VERSION && h.header_bytes == sizeof h;
+    if (ok && (memcmp(h.mesh_hash, mesh_hash, 20) || h.nverts != nv || h.ntri != nt)) {
+        fclose(f);
+        return fail(LSG_ERR_INVALID, "tree file %s: built for another mesh", path);
+    }
+    const uint64_t want = 4ull * h.nvalid + sizeof(Node) * (uint64_t)h.nnodes + sizeof(Node4) * (uint64_t)h.nnodes4;
+    ok = ok && h.payload_bytes == want && h.nvalid <= nt;
+    if (ok) {
+        payload.resize(want);
+        ok = !want || fread(payload.data(), 1, want, f) == want;
+        uint8_t extra;
+        ok = ok && fread(&extra, 1, 1, f) == 0;      // nothing after the payload
+    }
+    fclose(f);
+    if (!ok) return fail(LSG_ERR_INVALID, "tree file %s: not a tree file of format %u, or truncated", path, LSG_TREE_VERSION);
+    if (payload_checksum(payload.data(), payload.size()) != h.payload_sum)
+        return fail(LSG_ERR_INVALID, "tree file %s: damaged (payload checksum)", path);
+    return LSG_OK;
+}
+
+static int load_from_file(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt, const uint8_t hash[20], const char* path,
+                          uint32_t want_leaf, uint32_t want_flags, lsg_collider** out)
+{
+    TreeFileHeader h;
+    std::vector<uint8_t> payload;
+    int st = read_tree_file(path, hash, nv, nt, h, payload);
+    if (st) return st;
+    if ((want_leaf && h.leaf != want_leaf) || h.flags != want_flags)
+        return fail(LSG_ERR_INVALID, "tree file %s: built with leaf size %u flags %u", path, h.leaf, h.flags);
+    if (!stack_size_for_depth(h.depth) || h.stack_need + 1 > (uint32_t)stack_size_for_depth(h.depth))
+        return fail(LSG_ERR_INVALID, "tree file %s: depth %u does not fit the traversal stack", path, h.depth);
+    const uint32_t* order = (const uint32_t*)payload.data();
+    for (uint32_t i = 0; i < h.nvalid; i++)
+        if (order[i] >= nt) return fail(LSG_ERR_INVALID, "tree file %s: leaf order names triangle %u", path, order[i]);
+    const Node* nodes = (const Node*)(payload.data() + 4ull * h.nvalid);
+    const Node4* nodes4 = (const Node4*)(payload.data() + 4ull * h.nvalid + sizeof(Node) * (uint64_t)h.nnodes);
+    lsg_collider* c = new (std::nothrow) lsg_collider;
+    if (!c) return fail(LSG_ERR_OOM, "lsg_collider_load: host allocation failed");
+    if ((st = load_tree(c->impl, verts, nv, idx, nt, h, order, nodes, nodes4))) { free_impl(c->impl); delete c; return st; }
+    *out = c;
+    return LSG_OK;
+}
+
+static int save_to_file(const lsg_collider* c, const char* path, uint32_t flags, uint64_t min_bytes)
+{
+    const ColliderImpl& C = c->impl;
+    TreeFileHeader h;
+    memset(&h, 0, sizeof h);
+    memcpy(h.magic, kTreeMagic, 8);
+    h.version = LSG_TREE_VERSION; h.header_bytes = sizeof h;
+    h.nverts = C.nverts; h.ntri = C.ntri; h.leaf = C.leaf; h.flags = flags; h.nvalid = C.nvalid;
+    h.nnodes = C.nnodes; h.nnodes4 = C.nnodes4; h.depth = C.depth; h.stack_need = C.stack_need;
+    h.root = C.root; h.root4 = C.root4;
+    memcpy(h.bmin, C.bmin, sizeof h.bmin); memcpy(h.bmax, C.bmax, sizeof h.bmax); h.margin = C.margin;
+    h.payload_bytes = 4ull * C.nvalid + sizeof(Node) * (uint64_t)C.nnodes + sizeof(Node4) * (uint64_t)C.nnodes4;
+    if (sizeof h + h.payload_bytes < min_bytes) return LSG_OK;    // small trees build faster than a file loads
+    std::vector<uint8_t> payload(h.payload_bytes);
+    if (C.nvalid) {
+        std::vector<TriRest> m(C.nvalid);
+        LSG_HIP(hipMemcpy(m.data(), C.d_rest, sizeof(TriRest) * (size_t)C.nvalid, hipMemcpyDeviceToHost));
+        uint32_t* order = (uint32_t*)payload.data();
+        for (uint32_t i = 0; i < C.nvalid; i++) order[i] = m[i].meta.x;
+    }
+    if (C.nnodes) LSG_HIP(hipMemcpy(payload.data() + 4ull * C.nvalid, C.d_nodes, sizeof(Node) * (size_t)C.nnodes, hipMemcpyDeviceToHost));
+    if (C.nnodes4)
+        LSG_HIP(hipMemcpy(payload.data() + 4ull * C.nvalid + sizeof(Node) * (size_t)C.nnodes, C.d_nodes4, sizeof(Node4) * (size_t)C.nnodes4,
+                          hipMemcpyDeviceToHost));
+    // the mesh hash from the device copy of the mesh
+    std::vector<float> v(3 * (size_t)C.nverts);
+    std::vector<uint32_t> ix(3 * (size_t)C.ntri);
+    if (C.nverts) LSG_HIP(hipMemcpy(v.data(), C.d_verts, 12 * (size_t)C.nverts, hipMemcpyDeviceToHost));
+    if (C.ntri) LSG_HIP(hipMemcpy(ix.data(), C.d_idx, 12 * (size_t)C.ntri, hipMemcpyDeviceToHost));
+    sha1(v.data(), 4 * v.size(), ix.data(), 4 * ix.size(), h.mesh_hash);
+    h.payload_sum = payload_checksum(payload.data(), payload.size());
+    // written under a temporary name and renamed, so a reader never sees half a file
+    const std::string tmp = std::string(path) + ".tmp";
+    FILE* f = fopen(tmp.c_str(), "wb");
+    if (!f) return fail(LSG_ERR_INVALID, "tree file %s: cannot create", path);
+    bool ok = fwrite(&h, sizeof h, 1, f) == 1 && (payload.empty() || fwrite(payload.data(), 1, payload.size(), f) == payload.size());
+    ok = (fclose(f) == 0) && ok;
+    if (!ok || rename(tmp.c_str(), path)) {
+        remove(tmp.c_str());
+        return fail(LSG_ERR_INVALID, "tree file %s: write failed", path);
+    }
+    return LSG_OK;
+}
+
 } // namespace lsg
 
 using namespace lsg;
 
 extern "C" {
+
+int lsg_collider_save(const lsg_collider* c, const char* path)
+{
+    if (!c || !path) return fail(LSG_ERR_INVALID, "lsg_collider_save: null argument");
+    return save_to_file(c, path, c->impl.flags, 0);
+}
+
+int lsg_collider_load(const float* verts, uint32_t nverts, const uint32_t* indices, uint32_t ntri, const char* path, lsg_collider** out)
+{
+    if (!out) return fail(LSG_ERR_INVALID, "lsg_collider_load: out is null");
+    *out = nullptr;
+    if (!path || (!verts && nverts) || (!indices && ntri)) return fail(LSG_ERR_INVALID, "lsg_collider_load: null argument");
+    if (!device_ready()) return fail(LSG_ERR_NO_DEVICE, "lsg_collider_load: no gfx950 device");
+    uint8_t hash[20];
+    sha1(verts, 12 * (size_t)nverts, indices, 12 * (size_t)ntri, hash);
+    TreeFileHeader h;
+    std::vector<uint8_t> payload;
+    int st = read_tree_file(path, hash, nverts, ntri, h, payload);
+    if (st) return st;
+    return load_from_file(verts, nverts, indices, ntri, hash, path, 0, h.flags, out);
+}
+
+int lsg_collider_create_cached(const float* verts, uint32_t nverts, const uint32_t* indices, uint32_t ntri, const lsg_build_params* params,
+                               const volatile int* aborting, const char* cache_location, lsg_collider** out, int32_t* from_cache)
+{
+    if (from_cache) *from_cache = 0;
+    if (!out) return fail(LSG_ERR_INVALID, "lsg_collider_create_cached: out is null");
+    *out = nullptr;
+    const bool caching = !(cache_location && !strcmp(cache_location, "*"));
+    std::string path;
+    const uint32_t want_leaf = params ? params->leaf_size : 0u;
+    const uint32_t want_flags = params ? (params->flags & LSG_BUILD_LBVH_ONLY) : 0u;
+    if (caching && (verts || !nverts) && (indices || !ntri)) {
+        uint8_t hash[20];
+        sha1(verts, 12 * (size_t)nverts, indices, 12 * (size_t)ntri, hash);
+        path = cache_file_name(hash, LSG_TREE_VERSION, cache_location, ".lsgbvh");
+        if (device_ready() && load_from_file(verts, nverts, indices, ntri, hash, path.c_str(), want_leaf, want_flags, out) == LSG_OK) {
+            if (from_cache) *from_cache = 1;
+            return LSG_OK;
+        }
+    }
+    int st = lsg_collider_create(verts, nverts, indices, ntri, params, aborting, out);
+    if (st || !caching) return st;
+    // a cache that cannot be written leaves the collider as built (the reference ignores a failed save)
+    (void)save_to_file(*out, path.c_str(), want_flags, 1024);
+    return LSG_OK;
+}
 
 int lsg_collider_create(const float* verts, uint32_t nverts, const uint32_t* indices, uint32_t ntri,
                         const lsg_build_params* params, const volatile int* aborting, lsg_collider** out)
@@ -683,6 +915,7 @@ int lsg_collider_create(const float* verts, uint32_t nverts, const uint32_t* ind
         st = build(c->impl, verts, nverts, indices, ntri, leaf, sah, aborting);
     }
     if (st) { free_impl(c->impl); delete c; return st; }
+    c->impl.flags = sah ? 0u : (uint32_t)LSG_BUILD_LBVH_ONLY;
     *out = c;
     return LSG_OK;
 }

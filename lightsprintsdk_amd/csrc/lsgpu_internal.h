@@ -126,6 +126,7 @@ struct ColliderImpl {
     Node* d_nodes = nullptr;
     float4* d_plane = nullptr; TriRest* d_rest = nullptr;
     uint32_t* d_slot = nullptr;   // leaf-order slot of each original triangle
+    uint32_t flags = 0;           // lsg_build_params flags the tree was built with
     uint64_t bytes = 0;
     double build_ms = 0;
     ColliderView view() const
@@ -152,6 +153,10 @@ size_t sort_tmp_words(uint32_t n);
 // in keys/vals, keys_tmp/vals_tmp are n-word scratch
 int sort_pairs_u32(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t n, int bits,
                    uint32_t* tmp, hipStream_t st);
+// SHA-1 of the concatenation a || b (lsgpu_hash.cpp)
+void sha1(const void* a, size_t na, const void* b, size_t nb, uint8_t out[20]);
+// RRHash::getFileName's cache file name for a hash (lsgpu_hash.cpp)
+std::string cache_file_name(const uint8_t hash[20], uint32_t version, const char* location, const char* extension);
 // SAH rebuild of a BVH2's inner nodes over its leaves (lsgpu_sah.cpp): 0 = rebuilt,
 // 1 = fewer than two leaves, 2 = malformed tree
 int sah_rebuild(const float* nodes, uint32_t nnodes, int32_t root, float* out, uint32_t* out_nodes, int32_t* out_root,

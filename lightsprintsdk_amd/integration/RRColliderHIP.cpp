@@ -33,7 +33,7 @@ static bool flattenMesh(const RRMesh* mesh, std::vector<float>& v, std::vector<u
 	return true;
 }
 
-bool RRColliderHIP::build(bool& aborting)
+bool RRColliderHIP::build(bool& aborting, const char* cacheLocation)
 {
 	std::vector<float> v;
 	std::vector<uint32_t> idx;
@@ -41,7 +41,8 @@ bool RRColliderHIP::build(bool& aborting)
 	lsg_build_params bp = { leaf, 0 };
 	volatile int abortFlag = aborting ? 1 : 0;
 	lsg_collider* c = nullptr;
-	if (lsg_collider_create(v.data(), (uint32_t)(v.size() / 3), idx.data(), (uint32_t)(idx.size() / 3), &bp, &abortFlag, &c) != LSG_OK)
+	if (lsg_collider_create_cached(v.data(), (uint32_t)(v.size() / 3), idx.data(), (uint32_t)(idx.size() / 3), &bp, &abortFlag,
+	                               cacheLocation, &c, nullptr) != LSG_OK)
 	{
 		// RRCollider::create reports a failed builder and falls back to IT_LINEAR
 		// (RRCollider.cpp:216-239); the device's reason goes to the same log
@@ -61,12 +62,12 @@ bool RRColliderHIP::build(bool& aborting)
 	return true;
 }
 
-RRColliderHIP* RRColliderHIP::create(const RRMesh* mesh, bool& aborting, unsigned leafSize)
+RRColliderHIP* RRColliderHIP::create(const RRMesh* mesh, bool& aborting, unsigned leafSize, const char* cacheLocation)
 {
 	if (!mesh || lsg_device_count() <= 0) return nullptr;
 	RRColliderHIP* c = new RRColliderHIP(mesh);
 	c->leaf = leafSize;
-	if (!c->build(aborting)) { delete c; return nullptr; }
+	if (!c->build(aborting, cacheLocation)) { delete c; return nullptr; }
 	return c;
 }
 
@@ -77,9 +78,10 @@ RRColliderHIP::~RRColliderHIP()
 
 void RRColliderHIP::update()
 {
-	// RRCollider.h:185-190: a BVH collider rereads the mesh fully
+	// RRCollider.h:185-190: a BVH collider rereads the mesh fully (a changed mesh has
+	// another hash, so no cached tree of the old one is taken)
 	bool aborting = false;
-	build(aborting);
+	build(aborting, "*");
 }
 
 size_t RRColliderHIP::getMemoryOccupied() const
@@ -174,11 +176,12 @@ unsigned RRColliderHIP::intersectBatch(RRRay* rays, unsigned numRays) const
 RRCollider* builder(const RRMesh* mesh, const RRObjects* objects, RRCollider::IntersectTechnique technique,
                     bool& aborting, const char* cacheLocation, void* buildParams)
 {
-	(void)technique; (void)cacheLocation; (void)buildParams;
+	(void)technique; (void)buildParams;
 	// multi-object colliders stay with the reference (defaultBuilder -> createMultiCollider);
-	// the solver builds its collider from the multiObject's single mesh (RRObjectMulti.h:74)
+	// the solver builds its collider from the multiObject's single mesh (RRObjectMulti.h:74).
+	// The tree is cached as the reference caches its BSP (IntersectBsp.h:213-291).
 	if (!mesh || objects) return nullptr;
-	return RRColliderHIP::create(mesh, aborting);
+	return RRColliderHIP::create(mesh, aborting, 0, cacheLocation);
 }
 
 void registerTechniques(bool overrideBsp)

@@ -26,7 +26,10 @@ class RRColliderHIP : public rr::RRCollider
 public:
 	// Builds the LBVH on the GPU from the mesh; nullptr when the device is
 	// missing or the build fails (the error text is in lsg_last_error()).
-	static RRColliderHIP* create(const rr::RRMesh* mesh, bool& aborting, unsigned leafSize = 0);   // 0: library default
+	// cacheLocation as RRCollider::create's (RRCollider.h:179-183): nullptr = tree files in the
+	// temp directory, "*" = no cache, else the cache directory (lsg_collider_create_cached)
+	static RRColliderHIP* create(const rr::RRMesh* mesh, bool& aborting, unsigned leafSize = 0,   // 0: library default
+	                             const char* cacheLocation = "*");
 	~RRColliderHIP() override;
 
 	// RRCollider virtuals (RRCollider.h:190-267)
@@ -45,7 +48,7 @@ public:
 
 private:
 	RRColliderHIP(const rr::RRMesh* m) : mesh(m) {}
-	bool build(bool& aborting);
+	bool build(bool& aborting, const char* cacheLocation);
 	void fill(rr::RRRay& ray, const lsg_host::Hit& h) const;
 	const rr::RRMesh* mesh;
 	lsg_collider* col = nullptr;

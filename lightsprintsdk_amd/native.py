@@ -90,6 +90,11 @@ SIGNATURES = {
     "lsg_event_elapsed_ms": (ctypes.c_int, [P, P, ctypes.POINTER(ctypes.c_float)]),
     "lsg_collider_create": (ctypes.c_int, [P, U32, P, U32, ctypes.POINTER(BuildParams), P, ctypes.POINTER(P)]),
     "lsg_collider_destroy": (None, [P]),
+    "lsg_mesh_hash": (ctypes.c_int, [P, U32, P, U32, P]),
+    "lsg_cache_file_name": (ctypes.c_int, [P, U32, ctypes.c_char_p, ctypes.c_char_p, P, U32]),
+    "lsg_collider_save": (ctypes.c_int, [P, ctypes.c_char_p]),
+    "lsg_collider_load": (ctypes.c_int, [P, U32, P, U32, ctypes.c_char_p, P]),
+    "lsg_collider_create_cached": (ctypes.c_int, [P, U32, P, U32, P, P, ctypes.c_char_p, P, P]),
     "lsg_collider_info_get": (ctypes.c_int, [P, ctypes.POINTER(ColliderInfo)]),
     "lsg_collider_download": (ctypes.c_int, [P, P, P]),
     "lsg_bvh_rebuild_sah": (ctypes.c_int, [P, U32, I32, P, P, P, P]),
@@ -125,6 +130,8 @@ SIGNATURES = {
     "lsg_bake_run_layers": (ctypes.c_int, [P, ctypes.POINTER(GatherParams), ctypes.POINTER(GatherParams), ctypes.POINTER(FilterParams), U32, P]),
     "lsg_bake_download_layer": (ctypes.c_int, [P, U32, U32, P]),
     "lsg_bake_download_layer_rgba": (ctypes.c_int, [P, U32, U32, P]),
+    "lsg_format_bytes": (U32, [ctypes.c_int32]),
+    "lsg_lightmap_encode": (ctypes.c_int, [P, U32, ctypes.c_int32, ctypes.c_int32, ctypes.c_float, ctypes.c_float, P, P, P]),
     "lsg_stitch_vertices": (ctypes.c_int, [P, U32, P, U32, P, ctypes.c_float, ctypes.c_float, P, P, P]),
     "lsg_vertex_plan_create": (ctypes.c_int, [P, P, P, U32, P]),
     "lsg_vertex_plan_destroy": (None, [P]),
