@@ -279,9 +279,9 @@ def bake_cpu_baseline(sb, threads: int, target_s: float = 12.0):
 
 # ---- learned + fallback vs plain on the held-out box-room scenes -------------------------
 
-def learned_summary(reps: int = 2, log=None):
+def learned_summary(reps: int = 3, log=None):
     """The committed surrogate on the 5 held-out box-room scenes (the plain bake of each
-    is the reference and is timed beside it)."""
+    is the reference and is timed beside it; median of `reps` runs of each)."""
     from lightsprintsdk_amd import learned as Lr
     got = Lr.load_weights()
     if got is None:
@@ -300,7 +300,10 @@ def learned_summary(reps: int = 2, log=None):
                 fallback_mean=round(float(np.mean([r["fallback"] for r in rows])), 4),
                 bound=dict(kind="per-scene: largest tau whose estimated kept-texel RMSE (second 16-ray gather) "
                                 "is within target", target=bound["target"]),
-                training=dict(scenes=20, samples=meta.get("samples"), epochs=meta.get("epochs"),
+                timing=f"median of {reps} runs per scene, plain and learned alternating",
+                training=dict(source="metadata of the shipped surrogate (trained offline by python -m "
+                                     "lightsprintsdk_amd.learned train; not run or timed by this command)",
+                              scenes=20, samples=meta.get("samples"), epochs=meta.get("epochs"),
                               train_s=meta.get("train_s")),
                 scenes=[{k: (round(v, 5) if isinstance(v, float) else v) for k, v in r.items()} for r in rows])
 
