@@ -270,6 +270,55 @@ int  lsg_scene_get_exitance(const lsg_scene* scene, float* exitance);
 int  lsg_scene_set_indirect(lsg_scene* scene, const float* corners, const float* albedo, const uint8_t* sides);
 int  lsg_scene_set_lights(lsg_scene* scene, const lsg_light* lights, uint32_t nlights);
 
+/* Point materials: materials read at a ray's hit point (RRObject::getPointMaterial,
+ * RRObject.cpp:198-303, in RRCollisionHandlerFinalGathering::collides, pathtracer.h:194-226)
+ * when the gather's quality exceeds their minimal quality: hemisphere rays at quality
+ * (nearest texel), shadow rays at 2 x quality, interpolating texels bilinearly when
+ * quality / 10 exceeds it (gather.cpp:175-180, 366-370).  Supported: a diffuse texture, an
+ * emissive texture and keyed transmittance; a transparent texel is passed through by every
+ * ray, an opaque one is hit with its diffuse colour.  Refused: unkeyed (partial)
+ * transmittance, a keyed material that emits, and a gather whose quality does not read
+ * every keyed material's point materials (below it the reference transmits partially). */
+#define LSG_PM_TRANSMITTANCE_IN_ALPHA 1u   /* specularTransmittanceInAlpha */
+#define LSG_PM_KEYED                  2u   /* specularTransmittanceKeyed */
+#define LSG_PM_INVERTED               4u   /* specularTransmittanceMapInverted */
+typedef struct lsg_point_material {
+    int32_t  diffuse_texture;        /* diffuseReflectance.texture: index into the textures, or -1 */
+    int32_t  emissive_texture;       /* diffuseEmittance.texture, or -1 */
+    int32_t  transmittance_texture;  /* specularTransmittance.texture (may be the diffuse one), or -1 */
+    uint32_t flags;                  /* LSG_PM_* */
+    float    threshold;              /* specularTransmittanceThreshold */
+    uint32_t min_quality;            /* minimalQualityForPointMaterials */
+    float    albedo[3];              /* diffuseReflectance.colorLinear of the per-triangle material */
+    float    emittance[3];           /* diffuseEmittance.colorLinear x materialEmittanceMultiplier */
+    float    emittance_multiplier;   /* materialEmittanceMultiplier, applied to emissive texels */
+    uint32_t emit_sides;             /* bit 0: sideBits[0].emitTo && legal, bit 1: sideBits[1] */
+} lsg_point_material;
+/* A texture as RRBufferInMemory stores it: width x height elements of an LSG_FORMAT_*
+ * format (RGB, BGR, RGBA, RGBF, RGBAF, LUMINANCE, LUMINANCEF), rows from the first.  With a
+ * colour space other than the sRGB power of lsg_scene_set_point_materials' gamma, the caller
+ * converts: linear = getElement(i, colorSpace) of every texel (RRBufferInMemory.cpp:295-360),
+ * linear_stored = getElement(i, nullptr) with colorSpace->toLinear applied to rgb (what the
+ * transmittance-in-alpha path reads, RRObject.cpp:240-244); NULL = computed from gamma. */
+typedef struct lsg_texture {
+    uint32_t width, height;
+    int32_t  format;
+    int32_t  scaled;                 /* RRBuffer::getScaled: stored in the solver's colour space */
+    const void* data;                /* host */
+    const float* linear;             /* host float[width*height*4] or NULL */
+    const float* linear_stored;      /* host float[width*height*4] or NULL */
+} lsg_texture;
+/* tri_material: host uint32[ntri], each triangle's material or 0xFFFFFFFF (none); tri_uv:
+ * host float[ntri*6], the material uv of each triangle's corners; gamma: the power of the
+ * solver's colour space (RRColorSpace::create_sRGB, 0.45), 0 = none.  The exitance table
+ * entry of a triangle with a material holds the side's direct irradiance (the exitance
+ * kernels of lsg_scene_exitance_device and the bake plans write it so, and a caller of
+ * lsg_scene_set_exitance does the same): the hit multiplies it by the diffuse colour and
+ * adds emittance and indirect light itself.  tri_material NULL removes the materials. */
+int  lsg_scene_set_point_materials(lsg_scene* scene, const uint32_t* tri_material, const float* tri_uv,
+                                   const lsg_point_material* materials, uint32_t nmaterials,
+                                   const lsg_texture* textures, uint32_t ntextures, float gamma);
+
 /* The solver's environment (RRSolver::setEnvironment), which hemisphere rays that leave
  * the scene sample (PathtracerWorker::getIncidentRadiance, pathtracer.cpp:161-172), with a
  * nearest-texel lookup as RRBufferInMemory::getElementAtDirection

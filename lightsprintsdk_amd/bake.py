@@ -48,6 +48,30 @@ def rasterize(tri_uv: np.ndarray, tri_ids: np.ndarray, width: int, height: int, 
     return tex, sub
 
 
+# lsg_point_material (include/lsgpu.h): a material read at the hit point
+POINT_MATERIAL_DTYPE = np.dtype([("diffuse_texture", "<i4"), ("emissive_texture", "<i4"), ("transmittance_texture", "<i4"),
+                                 ("flags", "<u4"), ("threshold", "<f4"), ("min_quality", "<u4"), ("albedo", "<f4", 3),
+                                 ("emittance", "<f4", 3), ("emittance_multiplier", "<f4"), ("emit_sides", "<u4")])
+PM_TRANSMITTANCE_IN_ALPHA, PM_KEYED, PM_INVERTED = 1, 2, 4
+NO_MATERIAL = 0xFFFFFFFF
+# RRBufferFormat of a texture (include/lsgpu.h LSG_FORMAT_*)
+FORMAT_RGB, FORMAT_BGR, FORMAT_RGBA, FORMAT_RGBF, FORMAT_RGBAF, FORMAT_LUMINANCE, FORMAT_LUMINANCEF = 0, 1, 2, 3, 4, 9, 10
+_FORMAT_DTYPE = {0: (np.uint8, 3), 1: (np.uint8, 3), 2: (np.uint8, 4), 3: (np.float32, 3), 4: (np.float32, 4),
+                 9: (np.uint8, 1), 10: (np.float32, 1)}
+
+
+def texture(fmt: int, data, scaled: bool = True) -> dict:
+    """A texture as RRBufferInMemory stores it: data [height, width, channels] of the
+    format's element type (bytes for RGB, BGR, RGBA, LUMINANCE; float32 otherwise)."""
+    dt, ch = _FORMAT_DTYPE[fmt]
+    a = np.ascontiguousarray(data, dt)
+    if a.ndim == 2:
+        a = a[:, :, None]
+    if a.ndim != 3 or a.shape[2] != ch:
+        raise ValueError(f"format {fmt} takes [height, width, {ch}] {np.dtype(dt).name} texels, got {a.shape} {a.dtype}")
+    return dict(format=fmt, scaled=bool(scaled), data=a, width=a.shape[1], height=a.shape[0])
+
+
 def lights_array(lights) -> np.ndarray:
     a = np.zeros(len(lights), LIGHT_DTYPE)
     for k, l in enumerate(lights):
@@ -187,6 +211,31 @@ class GatherScene:
         self.side_bits = (self.side_bits & np.uint8(~(8 | 16) & 0xFF)) | np.where(b & 1, 8, 0).astype(np.uint8) \
             | np.where(b & 2, 16, 0).astype(np.uint8)
 
+    def set_point_materials(self, tri_material=None, tri_uv=None, materials=None, textures=(), gamma=0.45):
+        """Point materials (RRObject::getPointMaterial): tri_material [ntri] uint32 material
+        index or NO_MATERIAL, tri_uv [ntri, 6] material uv of the corners, materials a
+        POINT_MATERIAL_DTYPE array, textures a list of texture() dicts, gamma the solver's
+        sRGB power (0 = no colour space).  tri_material None removes them.  A triangle with
+        a material keeps its direct irradiance in the exitance table."""
+        L = N.lib()
+        if tri_material is None:
+            N.check(L.lsg_scene_set_point_materials(self._h, None, None, None, 0, None, 0, 0.0), "set_point_materials")
+            self.pm = None
+            return
+        tm = np.ascontiguousarray(tri_material, np.uint32).reshape(len(self.idx))
+        uv = np.ascontiguousarray(tri_uv, np.float32).reshape(len(self.idx), 6)
+        mats = np.ascontiguousarray(materials, POINT_MATERIAL_DTYPE).reshape(-1)
+        texs = [t if isinstance(t, dict) else texture(*t) for t in textures]
+        arr = (N.Texture * max(len(texs), 1))()
+        for k, t in enumerate(texs):
+            arr[k] = N.Texture(t["width"], t["height"], t["format"], int(t["scaled"]), t["data"].ctypes.data_as(ctypes.c_void_p),
+                               None, None)
+        N.check(L.lsg_scene_set_point_materials(self._h, tm.ctypes.data_as(ctypes.c_void_p), uv.ctypes.data_as(ctypes.c_void_p),
+                                                mats.ctypes.data_as(ctypes.c_void_p), len(mats),
+                                                ctypes.cast(arr, ctypes.c_void_p), len(texs), float(gamma)),
+                "set_point_materials")
+        self.pm = dict(tri_mat=tm, tri_uv=uv, mats=mats, textures=texs, gamma=float(gamma))
+
     def gather_layers_device(self, dtex: DeviceBuffer, ntex: int, dsub: DeviceBuffer, nsub: int, params: GatherParams,
                              douts, stream=None):
         """processTexel into the output layers of updateLightmap: douts = 5 DeviceBuffers
@@ -225,7 +274,7 @@ class GatherScene:
     def oracle_arrays(self, exitance):
         return dict(verts=self.verts, idx=self.idx, normals=self.normals, side_bits=self.side_bits,
                     tangents=getattr(self, "tangents", None), bitangents=getattr(self, "bitangents", None),
-                    env=getattr(self, "env", None), ind=getattr(self, "ind", None),
+                    env=getattr(self, "env", None), ind=getattr(self, "ind", None), pm=getattr(self, "pm", None),
                     exitance=np.ascontiguousarray(exitance, np.float32).reshape(len(self.idx), 8),
                     lights=np.array([(l["type"], l["position"], l["direction"], l["color"], l["cast_shadows"])
                                      for l in self.lights], dtype=[("type", "<i4"), ("pos", "<f4", 3), ("dir", "<f4", 3),

@@ -20,28 +20,36 @@
 
 namespace lsg {
 
+// a triangle with a point material keeps its irradiance in the table: the gather's hit
+// multiplies it by the material's diffuse colour and adds the emittance itself
+__device__ inline bool has_material(const uint32_t* mat, uint32_t t) { return mat && mat[t] != 0xFFFFFFFFu; }
+
 __global__ __launch_bounds__(256) void k_exitance(uint32_t nt, const float4* __restrict__ irr, const float* __restrict__ albedo,
-                                                  const float* __restrict__ emit, float4* __restrict__ ex)
+                                                  const float* __restrict__ emit, const uint32_t* __restrict__ mat,
+                                                  float4* __restrict__ ex)
 {
     const uint32_t t = blockIdx.x * 256 + threadIdx.x;
     if (t >= nt) return;
     // PathtracerWorker::getIncidentRadiance at depth 0 with solver shortcuts
     // (pathtracer.cpp:235-316): direct irradiance * diffuse reflectance + emittance
     const float4 e = irr[t];
-    ex[2 * (size_t)t] = make_float4(e.x * albedo[3 * (size_t)t] + emit[3 * (size_t)t],
-                                    e.y * albedo[3 * (size_t)t + 1] + emit[3 * (size_t)t + 1],
-                                    e.z * albedo[3 * (size_t)t + 2] + emit[3 * (size_t)t + 2], 0.f);
+    ex[2 * (size_t)t] = has_material(mat, t) ? make_float4(e.x, e.y, e.z, 0.f)
+                        : make_float4(e.x * albedo[3 * (size_t)t] + emit[3 * (size_t)t],
+                                      e.y * albedo[3 * (size_t)t + 1] + emit[3 * (size_t)t + 1],
+                                      e.z * albedo[3 * (size_t)t + 2] + emit[3 * (size_t)t + 2], 0.f);
     ex[2 * (size_t)t + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 // exitance = emittance only, the light the first gather's hemisphere rays may see: the
 // first gather is the direct illumination (lights, environment, emitters), and a table
 // left over from an earlier run would add a bounce per run
-__global__ __launch_bounds__(256) void k_emittance_only(uint32_t nt, const float* __restrict__ emit, float4* __restrict__ ex)
+__global__ __launch_bounds__(256) void k_emittance_only(uint32_t nt, const float* __restrict__ emit, const uint32_t* __restrict__ mat,
+                                                        float4* __restrict__ ex)
 {
     const uint32_t t = blockIdx.x * 256 + threadIdx.x;
     if (t >= nt) return;
-    ex[2 * (size_t)t] = make_float4(emit[3 * (size_t)t], emit[3 * (size_t)t + 1], emit[3 * (size_t)t + 2], 0.f);
+    ex[2 * (size_t)t] = has_material(mat, t) ? make_float4(0.f, 0.f, 0.f, 0.f)
+                        : make_float4(emit[3 * (size_t)t], emit[3 * (size_t)t + 1], emit[3 * (size_t)t + 2], 0.f);
     ex[2 * (size_t)t + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
@@ -50,15 +58,16 @@ __global__ __launch_bounds__(256) void k_emittance_only(uint32_t nt, const float
 // own hemisphere part (parts[2t])
 __global__ __launch_bounds__(256) void k_exitance_bounce(uint32_t nt, const float4* __restrict__ parts, const float4* __restrict__ hemi,
                                                          const float* __restrict__ albedo, const float* __restrict__ emit,
-                                                         float4* __restrict__ ex)
+                                                         const uint32_t* __restrict__ mat, float4* __restrict__ ex)
 {
     const uint32_t t = blockIdx.x * 256 + threadIdx.x;
     if (t >= nt) return;
     const float4 h = hemi ? hemi[t] : parts[2 * (size_t)t];
     const float4 l = parts[2 * (size_t)t + 1];
-    ex[2 * (size_t)t] = make_float4((h.x + l.x) * albedo[3 * (size_t)t] + emit[3 * (size_t)t],
-                                    (h.y + l.y) * albedo[3 * (size_t)t + 1] + emit[3 * (size_t)t + 1],
-                                    (h.z + l.z) * albedo[3 * (size_t)t + 2] + emit[3 * (size_t)t + 2], 0.f);
+    ex[2 * (size_t)t] = has_material(mat, t) ? make_float4(h.x + l.x, h.y + l.y, h.z + l.z, 0.f)
+                        : make_float4((h.x + l.x) * albedo[3 * (size_t)t] + emit[3 * (size_t)t],
+                                      (h.y + l.y) * albedo[3 * (size_t)t + 1] + emit[3 * (size_t)t + 1],
+                                      (h.z + l.z) * albedo[3 * (size_t)t + 2] + emit[3 * (size_t)t + 2], 0.f);
     ex[2 * (size_t)t + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
@@ -137,16 +146,17 @@ static int run_first(BakeImpl& B, const lsg_gather_params* first, hipStream_t st
     GatherArgs A;
     int stack = 0, e;
     if ((e = gather_args(B.scene, B.d_ttex, B.ntri, B.d_tsub, first, (float*)B.d_irr, A, stack))) return e;
-    k_emittance_only<<<nblk(B.ntri), 256, 0, st>>>(B.ntri, B.d_emit, B.scene->impl.d_exit);
+    k_emittance_only<<<nblk(B.ntri), 256, 0, st>>>(B.ntri, B.d_emit, B.scene->impl.d_tri_mat, B.scene->impl.d_exit);
     if (!B.bounces) {
         if ((e = launch_gather(A, stack, st))) return e;
-        k_exitance<<<nblk(B.ntri), 256, 0, st>>>(B.ntri, B.d_irr, B.d_albedo, B.d_emit, B.scene->impl.d_exit);
+        k_exitance<<<nblk(B.ntri), 256, 0, st>>>(B.ntri, B.d_irr, B.d_albedo, B.d_emit, B.scene->impl.d_tri_mat, B.scene->impl.d_exit);
         return LSG_OK;
     }
     // radiosity: keep the first gather's light part, regather the hemisphere each bounce
     A.parts = B.d_parts;
     if ((e = launch_gather(A, stack, st))) return e;
-    k_exitance_bounce<<<nblk(B.ntri), 256, 0, st>>>(B.ntri, B.d_parts, nullptr, B.d_albedo, B.d_emit, B.scene->impl.d_exit);
+    k_exitance_bounce<<<nblk(B.ntri), 256, 0, st>>>(B.ntri, B.d_parts, nullptr, B.d_albedo, B.d_emit, B.scene->impl.d_tri_mat,
+                                                     B.scene->impl.d_exit);
     lsg_gather_params bp = *first;
     bp.hemisphere = 1;
     bp.light_multiplier = 0.f;
@@ -157,7 +167,8 @@ static int run_first(BakeImpl& B, const lsg_gather_params* first, hipStream_t st
         GatherArgs H;
         if ((e = gather_args(B.scene, B.d_ttex, B.ntri, B.d_tsub, &bp, (float*)B.d_hemi, H, stack))) return e;
         if ((e = launch_gather(H, stack, st))) return e;
-        k_exitance_bounce<<<nblk(B.ntri), 256, 0, st>>>(B.ntri, B.d_parts, B.d_hemi, B.d_albedo, B.d_emit, B.scene->impl.d_exit);
+        k_exitance_bounce<<<nblk(B.ntri), 256, 0, st>>>(B.ntri, B.d_parts, B.d_hemi, B.d_albedo, B.d_emit, B.scene->impl.d_tri_mat,
+                                                         B.scene->impl.d_exit);
     }
     return LSG_OK;
 }
@@ -191,8 +202,8 @@ int lsg_scene_exitance_device(lsg_scene* scene, const float* irr, const float* a
     const uint32_t nt = scene->impl.col->impl.ntri;
     if (!nt) return LSG_OK;
     hipStream_t st = (hipStream_t)stream;
-    if (irr) k_exitance<<<nblk(nt), 256, 0, st>>>(nt, (const float4*)irr, albedo, emit, scene->impl.d_exit);
-    else k_emittance_only<<<nblk(nt), 256, 0, st>>>(nt, emit, scene->impl.d_exit);
+    if (irr) k_exitance<<<nblk(nt), 256, 0, st>>>(nt, (const float4*)irr, albedo, emit, scene->impl.d_tri_mat, scene->impl.d_exit);
+    else k_emittance_only<<<nblk(nt), 256, 0, st>>>(nt, emit, scene->impl.d_tri_mat, scene->impl.d_exit);
     LSG_HIP(hipGetLastError());
     return LSG_OK;
 }
@@ -316,7 +327,9 @@ int lsg_bake_run(lsg_bake* b, const lsg_gather_params* first, const lsg_gather_p
     {
         // the captured kernels hold the environment's buffer and shape as arguments
         const SceneImpl& S = B.scene->impl;
-        const uintptr_t env[4] = {(uintptr_t)S.d_env, S.env_type, S.env_w, S.env_h};
+        // (and the indirect records and point materials likewise)
+        const uintptr_t env[7] = {(uintptr_t)S.d_env, S.env_type, S.env_w, S.env_h, (uintptr_t)S.d_ind, (uintptr_t)S.d_tri_mat,
+                                  (uintptr_t)S.pm_version};
         const uint8_t* e8 = (const uint8_t*)env;
         key.insert(key.end(), e8, e8 + sizeof env);
     }

@@ -1,6 +1,7 @@
 // lsgpu_gather.h -- internal interface of the per-texel gather (lsgpu_gather.hip).
 #pragma once
 #include "lsgpu_internal.h"
+#include "lsgpu_material.h"
 #include <vector>
 
 namespace lsg {
@@ -20,6 +21,7 @@ struct SceneView {
     uint32_t env_type, env_w, env_h;   // LSG_ENV_*; env_type 0 = rays that leave return 0
     const float4* ind;           // 3 per triangle: corner k = (indirect irradiance rgb, albedo channel k), or null
     const uint32_t* slot;        // leaf-order slot of each triangle (hit (u, v) recomputed from its record)
+    PMView pm;                   // point materials (pm.tri_mat null = none)
 };
 
 struct GatherArgs {
@@ -71,7 +73,21 @@ struct SceneImpl {
     std::vector<uint8_t> h_side;       // side bits as given (the indirect bits are merged in)
     void* d_walk = nullptr;            // subtexel walk records of the last gather launch (32 B each)
     uint32_t walk_cap = 0;
+    // point materials (lsg_scene_set_point_materials); d_tri_mat null = none
+    uint32_t* d_tri_mat = nullptr;
+    float* d_tri_uv = nullptr;
+    lsg_point_material* d_mats = nullptr;
+    TexInfo* d_tex = nullptr;
+    float4* d_texels = nullptr;
+    std::vector<lsg_point_material> h_mats;   // host copy: launches check the keyed materials' quality
+    uint64_t pm_version = 0;                  // bumped on every change (bake graphs key on it)
 };
+
+// the gather's quality reads the point materials of every keyed material (a keyed material
+// below its minimal quality would be the reference's partially transmitting per-triangle
+// material, which is not modelled)
+int check_point_quality(const SceneImpl& S, uint32_t quality);
+void free_point_materials(SceneImpl& S);
 
 // walk-record scratch of the scene for a launch of n texels (grown on demand; not while
 // a stream is captured)

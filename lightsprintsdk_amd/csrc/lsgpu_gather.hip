@@ -36,6 +36,14 @@
 
 namespace lsg {
 
+constexpr int kFeatEnv = 1, kFeatMats = 2;
+
+// the kernel instantiation a launch needs
+static inline int features(const GatherArgs& A)
+{
+    return (A.sc.env_type ? kFeatEnv : 0) | (A.sc.pm.tri_mat ? kFeatMats : 0);
+}
+
 
 
 __device__ inline uint32_t lowbias32(uint32_t x)
@@ -113,9 +121,12 @@ template <int G> __device__ inline uint32_t group_sum_u(uint32_t v)
 // RRCollisionHandlerFinalGathering::collides (pathtracer.h:117-242) for opaque
 // materials: mode 0 = hemisphere (accept the rendered side), 1 = shadow ray
 // (any shadow caster, both sides [#45]).
-struct GatherFilter {
+template <bool MATS = false>
+struct GatherFilterT {
     const float* verts; const uint32_t* idx; const uint8_t* side;
     uint32_t shooter; float near_t; int mode; float rug_below;
+    PMView pm;              // MATS: point materials of the scene
+    uint32_t qpoint, qinterp;   // MATS: the quality this ray kind compares with minimalQualityForPointMaterials
     __device__ bool same_location(uint32_t tri) const
     {
         const uint32_t* si = idx + 3 * (size_t)shooter;
@@ -131,17 +142,48 @@ struct GatherFilter {
         }
         return true;
     }
-    __device__ bool accept(uint32_t tri, bool front, float t) const
+    __device__ bool accept(uint32_t tri, bool front, float t, float u, float v) const
     {
         if (tri == shooter) return false;                              // pathtracer.h:133
         if (t < near_t && same_location(tri)) return false;             // pathtracer.h:139-166
         const uint32_t m = side[tri];
-        return mode == 0 ? ((m & (front ? 1u : 2u)) != 0) : ((m & 4u) != 0);
+        const bool ok = mode == 0 ? ((m & (front ? 1u : 2u)) != 0) : ((m & 4u) != 0);
+        if constexpr (MATS) {
+            // a keyed texel that is fully transparent lets the ray through (pathtracer.h:194-226,
+            // RRObject.cpp:225-275)
+            const uint32_t mi = ok ? pm.tri_mat[tri] : 0xffffffffu;
+            if (mi != 0xffffffffu) {
+                const lsg_point_material& mat = pm.mats[mi];
+                if (qpoint > mat.min_quality && mat.transmittance_texture >= 0
+                    && pm_transparent(pm, mat, pm_uv(pm, tri, u, v), qinterp > mat.min_quality))
+                    return false;
+            }
+        }
+        return ok;
     }
     // shadow rays only need to know whether a blocker lies below rugDistance
     __device__ bool done(float t) const { return mode == 1 && t <= rug_below; }
     __device__ float clamp(float best) const { return mode == 1 ? fminf(best, rug_below) : best; }
 };
+using GatherFilter = GatherFilterT<false>;
+
+// the filter of a hemisphere (mode 0) or shadow (mode 1) ray of shooter tri; point
+// materials are read at quality (hemisphere, nearest texel) or 2 x quality (shadow rays,
+// bilinear when quality / 10 exceeds the material's minimum), gather.cpp:175-180, 366-370
+template <int FEAT>
+__device__ inline GatherFilterT<(FEAT & kFeatMats) != 0> gather_filter(const GatherArgs& A, uint32_t tri, float near_t, int mode,
+                                                                       float rug_below)
+{
+    GatherFilterT<(FEAT & kFeatMats) != 0> f;
+    f.verts = A.sc.verts; f.idx = A.sc.idx; f.side = A.sc.side;
+    f.shooter = tri; f.near_t = near_t; f.mode = mode; f.rug_below = rug_below;
+    if constexpr ((FEAT & kFeatMats) != 0) {
+        f.pm = A.sc.pm;
+        f.qpoint = mode == 0 ? A.quality : 2 * A.quality;
+        f.qinterp = mode == 0 ? 0u : A.quality / 10;
+    }
+    return f;
+}
 
 // The shooter of subtexel j: seeded jitter -> origin o on the subtexel's triangle tri,
 // and the normalised interpolated vertex normal at the subtexel centroid
@@ -363,19 +405,58 @@ __global__ __launch_bounds__(256) void k_subtexel_walk(const lsg_texel* __restri
 // at the hit's hitPoint2d (pathtracer.cpp:326-348, Triangle::getPointMeasure
 // rrcore.cpp:262-273).  (u, v) is recomputed by the hit's own triangle test on the
 // recorded distance, bit-identical to the traversal's.
-__device__ inline float4 hit_exitance(const SceneView& S, int tri, bool front, const float3 o, const float3 d, float t)
+template <bool MATS = false>
+__device__ inline float4 hit_exitance(const SceneView& S, int tri, bool front, const float3 o, const float3 d, float t,
+                                      uint32_t quality = 0)
 {
     float4 e = S.exitance[2 * (size_t)tri + (front ? 0 : 1)];
-    if (S.ind && (S.side[tri] & (front ? LSG_SIDE_FRONT_INDIRECT : LSG_SIDE_BACK_INDIRECT))) {
-        const TriRest* tr = S.col.rest + S.slot[tri];
-        float u = 0.f, v = 0.f;
-        srlnp_test(tr->sv, tr->ev, (int)tr->meta.y, o, d, t, u, v);
-        const float4 c0 = S.ind[3 * (size_t)tri], c1 = S.ind[3 * (size_t)tri + 1], c2 = S.ind[3 * (size_t)tri + 2];
-        const float w = 1 - u - v;
-        e.x = e.x + (c0.x * w + c1.x * u + c2.x * v) * c0.w;
-        e.y = e.y + (c0.y * w + c1.y * u + c2.y * v) * c1.w;
-        e.z = e.z + (c0.z * w + c1.z * u + c2.z * v) * c2.w;
+    const bool ind = S.ind && (S.side[tri] & (front ? LSG_SIDE_FRONT_INDIRECT : LSG_SIDE_BACK_INDIRECT));
+    const lsg_point_material* mat = nullptr;
+    if constexpr (MATS) {
+        const uint32_t mi = S.pm.tri_mat[tri];
+        if (mi != 0xffffffffu) mat = S.pm.mats + mi;
     }
+    if (!ind && !mat) return e;
+    const bool point = mat && quality > mat->min_quality;
+    float u = 0.f, v = 0.f;
+    if (ind || point) {
+        const TriRest* tr = S.col.rest + S.slot[tri];
+        srlnp_test(tr->sv, tr->ev, (int)tr->meta.y, o, d, t, u, v);
+    }
+    const float w = 1 - u - v;
+    float ix = 0.f, iy = 0.f, iz = 0.f, ax = 0.f, ay = 0.f, az = 0.f;
+    if (ind) {
+        const float4 c0 = S.ind[3 * (size_t)tri], c1 = S.ind[3 * (size_t)tri + 1], c2 = S.ind[3 * (size_t)tri + 2];
+        ix = c0.x * w + c1.x * u + c2.x * v; iy = c0.y * w + c1.y * u + c2.y * v; iz = c0.z * w + c1.z * u + c2.z * v;
+        ax = c0.w; ay = c1.w; az = c2.w;
+    }
+    if (!mat) {
+        e.x = e.x + ix * ax; e.y = e.y + iy * ay; e.z = e.z + iz * az;
+        return e;
+    }
+    // a triangle with a material: its table entry holds the side's direct irradiance, and
+    // the hit composes the exitance as getIncidentRadiance does (pathtracer.cpp:235-262,
+    // 306-316, 326-348): direct x diffuse colour, + emittance on a side that emits, +
+    // indirect x diffuse colour; the colours are the texel's where the quality reads
+    // point materials, else the triangle's
+    float2 muv = make_float2(0.f, 0.f);
+    if (point) muv = pm_uv(S.pm, (uint32_t)tri, u, v);
+    float rx = mat->albedo[0], ry = mat->albedo[1], rz = mat->albedo[2];
+    if (point && mat->diffuse_texture >= 0) {
+        const float4 c = pm_diffuse(S.pm, *mat, muv);
+        rx = c.x; ry = c.y; rz = c.z;
+    }
+    e.x = e.x * rx; e.y = e.y * ry; e.z = e.z * rz;
+    if (mat->emit_sides & (front ? 1u : 2u)) {
+        if (point && mat->emissive_texture >= 0) {
+            const float4 c = pm_emissive(S.pm, *mat, muv);
+            const float k = mat->emittance_multiplier;
+            e.x = e.x + c.x * k; e.y = e.y + c.y * k; e.z = e.z + c.z * k;
+        } else {
+            e.x = e.x + mat->emittance[0]; e.y = e.y + mat->emittance[1]; e.z = e.z + mat->emittance[2];
+        }
+    }
+    if (ind) { e.x = e.x + ix * rx; e.y = e.y + iy * ry; e.z = e.z + iz * rz; }
     return e;
 }
 
@@ -479,9 +560,10 @@ __device__ inline float3 layer_dir(int i, const float3& t, const float3& b, cons
 // (gather.cpp:284-305 hemisphere, :453-481 lights), and the bent normal
 // (gather.cpp:306, :482, :809-830), into A.layer[0..4].
 //
-// ENV: hemisphere rays that leave the scene sample the environment (a separate
-// instantiation, so that scenes without one keep the smaller register footprint).
-template <int STACK, bool STATS, int G = 64, bool LAYERS = false, bool ENV = false>
+// FEAT: kFeatEnv = hemisphere rays that leave the scene sample the environment;
+// kFeatMats = point materials (textures and keyed transparency read at hits).  Each is a
+// separate instantiation, so that scenes without them keep the smaller register footprint.
+template <int STACK, bool STATS, int G = 64, bool LAYERS = false, int FEAT = 0>
 __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, int32_t* mystk, uint32_t* lq, uint16_t* lqj,
                              int stride, float* subc = nullptr)
 {
@@ -562,7 +644,7 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
 #pragma unroll
                     for (int i = 0; i < (LAYERS ? 4 : 1); i++) ld[i] = layer_dir(i, tg, bt, make_float3(nx, ny, nz));
                 }
-                GatherFilter flt{A.sc.verts, A.sc.idx, A.sc.side, tri, near_t, 1, rug_below};
+                auto flt = gather_filter<FEAT>(A, tri, near_t, 1, rug_below);
                 for (uint32_t li = 0; li < nl; li++) {
                     const lsg_light L = A.lights[li];
                     float dx, dy, dz;
@@ -577,7 +659,7 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
                     if (L.cast_shadows) {
                         TraceResult r;
                         uint32_t nv_ = 0, nt_ = 0;
-                        trace_closest<STACK, STATS, GatherFilter, false>(flt, A.sc.col, o, make_float3(dx, dy, dz), A.tmin, dsz, mystk, stride, r, nv_, nt_);
+                        trace_closest<STACK, STATS, decltype(flt), false>(flt, A.sc.col, o, make_float3(dx, dy, dz), A.tmin, dsz, mystk, stride, r, nv_, nt_);
                         if (STATS) { stS[0]++; stS[1] += nv_; stS[2] += nt_; }
                         blocked = r.tri >= 0; bt = r.t;
                     }
@@ -688,7 +770,7 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
                     tx = ox / olen; ty = oy / olen; tz = oz / olen;
                     bx = ny * tz - nz * ty; by = -nx * tz + nz * tx; bz = nx * ty - ny * tx;
                 }
-                GatherFilter flt{A.sc.verts, A.sc.idx, A.sc.side, tri, near_t, 0, rug_below};
+                auto flt = gather_filter<FEAT>(A, tri, near_t, 0, rug_below);
                 const float x = my_x, y = my_y;
                 const float dist = x * x + y * y;
                 const float cosa = sqrtf(1 - dist);                          // gather.cpp:125-126
@@ -724,8 +806,8 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
                         hr += e.x; hg += e.y; hb += e.z;
                     }
                 };
-                if (r.tri >= 0) add(hit_exitance(A.sc, r.tri, r.front, o, d, r.t));
-                else if constexpr (ENV) {
+                if (r.tri >= 0) add(hit_exitance<(FEAT & kFeatMats) != 0>(A.sc, r.tri, r.front, o, d, r.t, A.quality));
+                else if constexpr ((FEAT & kFeatEnv) != 0) {
                     const float4 v = env_at(A.sc, d);
                     add(make_float4(v.x * A.env_mult, v.y * A.env_mult, v.z * A.env_mult, 0.f));
                 }
@@ -810,7 +892,7 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
 // batch, refills pause until every lane is done, and the rest at the series' end.
 constexpr uint32_t kGatherRefill = 16;
 
-template <int STACK, bool ENV>
+template <int STACK, int FEAT>
 __device__ void gather_texel_refill(const GatherArgs& A, uint32_t ti, uint32_t lane, int32_t* mystk, uint32_t* lq, uint16_t* lqj,
                                     int stride, float* subc)
 {
@@ -861,7 +943,7 @@ __device__ void gather_texel_refill(const GatherArgs& A, uint32_t ti, uint32_t l
                 const uint32_t s = lq0[qi], sj = lqj0[qi];
                 float3 o; float nx, ny, nz; uint32_t tri;
                 shooter_cached(A, ts, sj, key, shooter_base + s, subc, o, nx, ny, nz, tri);
-                GatherFilter flt{A.sc.verts, A.sc.idx, A.sc.side, tri, near_t, 1, rug_below};
+                auto flt = gather_filter<FEAT>(A, tri, near_t, 1, rug_below);
                 for (uint32_t li = 0; li < nl; li++) {
                     const lsg_light L = A.lights[li];
                     float dx, dy, dz;
@@ -877,7 +959,7 @@ __device__ void gather_texel_refill(const GatherArgs& A, uint32_t ti, uint32_t l
                         TraceResult r;
                         uint32_t nv_ = 0, nt_ = 0;
                         // min/max visit: this kernel keeps a hemisphere ray's state alongside
-                        trace_closest<STACK, false, GatherFilter, false, false>(flt, A.sc.col, o, make_float3(dx, dy, dz), A.tmin, dsz, mystk, stride, r, nv_, nt_);
+                        trace_closest<STACK, false, decltype(flt), false, false>(flt, A.sc.col, o, make_float3(dx, dy, dz), A.tmin, dsz, mystk, stride, r, nv_, nt_);
                         blocked = r.tri >= 0; bt = r.t;
                     }
                     if (!blocked) {
@@ -981,7 +1063,7 @@ __device__ void gather_texel_refill(const GatherArgs& A, uint32_t ti, uint32_t l
                     if (trace_begin(S, A.sc.col, o, d, A.tmin, A.locality)) busy = true;
                     else {
                         // the ray leaves the scene box: environment or nothing (pathtracer.cpp:156-173)
-                        if constexpr (ENV) {
+                        if constexpr ((FEAT & kFeatEnv) != 0) {
                             const float4 v = env_at(A.sc, d);
                             hr += v.x * A.env_mult; hg += v.y * A.env_mult; hb += v.z * A.env_mult;
                         }
@@ -993,15 +1075,15 @@ __device__ void gather_texel_refill(const GatherArgs& A, uint32_t ti, uint32_t l
                 // trace until the ray ends or enough lanes are free to hand out more shooters
                 const bool refill = next < n && qtail - qhead < 64u;
                 const uint32_t yield_at = refill ? 64u - kGatherRefill : 0u;
-                GatherFilter flt{A.sc.verts, A.sc.idx, A.sc.side, stri, near_t, 0, rug_below};
+                auto flt = gather_filter<FEAT>(A, stri, near_t, 0, rug_below);
                 if (trace_resume<STACK>(S, flt, A.sc.col, mystk, stride, yield_at, ok)) {
                     busy = false;
                     // a hit reads the exitance of the side it sees; a ray that leaves the
                     // scene returns environment * multiplier, or nothing
                     if (S.bi >= 0) {
-                        const float4 e = hit_exitance(A.sc, S.bi, S.bf, S.o, S.d, S.best);
+                        const float4 e = hit_exitance<(FEAT & kFeatMats) != 0>(A.sc, S.bi, S.bf, S.o, S.d, S.best, A.quality);
                         hr += e.x; hg += e.y; hb += e.z;
-                    } else if constexpr (ENV) {
+                    } else if constexpr ((FEAT & kFeatEnv) != 0) {
                         const float4 v = env_at(A.sc, S.d);
                         hr += v.x * A.env_mult; hg += v.y * A.env_mult; hb += v.z * A.env_mult;
                     }
@@ -1046,7 +1128,7 @@ __device__ void gather_texel_refill(const GatherArgs& A, uint32_t ti, uint32_t l
 // lanes idle while a texel's shadow rays run; nothing is shared between lanes.  The rays
 // are those of gather_texel (the same filler indices in shooter order, the same origins
 // and shots), summed per lane instead of across a group.
-template <int STACK, bool ENV>
+template <int STACK, int FEAT>
 __device__ void gather_texel_lane(const GatherArgs& A, uint32_t ti, int32_t* stk, int stride)
 {
     const lsg_texel T = A.texels[ti];
@@ -1099,23 +1181,23 @@ __device__ void gather_texel_lane(const GatherArgs& A, uint32_t ti, int32_t* stk
                 const float dist = x * x + y * y;
                 const float cosa = sqrtf(1 - dist);                          // gather.cpp:125-126
                 const float3 d = make_float3(nx * cosa + tx * x + bx * y, ny * cosa + ty * x + by * y, nz * cosa + tz * x + bz * y);
-                GatherFilter flt{A.sc.verts, A.sc.idx, A.sc.side, tri, near_t, 0, rug_below};
+                auto flt = gather_filter<FEAT>(A, tri, near_t, 0, rug_below);
                 TraceResult r;
                 uint32_t nv_ = 0, nt_ = 0;
                 // min/max visit: measured faster here (the near/far form raised this kernel
                 // from 109 to 122 VGPRs and the 10M-triangle city's cheap gathers slowed)
-                trace_closest<STACK, false, GatherFilter, true, false>(flt, A.sc.col, o, d, A.tmin, A.locality, stk, stride, r, nv_, nt_);
+                trace_closest<STACK, false, decltype(flt), true, false>(flt, A.sc.col, o, d, A.tmin, A.locality, stk, stride, r, nv_, nt_);
                 if (r.tri >= 0) {
-                    const float4 e = hit_exitance(A.sc, r.tri, r.front, o, d, r.t);
+                    const float4 e = hit_exitance<(FEAT & kFeatMats) != 0>(A.sc, r.tri, r.front, o, d, r.t, A.quality);
                     hr += e.x; hg += e.y; hb += e.z;
-                } else if constexpr (ENV) {
+                } else if constexpr ((FEAT & kFeatEnv) != 0) {
                     const float4 v = env_at(A.sc, d);
                     hr += v.x * A.env_mult; hg += v.y * A.env_mult; hb += v.z * A.env_mult;
                 }
                 relH++;
             }
             if (doL) {
-                GatherFilter flt{A.sc.verts, A.sc.idx, A.sc.side, tri, near_t, 1, rug_below};
+                auto flt = gather_filter<FEAT>(A, tri, near_t, 1, rug_below);
                 for (uint32_t li = 0; li < nl; li++) {
                     const lsg_light L = A.lights[li];
                     float dx, dy, dz;
@@ -1130,7 +1212,7 @@ __device__ void gather_texel_lane(const GatherArgs& A, uint32_t ti, int32_t* stk
                     if (L.cast_shadows) {
                         TraceResult r;
                         uint32_t nv_ = 0, nt_ = 0;
-                        trace_closest<STACK, false, GatherFilter, false, false>(flt, A.sc.col, o, make_float3(dx, dy, dz), A.tmin, dsz, stk, stride, r, nv_, nt_);
+                        trace_closest<STACK, false, decltype(flt), false, false>(flt, A.sc.col, o, make_float3(dx, dy, dz), A.tmin, dsz, stk, stride, r, nv_, nt_);
                         blocked = r.tri >= 0; bt = r.t;
                     }
                     if (!blocked) {
@@ -1172,20 +1254,20 @@ __device__ void gather_texel_lane(const GatherArgs& A, uint32_t ti, int32_t* stk
 constexpr uint32_t kLaneRounds = 16;
 
 // low ray counts without output layers: one texel per lane, 256 per block
-template <int STACK, bool ENV = false>
+template <int STACK, int FEAT = 0>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_gather_lanes(GatherArgs A)
 {
     __shared__ int32_t stk[STACK * 256];
     const uint32_t tid = threadIdx.x;
     const uint32_t ti = blockIdx.x * 256 + tid;
     if (ti >= A.ntex) return;
-    gather_texel_lane<STACK, ENV>(A, ti, stk + tid, 256);
+    gather_texel_lane<STACK, FEAT>(A, ti, stk + tid, 256);
 }
 
 // One texel per wave, 4 per block.  The plain and environment kernels are capped at 128
 // VGPRs (4 waves per SIMD, measured faster than 3 with more registers); the layered ones
 // hold 12 more sums per lane and are left uncapped rather than spill.
-template <int STACK, bool STATS, bool LAYERS, bool ENV>
+template <int STACK, bool STATS, bool LAYERS, int FEAT>
 __device__ inline void gather_block(const GatherArgs& A)
 {
     __shared__ int32_t stk[STACK * 256];
@@ -1196,24 +1278,24 @@ __device__ inline void gather_block(const GatherArgs& A)
     // wave-uniform: kept in a scalar register, the texel record is read by scalar loads
     const uint32_t ti = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4u + (tid >> 6)));
     if (ti >= A.ntex) return;                 // whole wavefront leaves together
-    gather_texel<STACK, STATS, 64, LAYERS, ENV>(A, ti, lane, stk + tid, lq + tid, lqj + tid, 256,
+    gather_texel<STACK, STATS, 64, LAYERS, FEAT>(A, ti, lane, stk + tid, lq + tid, lqj + tid, 256,
                                                 subc + (tid >> 6) * kFrame * kSubCache);
 }
 
-template <int STACK, bool STATS, bool ENV = false>
+template <int STACK, bool STATS, int FEAT = 0>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_gather(GatherArgs A)
 {
-    gather_block<STACK, STATS, false, ENV>(A);
+    gather_block<STACK, STATS, false, FEAT>(A);
 }
 
-template <int STACK, bool ENV>
+template <int STACK, int FEAT>
 __global__ __launch_bounds__(256) void k_gather_layers(GatherArgs A)
 {
-    gather_block<STACK, false, true, ENV>(A);
+    gather_block<STACK, false, true, FEAT>(A);
 }
 
 // low ray counts (at most G shooters per series): 256/G texels per block, G lanes each
-template <int STACK, int G, bool LAYERS, bool ENV>
+template <int STACK, int G, bool LAYERS, int FEAT>
 __device__ inline void gather_groups(const GatherArgs& A)
 {
     __shared__ int32_t stk[STACK * 256];
@@ -1222,19 +1304,19 @@ __device__ inline void gather_groups(const GatherArgs& A)
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const uint32_t ti = blockIdx.x * (256u / G) + tid / G;
     if (ti >= A.ntex) return;                 // the whole group leaves together
-    gather_texel<STACK, false, G, LAYERS, ENV>(A, ti, lane, stk + tid, lq + tid, lqj + tid, 256);
+    gather_texel<STACK, false, G, LAYERS, FEAT>(A, ti, lane, stk + tid, lq + tid, lqj + tid, 256);
 }
 
-template <int STACK, int G, bool ENV = false>
+template <int STACK, int G, int FEAT = 0>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_gather_grouped(GatherArgs A)
 {
-    gather_groups<STACK, G, false, ENV>(A);
+    gather_groups<STACK, G, false, FEAT>(A);
 }
 
-template <int STACK, int G, bool ENV>
+template <int STACK, int G, int FEAT>
 __global__ __launch_bounds__(256) void k_gather_grouped_layers(GatherArgs A)
 {
-    gather_groups<STACK, G, true, ENV>(A);
+    gather_groups<STACK, G, true, FEAT>(A);
 }
 
 __global__ void k_queue_reset(uint32_t* counters, uint32_t n)
@@ -1244,7 +1326,7 @@ __global__ void k_queue_reset(uint32_t* counters, uint32_t n)
 
 // Capped at 128 VGPRs it runs 4 waves per SIMD, measured faster than 3 with more
 // registers; the chunked form fits, the refill form (REFILL) spills 44 bytes.
-template <int STACK, int WAVES, bool ENV = false, bool REFILL = false>
+template <int STACK, int WAVES, int FEAT = 0, bool REFILL = false>
 __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4))) void k_gather_persistent(GatherArgs A, WorkQueue Q)
 {
     __shared__ int32_t stk[STACK * 64 * WAVES];
@@ -1265,9 +1347,9 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4)))
             if (tile >= t1) break;           // this queue is exhausted for every lane of the wave
             const uint32_t a = tile * Q.tile, b = min(A.ntex, a + Q.tile);
             for (uint32_t ti = a; ti < b; ti++)
-                if constexpr (REFILL) gather_texel_refill<STACK, ENV>(A, ti, lane, stk + tid, lq + tid, lqj + tid, 64 * WAVES,
+                if constexpr (REFILL) gather_texel_refill<STACK, FEAT>(A, ti, lane, stk + tid, lq + tid, lqj + tid, 64 * WAVES,
                                                                      subc + (tid >> 6) * kFrame * kSubCache);
-                else gather_texel<STACK, false, 64, false, ENV>(A, ti, lane, stk + tid, lq + tid, lqj + tid, 64 * WAVES,
+                else gather_texel<STACK, false, 64, false, FEAT>(A, ti, lane, stk + tid, lq + tid, lqj + tid, 64 * WAVES,
                                                                 subc + (tid >> 6) * kFrame * kSubCache);
         }
     }
@@ -1333,13 +1415,19 @@ int gather_args(lsg_scene* s, const lsg_texel* texels, uint32_t ntexels, const l
     for (int i = 0; i < 5; i++) A.layer[i] = nullptr;
     A.dirs = 0; A.layered = 0;
     A.texels = texels; A.ntex = ntexels; A.subs = subs; A.out = (float4*)out_rgba; A.stats = nullptr; A.parts = nullptr;
-    A.quality = p->quality;
+    A.quality = p->quality > 1 ? p->quality : 1;      // gather.cpp:865
     A.hemi_rays = (p->hemisphere || env) ? (p->quality > 1 ? p->quality : 1) : 0;    // gather.cpp:197
     A.rounds = (p->light_multiplier != 0 && s->impl.nlights) ? (p->light_rounds ? p->light_rounds : p->quality / 10 + 1) : 0; // gather.cpp:396
     A.tmin = p->ray_length_min; A.locality = p->locality; A.rug = p->rug_distance;
     A.light_mult = p->light_multiplier; A.seed = p->seed;
     A.lights = s->impl.d_lights; A.nlights = s->impl.nlights;
     A.sc.ind = s->impl.d_ind; A.sc.slot = C.d_slot;
+    A.sc.pm.tri_mat = s->impl.d_tri_mat; A.sc.pm.tri_uv = s->impl.d_tri_uv; A.sc.pm.mats = s->impl.d_mats;
+    A.sc.pm.tex = s->impl.d_tex; A.sc.pm.texels = s->impl.d_texels;
+    if (s->impl.d_tri_mat) {
+        const int e = check_point_quality(s->impl, A.quality);
+        if (e) return e;
+    }
     A.walk = nullptr; A.walk_buf = nullptr; A.walk_cap = 0;
     return LSG_OK;
 }
@@ -1357,19 +1445,19 @@ int scene_walk_scratch(lsg_scene* s, uint32_t n, GatherArgs& A)
     return LSG_OK;
 }
 
-template <bool ENV, bool REFILL>
+template <int FEAT, bool REFILL>
 static void launch_persistent(const GatherArgs& A, int stack, const WorkQueue& Q, uint32_t waves_per_block, uint32_t blocks,
                               hipStream_t st)
 {
     if (waves_per_block == 8) {
-        if (stack <= 32) k_gather_persistent<32, 8, ENV, REFILL><<<blocks, 512, 0, st>>>(A, Q);
-        else k_gather_persistent<64, 8, ENV, REFILL><<<blocks, 512, 0, st>>>(A, Q);
+        if (stack <= 32) k_gather_persistent<32, 8, FEAT, REFILL><<<blocks, 512, 0, st>>>(A, Q);
+        else k_gather_persistent<64, 8, FEAT, REFILL><<<blocks, 512, 0, st>>>(A, Q);
     } else if (waves_per_block == 2) {
-        if (stack <= 32) k_gather_persistent<32, 2, ENV, REFILL><<<blocks, 128, 0, st>>>(A, Q);
-        else k_gather_persistent<64, 2, ENV, REFILL><<<blocks, 128, 0, st>>>(A, Q);
+        if (stack <= 32) k_gather_persistent<32, 2, FEAT, REFILL><<<blocks, 128, 0, st>>>(A, Q);
+        else k_gather_persistent<64, 2, FEAT, REFILL><<<blocks, 128, 0, st>>>(A, Q);
     } else {
-        if (stack <= 32) k_gather_persistent<32, 4, ENV, REFILL><<<blocks, 256, 0, st>>>(A, Q);
-        else k_gather_persistent<64, 4, ENV, REFILL><<<blocks, 256, 0, st>>>(A, Q);
+        if (stack <= 32) k_gather_persistent<32, 4, FEAT, REFILL><<<blocks, 256, 0, st>>>(A, Q);
+        else k_gather_persistent<64, 4, FEAT, REFILL><<<blocks, 256, 0, st>>>(A, Q);
     }
 }
 
@@ -1384,12 +1472,20 @@ int launch_gather_persistent(const GatherArgs& A, int stack, const WorkQueue& Q,
     }
     // shooters per series above 32: the texel's lanes may take shooters as they free up
     refill = refill && std::max(B.hemi_rays, B.rounds) > 32u;
-    if (B.sc.env_type) {
-        if (refill) launch_persistent<true, true>(B, stack, Q, waves_per_block, blocks, st);
-        else launch_persistent<true, false>(B, stack, Q, waves_per_block, blocks, st);
+    if (B.sc.pm.tri_mat) {
+        if (B.sc.env_type) {
+            if (refill) launch_persistent<kFeatEnv | kFeatMats, true>(B, stack, Q, waves_per_block, blocks, st);
+            else launch_persistent<kFeatEnv | kFeatMats, false>(B, stack, Q, waves_per_block, blocks, st);
+        } else {
+            if (refill) launch_persistent<kFeatMats, true>(B, stack, Q, waves_per_block, blocks, st);
+            else launch_persistent<kFeatMats, false>(B, stack, Q, waves_per_block, blocks, st);
+        }
+    } else if (B.sc.env_type) {
+        if (refill) launch_persistent<kFeatEnv, true>(B, stack, Q, waves_per_block, blocks, st);
+        else launch_persistent<kFeatEnv, false>(B, stack, Q, waves_per_block, blocks, st);
     } else {
-        if (refill) launch_persistent<false, true>(B, stack, Q, waves_per_block, blocks, st);
-        else launch_persistent<false, false>(B, stack, Q, waves_per_block, blocks, st);
+        if (refill) launch_persistent<0, true>(B, stack, Q, waves_per_block, blocks, st);
+        else launch_persistent<0, false>(B, stack, Q, waves_per_block, blocks, st);
     }
     LSG_HIP(hipGetLastError());
     return LSG_OK;
@@ -1457,7 +1553,7 @@ int walk_records(lsg_scene* s, GatherArgs& A)
     return scene_walk_scratch(s, n, A);
 }
 
-template <bool LAYERS, bool ENV>
+template <bool LAYERS, int FEAT>
 static void launch_gather_kernels(GatherArgs A, int stack, hipStream_t st)
 {
     // shooters per series = max(hemisphere rays, light rounds); a group of G >= that
@@ -1473,41 +1569,51 @@ static void launch_gather_kernels(GatherArgs A, int stack, hipStream_t st)
         const unsigned grid = (unsigned)(((uint64_t)A.ntex * G + 255) / 256);
         if constexpr (LAYERS) {
             if (G == 16) {
-                if (stack <= 32) k_gather_grouped_layers<32, 16, ENV><<<grid, 256, 0, st>>>(A);
-                else k_gather_grouped_layers<64, 16, ENV><<<grid, 256, 0, st>>>(A);
+                if (stack <= 32) k_gather_grouped_layers<32, 16, FEAT><<<grid, 256, 0, st>>>(A);
+                else k_gather_grouped_layers<64, 16, FEAT><<<grid, 256, 0, st>>>(A);
             } else {
-                if (stack <= 32) k_gather_grouped_layers<32, 32, ENV><<<grid, 256, 0, st>>>(A);
-                else k_gather_grouped_layers<64, 32, ENV><<<grid, 256, 0, st>>>(A);
+                if (stack <= 32) k_gather_grouped_layers<32, 32, FEAT><<<grid, 256, 0, st>>>(A);
+                else k_gather_grouped_layers<64, 32, FEAT><<<grid, 256, 0, st>>>(A);
             }
         } else if (lanes) {
             const unsigned lgrid = (A.ntex + 255) / 256;
-            if (stack <= 32) k_gather_lanes<32, ENV><<<lgrid, 256, 0, st>>>(A);
-            else k_gather_lanes<64, ENV><<<lgrid, 256, 0, st>>>(A);
+            if (stack <= 32) k_gather_lanes<32, FEAT><<<lgrid, 256, 0, st>>>(A);
+            else k_gather_lanes<64, FEAT><<<lgrid, 256, 0, st>>>(A);
         } else if (G == 16) {
-            if (stack <= 32) k_gather_grouped<32, 16, ENV><<<grid, 256, 0, st>>>(A);
-            else k_gather_grouped<64, 16, ENV><<<grid, 256, 0, st>>>(A);
+            if (stack <= 32) k_gather_grouped<32, 16, FEAT><<<grid, 256, 0, st>>>(A);
+            else k_gather_grouped<64, 16, FEAT><<<grid, 256, 0, st>>>(A);
         } else {
-            if (stack <= 32) k_gather_grouped<32, 32, ENV><<<grid, 256, 0, st>>>(A);
-            else k_gather_grouped<64, 32, ENV><<<grid, 256, 0, st>>>(A);
+            if (stack <= 32) k_gather_grouped<32, 32, FEAT><<<grid, 256, 0, st>>>(A);
+            else k_gather_grouped<64, 32, FEAT><<<grid, 256, 0, st>>>(A);
         }
         return;
     }
     const unsigned grid = (A.ntex + 3) / 4;
     if constexpr (LAYERS) {
-        if (stack <= 32) k_gather_layers<32, ENV><<<grid, 256, 0, st>>>(A);
-        else k_gather_layers<64, ENV><<<grid, 256, 0, st>>>(A);
+        if (stack <= 32) k_gather_layers<32, FEAT><<<grid, 256, 0, st>>>(A);
+        else k_gather_layers<64, FEAT><<<grid, 256, 0, st>>>(A);
     } else {
-        if (stack <= 32) k_gather<32, false, ENV><<<grid, 256, 0, st>>>(A);
-        else k_gather<64, false, ENV><<<grid, 256, 0, st>>>(A);
+        if (stack <= 32) k_gather<32, false, FEAT><<<grid, 256, 0, st>>>(A);
+        else k_gather<64, false, FEAT><<<grid, 256, 0, st>>>(A);
     }
 }
 
 static int launch_gather_one(const GatherArgs& A, int stack, hipStream_t st)
 {
     if (!A.ntex) return LSG_OK;
-    const bool env = A.sc.env_type != 0;
-    if (A.layered) { if (env) launch_gather_kernels<true, true>(A, stack, st); else launch_gather_kernels<true, false>(A, stack, st); }
-    else { if (env) launch_gather_kernels<false, true>(A, stack, st); else launch_gather_kernels<false, false>(A, stack, st); }
+    switch (features(A)) {
+    case 0: if (A.layered) launch_gather_kernels<true, 0>(A, stack, st); else launch_gather_kernels<false, 0>(A, stack, st); break;
+    case kFeatEnv:
+        if (A.layered) launch_gather_kernels<true, kFeatEnv>(A, stack, st); else launch_gather_kernels<false, kFeatEnv>(A, stack, st);
+        break;
+    case kFeatMats:
+        if (A.layered) launch_gather_kernels<true, kFeatMats>(A, stack, st); else launch_gather_kernels<false, kFeatMats>(A, stack, st);
+        break;
+    default:
+        if (A.layered) launch_gather_kernels<true, kFeatEnv | kFeatMats>(A, stack, st);
+        else launch_gather_kernels<false, kFeatEnv | kFeatMats>(A, stack, st);
+        break;
+    }
     LSG_HIP(hipGetLastError());
     return LSG_OK;
 }
@@ -1611,6 +1717,7 @@ void lsg_scene_destroy(lsg_scene* s)
     dfree(s->impl.d_vnormal); dfree(s->impl.d_side); dfree(s->impl.d_exit); dfree(s->impl.d_lights);
     dfree(s->impl.d_vtan); dfree(s->impl.d_vbit); dfree(s->impl.d_env); dfree(s->impl.d_lmaps);
     dfree(s->impl.d_ind); dfree(s->impl.d_walk);
+    free_point_materials(s->impl);
     delete s;
 }
 
@@ -1702,12 +1809,20 @@ static int gather_launch(lsg_scene* s, const lsg_texel* texels, uint32_t ntexels
     if ((st = scene_walk_scratch(s, ntexels, A))) return st;
     k_subtexel_walk<<<(ntexels + 255) / 256, 256, 0, hs>>>(texels, ntexels, subs, first_series(A), (WalkRec*)A.walk_buf);
     A.walk = (const WalkRec*)A.walk_buf;
-    if (A.sc.env_type) {
-        if (stack <= 32) k_gather<32, true, true><<<grid, 256, 0, hs>>>(A);
-        else k_gather<64, true, true><<<grid, 256, 0, hs>>>(A);
-    } else {
-        if (stack <= 32) k_gather<32, true><<<grid, 256, 0, hs>>>(A);
-        else k_gather<64, true><<<grid, 256, 0, hs>>>(A);
+    switch (features(A)) {
+    case 0:
+        if (stack <= 32) k_gather<32, true, 0><<<grid, 256, 0, hs>>>(A); else k_gather<64, true, 0><<<grid, 256, 0, hs>>>(A);
+        break;
+    case kFeatEnv:
+        if (stack <= 32) k_gather<32, true, kFeatEnv><<<grid, 256, 0, hs>>>(A); else k_gather<64, true, kFeatEnv><<<grid, 256, 0, hs>>>(A);
+        break;
+    case kFeatMats:
+        if (stack <= 32) k_gather<32, true, kFeatMats><<<grid, 256, 0, hs>>>(A); else k_gather<64, true, kFeatMats><<<grid, 256, 0, hs>>>(A);
+        break;
+    default:
+        if (stack <= 32) k_gather<32, true, kFeatEnv | kFeatMats><<<grid, 256, 0, hs>>>(A);
+        else k_gather<64, true, kFeatEnv | kFeatMats><<<grid, 256, 0, hs>>>(A);
+        break;
     }
     LSG_HIP(hipGetLastError());
     return LSG_OK;

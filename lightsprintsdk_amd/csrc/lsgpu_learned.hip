@@ -364,8 +364,10 @@ int lsg_learned_features(lsg_scene* s, const lsg_texel* texels, uint32_t n, cons
         if (k && maps[k].rgba_base < maps[k - 1].rgba_base + maps[k - 1].width * maps[k - 1].height)
             return fail(LSG_ERR_INVALID, "lsg_learned_features: lightmaps must be sorted by rgba_base and disjoint");
     }
+    if (s->impl.d_tri_mat)
+        return fail(LSG_ERR_INVALID, "lsg_learned_features: the learned features read flat exitance; scenes with point materials are not supported");
     const ColliderImpl& C = s->impl.col->impl;
-    SceneView sv;
+    SceneView sv{};
     sv.col = C.view(); sv.verts = C.d_verts; sv.idx = C.d_idx; sv.vnormal = s->impl.d_vnormal;
     sv.side = s->impl.d_side; sv.exitance = s->impl.d_exit;
     hipStream_t st = (hipStream_t)stream;

@@ -8,8 +8,8 @@
 // lowest triangle index -- independent of traversal order.
 //
 // A Filter plays the role of RRCollisionHandler::collides (RRCollider.h:59): it
-// sees each candidate (triangle, side, distance) and may reject it, and it may end
-// the search early or narrow the interval once a hit is accepted.
+// sees each candidate (triangle, side, distance, hitPoint2d) and may reject it, and it
+// may end the search early or narrow the interval once a hit is accepted.
 #pragma once
 #include "lsgpu_internal.h"
 
@@ -18,7 +18,7 @@ namespace lsg {
 struct TraceResult { int tri; float t, u, v; bool front; uint32_t slot; };
 
 struct NoFilter {
-    __device__ bool accept(uint32_t, bool, float) const { return true; }
+    __device__ bool accept(uint32_t, bool, float, float, float) const { return true; }
     // after accepting a hit at t: true = stop searching
     __device__ bool done(float) const { return false; }
     // after accepting a hit: upper bound of the interval still worth searching
@@ -222,7 +222,7 @@ __device__ inline bool trace_closest(const Filter& flt, const ColliderView& c, c
                 float u, v;
                 if (!srlnp_test(tr->sv, tr->ev, (int)mt.y, o, d, t, u, v)) continue;
                 const bool fr = ndd < 0;
-                if (!flt.accept(mt.x, fr, t)) continue;
+                if (!flt.accept(mt.x, fr, t, u, v)) continue;
                 best = t; bi = (int)mt.x; bu = u; bv = v; bf = fr; bslot = p;
                 bound = flt.clamp(best);
                 if (flt.done(t)) { stop = true; break; }
@@ -294,7 +294,7 @@ __device__ inline bool trace_resume(TraceState& S, const Filter& flt, const Coll
                 float u, v;
                 if (!srlnp_test(tr->sv, tr->ev, (int)mt.y, S.o, S.d, t, u, v)) continue;
                 const bool fr = ndd < 0;
-                if (!flt.accept(mt.x, fr, t)) continue;
+                if (!flt.accept(mt.x, fr, t, u, v)) continue;
                 S.best = t; S.bi = (int)mt.x; S.bu = u; S.bv = v; S.bf = fr;
                 S.bound = flt.clamp(S.best);
                 if (flt.done(t)) { stop = true; break; }

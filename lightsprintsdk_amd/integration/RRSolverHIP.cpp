@@ -5,11 +5,120 @@
 #include "Lightsprint/RRLight.h"
 #include "Lightsprint/RRMaterial.h"
 #include "Lightsprint/RRObject.h"
+#include <climits>
 #include <vector>
 
 namespace rr_hip {
 
 using namespace rr;
+
+namespace {
+
+// textured materials that ask for point materials (minimalQualityForPointMaterials < UINT_MAX,
+// RRObject.cpp:299-301) as the GPU scene's point materials: their textures, read with the
+// solver's colour space, and the texcoord channel all their textured properties share
+struct PointMaterials
+{
+	std::vector<const RRMaterial*> source;
+	std::vector<lsg_point_material> mats;
+	std::vector<unsigned> channel;
+	std::vector<RRBuffer*> texSource;
+	std::vector<lsg_texture> tex;
+	std::vector<std::vector<float>> linear, linearStored;
+
+	static bool wants(const RRMaterial* m)
+	{
+		return m && m->minimalQualityForPointMaterials < UINT_MAX
+			&& (m->diffuseReflectance.texture || m->diffuseEmittance.texture || m->specularTransmittance.texture);
+	}
+
+	int texture(RRBuffer* b, const RRColorSpace* cs)
+	{
+		if (!b) return -1;
+		for (size_t i = 0; i < texSource.size(); i++)
+			if (texSource[i] == b) return (int)i;
+		const RRBufferFormat f = b->getFormat();
+		if (b->getType() != BT_2D_TEXTURE || !(f == BF_RGB || f == BF_BGR || f == BF_RGBA || f == BF_RGBF || f == BF_RGBAF
+			|| f == BF_LUMINANCE || f == BF_LUMINANCEF))
+		{
+			RRReporter::report(ERRO, "RRSolverHIP: point materials need 2D textures of per-texel formats.\n");
+			return -2;
+		}
+		const unsigned n = b->getWidth() * b->getHeight();
+		std::vector<float> lin(4 * (size_t)n), stored(4 * (size_t)n);
+		for (unsigned i = 0; i < n; i++)
+		{
+			const RRVec4 a = b->getElement(i, cs);
+			RRVec4 c = b->getElement(i, nullptr);
+			if (cs) cs->toLinear(c);
+			for (int k = 0; k < 4; k++) { lin[4 * (size_t)i + k] = a[k]; stored[4 * (size_t)i + k] = c[k]; }
+		}
+		linear.push_back(std::move(lin));
+		linearStored.push_back(std::move(stored));
+		lsg_texture t = {};
+		t.width = b->getWidth();
+		t.height = b->getHeight();
+		t.format = (int32_t)f;           // RRBufferFormat values are the LSG_FORMAT_* values
+		t.scaled = b->getScaled() ? 1 : 0;
+		t.data = b->lock(BL_READ);
+		texSource.push_back(b);
+		tex.push_back(t);
+		return (int)tex.size() - 1;
+	}
+
+	// index of m's point material, -1 when it is not supported
+	int material(const RRMaterial* m, float emitMultiplier, const RRColorSpace* cs)
+	{
+		for (size_t i = 0; i < source.size(); i++)
+			if (source[i] == m) return (int)i;
+		// the textured properties are mapped by one texcoord channel
+		unsigned ch = UINT_MAX;
+		for (const RRMaterial::Property* p : {&m->diffuseReflectance, &m->diffuseEmittance, &m->specularTransmittance})
+			if (p->texture)
+			{
+				if (ch != UINT_MAX && p->texcoord != ch)
+				{
+					RRReporter::report(ERRO, "RRSolverHIP: point material textures must share one texcoord channel.\n");
+					return -1;
+				}
+				ch = p->texcoord;
+			}
+		lsg_point_material pm = {};
+		pm.diffuse_texture = texture(m->diffuseReflectance.texture, cs);
+		pm.emissive_texture = texture(m->diffuseEmittance.texture, cs);
+		pm.transmittance_texture = texture(m->specularTransmittance.texture, cs);
+		if (pm.diffuse_texture < -1 || pm.emissive_texture < -1 || pm.transmittance_texture < -1) return -1;
+		if (pm.transmittance_texture >= 0 && m->refractionIndex != 1)
+		{
+			RRReporter::report(ERRO, "RRSolverHIP: keyed transmittance with refraction index %f is not supported.\n", m->refractionIndex);
+			return -1;
+		}
+		pm.flags = (m->specularTransmittanceInAlpha ? LSG_PM_TRANSMITTANCE_IN_ALPHA : 0u)
+			| (m->specularTransmittanceKeyed ? LSG_PM_KEYED : 0u) | (m->specularTransmittanceMapInverted ? LSG_PM_INVERTED : 0u);
+		pm.threshold = m->specularTransmittanceThreshold;
+		pm.min_quality = m->minimalQualityForPointMaterials;
+		for (int c = 0; c < 3; c++)
+		{
+			pm.albedo[c] = m->diffuseReflectance.colorLinear[c];
+			pm.emittance[c] = m->diffuseEmittance.colorLinear[c] * emitMultiplier;
+		}
+		pm.emittance_multiplier = emitMultiplier;
+		pm.emit_sides = (m->sideBits[0].emitTo && m->sideBits[0].legal ? 1u : 0u) | (m->sideBits[1].emitTo && m->sideBits[1].legal ? 2u : 0u);
+		source.push_back(m);
+		mats.push_back(pm);
+		channel.push_back(ch);
+		return (int)mats.size() - 1;
+	}
+
+	void unlockAll()
+	{
+		for (RRBuffer* b : texSource) b->unlock();
+		texSource.clear();
+	}
+	~PointMaterials() { unlockAll(); }
+};
+
+} // namespace
 
 RRSolverHIP::~RRSolverHIP()
 {
@@ -52,6 +161,10 @@ bool RRSolverHIP::buildScene(const UpdateParameters& params)
 	albedo.assign(3 * (size_t)nt, 0.f);
 	emittance.assign(3 * (size_t)nt, 0.f);
 	const float emitMultiplier = params.indirect.materialEmittanceMultiplier;
+	PointMaterials pms;
+	std::vector<uint32_t> triMat(nt, 0xFFFFFFFFu);
+	std::vector<float> triUv(6 * (size_t)nt, 0.f);
+	bool anyPoint = false;
 	for (unsigned t = 0; t < nt; t++)
 	{
 		RRMesh::Triangle tr;
@@ -72,12 +185,26 @@ bool RRSolverHIP::buildScene(const UpdateParameters& params)
 			idx[3 * (size_t)t + k] = 3 * t + k;
 		}
 		const RRMaterial* m = multi->getTriangleMaterial(t, nullptr, nullptr);
+		// a textured material read per hit point: its triangles keep their direct irradiance
+		// in the table, the GPU hit applies the texel's colour and emittance
+		int pmi = -1;
+		if (PointMaterials::wants(m))
+		{
+			pmi = pms.material(m, emitMultiplier, getColorSpace());
+			if (pmi < 0) return false;
+			triMat[t] = (uint32_t)pmi;
+			RRMesh::TriangleMapping map;
+			if (mesh->getTriangleMapping(t, map, pms.channel[pmi]))
+				for (int k = 0; k < 3; k++) { triUv[6 * (size_t)t + 2 * k] = map.uv[k][0]; triUv[6 * (size_t)t + 2 * k + 1] = map.uv[k][1]; }
+			anyPoint = true;
+		}
 		if (m)
 		{
 			// bit0/bit1: front/back side seen by gather rays (pathtracer.h:101-103),
-			// bit2: casts shadows (opaque)
+			// bit2: casts shadows (opaque, or keyed per point: its opaque texels shadow)
+			const bool keyed = pmi >= 0 && pms.mats[pmi].transmittance_texture >= 0;
 			side[t] = (m->sideBits[0].renderFrom ? 1 : 0) | (m->sideBits[1].renderFrom ? 2 : 0)
-			        | (m->specularTransmittance.color == RRVec3(0) ? 4 : 0);
+			        | (m->specularTransmittance.color == RRVec3(0) || keyed ? 4 : 0);
 			for (int c = 0; c < 3; c++)
 			{
 				albedo[3 * (size_t)t + c] = m->diffuseReflectance.colorLinear[c];
@@ -99,8 +226,9 @@ bool RRSolverHIP::buildScene(const UpdateParameters& params)
 		{
 			if (!m->sideBits[s].legal) continue;
 			for (int c = 0; c < 3; c++)
-				exitance[8 * (size_t)t + 4 * s + c] = direct[c] * albedo[3 * (size_t)t + c]
-				                                    + (m->sideBits[s].emitTo ? emittance[3 * (size_t)t + c] : 0.f);
+				exitance[8 * (size_t)t + 4 * s + c] = pmi >= 0 ? direct[c]
+				                                    : direct[c] * albedo[3 * (size_t)t + c]
+				                                      + (m->sideBits[s].emitTo ? emittance[3 * (size_t)t + c] : 0.f);
 			if (diffuse) { indirectSides[t] |= (uint8_t)(1u << s); anyIndirect = true; }
 		}
 		if (anyIndirect)
@@ -156,6 +284,25 @@ bool RRSolverHIP::buildScene(const UpdateParameters& params)
 		RRReporter::report(ERRO, "RRSolverHIP: %s\n", lsg_last_error());
 		releaseScene();
 		return false;
+	}
+	// point materials: the textures in their stored form plus their texels as the solver's
+	// colour space makes them linear (getElement with and without it)
+	if (anyPoint)
+	{
+		for (size_t i = 0; i < pms.tex.size(); i++)
+		{
+			pms.tex[i].linear = pms.linear[i].data();
+			pms.tex[i].linear_stored = pms.linearStored[i].data();
+		}
+		const int e = lsg_scene_set_point_materials(scene, triMat.data(), triUv.data(), pms.mats.data(), (uint32_t)pms.mats.size(),
+		                                            pms.tex.data(), (uint32_t)pms.tex.size(), 0.f);
+		pms.unlockAll();
+		if (e != LSG_OK)
+		{
+			RRReporter::report(ERRO, "RRSolverHIP: %s\n", lsg_last_error());
+			releaseScene();
+			return false;
+		}
 	}
 	// the mesh's vertex frames (RRMesh::TangentBasis), which directional layers use
 	if (lsg_scene_set_tangents(scene, tangents.data(), bitangents.data()) != LSG_OK)

@@ -67,6 +67,11 @@ class BakeInfo(ctypes.Structure):
                 ("rgba", P), ("rgbf", P), ("texels", P), ("subtexels", P)]
 
 
+class Texture(ctypes.Structure):
+    _fields_ = [("width", U32), ("height", U32), ("format", I32), ("scaled", I32), ("data", P), ("linear", P),
+                ("linear_stored", P)]
+
+
 BAKE_USE_GRAPH = 1
 
 # name -> (restype, argtypes); restype None means int status checked by check()
@@ -109,6 +114,7 @@ SIGNATURES = {
     "lsg_scene_exitance_device": (ctypes.c_int, [P, P, P, P, P]),
     "lsg_scene_set_lights": (ctypes.c_int, [P, P, U32]),
     "lsg_scene_set_indirect": (ctypes.c_int, [P, P, P, P]),
+    "lsg_scene_set_point_materials": (ctypes.c_int, [P, P, P, P, U32, P, U32, ctypes.c_float]),
     "lsg_gather_device": (ctypes.c_int, [P, P, U32, P, U32, ctypes.POINTER(GatherParams), P, P]),
     "lsg_gather_stats": (ctypes.c_int, [P, P, U32, P, U32, ctypes.POINTER(GatherParams), P, ctypes.POINTER(ctypes.c_double)]),
     "lsg_scene_set_tangents": (ctypes.c_int, [P, P, P]),

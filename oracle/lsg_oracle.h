@@ -23,6 +23,13 @@
  *       uniform-exitance enclosure, emitter form factor); reference bake is
  *       unbuildable here (needs boost), so bit-level parity with
  *       processTexel itself is "parity unpinned" beyond those closed forms.
+ *   lsgo_texture_at and the point materials of lsgo_gather_pm -- pinned:
+ *       texel lookups against hand-placed texels (nearest texel, bilinear
+ *       weights at known offsets, wrap-around), byte formats against the
+ *       reference colour space's byte table and toLinear compiled from
+ *       RRColorSpace.cpp, and whole gathers against closed forms (a keyed
+ *       checkerboard occluder passes half the light, a textured wall
+ *       reflects its texel colour), each with planted mistakes.
  *   lsgo_vertex_corners / lsgo_smooth_vertices -- pinned: closed forms of
  *       the angle x area weights (equal fans, a right-angle corner, constant and
  *       linear fields) with planted weight mistakes; the grouping they take is
@@ -91,6 +98,39 @@ void lsgo_bvh_traverse(const float* nodes, uint32_t nnodes, int32_t root_ref,
 
 uint32_t lsgo_lbvh_build(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_t ntri,
                          uint32_t leafmax, float* nodes_out, uint32_t* prim_out, int32_t* root_ref, int32_t* depth_out);
+
+/* Point materials of the gather (RRObject::getPointMaterial, RRObject.cpp:196-303): a
+ * material per triangle (0xFFFFFFFF = none), the material uv of each triangle's corners,
+ * and textures as RRBufferInMemory stores them (the oracle converts colours itself, from
+ * gamma).  lsgo_pmat is field for field the library's lsg_point_material, so one array
+ * serves both. */
+typedef struct lsgo_pmat {
+    int32_t diffuse_texture, emissive_texture, transmittance_texture;
+    uint32_t flags;                 /* 1 transmittance in alpha, 2 keyed, 4 inverted */
+    float threshold;
+    uint32_t min_quality;
+    float albedo[3];                /* diffuseReflectance.colorLinear */
+    float emittance[3];             /* diffuseEmittance.colorLinear x materialEmittanceMultiplier */
+    float emittance_multiplier;
+    uint32_t emit_sides;            /* bit 0 front, bit 1 back */
+} lsgo_pmat;
+typedef struct lsgo_ptex {
+    uint32_t width, height;
+    int32_t format;                 /* BF_RGB 0, BGR 1, RGBA 2, RGBF 3, RGBAF 4, LUMINANCE 9, LUMINANCEF 10 */
+    int32_t scaled;
+    const void* data;
+} lsgo_ptex;
+typedef struct lsgo_pm {
+    const uint32_t* tri_mat;
+    const float* tri_uv;
+    const lsgo_pmat* mats;
+    const lsgo_ptex* tex;
+    float gamma;                    /* RRColorSpace::create_sRGB power, 0 = no colour space */
+} lsgo_pm;
+
+/* RRBufferInMemory::getElementAtPosition of a 2D texture (RRBufferInMemory.cpp:372-408),
+ * with the colour space of gamma (0 = none) applied as getElement applies it. */
+void lsgo_texture_at(const lsgo_ptex* t, float u, float v, float gamma, int interpolated, float* rgba);
 
 /* HomogenousFiller2 (RRSolver/gather.cpp:57-111). */
 void  lsgo_filler_triangle_point(uint32_t n, float* a, float* b);

@@ -20,6 +20,9 @@
  * albedo plus emittance, pathtracer.cpp:235-262, 306-316) and, where the scene gives
  * per-corner indirect irradiance, adds the albedo times its interpolation at the hit's
  * hitPoint2d (pathtracer.cpp:326-348, Triangle::getPointMeasure rrcore.cpp:262-273).
+ * With point materials (lsgo_gather_pm) a triangle's diffuse colour and emittance are read
+ * at the hit point where the quality exceeds the material's minimum, and keyed texels
+ * that are fully transparent are passed by hemisphere and shadow rays.
  *
  * Geometry (ray origins, directions, hit tests) is evaluated in binary32 in the
  * reference's operation order, so hit decisions agree bit for bit with the GPU;
@@ -122,12 +125,131 @@ void lsgo_subtexel_walk(const float* areas, uint32_t m, const uint32_t* series_n
     }
 }
 
+/* ---- point materials ---------------------------------------------------------------
+ * RRGammaScaler (RRColorSpace.cpp:24-80): toLinear x -> sign(x) |x|^(1/gamma) with 1/gamma
+ * kept in binary32 and pow taken in binary64 (the build calls the double pow for float
+ * arguments), rounded to binary32; getLinear of a byte reads the table
+ * pow(float(i) / 255, 1/gamma). */
+static float cs_to_linear(float a, float gamma)
+{
+    const float inv = 1 / gamma;
+    return a >= 0 ? (float)pow((double)a, (double)inv) : -(float)pow((double)-a, (double)inv);
+}
+
+static float byte_to_float(uint8_t b) { return (float)b * 0.003921568627450980392156862745098f; }  /* RR_BYTE2FLOAT */
+
+/* RRBufferInMemory::getElement (RRBufferInMemory.cpp:295-360) */
+static void get_element(const lsgo_ptex* t, size_t i, float gamma, float* r)
+{
+    const uint8_t* b = (const uint8_t*)t->data;
+    const float* f = (const float*)t->data;
+    if (t->scaled && gamma > 0 && (t->format == 0 || t->format == 2)) {
+        const size_t k = t->format == 0 ? 3 * i : 4 * i;
+        for (int c = 0; c < 3; c++) r[c] = (float)pow((double)((float)b[k + c] / 255), (double)(1 / gamma));
+        r[3] = t->format == 0 ? 1.f : byte_to_float(b[k + 3]);
+        return;
+    }
+    switch (t->format) {
+    case 0: r[0] = byte_to_float(b[3 * i]); r[1] = byte_to_float(b[3 * i + 1]); r[2] = byte_to_float(b[3 * i + 2]); r[3] = 1; break;
+    case 1: r[0] = byte_to_float(b[3 * i + 2]); r[1] = byte_to_float(b[3 * i + 1]); r[2] = byte_to_float(b[3 * i]); r[3] = 1; break;
+    case 2: for (int c = 0; c < 4; c++) r[c] = byte_to_float(b[4 * i + c]); break;
+    case 3: r[0] = f[3 * i]; r[1] = f[3 * i + 1]; r[2] = f[3 * i + 2]; r[3] = 1; break;
+    case 4: for (int c = 0; c < 4; c++) r[c] = f[4 * i + c]; break;
+    case 9: r[0] = r[1] = r[2] = byte_to_float(b[i]); r[3] = 1; break;
+    default: r[0] = r[1] = r[2] = f[i]; r[3] = 1; break;
+    }
+    if (gamma > 0 && t->scaled)
+        for (int c = 0; c < 3; c++) r[c] = cs_to_linear(r[c], gamma);
+}
+
+/* RRBufferInMemory::getElementAtPosition (RRBufferInMemory.cpp:372-408), depth 1 */
+void lsgo_texture_at(const lsgo_ptex* t, float u, float v, float gamma, int interpolated, float* rgba)
+{
+    const uint32_t w = t->width, h = t->height;
+    if (!interpolated) {
+        const float px = fmodf(u, 1) + 2, py = fmodf(v, 1) + 2;
+        const uint32_t x = (uint32_t)(px * (float)w) % w, y = (uint32_t)(py * (float)h) % h;
+        get_element(t, x + (size_t)y * w, gamma, rgba);
+        return;
+    }
+    const float px = (fmodf(u, 1) + 2) * (float)w - 0.5f, py = (fmodf(v, 1) + 2) * (float)h - 0.5f;
+    const float rx = fmodf(px, 1), ry = fmodf(py, 1);
+    const uint32_t cx = (uint32_t)px, cy = (uint32_t)py;
+    float e00[4], e01[4], e10[4], e11[4];
+    get_element(t, (cx % w) + (size_t)(cy % h) * w, gamma, e00);
+    get_element(t, (cx % w) + (size_t)((cy + 1) % h) * w, gamma, e01);
+    get_element(t, ((cx + 1) % w) + (size_t)(cy % h) * w, gamma, e10);
+    get_element(t, ((cx + 1) % w) + (size_t)((cy + 1) % h) * w, gamma, e11);
+    const float w00 = (1 - rx) * (1 - ry), w01 = (1 - rx) * ry, w10 = rx * (1 - ry), w11 = rx * ry;
+    for (int c = 0; c < 4; c++) {
+        float a = e00[c] * w00;
+        a = a + e01[c] * w01;
+        a = a + e10[c] * w10;
+        rgba[c] = a + e11[c] * w11;
+    }
+}
+
+typedef struct { float dif[3], emi[3]; int transparent; } pmat_t;
+
+/* RRObject::getPointMaterial (RRObject.cpp:285-303) with updatePointMaterial (:196-281), for
+ * the properties this gather reads: diffuse colour, emittance (x the emittance multiplier,
+ * which the pathtracer applies, pathtracer.cpp:313), and whether keying made the texel
+ * fully transparent (renderFrom and catchFrom cleared) */
+static void point_material(const lsgo_pm* pm, uint32_t tri, float u, float v, int interpolated, pmat_t* m)
+{
+    const lsgo_pmat* M = pm->mats + pm->tri_mat[tri];
+    const float* c = pm->tri_uv + 6 * (size_t)tri;
+    const float w = 1 - u - v;
+    const float mu = c[0] * w + c[2] * u + c[4] * v, mv = c[1] * w + c[3] * u + c[5] * v;
+    const float g = pm->gamma;
+    float rgba[4];
+    for (int k = 0; k < 3; k++) { m->dif[k] = M->albedo[k]; m->emi[k] = M->emittance[k]; }
+    m->transparent = 0;
+    if (M->emissive_texture >= 0) {
+        lsgo_texture_at(pm->tex + M->emissive_texture, mu, mv, g, interpolated, rgba);
+        for (int k = 0; k < 3; k++) m->emi[k] = rgba[k] * M->emittance_multiplier;
+    }
+    if (M->diffuse_texture >= 0 && M->transmittance_texture == M->diffuse_texture && (M->flags & 1)) {
+        /* optimized path: transmittance in the diffuse map's alpha */
+        lsgo_texture_at(pm->tex + M->diffuse_texture, mu, mv, 0, interpolated, rgba);
+        float t = 1 - rgba[3];
+        if (M->flags & 4) t = 1 - t;
+        if (M->flags & 2) t = t >= M->threshold ? 1.f : 0.f;
+        m->transparent = t == 1;
+        float tl = t;
+        for (int k = 0; k < 3; k++) m->dif[k] = rgba[k];
+        if (g > 0) {
+            for (int k = 0; k < 3; k++) m->dif[k] = cs_to_linear(m->dif[k], g);
+            tl = cs_to_linear(t, g);
+        }
+        for (int k = 0; k < 3; k++) m->dif[k] *= 1 - tl;
+        return;
+    }
+    if (M->diffuse_texture >= 0) {
+        lsgo_texture_at(pm->tex + M->diffuse_texture, mu, mv, g, interpolated, rgba);
+        for (int k = 0; k < 3; k++) m->dif[k] = rgba[k];
+    }
+    if (M->transmittance_texture >= 0) {
+        lsgo_texture_at(pm->tex + M->transmittance_texture, mu, mv, 0, interpolated, rgba);
+        float t[3];
+        for (int k = 0; k < 3; k++) t[k] = (M->flags & 1) ? 1 - rgba[3] : rgba[k];
+        if (M->flags & 4) for (int k = 0; k < 3; k++) t[k] = 1 - t[k];
+        if (M->flags & 2) {
+            const float avg = (t[0] + t[1] + t[2]) * 0.33333333333333f;
+            for (int k = 0; k < 3; k++) t[k] = avg >= M->threshold ? 1.f : 0.f;
+        }
+        m->transparent = t[0] == 1 && t[1] == 1 && t[2] == 1;
+        for (int k = 0; k < 3; k++) m->dif[k] *= 1 - (g > 0 ? cs_to_linear(t[k], g) : t[k]);
+    }
+}
+
 /* ---- ray casting with the final-gathering handler rules ------------------------ */
 typedef struct {
     const float* verts; const uint32_t* idx; uint32_t ntri;
     const lsgo_tri* tris; const uint8_t* side_mask;
     const float* nodes; int32_t root_ref; const uint32_t* prim; uint32_t nprim;
     float bmn[3], bmx[3];
+    const lsgo_pm* pm; uint32_t quality;     /* point materials (NULL = none), max(1, quality) */
 } scene_t;
 
 /* pathtracer.h:139-166: tri has all three vertices equal to the shooter's */
@@ -145,14 +267,29 @@ static int same_location(const scene_t* sc, uint32_t tri, uint32_t shooter)
     return 1;
 }
 
-/* mode 0 = hemisphere (accept rendered side), 1 = shadow (any shadow caster) */
-static int accept_hit(const scene_t* sc, int mode, uint32_t tri, int front, float t, uint32_t shooter, float tmin)
+/* mode 0 = hemisphere (accept rendered side), 1 = shadow (any shadow caster); with point
+ * materials a texel that keying made fully transparent is passed (pathtracer.h:194-216:
+ * hemisphere rays read point materials at quality, shadow rays at 2 x quality,
+ * interpolated when quality / 10 exceeds the material's minimum, gather.cpp:175-180, 366-370) */
+static int accept_hit(const scene_t* sc, int mode, uint32_t tri, int front, float t, uint32_t shooter, float tmin,
+                      float u, float v)
 {
     if (tri == shooter) return 0;                                   /* pathtracer.h:133 */
     if (t < 1000 * tmin && same_location(sc, tri, shooter)) return 0; /* pathtracer.h:139-166 */
     uint8_t m = sc->side_mask[tri];
-    if (mode == 0) return (m & (front ? 1 : 2)) != 0;                /* TEST_BIT, renderFrom of hit side */
-    return (m & 4) != 0;                                             /* shadow caster, both sides [#45] */
+    int ok = mode == 0 ? (m & (front ? 1 : 2)) != 0                 /* TEST_BIT, renderFrom of hit side */
+                       : (m & 4) != 0;                              /* shadow caster, both sides [#45] */
+    if (ok && sc->pm && sc->pm->tri_mat[tri] != 0xFFFFFFFFu) {
+        const lsgo_pmat* M = sc->pm->mats + sc->pm->tri_mat[tri];
+        const uint32_t qpoint = mode == 0 ? sc->quality : 2 * sc->quality;
+        const uint32_t qinterp = mode == 0 ? 0 : sc->quality / 10;
+        if (qpoint > M->min_quality) {
+            pmat_t P;
+            point_material(sc->pm, tri, u, v, qinterp > M->min_quality, &P);
+            if (P.transparent) ok = 0;
+        }
+    }
+    return ok;
 }
 
 static inline void slab6(const float* b, const float* o, const float* idir, float tlo, float thi, float* tn, float* tf)
@@ -169,7 +306,7 @@ static void consider(const scene_t* sc, int mode, uint32_t t, const float* o, co
 {
     float th, u, v; int f;
     if (!lsgo_srlnp_hit(&sc->tris[t], o, d, lo, *best, &th, &u, &v, &f)) return;
-    if (!accept_hit(sc, mode, t, f, th, shooter, tmin)) return;
+    if (!accept_hit(sc, mode, t, f, th, shooter, tmin, u, v)) return;
     if (*bi < 0 || th < *best || (th == *best && (int)t < *bi)) { *best = th; *bi = (int)t; *bf = f; buv[0] = u; buv[1] = v; }
 }
 
@@ -365,7 +502,8 @@ static void gather_core(const float* verts, uint32_t nverts, const uint32_t* idx
                         const float* exitance, const float* ind, const lsgo_light* lights, uint32_t nlights, const lsgo_gparams* p,
                         const uint32_t* texels, uint32_t ntex, const float* subs,
                         const float* nodes, int32_t root_ref, const uint32_t* prim, uint32_t nprim,
-                        const lsgo_env* env_in, double* out_rgba, double* const* layers, lsgo_gstats* stats, int threads)
+                        const lsgo_env* env_in, const lsgo_pm* pm, double* out_rgba, double* const* layers, lsgo_gstats* stats,
+                        int threads)
 {
     const int dirs = layers && (layers[1] || layers[2] || layers[3]);
     /* the environment is sampled only with a nonzero multiplier (gather.cpp:149) */
@@ -373,6 +511,7 @@ static void gather_core(const float* verts, uint32_t nverts, const uint32_t* idx
     scene_t sc;
     sc.verts = verts; sc.idx = idx; sc.ntri = ntri; sc.side_mask = side_mask;
     sc.nodes = nodes; sc.root_ref = root_ref; sc.prim = prim; sc.nprim = nprim;
+    sc.pm = pm; sc.quality = p->quality > 1 ? p->quality : 1;      /* gather.cpp:865 */
     lsgo_tri* tris = (lsgo_tri*)malloc(sizeof(lsgo_tri) * (ntri ? ntri : 1));
     for (uint32_t t = 0; t < ntri; t++)
         lsgo_srlnp_setup(verts + 3 * (size_t)idx[3 * t], verts + 3 * (size_t)idx[3 * t + 1], verts + 3 * (size_t)idx[3 * t + 2], &tris[t]);
@@ -455,7 +594,29 @@ static void gather_core(const float* verts, uint32_t nverts, const uint32_t* idx
                      * (pathtracer.cpp:156-173) */
                     float envc[3], hitc[3];
                     const float* e = NULL;
-                    if (hit >= 0) {
+                    if (hit >= 0 && pm && pm->tri_mat[hit] != 0xFFFFFFFFu) {
+                        /* a triangle with a material: the table holds the side's direct
+                         * irradiance; getIncidentRadiance composes direct x diffuse colour,
+                         * + emittance where the side emits, + indirect x diffuse colour
+                         * (pathtracer.cpp:235-262, 306-316, 326-348) */
+                        const lsgo_pmat* M = pm->mats + pm->tri_mat[hit];
+                        const float* tab = exitance + 4 * (2 * (size_t)hit + (fr ? 0 : 1));
+                        float dif[3], emi[3];
+                        for (int k = 0; k < 3; k++) { dif[k] = M->albedo[k]; emi[k] = M->emittance[k]; }
+                        if (sc.quality > M->min_quality) {
+                            pmat_t P;
+                            point_material(pm, (uint32_t)hit, huv[0], huv[1], 0, &P);
+                            for (int k = 0; k < 3; k++) { dif[k] = P.dif[k]; emi[k] = P.emi[k]; }
+                        }
+                        for (int k = 0; k < 3; k++) hitc[k] = tab[k] * dif[k];
+                        if (M->emit_sides & (fr ? 1u : 2u)) for (int k = 0; k < 3; k++) hitc[k] = hitc[k] + emi[k];
+                        if (ind && (side_mask[hit] & (fr ? 8 : 16))) {
+                            const float* c = ind + 12 * (size_t)hit;
+                            const float u = huv[0], v = huv[1], w = 1 - u - v;
+                            for (int k = 0; k < 3; k++) hitc[k] = hitc[k] + (c[k] * w + c[4 + k] * u + c[8 + k] * v) * dif[k];
+                        }
+                        e = hitc;
+                    } else if (hit >= 0) {
                         e = exitance + 4 * (2 * (size_t)hit + (fr ? 0 : 1));
                         if (ind && (side_mask[hit] & (fr ? 8 : 16))) {
                             /* + albedo * (I0 (1-u-v) + I1 u + I2 v), pathtracer.cpp:346, rrcore.cpp:266-268 */
@@ -568,7 +729,7 @@ void lsgo_gather(const float* verts, uint32_t nverts, const uint32_t* idx, uint3
                  double* out_rgba, lsgo_gstats* stats, int threads)
 {
     gather_core(verts, nverts, idx, ntri, vnormals, NULL, NULL, side_mask, exitance, NULL, lights, nlights, p, texels, ntex,
-                subs, nodes, root_ref, prim, nprim, NULL, out_rgba, NULL, stats, threads);
+                subs, nodes, root_ref, prim, nprim, NULL, NULL, out_rgba, NULL, stats, threads);
 }
 
 /* lsgo_gather with an environment (NULL = none) and per-corner indirect irradiance (NULL =
@@ -582,7 +743,7 @@ void lsgo_gather_env(const float* verts, uint32_t nverts, const uint32_t* idx, u
                      const lsgo_env* env, double* out_rgba, lsgo_gstats* stats, int threads)
 {
     gather_core(verts, nverts, idx, ntri, vnormals, NULL, NULL, side_mask, exitance, ind, lights, nlights, p, texels, ntex,
-                subs, nodes, root_ref, prim, nprim, env, out_rgba, NULL, stats, threads);
+                subs, nodes, root_ref, prim, nprim, env, NULL, out_rgba, NULL, stats, threads);
 }
 
 /* lsgo_gather with the output layers of updateLightmap.  vtan/vbit: per-vertex frames
@@ -604,6 +765,29 @@ void lsgo_gather_layers(const float* verts, uint32_t nverts, const uint32_t* idx
         vtan = ft; vbit = fb;
     }
     gather_core(verts, nverts, idx, ntri, vnormals, vtan, vbit, side_mask, exitance, ind, lights, nlights, p, texels, ntex,
-                subs, nodes, root_ref, prim, nprim, env, NULL, layers, stats, threads);
+                subs, nodes, root_ref, prim, nprim, env, NULL, NULL, layers, stats, threads);
+    free(ft); free(fb);
+}
+
+/* lsgo_gather_layers with point materials (pm NULL = none); out_rgba: the lightmap alone
+ * (layers NULL), else layers as lsgo_gather_layers. */
+void lsgo_gather_pm(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_t ntri,
+                    const float* vnormals, const float* vtan, const float* vbit, const uint8_t* side_mask,
+                    const float* exitance, const float* ind, const lsgo_light* lights, uint32_t nlights,
+                    const lsgo_gparams* p, const uint32_t* texels, uint32_t ntex, const float* subs,
+                    const float* nodes, int32_t root_ref, const uint32_t* prim, uint32_t nprim,
+                    const lsgo_env* env, const lsgo_pm* pm, double* out_rgba, double* const* layers, lsgo_gstats* stats,
+                    int threads)
+{
+    float* ft = NULL;
+    float* fb = NULL;
+    if (layers && (!vtan || !vbit)) {
+        ft = (float*)malloc(12 * (size_t)(nverts ? nverts : 1));
+        fb = (float*)malloc(12 * (size_t)(nverts ? nverts : 1));
+        lsgo_default_frames(vnormals, nverts, ft, fb);
+        vtan = ft; vbit = fb;
+    }
+    gather_core(verts, nverts, idx, ntri, vnormals, vtan, vbit, side_mask, exitance, ind, lights, nlights, p, texels, ntex,
+                subs, nodes, root_ref, prim, nprim, env, pm, layers ? NULL : out_rgba, layers, stats, threads);
     free(ft); free(fb);
 }

@@ -268,6 +268,9 @@ def gather(scene_arrays, params, texels, subs, tree=None, threads=0):
         nprim = len(prim)
     else:
         np_, pp_, root, nprim = None, None, 0, 0
+    if scene_arrays.get("pm") is not None:
+        _gather_pm(lib(), scene_arrays, v, i, vn, sb, ex, lights, gp, tx, sbs, tree, out, None, gs, threads)
+        return out, gs
     ev, keep = _env(scene_arrays)
     ind = _ind(scene_arrays)
     L = lib()
@@ -278,6 +281,71 @@ def gather(scene_arrays, params, texels, subs, tree=None, threads=0):
                       _p(lights), len(lights), ctypes.byref(gp), _p(tx), len(tx), _p(sbs), np_, root, pp_, nprim,
                       ctypes.byref(ev) if ev is not None else None, _p(out), ctypes.byref(gs), threads)
     return out, gs
+
+
+class _PTex(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("format", ctypes.c_int32), ("scaled", ctypes.c_int32),
+                ("data", ctypes.c_void_p)]
+
+
+class _PM(ctypes.Structure):
+    _fields_ = [("tri_mat", ctypes.c_void_p), ("tri_uv", ctypes.c_void_p), ("mats", ctypes.c_void_p), ("tex", ctypes.c_void_p),
+                ("gamma", ctypes.c_float)]
+
+
+def _pm(scene_arrays):
+    """Point materials of the scene as lsgo_pm (None = none), with the arrays it points to."""
+    pm = scene_arrays.get("pm")
+    if pm is None:
+        return None, None
+    tm = np.ascontiguousarray(pm["tri_mat"], np.uint32)
+    uv = np.ascontiguousarray(pm["tri_uv"], np.float32)
+    mats = np.ascontiguousarray(pm["mats"])
+    texs = pm["textures"]
+    arr = (_PTex * max(len(texs), 1))()
+    datas = []
+    for k, t in enumerate(texs):
+        d = np.ascontiguousarray(t["data"])
+        datas.append(d)
+        arr[k] = _PTex(t["width"], t["height"], t["format"], int(t["scaled"]), d.ctypes.data)
+    c = _PM(tm.ctypes.data, uv.ctypes.data, mats.ctypes.data, ctypes.cast(arr, ctypes.c_void_p), pm["gamma"])
+    return c, (tm, uv, mats, arr, datas)
+
+
+def texture_at(tex, u, v, gamma=0.0, interpolated=False):
+    """lsgo_texture_at: RRBufferInMemory::getElementAtPosition of a texture() dict."""
+    L = lib()
+    L.lsgo_texture_at.argtypes = [ctypes.c_void_p, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_void_p]
+    L.lsgo_texture_at.restype = None
+    d = np.ascontiguousarray(tex["data"])
+    t = _PTex(tex["width"], tex["height"], tex["format"], int(tex["scaled"]), d.ctypes.data)
+    out = np.zeros(4, np.float32)
+    L.lsgo_texture_at(ctypes.byref(t), u, v, gamma, int(interpolated), out.ctypes.data)
+    return out
+
+
+def _gather_pm(L, scene_arrays, v, i, vn, sb, ex, lights, gp, tx, sbs, tree, out, layer_ptrs, gs, threads):
+    P, U = ctypes.c_void_p, ctypes.c_uint32
+    L.lsgo_gather_pm.argtypes = [P, U, P, U, P, P, P, P, P, P, P, U, P, P, U, P, P, ctypes.c_int32, P, U, P, P, P, P, P,
+                                 ctypes.c_int]
+    L.lsgo_gather_pm.restype = None
+    if tree is not None:
+        nodes = np.ascontiguousarray(tree["nodes"], dtype=np.float32)
+        prim = np.ascontiguousarray(tree["prim"], dtype=np.uint32)
+        np_, pp_, root, nprim = _p(nodes), _p(prim), tree["root"], len(prim)
+    else:
+        np_, pp_, root, nprim = None, None, 0, 0
+    vt, vb = scene_arrays.get("tangents"), scene_arrays.get("bitangents")
+    vt = None if vt is None else np.ascontiguousarray(vt, dtype=np.float32)
+    vb = None if vb is None else np.ascontiguousarray(vb, dtype=np.float32)
+    ev, keep = _env(scene_arrays)
+    ind = _ind(scene_arrays)
+    pm, keep_pm = _pm(scene_arrays)
+    L.lsgo_gather_pm(_p(v), len(v), _p(i), len(i), _p(vn), _p(vt) if vt is not None else None,
+                     _p(vb) if vb is not None else None, _p(sb), _p(ex), _p(ind) if ind is not None else None,
+                     _p(lights), len(lights), ctypes.byref(gp), _p(tx), len(tx), _p(sbs), np_, root, pp_, nprim,
+                     ctypes.byref(ev) if ev is not None else None, ctypes.byref(pm) if pm is not None else None,
+                     _p(out) if out is not None else None, layer_ptrs, ctypes.byref(gs), threads)
 
 
 def _ind(scene_arrays):
@@ -401,6 +469,9 @@ def gather_layers(scene_arrays, params, texels, subs, layers=(0, 1, 2, 3, 4), tr
                  params["rug_distance"], params["light_multiplier"], params["seed"], params.get("light_rounds", 0),
                  params.get("environment_multiplier", 0.0))
     gs = GStats()
+    if scene_arrays.get("pm") is not None:
+        _gather_pm(L, scene_arrays, v, i, vn, sb, ex, lights, gp, tx, sbs, tree, None, ptrs, gs, threads)
+        return outs
     if tree is not None:
         nodes = np.ascontiguousarray(tree["nodes"], dtype=np.float32)
         prim = np.ascontiguousarray(tree["prim"], dtype=np.uint32)
