@@ -563,9 +563,17 @@ __device__ inline float3 layer_dir(int i, const float3& t, const float3& b, cons
 // FEAT: kFeatEnv = hemisphere rays that leave the scene sample the environment;
 // kFeatMats = point materials (textures and keyed transparency read at hits).  Each is a
 // separate instantiation, so that scenes without them keep the smaller register footprint.
+// direction-binned hemisphere order (gather_texel): series of at most kPermMaxRays rays,
+// whose candidates fit kPermWords words of 64 (1024 rays need about 1700 candidates, and any
+// 2048 consecutive filler indices hold more than 1100 points of the disk); per wave
+// kPermScr words of LDS: the words' acceptance masks, the columns' bits, the key starts and
+// the words' prefix counts
+constexpr uint32_t kPermWords = 32, kPermMaxRays = 1024;
+constexpr uint32_t kPermScr = 2 * kPermWords + 64 + (64 + kPermWords) / 2;
+
 template <int STACK, bool STATS, int G = 64, bool LAYERS = false, int FEAT = 0>
 __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, int32_t* mystk, uint32_t* lq, uint16_t* lqj,
-                             int stride, float* subc = nullptr)
+                             int stride, float* subc = nullptr, uint32_t* pscr = nullptr)
 {
     // one texel per wave: the frames of its first subtexels are kept in the wave's LDS
     // (when the caller gives one)
@@ -704,7 +712,155 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
             }
             __builtin_amdgcn_wave_barrier();
         };
-        for (uint32_t base = 0; base < n; base += G) {
+        // one hemisphere ray: shooter s of the series (origin in subtexel myj, jitter index
+        // shooter_base + s) in the direction of filler point (x, y)
+        auto shoot_h = [&](uint32_t s, uint32_t myj, float x, float y) {
+            float3 o; float nx, ny, nz; uint32_t tri;
+            float3 tg, bt;
+            float tx, ty, tz, bx, by, bz;
+            if (kCache) shooter_cached(A, ts, myj, key, shooter_base + s, subc, o, nx, ny, nz, tri);
+            else shooter_at<LAYERS>(A, ts, myj, key, shooter_base + s, o, nx, ny, nz, tri, &tg, &bt);
+            const uint32_t sj = kCache ? myj : kSubCache;
+            if (kCache && sj < kSubCache) {
+                const float* f = subc + kFrame * sj;
+                tx = f[3]; ty = f[4]; tz = f[5]; bx = f[6]; by = f[7]; bz = f[8];
+            } else {
+                float ox, oy, oz;                                           // RRMathPrivate.h:48-53
+                if (fabsf(nx) > fabsf(nz)) { ox = -ny; oy = nx; oz = 0.f; } else { ox = 0.f; oy = -nz; oz = ny; }
+                const float olen = sqrtf(ox * ox + oy * oy + oz * oz);
+                tx = ox / olen; ty = oy / olen; tz = oz / olen;
+                bx = ny * tz - nz * ty; by = -nx * tz + nz * tx; bz = nx * ty - ny * tx;
+            }
+            auto flt = gather_filter<FEAT>(A, tri, near_t, 0, rug_below);
+            const float dist = x * x + y * y;
+            const float cosa = sqrtf(1 - dist);                          // gather.cpp:125-126
+            const float3 d = make_float3(nx * cosa + tx * x + bx * y, ny * cosa + ty * x + by * y, nz * cosa + tz * x + bz * y);
+            TraceResult r;
+            uint32_t nv_ = 0, nt_ = 0;
+            trace_closest<STACK, STATS>(flt, A.sc.col, o, d, A.tmin, A.locality, mystk, stride, r, nv_, nt_);
+            if (STATS) { stH[0]++; stH[1] += nv_; stH[2] += nt_; }
+            // a hit reads the exitance of the side it sees; a ray that leaves the scene
+            // returns environment * multiplier, or nothing (pathtracer.cpp:156-173)
+            auto add = [&](const float4& e) {
+                if constexpr (LAYERS) {
+                    if (dirs) {
+                        // gather.cpp:284-305: dirs near the normal are picked more often,
+                        // so each layer weighs by its own cosine over the normal's
+                        const float ni1 = d.x * nx + d.y * ny + d.z * nz;
+                        if (ni1 > 0) {
+                            const float inv = 1 / ni1;
+#pragma unroll
+                            for (int i = 0; i < 4; i++) {
+                                const float3 l = layer_dir(i, tg, bt, make_float3(nx, ny, nz));
+                                const float n2 = d.x * l.x + d.y * l.y + d.z * l.z;
+                                if (n2 > 0) {
+                                    const float k = n2 * inv;
+                                    hl[3 * i] += e.x * k; hl[3 * i + 1] += e.y * k; hl[3 * i + 2] += e.z * k;
+                                }
+                            }
+                        }
+                    } else { hl[0] += e.x; hl[1] += e.y; hl[2] += e.z; }
+                    const float w = (fabsf(e.x) + fabsf(e.y) + fabsf(e.z)) * 0.33333333333333f;   // gather.cpp:306
+                    bn[0] += d.x * w; bn[1] += d.y * w; bn[2] += d.z * w;
+                } else {
+                    hr += e.x; hg += e.y; hb += e.z;
+                }
+            };
+            if (r.tri >= 0) add(hit_exitance<(FEAT & kFeatMats) != 0>(A.sc, r.tri, r.front, o, d, r.t, A.quality));
+            else if constexpr ((FEAT & kFeatEnv) != 0) {
+                const float4 v = env_at(A.sc, d);
+                add(make_float4(v.x * A.env_mult, v.y * A.env_mult, v.z * A.env_mult, 0.f));
+            }
+            cRelH++;
+        };
+        // direction-binned order: the series' hemisphere rays traced 64 at a time in the
+        // order of their filler cell (coarse base-4 digits of the candidate index first), so a
+        // wave's lanes trace neighbouring directions; every shooter keeps the filler point the
+        // sequential hand-out gives it, only the order of the sums changes
+        const bool permH = pscr != nullptr && G == 64 && shootH && allH && shooter_base == 0 && n == hemi_rays && n > 64u
+                           && n <= kPermMaxRays && (m == 1 || (A.walk && walk_has(A.walk + ti)));
+        bool permDone = false;
+        if (permH) {
+            uint64_t* const pmask = (uint64_t*)pscr;                       // word w: accepted among candidates 64w..64w+63
+            uint32_t* const pcol = pscr + 2 * kPermWords;                   // column c: bit w = candidate 64w+c accepted
+            uint16_t* const pstart = (uint16_t*)(pscr + 2 * kPermWords + 64);   // 64 key starts, then kPermWords word prefixes
+            uint16_t* const pprefix = pstart + 64;
+            const uint32_t f0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)filler);   // wave-uniform
+            uint32_t acc = 0, W = 0, colbits = 0, lastw_full = 0, lastbit = 0;
+            __builtin_amdgcn_wave_barrier();
+            while (acc < n && W < kPermWords) {
+                float cx, cy;
+                filler_point(f0 + 64u * W + lane, cx, cy);
+                const bool ok = !(cx * cx + cy * cy >= 1.f);
+                uint64_t am = __ballot(ok);
+                const uint32_t na = (uint32_t)__popcll(am);
+                lastw_full = acc + na == n;
+                if (acc + na > n) {
+                    lastbit = nth_set_bit(am, n - acc - 1u);
+                    am &= lastbit == 63u ? ~0ull : ((1ull << (lastbit + 1u)) - 1ull);
+                }
+                if (lane == 0) { pmask[W] = am; pprefix[W] = (uint16_t)acc; }
+                colbits |= (uint32_t)((am >> lane) & 1ull) << W;
+                acc += (uint32_t)__popcll(am);
+                W++;
+            }
+            if (acc == n) {
+                permDone = true;
+                filler = lastw_full ? f0 + 64u * W : f0 + 64u * (W - 1u) + lastbit + 1u;
+                // key of this lane's column: the digits of its candidates' index mod 64, coarsest first
+                const uint32_t c = (f0 + lane) & 63u;
+                const uint32_t key = ((c & 3u) << 4) | (((c >> 2) & 3u) << 2) | ((c >> 4) & 3u);
+                pcol[lane] = colbits;
+                pstart[key] = (uint16_t)__popc(colbits);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                const uint32_t kc = pstart[lane];                           // count of key = lane
+                uint32_t incl = kc;
+                for (uint32_t o = 1; o < 64; o <<= 1) {
+                    const uint32_t v = (uint32_t)__shfl_up((int)incl, o);
+                    if (lane >= o) incl += v;
+                }
+                __builtin_amdgcn_wave_barrier();
+                pstart[lane] = (uint16_t)(incl - kc);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                for (uint32_t base = 0; base < n; base += 64u) {
+                    const uint32_t P = base + lane;
+                    if (P < n) {
+                        uint32_t k = 0;
+                        for (uint32_t step = 32; step; step >>= 1)
+                            if (pstart[k + step] <= P) k += step;
+                        const uint32_t t = P - pstart[k];
+                        const uint32_t cm = (k >> 4) | (((k >> 2) & 3u) << 2) | ((k & 3u) << 4);
+                        const uint32_t col = (cm - f0) & 63u;
+                        const uint32_t w = nth_set_bit((uint64_t)pcol[col], t);
+                        const uint32_t i = 64u * w + col;
+                        const uint32_t sh = pprefix[w] + (uint32_t)__popcll(pmask[w] & lanemask_lt(col));
+                        float x, y;
+                        filler_point(f0 + i, x, y);
+                        const uint32_t myj = m > 1 ? walk_pick(A.walk + ti, m, sh) : 0u;
+                        shoot_h(sh, myj, x, y);
+                    }
+                }
+                // the series' light shooters, in order
+                for (uint32_t base = 0; base < n; base += 64u) {
+                    const uint32_t sq = base + lane;
+                    const bool doL = sq < n && shootL && (allL || shots_before(sq + 1, n, rounds) > shots_before(sq, n, rounds));
+                    const uint64_t lm = __ballot(doL);
+                    if (doL) {
+                        const uint32_t e = qtail + (uint32_t)__popcll(lm & lanemask_lt(lane));
+                        const uint32_t qi = (e & 63u) + ((e / 64u) & 1u) * stride;
+                        lq0[qi] = sq;
+                        lqj0[qi] = (uint16_t)(m > 1 ? walk_pick(A.walk + ti, m, sq) : 0u);
+                    }
+                    qtail += (uint32_t)__popcll(lm);
+                    while (qtail - qhead >= 64u) { light_batch(64u); qhead += 64u; }
+                }
+            }
+        }
+        if (!permDone) for (uint32_t base = 0; base < n; base += G) {
             const uint32_t s = base + lane;
             const bool active = s < n;
             bool doH = false, doL = false;
@@ -753,66 +909,7 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
                 }
                 qtail += (uint32_t)__popcll(lm);
             }
-            if (doH) {
-                float3 o; float nx, ny, nz; uint32_t tri;
-                float3 tg, bt;
-                float tx, ty, tz, bx, by, bz;
-                if (kCache) shooter_cached(A, ts, myj, key, shooter_base + s, subc, o, nx, ny, nz, tri);
-                else shooter_at<LAYERS>(A, ts, myj, key, shooter_base + s, o, nx, ny, nz, tri, &tg, &bt);
-                const uint32_t sj = kCache ? myj : kSubCache;
-                if (kCache && sj < kSubCache) {
-                    const float* f = subc + kFrame * sj;
-                    tx = f[3]; ty = f[4]; tz = f[5]; bx = f[6]; by = f[7]; bz = f[8];
-                } else {
-                    float ox, oy, oz;                                           // RRMathPrivate.h:48-53
-                    if (fabsf(nx) > fabsf(nz)) { ox = -ny; oy = nx; oz = 0.f; } else { ox = 0.f; oy = -nz; oz = ny; }
-                    const float olen = sqrtf(ox * ox + oy * oy + oz * oz);
-                    tx = ox / olen; ty = oy / olen; tz = oz / olen;
-                    bx = ny * tz - nz * ty; by = -nx * tz + nz * tx; bz = nx * ty - ny * tx;
-                }
-                auto flt = gather_filter<FEAT>(A, tri, near_t, 0, rug_below);
-                const float x = my_x, y = my_y;
-                const float dist = x * x + y * y;
-                const float cosa = sqrtf(1 - dist);                          // gather.cpp:125-126
-                const float3 d = make_float3(nx * cosa + tx * x + bx * y, ny * cosa + ty * x + by * y, nz * cosa + tz * x + bz * y);
-                TraceResult r;
-                uint32_t nv_ = 0, nt_ = 0;
-                trace_closest<STACK, STATS>(flt, A.sc.col, o, d, A.tmin, A.locality, mystk, stride, r, nv_, nt_);
-                if (STATS) { stH[0]++; stH[1] += nv_; stH[2] += nt_; }
-                // a hit reads the exitance of the side it sees; a ray that leaves the scene
-                // returns environment * multiplier, or nothing (pathtracer.cpp:156-173)
-                auto add = [&](const float4& e) {
-                    if constexpr (LAYERS) {
-                        if (dirs) {
-                            // gather.cpp:284-305: dirs near the normal are picked more often,
-                            // so each layer weighs by its own cosine over the normal's
-                            const float ni1 = d.x * nx + d.y * ny + d.z * nz;
-                            if (ni1 > 0) {
-                                const float inv = 1 / ni1;
-#pragma unroll
-                                for (int i = 0; i < 4; i++) {
-                                    const float3 l = layer_dir(i, tg, bt, make_float3(nx, ny, nz));
-                                    const float n2 = d.x * l.x + d.y * l.y + d.z * l.z;
-                                    if (n2 > 0) {
-                                        const float k = n2 * inv;
-                                        hl[3 * i] += e.x * k; hl[3 * i + 1] += e.y * k; hl[3 * i + 2] += e.z * k;
-                                    }
-                                }
-                            }
-                        } else { hl[0] += e.x; hl[1] += e.y; hl[2] += e.z; }
-                        const float w = (fabsf(e.x) + fabsf(e.y) + fabsf(e.z)) * 0.33333333333333f;   // gather.cpp:306
-                        bn[0] += d.x * w; bn[1] += d.y * w; bn[2] += d.z * w;
-                    } else {
-                        hr += e.x; hg += e.y; hb += e.z;
-                    }
-                };
-                if (r.tri >= 0) add(hit_exitance<(FEAT & kFeatMats) != 0>(A.sc, r.tri, r.front, o, d, r.t, A.quality));
-                else if constexpr ((FEAT & kFeatEnv) != 0) {
-                    const float4 v = env_at(A.sc, d);
-                    add(make_float4(v.x * A.env_mult, v.y * A.env_mult, v.z * A.env_mult, 0.f));
-                }
-                cRelH++;
-            }
+            if (doH) shoot_h(s, myj, my_x, my_y);
             while (qtail - qhead >= (uint32_t)G) { light_batch((uint32_t)G); qhead += (uint32_t)G; }
         }
         while (qtail != qhead) {
@@ -1274,12 +1371,13 @@ __device__ inline void gather_block(const GatherArgs& A)
     __shared__ uint32_t lq[2 * 256];
     __shared__ uint16_t lqj[2 * 256];
     __shared__ float subc[4 * kFrame * kSubCache];
+    __shared__ uint32_t pscr[4 * kPermScr];
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     // wave-uniform: kept in a scalar register, the texel record is read by scalar loads
     const uint32_t ti = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4u + (tid >> 6)));
     if (ti >= A.ntex) return;                 // whole wavefront leaves together
     gather_texel<STACK, STATS, 64, LAYERS, FEAT>(A, ti, lane, stk + tid, lq + tid, lqj + tid, 256,
-                                                subc + (tid >> 6) * kFrame * kSubCache);
+                                                subc + (tid >> 6) * kFrame * kSubCache, pscr + (tid >> 6) * kPermScr);
 }
 
 template <int STACK, bool STATS, int FEAT = 0>
@@ -1333,6 +1431,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4)))
     __shared__ uint32_t lq[2 * 64 * WAVES];
     __shared__ uint16_t lqj[2 * 64 * WAVES];
     __shared__ float subc[WAVES * kFrame * kSubCache];
+    __shared__ uint32_t pscr[REFILL ? 1 : WAVES * kPermScr];
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const uint32_t home = Q.nqueues > 1 ? blockIdx.x % Q.nqueues : 0;
     for (uint32_t qi = 0; qi < Q.nqueues; qi++) {
@@ -1350,7 +1449,8 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4)))
                 if constexpr (REFILL) gather_texel_refill<STACK, FEAT>(A, ti, lane, stk + tid, lq + tid, lqj + tid, 64 * WAVES,
                                                                      subc + (tid >> 6) * kFrame * kSubCache);
                 else gather_texel<STACK, false, 64, false, FEAT>(A, ti, lane, stk + tid, lq + tid, lqj + tid, 64 * WAVES,
-                                                                subc + (tid >> 6) * kFrame * kSubCache);
+                                                                subc + (tid >> 6) * kFrame * kSubCache,
+                                                                pscr + (tid >> 6) * kPermScr);
         }
     }
 }
