@@ -571,6 +571,79 @@ __device__ inline float3 layer_dir(int i, const float3& t, const float3& b, cons
 constexpr uint32_t kPermWords = 32, kPermMaxRays = 1024;
 constexpr uint32_t kPermScr = 2 * kPermWords + 64 + (64 + kPermWords) / 2;
 
+// The acceptance pass of a direction-binned series of n hemisphere rays whose candidates
+// start at f0: word w holds the acceptance of candidates f0+64w .. f0+64w+63 (only the first
+// n accepted count), column c the acceptance of candidates f0+64w+c over w.  Columns are
+// ordered by the key of their candidates' index mod 64 (base-4 digits, coarsest first: the
+// filler's coarse cells first), and the key starts give each key's first position in the
+// binned order.  Returns false (nothing usable) when the candidates do not fit kPermWords
+// words; else sets filler past the series as the sequential hand-out leaves it.
+__device__ inline bool perm_prepare(uint32_t* pscr, uint32_t f0, uint32_t n, uint32_t lane, uint32_t& filler)
+{
+    uint64_t* const pmask = (uint64_t*)pscr;
+    uint32_t* const pcol = pscr + 2 * kPermWords;
+    uint16_t* const pstart = (uint16_t*)(pscr + 2 * kPermWords + 64);
+    uint16_t* const pprefix = pstart + 64;
+    uint32_t acc = 0, W = 0, colbits = 0, lastw_full = 0, lastbit = 0;
+    __builtin_amdgcn_wave_barrier();
+    while (acc < n && W < kPermWords) {
+        float cx, cy;
+        filler_point(f0 + 64u * W + lane, cx, cy);
+        const bool ok = !(cx * cx + cy * cy >= 1.f);
+        uint64_t am = __ballot(ok);
+        const uint32_t na = (uint32_t)__popcll(am);
+        lastw_full = acc + na == n;
+        if (acc + na > n) {
+            lastbit = nth_set_bit(am, n - acc - 1u);
+            am &= lastbit == 63u ? ~0ull : ((1ull << (lastbit + 1u)) - 1ull);
+        }
+        if (lane == 0) { pmask[W] = am; pprefix[W] = (uint16_t)acc; }
+        colbits |= (uint32_t)((am >> lane) & 1ull) << W;
+        acc += (uint32_t)__popcll(am);
+        W++;
+    }
+    if (acc != n) return false;
+    filler = lastw_full ? f0 + 64u * W : f0 + 64u * (W - 1u) + lastbit + 1u;
+    const uint32_t c = (f0 + lane) & 63u;
+    const uint32_t key = ((c & 3u) << 4) | (((c >> 2) & 3u) << 2) | ((c >> 4) & 3u);
+    pcol[lane] = colbits;
+    pstart[key] = (uint16_t)__popc(colbits);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint32_t kc = pstart[lane];                           // count of key = lane
+    uint32_t incl = kc;
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+        const uint32_t v = (uint32_t)__shfl_up((int)incl, o);
+        if (lane >= o) incl += v;
+    }
+    __builtin_amdgcn_wave_barrier();
+    pstart[lane] = (uint16_t)(incl - kc);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    return true;
+}
+
+// position P (< n) of the binned order: the shooter (its rank among the accepted
+// candidates, as the sequential hand-out ranks them) and the candidate's filler index
+__device__ inline void perm_pick(const uint32_t* pscr, uint32_t f0, uint32_t P, uint32_t& shooter, uint32_t& cand)
+{
+    const uint64_t* const pmask = (const uint64_t*)pscr;
+    const uint32_t* const pcol = pscr + 2 * kPermWords;
+    const uint16_t* const pstart = (const uint16_t*)(pscr + 2 * kPermWords + 64);
+    const uint16_t* const pprefix = pstart + 64;
+    uint32_t k = 0;
+    for (uint32_t step = 32; step; step >>= 1)
+        if (pstart[k + step] <= P) k += step;
+    const uint32_t t = P - pstart[k];
+    const uint32_t cm = (k >> 4) | (((k >> 2) & 3u) << 2) | ((k & 3u) << 4);
+    const uint32_t col = (cm - f0) & 63u;
+    const uint32_t w = nth_set_bit((uint64_t)pcol[col], t);
+    shooter = pprefix[w] + (uint32_t)__popcll(pmask[w] & lanemask_lt(col));
+    cand = f0 + 64u * w + col;
+}
+
 template <int STACK, bool STATS, int G = 64, bool LAYERS = false, int FEAT = 0>
 __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, int32_t* mystk, uint32_t* lq, uint16_t* lqj,
                              int stride, float* subc = nullptr, uint32_t* pscr = nullptr)
@@ -781,65 +854,16 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
                            && n <= kPermMaxRays && (m == 1 || (A.walk && walk_has(A.walk + ti)));
         bool permDone = false;
         if (permH) {
-            uint64_t* const pmask = (uint64_t*)pscr;                       // word w: accepted among candidates 64w..64w+63
-            uint32_t* const pcol = pscr + 2 * kPermWords;                   // column c: bit w = candidate 64w+c accepted
-            uint16_t* const pstart = (uint16_t*)(pscr + 2 * kPermWords + 64);   // 64 key starts, then kPermWords word prefixes
-            uint16_t* const pprefix = pstart + 64;
             const uint32_t f0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)filler);   // wave-uniform
-            uint32_t acc = 0, W = 0, colbits = 0, lastw_full = 0, lastbit = 0;
-            __builtin_amdgcn_wave_barrier();
-            while (acc < n && W < kPermWords) {
-                float cx, cy;
-                filler_point(f0 + 64u * W + lane, cx, cy);
-                const bool ok = !(cx * cx + cy * cy >= 1.f);
-                uint64_t am = __ballot(ok);
-                const uint32_t na = (uint32_t)__popcll(am);
-                lastw_full = acc + na == n;
-                if (acc + na > n) {
-                    lastbit = nth_set_bit(am, n - acc - 1u);
-                    am &= lastbit == 63u ? ~0ull : ((1ull << (lastbit + 1u)) - 1ull);
-                }
-                if (lane == 0) { pmask[W] = am; pprefix[W] = (uint16_t)acc; }
-                colbits |= (uint32_t)((am >> lane) & 1ull) << W;
-                acc += (uint32_t)__popcll(am);
-                W++;
-            }
-            if (acc == n) {
+            if (perm_prepare(pscr, f0, n, lane, filler)) {
                 permDone = true;
-                filler = lastw_full ? f0 + 64u * W : f0 + 64u * (W - 1u) + lastbit + 1u;
-                // key of this lane's column: the digits of its candidates' index mod 64, coarsest first
-                const uint32_t c = (f0 + lane) & 63u;
-                const uint32_t key = ((c & 3u) << 4) | (((c >> 2) & 3u) << 2) | ((c >> 4) & 3u);
-                pcol[lane] = colbits;
-                pstart[key] = (uint16_t)__popc(colbits);
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                const uint32_t kc = pstart[lane];                           // count of key = lane
-                uint32_t incl = kc;
-                for (uint32_t o = 1; o < 64; o <<= 1) {
-                    const uint32_t v = (uint32_t)__shfl_up((int)incl, o);
-                    if (lane >= o) incl += v;
-                }
-                __builtin_amdgcn_wave_barrier();
-                pstart[lane] = (uint16_t)(incl - kc);
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 for (uint32_t base = 0; base < n; base += 64u) {
                     const uint32_t P = base + lane;
                     if (P < n) {
-                        uint32_t k = 0;
-                        for (uint32_t step = 32; step; step >>= 1)
-                            if (pstart[k + step] <= P) k += step;
-                        const uint32_t t = P - pstart[k];
-                        const uint32_t cm = (k >> 4) | (((k >> 2) & 3u) << 2) | ((k & 3u) << 4);
-                        const uint32_t col = (cm - f0) & 63u;
-                        const uint32_t w = nth_set_bit((uint64_t)pcol[col], t);
-                        const uint32_t i = 64u * w + col;
-                        const uint32_t sh = pprefix[w] + (uint32_t)__popcll(pmask[w] & lanemask_lt(col));
+                        uint32_t sh, cand;
+                        perm_pick(pscr, f0, P, sh, cand);
                         float x, y;
-                        filler_point(f0 + i, x, y);
+                        filler_point(cand, x, y);
                         const uint32_t myj = m > 1 ? walk_pick(A.walk + ti, m, sh) : 0u;
                         shoot_h(sh, myj, x, y);
                     }
@@ -991,7 +1015,7 @@ constexpr uint32_t kGatherRefill = 16;
 
 template <int STACK, int FEAT>
 __device__ void gather_texel_refill(const GatherArgs& A, uint32_t ti, uint32_t lane, int32_t* mystk, uint32_t* lq, uint16_t* lqj,
-                                    int stride, float* subc)
+                                    int stride, float* subc, uint32_t* pscr)
 {
     const lsg_texel T = A.texels[ti];
     const uint32_t m = T.sub_count_flags & 0xffffu, flags = (T.sub_count_flags >> 16) & 0xffu;
@@ -1077,6 +1101,15 @@ __device__ void gather_texel_refill(const GatherArgs& A, uint32_t ti, uint32_t l
             }
             __builtin_amdgcn_wave_barrier();
         };
+        // hemisphere rays in the direction-binned order (as gather_texel): free lanes take
+        // the next positions of that order
+        uint32_t f0 = 0;
+        bool perm = false;
+        if (shootH && allH && shooter_base == 0 && n == hemi_rays && n > 64u && n <= kPermMaxRays
+            && (m == 1 || (A.walk && walk_has(A.walk + ti)))) {
+            f0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)filler);
+            perm = perm_prepare(pscr, f0, n, lane, filler);
+        }
         uint32_t next = 0;                         // next shooter to hand out (wave-uniform)
         bool busy = false, ok = true;
         uint32_t stri = 0;                         // the lane's shooter triangle
@@ -1091,15 +1124,26 @@ __device__ void gather_texel_refill(const GatherArgs& A, uint32_t ti, uint32_t l
                 const uint32_t rank = (uint32_t)__popcll(fm & lanemask_lt(lane));
                 const uint32_t take = min((uint32_t)__popcll(fm), n - next);
                 const bool mine = !busy && rank < take;
-                const uint32_t s = next + rank;
+                uint32_t s = next + rank;
+                float my_x = 0.f, my_y = 0.f;
                 bool doH = false, doL = false;
-                if (mine && shootH) doH = allH || shots_before(s + 1, n, hemi_rays) > shots_before(s, n, hemi_rays);
-                if (mine && shootL) doL = allL || shots_before(s + 1, n, rounds) > shots_before(s, n, rounds);
+                if (perm) {
+                    // binned order: position s gives the shooter and its filler point
+                    if (mine) {
+                        uint32_t cand;
+                        perm_pick(pscr, f0, s, s, cand);
+                        filler_point(cand, my_x, my_y);
+                        doH = true;
+                        if (shootL) doL = allL || shots_before(s + 1, n, rounds) > shots_before(s, n, rounds);
+                    }
+                } else {
+                    if (mine && shootH) doH = allH || shots_before(s + 1, n, hemi_rays) > shots_before(s, n, hemi_rays);
+                    if (mine && shootL) doL = allL || shots_before(s + 1, n, rounds) > shots_before(s, n, rounds);
+                }
                 // filler points in shooter order (GetCirclePoint, gather.cpp:85-90)
                 const uint64_t hmask = __ballot(doH);
-                const uint32_t need = (uint32_t)__popcll(hmask);
+                const uint32_t need = perm ? 0u : (uint32_t)__popcll(hmask);
                 const uint32_t myrank = (uint32_t)__popcll(hmask & lanemask_lt(lane));
-                float my_x = 0.f, my_y = 0.f;
                 uint32_t got = 0;
                 while (got < need) {
                     const uint32_t cand = filler + lane;
@@ -1431,7 +1475,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4)))
     __shared__ uint32_t lq[2 * 64 * WAVES];
     __shared__ uint16_t lqj[2 * 64 * WAVES];
     __shared__ float subc[WAVES * kFrame * kSubCache];
-    __shared__ uint32_t pscr[REFILL ? 1 : WAVES * kPermScr];
+    __shared__ uint32_t pscr[WAVES * kPermScr];
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const uint32_t home = Q.nqueues > 1 ? blockIdx.x % Q.nqueues : 0;
     for (uint32_t qi = 0; qi < Q.nqueues; qi++) {
@@ -1447,7 +1491,8 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4)))
             const uint32_t a = tile * Q.tile, b = min(A.ntex, a + Q.tile);
             for (uint32_t ti = a; ti < b; ti++)
                 if constexpr (REFILL) gather_texel_refill<STACK, FEAT>(A, ti, lane, stk + tid, lq + tid, lqj + tid, 64 * WAVES,
-                                                                     subc + (tid >> 6) * kFrame * kSubCache);
+                                                                     subc + (tid >> 6) * kFrame * kSubCache,
+                                                                     pscr + (tid >> 6) * kPermScr);
                 else gather_texel<STACK, false, 64, false, FEAT>(A, ti, lane, stk + tid, lq + tid, lqj + tid, 64 * WAVES,
                                                                 subc + (tid >> 6) * kFrame * kSubCache,
                                                                 pscr + (tid >> 6) * kPermScr);
