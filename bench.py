@@ -42,17 +42,20 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 NODE_BYTES = 128          # one 4-wide node per traversal step (n_node counts these)
 # L2-to-fabric bytes per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by
-# the same counters on a 1 GiB copy (x2.0 and x1.0): profiles/pmc_traffic_k_gather_k_trace_closest.md
-TRAFFIC_BYTES = {"k_gather": 4.668e9, "k_trace_closest": 1.194e9}
+# the same counters on a 1 GiB copy (x2.0 and x1.0):
+# profiles/pmc_traffic_k_gather_persistent_plan_bake_box_room_50k.md,
+# profiles/pmc_traffic_k_trace_closest_collider_ico100k.md
+TRAFFIC_BYTES = {"k_gather_persistent": 3.372e9, "k_trace_closest": 1.194e9}
 # VALU issue: a SIMD issues one wave64 VALU instruction per 4 cycles (16 lanes wide; the
 # SQ counters give one quad-cycle of SQ_ACTIVE_INST_VALU per SQ_INSTS_VALU), so MI355X
 # issues 256 CUs x 4 SIMDs x 2.4 GHz / 4 = 614.4 G wave64 VALU instructions per second.
 VALU_PEAK_G = 256 * 4 * 2.4 / 4
 # wave64 VALU instructions per launch and active lanes per instruction, from rocprofv3
 # --pmc SQ_INSTS_VALU SQ_ACTIVE_INST_VALU SQ_THREAD_CYCLES_VALU:
-# profiles/pmc_sq_valu_k_gather_k_trace_closest.md
-VALU_INSTS = {"k_gather": 52.0965e9, "k_trace_closest": 289.757e6}
-LANE_UTIL = {"k_gather": 0.6285, "k_trace_closest": 0.1342}
+# profiles/pmc_sq_valu_k_gather_persistent_plan_bake_box_room_50k.md,
+# profiles/pmc_sq_valu_k_trace_closest_collider_ico100k.md
+VALU_INSTS = {"k_gather_persistent": 47.8033e9, "k_trace_closest": 289.757e6}
+LANE_UTIL = {"k_gather_persistent": 0.7067, "k_trace_closest": 0.1342}
 METRIC = "closest-hit Mrays/s; baked texels/s at ≤2% RMSE, plain vs learned gather"
 
 
@@ -123,6 +126,9 @@ def collider_workload(steps, warmup, cpu_threads, with_cpu=True):
 def _valu(kernel, kernel_ms):
     """VALU issue of a kernel: its profiled instruction count per launch over the live
     launch time, against the chip's issue peak."""
+    if kernel not in VALU_INSTS:
+        return dict(achieved=None, peak=VALU_PEAK_G, unit="G VALU instr/s", frac=None,
+                    note="no profiled instruction count for this kernel")
     g = VALU_INSTS[kernel] / (kernel_ms * 1e-3) / 1e9
     return dict(achieved=round(g, 1), peak=VALU_PEAK_G, unit="G VALU instr/s", frac=round(g / VALU_PEAK_G, 4),
                 insts_per_launch=VALU_INSTS[kernel], lanes_active=LANE_UTIL[kernel])
@@ -173,68 +179,77 @@ def collider_cpu_baseline(V, I, R, threads, col):
 # ---- plain bake: box room, ~50k triangles, 21 lightmaps of 256x256 ----------------------------------------------------------------
 
 def bake_workload(steps: int, warmup: int, cpu_threads: int):
-    """Plain-gather bake of the box room; one step = final gather + filters + RGBF writeback."""
+    """Lightmap bake of the box room through the bake plan (lsg_bake_*, the path the solver
+    adapter's updateLightmaps takes): one step = final gather (persistent, XCD-local tile
+    queues, launch shape from the autotuner) + filter chain + RGBF writeback, replayed from a
+    hipGraph.  The exitance table comes from the plan's first gather, run once before."""
     from lightsprintsdk_amd import native as N, scenes as S
-    from lightsprintsdk_amd.bake import SceneBake, GatherParams, box_room_scene
+    from lightsprintsdk_amd.bake import BakePlan, GatherParams, box_room_scene, autotune, TEXEL_DTYPE, SUBTEXEL_DTYPE
     sc = box_room_scene()
     t0 = time.perf_counter()
-    sb = SceneBake(sc, GatherParams(quality=1024, ray_length_min=S.minimal_safe_distance(sc.verts), seed=1))
+    plan = BakePlan(sc, GatherParams(quality=1024, ray_length_min=S.minimal_safe_distance(sc.verts), seed=1))
     setup_s = time.perf_counter() - t0
+    L = N.lib()
     t0 = time.perf_counter()
-    sb.first_gather()
+    plan.run(first=True, graph=False)
+    N.check(L.lsg_synchronize(), "sync")
     first_s = time.perf_counter() - t0
-    L, s, e0, e1 = _events()
+    cache = os.environ.get("LSG_TUNING_CACHE", os.path.expanduser("~/.cache/lsgpu_tuning.json"))
+    os.makedirs(os.path.dirname(cache), exist_ok=True)
+    t0 = time.perf_counter()
+    tune = autotune(plan, sample_texels=150_000, cache_path=cache)
+    tune_s = time.perf_counter() - t0
     for _ in range(warmup):
-        sb.bake(s)
+        plan.run(first=False, graph=True)
     N.check(L.lsg_synchronize(), "sync")
     t0 = time.perf_counter()
     for _ in range(steps):
-        sb.bake(s)                     # clear + final gather + filter chain + RGBF writeback
+        plan.run(first=False, graph=True)      # final gather + filter chain + RGBF writeback
     N.check(L.lsg_synchronize(), "sync")
     wall = time.perf_counter() - t0
-    # the dominant kernel alone, timed with HIP events on the stream it runs on
-    kreps = max(2, min(steps, 5))
-    N.check(L.lsg_event_record(e0, s), "ev")
-    for _ in range(kreps):
-        sb.final_gather(s)
-    N.check(L.lsg_event_record(e1, s), "ev")
-    N.check(L.lsg_synchronize(), "sync")
-    ms = ctypes.c_float()
-    N.check(L.lsg_event_elapsed_ms(e0, e1, ctypes.byref(ms)), "elapsed")
-    kernel_ms = ms.value / kreps
-    ntex = len(sb.texels)
+    # the dominant kernel alone (every texel, the tuned shape), HIP events on its stream
+    kernel_ms = float(np.mean([plan.sample_ms(1) for _ in range(3)]))
+    inf = plan.info()
+    ntex, nsub = int(inf.num_texels), int(inf.num_subtexels)
+    texels = np.empty(ntex, TEXEL_DTYPE)
+    subs = np.empty(nsub, SUBTEXEL_DTYPE)
+    N.check(L.lsg_memcpy_d2h(texels.ctypes.data_as(ctypes.c_void_p), inf.texels, texels.nbytes), "d2h")
+    N.check(L.lsg_memcpy_d2h(subs.ctypes.data_as(ctypes.c_void_p), inf.subtexels, subs.nbytes), "d2h")
     texels_per_s = ntex / (wall / steps)
     # traversal work of the same gather (separate counting launch)
     st = (ctypes.c_double * 6)()
-    p = sb.params.c()
-    N.check(L.lsg_gather_stats(sb.gs._h, sb.d_tex.ptr, ntex, sb.d_sub.ptr, len(sb.subs), ctypes.byref(p), sb.d_out.ptr, st),
+    p = plan.params.c()
+    N.check(L.lsg_gather_stats(plan.gs._h, inf.texels, ntex, inf.subtexels, nsub, ctypes.byref(p), inf.rgba, st),
             "gather_stats")
+    plan.run(first=False, graph=False)            # the counting launch wrote the work buffer
     hr, hn, ht, sr, sn, stt = list(st)
     b_ray = 48 + NODE_BYTES * hn / hr + 48 * ht / hr
     b_shadow = 48 + NODE_BYTES * sn / max(sr, 1) + 48 * stt / max(sr, 1)
-    n_sub = len(sb.subs) / ntex
+    n_sub = nsub / ntex
     R = hr / ntex
     Lr = sr / ntex
     b_texel = R * (b_ray + 12) + Lr * b_shadow + 32 * n_sub + 12
     achieved = b_texel * ntex / (kernel_ms * 1e-3) / 1e9
     rays_per_s = (hr + sr) / (kernel_ms * 1e-3)
-    sb.bake(s)
-    maps = sb.lightmaps_rgbf()
-    lit = np.concatenate([m.reshape(-1, 3) for m in maps])[sb.texels["out_index"]]
+    lit = np.concatenate([plan.lightmap(k).reshape(-1, 3) for k in range(len(sc.objects))])
+    kern = "k_gather_persistent"
     res = dict(
         metric=METRIC, value=round(texels_per_s, 1), unit="texels/s", ms_per_step=round(wall / steps * 1e3, 3),
         dtype="f32",
-        config=dict(workload="plain lightmap bake: box room 10x10x4 m + 20 boxes, 21 lightmaps 256x256 RGBF, "
+        config=dict(workload="lightmap bake (bake plan): box room 10x10x4 m + 20 boxes, 21 lightmaps 256x256 RGBF, "
                              "1024 hemisphere rays + 103 shadow rounds per texel (point light + emissive quad); "
-                             "step = final gather + filter chain + RGBF writeback",
-                    triangles=int(sc.ntri), lightmaps=len(sc.objects), texels=int(ntex), subtexels=int(len(sb.subs)),
+                             "step = final gather + filter chain + RGBF writeback, hipGraph replay",
+                    triangles=int(sc.ntri), lightmaps=len(sc.objects), texels=ntex, subtexels=nsub,
                     quality=1024, hemisphere_rays=int(hr), shadow_rays=int(sr), rays_per_s=round(rays_per_s, 1),
-                    mrays_per_s=round(rays_per_s / 1e6, 2), first_gather_s=round(first_s, 4), setup_s=round(setup_s, 3),
-                    mean_irradiance=float(lit.mean())),
-        roofline=dict(bound="valu", **{k: v for k, v in _valu("k_gather", kernel_ms).items()},
-                      traffic=TRAFFIC_BYTES["k_gather"], kernel="k_gather", kernel_ms=round(kernel_ms, 4),
-                      hbm=dict(achieved_gbs=round(TRAFFIC_BYTES["k_gather"] / (kernel_ms * 1e-3) / 1e9, 1), peak_gbs=HBM_PEAK_GBS,
-                               frac=round(TRAFFIC_BYTES["k_gather"] / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+                    mrays_per_s=round(rays_per_s / 1e6, 2), first_run_s=round(first_s, 4), setup_s=round(setup_s, 3),
+                    tuning=dict(tile=inf.tuning.tile, waves_per_block=inf.tuning.waves_per_block,
+                                blocks_per_cu=inf.tuning.blocks_per_cu, xcd_queues=inf.tuning.xcd_queues,
+                                lane_refill=inf.tuning.lane_refill),
+                    autotune=tune, autotune_s=round(tune_s, 2), mean_irradiance=float(lit.mean())),
+        roofline=dict(bound="valu", **{k: v for k, v in _valu(kern, kernel_ms).items()},
+                      traffic=TRAFFIC_BYTES.get(kern), kernel=kern, kernel_ms=round(kernel_ms, 4),
+                      hbm=dict(achieved_gbs=round(TRAFFIC_BYTES.get(kern, 0) / (kernel_ms * 1e-3) / 1e9, 1), peak_gbs=HBM_PEAK_GBS,
+                               frac=round(TRAFFIC_BYTES.get(kern, 0) / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
                                note="L2-to-fabric bytes per launch (FETCH_SIZE/WRITE_SIZE); the scene stays in L2/MALL"),
                       model=dict(bytes_per_texel=round(b_texel, 1), bytes_per_hemisphere_ray=round(b_ray, 2),
                                  bytes_per_shadow_ray=round(b_shadow, 2), n_node_hemi=round(hn / hr, 3),
@@ -244,31 +259,31 @@ def bake_workload(steps: int, warmup: int, cpu_threads: int):
                                  note="node and triangle bytes the traversal reads, most of them cache hits: "
                                       "a work measure, not HBM traffic")),
     )
-    res["cpu_baseline"] = bake_cpu_baseline(sb, cpu_threads)
+    res["cpu_baseline"] = bake_cpu_baseline(plan.gs, texels, subs, plan.params, plan.exitance(), cpu_threads)
     return res
 
 
-def bake_cpu_baseline(sb, threads: int, target_s: float = 12.0):
+def bake_cpu_baseline(gs, texels, subs, params, exitance, threads: int, target_s: float = 12.0):
     """The CPU oracle's FP64 statement of processTexel (a port: the reference's RRSolver
     needs boost and is not built here) on a bounded, evenly spread sample of the same
     texels, all host cores; with the per-core rate and, where the compiled reference
     collider exists, the reference-equivalent rate port / rho (rho measured on the collider,
     which is most of a gather's work)."""
     O = _oracle()
-    tree = sb.gs.col.download_tree()
-    arrays = sb.gs.oracle_arrays(sb.exitance)
+    tree = gs.col.download_tree()
+    arrays = gs.oracle_arrays(exitance)
     n0 = 64 * threads
-    sel = np.linspace(0, len(sb.texels) - 1, n0).astype(np.int64)
+    sel = np.linspace(0, len(texels) - 1, n0).astype(np.int64)
     t0 = time.perf_counter()
-    O.gather(arrays, sb.params.as_dict(), sb.texels[sel].view(np.uint32).reshape(-1, 4), sb.subs, tree=tree, threads=threads)
+    O.gather(arrays, params.as_dict(), texels[sel].view(np.uint32).reshape(-1, 4), subs, tree=tree, threads=threads)
     dt = time.perf_counter() - t0
-    n1 = int(min(len(sb.texels), max(n0, n0 * target_s / max(dt, 1e-3))))
-    sel = np.linspace(0, len(sb.texels) - 1, n1).astype(np.int64)
+    n1 = int(min(len(texels), max(n0, n0 * target_s / max(dt, 1e-3))))
+    sel = np.linspace(0, len(texels) - 1, n1).astype(np.int64)
     t0 = time.perf_counter()
-    O.gather(arrays, sb.params.as_dict(), sb.texels[sel].view(np.uint32).reshape(-1, 4), sb.subs, tree=tree, threads=threads)
+    O.gather(arrays, params.as_dict(), texels[sel].view(np.uint32).reshape(-1, 4), subs, tree=tree, threads=threads)
     dt = time.perf_counter() - t0
     out = dict(value=round(n1 / dt, 1), unit="texels/s", cores=threads, kind="port",
-               sample=f"{n1} of {len(sb.texels)} texels evenly spread, oracle C gather (FP64 sums) over the same tree, "
+               sample=f"{n1} of {len(texels)} texels evenly spread, oracle C gather (FP64 sums) over the same tree, "
                       f"{dt:.1f} s on all {threads} host cores", per_core=round(n1 / dt / threads, 1))
     cal = cpu_calibration()
     if cal:
