@@ -317,34 +317,40 @@ __global__ __launch_bounds__(256) void k_write_tris(uint32_t n, const uint32_t* 
 // ---------------------------------------------------------------------------------------
 // traversal
 
-// Closest hit of a device ray batch.  Each wave owns kRaysPerLane * 64 consecutive rays
-// (lane L: rays base + j*64 + L).  A lane whose ray is finished waits until kRefill
-// lanes are free; then they all load their next rays together while the others keep
-// tracing (trace_resume yields) -- incoherent rays end at very different depths, and
-// without refill most lanes of a wave would idle behind the longest ray.
-constexpr uint32_t kRaysPerLane = 4;
-constexpr uint32_t kRefill = 16;
+// Closest hit of a device ray batch.  The waves of a launch take rays from one queue (a
+// counter in device memory): whenever kRefill of a wave's lanes are free, the wave takes that
+// many consecutive rays with one atomic add and hands them to the free lanes in lane order,
+// while the other lanes keep tracing (trace_resume yields and keeps their state).  No wave
+// idles while rays remain, whichever rays turn out long.  Rays that miss the scene box are
+// finished at once.
+constexpr uint32_t kRefill = 24;
+constexpr uint32_t kRayQueues = 256;      // counters per collider: launches in flight at once
+
+__global__ void k_ray_queue_reset(unsigned long long* q) { if (threadIdx.x == 0) *q = 0ull; }
 
 template <int STACK>
 __global__ __launch_bounds__(256) void k_trace_closest(ColliderView c, const float4* __restrict__ ro, const float4* __restrict__ rd,
                                                        uint32_t n, int4* __restrict__ hits, uint8_t* __restrict__ front,
-                                                       uint32_t* __restrict__ overflow)
+                                                       uint32_t* __restrict__ overflow, unsigned long long* __restrict__ queue)
 {
     __shared__ int32_t stk[STACK * 256];
     const uint32_t lane = threadIdx.x & 63u;
-    const uint64_t base = ((uint64_t)blockIdx.x * 4u + (threadIdx.x >> 6)) * (64u * kRaysPerLane) + lane;
-    uint32_t j = 0;                 // next ray slot of this lane
     bool busy = false, ok = true;
-    uint64_t ray = 0;
+    bool done = false;              // the queue is empty (wave-uniform)
+    uint32_t ray = 0;
     TraceState S;
     for (;;) {
-        const bool can = !busy && j < kRaysPerLane && base + 64ull * j < n;
-        const uint64_t want = __ballot(can), bm = __ballot(busy);
-        if (!want && !bm) break;
-        if ((uint32_t)__popcll(want) >= kRefill || !bm) {
-            // start the next ray; rays that miss the scene box are finished at once
-            while (!busy && j < kRaysPerLane && base + 64ull * j < n) {
-                ray = base + 64ull * (j++);
+        const uint64_t bm = __ballot(busy);
+        const uint32_t nfree = 64u - (uint32_t)__popcll(bm);
+        if (!done && (nfree >= kRefill || !bm)) {
+            unsigned long long q0 = 0;
+            if (lane == 0) q0 = atomicAdd(queue, (unsigned long long)nfree);
+            q0 = ((unsigned long long)(uint32_t)__builtin_amdgcn_readfirstlane((int)(q0 >> 32)) << 32)
+                 | (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)q0);
+            if (q0 >= n || n - q0 <= nfree) done = true;
+            const uint32_t rank = (uint32_t)__popcll(~bm & (lane ? (~0ull >> (64 - lane)) : 0ull));   // free lanes below
+            if (!busy && q0 < n && rank < n - q0) {
+                ray = (uint32_t)q0 + rank;
                 const float4 a = ro[ray], b = rd[ray];
                 if (trace_begin(S, c, make_float3(a.x, a.y, a.z), make_float3(b.x, b.y, b.z), a.w, b.w)) busy = true;
                 else {
@@ -352,13 +358,11 @@ __global__ __launch_bounds__(256) void k_trace_closest(ColliderView c, const flo
                     if (front) front[ray] = 0;
                 }
             }
+            continue;               // lanes whose rays missed the box may take more at once
         }
-        const uint32_t nwait = (uint32_t)__popcll(__ballot(!busy && j < kRaysPerLane && base + 64ull * j < n));
+        if (!bm) break;             // queue empty and nothing in flight
         if (!busy) continue;
-        const uint32_t ntr = (uint32_t)__popcll(__ballot(1));
-        const uint32_t need = nwait < kRefill ? kRefill - nwait : 1u;
-        const bool more = want != 0 || nwait != 0;
-        const uint32_t yield_at = (more && ntr > need) ? ntr - need : 0u;
+        const uint32_t yield_at = done ? 0u : 64u - kRefill;
         if (!trace_resume<STACK>(S, NoFilter(), c, stk + threadIdx.x, 256, yield_at, ok)) continue;
         busy = false;
         hits[ray] = S.bi >= 0 ? make_int4(S.bi, __float_as_int(S.best), __float_as_int(S.bu), __float_as_int(S.bv))
@@ -419,7 +423,7 @@ template <class T> static int dalloc(T** p, size_t n, uint64_t* acc = nullptr)
 static void free_impl(ColliderImpl& c)
 {
     dfree(c.d_verts); dfree(c.d_idx); dfree(c.d_nodes); dfree(c.d_nodes4);
-    dfree(c.d_plane); dfree(c.d_rest); dfree(c.d_slot);
+    dfree(c.d_plane); dfree(c.d_rest); dfree(c.d_slot); dfree(c.d_queue);
     c = ColliderImpl();
 }
 
@@ -598,6 +602,7 @@ static int build(ColliderImpl& C, const float* verts, uint32_t nv, const uint32_
     CK(dalloc(&C.d_plane, n, &C.bytes)); CK(dalloc(&C.d_rest, n, &C.bytes));
     CK(dalloc(&C.d_slot, nt, &C.bytes));
     CKH(hipMemset(C.d_slot, 0, sizeof(uint32_t) * (size_t)(nt ? nt : 1)));   // degenerate triangles: never hit
+    CK(dalloc(&C.d_queue, kRayQueues, &C.bytes));
     CK(dalloc(&d_plo, n)); CK(dalloc(&d_phi, n));
     if (n) k_write_tris<<<blocks(n), 256>>>(n, d_vals, d_plane_u, d_sv_u, d_ev_u, d_bytes, d_blo_u, d_bhi_u,
                                             C.d_plane, C.d_rest, d_plo, d_phi, C.d_slot);
@@ -724,6 +729,7 @@ static int load_tree(ColliderImpl& C, const float* verts, uint32_t nv, const uin
     CK(dalloc(&C.d_plane, n, &C.bytes)); CK(dalloc(&C.d_rest, n, &C.bytes));
     CK(dalloc(&C.d_slot, nt, &C.bytes));
     CKH(hipMemset(C.d_slot, 0, sizeof(uint32_t) * N));
+    CK(dalloc(&C.d_queue, kRayQueues, &C.bytes));
     if (n) k_write_tris<<<blocks(n), 256>>>(n, d_order, d_plane_u, d_sv_u, d_ev_u, d_bytes, d_blo_u, d_bhi_u,
                                             C.d_plane, C.d_rest, d_plo, d_phi, C.d_slot);
     CKH(hipGetLastError());
@@ -980,10 +986,24 @@ int lsg_collider_intersect_device(const lsg_collider* c, const void* ro, const v
     if (!stack) return fail(LSG_ERR_LIMIT, "tree too deep");
     uint32_t* d_over = nullptr;   // stack >= depth+1 by construction; the kernel guard only prevents overrun
     hipStream_t s = (hipStream_t)stream;
+    // a counter of this collider's ring (launches in flight on one collider take different
+    // ones), zeroed on the launch's stream; waves enough to fill every CU at 4 per SIMD
+    unsigned long long* q = C.d_queue + (__atomic_fetch_add(&C.queue_next, 1u, __ATOMIC_RELAXED) % kRayQueues);
+    k_ray_queue_reset<<<1, 64, 0, s>>>(q);
+    // resident blocks per CU as the kernel's registers and LDS allow, queried once
+    static int cus = 0, per32 = 0, per64 = 0;
+    if (!cus) {
+        int v = 0;
+        cus = (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, 0) == hipSuccess && v > 0) ? v : 256;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per32, k_trace_closest<32>, 256, 0) != hipSuccess || per32 < 1) per32 = 4;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per64, k_trace_closest<64>, 256, 0) != hipSuccess || per64 < 1) per64 = 2;
+    }
+    const unsigned resident = (unsigned)cus * (unsigned)(stack <= 32 ? per32 : per64);
+    const unsigned nb = std::max(1u, std::min(blocks(n, 256u * 4u), resident));
     if (stack <= 32)
-        k_trace_closest<32><<<blocks(n, 256 * kRaysPerLane), 256, 0, s>>>(C.view(), (const float4*)ro, (const float4*)rd, n, (int4*)hits, (uint8_t*)front, d_over);
+        k_trace_closest<32><<<nb, 256, 0, s>>>(C.view(), (const float4*)ro, (const float4*)rd, n, (int4*)hits, (uint8_t*)front, d_over, q);
     else
-        k_trace_closest<64><<<blocks(n, 256 * kRaysPerLane), 256, 0, s>>>(C.view(), (const float4*)ro, (const float4*)rd, n, (int4*)hits, (uint8_t*)front, d_over);
+        k_trace_closest<64><<<nb, 256, 0, s>>>(C.view(), (const float4*)ro, (const float4*)rd, n, (int4*)hits, (uint8_t*)front, d_over, q);
     LSG_HIP(hipGetLastError());
     return LSG_OK;
 }

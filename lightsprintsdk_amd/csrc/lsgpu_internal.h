@@ -127,6 +127,8 @@ struct ColliderImpl {
     float4* d_plane = nullptr; TriRest* d_rest = nullptr;
     uint32_t* d_slot = nullptr;   // leaf-order slot of each original triangle
     uint32_t flags = 0;           // lsg_build_params flags the tree was built with
+    unsigned long long* d_queue = nullptr;   // ray-queue counters of closest-hit launches (a ring)
+    mutable uint32_t queue_next = 0;   // next counter a launch takes (host atomic)
     uint64_t bytes = 0;
     double build_ms = 0;
     ColliderView view() const
