@@ -1003,15 +1003,16 @@ __device__ void gather_texel(const GatherArgs& A, uint32_t ti, uint32_t wlane, i
 }
 
 // processTexel by one wave with refill (at most 64 lanes per series chunk replaced by a
-// pool): lanes take the series' shooters in order as they become free, trace their
-// hemisphere rays with the resumable traversal, and the wave hands out the next shooters
-// whenever kGatherRefill lanes are free, instead of waiting for the slowest ray of every
-// 64.  Shooter s still gets the filler point that sequential order gives it (free lanes
-// take consecutive shooters in lane order, and the hand-out ranks them in that order).
+// pool): lanes take the series' shooters as they become free (in the direction-binned
+// order where gather_texel uses it, else in sequence), trace their hemisphere rays with the
+// resumable traversal, and the wave hands out the next shooters whenever kGatherRefill
+// lanes are free, instead of waiting for the slowest ray of every 64 (a sweep over 8-48 on
+// the batched bake picked 32).  Shooter s still gets the filler point that sequential order
+// gives it.
 // Light shooters are queued as before; their shadow rays run only while no hemisphere
 // ray is in flight (the lanes' LDS stacks are free then): when the queue holds a full
 // batch, refills pause until every lane is done, and the rest at the series' end.
-constexpr uint32_t kGatherRefill = 16;
+constexpr uint32_t kGatherRefill = 32;
 
 template <int STACK, int FEAT>
 __device__ void gather_texel_refill(const GatherArgs& A, uint32_t ti, uint32_t lane, int32_t* mystk, uint32_t* lq, uint16_t* lqj,
