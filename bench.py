@@ -45,7 +45,7 @@ NODE_BYTES = 128          # one 4-wide node per traversal step (n_node counts th
 # the same counters on a 1 GiB copy (x2.0 and x1.0):
 # profiles/pmc_traffic_k_gather_persistent_plan_bake_box_room_50k.md,
 # profiles/pmc_traffic_k_trace_closest_collider_ico100k.md
-TRAFFIC_BYTES = {"k_gather_persistent": 3.372e9, "k_trace_closest": 1.194e9}
+TRAFFIC_BYTES = {"k_gather_persistent": 3.372e9, "k_trace_closest": 1.068e9}
 # VALU issue: a SIMD issues one wave64 VALU instruction per 4 cycles (16 lanes wide; the
 # SQ counters give one quad-cycle of SQ_ACTIVE_INST_VALU per SQ_INSTS_VALU), so MI355X
 # issues 256 CUs x 4 SIMDs x 2.4 GHz / 4 = 614.4 G wave64 VALU instructions per second.
@@ -54,8 +54,8 @@ VALU_PEAK_G = 256 * 4 * 2.4 / 4
 # --pmc SQ_INSTS_VALU SQ_ACTIVE_INST_VALU SQ_THREAD_CYCLES_VALU:
 # profiles/pmc_sq_valu_k_gather_persistent_plan_bake_box_room_50k.md,
 # profiles/pmc_sq_valu_k_trace_closest_collider_ico100k.md
-VALU_INSTS = {"k_gather_persistent": 47.8033e9, "k_trace_closest": 289.757e6}
-LANE_UTIL = {"k_gather_persistent": 0.7067, "k_trace_closest": 0.1342}
+VALU_INSTS = {"k_gather_persistent": 47.8033e9, "k_trace_closest": 207.612e6}
+LANE_UTIL = {"k_gather_persistent": 0.7067, "k_trace_closest": 0.1960}
 METRIC = "closest-hit Mrays/s; baked texels/s at ≤2% RMSE, plain vs learned gather"
 
 
