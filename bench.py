@@ -231,7 +231,8 @@ def bake_workload(steps: int, warmup: int, cpu_threads: int):
     b_texel = R * (b_ray + 12) + Lr * b_shadow + 32 * n_sub + 12
     achieved = b_texel * ntex / (kernel_ms * 1e-3) / 1e9
     rays_per_s = (hr + sr) / (kernel_ms * 1e-3)
-    lit = np.concatenate([plan.lightmap(k).reshape(-1, 3) for k in range(len(sc.objects))])
+    # the filtered lightmaps in slot order (objects' slots are consecutive), at the covered texels
+    lit = np.concatenate([plan.lightmap(k).reshape(-1, 3) for k in range(len(sc.objects))])[texels["out_index"]]
     kern = "k_gather_persistent"
     res = dict(
         metric=METRIC, value=round(texels_per_s, 1), unit="texels/s", ms_per_step=round(wall / steps * 1e3, 3),
