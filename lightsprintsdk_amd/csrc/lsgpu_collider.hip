@@ -435,83 +435,6 @@ static inline unsigned blocks(uint64_t n, unsigned per = 256) { return (unsigned
 // at most H stack entries, so with P entries already on the stack a node may take the
 // children ch when P + |ch| - 1 + max H(inner child) <= budget.  With budget =
 // STACK - 1 >= BVH2 depth this holds at the root, and by construction below it.
-static int collapse4(ColliderImpl& C, uint32_t budget, const std::vector<Node>& n2)
-{
-    C.stack_need = 0;
-    if (C.root < 0 || !C.nnodes) {
-        C.root4 = C.root; C.nnodes4 = 0;
-        return dalloc(&C.d_nodes4, 1, &C.bytes);
-    }
-    struct Ch { int32_t ref; float b[6]; };
-    auto bits = [](float x) { int32_t r; memcpy(&r, &x, 4); return r; };
-    auto kids = [&](int32_t i, Ch* out) {
-        const Node& q = n2[(size_t)i];
-        out[0] = Ch{bits(q.q3.x), {q.q0.x, q.q0.y, q.q0.z, q.q0.w, q.q1.x, q.q1.y}};
-        out[1] = Ch{bits(q.q3.y), {q.q1.z, q.q1.w, q.q2.x, q.q2.y, q.q2.z, q.q2.w}};
-    };
-    auto area = [](const Ch& c) {
-        const float dx = c.b[1] - c.b[0], dy = c.b[3] - c.b[2], dz = c.b[5] - c.b[4];
-        return dx * dy + dy * dz + dz * dx;
-    };
-    // heights of the reachable inner nodes (children come after their parent in `order`)
-    std::vector<uint8_t> H(C.nnodes, 0);
-    std::vector<int32_t> order;
-    order.reserve(C.nnodes);
-    order.push_back(C.root);
-    for (size_t k = 0; k < order.size(); k++) {
-        Ch c2[2]; kids(order[k], c2);
-        for (int j = 0; j < 2; j++) if (c2[j].ref >= 0) order.push_back(c2[j].ref);
-    }
-    for (size_t k = order.size(); k-- > 0;) {
-        Ch c2[2]; kids(order[k], c2);
-        uint32_t h = 0;
-        for (int j = 0; j < 2; j++) if (c2[j].ref >= 0) h = std::max<uint32_t>(h, H[(size_t)c2[j].ref]);
-        H[(size_t)order[k]] = (uint8_t)std::min<uint32_t>(h + 1, 255);
-    }
-    if (H[(size_t)C.root] > budget) return fail(LSG_ERR_LIMIT, "collapse4: tree height %u over stack budget %u", H[(size_t)C.root], budget);
-    std::vector<Node4> n4(1);
-    n4.reserve(order.size() / 2 + 1);
-    struct Work { int32_t i2; uint32_t slot, P; };
-    std::vector<Work> work{{C.root, 0u, 0u}};
-    while (!work.empty()) {
-        const Work w = work.back(); work.pop_back();
-        Ch ch[4]; kids(w.i2, ch);
-        int m = 2;
-        while (m < 4) {
-            int best = -1; float ba = -1.f;
-            for (int k = 0; k < m; k++) if (ch[k].ref >= 0 && area(ch[k]) > ba) { ba = area(ch[k]); best = k; }
-            if (best < 0) break;
-            Ch g[2]; kids(ch[best].ref, g);
-            uint32_t hmax = 0;
-            for (int k = 0; k < m; k++) if (k != best && ch[k].ref >= 0) hmax = std::max<uint32_t>(hmax, H[(size_t)ch[k].ref]);
-            for (int j = 0; j < 2; j++) if (g[j].ref >= 0) hmax = std::max<uint32_t>(hmax, H[(size_t)g[j].ref]);
-            if (w.P + (uint32_t)m + hmax > budget) break;     // one child more = one push more
-            ch[best] = g[0]; ch[m++] = g[1];
-        }
-        const uint32_t P = w.P + (uint32_t)m - 1;
-        C.stack_need = std::max(C.stack_need, P);
-        Node4 nd;
-        float* f = (float*)nd.q;
-        int32_t* r = (int32_t*)&nd.q[6];
-        for (int k = 0; k < 4; k++) {
-            for (int a = 0; a < 6; a++) f[4 * a + k] = k < m ? ch[k].b[a] : 0.f;
-            if (k >= m) { r[k] = kNoChild; continue; }
-            if (ch[k].ref < 0) { r[k] = ch[k].ref; continue; }
-            r[k] = (int32_t)n4.size();
-            work.push_back({ch[k].ref, (uint32_t)n4.size(), P});
-            n4.emplace_back();
-        }
-        nd.q[7] = make_float4(0.f, 0.f, 0.f, 0.f);
-        n4[w.slot] = nd;
-    }
-    C.nnodes4 = (uint32_t)n4.size();
-    C.root4 = 0;
-    int st = dalloc(&C.d_nodes4, n4.size(), &C.bytes);
-    if (st) return st;
-    LSG_HIP(hipMemcpy(C.d_nodes4, n4.data(), sizeof(Node4) * n4.size(), hipMemcpyHostToDevice));
-    return LSG_OK;
-}
-
 static int build(ColliderImpl& C, const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt, uint32_t leafmax,
                  bool sah, const volatile int* aborting)
 {
@@ -625,32 +548,37 @@ static int build(ColliderImpl& C, const float* verts, uint32_t nv, const uint32_
         CKH(hipGetLastError());
         CKH(hipMemcpy(&C.depth, d_depth, 4, hipMemcpyDeviceToHost));
         C.root = 0;
+        if (sah) {
+            // inner nodes rebuilt over the same leaves by SAH on the GPU (the host's
+            // lsg_bvh_rebuild_sah gives the same tree); kept unless the new tree would need
+            // the larger traversal stack
+            Node* s2 = nullptr;
+            uint32_t nn = 0, d = 0;
+            st = sah_rebuild_device(d_first, d_last, d_split, (int)n, leafmax, C.d_nodes, &s2, &nn, &d);
+            if (st > 1) { cleanup(); return st; }
+            if (st == 0) {
+                const int have = stack_size_for_depth(C.depth), want = stack_size_for_depth(d);
+                if (want && (!have || want <= have)) {
+                    dfree(C.d_nodes);
+                    C.bytes += sizeof(Node) * (uint64_t)nn;
+                    C.bytes -= sizeof(Node) * (uint64_t)(C.nnodes ? C.nnodes : 1);
+                    C.d_nodes = s2;
+                    C.nnodes = nn; C.root = 0; C.depth = d;
+                } else dfree(s2);
+            }
+            st = 0;
+        }
     }
     CKH(hipDeviceSynchronize());
     cleanup();
 #undef CK
 #undef CKH
-    std::vector<Node> n2(C.nnodes);
-    if (C.nnodes) LSG_HIP(hipMemcpy(n2.data(), C.d_nodes, sizeof(Node) * n2.size(), hipMemcpyDeviceToHost));
-    if (sah && C.root >= 0 && C.nnodes) {
-        // inner nodes rebuilt over the same leaves by SAH; kept unless the new tree would
-        // need the larger traversal stack
-        std::vector<Node> s2(C.nnodes);
-        uint32_t nn = 0, d = 0;
-        int32_t r = 0;
-        if (sah_rebuild((const float*)n2.data(), C.nnodes, C.root, (float*)s2.data(), &nn, &r, &d) == 0) {
-            const int have = stack_size_for_depth(C.depth), want = stack_size_for_depth(d);
-            if (want && (!have || want <= have)) {
-                s2.resize(nn);
-                LSG_HIP(hipMemcpy(C.d_nodes, s2.data(), sizeof(Node) * nn, hipMemcpyHostToDevice));
-                n2.swap(s2);
-                C.nnodes = nn; C.root = r; C.depth = d;
-            }
-        }
-    }
     if (!stack_size_for_depth(C.depth))
         return fail(LSG_ERR_LIMIT, "BVH depth %u exceeds the traversal stack (63)", C.depth);
-    if ((st = collapse4(C, (uint32_t)stack_size_for_depth(C.depth) - 1, n2))) return st;
+    if (C.root < 0 || !C.nnodes) {
+        C.root4 = C.root; C.nnodes4 = 0; C.stack_need = 0;
+        if ((st = dalloc(&C.d_nodes4, 1, &C.bytes))) return st;
+    } else if ((st = collapse4_device(C, (uint32_t)stack_size_for_depth(C.depth) - 1))) return st;
     C.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return LSG_OK;
 }

@@ -159,6 +159,14 @@ int sort_pairs_u32(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t*
 void sha1(const void* a, size_t na, const void* b, size_t nb, uint8_t out[20]);
 // RRHash::getFileName's cache file name for a hash (lsgpu_hash.cpp)
 std::string cache_file_name(const uint8_t hash[20], uint32_t version, const char* location, const char* extension);
+// The same rebuild on the GPU (lsgpu_sah.hip) from the LBVH's Karras arrays and nodes:
+// *out_nodes (device, *out_count nodes, root 0) and its depth; 0 = rebuilt, 1 = fewer than
+// two leaves, else an LSG_ERR_* code
+int sah_rebuild_device(const int* d_first, const int* d_last, const int* d_split, int nprim, uint32_t leafmax, const Node* d_nodes,
+                       Node** out_nodes, uint32_t* out_count, uint32_t* out_depth);
+// The BVH4 of C's BVH2 on the GPU (lsgpu_sah.hip): C.d_nodes4, nnodes4, root4, stack_need
+struct ColliderImpl;
+int collapse4_device(ColliderImpl& C, uint32_t budget);
 // SAH rebuild of a BVH2's inner nodes over its leaves (lsgpu_sah.cpp): 0 = rebuilt,
 // 1 = fewer than two leaves, 2 = malformed tree
 int sah_rebuild(const float* nodes, uint32_t nnodes, int32_t root, float* out, uint32_t* out_nodes, int32_t* out_root,

@@ -10,6 +10,9 @@
 // reference's BSP/kd build (createAndSaveBsp, RRCollider/bsp.cpp:1192), whose kd split
 // search weighs box surface areas as well (find_best_root_kd, bsp.cpp:388-574).
 //
+// The leaves enter in the order of their triangles (the LBVH's Morton order); the build
+// only reorders them by stable partitions, so the result depends on the leaves alone.
+//
 // Node layout as lsg_collider_download: 16 floats per inner node, child 0 box
 // (lo.x hi.x lo.y hi.y lo.z hi.z), child 1 box, child refs as int bits, two spare.
 // Output nodes are numbered in preorder (root = 0, left subtree before right), so the
@@ -214,6 +217,11 @@ int sah_rebuild(const float* nodes, uint32_t nnodes, int32_t root, float* out, u
     }
     const uint32_t n = (uint32_t)L.size();
     if (n < 2) return 1;
+    // leaves in the order of their triangles (the LBVH's Morton order): a leaf's start
+    // index in the leaf-ordered triangles is unique
+    std::sort(L.begin(), L.end(), [](const Leaf& a, const Leaf& b) {
+        return (((uint32_t)a.ref >> 4) & 0x7ffffffu) < (((uint32_t)b.ref >> 4) & 0x7ffffffu);
+    });
     std::vector<Leaf> tmp(n);
     Task t0;
     t0.begin = 0; t0.end = n; t0.parent = -1; t0.side = 0; t0.depth = 1;
