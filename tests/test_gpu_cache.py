@@ -108,7 +108,7 @@ def test_gpu_cache_small_trees_and_star(gpu, tmp_path):
 
 
 def test_gpu_cache_large_mesh_load_time(gpu, tmp_path):
-    """The 10M-triangle city: the cached tree skips the GPU build and the host SAH rebuild."""
+    """The 10M-triangle city: the cached tree skips the GPU build (LBVH, SAH rebuild, collapse)."""
     import time
     from lightsprintsdk_amd.collider import RRCollider
     sc = S.city_terrain(seed=5, sun_direction=S.sun_sweep(0, 60))
