@@ -391,8 +391,26 @@ typedef struct lsg_filter_params {
     float   smoothing_amount;   /* lightmapSmooth sigma */
     float   spread_foreground;  /* lightmapGrow steps */
     float   background[4];      /* lightmapFillBackground colour */
-    int32_t wrap;               /* must be 0 (wrap=true is not supported) */
+    int32_t wrap;               /* neighbours across opposite borders (RRBuffer.cpp:676-680, 822-826, 904-922) */
 } lsg_filter_params;
+
+/* One unwrap seam (UnwrapSeam, RRBuffer.cpp:529-533): a mesh edge shared by two
+ * triangles whose lightmap uvs differ: edge1 = (u, v) of its two ends in the first
+ * triangle's unwrap, edge2 = the same two ends in the second's. */
+typedef struct lsg_unwrap_seam {
+    float edge1[4];
+    float edge2[4];
+} lsg_unwrap_seam;
+
+/* The seams lightmapSmooth smooths for an object (UnwrapSeams, RRBuffer.cpp:536-625):
+ * vertices stitched as RRMesh::createOptimizedVertices(0, 1 degree, 0, NULL) stitches
+ * them (lsg_stitch_vertices), then every triangle edge that a later triangle shares at
+ * both stitched ends with different uvs there, in the reference's order.  Host function.
+ * verts, idx, corner_normals as lsg_stitch_vertices; tri_uv: float[ntri*6], the lightmap
+ * uv of every triangle corner; seams: out, the first cap seams (NULL with cap 0 to count);
+ * *nseams out: how many there are. */
+int  lsg_unwrap_seams(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_t ntri, const float* corner_normals,
+                      const float* tri_uv, lsg_unwrap_seam* seams, uint32_t cap, uint32_t* nseams);
 
 /* The filter chain updateLightmap runs on every gathered buffer (lightmap.cpp:852-876):
  * lightmapSmooth -> lightmapGrowForBilinearInterpolation -> lightmapGrow ->
@@ -409,6 +427,11 @@ typedef struct lsg_filter_plan lsg_filter_plan;
 int  lsg_filter_plan_create(const lsg_lightmap* maps, uint32_t nmaps, lsg_filter_plan** out);
 void lsg_filter_plan_destroy(lsg_filter_plan* plan);
 int  lsg_filter_plan_run(lsg_filter_plan* plan, float* rgba, float* rgbf, const lsg_filter_params* params, void* stream);
+/* The objects' unwrap seams, smoothed after every blur pass of lightmapSmooth
+ * (RRBuffer.cpp:734-777, the seams of lightmapSmooth's object argument): lightmap m's
+ * seams are seams[begin[m] .. begin[m+1]) (host arrays of the plan's nmaps + 1 offsets,
+ * copied); seams == NULL removes them. */
+int  lsg_filter_plan_set_seams(lsg_filter_plan* plan, const lsg_unwrap_seam* seams, const uint32_t* begin);
 
 /* No counterpart in the reference.  For callers that cannot allocate device memory
  * (tight serving loops, harnesses that forbid hipMalloc): lsg_filter_lightmaps with the
@@ -493,6 +516,9 @@ int  lsg_bake_run(lsg_bake* bake, const lsg_gather_params* first, const lsg_gath
  * Takes effect on the next lsg_bake_run with first != NULL (graph replays included);
  * 0 bounces (the default) = direct light only. */
 int  lsg_bake_set_bounces(lsg_bake* bake, uint32_t bounces, uint32_t quality);
+/* The unwrap seams of one object's lightmap (host array, copied; nseams 0 removes them),
+ * smoothed by the filter chain of later runs as lsg_filter_plan_set_seams describes. */
+int  lsg_bake_set_seams(lsg_bake* bake, uint32_t object, const lsg_unwrap_seam* seams, uint32_t nseams);
 /* times the persistent final gather over every stride-th texel (autotuner probe) */
 int  lsg_bake_gather_sample(lsg_bake* bake, const lsg_gather_params* params, uint32_t stride, float* ms);
 /* copies object `object`'s RGBF lightmap (width*height*3 floats) to the host */

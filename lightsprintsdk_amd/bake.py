@@ -392,6 +392,20 @@ class SceneBake:
             self._fplan = h.value
         return self._fplan
 
+    def set_seams(self, seams):
+        """Unwrap seams per lightmap (a list of float32 [n, 8] arrays, object_seams), smoothed
+        by every later filter(); None removes them."""
+        plan = self._filter_plan()
+        if seams is None:
+            N.check(N.lib().lsg_filter_plan_set_seams(plan, None, None), "lsg_filter_plan_set_seams")
+            return
+        assert len(seams) == len(self.slots)
+        arrs = [np.ascontiguousarray(a, np.float32).reshape(-1, 8) for a in seams]
+        begin = np.cumsum([0] + [len(a) for a in arrs]).astype(np.uint32)
+        allv = np.ascontiguousarray(np.concatenate(arrs) if arrs else np.zeros((0, 8), np.float32))
+        N.check(N.lib().lsg_filter_plan_set_seams(plan, allv.ctypes.data_as(ctypes.c_void_p) if len(allv) else None,
+                                                  begin.ctypes.data_as(ctypes.c_void_p)), "lsg_filter_plan_set_seams")
+
     def filter(self, stream=None, rgba=None, rgbf=None):
         """updateLightmap's filter chain + RGBF writeback on every lightmap (GPU)."""
         fp = self.filtering.c()
@@ -470,6 +484,36 @@ def stitch_vertices(verts, idx, corner_normals, weld_distance=0.0, max_smooth_an
                                         group.ctypes.data_as(ctypes.c_void_p), rep.ctypes.data_as(ctypes.c_void_p),
                                         ctypes.byref(ng)), "lsg_stitch_vertices")
     return group, rep[:ng.value].copy()
+
+
+def unwrap_seams(verts, idx, corner_normals, tri_uv) -> np.ndarray:
+    """The unwrap seams lightmapSmooth smooths for one object (lsg_unwrap_seams,
+    UnwrapSeams, RRBuffer.cpp:536-625): float32 [n, 8] per seam, the edge's two ends in
+    the first triangle's unwrap (u0, v0, u1, v1), then in the second's."""
+    v = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
+    i = np.ascontiguousarray(idx, np.uint32).reshape(-1, 3)
+    n = np.ascontiguousarray(corner_normals, np.float32).reshape(-1, 9)
+    uv = np.ascontiguousarray(tri_uv, np.float32).reshape(-1, 6)
+    assert len(n) == len(i) and len(uv) == len(i)
+    args = [v.ctypes.data_as(ctypes.c_void_p), len(v), i.ctypes.data_as(ctypes.c_void_p), len(i),
+            n.ctypes.data_as(ctypes.c_void_p), uv.ctypes.data_as(ctypes.c_void_p)]
+    cnt = ctypes.c_uint32()
+    N.check(N.lib().lsg_unwrap_seams(*args, None, 0, ctypes.byref(cnt)), "lsg_unwrap_seams")
+    out = np.zeros((cnt.value, 8), np.float32)
+    if cnt.value:
+        N.check(N.lib().lsg_unwrap_seams(*args, out.ctypes.data_as(ctypes.c_void_p), cnt.value, ctypes.byref(cnt)),
+                "lsg_unwrap_seams")
+    return out
+
+
+def object_seams(scene: S.BakeScene, k: int) -> np.ndarray:
+    """The unwrap seams of lightmapped object k of a scene: its own triangles, their
+    vertices and per-vertex normals (the object's mesh, as lightmapSmooth is given it)."""
+    ob = scene.objects[k]
+    tri = np.ascontiguousarray(scene.idx[ob.tri_begin:ob.tri_begin + ob.tri_count], np.uint32)
+    used, inv = np.unique(tri, return_inverse=True)
+    return unwrap_seams(scene.verts[used], inv.reshape(-1, 3), scene.normals[tri].reshape(-1, 9),
+                        scene.tri_uv[ob.tri_begin:ob.tri_begin + ob.tri_count])
 
 
 class VertexPlan:
@@ -615,6 +659,13 @@ class BakePlan:
         N.check(N.lib().lsg_bake_run(self._h, ctypes.byref(fp) if first else None, ctypes.byref(p),
                                      ctypes.byref(flt) if filtering else None, N.BAKE_USE_GRAPH if graph else 0, stream),
                 "lsg_bake_run")
+
+    def set_seams(self, k, seams):
+        """Unwrap seams of object k's lightmap (float32 [n, 8], object_seams), smoothed by
+        the filter chain of later runs; an empty array removes them."""
+        a = np.ascontiguousarray(seams, np.float32).reshape(-1, 8)
+        N.check(N.lib().lsg_bake_set_seams(self._h, k, a.ctypes.data_as(ctypes.c_void_p) if len(a) else None, len(a)),
+                "lsg_bake_set_seams")
 
     def set_bounces(self, bounces=0, quality=0):
         """Radiosity on the GPU (lsg_bake_set_bounces): after the first gather, `bounces`

@@ -14,6 +14,7 @@
 #include "lsgpu_internal.h"
 #include "lsgpu_gather.h"
 #include "lsgpu_filter.h"
+#include <cmath>
 #include <vector>
 #include <cstring>
 #include <chrono>
@@ -103,8 +104,11 @@ struct BakeImpl {
     float4* d_hemi = nullptr;          // a bounce's hemisphere gather per triangle
     float4* d_lrgba[4] = {nullptr, nullptr, nullptr, nullptr};   // layers 1..4 (layer 0 = d_rgba / d_rgbf)
     float* d_lrgbf[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<std::vector<lsg_unwrap_seam>> seam_lists;   // per lightmap (object), host copies
+    SeamStore seams;
     ~BakeImpl()
     {
+        seams_free(seams);
         for (int i = 0; i < 4; i++) { dfree(d_lrgba[i]); dfree(d_lrgbf[i]); }
         dfree(d_parts); dfree(d_hemi);
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -182,7 +186,8 @@ static int run_all(BakeImpl& B, const lsg_gather_params* first, const lsg_gather
     if (B.nslots) k_zero_f4<<<nblk(B.nslots), 256, 0, st>>>(B.d_rgba, B.nslots);   // a plan without lightmaps only gathers the solution
     if ((e = run_gather_final(B, fin, B.d_tex, B.ntex, st))) return e;
     if (fp) {
-        if ((e = filter_run(B.d_maps, B.nmaps, B.ftotal, B.d_rgba, B.d_rgbf, *fp, B.S, st, true, fixed_grow, steps_used))) return e;
+        if ((e = filter_run(B.d_maps, B.nmaps, B.ftotal, B.d_rgba, B.d_rgbf, *fp, B.S, st, true, fixed_grow, steps_used, &B.seams.view)))
+            return e;
     }
     LSG_HIP(hipGetLastError());
     return LSG_OK;
@@ -309,7 +314,6 @@ int lsg_bake_run(lsg_bake* b, const lsg_gather_params* first, const lsg_gather_p
                  const lsg_filter_params* filtering, uint32_t flags, void* stream)
 {
     if (!b || !final_params) return fail(LSG_ERR_INVALID, "lsg_bake_run: null argument");
-    if (filtering && filtering->wrap) return fail(LSG_ERR_INVALID, "lsg_bake_run: wrap=true is not supported");
     BakeImpl& B = b->impl;
     hipStream_t st = (hipStream_t)stream;
     if (!(flags & LSG_BAKE_USE_GRAPH)) {
@@ -327,9 +331,9 @@ int lsg_bake_run(lsg_bake* b, const lsg_gather_params* first, const lsg_gather_p
     {
         // the captured kernels hold the environment's buffer and shape as arguments
         const SceneImpl& S = B.scene->impl;
-        // (and the indirect records and point materials likewise)
-        const uintptr_t env[7] = {(uintptr_t)S.d_env, S.env_type, S.env_w, S.env_h, (uintptr_t)S.d_ind, (uintptr_t)S.d_tri_mat,
-                                  (uintptr_t)S.pm_version};
+        // (and the indirect records, point materials and unwrap seams likewise)
+        const uintptr_t env[8] = {(uintptr_t)S.d_env, S.env_type, S.env_w, S.env_h, (uintptr_t)S.d_ind, (uintptr_t)S.d_tri_mat,
+                                  (uintptr_t)S.pm_version, (uintptr_t)B.seams.version};
         const uint8_t* e8 = (const uint8_t*)env;
         key.insert(key.end(), e8, e8 + sizeof env);
     }
@@ -378,7 +382,6 @@ int lsg_bake_run_layers(lsg_bake* b, const lsg_gather_params* first, const lsg_g
 {
     if (!b || !final_params) return fail(LSG_ERR_INVALID, "lsg_bake_run_layers: null argument");
     if (!layer_mask || layer_mask >= (1u << LSG_NUM_LAYERS)) return fail(LSG_ERR_INVALID, "lsg_bake_run_layers: layer_mask %#x", layer_mask);
-    if (filtering && filtering->wrap) return fail(LSG_ERR_INVALID, "lsg_bake_run_layers: wrap=true is not supported");
     BakeImpl& B = b->impl;
     hipStream_t st = (hipStream_t)stream;
     float4* rgba[LSG_NUM_LAYERS];
@@ -400,7 +403,8 @@ int lsg_bake_run_layers(lsg_bake* b, const lsg_gather_params* first, const lsg_g
         for (int i = 0; i < LSG_NUM_LAYERS; i++) {
             if (!rgba[i]) continue;
             uint32_t used = 0;
-            if ((e = filter_run(B.d_maps, B.nmaps, B.ftotal, rgba[i], rgbf[i], *filtering, B.S, st, true, -1, &used))) return e;
+            if ((e = filter_run(B.d_maps, B.nmaps, B.ftotal, rgba[i], rgbf[i], *filtering, B.S, st, true, -1, &used, &B.seams.view)))
+                return e;
             if (i == 0) B.grow_steps = used;
         }
     }
@@ -433,6 +437,28 @@ int lsg_bake_download_layer_rgba(const lsg_bake* b, uint32_t object, uint32_t la
     const lsg_lightmap& m = B.maps[object];
     LSG_HIP(hipMemcpy(rgba, src + m.rgba_base, 16 * (size_t)m.width * m.height, hipMemcpyDeviceToHost));
     return LSG_OK;
+}
+
+int lsg_bake_set_seams(lsg_bake* b, uint32_t object, const lsg_unwrap_seam* seams, uint32_t nseams)
+{
+    if (!b) return fail(LSG_ERR_INVALID, "lsg_bake_set_seams: null plan");
+    BakeImpl& B = b->impl;
+    if (object >= B.nmaps) return fail(LSG_ERR_INVALID, "lsg_bake_set_seams: object %u of %u", object, B.nmaps);
+    if (nseams && !seams) return fail(LSG_ERR_INVALID, "lsg_bake_set_seams: null seams");
+    for (uint32_t i = 0; i < nseams; i++)
+        for (int k = 0; k < 4; k++)
+            if (!std::isfinite(seams[i].edge1[k]) || !std::isfinite(seams[i].edge2[k]))
+                return fail(LSG_ERR_INVALID, "lsg_bake_set_seams: seam %u has a coordinate that is not finite", i);
+    B.seam_lists.resize(B.nmaps);
+    B.seam_lists[object].assign(seams, seams + nseams);
+    std::vector<lsg_unwrap_seam> all;
+    std::vector<uint32_t> begin(B.nmaps + 1, 0);
+    for (uint32_t m = 0; m < B.nmaps; m++) {
+        all.insert(all.end(), B.seam_lists[m].begin(), B.seam_lists[m].end());
+        begin[m + 1] = (uint32_t)all.size();
+    }
+    B.graph_key.clear();
+    return seams_set(B.seams, B.nmaps, all.empty() ? nullptr : all.data(), begin.data());
 }
 
 int lsg_bake_set_bounces(lsg_bake* b, uint32_t bounces, uint32_t quality)

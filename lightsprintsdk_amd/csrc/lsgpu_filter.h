@@ -21,6 +21,29 @@ struct FilterScratch {
     hipEvent_t ev[2] = {nullptr, nullptr};
 };
 
+// Unwrap seams of a batch's lightmaps, on the device: map m's seams are
+// seams[begin[m] .. begin[m+1]); `with` lists the maps that have any (one workgroup each).
+struct SeamSet {
+    const lsg_unwrap_seam* seams = nullptr;
+    const uint32_t* begin = nullptr;
+    const uint32_t* with = nullptr;
+    uint32_t nwith = 0;
+};
+
+// a SeamSet's device arrays, owned; version counts the changes (captured graphs hold
+// the arrays and the grid)
+struct SeamStore {
+    lsg_unwrap_seam* d_seams = nullptr;
+    uint32_t* d_begin = nullptr;
+    uint32_t* d_with = nullptr;
+    SeamSet view;
+    uint32_t version = 0;
+};
+// copies host seams (begin: nmaps + 1 offsets) into S, replacing what it held;
+// seams == nullptr or no seams at all leaves S empty
+int seams_set(SeamStore& S, uint32_t nmaps, const lsg_unwrap_seam* seams, const uint32_t* begin);
+void seams_free(SeamStore& S);
+
 int maps_upload(const lsg_lightmap* maps, uint32_t nmaps, FilterMap** d_maps, uint32_t* total);
 int scratch_alloc(FilterScratch& S, uint32_t total);
 void scratch_free(FilterScratch& S);
@@ -29,6 +52,6 @@ void scratch_free(FilterScratch& S);
 // capture), S.changed[0] = whether the last step still changed a texel.
 int filter_run(const FilterMap* d_maps, uint32_t nmaps, uint32_t total, float4* rgba, float* rgbf,
                const lsg_filter_params& fp, FilterScratch& S, hipStream_t s, bool rgba_out,
-               int fixed_steps, uint32_t* steps_used);
+               int fixed_steps, uint32_t* steps_used, const SeamSet* seams = nullptr);
 
 } // namespace lsg

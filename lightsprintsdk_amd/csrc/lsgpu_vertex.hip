@@ -275,6 +275,65 @@ int lsg_stitch_vertices(const float* verts, uint32_t nverts, const uint32_t* idx
     return LSG_OK;
 }
 
+// Unwrap seams (UnwrapSeams, RRBuffer.cpp:536-625).  Vertices are stitched as the
+// reference stitches them for this purpose (createOptimizedVertices(0, RR_DEG2RAD(1), 0,
+// NULL): equal positions, normals within one degree, uvs ignored).  Each stitched vertex
+// lists the triangles at it in triangle order, a triangle once per corner there (a
+// counting sort of the corners).  Then for every triangle t1 and edge e = (corner e,
+// corner e+1) = (a, b), the triangles t2 > t1 listed at b are matched against the list at
+// a by one cursor that only moves forward, as the reference's merge does (so a triangle
+// listed twice at b matches twice); a match whose uvs at a and b differ from t1's is a
+// seam.  The order of the seams, and the smoothing that reads them in that order, are
+// therefore the reference's.
+int lsg_unwrap_seams(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_t ntri, const float* corner_normals,
+                     const float* tri_uv, lsg_unwrap_seam* seams, uint32_t cap, uint32_t* nseams)
+{
+    if (!verts || !idx || !corner_normals || !tri_uv || !nseams || (cap && !seams))
+        return fail(LSG_ERR_INVALID, "lsg_unwrap_seams: null argument");
+    std::vector<uint32_t> group(nverts);
+    uint32_t ng = 0;
+    const float one_degree = 1 * ((float)3.14159265358979323846 / 180);
+    if (int st = lsg_stitch_vertices(verts, nverts, idx, ntri, corner_normals, 0.f, one_degree, group.data(), nullptr, &ng)) return st;
+    std::vector<uint32_t> head(ng + 1, 0), at(3 * (size_t)ntri);
+    for (size_t c = 0; c < 3 * (size_t)ntri; c++) head[group[idx[c]] + 1]++;
+    for (uint32_t g = 0; g < ng; g++) head[g + 1] += head[g];
+    {
+        std::vector<uint32_t> put(head.begin(), head.end() - 1);
+        for (size_t c = 0; c < 3 * (size_t)ntri; c++) at[put[group[idx[c]]]++] = (uint32_t)(c / 3);
+    }
+    auto corner_group = [&](uint32_t t, int k) { return group[idx[3 * (size_t)t + k]]; };
+    uint32_t n = 0;
+    for (uint32_t t1 = 0; t1 < ntri; t1++) {
+        const float* uv1 = tri_uv + 6 * (size_t)t1;
+        for (int e = 0; e < 3; e++) {
+            const int f = (e + 1) % 3;
+            const uint32_t a = corner_group(t1, e), b = corner_group(t1, f);
+            const uint32_t* la = at.data() + head[a];
+            const uint32_t na = head[a + 1] - head[a];
+            uint32_t cur = 0;
+            for (uint32_t j = head[b]; j < head[b + 1]; j++) {
+                const uint32_t t2 = at[j];
+                if (t2 <= t1) continue;
+                while (cur + 1 < na && la[cur] < t2) cur++;
+                if (la[cur] != t2) continue;
+                const int ka = corner_group(t2, 0) == a ? 0 : (corner_group(t2, 1) == a ? 1 : 2);
+                const int kb = corner_group(t2, 0) == b ? 0 : (corner_group(t2, 1) == b ? 1 : 2);
+                const float* uv2 = tri_uv + 6 * (size_t)t2;
+                lsg_unwrap_seam sm;
+                sm.edge1[0] = uv1[2 * e]; sm.edge1[1] = uv1[2 * e + 1]; sm.edge1[2] = uv1[2 * f]; sm.edge1[3] = uv1[2 * f + 1];
+                sm.edge2[0] = uv2[2 * ka]; sm.edge2[1] = uv2[2 * ka + 1]; sm.edge2[2] = uv2[2 * kb]; sm.edge2[3] = uv2[2 * kb + 1];
+                const bool same = sm.edge1[0] == sm.edge2[0] && sm.edge1[1] == sm.edge2[1] && sm.edge1[2] == sm.edge2[2] &&
+                                  sm.edge1[3] == sm.edge2[3];
+                if (same) continue;
+                if (n < cap) seams[n] = sm;
+                n++;
+            }
+        }
+    }
+    *nseams = n;
+    return LSG_OK;
+}
+
 int lsg_vertex_plan_create(const lsg_scene* scene, const uint32_t* corner_group, const float* group_positions, uint32_t ngroups,
                            lsg_vertex_plan** out)
 {

@@ -118,6 +118,55 @@ struct PointMaterials
 	~PointMaterials() { unlockAll(); }
 };
 
+// The seams lightmapSmooth smooths for an object (UnwrapSeams, RRBuffer.cpp:536-625): its
+// own mesh and the lightmap texcoord of its first face group's material; none without
+// them.  The library stitches and searches (lsg_unwrap_seams).
+std::vector<lsg_unwrap_seam> objectSeams(const RRObject* object)
+{
+	std::vector<lsg_unwrap_seam> out;
+	if (!object || !object->faceGroups.size() || !object->faceGroups[0].material) return out;
+	const unsigned texcoord = object->faceGroups[0].material->lightmap.texcoord;
+	const RRMesh* mesh = object->getCollider() ? object->getCollider()->getMesh() : nullptr;
+	RRMesh::TriangleMapping map;
+	if (!mesh || !mesh->getTriangleMapping(0, map, texcoord)) return out;
+	const unsigned nv = mesh->getNumVertices(), nt = mesh->getNumTriangles();
+	std::vector<float> verts(3 * (size_t)nv), normals(9 * (size_t)nt), uv(6 * (size_t)nt, 0.f);
+	std::vector<uint32_t> idx(3 * (size_t)nt);
+	for (unsigned v = 0; v < nv; v++)
+	{
+		RRMesh::Vertex p;
+		mesh->getVertex(v, p);
+		for (int c = 0; c < 3; c++) verts[3 * (size_t)v + c] = p[c];
+	}
+	for (unsigned t = 0; t < nt; t++)
+	{
+		RRMesh::Triangle tri;
+		RRMesh::TriangleNormals tn;
+		mesh->getTriangle(t, tri);
+		mesh->getTriangleNormals(t, tn);
+		const bool mapped = mesh->getTriangleMapping(t, map, texcoord);
+		for (int k = 0; k < 3; k++)
+		{
+			idx[3 * (size_t)t + k] = tri[k];
+			for (int c = 0; c < 3; c++) normals[9 * (size_t)t + 3 * k + c] = tn.vertex[k].normal[c];
+			if (mapped) { uv[6 * (size_t)t + 2 * k] = map.uv[k][0]; uv[6 * (size_t)t + 2 * k + 1] = map.uv[k][1]; }
+		}
+	}
+	uint32_t n = 0;
+	int st = lsg_unwrap_seams(verts.data(), nv, idx.data(), nt, normals.data(), uv.data(), nullptr, 0, &n);
+	if (st == LSG_OK && n)
+	{
+		out.resize(n);
+		st = lsg_unwrap_seams(verts.data(), nv, idx.data(), nt, normals.data(), uv.data(), out.data(), n, &n);
+	}
+	if (st != LSG_OK)
+	{
+		RR_LIMITED_TIMES(10, RRReporter::report(WARN, "Unwrap seams won't be filtered: %s\n", lsg_last_error()));
+		out.clear();
+	}
+	return out;
+}
+
 } // namespace
 
 RRSolverHIP::~RRSolverHIP()
@@ -425,6 +474,19 @@ unsigned RRSolverHIP::bake(const std::vector<Target>& pixel, const std::vector<T
 		if (!filtering) filtering = &defaults;
 		lsg_filter_params f = { filtering->smoothingAmount, (float)filtering->spreadForegroundColor, {0, 0, 0, 0},
 		                        filtering->wrap ? 1 : 0 };
+		// lightmapSmooth smooths each object's unwrap seams after every blur pass
+		// (lightmap.cpp:865 passes the object); sigma 0 returns before them
+		if (filtering->smoothingAmount > 0)
+			for (unsigned k = 0; k < numTargets; k++)
+			{
+				const std::vector<lsg_unwrap_seam> seams = objectSeams(objects[pixel[k].object]);
+				if (!seams.empty() && lsg_bake_set_seams(plan, k, seams.data(), (uint32_t)seams.size()) != LSG_OK)
+				{
+					RRReporter::report(ERRO, "RRSolverHIP: %s\n", lsg_last_error());
+					lsg_bake_destroy(plan);
+					return 0;
+				}
+			}
 		// GPU radiosity: first gather at half quality (gather.cpp:1083-1088), lights and the
 		// environment only, then the bounces; otherwise the reference solver's exitance is used
 		lsg_gather_params first = g;

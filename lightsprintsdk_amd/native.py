@@ -124,6 +124,8 @@ SIGNATURES = {
     "lsg_filter_plan_create": (ctypes.c_int, [P, U32, ctypes.POINTER(P)]),
     "lsg_filter_plan_destroy": (None, [P]),
     "lsg_filter_plan_run": (ctypes.c_int, [P, P, P, ctypes.POINTER(FilterParams), P]),
+    "lsg_filter_plan_set_seams": (ctypes.c_int, [P, P, P]),
+    "lsg_unwrap_seams": (ctypes.c_int, [P, U32, P, U32, P, P, P, U32, P]),
     "lsg_filter_lightmaps_noalloc": (ctypes.c_int, [P, U32, P, P, ctypes.POINTER(FilterParams), P]),
     "lsg_bake_create": (ctypes.c_int, [P, P, U32, P, P, ctypes.POINTER(P)]),
     "lsg_bake_destroy": (None, [P]),
@@ -133,6 +135,7 @@ SIGNATURES = {
     "lsg_bake_gather_sample": (ctypes.c_int, [P, ctypes.POINTER(GatherParams), U32, ctypes.POINTER(ctypes.c_float)]),
     "lsg_bake_download": (ctypes.c_int, [P, U32, P]),
     "lsg_bake_set_bounces": (ctypes.c_int, [P, U32, U32]),
+    "lsg_bake_set_seams": (ctypes.c_int, [P, U32, P, U32]),
     "lsg_bake_run_layers": (ctypes.c_int, [P, ctypes.POINTER(GatherParams), ctypes.POINTER(GatherParams), ctypes.POINTER(FilterParams), U32, P]),
     "lsg_bake_download_layer": (ctypes.c_int, [P, U32, U32, P]),
     "lsg_bake_download_layer_rgba": (ctypes.c_int, [P, U32, U32, P]),

@@ -541,6 +541,39 @@ def filter_lightmap(rgba, sigma=1.0, spread=1000.0, bg=(0, 0, 0, 0), library=Non
     return img, out
 
 
+def filter_lightmap_ex(rgba, sigma=1.0, spread=1000.0, bg=(0, 0, 0, 0), wrap=False, seams=None, library=None):
+    """The filter chain with FilteringParameters::wrap and the unwrap seams of the map's
+    object (float32 [n, 8], or None) smoothed after every blur pass."""
+    L = lib() if library is None else library
+    L.lsgo_filter_lightmap_ex.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_float,
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint32]
+    img = np.ascontiguousarray(rgba, dtype=np.float32).copy()
+    h, w = img.shape[:2]
+    b = np.asarray(bg, np.float32)
+    out = np.zeros((h, w, 3), np.float32)
+    sm = None if seams is None else np.ascontiguousarray(seams, np.float32).reshape(-1, 8)
+    L.lsgo_filter_lightmap_ex(_p(img), w, h, sigma, spread, _p(b), _p(out), 1 if wrap else 0,
+                              None if sm is None else _p(sm), 0 if sm is None else len(sm))
+    return img, out
+
+
+def unwrap_seams_from_corners(corners, nverts, tri_uv, library=None):
+    """UnwrapSeams's search over a stitched mesh (the stitched vertex of every corner,
+    [ntri*3]): float32 [n, 8]."""
+    L = lib() if library is None else library
+    L.lsgo_unwrap_seams.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
+                                    ctypes.c_uint32]
+    L.lsgo_unwrap_seams.restype = ctypes.c_uint32
+    c = np.ascontiguousarray(corners, np.uint32).reshape(-1)
+    uv = np.ascontiguousarray(tri_uv, np.float32).reshape(-1, 6)
+    nt = len(uv)
+    n = L.lsgo_unwrap_seams(_p(c), int(nverts), nt, _p(uv), None, 0)
+    out = np.zeros((n, 8), np.float32)
+    if n:
+        L.lsgo_unwrap_seams(_p(c), int(nverts), nt, _p(uv), _p(out), n)
+    return out
+
+
 def mlp_num_params():
     L = lib()
     L.lsgo_mlp_num_params.restype = ctypes.c_uint32

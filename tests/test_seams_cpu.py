@@ -1,0 +1,373 @@
+"""Wrapped filtering and unwrap-seam smoothing, pinned on the CPU.
+
+The oracle's filter chain (oracle/lsg_oracle_filters.c) with FilteringParameters::wrap
+and with an object's unwrap seams is checked against facts the reference's code fixes,
+derived by hand here rather than from the oracle:
+
+- lightmapGrow with wrap (RRBuffer/RRBuffer.cpp:912-927) takes the 8 neighbours as flat
+  texel numbers k + offset in 32-bit unsigned arithmetic reduced modulo the texel count:
+  a row's end continues on the next row, and where the count does not divide 2^32 a sum
+  that wrapped below zero lands elsewhere than k + offset + count;
+- lightmapSmooth's blur flags and lightmapGrowForBilinearInterpolation with wrap see the
+  neighbour across the opposite border of the same row or column (:676-680, 822-826);
+- the seam walk (:734-777): the texels under both edges of a seam end up at the mean of
+  the two sides, seams run in order (a seam reads what the seams before it wrote) and
+  after every blur pass, and points outside [0,1)^2 or on unused texels are skipped;
+- the seam search (UnwrapSeams, :536-625) on meshes whose seams are known.
+
+Copies of the oracle with a mistake planted in these parts must fail the checks.  The
+library's own seam search (lsg_unwrap_seams, host code) is compared with the oracle's,
+and, where the reference's vertex stitching is compiled (oracle/_ref/stitch_ref), with
+the oracle's search over the reference's own stitched mesh."""
+import ctypes
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FILTERS_C = os.path.join(ROOT, "oracle", "lsg_oracle_filters.c")
+
+EDGE_ONLY = (16 | 32 | 64 | 128) / 255.0     # every EDGE flag, no QUADRANT flag
+QUAD_ONLY = 15 / 255.0                       # every QUADRANT flag, no EDGE flag: no blur neighbour
+Y0_ONLY = 64 / 255.0                         # EDGE_Y0 only: no blur neighbour, no quadrant
+BG = (-1.0, -2.0, -3.0, 0.0)
+
+
+def _map(w, h):
+    return np.zeros((h, w, 4), np.float32)
+
+
+# ---- wrap ------------------------------------------------------------------------------
+
+def check_grow_wrap_flat(filt):
+    """4x4 map, one used texel at flat index 3 (x 3, y 0).  One lightmapGrow step with wrap
+    fills exactly the texels i with 3 in {i+1, i-1, i+4, i-4, i+5, i-3, i+3, i-5} mod 16
+    (16 divides 2^32): {0, 2, 4, 6, 7, 8, 14, 15}.  Per-row/column wrap would fill (0,3)
+    = 12 instead of (0,2) = 8; clamping would leave 0 and 8 empty."""
+    m = _map(4, 4)
+    a = np.float32([0.5, 0.25, 0.125])
+    m[0, 3, :3] = a
+    m[0, 3, 3] = EDGE_ONLY
+    _, rgbf = filt(m, sigma=0.0, spread=1.0, bg=BG, wrap=True)
+    got = rgbf.reshape(16, 3)
+    filled = {0, 2, 3, 4, 6, 7, 8, 14, 15}
+    for i in range(16):
+        want = a if i in filled else np.float32(BG[:3])
+        if not np.array_equal(got[i], want):
+            return 1.0
+    return 0.0
+
+
+def check_grow_wrap_unsigned(filt):
+    """3x3 map (9 does not divide 2^32; 2^32 = 4 mod 9), one used texel at 8.  For i < 5
+    the sums i-1, i-3, i-2, i-4 wrap below zero and land on (i + 3), (i + 1), (i + 2),
+    i mod 9 instead of i + 8, i + 6, i + 7, i + 5, so after one step only 4, 5, 6, 7 are
+    filled (i + 1, i + 3, i + 4, i + 2 reach 8 from 7, 5, 4, 6); texels 0-3, which
+    taking the sums modulo 9 properly would fill, stay empty until the second step."""
+    m = _map(3, 3)
+    b = np.float32([0.25, 0.5, 1.0])
+    m[2, 2, :3] = b
+    m[2, 2, 3] = EDGE_ONLY
+    _, one = filt(m, sigma=0.0, spread=1.0, bg=BG, wrap=True)
+    _, two = filt(m, sigma=0.0, spread=2.0, bg=BG, wrap=True)
+    one, two = one.reshape(9, 3), two.reshape(9, 3)
+    for i in range(9):
+        want = b if i >= 4 else np.float32(BG[:3])
+        if not np.array_equal(one[i], want):
+            return 1.0
+        if not np.array_equal(two[i], b):
+            return 1.0
+    return 0.0
+
+
+def _passes(sigma):
+    """The pass variances of lightmapSmooth in binary32 (min(remaining, 0.34) each)."""
+    s2sum = np.float32(sigma) * np.float32(sigma)
+    out = []
+    while s2sum > 0:
+        s2 = min(s2sum, np.float32(0.34))
+        out.append(float(s2))
+        s2sum = np.float32(s2sum - s2)
+    return out
+
+
+def check_smooth_wrap(filt):
+    """4x3 map, used texels (0,1) = P and (3,1) = Q with every edge flag; with wrap each is
+    the other's side neighbour across the border, so each pass mixes them with the side
+    weight exp(-1/(2 s2)); without wrap they have no neighbour and stay."""
+    m = _map(4, 3)
+    p, q = np.array([1.0, 0.0, 0.5]), np.array([0.0, 1.0, 0.25])
+    m[1, 0, :3], m[1, 3, :3] = p, q
+    m[1, 0, 3] = m[1, 3, 3] = EDGE_ONLY
+    out, _ = filt(m, sigma=1.0, spread=0.0, bg=BG, wrap=True)
+    a, b = p.copy(), q.copy()
+    for s2 in _passes(1.0):
+        k = np.exp(-0.5 / s2)
+        a, b = (a + k * b) / (1 + k), (b + k * a) / (1 + k)
+    err = max(np.max(np.abs(out[1, 0, :3] - a)), np.max(np.abs(out[1, 3, :3] - b)))
+    plain, _ = filt(m, sigma=1.0, spread=0.0, bg=BG, wrap=False)
+    err = max(err, np.max(np.abs(plain[1, 0, :3] - p)), np.max(np.abs(plain[1, 3, :3] - q)))
+    return float(err)
+
+
+def check_grow_bilinear_wrap(filt):
+    """4x3 map, one used texel (3,1) with every flag.  lightmapGrowForBilinearInterpolation
+    with wrap fills its 8 neighbours, columns 2 and 0 (across the border), with its
+    colour; column 1 stays empty (texels filled in the pass count for nothing)."""
+    m = _map(4, 3)
+    q = np.float32([0.25, 0.5, 0.75])
+    m[1, 3, :3] = q
+    m[1, 3, 3] = 1.0
+    _, wr = filt(m, sigma=0.0, spread=0.0, bg=BG, wrap=True)
+    _, nw = filt(m, sigma=0.0, spread=0.0, bg=BG, wrap=False)
+    for y in range(3):
+        for x in range(4):
+            want_w = q if x in (0, 2, 3) else np.float32(BG[:3])
+            want_n = q if x in (2, 3) else np.float32(BG[:3])
+            if not (np.array_equal(wr[y, x], want_w) and np.array_equal(nw[y, x], want_n)):
+                return 1.0
+    return 0.0
+
+
+# ---- seams -----------------------------------------------------------------------------
+
+def _seam(e1a, e1b, e2a, e2b, n=8):
+    return np.float32([e1a[0] / n, e1a[1] / n, e1b[0] / n, e1b[1] / n, e2a[0] / n, e2a[1] / n, e2b[0] / n, e2b[1] / n])
+
+
+def check_seam_columns(filt):
+    """8x8 map, every texel used with no blur neighbour.  A seam runs down column 1 (P) in
+    one unwrap and column 6 (Q) in the other, P + Q = (1, 1, 1): every texel of both
+    columns becomes the mean of the two sides, 0.5, and stays there; the rest is
+    untouched."""
+    m = _map(8, 8)
+    m[:, :, :3] = np.float32([0.1, 0.2, 0.3])
+    m[:, :, 3] = QUAD_ONLY
+    m[:, 1, :3] = np.float32([0.25, 0.5, 1.0])
+    m[:, 6, :3] = np.float32([0.75, 0.5, 0.0])
+    seams = np.stack([_seam((1.5, 0.25), (1.5, 7.75), (6.5, 0.25), (6.5, 7.75))])
+    _, rgbf = filt(m, sigma=1.0, spread=0.0, bg=BG, seams=seams)
+    want = m[:, :, :3].copy()
+    want[:, 1] = want[:, 6] = 0.5
+    return 0.0 if np.array_equal(rgbf, want) else 1.0
+
+
+def check_seam_order(filt):
+    """Texels a = (1,1), b = (5,1), c = (1,5) with values 1, 0, 0; seam A joins a and b,
+    then seam B joins a and c, each inside single texels.  Per pass A sets a, b to their
+    mean and B then a, c to theirs: after the three passes of sigma 1, a = c = 0.328125
+    and b = 0.34375 (in the other order a = b = 0.328125, c = 0.34375)."""
+    m = _map(8, 8)
+    m[:, :, :3] = 0.3
+    m[:, :, 3] = QUAD_ONLY
+    m[1, 1, :3], m[1, 5, :3], m[5, 1, :3] = 1.0, 0.0, 0.0
+    seams = np.stack([_seam((1.25, 1.5), (1.75, 1.5), (5.25, 1.5), (5.75, 1.5)),
+                      _seam((1.25, 1.5), (1.75, 1.5), (1.25, 5.5), (1.75, 5.5))])
+    _, rgbf = filt(m, sigma=1.0, spread=0.0, bg=BG, seams=seams)
+    ok = (np.all(rgbf[1, 1] == 0.328125) and np.all(rgbf[5, 1] == 0.328125) and np.all(rgbf[1, 5] == 0.34375))
+    return 0.0 if ok else 1.0
+
+
+def check_seam_skips(filt):
+    """A seam from v = -0.5 to 0.5 down columns 2 (P) and 5 (Q) of an 8x8 map whose texel
+    (5,1) is unused: rows 0, 2, 3 and 4 (v = 0.5 at the last point) of both columns
+    become the mean; row 1 is skipped (P stays, (5,1) gets the background), rows 5-7 are
+    off the seam."""
+    m = _map(8, 8)
+    m[:, :, :3] = 0.3
+    m[:, :, 3] = Y0_ONLY
+    m[:, 2, :3] = np.float32([0.25, 0.5, 1.0])
+    m[:, 5, :3] = np.float32([0.75, 0.5, 0.0])
+    m[1, 5] = 0.0
+    seams = np.stack([_seam((2.5, -4.0), (2.5, 4.0), (5.5, -4.0), (5.5, 4.0))])
+    _, rgbf = filt(m, sigma=1.0, spread=0.0, bg=BG, seams=seams)
+    want = m[:, :, :3].copy()
+    want[1, 5] = BG[:3]
+    for r in (0, 2, 3, 4):
+        want[r, 2] = want[r, 5] = 0.5
+    return 0.0 if np.array_equal(rgbf, want) else 1.0
+
+
+CHECKS = [check_grow_wrap_flat, check_grow_wrap_unsigned, check_smooth_wrap, check_grow_bilinear_wrap,
+          check_seam_columns, check_seam_order, check_seam_skips]
+
+
+@pytest.mark.parametrize("check", CHECKS, ids=[c.__name__ for c in CHECKS])
+def test_filter_closed_forms(oracle, check):
+    assert check(oracle.filter_lightmap_ex) < 2e-6
+
+
+# (substring of the oracle, planted replacement, check that must fail)
+MUTATIONS = [
+    ("src[(i - 1u) % size]", "src[(i + size - 1u) % size]", check_grow_wrap_unsigned),
+    ("src[(i - 1u - w) % size]", "src[(i - 1u + w) % size]", check_grow_wrap_flat),
+    ("if (wrap) { x = (int)(((uint32_t)x + w) % w); y = (int)(((uint32_t)y + h) % h); }\n                            if (x >= 0",
+     "if (x >= 0", check_smooth_wrap),
+    ("if (wrap) { x = (int)(((uint32_t)x + w) % w); y = (int)(((uint32_t)y + h) % h); }\n                    else if",
+     "if", check_grow_bilinear_wrap),
+    ("const v4 both = add4(dst[i1], dst[i2]);", "const v4 both = add4(dst[i1], dst[i1]);", check_seam_columns),
+    ("acc2 = add4(acc2, both);", "acc2 = add4(acc2, dst[i2]);", check_seam_columns),
+    ("const float* e = seams + 8 * (size_t)s;", "const float* e = seams + 8 * (size_t)(nseams - 1 - s);", check_seam_order),
+    ("if (!(dst[i1].v[3] != 0 && dst[i2].v[3] != 0)) continue;", "", check_seam_skips),
+]
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="needs gcc to build the planted copies")
+@pytest.mark.parametrize("k", range(len(MUTATIONS)))
+def test_planted_mistakes_fail(oracle, tmp_path, k):
+    old, new, check = MUTATIONS[k]
+    src = open(FILTERS_C).read()
+    assert src.count(old) == 1, "the planted site moved: " + old
+    c = tmp_path / "mutant.c"
+    c.write_text(src.replace(old, new))
+    so = tmp_path / "mutant.so"
+    subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-std=c11", "-o", str(so), str(c), "-lm"],
+                   check=True)
+    lib = ctypes.CDLL(str(so))
+    err = check(lambda img, **kw: oracle.filter_lightmap_ex(img, library=lib, **kw))
+    assert err > 1e-3, f"planted mistake survived: {new!r}"
+
+
+# ---- seam search -----------------------------------------------------------------------
+
+def _lib_seams(v, i, n, uv):
+    from lightsprintsdk_amd import bake as B
+    return B.unwrap_seams(v, i, n, uv)
+
+
+def _quad_two_charts():
+    v = np.float32([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]])
+    i = np.uint32([[0, 1, 2], [0, 2, 3]])
+    n = np.tile(np.float32([0, 0, 1]), (2, 3))
+    uv = np.float32([[0.1, 0.1, 0.4, 0.1, 0.4, 0.4], [0.6, 0.6, 0.9, 0.9, 0.6, 0.9]])
+    return v, i, n, uv
+
+
+def test_seams_quad_in_two_charts():
+    """The diagonal of a quad whose two triangles lie in different charts: one seam, the
+    first triangle's edge from corner 2 to corner 0 and the same two vertices' uvs in the
+    second triangle."""
+    s = _lib_seams(*_quad_two_charts())
+    np.testing.assert_array_equal(s, np.float32([[0.4, 0.4, 0.1, 0.1, 0.9, 0.9, 0.6, 0.6]]))
+
+
+def test_seams_continuous_unwrap_and_cube_faces():
+    """A quad with one continuous unwrap has no seam; nor does a cube whose faces have
+    their own vertices (normals 90 degrees apart are never stitched)."""
+    v, i, n, uv = _quad_two_charts()
+    uv = np.float32([[0.1, 0.1, 0.4, 0.1, 0.4, 0.4], [0.1, 0.1, 0.4, 0.4, 0.1, 0.4]])
+    assert len(_lib_seams(v, i, n, uv)) == 0
+    verts, idx, nrm, uvs = [], [], [], []
+    for axis in range(3):
+        for sign in (-1.0, 1.0):
+            nv = np.zeros(3, np.float32)
+            nv[axis] = sign
+            a, b = (axis + 1) % 3, (axis + 2) % 3
+            base = len(verts)
+            for du, dv in ((0, 0), (1, 0), (1, 1), (0, 1)):
+                p = np.zeros(3, np.float32)
+                p[axis] = sign
+                p[a], p[b] = 2 * du - 1, 2 * dv - 1
+                verts.append(p)
+            idx += [[base, base + 1, base + 2], [base, base + 2, base + 3]]
+            nrm += [np.tile(nv, 3)] * 2
+            k = len(uvs) // 2
+            uvs += [[0.1 * k, 0.0, 0.1 * k + 0.05, 0.0, 0.1 * k + 0.05, 0.05],
+                    [0.1 * k, 0.0, 0.1 * k + 0.05, 0.05, 0.1 * k, 0.05]]
+    assert len(_lib_seams(np.float32(verts), np.uint32(idx), np.float32(nrm), np.float32(uvs))) == 0
+
+
+def _cylinder(segments=6, bands=2):
+    """Side of a cylinder unwrapped around: the column of vertices at u = 1 repeats the
+    positions and normals of the column at u = 0."""
+    v, n, uvv = [], [], []
+    for j in range(bands + 1):
+        for s in range(segments + 1):
+            a = 2 * np.pi * (s % segments) / segments
+            v.append([np.cos(a), np.sin(a), j])
+            n.append([np.cos(a), np.sin(a), 0.0])
+            uvv.append([s / segments, j / bands])
+    v, n, uvv = np.float32(v), np.float32(n), np.float32(uvv)
+    idx = []
+    row = segments + 1
+    for j in range(bands):
+        for s in range(segments):
+            a, b, c, d = j * row + s, j * row + s + 1, (j + 1) * row + s + 1, (j + 1) * row + s
+            idx += [[a, b, c], [a, c, d]]
+    idx = np.uint32(idx)
+    return v, idx, n[idx].reshape(-1, 9), uvv[idx].reshape(-1, 6)
+
+
+def test_seams_cylinder_wrap_line():
+    """A cylinder's wrap line: one seam per band, u = 0 on one side and u = 1 on the other,
+    at the band's two heights."""
+    s = _lib_seams(*_cylinder(6, 2))
+    assert len(s) == 2
+    got = set()
+    for e in s:
+        us = {(float(e[0]), float(e[2])), (float(e[4]), float(e[6]))}
+        assert us == {(0.0, 0.0), (1.0, 1.0)}, e
+        assert {float(e[1]), float(e[3])} == {float(e[5]), float(e[7])}
+        got.add(tuple(sorted((float(e[1]), float(e[3])))))
+    assert got == {(0.0, 0.5), (0.5, 1.0)}
+
+
+@pytest.mark.parametrize("degrees,count", [(0.5, 1), (2.0, 0)])
+def test_seams_stitch_within_one_degree(degrees, count):
+    """Two triangles with their own vertices meeting along an edge, in different charts:
+    the copies are stitched (and the edge is a seam) only while their normals are within
+    one degree."""
+    t = np.radians(degrees)
+    v = np.float32([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 0, 0], [1, 1, 0], [0, 1, 0]])
+    i = np.uint32([[0, 1, 2], [3, 4, 5]])
+    n = np.float32([[0, 0, 1] * 3, [0, np.sin(t), np.cos(t)] * 3])
+    uv = np.float32([[0.1, 0.1, 0.4, 0.1, 0.4, 0.4], [0.6, 0.6, 0.9, 0.9, 0.6, 0.9]])
+    assert len(_lib_seams(v, i, n, uv)) == count
+
+
+def _meshes():
+    from lightsprintsdk_amd import scenes as S
+    out = [("quad", *_quad_two_charts()), ("cylinder", *_cylinder(16, 5))]
+    sc = S.object_field(seed=11, n_objects=12, target_tris=30_000, lightmap_range=(32, 128))
+    for k in range(min(6, len(sc.objects))):
+        ob = sc.objects[k]
+        tri = np.ascontiguousarray(sc.idx[ob.tri_begin:ob.tri_begin + ob.tri_count])
+        used, inv = np.unique(tri, return_inverse=True)
+        out.append((f"field{k}", sc.verts[used], inv.reshape(-1, 3).astype(np.uint32), sc.normals[tri].reshape(-1, 9),
+                    sc.tri_uv[ob.tri_begin:ob.tri_begin + ob.tri_count]))
+    rng = np.random.default_rng(5)
+    v, i, n, uv = _cylinder(8, 3)
+    perm = rng.permutation(len(i))            # triangle order decides the seams' order
+    out.append(("cylinder_shuffled", v, i[perm], n[perm], uv[perm]))
+    return out
+
+
+def test_library_seams_equal_oracle_search():
+    """lsg_unwrap_seams equals the oracle's search over the library's own stitching, seam
+    for seam in order."""
+    import oracle_py as O
+    from lightsprintsdk_amd import bake as B
+    total = 0
+    for name, v, i, n, uv in _meshes():
+        group, _ = B.stitch_vertices(v, i, n, 0.0, 1 * (np.float32(np.pi) / np.float32(180)))
+        ref = O.unwrap_seams_from_corners(group[i.reshape(-1)], int(group[group != 0xFFFFFFFF].max()) + 1, uv)
+        got = _lib_seams(v, i, n, uv)
+        np.testing.assert_array_equal(got, ref, err_msg=name)
+        total += len(got)
+    assert total >= 9                       # the quad's and the cylinders' (the field's objects have none)
+
+
+def test_library_seams_equal_search_over_reference_stitching():
+    """The same against the oracle's search over the reference's own stitched mesh
+    (createOptimizedVertices(0, RR_DEG2RAD(1), 0, NULL), compiled from its sources)."""
+    import oracle_py as O
+    if not O.have_stitch_ref():
+        pytest.skip("oracle/_ref/stitch_ref is built only where the reference sources exist")
+    angle = 1 * (np.float32(np.pi) / np.float32(180))
+    for name, v, i, n, uv in _meshes():
+        corners, pos = O.reference_stitch(v, i, n, 0.0, float(angle))
+        ref = O.unwrap_seams_from_corners(corners, len(pos), uv)
+        np.testing.assert_array_equal(_lib_seams(v, i, n, uv), ref, err_msg=name)
