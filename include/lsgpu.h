@@ -329,6 +329,18 @@ int  lsg_scene_set_point_materials(lsg_scene* scene, const uint32_t* tri_materia
 #define LSG_ENV_CUBE      1   /* BT_CUBE_TEXTURE: 6 faces of width x width, +x,-x,+y,-y,+z,-z */
 #define LSG_ENV_PANORAMA  2   /* BT_2D_TEXTURE: 360 x 180 degree panorama, width x height */
 int  lsg_scene_set_environment(lsg_scene* scene, int32_t type, uint32_t width, uint32_t height, const float* rgb);
+/* The solver's second environment, both environments' rotations and the blend factor
+ * (RRSolver::setEnvironment(env0, env1, angleRad0, angleRad1), setEnvironmentBlendFactor).
+ * The gather samples RRBuffer::createEnvironmentBlend of them (pathtracer.cpp:29-34,
+ * RRBufferBlend.cpp:21-66): a ray that leaves the scene in direction d returns
+ *   env0(R(angle0) d) * (1 - blend) + env1(R(angle1) d) * blend,
+ * R(a) = RRMatrix3x4::rotationByYawPitchRoll((a, 0, 0)), an absent environment giving 0;
+ * with no second environment and angle0 == 0 it returns env0(d) whatever the blend
+ * factor.  The first environment (lsg_scene_set_environment) alone decides whether the
+ * hemisphere is gathered (gather.cpp:149, 197).  type1 LSG_ENV_NONE: no second
+ * environment (rgb1 may be NULL). */
+int  lsg_scene_set_environment_blend(lsg_scene* scene, int32_t type1, uint32_t width1, uint32_t height1, const float* rgb1,
+                                     float angle0, float angle1, float blend_factor);
 
 /* The per-texel final gather, processTexel (RRSolver/gather.cpp:600-836) for every
  * texel at once.  texels/subs are device arrays; out_rgba is a device float4 array

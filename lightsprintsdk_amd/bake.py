@@ -174,6 +174,31 @@ class GatherScene:
                 "set_environment")
         self.env = dict(type=kind, width=width, height=height, rgb=rgb)
 
+    def _oracle_env(self):
+        """The environment as the oracle takes it (with the blend's second environment)."""
+        env = getattr(self, "env", None)
+        blend = getattr(self, "env_blend", None)
+        if blend is None:
+            return env
+        env = dict(env) if env else dict(type=0, width=0, height=0, rgb=np.zeros((0, 3), np.float32))
+        env["blend"] = blend
+        return env
+
+    def set_environment_blend(self, kind1=0, width1=0, height1=0, rgb1=None, angle0=0.0, angle1=0.0, blend=0.0):
+        """RRSolver::setEnvironment(env0, env1, angle0, angle1) with setEnvironmentBlendFactor(blend):
+        the second environment (kind1 ENV_NONE: none), the yaw of each and the blend factor;
+        rays that leave the scene see env0 rotated * (1 - blend) + env1 rotated * blend
+        (lsg_scene_set_environment_blend)."""
+        env1 = None
+        ptr = None
+        if kind1:
+            rgb1 = np.ascontiguousarray(rgb1, np.float32).reshape(-1, 3)
+            env1 = dict(type=kind1, width=width1, height=height1, rgb=rgb1)
+            ptr = rgb1.ctypes.data_as(ctypes.c_void_p)
+        N.check(N.lib().lsg_scene_set_environment_blend(self._h, kind1, width1, height1, ptr, float(angle0), float(angle1),
+                                                        float(blend)), "set_environment_blend")
+        self.env_blend = dict(env1=env1, angle0=float(angle0), angle1=float(angle1), blend=float(blend))
+
     def set_tangents(self, tangents=None, bitangents=None):
         """Per-vertex tangent frames for directional lightmaps (RRMesh::TangentBasis);
         None = each vertex's frame built from its normal (buildBasisFromNormal)."""
@@ -274,7 +299,7 @@ class GatherScene:
     def oracle_arrays(self, exitance):
         return dict(verts=self.verts, idx=self.idx, normals=self.normals, side_bits=self.side_bits,
                     tangents=getattr(self, "tangents", None), bitangents=getattr(self, "bitangents", None),
-                    env=getattr(self, "env", None), ind=getattr(self, "ind", None), pm=getattr(self, "pm", None),
+                    env=self._oracle_env(), ind=getattr(self, "ind", None), pm=getattr(self, "pm", None),
                     exitance=np.ascontiguousarray(exitance, np.float32).reshape(len(self.idx), 8),
                     lights=np.array([(l["type"], l["position"], l["direction"], l["color"], l["cast_shadows"])
                                      for l in self.lights], dtype=[("type", "<i4"), ("pos", "<f4", 3), ("dir", "<f4", 3),

@@ -332,8 +332,8 @@ int lsg_bake_run(lsg_bake* b, const lsg_gather_params* first, const lsg_gather_p
         // the captured kernels hold the environment's buffer and shape as arguments
         const SceneImpl& S = B.scene->impl;
         // (and the indirect records, point materials and unwrap seams likewise)
-        const uintptr_t env[8] = {(uintptr_t)S.d_env, S.env_type, S.env_w, S.env_h, (uintptr_t)S.d_ind, (uintptr_t)S.d_tri_mat,
-                                  (uintptr_t)S.pm_version, (uintptr_t)B.seams.version};
+        const uintptr_t env[9] = {(uintptr_t)S.d_env, S.env_type, S.env_w, S.env_h, (uintptr_t)S.d_ind, (uintptr_t)S.d_tri_mat,
+                                  (uintptr_t)S.pm_version, (uintptr_t)B.seams.version, (uintptr_t)S.env_version};
         const uint8_t* e8 = (const uint8_t*)env;
         key.insert(key.end(), e8, e8 + sizeof env);
     }

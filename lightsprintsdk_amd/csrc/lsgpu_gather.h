@@ -6,6 +6,9 @@
 
 namespace lsg {
 
+// SceneView::env_type of a blend of two environments (RRBufferBlend)
+constexpr uint32_t kEnvBlend = 3;
+
 constexpr int kMaxLights = 16;
 
 struct SceneView {
@@ -18,7 +21,14 @@ struct SceneView {
     const float* vtangent;       // 3 floats per vertex (output layers only)
     const float* vbitangent;     // 3 floats per vertex (output layers only)
     const float4* env;           // environment texels, linear rgb, or null
-    uint32_t env_type, env_w, env_h;   // LSG_ENV_*; env_type 0 = rays that leave return 0
+    uint32_t env_type, env_w, env_h;   // LSG_ENV_* or kEnvBlend; env_type 0 = rays that leave return 0
+    // kEnvBlend: env/env_w/env_h are the first environment (of type env0_type, 0 = none),
+    // env1... the second; each is looked up in its direction rotated by its 3x3 matrix
+    // (row-major, env_rot[0..8] and [9..17]) and weighted 1 - env_blend and env_blend
+    const float4* env1;
+    uint32_t env0_type, env1_type, env1_w, env1_h;
+    float env_rot[18];
+    float env_blend;
     const float4* ind;           // 3 per triangle: corner k = (indirect irradiance rgb, albedo channel k), or null
     const uint32_t* slot;        // leaf-order slot of each triangle (hit (u, v) recomputed from its record)
     PMView pm;                   // point materials (pm.tri_mat null = none)
@@ -66,6 +76,12 @@ struct SceneImpl {
     float* d_vbit = nullptr;
     float4* d_env = nullptr;           // environment texels, linear rgb
     uint32_t env_type = 0, env_w = 0, env_h = 0;
+    float4* d_env1 = nullptr;          // the second environment (lsg_scene_set_environment_blend)
+    uint32_t env1_type = 0, env1_w = 0, env1_h = 0;
+    bool env_blended = false;          // RRBuffer::createEnvironmentBlend made a blend
+    float env_rot[18] = {};            // yaw rotations of the two environments
+    float env_blend = 0;
+    uint32_t env_version = 0;          // counts environment changes (captured graphs hold them)
     lsg_lightmap* d_lmaps = nullptr;   // lightmap list of the last learned-feature pass
     uint32_t lmaps_cap = 0;
     std::vector<lsg_lightmap> h_lmaps; // host copy it was uploaded from

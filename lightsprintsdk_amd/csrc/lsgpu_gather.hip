@@ -31,6 +31,7 @@
 #include "lsgpu_trace.h"
 #include "lsgpu_gather.h"
 #include <atomic>
+#include <cmath>
 #include <vector>
 #include <cstring>
 
@@ -487,11 +488,10 @@ __device__ inline float asin_as(float x)
 //   -y: sc=+x tc=-z   +z: sc=+x tc=-y   -z: sc=-x tc=-y.
 // Panoramas map the yaw of (x, z) to columns and asin(y) to rows, in float as the
 // reference, with asin_as for asin.
-__device__ inline float4 env_at(const SceneView& S, const float3& d)
+__device__ inline float4 env_texel(const float4* env, uint32_t type, uint32_t w, uint32_t h, const float3& d)
 {
     uint32_t idx;
-    const uint32_t w = S.env_w, h = S.env_h;
-    if (S.env_type == LSG_ENV_CUBE) {
+    if (type == LSG_ENV_CUBE) {
         const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
         const uint32_t axis = (ax >= ay && ax >= az) ? 0u : ((ay >= ax && ay >= az) ? 1u : 2u);
         const float m = axis == 0 ? d.x : (axis == 1 ? d.y : d.z);
@@ -526,7 +526,28 @@ __device__ inline float4 env_at(const SceneView& S, const float3& d)
         const uint32_t c1 = (uint32_t)fmaxf(0.f, (el * (1.0f / pi) + 0.5f) * (float)h) % h;
         idx = c0 + c1 * w;
     }
-    return S.env[idx];
+    return env[idx];
+}
+
+// RRMatrix3x4::transformDirection (RRMath.cpp:212-219) by a row-major 3x3
+__device__ inline float3 rotate3(const float* m, const float3& a)
+{
+    return make_float3(a.x * m[0] + a.y * m[1] + a.z * m[2], a.x * m[3] + a.y * m[4] + a.z * m[5],
+                       a.x * m[6] + a.y * m[7] + a.z * m[8]);
+}
+
+// What a ray that leaves the scene in direction d sees (before the multiplier): the
+// environment's texel, or for a blend (RRBufferBlend::getElementAtDirection,
+// RRBufferBlend.cpp:49-66) color0 * (1 - blend) + color1 * blend of the two environments
+// looked up in their rotated directions, an absent one giving 0
+__device__ inline float4 env_at(const SceneView& S, const float3& d)
+{
+    if (S.env_type != kEnvBlend) return env_texel(S.env, S.env_type, S.env_w, S.env_h, d);
+    float4 c0 = make_float4(0.f, 0.f, 0.f, 0.f), c1 = c0;
+    if (S.env0_type) c0 = env_texel(S.env, S.env0_type, S.env_w, S.env_h, rotate3(S.env_rot, d));
+    if (S.env1_type) c1 = env_texel(S.env1, S.env1_type, S.env1_w, S.env1_h, rotate3(S.env_rot + 9, d));
+    const float k0 = 1 - S.env_blend, k1 = S.env_blend;
+    return make_float4(c0.x * k0 + c1.x * k1, c0.y * k0 + c1.y * k1, c0.z * k0 + c1.z * k1, c0.w * k0 + c1.w * k1);
 }
 
 // g_lightmapDirections[i] (gather.cpp:43-49) in the skewed frame (t, b, n), normalised
@@ -1553,10 +1574,18 @@ int gather_args(lsg_scene* s, const lsg_texel* texels, uint32_t ntexels, const l
     A.sc.verts = C.d_verts; A.sc.idx = C.d_idx;
     A.sc.vnormal = s->impl.d_vnormal; A.sc.side = s->impl.d_side; A.sc.exitance = s->impl.d_exit;
     A.sc.vtangent = s->impl.d_vtan; A.sc.vbitangent = s->impl.d_vbit;
-    // the environment is sampled only with a nonzero multiplier (gather.cpp:149)
-    const bool env = s->impl.env_type != LSG_ENV_NONE && p->environment_multiplier != 0;
-    A.sc.env = s->impl.d_env; A.sc.env_type = env ? s->impl.env_type : 0u;
-    A.sc.env_w = s->impl.env_w; A.sc.env_h = s->impl.env_h;
+    // the environment is sampled only with a nonzero multiplier (gather.cpp:149); the first
+    // environment decides the hemisphere gather (gather.cpp:149, 197), the blend of both
+    // what a ray that leaves sees (pathtracer.cpp:29-34)
+    const SceneImpl& SI = s->impl;
+    const bool env = SI.env_type != LSG_ENV_NONE && p->environment_multiplier != 0;
+    const bool blend = SI.env_blended && (SI.env_type != LSG_ENV_NONE || SI.env1_type != LSG_ENV_NONE) && p->environment_multiplier != 0;
+    A.sc.env = SI.d_env; A.sc.env_type = blend ? kEnvBlend : (env ? SI.env_type : 0u);
+    A.sc.env_w = SI.env_w; A.sc.env_h = SI.env_h;
+    A.sc.env1 = SI.d_env1; A.sc.env0_type = SI.env_type; A.sc.env1_type = SI.env1_type;
+    A.sc.env1_w = SI.env1_w; A.sc.env1_h = SI.env1_h;
+    for (int i = 0; i < 18; i++) A.sc.env_rot[i] = SI.env_rot[i];
+    A.sc.env_blend = SI.env_blend;
     A.env_mult = p->environment_multiplier;
     for (int i = 0; i < 5; i++) A.layer[i] = nullptr;
     A.dirs = 0; A.layered = 0;
@@ -1861,7 +1890,7 @@ void lsg_scene_destroy(lsg_scene* s)
 {
     if (!s) return;
     dfree(s->impl.d_vnormal); dfree(s->impl.d_side); dfree(s->impl.d_exit); dfree(s->impl.d_lights);
-    dfree(s->impl.d_vtan); dfree(s->impl.d_vbit); dfree(s->impl.d_env); dfree(s->impl.d_lmaps);
+    dfree(s->impl.d_vtan); dfree(s->impl.d_vbit); dfree(s->impl.d_env); dfree(s->impl.d_env1); dfree(s->impl.d_lmaps);
     dfree(s->impl.d_ind); dfree(s->impl.d_walk);
     free_point_materials(s->impl);
     delete s;
@@ -1873,6 +1902,7 @@ int lsg_scene_set_environment(lsg_scene* s, int32_t type, uint32_t width, uint32
     SceneImpl& S = s->impl;
     if (type == LSG_ENV_NONE) {
         dfree(S.d_env); S.d_env = nullptr; S.env_type = 0; S.env_w = S.env_h = 0;
+        S.env_version++;
         return LSG_OK;
     }
     if (type != LSG_ENV_CUBE && type != LSG_ENV_PANORAMA) return fail(LSG_ERR_INVALID, "lsg_scene_set_environment: type %d", type);
@@ -1888,6 +1918,52 @@ int lsg_scene_set_environment(lsg_scene* s, int32_t type, uint32_t width, uint32
     if (e != hipSuccess) { dfree(d); return hip_fail(e, "h2d", __LINE__); }
     dfree(S.d_env);
     S.d_env = d; S.env_type = (uint32_t)type; S.env_w = width; S.env_h = height;
+    S.env_version++;
+    return LSG_OK;
+}
+
+// RRMatrix3x4::rotationByYawPitchRoll((yaw, 0, 0)) (RRMath.cpp:457-470): the reference's
+// double expressions, rounded to float entry by entry
+static void yaw_rotation(float yaw, float* m)
+{
+    const double pitch = 0, roll = 0;
+    const double c1 = cos((double)yaw), c2 = cos(pitch), c3 = cos(roll);
+    const double s1 = sin((double)yaw), s2 = sin(pitch), s3 = sin(roll);
+    m[0] = (float)(c1 * c3 + s1 * s2 * s3); m[1] = (float)(c3 * s1 * s2 - c1 * s3); m[2] = (float)(c2 * s1);
+    m[3] = (float)(c2 * s3); m[4] = (float)(c2 * c3); m[5] = (float)(-s2);
+    m[6] = (float)(c1 * s2 * s3 - c3 * s1); m[7] = (float)(s1 * s3 + c1 * c3 * s2); m[8] = (float)(c1 * c2);
+}
+
+int lsg_scene_set_environment_blend(lsg_scene* s, int32_t type1, uint32_t width1, uint32_t height1, const float* rgb1,
+                                    float angle0, float angle1, float blend_factor)
+{
+    if (!s) return fail(LSG_ERR_INVALID, "lsg_scene_set_environment_blend: null scene");
+    if (!std::isfinite(angle0) || !std::isfinite(angle1) || !std::isfinite(blend_factor))
+        return fail(LSG_ERR_INVALID, "lsg_scene_set_environment_blend: angles and blend factor must be finite");
+    SceneImpl& S = s->impl;
+    float4* d = nullptr;
+    if (type1 != LSG_ENV_NONE) {
+        if (type1 != LSG_ENV_CUBE && type1 != LSG_ENV_PANORAMA) return fail(LSG_ERR_INVALID, "lsg_scene_set_environment_blend: type %d", type1);
+        if (!rgb1 || !width1 || !height1) return fail(LSG_ERR_INVALID, "lsg_scene_set_environment_blend: empty environment");
+        if (type1 == LSG_ENV_CUBE && width1 != height1) return fail(LSG_ERR_INVALID, "lsg_scene_set_environment_blend: cube faces must be square");
+        const uint64_t n = (uint64_t)width1 * height1 * (type1 == LSG_ENV_CUBE ? 6u : 1u);
+        if (n > (1ull << 28)) return fail(LSG_ERR_LIMIT, "lsg_scene_set_environment_blend: %llu texels", (unsigned long long)n);
+        std::vector<float4> t((size_t)n);
+        for (size_t i = 0; i < (size_t)n; i++) t[i] = make_float4(rgb1[3 * i], rgb1[3 * i + 1], rgb1[3 * i + 2], 0.f);
+        LSG_HIP(hipMalloc(&d, 16 * (size_t)n));
+        hipError_t e = hipMemcpy(d, t.data(), 16 * (size_t)n, hipMemcpyHostToDevice);
+        if (e != hipSuccess) { dfree(d); return hip_fail(e, "h2d", __LINE__); }
+    }
+    dfree(S.d_env1);
+    S.d_env1 = d;
+    S.env1_type = d ? (uint32_t)type1 : 0u; S.env1_w = d ? width1 : 0u; S.env1_h = d ? height1 : 0u;
+    // RRBuffer::createEnvironmentBlend (RRBufferBlend.cpp:21-30): no blend without a second
+    // environment and a rotation of the first
+    S.env_blended = d != nullptr || angle0 != 0;
+    yaw_rotation(angle0, S.env_rot);
+    yaw_rotation(angle1, S.env_rot + 9);
+    S.env_blend = blend_factor;
+    S.env_version++;
     return LSG_OK;
 }
 

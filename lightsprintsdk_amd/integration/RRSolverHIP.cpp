@@ -360,39 +360,60 @@ bool RRSolverHIP::buildScene(const UpdateParameters& params)
 		releaseScene();
 		return false;
 	}
-	// the environment that gather rays leaving the scene sample (gather.cpp:149,
-	// pathtracer.cpp:161-172): its texels in linear colour, as getElementAtDirection
-	// reads them with the solver's colour space
-	const RRBuffer* env = params.indirect.environmentMultiplier ? getEnvironment(0) : nullptr;
-	if (env)
+	// the environments that gather rays leaving the scene sample (gather.cpp:149,
+	// pathtracer.cpp:29-34, 161-172): both of the solver's, their rotations and the blend
+	// factor; texels in linear colour, as getElementAtDirection reads them with the solver's
+	// colour space
+	if (params.indirect.environmentMultiplier)
 	{
-		const bool cube = env->getType() == BT_CUBE_TEXTURE;
-		const unsigned w = env->getWidth(), h = env->getHeight();
-		const size_t n = (size_t)w * h * (cube ? 6 : 1);
-		std::vector<float> rgb(3 * n);
-		for (size_t i = 0; i < n; i++)
+		RRReal angle0 = 0, angle1 = 0;
+		const RRBuffer* env0 = getEnvironment(0, &angle0);
+		const RRBuffer* env1 = getEnvironment(1, &angle1);
+		auto texels = [&](const RRBuffer* env, std::vector<float>& rgb)
 		{
-			const RRVec4 c = env->getElement((unsigned)i, getColorSpace());
-			for (int k = 0; k < 3; k++) rgb[3 * i + k] = c[k];
-		}
-		if (lsg_scene_set_environment(scene, cube ? LSG_ENV_CUBE : LSG_ENV_PANORAMA, w, h, rgb.data()) != LSG_OK)
+			const bool cube = env->getType() == BT_CUBE_TEXTURE;
+			const size_t n = (size_t)env->getWidth() * env->getHeight() * (cube ? 6 : 1);
+			rgb.resize(3 * n);
+			for (size_t i = 0; i < n; i++)
+			{
+				const RRVec4 c = env->getElement((unsigned)i, getColorSpace());
+				for (int k = 0; k < 3; k++) rgb[3 * i + k] = c[k];
+			}
+			return cube ? LSG_ENV_CUBE : LSG_ENV_PANORAMA;
+		};
+		std::vector<float> rgb;
+		if (env0 && lsg_scene_set_environment(scene, texels(env0, rgb), env0->getWidth(), env0->getHeight(), rgb.data()) != LSG_OK)
 		{
 			RRReporter::report(ERRO, "RRSolverHIP: %s\n", lsg_last_error());
 			releaseScene();
 			return false;
 		}
+		if (env1 || angle0)
+		{
+			const int type1 = env1 ? texels(env1, rgb) : LSG_ENV_NONE;
+			if (lsg_scene_set_environment_blend(scene, type1, env1 ? env1->getWidth() : 0, env1 ? env1->getHeight() : 0,
+			                                    env1 ? rgb.data() : nullptr, angle0, angle1, getEnvironmentBlendFactor()) != LSG_OK)
+			{
+				RRReporter::report(ERRO, "RRSolverHIP: %s\n", lsg_last_error());
+				releaseScene();
+				return false;
+			}
+		}
 	}
 	return true;
 }
 
-// environments the GPU samples: none, or one cube map or panorama without rotation or
-// blending (RRBuffer::createEnvironmentBlend returns it unchanged then, RRBufferBlend.cpp:21-29)
+// environments the GPU samples: cube maps and panoramas, rotated and blended as
+// RRBuffer::createEnvironmentBlend does (RRBufferBlend.cpp:21-66)
 bool RRSolverHIP::environmentSupported(const UpdateParameters& params) const
 {
-	RRReal angle0 = 0;
-	const RRBuffer* env = getEnvironment(0, &angle0);
-	if (!env || params.indirect.environmentMultiplier == 0) return true;
-	return !getEnvironment(1) && angle0 == 0 && (env->getType() == BT_CUBE_TEXTURE || env->getType() == BT_2D_TEXTURE);
+	if (params.indirect.environmentMultiplier == 0) return true;
+	for (unsigned i = 0; i < 2; i++)
+	{
+		const RRBuffer* env = getEnvironment(i);
+		if (env && env->getType() != BT_CUBE_TEXTURE && env->getType() != BT_2D_TEXTURE) return false;
+	}
+	return true;
 }
 
 void RRSolverHIP::setStaticObjects(const RRObjects& objects, const SmoothingParameters* smoothing, const char* cacheLocation,
@@ -648,7 +669,7 @@ unsigned RRSolverHIP::updateLightmaps(int layerLightmap, int layerDirectionalLig
 	{
 		if (referenceFallback) return RRSolver::updateLightmaps(layerLightmap, layerDirectionalLightmap, layerBentNormals, params, filtering);
 		RRReporter::report(ERRO, "RRSolverHIP::updateLightmaps: unsupported request (needs quality>0, "
-		                         "no rotated or blended environment, a gfx950 device).\n");
+		                         "cube-map or panorama environments, a gfx950 device).\n");
 		return 0;
 	}
 	// indirect solution first, by the reference's own solver (RRSolver.h:731-734), unless

@@ -119,6 +119,7 @@ SIGNATURES = {
     "lsg_gather_stats": (ctypes.c_int, [P, P, U32, P, U32, ctypes.POINTER(GatherParams), P, ctypes.POINTER(ctypes.c_double)]),
     "lsg_scene_set_tangents": (ctypes.c_int, [P, P, P]),
     "lsg_scene_set_environment": (ctypes.c_int, [P, I32, U32, U32, P]),
+    "lsg_scene_set_environment_blend": (ctypes.c_int, [P, I32, U32, U32, P, ctypes.c_float, ctypes.c_float, ctypes.c_float]),
     "lsg_gather_layers_device": (ctypes.c_int, [P, P, U32, P, U32, ctypes.POINTER(GatherParams), P, P]),
     "lsg_filter_lightmaps": (ctypes.c_int, [P, U32, P, P, ctypes.POINTER(FilterParams), P]),
     "lsg_filter_plan_create": (ctypes.c_int, [P, U32, ctypes.POINTER(P)]),

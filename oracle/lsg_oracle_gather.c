@@ -62,6 +62,12 @@ typedef struct lsgo_env {
     int32_t type;
     uint32_t w, h;
     const float* rgb;
+    /* blended != 0: the solver also has env1 (NULL or type 0 = none), the two angles and
+     * the blend factor (RRSolver::setEnvironment, setEnvironmentBlendFactor), and the
+     * gather reads RRBuffer::createEnvironmentBlend of them (RRBufferBlend.cpp:21-66) */
+    const struct lsgo_env* env1;
+    float angle0, angle1, blend;
+    int32_t blended;
 } lsgo_env;
 
 typedef struct lsgo_gstats {
@@ -429,6 +435,53 @@ uint32_t lsgo_env_index(const lsgo_env* env, const float* d)
     return c0 + c1 * w;
 }
 
+/* RRMatrix3x4::rotationByYawPitchRoll((yaw, 0, 0)), RRMath.cpp:457-470: the double
+ * expressions, each entry rounded to float */
+static void yaw_matrix(float yaw, float* m)
+{
+    const double c1 = cos((double)yaw), c2 = cos(0.0), c3 = cos(0.0);
+    const double s1 = sin((double)yaw), s2 = sin(0.0), s3 = sin(0.0);
+    m[0] = (float)(c1 * c3 + s1 * s2 * s3); m[1] = (float)(c3 * s1 * s2 - c1 * s3); m[2] = (float)(c2 * s1);
+    m[3] = (float)(c2 * s3); m[4] = (float)(c2 * c3); m[5] = (float)(-s2);
+    m[6] = (float)(c1 * s2 * s3 - c3 * s1); m[7] = (float)(s1 * s3 + c1 * c3 * s2); m[8] = (float)(c1 * c2);
+}
+
+/* createEnvironmentBlend (RRBufferBlend.cpp:21-30) makes a blend unless there is no
+ * second environment and the first is not rotated */
+static int blend_on(const lsgo_env* env)
+{
+    return env->blended && ((env->env1 && env->env1->type) || env->angle0 != 0);
+}
+
+/* RRBuffer::getElementAtDirection of the environment, or of the blend
+ * (RRBufferBlend::getElementAtDirection, RRBufferBlend.cpp:49-66): this(R(angle0) d) *
+ * (1 - blend) + env1(R(angle1) d) * blend, an absent one giving 0, R(a) =
+ * RRMatrix3x4::rotationByYawPitchRoll((a, 0, 0)); rgb */
+static void env_color(const lsgo_env* env, const float* d, float* out)
+{
+    if (!blend_on(env)) {
+        const float* t = env->rgb + 3 * (size_t)lsgo_env_index(env, d);
+        for (int c = 0; c < 3; c++) out[c] = t[c];
+        return;
+    }
+    float c0[3] = {0, 0, 0}, c1[3] = {0, 0, 0};
+    const lsgo_env* e[2] = {env, env->env1};
+    const float ang[2] = {env->angle0, env->angle1};
+    float* col[2] = {c0, c1};
+    for (int k = 0; k < 2; k++) {
+        if (!e[k] || !e[k]->type) continue;
+        float m[9];
+        yaw_matrix(ang[k], m);
+        /* RRMatrix3x4::transformDirection, RRMath.cpp:212-219 */
+        const float r[3] = {d[0] * m[0] + d[1] * m[1] + d[2] * m[2], d[0] * m[3] + d[1] * m[4] + d[2] * m[5],
+                            d[0] * m[6] + d[1] * m[7] + d[2] * m[8]};
+        const float* t = e[k]->rgb + 3 * (size_t)lsgo_env_index(e[k], r);
+        for (int c = 0; c < 3; c++) col[k][c] = t[c];
+    }
+    const float k0 = 1 - env->blend, k1 = env->blend;
+    for (int c = 0; c < 3; c++) out[c] = c0[c] * k0 + c1[c] * k1;
+}
+
 /* Skewed-normalised tangent and bitangent at the subtexel centroid: the normalised
  * barycentric blend of the vertex frames (gather.cpp:731-732). */
 static void skewed_frame(const float* vt, const float* vb, const uint32_t* tidx, const float* sub, float* tg, float* bg)
@@ -506,8 +559,12 @@ static void gather_core(const float* verts, uint32_t nverts, const uint32_t* idx
                         int threads)
 {
     const int dirs = layers && (layers[1] || layers[2] || layers[3]);
-    /* the environment is sampled only with a nonzero multiplier (gather.cpp:149) */
-    const lsgo_env* env = (env_in && env_in->type && p->environment_multiplier != 0) ? env_in : NULL;
+    /* the environment is sampled only with a nonzero multiplier (gather.cpp:149); the first
+     * one decides the hemisphere gather (gather.cpp:149, 197), the blend of both what a
+     * ray that leaves sees (pathtracer.cpp:29-34) */
+    const int env0 = env_in && env_in->type && p->environment_multiplier != 0;
+    const lsgo_env* env = (env_in && p->environment_multiplier != 0 &&
+                           (env_in->type || (blend_on(env_in) && env_in->env1 && env_in->env1->type))) ? env_in : NULL;
     scene_t sc;
     sc.verts = verts; sc.idx = idx; sc.ntri = ntri; sc.side_mask = side_mask;
     sc.nodes = nodes; sc.root_ref = root_ref; sc.prim = prim; sc.nprim = nprim;
@@ -520,7 +577,7 @@ static void gather_core(const float* verts, uint32_t nverts, const uint32_t* idx
     lsgo_box_init(sc.bmn, sc.bmx);
 
     const uint32_t q = p->quality;
-    const uint32_t hemi_rays = (p->hemisphere || env) ? (q > 1 ? q : 1) : 0;        /* gather.cpp:197 */
+    const uint32_t hemi_rays = (p->hemisphere || env0) ? (q > 1 ? q : 1) : 0;       /* gather.cpp:197 */
     const uint32_t rounds = (p->light_multiplier != 0 && nlights) ? (p->light_rounds ? p->light_rounds : q / 10 + 1) : 0;  /* gather.cpp:396 */
     const uint32_t light_rays = nlights * rounds;
     double nh = 0, ns = 0, nser = 0;
@@ -626,8 +683,8 @@ static void gather_core(const float* verts, uint32_t nverts, const uint32_t* idx
                             e = hitc;
                         }
                     } else if (env) {
-                        const float* t = env->rgb + 3 * (size_t)lsgo_env_index(env, d);
-                        for (int c = 0; c < 3; c++) envc[c] = t[c] * p->environment_multiplier;
+                        env_color(env, d, envc);
+                        for (int c = 0; c < 3; c++) envc[c] = envc[c] * p->environment_multiplier;
                         e = envc;
                     }
                     if (e) {

@@ -49,7 +49,9 @@ class GParams(ctypes.Structure):
 
 
 class Env(ctypes.Structure):
-    _fields_ = [("type", ctypes.c_int32), ("w", ctypes.c_uint32), ("h", ctypes.c_uint32), ("rgb", ctypes.c_void_p)]
+    _fields_ = [("type", ctypes.c_int32), ("w", ctypes.c_uint32), ("h", ctypes.c_uint32), ("rgb", ctypes.c_void_p),
+                ("env1", ctypes.c_void_p), ("angle0", ctypes.c_float), ("angle1", ctypes.c_float), ("blend", ctypes.c_float),
+                ("blended", ctypes.c_int32)]
 
 
 ENV_CUBE, ENV_PANORAMA = 1, 2
@@ -61,7 +63,16 @@ def _env(scene_arrays):
     if not e:
         return None, None
     rgb = np.ascontiguousarray(e["rgb"], dtype=np.float32)
-    return Env(e["type"], e["width"], e["height"], rgb.ctypes.data_as(ctypes.c_void_p)), rgb
+    ev = Env(e["type"], e["width"], e["height"], rgb.ctypes.data_as(ctypes.c_void_p))
+    keep = [rgb]
+    b = e.get("blend")       # dict(env1=dict or None, angle0, angle1, blend): RRSolver's second environment
+    if b:
+        if b.get("env1"):
+            e1, k1 = _env(dict(env=b["env1"]))
+            keep += [e1, k1]
+            ev.env1 = ctypes.cast(ctypes.pointer(e1), ctypes.c_void_p)
+        ev.angle0, ev.angle1, ev.blend, ev.blended = b["angle0"], b["angle1"], b["blend"], 1
+    return ev, keep
 
 
 def env_index(env, d):
