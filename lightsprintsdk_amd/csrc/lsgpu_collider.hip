@@ -233,32 +233,29 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ key
     if (i == 0) parent_node[0] = -1;
 }
 
-// Bottom-up union of child boxes; the second thread to arrive at a node computes it.
-__global__ __launch_bounds__(256) void k_refit(int n, const int* __restrict__ first, const int* __restrict__ last, const int* __restrict__ split,
-                                               const int* __restrict__ parent_node, const int* __restrict__ parent_leaf,
-                                               const float4* __restrict__ plo, const float4* __restrict__ phi,
-                                               float4* nlo, float4* nhi, uint32_t* arrive)
+// Bottom-up union of child boxes, one tree level per launch: launch k computes the nodes
+// whose inner children were finished by earlier launches (done[c] in 1..k-1; leaf children
+// are always ready) and marks them done[i] = k.  A child finished in the same launch reads
+// as 0 or k, so a node only reads boxes written before its launch began: no fences, no
+// arrival counters (a device-scope fence per climb step writes back and invalidates a
+// whole L2).  The root, node 0, is finished at launch height + 1.
+__global__ __launch_bounds__(256) void k_refit_level(int n, const int* __restrict__ first, const int* __restrict__ last,
+                                                     const int* __restrict__ split, const float4* __restrict__ plo,
+                                                     const float4* __restrict__ phi, float4* __restrict__ nlo, float4* __restrict__ nhi,
+                                                     uint32_t* __restrict__ done, uint32_t k)
 {
-    int p = blockIdx.x * 256 + threadIdx.x;
-    if (p >= n) return;
-    int node = parent_leaf[p];
-    while (node >= 0) {
-        __threadfence();
-        uint32_t old = atomicAdd(&arrive[node], 1u);
-        if (old == 0) return;
-        __threadfence();
-        int g = split[node];
-        float4 al, ah, bl, bh;
-        if (first[node] == g) { al = plo[g]; ah = phi[g]; }
-        else { al = nlo[g]; ah = nhi[g]; }
-        if (last[node] == g + 1) { bl = plo[g + 1]; bh = phi[g + 1]; }
-        else { bl = nlo[g + 1]; bh = nhi[g + 1]; }
-        float4 lo = make_float4(fminf(al.x, bl.x), fminf(al.y, bl.y), fminf(al.z, bl.z), 0.f);
-        float4 hi = make_float4(fmaxf(ah.x, bh.x), fmaxf(ah.y, bh.y), fmaxf(ah.z, bh.z), 0.f);
-        nlo[node] = lo;
-        nhi[node] = hi;
-        node = parent_node[node];
-    }
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n - 1 || done[i]) return;
+    const int g = split[i];
+    const bool la = first[i] == g, lb = last[i] == g + 1;
+    if (!la) { const uint32_t d = done[g]; if (d == 0 || d >= k) return; }
+    if (!lb) { const uint32_t d = done[g + 1]; if (d == 0 || d >= k) return; }
+    float4 al, ah, bl, bh;
+    if (la) { al = plo[g]; ah = phi[g]; } else { al = nlo[g]; ah = nhi[g]; }
+    if (lb) { bl = plo[g + 1]; bh = phi[g + 1]; } else { bl = nlo[g + 1]; bh = nhi[g + 1]; }
+    nlo[i] = make_float4(fminf(al.x, bl.x), fminf(al.y, bl.y), fminf(al.z, bl.z), 0.f);
+    nhi[i] = make_float4(fmaxf(ah.x, bh.x), fmaxf(ah.y, bh.y), fmaxf(ah.z, bh.z), 0.f);
+    done[i] = k;
 }
 
 __global__ __launch_bounds__(256) void k_write_nodes(int n, uint32_t leafmax, float margin, const int* __restrict__ first, const int* __restrict__ last,
@@ -542,7 +539,17 @@ static int build(ColliderImpl& C, const float* verts, uint32_t nv, const uint32_
         CKH(hipMemset(d_arrive, 0, 4 * (size_t)ni));
         CKH(hipMemset(d_depth, 0, 4));
         k_karras<<<blocks(ni), 256>>>(d_keys, (int)n, d_first, d_last, d_split, d_pn, d_pl);
-        k_refit<<<blocks(n), 256>>>((int)n, d_first, d_last, d_split, d_pn, d_pl, d_plo, d_phi, d_nlo, d_nhi, d_arrive);
+        // one launch per level until the root is done (checked every 8 launches; launches
+        // after that find every node done)
+        for (uint32_t k = 1;; k++) {
+            k_refit_level<<<blocks(ni), 256>>>((int)n, d_first, d_last, d_split, d_plo, d_phi, d_nlo, d_nhi, d_arrive, k);
+            if (k % 8 == 0) {
+                uint32_t root_done = 0;
+                CKH(hipMemcpy(&root_done, d_arrive, 4, hipMemcpyDeviceToHost));
+                if (root_done) break;
+                if (k > 4096) { cleanup(); return fail(LSG_ERR_INVALID, "refit: no progress"); }
+            }
+        }
         k_write_nodes<<<blocks(ni), 256>>>((int)n, leafmax, C.margin, d_first, d_last, d_split, d_pn, d_plo, d_phi, d_nlo, d_nhi,
                                            C.d_nodes, d_depth);
         CKH(hipGetLastError());

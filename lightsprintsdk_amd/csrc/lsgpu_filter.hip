@@ -461,8 +461,7 @@ __global__ __launch_bounds__(64) void k_seam_pass(const FilterMap* __restrict__ 
                 seam_put(d, l2, a2);
             }
         }
-        __threadfence();
-        __syncthreads();
+        __syncthreads();   // a workgroup barrier orders the lanes' writes for the next seam (one CU)
     }
 }
 

@@ -10,6 +10,8 @@
 // reductions are exact (min/max over floats as order-preserving integers), the split
 // arithmetic is the host's, and nodes are numbered in preorder (left child = node + 1, right
 // child = node + leaves on the left), so the result equals the host's node for node.
+// Tasks of 2^15 leaves or more (the top levels) take the same steps across the chip:
+// many workgroups bin, one lane splits, a scan of left/right flags partitions.
 #include "lsgpu_internal.h"
 #include <vector>
 
@@ -94,6 +96,139 @@ __global__ void k_sah_root_task(const int* __restrict__ cb, uint32_t n, SahTask*
     *t = r;
 }
 
+// bins per axis of a task and the binning scale (0 on an axis whose centroids coincide)
+__device__ inline int task_bins(const SahTask& T, bool* ok, float* sc)
+{
+    const uint32_t m = T.end - T.begin;
+    const int nb = (int)min((uint32_t)kSahBins, max(4u, m));
+    for (int a = 0; a < 3; a++) {
+        ok[a] = T.chi[a] > T.clo[a];
+        sc[a] = ok[a] ? (float)nb / (T.chi[a] - T.clo[a]) : 0.f;
+    }
+    return nb;
+}
+
+__device__ inline int bin_of(const SahLeaf& f, const SahTask& T, int a, float sc, int nb)
+{
+    return min((int)((f.c[a] - T.clo[a]) * sc), nb - 1);
+}
+
+// the host's split search (lsgpu_sah.cpp) on a task's bins: dec = (axis or -1 for halves,
+// last bin on the left, leaves on the left)
+__device__ inline void sah_choose(const int (*cnt)[kSahBins], const int (*box)[kSahBins][6], int nb, const bool* ok, uint32_t m,
+                                  int* dec)
+{
+    float best = 0.f;
+    int bax = -1, bk = -1;
+    for (int a = 0; a < 3; a++) {
+        if (!ok[a]) continue;
+        float ra[kSahBins];
+        uint32_t rc[kSahBins];
+        float acc[6] = {3.0e38f, -3.0e38f, 3.0e38f, -3.0e38f, 3.0e38f, -3.0e38f};
+        uint32_t c = 0;
+        for (int k = nb - 1; k >= 0; k--) {
+            for (int e = 0; e < 6; e++) {
+                const float v = o2f(box[a][k][e]);
+                acc[e] = (e & 1) ? fmaxf(acc[e], v) : fminf(acc[e], v);
+            }
+            c += (uint32_t)cnt[a][k];
+            const float dx = acc[1] - acc[0], dy = acc[3] - acc[2], dz = acc[5] - acc[4];
+            ra[k] = c ? dx * dy + dy * dz + dz * dx : 0.f;
+            rc[k] = c;
+        }
+        for (int e = 0; e < 6; e++) acc[e] = (e & 1) ? -3.0e38f : 3.0e38f;
+        c = 0;
+        for (int k = 0; k < nb - 1; k++) {
+            for (int e = 0; e < 6; e++) {
+                const float v = o2f(box[a][k][e]);
+                acc[e] = (e & 1) ? fmaxf(acc[e], v) : fminf(acc[e], v);
+            }
+            c += (uint32_t)cnt[a][k];
+            if (!c || !rc[k + 1]) continue;
+            const float dx = acc[1] - acc[0], dy = acc[3] - acc[2], dz = acc[5] - acc[4];
+            const float cost = (dx * dy + dy * dz + dz * dx) * (float)c + ra[k + 1] * (float)rc[k + 1];
+            if (bax < 0 || cost < best) { best = cost; bax = a; bk = k; }
+        }
+    }
+    uint32_t nl = m / 2;
+    if (bax >= 0) {
+        nl = 0;
+        for (int k = 0; k <= bk; k++) nl += (uint32_t)cnt[bax][k];
+    }
+    dec[0] = bax; dec[1] = bk; dec[2] = (int)nl;
+}
+
+// a split task's node (children's boxes, a leaf child's ref from dst) and its children of
+// two or more leaves as the next level's tasks
+__device__ inline void sah_emit(const SahTask& T, uint32_t nl, const int (*child)[12], const SahLeaf* dst, Node* out, SahTask* next,
+                                uint32_t* nnext, uint32_t* depth)
+{
+    atomicMax(depth, T.depth);
+    float nodef[16];
+    for (int s = 0; s < 2; s++) {
+        const uint32_t b = s == 0 ? T.begin : T.begin + nl, e = s == 0 ? T.begin + nl : T.end;
+        const uint32_t ms = e - b;
+        const uint32_t id = s == 0 ? T.id + 1 : T.id + nl;
+        const int32_t ref = ms == 1 ? dst[b].ref : (int32_t)id;
+        for (int k = 0; k < 6; k++) nodef[6 * s + k] = o2f(child[s][k]);
+        nodef[12 + s] = __int_as_float(ref);
+        if (ms >= 2) {
+            SahTask c;
+            c.begin = b; c.end = e; c.id = id; c.depth = T.depth + 1;
+            for (int a = 0; a < 3; a++) { c.clo[a] = o2f(child[s][6 + a]); c.chi[a] = o2f(child[s][9 + a]); }
+            next[atomicAdd(nnext, 1u)] = c;
+        }
+    }
+    nodef[14] = 0.f; nodef[15] = 0.f;
+    Node nd;
+    nd.q0 = make_float4(nodef[0], nodef[1], nodef[2], nodef[3]);
+    nd.q1 = make_float4(nodef[4], nodef[5], nodef[6], nodef[7]);
+    nd.q2 = make_float4(nodef[8], nodef[9], nodef[10], nodef[11]);
+    nd.q3 = make_float4(nodef[12], nodef[13], nodef[14], nodef[15]);
+    out[T.id] = nd;
+}
+
+__device__ inline void bins_init(int* cnt, int* box, uint32_t tid, uint32_t nthreads)
+{
+    const int kLo = f2o(3.0e38f), kHi = f2o(-3.0e38f);
+    for (uint32_t k = tid; k < 3 * kSahBins; k += nthreads) {
+        cnt[k] = 0;
+        for (int e = 0; e < 6; e++) box[6 * k + e] = (e & 1) ? kHi : kLo;
+    }
+}
+
+__device__ inline void child_init(int* child, uint32_t tid)
+{
+    // per child: box lo/hi x y z (6), centroid lo x y z, hi x y z (6)
+    if (tid < 24) child[tid] = ((tid % 12) < 6 ? ((tid & 1) ? f2o(-3.0e38f) : f2o(3.0e38f)) : ((tid % 12) < 9 ? f2o(3.0e38f) : f2o(-3.0e38f)));
+}
+
+__device__ inline void bin_leaf(const SahLeaf& f, const SahTask& T, const bool* ok, const float* sc, int nb, int (*cnt)[kSahBins],
+                                int (*box)[kSahBins][6])
+{
+    for (int a = 0; a < 3; a++) {
+        if (!ok[a]) continue;
+        const int j = bin_of(f, T, a, sc[a], nb);
+        atomicAdd(&cnt[a][j], 1);
+        for (int e = 0; e < 6; e++) {
+            if (e & 1) atomicMax(&box[a][j][e], f2o(f.b[e]));
+            else atomicMin(&box[a][j][e], f2o(f.b[e]));
+        }
+    }
+}
+
+__device__ inline void child_add(int* cbx, const SahLeaf& f)
+{
+    for (int e = 0; e < 6; e++) {
+        if (e & 1) atomicMax(&cbx[e], f2o(f.b[e]));
+        else atomicMin(&cbx[e], f2o(f.b[e]));
+    }
+    for (int a = 0; a < 3; a++) {
+        atomicMin(&cbx[6 + a], f2o(f.c[a]));
+        atomicMax(&cbx[9 + a], f2o(f.c[a]));
+    }
+}
+
 // one node of a level per wave: bins, split, stable partition src -> dst, children
 __global__ __launch_bounds__(64) void k_sah_level(const SahTask* __restrict__ tasks, const SahLeaf* __restrict__ src,
                                                   SahLeaf* __restrict__ dst, Node* __restrict__ out, SahTask* __restrict__ next,
@@ -101,79 +236,20 @@ __global__ __launch_bounds__(64) void k_sah_level(const SahTask* __restrict__ ta
 {
     __shared__ int s_cnt[3][kSahBins];
     __shared__ int s_box[3][kSahBins][6];
-    __shared__ int s_child[2][12];          // per child: box lo/hi x y z (6), centroid lo x y z, hi x y z (6)
+    __shared__ int s_child[2][12];
     __shared__ int s_dec[3];                // axis (-1: halves), bin, leaves on the left
     const uint32_t lane = threadIdx.x;
     const SahTask T = tasks[blockIdx.x];
     const uint32_t m = T.end - T.begin;
-    const int nb = (int)min((uint32_t)kSahBins, max(4u, m));
     bool ok[3];
     float sc[3];
-    for (int a = 0; a < 3; a++) {
-        ok[a] = T.chi[a] > T.clo[a];
-        sc[a] = ok[a] ? (float)nb / (T.chi[a] - T.clo[a]) : 0.f;
-    }
-    const int kLo = f2o(3.0e38f), kHi = f2o(-3.0e38f);
-    for (uint32_t k = lane; k < 3 * kSahBins; k += 64) {
-        (&s_cnt[0][0])[k] = 0;
-        for (int e = 0; e < 6; e++) (&s_box[0][0][0])[6 * k + e] = (e & 1) ? kHi : kLo;
-    }
-    if (lane < 24) (&s_child[0][0])[lane] = ((lane % 12) < 6 ? ((lane & 1) ? kHi : kLo) : ((lane % 12) < 9 ? kLo : kHi));
+    const int nb = task_bins(T, ok, sc);
+    bins_init(&s_cnt[0][0], &s_box[0][0][0], lane, 64);
+    child_init(&s_child[0][0], lane);
     __syncthreads();
-    for (uint32_t k = T.begin + lane; k < T.end; k += 64) {
-        const SahLeaf f = src[k];
-        for (int a = 0; a < 3; a++) {
-            if (!ok[a]) continue;
-            const int j = min((int)((f.c[a] - T.clo[a]) * sc[a]), nb - 1);
-            atomicAdd(&s_cnt[a][j], 1);
-            for (int e = 0; e < 6; e++) {
-                if (e & 1) atomicMax(&s_box[a][j][e], f2o(f.b[e]));
-                else atomicMin(&s_box[a][j][e], f2o(f.b[e]));
-            }
-        }
-    }
+    for (uint32_t k = T.begin + lane; k < T.end; k += 64) bin_leaf(src[k], T, ok, sc, nb, s_cnt, s_box);
     __syncthreads();
-    if (lane == 0) {
-        // the host's split search (lsgpu_sah.cpp), on the same bins
-        float best = 0.f;
-        int bax = -1, bk = -1;
-        for (int a = 0; a < 3; a++) {
-            if (!ok[a]) continue;
-            float ra[kSahBins];
-            uint32_t rc[kSahBins];
-            float acc[6] = {3.0e38f, -3.0e38f, 3.0e38f, -3.0e38f, 3.0e38f, -3.0e38f};
-            uint32_t c = 0;
-            for (int k = nb - 1; k >= 0; k--) {
-                for (int e = 0; e < 6; e++) {
-                    const float v = o2f(s_box[a][k][e]);
-                    acc[e] = (e & 1) ? fmaxf(acc[e], v) : fminf(acc[e], v);
-                }
-                c += (uint32_t)s_cnt[a][k];
-                const float dx = acc[1] - acc[0], dy = acc[3] - acc[2], dz = acc[5] - acc[4];
-                ra[k] = c ? dx * dy + dy * dz + dz * dx : 0.f;
-                rc[k] = c;
-            }
-            for (int e = 0; e < 6; e++) acc[e] = (e & 1) ? -3.0e38f : 3.0e38f;
-            c = 0;
-            for (int k = 0; k < nb - 1; k++) {
-                for (int e = 0; e < 6; e++) {
-                    const float v = o2f(s_box[a][k][e]);
-                    acc[e] = (e & 1) ? fmaxf(acc[e], v) : fminf(acc[e], v);
-                }
-                c += (uint32_t)s_cnt[a][k];
-                if (!c || !rc[k + 1]) continue;
-                const float dx = acc[1] - acc[0], dy = acc[3] - acc[2], dz = acc[5] - acc[4];
-                const float cost = (dx * dy + dy * dz + dz * dx) * (float)c + ra[k + 1] * (float)rc[k + 1];
-                if (bax < 0 || cost < best) { best = cost; bax = a; bk = k; }
-            }
-        }
-        uint32_t nl = m / 2;
-        if (bax >= 0) {
-            nl = 0;
-            for (int k = 0; k <= bk; k++) nl += (uint32_t)s_cnt[bax][k];
-        }
-        s_dec[0] = bax; s_dec[1] = bk; s_dec[2] = (int)nl;
-    }
+    if (lane == 0) sah_choose(s_cnt, s_box, nb, ok, m, s_dec);
     __syncthreads();
     const int bax = s_dec[0], bk = s_dec[1];
     const uint32_t nl = (uint32_t)s_dec[2];
@@ -185,7 +261,7 @@ __global__ __launch_bounds__(64) void k_sah_level(const SahTask* __restrict__ ta
         bool left = false;
         if (valid) {
             f = src[k];
-            left = bax < 0 ? (k - T.begin < m / 2) : (min((int)((f.c[bax] - T.clo[bax]) * sc[bax]), nb - 1) <= bk);
+            left = bax < 0 ? (k - T.begin < m / 2) : (bin_of(f, T, bax, sc[bax], nb) <= bk);
         }
         const uint64_t lm = __ballot(valid && left), rm = __ballot(valid && !left);
         const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
@@ -193,81 +269,130 @@ __global__ __launch_bounds__(64) void k_sah_level(const SahTask* __restrict__ ta
             const uint32_t at = left ? T.begin + ldone + (uint32_t)__popcll(lm & below)
                                      : T.begin + nl + rdone + (uint32_t)__popcll(rm & below);
             dst[at] = f;
-            int* cbx = s_child[left ? 0 : 1];
-            for (int e = 0; e < 6; e++) {
-                if (e & 1) atomicMax(&cbx[e], f2o(f.b[e]));
-                else atomicMin(&cbx[e], f2o(f.b[e]));
-            }
-            for (int a = 0; a < 3; a++) {
-                atomicMin(&cbx[6 + a], f2o(f.c[a]));
-                atomicMax(&cbx[9 + a], f2o(f.c[a]));
-            }
+            child_add(s_child[left ? 0 : 1], f);
         }
         ldone += (uint32_t)__popcll(lm);
         rdone += (uint32_t)__popcll(rm);
     }
     __syncthreads();
-    if (lane == 0) {
-        atomicMax(depth, T.depth);
-        float nodef[16];
-        for (int s = 0; s < 2; s++) {
-            const uint32_t b = s == 0 ? T.begin : T.begin + nl, e = s == 0 ? T.begin + nl : T.end;
-            const uint32_t ms = e - b;
-            const uint32_t id = s == 0 ? T.id + 1 : T.id + nl;
-            const int32_t ref = ms == 1 ? dst[b].ref : (int32_t)id;
-            for (int k = 0; k < 6; k++) nodef[6 * s + k] = o2f(s_child[s][k]);
-            nodef[12 + s] = __int_as_float(ref);
-            if (ms >= 2) {
-                SahTask c;
-                c.begin = b; c.end = e; c.id = id; c.depth = T.depth + 1;
-                for (int a = 0; a < 3; a++) { c.clo[a] = o2f(s_child[s][6 + a]); c.chi[a] = o2f(s_child[s][9 + a]); }
-                next[atomicAdd(nnext, 1u)] = c;
-            }
+    if (lane == 0) sah_emit(T, nl, s_child, dst, out, next, nnext, depth);
+}
+
+// ---- one big task across the chip ------------------------------------------------------
+// The top levels have a handful of tasks over millions of leaves: each is binned by many
+// workgroups (LDS bins merged into global ones with the same exact integer min/max and
+// counts), split by one lane with the same search, partitioned stably by a scan of its
+// left/right flags, and emitted as k_sah_level emits.
+constexpr uint32_t kBigTask = 1u << 15;
+
+struct SahBig {
+    int cnt[3][kSahBins];
+    int box[3][kSahBins][6];
+    int child[2][12];
+    int dec[3];
+};
+
+__global__ __launch_bounds__(256) void k_big_init(SahBig* __restrict__ B)
+{
+    bins_init(&B->cnt[0][0], &B->box[0][0][0], threadIdx.x, 256);
+    child_init(&B->child[0][0], threadIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_big_bin(const SahTask* __restrict__ task, const SahLeaf* __restrict__ src, SahBig* __restrict__ B)
+{
+    __shared__ int s_cnt[3][kSahBins];
+    __shared__ int s_box[3][kSahBins][6];
+    const SahTask T = *task;
+    bool ok[3];
+    float sc[3];
+    const int nb = task_bins(T, ok, sc);
+    bins_init(&s_cnt[0][0], &s_box[0][0][0], threadIdx.x, 256);
+    __syncthreads();
+    for (uint32_t k = T.begin + blockIdx.x * 256 + threadIdx.x; k < T.end; k += gridDim.x * 256) bin_leaf(src[k], T, ok, sc, nb, s_cnt, s_box);
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < 3 * kSahBins; k += 256) {
+        const int c = (&s_cnt[0][0])[k];
+        if (!c) continue;
+        atomicAdd(&B->cnt[0][0] + k, c);
+        for (int e = 0; e < 6; e++) {
+            int* g = &B->box[0][0][0] + 6 * k + e;
+            const int v = (&s_box[0][0][0])[6 * k + e];
+            if (e & 1) atomicMax(g, v); else atomicMin(g, v);
         }
-        nodef[14] = 0.f; nodef[15] = 0.f;
-        Node nd;
-        nd.q0 = make_float4(nodef[0], nodef[1], nodef[2], nodef[3]);
-        nd.q1 = make_float4(nodef[4], nodef[5], nodef[6], nodef[7]);
-        nd.q2 = make_float4(nodef[8], nodef[9], nodef[10], nodef[11]);
-        nd.q3 = make_float4(nodef[12], nodef[13], nodef[14], nodef[15]);
-        out[T.id] = nd;
     }
+}
+
+__global__ void k_big_split(const SahTask* __restrict__ task, SahBig* __restrict__ B)
+{
+    if (threadIdx.x) return;
+    const SahTask T = *task;
+    bool ok[3];
+    float sc[3];
+    const int nb = task_bins(T, ok, sc);
+    sah_choose(B->cnt, B->box, nb, ok, T.end - T.begin, B->dec);
+}
+
+__global__ __launch_bounds__(256) void k_big_flag(const SahTask* __restrict__ task, const SahLeaf* __restrict__ src,
+                                                  const SahBig* __restrict__ B, uint32_t* __restrict__ flag)
+{
+    const SahTask T = *task;
+    const uint32_t m = T.end - T.begin;
+    bool ok[3];
+    float sc[3];
+    const int nb = task_bins(T, ok, sc);
+    const int bax = B->dec[0], bk = B->dec[1];
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < m; i += gridDim.x * 256)
+        flag[i] = (bax < 0 ? i < m / 2 : bin_of(src[T.begin + i], T, bax, sc[bax], nb) <= bk) ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(256) void k_big_scatter(const SahTask* __restrict__ task, const SahLeaf* __restrict__ src,
+                                                     SahLeaf* __restrict__ dst, const uint32_t* __restrict__ flag,
+                                                     const uint32_t* __restrict__ prefix, SahBig* __restrict__ B)
+{
+    __shared__ int s_child[2][12];
+    const SahTask T = *task;
+    const uint32_t m = T.end - T.begin, nl = (uint32_t)B->dec[2];
+    child_init(&s_child[0][0], threadIdx.x);
+    __syncthreads();
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < m; i += gridDim.x * 256) {
+        const SahLeaf f = src[T.begin + i];
+        const bool left = flag[i] != 0;
+        dst[left ? T.begin + prefix[i] : T.begin + nl + (i - prefix[i])] = f;
+        child_add(s_child[left ? 0 : 1], f);
+    }
+    __syncthreads();
+    if (threadIdx.x < 24) {
+        const uint32_t k = threadIdx.x, e = k % 12;
+        int* g = &B->child[0][0] + k;
+        const int v = (&s_child[0][0])[k];
+        const bool hi = e < 6 ? (e & 1) != 0 : e >= 9;
+        if (hi) atomicMax(g, v); else atomicMin(g, v);
+    }
+}
+
+__global__ void k_big_emit(const SahTask* __restrict__ task, const SahLeaf* __restrict__ dst, const SahBig* __restrict__ B,
+                           Node* __restrict__ out, SahTask* __restrict__ next, uint32_t* __restrict__ nnext, uint32_t* __restrict__ depth)
+{
+    if (threadIdx.x) return;
+    sah_emit(*task, (uint32_t)B->dec[2], B->child, dst, out, next, nnext, depth);
 }
 
 // ---- BVH2 -> BVH4 collapse ------------------------------------------------------------
 
 struct C4Work { int32_t i2; uint32_t slot, P; };
 
-__global__ __launch_bounds__(256) void k_c4_parent(uint32_t nn, const Node* __restrict__ nodes, int32_t* __restrict__ parent)
+// heights (inner nodes on the longest path down, a node over two leaves = 1), one level
+// per launch: launch k finishes the nodes whose inner children were finished by earlier
+// launches (H in 1..k-1), so H = k is the height, and every box of H read was written
+// before the launch began (no fences)
+__global__ __launch_bounds__(256) void k_c4_height(uint32_t nn, const Node* __restrict__ nodes, uint32_t* __restrict__ H, uint32_t k)
 {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= nn) return;
+    if (i >= nn || H[i]) return;
     const int32_t r0 = __float_as_int(nodes[i].q3.x), r1 = __float_as_int(nodes[i].q3.y);
-    if (r0 >= 0) parent[r0] = (int32_t)i;
-    if (r1 >= 0) parent[r1] = (int32_t)i;
-}
-
-// heights (inner nodes on the longest path down, a node over two leaves = 1), bottom-up:
-// a climb starts at each leaf child; the second arrival at a node computes it and goes on
-__global__ __launch_bounds__(256) void k_c4_height(uint32_t nn, const Node* __restrict__ nodes, const int32_t* __restrict__ parent,
-                                                   uint32_t* __restrict__ arrive, volatile uint32_t* H)
-{
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= nn) return;
-    const int32_t r0 = __float_as_int(nodes[i].q3.x), r1 = __float_as_int(nodes[i].q3.y);
-    const int climbs = (r0 < 0 ? 1 : 0) + (r1 < 0 ? 1 : 0);
-    for (int c = 0; c < climbs; c++) {
-        int32_t node = (int32_t)i;
-        while (node >= 0) {
-            __threadfence();
-            if (atomicAdd(&arrive[node], 1u) == 0u) break;        // the other child is not done yet
-            __threadfence();
-            const int32_t a = __float_as_int(nodes[node].q3.x), b = __float_as_int(nodes[node].q3.y);
-            const uint32_t ha = a >= 0 ? H[a] : 0u, hb = b >= 0 ? H[b] : 0u;
-            H[node] = (ha > hb ? ha : hb) + 1u;
-            node = parent[node];
-        }
-    }
+    if (r0 >= 0) { const uint32_t h = H[r0]; if (h == 0 || h >= k) return; }
+    if (r1 >= 0) { const uint32_t h = H[r1]; if (h == 0 || h >= k) return; }
+    H[i] = k;
 }
 
 __device__ inline float box_area(const float* b)
@@ -362,11 +487,12 @@ int sah_rebuild_device(const int* d_first, const int* d_last, const int* d_split
     uint32_t *flag = nullptr, *pos = nullptr, *tmp = nullptr, *total = nullptr, *cnt = nullptr, *dep = nullptr;
     int* cb = nullptr;
     SahLeaf *L0 = nullptr, *L1 = nullptr;
-    SahTask *T0 = nullptr, *T1 = nullptr;
+    SahTask *T0 = nullptr, *T1 = nullptr, *tsmall = nullptr;
+    SahBig* big = nullptr;
     Node* out = nullptr;
     auto cleanup = [&]() {
         sfree(flag); sfree(pos); sfree(tmp); sfree(total); sfree(cnt); sfree(dep); sfree(cb);
-        sfree(L0); sfree(L1); sfree(T0); sfree(T1);
+        sfree(L0); sfree(L1); sfree(T0); sfree(T1); sfree(tsmall); sfree(big);
     };
 #define SK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { cleanup(); sfree(out); return hip_fail(e_, #x, __LINE__); } } while (0)
     const unsigned gb = (unsigned)((ni + 255) / 256);
@@ -401,13 +527,50 @@ int sah_rebuild_device(const int* d_first, const int* d_last, const int* d_split
     SK(hipMalloc(&dep, 4));
     SK(hipMemset(dep, 0, 4));
     SK(hipGetLastError());
+    // tasks of at least kBigTask leaves go across the chip (the top levels); the host reads
+    // the task list while it is short to find them.  flag, pos and tmp are free by now and
+    // hold the big tasks' flags, their prefix and the scan's scratch.
+    constexpr uint32_t kHostTasks = 1024;
+    SK(hipMalloc(&big, sizeof(SahBig)));
+    SK(hipMalloc(&tsmall, sizeof(SahTask) * kHostTasks));
+    std::vector<SahTask> ht, small;
+    std::vector<uint32_t> bigs;
     uint32_t ntask = 1;
     SahLeaf *src = L0, *dst = L1;
     SahTask *tin = T0, *tout = T1;
     for (uint32_t level = 0; ntask; level++) {
         if (level > 4096) { cleanup(); sfree(out); return fail(LSG_ERR_INVALID, "sah_rebuild_device: no progress"); }
         SK(hipMemset(cnt, 0, 4));
-        k_sah_level<<<ntask, 64>>>(tin, src, dst, out, tout, cnt, dep);
+        bigs.clear();
+        small.clear();
+        if (ntask <= kHostTasks) {
+            ht.resize(ntask);
+            SK(hipMemcpy(ht.data(), tin, sizeof(SahTask) * ntask, hipMemcpyDeviceToHost));
+            for (uint32_t i = 0; i < ntask; i++) {
+                if (ht[i].end - ht[i].begin >= kBigTask) bigs.push_back(i);
+                else small.push_back(ht[i]);
+            }
+        }
+        if (bigs.empty()) {
+            k_sah_level<<<ntask, 64>>>(tin, src, dst, out, tout, cnt, dep);
+        } else {
+            if (!small.empty()) {
+                SK(hipMemcpy(tsmall, small.data(), sizeof(SahTask) * small.size(), hipMemcpyHostToDevice));
+                k_sah_level<<<(unsigned)small.size(), 64>>>(tsmall, src, dst, out, tout, cnt, dep);
+            }
+            for (uint32_t i : bigs) {
+                const SahTask* t = tin + i;
+                const uint32_t m = ht[i].end - ht[i].begin;
+                const unsigned g = std::min(2048u, (m + 255) / 256);
+                k_big_init<<<1, 256>>>(big);
+                k_big_bin<<<g, 256>>>(t, src, big);
+                k_big_split<<<1, 64>>>(t, big);
+                k_big_flag<<<g, 256>>>(t, src, big, flag);
+                if (int st = scan_exclusive_u32(flag, pos, m, nullptr, tmp, nullptr)) { cleanup(); sfree(out); return st; }
+                k_big_scatter<<<g, 256>>>(t, src, dst, flag, pos, big);
+                k_big_emit<<<1, 64>>>(t, dst, big, out, tout, cnt, dep);
+            }
+        }
         SK(hipGetLastError());
         SK(hipMemcpy(&ntask, cnt, 4, hipMemcpyDeviceToHost));
         if (ntask > cap) { cleanup(); sfree(out); return fail(LSG_ERR_INVALID, "sah_rebuild_device: %u tasks over %u", ntask, cap); }
@@ -422,34 +585,35 @@ int sah_rebuild_device(const int* d_first, const int* d_last, const int* d_split
     return 0;
 }
 
-// The BVH4 of a BVH2 (nodes, nn, root >= 0) on the GPU, breadth first: heights bottom-up,
+// The BVH4 of a BVH2 (nodes, nn, root >= 0) on the GPU, breadth first: heights bottom-up
+// (one launch per level),
 // then one level of BVH4 nodes per launch, slots in level order.  budget = the traversal
 // stack entries minus one.  Sets C.d_nodes4, C.nnodes4, C.root4 = 0, C.stack_need.
 int collapse4_device(ColliderImpl& C, uint32_t budget)
 {
     const uint32_t nn = C.nnodes;
-    int32_t* parent = nullptr;
-    uint32_t *arrive = nullptr, *H = nullptr, *cnt = nullptr, *prefix = nullptr, *pnext = nullptr, *tmp = nullptr, *total = nullptr,
+    uint32_t *H = nullptr, *cnt = nullptr, *prefix = nullptr, *pnext = nullptr, *tmp = nullptr, *total = nullptr,
              *need = nullptr;
     C4Work *w0 = nullptr, *w1 = nullptr;
     Node4* out = nullptr;
     auto cleanup = [&]() {
-        sfree(parent); sfree(arrive); sfree(H); sfree(cnt); sfree(prefix); sfree(pnext); sfree(tmp); sfree(total); sfree(need);
+        sfree(H); sfree(cnt); sfree(prefix); sfree(pnext); sfree(tmp); sfree(total); sfree(need);
         sfree(w0); sfree(w1);
     };
 #define SK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { cleanup(); sfree(out); return hip_fail(e_, #x, __LINE__); } } while (0)
     const unsigned gb = (nn + 255) / 256;
-    SK(hipMalloc(&parent, 4 * (size_t)nn));
-    SK(hipMalloc(&arrive, 4 * (size_t)nn));
     SK(hipMalloc(&H, 4 * (size_t)nn));
-    SK(hipMemset(parent, 0xff, 4 * (size_t)nn));
-    SK(hipMemset(arrive, 0, 4 * (size_t)nn));
     SK(hipMemset(H, 0, 4 * (size_t)nn));
-    k_c4_parent<<<gb, 256>>>(nn, C.d_nodes, parent);
-    k_c4_height<<<gb, 256>>>(nn, C.d_nodes, parent, arrive, H);
-    SK(hipGetLastError());
     uint32_t hroot = 0;
-    SK(hipMemcpy(&hroot, H + C.root, 4, hipMemcpyDeviceToHost));
+    for (uint32_t k = 1;; k++) {
+        k_c4_height<<<gb, 256>>>(nn, C.d_nodes, H, k);
+        if (k % 8 == 0) {
+            SK(hipMemcpy(&hroot, H + C.root, 4, hipMemcpyDeviceToHost));
+            if (hroot) break;
+            if (k > 4096) { cleanup(); return fail(LSG_ERR_INVALID, "collapse4: no progress"); }
+        }
+    }
+    SK(hipGetLastError());
     if (hroot > budget) { cleanup(); return fail(LSG_ERR_LIMIT, "collapse4: tree height %u over stack budget %u", hroot, budget); }
     SK(hipMalloc(&out, sizeof(Node4) * (size_t)nn));
     SK(hipMalloc(&w0, sizeof(C4Work) * (size_t)nn));

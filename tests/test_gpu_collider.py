@@ -32,12 +32,14 @@ def _same_hits(g, gf, o, h):
     np.testing.assert_array_equal(gf[m].astype(np.int32), o["front"][m])
 
 
-@pytest.mark.parametrize("leaf", [1, 4, 8])
-def test_gpu_tree_equals_oracle_tree(gpu, oracle, leaf):
+@pytest.mark.parametrize("freq,leaf", [(30, 1), (30, 4), (30, 8), (100, 1), (100, 2)])
+def test_gpu_tree_equals_oracle_tree(gpu, oracle, freq, leaf):
     """The GPU LBVH equals the oracle's node for node; the default build's tree equals
-    the host SAH rebuild of that LBVH (lsg_bvh_rebuild_sah, checked on the CPU)."""
+    the host SAH rebuild of that LBVH (lsg_bvh_rebuild_sah, checked on the CPU).  At
+    frequency 100 (200,000 triangles) the top tasks have more than 2^15 leaves and are
+    rebuilt across the chip."""
     from lightsprintsdk_amd.collider import rebuild_sah
-    V, I = scenes.icosphere(30, seed=11)
+    V, I = scenes.icosphere(freq, seed=11)
     c = _col(V, I, leaf, lbvh_only=True)
     t_gpu = c.download_tree()
     t_cpu = oracle.lbvh_build(V, I, leaf_size=leaf)
