@@ -142,25 +142,6 @@ __global__ __launch_bounds__(256) void k_rs_hist(const uint32_t* __restrict__ ke
 }
 
 // exclusive scan of m counters in place, one block of 1024 threads
-__global__ __launch_bounds__(1024) void k_rs_scan(uint32_t* __restrict__ a, uint32_t m)
-{
-    __shared__ uint32_t s[1024];
-    uint32_t chunk = (m + 1023) / 1024;
-    uint32_t b = threadIdx.x * chunk, e = min(m, b + chunk);
-    uint32_t sum = 0;
-    for (uint32_t i = b; i < e; i++) sum += a[i];
-    s[threadIdx.x] = sum;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        uint32_t v = threadIdx.x >= (uint32_t)off ? s[threadIdx.x - off] : 0;
-        __syncthreads();
-        s[threadIdx.x] += v;
-        __syncthreads();
-    }
-    uint32_t run = s[threadIdx.x] - sum;
-    for (uint32_t i = b; i < e; i++) { uint32_t v = a[i]; a[i] = run; run += v; }
-}
-
 __global__ __launch_bounds__(256) void k_rs_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                     uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                     uint32_t n, int shift, const uint32_t* __restrict__ offs)
@@ -477,13 +458,13 @@ static int build(ColliderImpl& C, const float* verts, uint32_t nv, const uint32_
     // per-triangle precompute + keys
     uint32_t *d_keys = nullptr, *d_vals = nullptr, *d_k2 = nullptr, *d_v2 = nullptr, *d_bytes = nullptr, *d_nvalid = nullptr;
     float4 *d_plane_u = nullptr, *d_sv_u = nullptr, *d_ev_u = nullptr, *d_blo_u = nullptr, *d_bhi_u = nullptr;
-    uint32_t* d_hist = nullptr;
+    uint32_t *d_hist = nullptr, *d_offs = nullptr, *d_stmp = nullptr;
     int *d_first = nullptr, *d_last = nullptr, *d_split = nullptr, *d_pn = nullptr, *d_pl = nullptr;
     float4 *d_plo = nullptr, *d_phi = nullptr, *d_nlo = nullptr, *d_nhi = nullptr;
     uint32_t *d_arrive = nullptr, *d_depth = nullptr;
     auto cleanup = [&]() {
         dfree(d_keys); dfree(d_vals); dfree(d_k2); dfree(d_v2); dfree(d_bytes); dfree(d_nvalid);
-        dfree(d_plane_u); dfree(d_sv_u); dfree(d_ev_u); dfree(d_blo_u); dfree(d_bhi_u); dfree(d_hist);
+        dfree(d_plane_u); dfree(d_sv_u); dfree(d_ev_u); dfree(d_blo_u); dfree(d_bhi_u); dfree(d_hist); dfree(d_offs); dfree(d_stmp);
         dfree(d_first); dfree(d_last); dfree(d_split); dfree(d_pn); dfree(d_pl);
         dfree(d_plo); dfree(d_phi); dfree(d_nlo); dfree(d_nhi); dfree(d_arrive); dfree(d_depth);
     };
@@ -508,11 +489,15 @@ static int build(ColliderImpl& C, const float* verts, uint32_t nv, const uint32_
     if (nt > 1) {
         unsigned nb = blocks(nt, RS_TILE);
         CK(dalloc(&d_hist, 256 * (size_t)nb));
+        CK(dalloc(&d_offs, 256 * (size_t)nb));
+        CK(dalloc(&d_stmp, scan_tmp_words(256 * nb)));
         uint32_t *ki = d_keys, *vi = d_vals, *ko = d_k2, *vo = d_v2;
         for (int shift = 0; shift < 32; shift += 8) {
             k_rs_hist<<<nb, 256>>>(ki, nt, shift, d_hist);
-            k_rs_scan<<<1, 1024>>>(d_hist, 256 * nb);
-            k_rs_scatter<<<nb, 256>>>(ki, vi, ko, vo, nt, shift, d_hist);
+            // digit-major counts -> each (digit, block)'s first output slot, by the
+            // device-wide scan
+            CK(scan_exclusive_u32(d_hist, d_offs, 256 * nb, nullptr, d_stmp, nullptr));
+            k_rs_scatter<<<nb, 256>>>(ki, vi, ko, vo, nt, shift, d_offs);
             std::swap(ki, ko); std::swap(vi, vo);
         }
         CKH(hipGetLastError());

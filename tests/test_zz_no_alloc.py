@@ -1,7 +1,8 @@
 """The allocation-free filter chain (lsg_filter_lightmaps_noalloc) through the C ABI:
 bit-identical to lsg_filter_lightmaps on two calls in a row, the second with a smaller
-batch (the static scratch keeps the first call's contents), and a clean LSG_ERR_LIMIT
-beyond its stated maximum."""
+batch (the static scratch keeps the first call's contents), with and without wrap (maps
+whose texel counts do and do not divide 2^32), and a clean LSG_ERR_LIMIT beyond its
+stated maximum."""
 import ctypes
 
 import numpy as np
@@ -57,14 +58,15 @@ def _run(fn, rgba, maps, nout, params):
     return out_rgba, out_rgbf
 
 
-def test_noalloc_filters_equal_the_allocating_entry_point(gpu):
-    params = N.FilterParams(1.0, 1000.0, (ctypes.c_float * 4)(0.25, 0.5, 0.125, 0.0), 0)
+@pytest.mark.parametrize("wrap", [0, 1])
+def test_noalloc_filters_equal_the_allocating_entry_point(gpu, wrap):
+    params = N.FilterParams(1.0, 1000.0, (ctypes.c_float * 4)(0.25, 0.5, 0.125, 0.0), wrap)
     cases = [([(256, 256), (64, 64), (128, 32), (100, 37), (17, 9)], 11),   # 84,609 texels
              ([(48, 40), (33, 21)], 12)]                                      # smaller second call
     print()
     for shapes, seed in cases:
         rgba, maps, nout = _maps(shapes, seed)
-        print(f"noalloc filter case: {len(shapes)} maps, {nout} texels, seed {seed}", flush=True)
+        print(f"noalloc filter case: {len(shapes)} maps, {nout} texels, seed {seed}, wrap {wrap}", flush=True)
         ref = _run("lsg_filter_lightmaps", rgba, maps, nout, params)
         got = _run("lsg_filter_lightmaps_noalloc", rgba, maps, nout, params)
         np.testing.assert_array_equal(got[0].view(np.uint32), ref[0].view(np.uint32))
