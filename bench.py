@@ -504,6 +504,10 @@ def main():
         res = bake_workload(a.steps, a.warmup, threads)
         if not a.no_extras:
             res["learned"] = learned_summary()
+            bt = batched_workload(2, 1)
+            res["batched"] = dict(workload=bt["config"]["workload"], texels_per_s=bt["value"], ms_per_step=bt["ms_per_step"],
+                                  steps=2, warmup=1, **{k: bt["config"][k] for k in ("triangles", "objects", "texels", "tuning",
+                                                                                      "setup_s", "autotune_s")})
             c = collider_workload(5, 1, threads, with_cpu=True)
             cb = c["cpu_baseline"]
             res["collider"] = dict(workload=c["config"]["workload"], mrays_per_s=c["value"],
