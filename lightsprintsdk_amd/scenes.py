@@ -290,9 +290,16 @@ def box_room(seed: int = 3, n_boxes: int = 20, target_tris: int = 50_000, lightm
 
 
 def minimal_safe_distance(verts: np.ndarray) -> float:
-    """RRSolver.cpp:333: (maxi-mini).avg()*1e-6 in float32."""
-    lo = verts.min(axis=0).astype(np.float32)
-    hi = verts.max(axis=0).astype(np.float32)
+    """RRSolver.cpp:333: (maxi-mini).avg()*1e-6 in float32, the box as RRMesh::getAABB
+    finds it (RRMesh.cpp:164-196): per axis over the finite coordinates only, an axis
+    without any (or no vertices at all) spanning 0."""
+    v = np.asarray(verts, np.float32).reshape(-1, 3)
+    lo = np.zeros(3, np.float32)
+    hi = np.zeros(3, np.float32)
+    for j in range(3):
+        c = v[:, j][np.isfinite(v[:, j])]
+        if len(c):
+            lo[j], hi[j] = c.min(), c.max()
     e = (hi - lo).astype(np.float32)
     avg = np.float32((np.float32(e[0] + e[1]) + e[2]) * np.float32(0.33333333333333))
     return float(np.float32(avg * np.float32(1e-6)))

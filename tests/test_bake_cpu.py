@@ -314,3 +314,16 @@ def test_city_terrain_scene_layout():
     # the sweep: unit vectors, elevation 15..75 degrees, one turn of azimuth
     el = [np.degrees(np.arcsin(-S.sun_sweep(k)[2])) for k in range(60)]
     assert min(el) >= 15 - 1e-6 and max(el) <= 75 + 1e-6
+
+
+def test_minimal_safe_distance_follows_the_solver():
+    """RRSolver's ray offset (RRSolver.cpp:333): the mean extent of the scene's box times
+    1e-6, the box over finite coordinates only (RRMesh::getAABB, RRMesh.cpp:164-196)."""
+    v = np.float32([[1, -2, 5], [4, 4, -4], [2, 0, 0]])          # extents 3, 6, 9: mean 6
+    assert S.minimal_safe_distance(v) == pytest.approx(6e-6, rel=1e-6)
+    w = np.vstack([v, np.float32([[np.inf, np.nan, 1e9]])])        # non-finite coordinates are skipped
+    assert S.minimal_safe_distance(w) == pytest.approx((3 + 6 + (1e9 + 4)) / 3 * 1e-6, rel=1e-6)
+    nan_axis = v.copy()
+    nan_axis[:, 2] = np.nan                                          # an axis without finite values spans 0
+    assert S.minimal_safe_distance(nan_axis) == pytest.approx(3e-6, rel=1e-6)
+    assert S.minimal_safe_distance(np.zeros((0, 3), np.float32)) == 0.0
