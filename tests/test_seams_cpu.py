@@ -342,6 +342,14 @@ def _meshes():
     v, i, n, uv = _cylinder(8, 3)
     perm = rng.permutation(len(i))            # triangle order decides the seams' order
     out.append(("cylinder_shuffled", v, i[perm], n[perm], uv[perm]))
+    # degenerate triangles: two corners on one (stitched) vertex list the triangle twice there
+    v, i, n, uv = _quad_two_charts()
+    v = np.vstack([v, v[2:3]])                # a copy of vertex 2, stitched to it
+    i = np.vstack([i, np.uint32([[0, 2, 4], [2, 4, 3], [4, 0, 2]])])
+    n = np.vstack([n, np.tile(np.float32([0, 0, 1]), (3, 3))])
+    uv = np.vstack([uv, np.float32([[0.1, 0.7, 0.2, 0.7, 0.2, 0.8], [0.3, 0.3, 0.35, 0.3, 0.3, 0.35],
+                                    [0.5, 0.1, 0.6, 0.1, 0.6, 0.2]])])
+    out.append(("degenerate", v, i, n, uv))
     return out
 
 
