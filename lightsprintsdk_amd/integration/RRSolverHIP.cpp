@@ -534,6 +534,12 @@ unsigned RRSolverHIP::bake(const std::vector<Target>& pixel, const std::vector<T
 			lsg_bake_destroy(plan);
 			return 0;
 		}
+		// an abort requested while the batch ran: the caller's buffers are left as they were
+		if (aborting)
+		{
+			lsg_bake_destroy(plan);
+			return 0;
+		}
 		// writeback of every layer into the caller's buffers (lightmap.cpp:866-876)
 		std::vector<float> rgba;
 		const unsigned version = getSolutionVersion();
@@ -664,6 +670,9 @@ unsigned RRSolverHIP::bakeVertices(const std::vector<Target>& vertex, const lsg_
 unsigned RRSolverHIP::updateLightmaps(int layerLightmap, int layerDirectionalLightmap, int layerBentNormals,
                                       const UpdateParameters* params, const FilteringParameters* filtering)
 {
+	// RRSolver::aborting set before the call: nothing is updated (lightmap.cpp:650-651,
+	// 1071); set while the GPU batch runs, it takes effect when the batch returns
+	if (aborting) return 0;
 	const bool supported = params && params->quality > 0 && environmentSupported(*params) && lsg_device_count() > 0;
 	if (!supported)
 	{
@@ -722,6 +731,7 @@ unsigned RRSolverHIP::updateLightmaps(int layerLightmap, int layerDirectionalLig
 unsigned RRSolverHIP::updateLightmap(int objectNumber, RRBuffer* lightmap, RRBuffer* directionalLightmap[3],
                                      RRBuffer* bentNormals, const UpdateParameters* params, const FilteringParameters* filtering)
 {
+	if (aborting) return 0;                  // lightmap.cpp:650-651
 	const RRObjects& objects = getStaticObjects();
 	Target t{(unsigned)objectNumber, {lightmap, directionalLightmap ? directionalLightmap[0] : nullptr,
 	                                  directionalLightmap ? directionalLightmap[1] : nullptr,
