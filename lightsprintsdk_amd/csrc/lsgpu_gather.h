@@ -9,6 +9,9 @@ namespace lsg {
 // SceneView::env_type of a blend of two environments (RRBufferBlend)
 constexpr uint32_t kEnvBlend = 3;
 
+// slots of a scene's ring of tile-queue counters for persistent gathers (GatherArgs::pq)
+constexpr uint32_t kQueueSlots = 64;
+
 constexpr int kMaxLights = 16;
 
 struct SceneView {
@@ -53,6 +56,10 @@ struct GatherArgs {
     // (walk_cap records) is where the launch functions write them; null = no records.
     const struct WalkRec* walk;
     void* walk_buf; uint32_t walk_cap;
+    // counters (8 words) for a persistent launch's tile queues, one slot of the scene's
+    // ring, or null: launches of many texels with more than 32 shooters per series then
+    // go through the persistent kernel (the same results as the one-wave-per-texel one)
+    uint32_t* pq;
 };
 
 // persistent launches: tiles of texels handed out by atomic counters, one queue per
@@ -82,6 +89,8 @@ struct SceneImpl {
     float env_rot[18] = {};            // yaw rotations of the two environments
     float env_blend = 0;
     uint32_t env_version = 0;          // counts environment changes (captured graphs hold them)
+    uint32_t* d_pq = nullptr;          // ring of kQueueSlots x 8 tile-queue counters for persistent gathers
+    uint32_t pq_next = 0;              // next slot (host atomic), so launches on different streams differ
     lsg_lightmap* d_lmaps = nullptr;   // lightmap list of the last learned-feature pass
     uint32_t lmaps_cap = 0;
     std::vector<lsg_lightmap> h_lmaps; // host copy it was uploaded from

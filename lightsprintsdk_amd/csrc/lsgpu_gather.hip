@@ -1604,6 +1604,12 @@ int gather_args(lsg_scene* s, const lsg_texel* texels, uint32_t ntexels, const l
         if (e) return e;
     }
     A.walk = nullptr; A.walk_buf = nullptr; A.walk_cap = 0;
+    A.pq = nullptr;
+    {
+        SceneImpl& SP = s->impl;
+        if (!SP.d_pq) LSG_HIP(hipMalloc(&SP.d_pq, sizeof(uint32_t) * 8 * kQueueSlots));
+        A.pq = SP.d_pq + 8 * (__atomic_fetch_add(&SP.pq_next, 1u, __ATOMIC_RELAXED) % kQueueSlots);
+    }
     return LSG_OK;
 }
 
@@ -1673,6 +1679,7 @@ static int launch_gather_one(const GatherArgs& A, int stack, hipStream_t st);
 // shooters (fewer launch drains for low-quality gathers), at most 2^25 texels (64
 // threads per texel stay below 2^31 threads per launch)
 constexpr uint32_t kLaunchTexels = 1u << 22;
+constexpr uint32_t kPersistentMinTexels = 16384;   // smaller launches stay one wave per texel
 
 uint32_t gather_chunk(const GatherArgs& A)
 {
@@ -1776,6 +1783,14 @@ static void launch_gather_kernels(GatherArgs A, int stack, hipStream_t st)
 static int launch_gather_one(const GatherArgs& A, int stack, hipStream_t st)
 {
     if (!A.ntex) return LSG_OK;
+    // many texels with more than 32 shooters per series: the persistent kernel (texel
+    // tiles from 8 per-XCD queues, direction-binned hemisphere order; bit-identical to
+    // k_gather), with the bake plan's default shape
+    if (!A.layered && A.pq && !A.parts && !A.stats && A.ntex >= kPersistentMinTexels && std::max(A.hemi_rays, A.rounds) > 32u) {
+        WorkQueue Q;
+        Q.counters = A.pq; Q.nqueues = 8; Q.ntiles = A.ntex; Q.tile = 1;
+        return launch_gather_persistent(A, stack, Q, 4, device_cus() * 4, false, st);
+    }
     switch (features(A)) {
     case 0: if (A.layered) launch_gather_kernels<true, 0>(A, stack, st); else launch_gather_kernels<false, 0>(A, stack, st); break;
     case kFeatEnv:
@@ -1890,7 +1905,7 @@ void lsg_scene_destroy(lsg_scene* s)
 {
     if (!s) return;
     dfree(s->impl.d_vnormal); dfree(s->impl.d_side); dfree(s->impl.d_exit); dfree(s->impl.d_lights);
-    dfree(s->impl.d_vtan); dfree(s->impl.d_vbit); dfree(s->impl.d_env); dfree(s->impl.d_env1); dfree(s->impl.d_lmaps);
+    dfree(s->impl.d_vtan); dfree(s->impl.d_vbit); dfree(s->impl.d_env); dfree(s->impl.d_env1); dfree(s->impl.d_pq); dfree(s->impl.d_lmaps);
     dfree(s->impl.d_ind); dfree(s->impl.d_walk);
     free_point_materials(s->impl);
     delete s;
