@@ -57,7 +57,7 @@ struct GatherArgs {
     const struct WalkRec* walk;
     void* walk_buf; uint32_t walk_cap;
     // counters (8 words) for a persistent launch's tile queues, one slot of the scene's
-    // ring, or null: launches of many texels with more than 32 shooters per series then
+    // ring, or null: launches of many texels with more than 32 hemisphere rays each then
     // go through the persistent kernel (the same results as the one-wave-per-texel one)
     uint32_t* pq;
 };

@@ -1783,10 +1783,11 @@ static void launch_gather_kernels(GatherArgs A, int stack, hipStream_t st)
 static int launch_gather_one(const GatherArgs& A, int stack, hipStream_t st)
 {
     if (!A.ntex) return LSG_OK;
-    // many texels with more than 32 shooters per series: the persistent kernel (texel
+    // many texels with more than 32 hemisphere rays each: the persistent kernel (texel
     // tiles from 8 per-XCD queues, direction-binned hemisphere order; bit-identical to
-    // k_gather), with the bake plan's default shape
-    if (!A.layered && A.pq && !A.parts && !A.stats && A.ntex >= kPersistentMinTexels && std::max(A.hemi_rays, A.rounds) > 32u) {
+    // k_gather), with the bake plan's default shape.  Gathers of light rays only (a first
+    // gather) stay one wave per texel, which is faster for them.
+    if (!A.layered && A.pq && !A.parts && !A.stats && A.ntex >= kPersistentMinTexels && A.hemi_rays > 32u) {
         WorkQueue Q;
         Q.counters = A.pq; Q.nqueues = 8; Q.ntiles = A.ntex; Q.tile = 1;
         return launch_gather_persistent(A, stack, Q, 4, device_cus() * 4, false, st);
