@@ -446,6 +446,18 @@ static void yaw_matrix(float yaw, float* m)
     m[6] = (float)(c1 * s2 * s3 - c3 * s1); m[7] = (float)(s1 * s3 + c1 * c3 * s2); m[8] = (float)(c1 * c2);
 }
 
+/* direction d turned by the yaw rotation of an environment: rotationByYawPitchRoll((yaw,
+ * 0, 0)) then RRMatrix3x4::transformDirection (RRMath.cpp:212-219) */
+void lsgo_env_rotate(float yaw, const float* d, float* r)
+{
+    float m[9];
+    yaw_matrix(yaw, m);
+    const float x = d[0] * m[0] + d[1] * m[1] + d[2] * m[2];
+    const float y = d[0] * m[3] + d[1] * m[4] + d[2] * m[5];
+    const float z = d[0] * m[6] + d[1] * m[7] + d[2] * m[8];
+    r[0] = x; r[1] = y; r[2] = z;
+}
+
 /* createEnvironmentBlend (RRBufferBlend.cpp:21-30) makes a blend unless there is no
  * second environment and the first is not rotated */
 static int blend_on(const lsgo_env* env)
@@ -470,11 +482,8 @@ static void env_color(const lsgo_env* env, const float* d, float* out)
     float* col[2] = {c0, c1};
     for (int k = 0; k < 2; k++) {
         if (!e[k] || !e[k]->type) continue;
-        float m[9];
-        yaw_matrix(ang[k], m);
-        /* RRMatrix3x4::transformDirection, RRMath.cpp:212-219 */
-        const float r[3] = {d[0] * m[0] + d[1] * m[1] + d[2] * m[2], d[0] * m[3] + d[1] * m[4] + d[2] * m[5],
-                            d[0] * m[6] + d[1] * m[7] + d[2] * m[8]};
+        float r[3];
+        lsgo_env_rotate(ang[k], d, r);
         const float* t = e[k]->rgb + 3 * (size_t)lsgo_env_index(e[k], r);
         for (int c = 0; c < 3; c++) col[k][c] = t[c];
     }

@@ -568,6 +568,38 @@ def filter_lightmap_ex(rgba, sigma=1.0, spread=1000.0, bg=(0, 0, 0, 0), wrap=Fal
     return img, out
 
 
+ROTATE_BIN = os.path.join(ORACLE_DIR, "_ref", "rotate_ref")
+
+
+def env_rotate(yaws, dirs, library=None):
+    """The oracle's yaw rotation of directions (lsgo_env_rotate): float32 [n, 3]."""
+    L = lib() if library is None else library
+    L.lsgo_env_rotate.argtypes = [ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p]
+    L.lsgo_env_rotate.restype = None
+    d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    out = np.zeros_like(d)
+    r = np.zeros(3, np.float32)
+    for k in range(len(d)):
+        dk = np.ascontiguousarray(d[k])
+        L.lsgo_env_rotate(float(yaws[k]), _p(dk), _p(r))
+        out[k] = r
+    return out
+
+
+def reference_rotate(yaws, dirs):
+    """The compiled reference's rotationByYawPitchRoll((yaw,0,0)).transformDirection
+    (oracle/_ref/rotate_ref) of each direction: float32 [n, 3]."""
+    d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    rec = np.concatenate([np.asarray(yaws, np.float32).reshape(-1, 1), d], axis=1).astype(np.float32)
+    with tempfile.TemporaryDirectory() as tmp:
+        fin, fout = os.path.join(tmp, "in.bin"), os.path.join(tmp, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(np.uint32([len(d)]).tobytes())
+            f.write(rec.tobytes())
+        subprocess.run([ROTATE_BIN, fin, fout], check=True, capture_output=True)
+        return np.fromfile(fout, np.float32).reshape(-1, 3)
+
+
 def unwrap_seams_from_corners(corners, nverts, tri_uv, library=None):
     """UnwrapSeams's search over a stitched mesh (the stitched vertex of every corner,
     [ntri*3]): float32 [n, 8]."""

@@ -10,8 +10,9 @@ env0 turns the hemisphere gather on (gather.cpp:149, 197).
 Checked against closed forms: uniform skies mix linearly; the no-blend rule; a cube sky
 lit on its +x face only, seen by an upward floor, gives the floor the cosine-weighted
 share of the face the rotation brings overhead (a quadrature of the cube face), and
-nothing when the rotation turns the face below the floor.  Copies of the oracle with a
-mistake planted in the blend must fail."""
+nothing when the rotation turns the face below the floor; the rotation itself equals the
+reference's, compiled from RRMath.cpp, bit for bit.  Copies of the oracle with a mistake
+planted in the blend must fail."""
 import ctypes
 import os
 import shutil
@@ -107,14 +108,30 @@ def test_blend_closed_forms(oracle, check, tol):
     assert check(lambda env, **kw: _gather(oracle, env, **kw)) < tol
 
 
+def test_rotation_equals_the_reference_bit_for_bit(oracle):
+    """The oracle's environment rotation equals the reference's own
+    rotationByYawPitchRoll((yaw, 0, 0)).transformDirection, compiled from RRMath.cpp
+    (oracle/_ref/rotate_ref), bit for bit on 3,000 random yaws and directions, including
+    multiples of pi/2 where the matrix entries round to signed zeros."""
+    if not os.path.exists(oracle.ROTATE_BIN):
+        pytest.skip("oracle/_ref/rotate_ref is built only where the reference sources exist")
+    rng = np.random.default_rng(21)
+    n = 3000
+    yaws = rng.uniform(-7.0, 7.0, n).astype(np.float32)
+    yaws[:400] = (np.pi / 2 * rng.integers(-4, 5, 400)).astype(np.float32)
+    d = rng.normal(size=(n, 3)).astype(np.float32)
+    d[:200, rng.integers(0, 3)] = 0.0
+    got = oracle.env_rotate(yaws, d)
+    ref = oracle.reference_rotate(yaws, d)
+    np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32))
+
+
 MUTATIONS = [
     ("const float k0 = 1 - env->blend, k1 = env->blend;", "const float k1 = 1 - env->blend, k0 = env->blend;", check_uniform_mix),
     ("return env->blended && ((env->env1 && env->env1->type) || env->angle0 != 0);", "return env->blended;", check_no_blend_rule),
     ("(p->hemisphere || env0)", "(p->hemisphere || env)", check_no_blend_rule),
-    ("const float r[3] = {d[0] * m[0] + d[1] * m[1] + d[2] * m[2], d[0] * m[3] + d[1] * m[4] + d[2] * m[5],\n"
-     "                            d[0] * m[6] + d[1] * m[7] + d[2] * m[8]};",
-     "const float r[3] = {d[0] * m[0] + d[1] * m[3] + d[2] * m[6], d[0] * m[1] + d[1] * m[4] + d[2] * m[7],\n"
-     "                            d[0] * m[2] + d[1] * m[5] + d[2] * m[8]};", check_rotation),
+    ("const float x = d[0] * m[0] + d[1] * m[1] + d[2] * m[2];", "const float x = d[0] * m[0] + d[1] * m[3] + d[2] * m[6];",
+     check_rotation),
 ]
 
 
