@@ -165,6 +165,9 @@ __device__ inline void sah_emit(const SahTask& T, uint32_t nl, const int (*child
 {
     atomicMax(depth, T.depth);
     float nodef[16];
+    // the children of two or more leaves become tasks, appended with one atomic
+    const uint32_t nkids = (nl >= 2 ? 1u : 0u) + (T.end - T.begin - nl >= 2 ? 1u : 0u);
+    uint32_t slot = nkids ? atomicAdd(nnext, nkids) : 0u;
     for (int s = 0; s < 2; s++) {
         const uint32_t b = s == 0 ? T.begin : T.begin + nl, e = s == 0 ? T.begin + nl : T.end;
         const uint32_t ms = e - b;
@@ -176,7 +179,7 @@ __device__ inline void sah_emit(const SahTask& T, uint32_t nl, const int (*child
             SahTask c;
             c.begin = b; c.end = e; c.id = id; c.depth = T.depth + 1;
             for (int a = 0; a < 3; a++) { c.clo[a] = o2f(child[s][6 + a]); c.chi[a] = o2f(child[s][9 + a]); }
-            next[atomicAdd(nnext, 1u)] = c;
+            next[slot++] = c;
         }
     }
     nodef[14] = 0.f; nodef[15] = 0.f;
